@@ -1,0 +1,138 @@
+/*
+ * bilu.h -- C ABI of the MI355X batched shared-pattern ILU(0)/ParILU(0)/ISAI
+ * preconditioned BiCGSTAB solver (libbilu.so).
+ *
+ * Plain C: pointers, sizes and an opaque pattern handle.  Device pointers are
+ * HIP device addresses owned by the caller; `stream` is a hipStream_t passed
+ * as void* (NULL = the default stream).  Every entry point returns a status
+ * code (BILU_OK = 0) and never aborts the process.
+ *
+ * Reference interfaces these replace (aggarwal2000/Merging-ILU0,
+ * Batched-ILU0-merged/ILU0_ISAI_bicgstab/):
+ *   bilu_solve            <- Batched_conv_ILU_app1_Preconditioned_BiCGSTAB_merged_Gpu
+ *                            (PrecondBiCGSTAB.h:26-27, merged.cu:610-650), _segregated_
+ *                            (PrecondBiCGSTAB.h:29-30), Batched_ILU_Preconditioned_BiCGSTAB_Gpu
+ *                            (PrecondBiCGSTAB.h:15-17), Batched_ILU_ISAI_... (:19),
+ *                            Batched_General_ISAI_... (:21-23)
+ *   bilu_ilu0             <- ComputeILU0Approach1 (ILU_0.h:11) / ILU_0_Factorization_Gpu (ILU_0.h:9)
+ *   bilu_parilu0          <- ParILU_0_Factorization_Gpu (parILU_0.h:7)
+ *   bilu_split_lu         <- Update_row_pointers_L_and_U_and_Allocate_Memory +
+ *                            Fill_L_and_U_col_idxs_and_vals (factorization.h:23-26)
+ *   bilu_isai             <- GenerateISAI_gpu (isai.h:9), power 1
+ *   bilu_pattern_create   <- Count_Missing_Diagonal_Elements + Find_locations_of_diagonal_elements
+ *                            (factorization.h:10,17), done once per pattern on the host
+ *   bilu_true_residual    <- KernelFillTrueResNorms (Bicgstab_common.hpp:294-342)
+ */
+#ifndef BILU_H
+#define BILU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---- */
+#define BILU_OK 0
+#define BILU_ERR_INVALID_ARG 1
+#define BILU_ERR_INVALID_PATTERN 2   /* unsorted / out-of-range columns, bad row_ptrs */
+#define BILU_ERR_MISSING_DIAGONAL 3  /* the ILU paths need a full diagonal (merged.cu:557-560) */
+#define BILU_ERR_TOO_LARGE 4         /* n, nnz or ISAI row size beyond what the kernels support */
+#define BILU_ERR_HIP 5               /* a HIP runtime call failed */
+#define BILU_ERR_NO_DEVICE 6
+
+/* ---- per-system convergence flags (conv output); 1/-1 keep the reference meaning ---- */
+#define BILU_CONVERGED 1
+#define BILU_NOT_CONVERGED (-1)   /* max_iter reached or rho == 0 breakdown */
+#define BILU_ZERO_PIVOT (-2)      /* zero pivot in ILU(0) / zero diagonal in ISAI */
+#define BILU_NONFINITE (-3)       /* NaN/Inf reached the residual */
+
+/* ---- preconditioners (PrecondBiCGSTAB.h:15-30) ---- */
+#define BILU_PRECOND_ILU0 0           /* exact ILU(0), in place */
+#define BILU_PRECOND_PARILU0 1        /* ParILU(0) Jacobi sweeps from L=tril(A)+I, U=triu(A) */
+#define BILU_PRECOND_ILU0_ISAI 2      /* exact ILU(0), then ISAI(L), ISAI(U) on the factor pattern */
+#define BILU_PRECOND_PARILU0_ISAI 3   /* ParILU(0), then ISAI(L), ISAI(U) */
+#define BILU_PRECOND_GENERAL_ISAI 4   /* ISAI(A), Gauss-Jordan per row */
+#define BILU_PRECOND_NONE 5           /* identity (testing) */
+
+#define BILU_TOL_ABSOLUTE 0   /* ||r|| < tol (reference, ATOL = 1e-11) */
+#define BILU_TOL_RELATIVE 1   /* ||r|| < tol * ||b|| */
+
+#define BILU_MERGED 0         /* factor and solve in one launch (merged.cu) */
+#define BILU_SEGREGATED 1     /* factor launch, factors through HBM, solve launch (segregated.cu) */
+
+#define BILU_ISAI_GENERAL 0
+#define BILU_ISAI_LOWER 1     /* strictly-lower values + unit diagonal */
+#define BILU_ISAI_UPPER 2     /* diagonal + strictly-upper values */
+
+typedef struct bilu_pattern_s *bilu_pattern_t;
+
+typedef struct {
+    int precond;          /* BILU_PRECOND_* */
+    int parilu_sweeps;    /* sweeps for the ParILU variants (>= 0) */
+    double tol;           /* default 1e-11 */
+    int tol_mode;         /* BILU_TOL_* */
+    int max_iter;         /* full BiCGSTAB iterations, default 500 */
+    int merge;            /* BILU_MERGED / BILU_SEGREGATED */
+    int true_res_restart; /* 1: restart from b - A x when the true residual misses tol */
+    int max_restarts;     /* bound on such restarts (default 3) */
+} bilu_opts;
+
+typedef struct {
+    int n, nnz;
+    int l_nnz, u_nnz;          /* reference L/U layout sizes (L stores its unit diagonal) */
+    int levels_lower, levels_upper;
+    int ilu_updates;           /* number of (i,k,j) ILU(0) updates */
+    int max_row_nnz;
+} bilu_pattern_info;
+
+/* defaults: ILU(0), 5 sweeps, tol 1e-11 absolute, 500 iterations, merged, no restart */
+void bilu_opts_default(bilu_opts *o);
+const char *bilu_version(void);
+const char *bilu_status_string(int status);
+
+/* Host arrays; columns sorted within each row.  Analyses the pattern once
+ * (diagonal pointers, level schedules, ILU update lists, ParILU dependency
+ * lists, ISAI gather maps) and uploads it to the current device. */
+int bilu_pattern_create(int n, int nnz, const int *row_ptrs, const int *col_idx,
+                        bilu_pattern_t *out);
+int bilu_pattern_destroy(bilu_pattern_t p);
+int bilu_pattern_get_info(bilu_pattern_t p, bilu_pattern_info *info);
+/* host copies of the reference L/U split patterns (sizes from get_info) */
+int bilu_pattern_lu_split(bilu_pattern_t p, int *l_row_ptrs, int *l_col_idx,
+                          int *u_row_ptrs, int *u_col_idx);
+
+/* Batched solve A_s x_s = b_s, s < batch.  d_A[batch*nnz], d_b/d_x[batch*n],
+ * per-system outputs d_iter (half steps, float), d_conv, d_iter_res (last
+ * recursive residual norm), d_true_res (||b - A x||_2); d_iter_res and
+ * d_true_res may be NULL. */
+int bilu_solve(bilu_pattern_t p, int batch, const double *d_A, const double *d_b,
+               double *d_x, const bilu_opts *opts, float *d_iter, int *d_conv,
+               double *d_iter_res, double *d_true_res, void *stream);
+
+/* Exact ILU(0) in place on the pattern: d_F[batch*nnz] <- factors of d_A
+ * (strict lower = L, unit diagonal implicit; upper = U).  d_status[batch]
+ * (may be NULL) receives 0 or BILU_ZERO_PIVOT. */
+int bilu_ilu0(bilu_pattern_t p, int batch, const double *d_A, double *d_F,
+              int *d_status, void *stream);
+/* ParILU(0): `sweeps` Jacobi sweeps from d_F_init (NULL: from A) */
+int bilu_parilu0(bilu_pattern_t p, int batch, const double *d_A, const double *d_F_init,
+                 double *d_F, int sweeps, void *stream);
+/* ||(L U - A) on the pattern||_F per system */
+int bilu_factor_residual(bilu_pattern_t p, int batch, const double *d_A, const double *d_F,
+                         double *d_res, void *stream);
+/* in-place factor -> reference L (unit diagonal stored) and U value arrays */
+int bilu_split_lu(bilu_pattern_t p, int batch, const double *d_F, double *d_L, double *d_U,
+                  void *stream);
+/* ISAI (power 1) of the typed view of d_vals; output on the pattern masked
+ * to the type's triangle (other positions are left untouched). */
+int bilu_isai(bilu_pattern_t p, int batch, const double *d_vals, int type, double *d_out,
+              int *d_status, void *stream);
+/* ||b - A x||_2 per system */
+int bilu_true_residual(bilu_pattern_t p, int batch, const double *d_A, const double *d_b,
+                       const double *d_x, double *d_res, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,208 @@
+"""Python binding of libbilu (ctypes over the C ABI in include/bilu.h).
+
+Device buffers are torch tensors on a HIP device (torch is used only for
+device memory and streams).  The HIP library is required: if libbilu.so is
+missing or cannot be loaded, every entry point raises -- there is no CPU
+fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libbilu.so")
+
+OK = 0
+CONVERGED, NOT_CONVERGED, ZERO_PIVOT, NONFINITE = 1, -1, -2, -3
+ILU0, PARILU0, ILU0_ISAI, PARILU0_ISAI, GENERAL_ISAI, NONE = 0, 1, 2, 3, 4, 5
+TOL_ABSOLUTE, TOL_RELATIVE = 0, 1
+MERGED, SEGREGATED = 0, 1
+ISAI_GENERAL, ISAI_LOWER, ISAI_UPPER = 0, 1, 2
+
+# every symbol include/bilu.h declares
+EXPORTS = [
+    "bilu_opts_default", "bilu_version", "bilu_status_string", "bilu_pattern_create",
+    "bilu_pattern_destroy", "bilu_pattern_get_info", "bilu_pattern_lu_split", "bilu_solve",
+    "bilu_ilu0", "bilu_parilu0", "bilu_factor_residual", "bilu_split_lu", "bilu_isai",
+    "bilu_true_residual",
+]
+
+
+class BiluError(RuntimeError):
+    pass
+
+
+class Opts(C.Structure):
+    _fields_ = [
+        ("precond", C.c_int),
+        ("parilu_sweeps", C.c_int),
+        ("tol", C.c_double),
+        ("tol_mode", C.c_int),
+        ("max_iter", C.c_int),
+        ("merge", C.c_int),
+        ("true_res_restart", C.c_int),
+        ("max_restarts", C.c_int),
+    ]
+
+
+class PatternInfo(C.Structure):
+    _fields_ = [(k, C.c_int) for k in ("n", "nnz", "l_nnz", "u_nnz", "levels_lower",
+                                        "levels_upper", "ilu_updates", "max_row_nnz")]
+
+
+_lib = None
+
+
+def lib():
+    """Load libbilu.so (raises if it is missing: no fallback path exists)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise BiluError(f"libbilu.so not built: {LIB_PATH} (run __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        vp, i32p = C.c_void_p, C.POINTER(C.c_int)
+        L.bilu_opts_default.argtypes = [C.POINTER(Opts)]
+        L.bilu_opts_default.restype = None
+        L.bilu_version.restype = C.c_char_p
+        L.bilu_status_string.argtypes = [C.c_int]
+        L.bilu_status_string.restype = C.c_char_p
+        L.bilu_pattern_create.argtypes = [C.c_int, C.c_int, i32p, i32p, C.POINTER(vp)]
+        L.bilu_pattern_destroy.argtypes = [vp]
+        L.bilu_pattern_get_info.argtypes = [vp, C.POINTER(PatternInfo)]
+        L.bilu_pattern_lu_split.argtypes = [vp, i32p, i32p, i32p, i32p]
+        L.bilu_solve.argtypes = [vp, C.c_int, vp, vp, vp, C.POINTER(Opts), vp, vp, vp, vp, vp]
+        L.bilu_ilu0.argtypes = [vp, C.c_int, vp, vp, vp, vp]
+        L.bilu_parilu0.argtypes = [vp, C.c_int, vp, vp, vp, C.c_int, vp]
+        L.bilu_factor_residual.argtypes = [vp, C.c_int, vp, vp, vp, vp]
+        L.bilu_split_lu.argtypes = [vp, C.c_int, vp, vp, vp, vp]
+        L.bilu_isai.argtypes = [vp, C.c_int, vp, C.c_int, vp, vp, vp]
+        L.bilu_true_residual.argtypes = [vp, C.c_int, vp, vp, vp, vp, vp]
+        _lib = L
+    return _lib
+
+
+def _check(rc, what):
+    if rc != OK:
+        msg = lib().bilu_status_string(rc).decode()
+        raise BiluError(f"{what} failed: {rc} ({msg})")
+
+
+def opts(precond=ILU0, parilu_sweeps=5, tol=1e-11, tol_mode=TOL_ABSOLUTE, max_iter=500,
+         merge=MERGED, true_res_restart=0, max_restarts=3) -> Opts:
+    return Opts(precond, parilu_sweeps, tol, tol_mode, max_iter, merge, true_res_restart,
+                max_restarts)
+
+
+def _ptr(t):
+    if t is None:
+        return None
+    return C.c_void_p(t.data_ptr())
+
+
+def _stream(stream):
+    if stream is None:
+        import torch
+        stream = torch.cuda.current_stream()
+    return C.c_void_p(stream.cuda_stream)
+
+
+class Pattern:
+    """Handle of an analysed shared CSR pattern on the current device."""
+
+    def __init__(self, n, row_ptrs, col_idx):
+        rp = np.ascontiguousarray(row_ptrs, dtype=np.int32)
+        ci = np.ascontiguousarray(col_idx, dtype=np.int32)
+        self.n = int(n)
+        self.nnz = int(rp[-1])
+        self.row_ptrs, self.col_idx = rp, ci
+        h = C.c_void_p()
+        _check(lib().bilu_pattern_create(self.n, self.nnz, rp.ctypes.data_as(C.POINTER(C.c_int)),
+                                         ci.ctypes.data_as(C.POINTER(C.c_int)), C.byref(h)),
+               "bilu_pattern_create")
+        self.h = h
+
+    @classmethod
+    def from_synth(cls, pat):
+        return cls(pat.n, pat.row_ptrs, pat.col_idx)
+
+    def info(self) -> PatternInfo:
+        inf = PatternInfo()
+        _check(lib().bilu_pattern_get_info(self.h, C.byref(inf)), "bilu_pattern_get_info")
+        return inf
+
+    def lu_split(self):
+        inf = self.info()
+        lrp = np.empty(self.n + 1, np.int32)
+        lci = np.empty(inf.l_nnz, np.int32)
+        urp = np.empty(self.n + 1, np.int32)
+        uci = np.empty(inf.u_nnz, np.int32)
+        P = C.POINTER(C.c_int)
+        _check(lib().bilu_pattern_lu_split(self.h, lrp.ctypes.data_as(P), lci.ctypes.data_as(P),
+                                           urp.ctypes.data_as(P), uci.ctypes.data_as(P)),
+               "bilu_pattern_lu_split")
+        return lrp, lci, urp, uci
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().bilu_pattern_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def solve(pat: Pattern, A, b, x, o: Opts | None = None, iters=None, conv=None, iter_res=None,
+          true_res=None, stream=None):
+    """Batched solve on device tensors; returns (iters, conv, iter_res, true_res)."""
+    import torch
+    B = b.shape[0]
+    dev = b.device
+    if iters is None:
+        iters = torch.empty(B, dtype=torch.float32, device=dev)
+    if conv is None:
+        conv = torch.empty(B, dtype=torch.int32, device=dev)
+    if iter_res is None:
+        iter_res = torch.empty(B, dtype=torch.float64, device=dev)
+    if true_res is None:
+        true_res = torch.empty(B, dtype=torch.float64, device=dev)
+    o = o or opts()
+    _check(lib().bilu_solve(pat.h, B, _ptr(A), _ptr(b), _ptr(x), C.byref(o), _ptr(iters),
+                            _ptr(conv), _ptr(iter_res), _ptr(true_res), _stream(stream)),
+           "bilu_solve")
+    return iters, conv, iter_res, true_res
+
+
+def ilu0(pat: Pattern, A, F, status=None, stream=None):
+    _check(lib().bilu_ilu0(pat.h, A.shape[0], _ptr(A), _ptr(F), _ptr(status), _stream(stream)),
+           "bilu_ilu0")
+
+
+def parilu0(pat: Pattern, A, F, sweeps, F_init=None, stream=None):
+    _check(lib().bilu_parilu0(pat.h, A.shape[0], _ptr(A), _ptr(F_init), _ptr(F), sweeps,
+                              _stream(stream)), "bilu_parilu0")
+
+
+def factor_residual(pat: Pattern, A, F, res, stream=None):
+    _check(lib().bilu_factor_residual(pat.h, A.shape[0], _ptr(A), _ptr(F), _ptr(res),
+                                      _stream(stream)), "bilu_factor_residual")
+
+
+def split_lu(pat: Pattern, F, L, U, stream=None):
+    _check(lib().bilu_split_lu(pat.h, F.shape[0], _ptr(F), _ptr(L), _ptr(U), _stream(stream)),
+           "bilu_split_lu")
+
+
+def isai(pat: Pattern, vals, typ, out, status=None, stream=None):
+    _check(lib().bilu_isai(pat.h, vals.shape[0], _ptr(vals), typ, _ptr(out), _ptr(status),
+                           _stream(stream)), "bilu_isai")
+
+
+def true_residual(pat: Pattern, A, b, x, res, stream=None):
+    _check(lib().bilu_true_residual(pat.h, A.shape[0], _ptr(A), _ptr(b), _ptr(x), _ptr(res),
+                                    _stream(stream)), "bilu_true_residual")

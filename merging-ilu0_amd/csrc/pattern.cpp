@@ -1,0 +1,154 @@
+// Host-side pattern analysis (once per pattern, outside any timed region).
+//
+// The reference recomputes parts of this inside every solve call: the
+// diagonal locator (factorization.cu:392-480, with a one-thread prefix sum
+// and a blocking D2H), the L/U split (factorization.cu:487-575), the ParILU
+// dependency graph (parILU_0_gpu.cu:118-220, host) and the ISAI dense index
+// maps (isai_gpu.cu:209-287).  Here they are derived once from the host CSR
+// arrays and cached with the pattern handle.
+#include "pattern.hpp"
+
+#include <algorithm>
+
+#include "../../include/bilu.h"
+
+namespace bilu {
+
+static int find_pos(const std::vector<int> &rp, const std::vector<int> &ci, int r, int c)
+{
+    auto b = ci.begin() + rp[r], e = ci.begin() + rp[r + 1];
+    auto it = std::lower_bound(b, e, c);
+    return (it != e && *it == c) ? int(it - ci.begin()) : -1;
+}
+
+static void build_levels(int n, const std::vector<int> &lev, std::vector<int> &ptr,
+                         std::vector<int> &rows)
+{
+    int nlev = 0;
+    for (int i = 0; i < n; i++) nlev = std::max(nlev, lev[i] + 1);
+    ptr.assign(nlev + 1, 0);
+    for (int i = 0; i < n; i++) ptr[lev[i] + 1]++;
+    for (int l = 0; l < nlev; l++) ptr[l + 1] += ptr[l];
+    rows.assign(n, 0);
+    std::vector<int> fill(ptr.begin(), ptr.end() - 1);
+    for (int i = 0; i < n; i++) rows[fill[lev[i]]++] = i;
+}
+
+static void build_isai(const PatternAnalysis &pa, int type, IsaiMap &M)
+{
+    const int n = pa.n;
+    M.beg.assign(n, 0);
+    M.m.assign(n, 0);
+    M.goff.assign(n + 1, 0);
+    M.rhs_pos.assign(n, 0);
+    M.gather.clear();
+    M.max_m = 0;
+    for (int i = 0; i < n; i++) {
+        int b, e;
+        if (type == BILU_ISAI_LOWER) { b = pa.rp[i]; e = pa.diag[i] + 1; }
+        else if (type == BILU_ISAI_UPPER) { b = pa.diag[i]; e = pa.rp[i + 1]; }
+        else { b = pa.rp[i]; e = pa.rp[i + 1]; }
+        const int m = e - b;
+        M.beg[i] = b;
+        M.m[i] = m;
+        M.max_m = std::max(M.max_m, m);
+        for (int q = 0; q < m; q++)
+            if (pa.ci[b + q] == i) M.rhs_pos[i] = q;
+        for (int t = 0; t < m; t++) {
+            for (int k = 0; k < m; k++) {
+                const int r = pa.ci[b + k], c = pa.ci[b + t];  // entry (J[k], J[t])
+                int g;
+                if (type == BILU_ISAI_LOWER && r == c) g = -2;
+                else if (type == BILU_ISAI_LOWER && r < c) g = -1;
+                else if (type == BILU_ISAI_UPPER && r > c) g = -1;
+                else g = find_pos(pa.rp, pa.ci, r, c);
+                M.gather.push_back(g);
+            }
+        }
+        M.goff[i + 1] = int(M.gather.size());
+    }
+}
+
+int analyse_pattern(int n, int nnz, const int *rp, const int *ci, PatternAnalysis &pa,
+                    std::string &why)
+{
+    if (n <= 0 || nnz <= 0 || !rp || !ci) { why = "empty pattern"; return BILU_ERR_INVALID_ARG; }
+    if (rp[0] != 0 || rp[n] != nnz) { why = "row_ptrs[0] must be 0 and row_ptrs[n] == nnz"; return BILU_ERR_INVALID_PATTERN; }
+    for (int i = 0; i < n; i++) {
+        if (rp[i + 1] < rp[i]) { why = "row_ptrs not monotone"; return BILU_ERR_INVALID_PATTERN; }
+        for (int p = rp[i]; p < rp[i + 1]; p++) {
+            if (ci[p] < 0 || ci[p] >= n) { why = "column index out of range"; return BILU_ERR_INVALID_PATTERN; }
+            if (p > rp[i] && ci[p] <= ci[p - 1]) { why = "columns not strictly increasing within a row"; return BILU_ERR_INVALID_PATTERN; }
+        }
+    }
+    pa.n = n;
+    pa.nnz = nnz;
+    pa.rp.assign(rp, rp + n + 1);
+    pa.ci.assign(ci, ci + nnz);
+    pa.diag.assign(n, -1);
+    pa.max_row = 0;
+    for (int i = 0; i < n; i++) {
+        pa.diag[i] = find_pos(pa.rp, pa.ci, i, i);
+        pa.max_row = std::max(pa.max_row, rp[i + 1] - rp[i]);
+        if (pa.diag[i] < 0) { why = "missing diagonal entry in row " + std::to_string(i); return BILU_ERR_MISSING_DIAGONAL; }
+    }
+    // level schedules
+    std::vector<int> lev(n, 0);
+    for (int i = 0; i < n; i++) {
+        int l = 0;
+        for (int p = rp[i]; p < pa.diag[i]; p++) l = std::max(l, lev[ci[p]] + 1);
+        lev[i] = l;
+    }
+    build_levels(n, lev, pa.lev_l_ptr, pa.lev_l_rows);
+    std::fill(lev.begin(), lev.end(), 0);
+    for (int i = n - 1; i >= 0; i--) {
+        int l = 0;
+        for (int p = pa.diag[i] + 1; p < rp[i + 1]; p++) l = std::max(l, lev[ci[p]] + 1);
+        lev[i] = l;
+    }
+    build_levels(n, lev, pa.lev_u_ptr, pa.lev_u_rows);
+    // ILU(0) update lists
+    pa.upd_ptr.assign(nnz + 1, 0);
+    pa.upd.clear();
+    for (int i = 0; i < n; i++) {
+        for (int p = rp[i]; p < rp[i + 1]; p++) {
+            const int k = ci[p];
+            if (k < i) {
+                for (int q = pa.diag[k] + 1; q < rp[k + 1]; q++) {
+                    const int dst = find_pos(pa.rp, pa.ci, i, ci[q]);
+                    if (dst >= 0) { pa.upd.push_back(dst); pa.upd.push_back(q); }
+                }
+            }
+            pa.upd_ptr[p + 1] = int(pa.upd.size() / 2);
+        }
+    }
+    // ParILU(0) dependency lists
+    pa.dep_ptr.assign(nnz + 1, 0);
+    pa.dep.clear();
+    for (int i = 0; i < n; i++) {
+        for (int e = rp[i]; e < rp[i + 1]; e++) {
+            const int j = ci[e];
+            const int kmax = std::min(i, j) - 1;
+            for (int p = rp[i]; p < pa.diag[i] && ci[p] <= kmax; p++) {
+                const int q = find_pos(pa.rp, pa.ci, ci[p], j);
+                if (q >= 0) { pa.dep.push_back(p); pa.dep.push_back(q); }
+            }
+            pa.dep_ptr[e + 1] = int(pa.dep.size() / 2);
+        }
+    }
+    // reference L/U split (factorization.cu:216-344)
+    pa.l_rp.assign(n + 1, 0);
+    pa.u_rp.assign(n + 1, 0);
+    pa.l_ci.clear(); pa.l_src.clear(); pa.u_ci.clear(); pa.u_src.clear();
+    for (int i = 0; i < n; i++) {
+        for (int p = rp[i]; p < pa.diag[i]; p++) { pa.l_ci.push_back(ci[p]); pa.l_src.push_back(p); }
+        pa.l_ci.push_back(i); pa.l_src.push_back(-1);
+        for (int p = pa.diag[i]; p < rp[i + 1]; p++) { pa.u_ci.push_back(ci[p]); pa.u_src.push_back(p); }
+        pa.l_rp[i + 1] = int(pa.l_ci.size());
+        pa.u_rp[i + 1] = int(pa.u_ci.size());
+    }
+    for (int t = 0; t < 3; t++) build_isai(pa, t, pa.isai[t]);
+    return BILU_OK;
+}
+
+}  // namespace bilu

@@ -1,0 +1,999 @@
+// Batched shared-pattern ILU(0) / ParILU(0) / ISAI preconditioned BiCGSTAB
+// for gfx950 (MI355X), and the C ABI declared in include/bilu.h.
+//
+// Execution model: one wave64 workgroup per system (grid = batch).  The
+// system's matrix values, factors and the vectors that are gathered from
+// (SpMV / triangular-solve inputs) live in LDS; the Krylov vectors live in
+// VGPRs in a fixed "row-cyclic" distribution (element i in lane i%64,
+// register slot i/64), so axpys are register-only and dot products are a
+// per-lane strided sum followed by a 6-step butterfly -- no LDS, no
+// barrier.  Scalars (rho, alpha, omega, norms) come out of the butterfly
+// identical in every lane, so all control flow is wave-uniform.
+// Pattern-derived schedules (level sets, ILU(0) update lists, ParILU
+// dependency lists, ISAI gather maps) are computed once on the host
+// (pattern.cpp) and read from global memory (L1/L2 resident: a few KB).
+//
+// Arithmetic order is fixed and mirrors the CPU oracle bit for bit:
+// explicit fma() everywhere a product is accumulated, compiled with
+// -ffp-contract=off; SpMV row sums and triangular-solve sums run in CSR
+// order (L left->right, U right->left as merged.cu:43-51,85-95); dots sum
+// idx = lane, lane+64, ... per lane and then combine lanes with a butterfly
+// whose lane-0 association equals a shfl_down tree (oracle dot width 64).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/bilu.h"
+#include "pattern.hpp"
+
+namespace {
+
+constexpr int WAVE = 64;
+constexpr int MAX_NR = 5;        // rows per lane in the register layout: n <= 320
+constexpr int ISAI_GROUP = 32;   // lanes per ISAI row (reference row_size_limit 32)
+
+enum PcKind { PC_LU = 0, PC_ISAI_LU = 1, PC_ISAI_GEN = 2, PC_NONE = 3 };
+enum FactorKind { FK_ILU = 0, FK_PARILU = 1, FK_LOAD = 2, FK_NONE = 3 };
+
+struct DevIsai {
+    const int *beg, *m, *goff, *gather, *rhs;
+};
+
+struct DevPattern {
+    int n, nnz, nlev_l, nlev_u;
+    const int *rp, *ci, *diag, *erow;
+    const int *lev_l_ptr, *lev_l_rows, *lev_u_ptr, *lev_u_rows;
+    const int *upd_ptr;
+    const int2 *upd;
+    const int *dep_ptr;
+    const int2 *dep;
+    const int *l_src, *u_src;
+    int l_nnz, u_nnz;
+    DevIsai isai[3];
+};
+
+struct SolveParams {
+    double tol;
+    int tol_mode, max_iter, restart, max_restarts;
+    int factor, sweeps;
+};
+
+// ---------------------------------------------------------------- wave ops
+__device__ __forceinline__ double wave_sum(double v)
+{
+    // butterfly; lane 0 associates exactly like a shfl_down tree and every
+    // lane ends with the same bits (IEEE addition is commutative)
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = v + __shfl_xor(v, off, WAVE);
+    return v;
+}
+
+template <int NR>
+struct Vec {
+    double e[NR];
+};
+
+template <int NR>
+__device__ __forceinline__ double vdot(const Vec<NR> &a, const Vec<NR> &b, int lane, int n)
+{
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < NR; k++)
+        if (lane + WAVE * k < n) s = fma(a.e[k], b.e[k], s);
+    return wave_sum(s);
+}
+
+template <int NR>
+__device__ __forceinline__ void to_lds(const Vec<NR> &v, double *dst, int lane, int n)
+{
+#pragma unroll
+    for (int k = 0; k < NR; k++)
+        if (lane + WAVE * k < n) dst[lane + WAVE * k] = v.e[k];
+}
+
+template <int NR>
+__device__ __forceinline__ void from_lds(Vec<NR> &v, const double *src, int lane, int n)
+{
+#pragma unroll
+    for (int k = 0; k < NR; k++) v.e[k] = (lane + WAVE * k < n) ? src[lane + WAVE * k] : 0.0;
+}
+
+// y = A x, rows in the register layout; A over positions [rp[i], rp[i+1])
+template <int NR>
+__device__ __forceinline__ void spmv(const DevPattern &P, const double *A, const double *x,
+                                     Vec<NR> &y, int lane)
+{
+#pragma unroll
+    for (int k = 0; k < NR; k++) {
+        const int i = lane + WAVE * k;
+        double acc = 0.0;
+        if (i < P.n) {
+            const int e = P.rp[i + 1];
+            for (int p = P.rp[i]; p < e; p++) acc = fma(A[p], x[P.ci[p]], acc);
+        }
+        y.e[k] = acc;
+    }
+}
+
+// out = U^{-1} L^{-1} vin on the in-place factor F, level by level
+__device__ void lu_apply(const DevPattern &P, const double *F, const double *vin, double *y,
+                         double *out, int lane)
+{
+    for (int l = 0; l < P.nlev_l; l++) {
+        const int b = P.lev_l_ptr[l], e = P.lev_l_ptr[l + 1];
+        for (int idx = b + lane; idx < e; idx += WAVE) {
+            const int i = P.lev_l_rows[idx];
+            double acc = 0.0;
+            const int pd = P.diag[i];
+            for (int p = P.rp[i]; p < pd; p++) acc = fma(F[p], y[P.ci[p]], acc);
+            y[i] = vin[i] - acc;
+        }
+        __syncthreads();
+    }
+    for (int l = 0; l < P.nlev_u; l++) {
+        const int b = P.lev_u_ptr[l], e = P.lev_u_ptr[l + 1];
+        for (int idx = b + lane; idx < e; idx += WAVE) {
+            const int i = P.lev_u_rows[idx];
+            double acc = 0.0;
+            const int pd = P.diag[i];
+            for (int p = P.rp[i + 1] - 1; p > pd; p--) acc = fma(F[p], out[P.ci[p]], acc);
+            out[i] = (y[i] - acc) / F[pd];
+        }
+        __syncthreads();
+    }
+}
+
+// out = aiU (aiL vin); aiL on [rp[i], diag[i]] of isL, aiU on [diag[i], rp[i+1])
+// of isU except its diagonal, which is kept in udiag[i]
+__device__ void isai_lu_apply(const DevPattern &P, const double *isL, const double *isU,
+                              const double *udiag, const double *vin, double *tmp, double *out,
+                              int lane)
+{
+    for (int i = lane; i < P.n; i += WAVE) {
+        double acc = 0.0;
+        const int pd = P.diag[i];
+        for (int p = P.rp[i]; p <= pd; p++) acc = fma(isL[p], vin[P.ci[p]], acc);
+        tmp[i] = acc;
+    }
+    __syncthreads();
+    for (int i = lane; i < P.n; i += WAVE) {
+        const int pd = P.diag[i];
+        double acc = fma(udiag[i], tmp[i], 0.0);
+        const int e = P.rp[i + 1];
+        for (int p = pd + 1; p < e; p++) acc = fma(isU[p], tmp[P.ci[p]], acc);
+        out[i] = acc;
+    }
+    __syncthreads();
+}
+
+__device__ void spmv_lds(const DevPattern &P, const double *M, const double *vin, double *out,
+                         int lane)
+{
+    for (int i = lane; i < P.n; i += WAVE) {
+        double acc = 0.0;
+        const int e = P.rp[i + 1];
+        for (int p = P.rp[i]; p < e; p++) acc = fma(M[p], vin[P.ci[p]], acc);
+        out[i] = acc;
+    }
+    __syncthreads();
+}
+
+// exact ILU(0) in place, rows level by level (IKJ order == the reference KIJ
+// order per entry, merged.cu:145-246); returns nonzero on a zero pivot
+__device__ int ilu_factor(const DevPattern &P, double *F, int lane)
+{
+    int bad = 0;
+    for (int l = 0; l < P.nlev_l; l++) {
+        const int b = P.lev_l_ptr[l], e = P.lev_l_ptr[l + 1];
+        for (int idx = b + lane; idx < e; idx += WAVE) {
+            const int i = P.lev_l_rows[idx];
+            const int pd = P.diag[i];
+            for (int p = P.rp[i]; p < pd; p++) {
+                const double ukk = F[P.diag[P.ci[p]]];
+                bad |= (ukk == 0.0);
+                const double lik = F[p] / ukk;
+                F[p] = lik;
+                const int ue = P.upd_ptr[p + 1];
+                for (int u = P.upd_ptr[p]; u < ue; u++) {
+                    const int2 d = P.upd[u];
+                    F[d.x] = fma(-lik, F[d.y], F[d.x]);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    for (int i = lane; i < P.n; i += WAVE) bad |= (F[P.diag[i]] == 0.0);
+    return __any(bad);
+}
+
+// Jacobi ParILU(0) sweeps (parILU_0_gpu.cu:18-57 formula); F holds the
+// initial guess on entry, the result on exit; W is scratch of nnz doubles
+__device__ void parilu_sweeps(const DevPattern &P, const double *A, double *F, double *W,
+                              int sweeps, int lane)
+{
+    for (int s = 0; s < sweeps; s++) {
+        for (int e = lane; e < P.nnz; e += WAVE) W[e] = F[e];
+        __syncthreads();
+        for (int e = lane; e < P.nnz; e += WAVE) {
+            const int i = P.erow[e], j = P.ci[e];
+            double sum = 0.0;
+            const int de = P.dep_ptr[e + 1];
+            for (int d = P.dep_ptr[e]; d < de; d++) {
+                const int2 pq = P.dep[d];
+                sum = fma(W[pq.x], W[pq.y], sum);
+            }
+            F[e] = (i > j) ? (A[e] - sum) / W[P.diag[j]] : A[e] - sum;
+        }
+        __syncthreads();
+    }
+}
+
+// ISAI (power 1) rows, two rows per wave (32 lanes each, lane t owns dense
+// column t of the block: the reference's "implicit transpose",
+// isai_gpu.cu:559-626).  Lower/upper: column-sweep triangular solves
+// (isai_gpu.cu:325-389); general: Gauss-Jordan with the reference pivot
+// choice (isai_gpu.cu:417-554).  vals: typed view input; out[pos] for the
+// row's positions (diagonal of UPPER goes to udiag when udiag != nullptr).
+__device__ __forceinline__ double view_val(const double *vals, int g)
+{
+    return g >= 0 ? vals[g] : (g == -2 ? 1.0 : 0.0);
+}
+
+__device__ int isai_tri(const DevPattern &P, int type, const double *vals, double *out,
+                        double *udiag, int lane)
+{
+    const DevIsai &M = P.isai[type];
+    const int grp = lane / ISAI_GROUP, t = lane % ISAI_GROUP;
+    int bad = 0;
+    for (int i0 = 0; i0 < P.n; i0 += 2) {
+        const int i = i0 + grp;
+        const bool act = i < P.n;
+        const int m = act ? M.m[i] : 0;
+        const int mo = __shfl_xor(m, ISAI_GROUP, WAVE);
+        const int mmax = m > mo ? m : mo;
+        const int *g = act ? M.gather + M.goff[i] : M.gather;
+        double s;
+        if (type == BILU_ISAI_LOWER) {
+            s = (t == m - 1) ? 1.0 : 0.0;
+            for (int c = mmax - 1; c >= 0; c--) {
+                const bool on = c < m;
+                if (on && t == c) {
+                    const double d = view_val(vals, g[t * m + t]);
+                    bad |= (d == 0.0);
+                    s = s / d;
+                }
+                const double bot = __shfl(s, c, ISAI_GROUP);
+                if (on && t < c) s = fma(-bot, view_val(vals, g[t * m + c]), s);
+            }
+        } else {
+            s = (t == 0) ? 1.0 : 0.0;
+            for (int c = 0; c < mmax; c++) {
+                const bool on = c < m;
+                if (on && t == c) {
+                    const double d = view_val(vals, g[t * m + t]);
+                    bad |= (d == 0.0);
+                    s = s / d;
+                }
+                const double top = __shfl(s, c, ISAI_GROUP);
+                if (on && t > c && t < m) s = fma(-top, view_val(vals, g[t * m + c]), s);
+            }
+        }
+        if (act && t < m) {
+            const int pos = M.beg[i] + t;
+            if (udiag != nullptr && t == 0) udiag[i] = s;
+            else out[pos] = s;
+        }
+    }
+    __syncthreads();
+    return __any(bad);
+}
+
+__device__ int isai_general(const DevPattern &P, const double *vals, double *out, int lane)
+{
+    const DevIsai &M = P.isai[BILU_ISAI_GENERAL];
+    const int grp = lane / ISAI_GROUP, t = lane % ISAI_GROUP;
+    int bad = 0;
+    for (int i0 = 0; i0 < P.n; i0 += 2) {
+        const int i = i0 + grp;
+        const bool act = i < P.n;
+        const int m = act ? M.m[i] : 0;
+        const int *g = act ? M.gather + M.goff[i] : M.gather;
+        double row[ISAI_GROUP];
+#pragma unroll
+        for (int k = 0; k < ISAI_GROUP; k++)
+            row[k] = (t < m && k < m) ? view_val(vals, g[t * m + k]) : 0.0;
+        double rhs = (act && t == M.rhs[i]) ? 1.0 : 0.0;
+#pragma unroll
+        for (int dp = 0; dp < ISAI_GROUP; dp++) {
+            const bool on = dp < m;  // uniform within the group
+            // pivot: reference 32-lane shfl_down tree (choose_pivot_row_1)
+            const double dval = __shfl(row[dp], dp, ISAI_GROUP);
+            double val = (t >= m || t < dp) ? dval : row[dp];
+            int piv = (t >= m || t < dp) ? dp : t;
+#pragma unroll
+            for (int off = ISAI_GROUP / 2; off > 0; off >>= 1) {
+                const double vo = __shfl_down(val, off, ISAI_GROUP);
+                const int po = __shfl_down(piv, off, ISAI_GROUP);
+                if (fabs(vo) > fabs(val)) { val = vo; piv = po; }
+            }
+            piv = __shfl(piv, 0, ISAI_GROUP);
+            if (!on) piv = dp;
+            if (piv != dp) {
+#pragma unroll
+                for (int c = 0; c < ISAI_GROUP; c++) {
+                    const double a = __shfl(row[c], dp, ISAI_GROUP);
+                    const double b = __shfl(row[c], piv, ISAI_GROUP);
+                    if (t == dp) row[c] = b;
+                    if (t == piv) row[c] = a;
+                }
+                const double a = __shfl(rhs, dp, ISAI_GROUP);
+                const double b = __shfl(rhs, piv, ISAI_GROUP);
+                if (t == dp) rhs = b;
+                if (t == piv) rhs = a;
+            }
+            const double de = __shfl(row[dp], dp, ISAI_GROUP);
+            if (on) {
+                bad |= (de == 0.0);
+                const double mult = row[dp] / de;
+#pragma unroll
+                for (int c = 0; c < ISAI_GROUP; c++) {
+                    const double key = __shfl(row[c], dp, ISAI_GROUP);
+                    if (t != dp && c < m) row[c] = fma(-mult, key, row[c]);
+                }
+                const double rk = __shfl(rhs, dp, ISAI_GROUP);
+                if (t != dp) rhs = fma(-mult, rk, rhs);
+            }
+        }
+        double d = 1.0;
+#pragma unroll
+        for (int k = 0; k < ISAI_GROUP; k++)
+            if (k == t) d = row[k];
+        if (act && t < m) out[M.beg[i] + t] = rhs / d;
+    }
+    __syncthreads();
+    return __any(bad);
+}
+
+// ------------------------------------------------------------- the solver
+// LDS layout (doubles): sA[nnz] | sF[nnz] | sW[nnz or 0] | sU[n or 0] | e0[n] e1[n] e2[n]
+template <int NR, int PC>
+__global__ __launch_bounds__(WAVE) void k_solve(DevPattern P, int batch, const double *__restrict__ gA,
+                                               const double *__restrict__ gb, double *__restrict__ gx,
+                                               const double *__restrict__ gF, SolveParams sp,
+                                               float *__restrict__ g_iter, int *__restrict__ g_conv,
+                                               double *__restrict__ g_ires, double *__restrict__ g_tres)
+{
+    extern __shared__ double lds[];
+    const int sys = blockIdx.x;
+    if (sys >= batch) return;
+    const int lane = threadIdx.x;
+    const int n = P.n, nnz = P.nnz;
+    const bool need_w = (sp.factor == FK_PARILU) || PC == PC_ISAI_LU || PC == PC_ISAI_GEN;
+    double *sA = lds;
+    double *sF = sA + nnz;
+    double *sW = sF + nnz;
+    double *sU = sW + (need_w ? nnz : 0);
+    double *e0 = sU + (PC == PC_ISAI_LU ? n : 0);
+    double *e1 = e0 + n;
+    double *e2 = e1 + n;
+
+    const double *Ap = gA + (size_t)sys * nnz;
+    for (int e = lane; e < nnz; e += WAVE) sA[e] = Ap[e];
+    if (sp.factor == FK_LOAD) {
+        const double *Fp = gF + (size_t)sys * nnz;
+        for (int e = lane; e < nnz; e += WAVE) sF[e] = Fp[e];
+    } else if (PC != PC_NONE) {
+        for (int e = lane; e < nnz; e += WAVE) sF[e] = Ap[e];
+    }
+    Vec<NR> b, x, r, rh, p, v, s, t, ph, sh;
+    const double *bp = gb + (size_t)sys * n;
+#pragma unroll
+    for (int k = 0; k < NR; k++) {
+        const int i = lane + WAVE * k;
+        b.e[k] = (i < n) ? bp[i] : 0.0;
+    }
+    __syncthreads();
+
+    // ---- preconditioner generation (fused: no HBM round trip) ----
+    int bad = 0;
+    if (PC == PC_LU || PC == PC_ISAI_LU) {
+        if (sp.factor == FK_ILU) bad = ilu_factor(P, sF, lane);
+        else if (sp.factor == FK_PARILU) parilu_sweeps(P, sA, sF, sW, sp.sweeps, lane);
+    }
+    if (PC == PC_ISAI_LU && !bad) {
+        bad |= isai_tri(P, BILU_ISAI_LOWER, sF, sW, nullptr, lane);
+        bad |= isai_tri(P, BILU_ISAI_UPPER, sF, sW, sU, lane);
+    } else if (PC == PC_ISAI_GEN) {
+        bad |= isai_general(P, sA, sW, lane);
+    }
+
+    float iter = 0.0f;
+    int conv = BILU_NOT_CONVERGED;
+    double iter_res = 0.0;
+#pragma unroll
+    for (int k = 0; k < NR; k++) {
+        x.e[k] = 0.0; v.e[k] = 0.0; p.e[k] = 0.0; r.e[k] = b.e[k]; rh.e[k] = b.e[k];
+        ph.e[k] = 0.0; sh.e[k] = 0.0; s.e[k] = 0.0; t.e[k] = 0.0;
+    }
+    const double bnorm = sqrt(vdot(b, b, lane, n));
+    const double res0 = bnorm;  // r0 = b - A*0
+    const double tol_abs = (sp.tol_mode == BILU_TOL_RELATIVE) ? sp.tol * bnorm : sp.tol;
+    iter_res = res0;
+
+    auto apply = [&](const Vec<NR> &in, Vec<NR> &out) {
+        if (PC == PC_NONE) {
+            to_lds(in, e2, lane, n);
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < NR; k++) out.e[k] = in.e[k];
+            return;
+        }
+        to_lds(in, e0, lane, n);
+        __syncthreads();
+        if (PC == PC_LU) lu_apply(P, sF, e0, e1, e2, lane);
+        else if (PC == PC_ISAI_LU) isai_lu_apply(P, sW, sW, sU, e0, e1, e2, lane);
+        else spmv_lds(P, sW, e0, e2, lane);
+        from_lds(out, e2, lane, n);
+    };
+
+    if (bad) {
+        conv = BILU_ZERO_PIVOT;
+        iter_res = 0.0;
+    } else if (!isfinite(bnorm)) {
+        conv = BILU_NONFINITE;
+    } else if (bnorm == 0.0) {
+        conv = BILU_CONVERGED;
+        iter_res = 0.0;
+    } else if (res0 < tol_abs) {
+        conv = BILU_CONVERGED;
+    } else {
+        double rho_old = 1.0, omega_old = 1.0, alpha = 1.0;
+        int restarts = 0;
+        while (iter < (float)sp.max_iter) {
+            const double rho_new = vdot(r, rh, lane, n);
+            if (rho_new == 0.0) break;
+            const double beta = (rho_new / rho_old) * (alpha / omega_old);
+#pragma unroll
+            for (int k = 0; k < NR; k++) p.e[k] = fma(beta, fma(-omega_old, v.e[k], p.e[k]), r.e[k]);
+            apply(p, ph);            // ph also sits in e2 for the gather
+            spmv(P, sA, e2, v, lane);
+            alpha = rho_new / vdot(rh, v, lane, n);
+#pragma unroll
+            for (int k = 0; k < NR; k++) s.e[k] = fma(-alpha, v.e[k], r.e[k]);
+            iter_res = sqrt(vdot(s, s, lane, n));
+            iter += 0.5f;
+            if (!isfinite(iter_res)) { conv = BILU_NONFINITE; break; }
+            const bool half_conv = iter_res < tol_abs;
+            if (half_conv) {
+#pragma unroll
+                for (int k = 0; k < NR; k++) x.e[k] = fma(alpha, ph.e[k], x.e[k]);
+            } else {
+                __syncthreads();     // e2 (ph) reads by spmv done before reuse
+                apply(s, sh);
+                spmv(P, sA, e2, t, lane);
+                const double ts = vdot(t, s, lane, n);
+                const double tt = vdot(t, t, lane, n);
+                const double omega_new = ts / tt;
+#pragma unroll
+                for (int k = 0; k < NR; k++) x.e[k] = fma(omega_new, sh.e[k], fma(alpha, ph.e[k], x.e[k]));
+                iter += 0.5f;
+#pragma unroll
+                for (int k = 0; k < NR; k++) r.e[k] = fma(-omega_new, t.e[k], s.e[k]);
+                iter_res = sqrt(vdot(r, r, lane, n));
+                rho_old = rho_new;
+                omega_old = omega_new;
+                if (!isfinite(iter_res)) { conv = BILU_NONFINITE; break; }
+            }
+            __syncthreads();
+            if (half_conv || iter_res < tol_abs) {
+                conv = BILU_CONVERGED;
+                if (!sp.restart || restarts >= sp.max_restarts) break;
+                // true residual b - A x; restart from it if it misses tol
+                to_lds(x, e0, lane, n);
+                __syncthreads();
+                Vec<NR> w;
+                spmv(P, sA, e0, w, lane);
+#pragma unroll
+                for (int k = 0; k < NR; k++) w.e[k] = b.e[k] - w.e[k];
+                const double tr = sqrt(vdot(w, w, lane, n));
+                __syncthreads();
+                if (tr < tol_abs) break;
+                restarts++;
+                conv = BILU_NOT_CONVERGED;
+#pragma unroll
+                for (int k = 0; k < NR; k++) { r.e[k] = w.e[k]; rh.e[k] = w.e[k]; p.e[k] = 0.0; v.e[k] = 0.0; }
+                rho_old = 1.0; omega_old = 1.0; alpha = 1.0;
+                iter_res = tr;
+            }
+        }
+    }
+    if (conv == BILU_ZERO_PIVOT) {
+#pragma unroll
+        for (int k = 0; k < NR; k++) x.e[k] = 0.0;
+    }
+    // fused epilogue: true residual ||b - A x|| from the LDS-resident A
+    __syncthreads();
+    to_lds(x, e0, lane, n);
+    __syncthreads();
+    Vec<NR> w;
+    spmv(P, sA, e0, w, lane);
+#pragma unroll
+    for (int k = 0; k < NR; k++) w.e[k] = b.e[k] - w.e[k];
+    const double tres = sqrt(vdot(w, w, lane, n));
+    double *xp = gx + (size_t)sys * n;
+#pragma unroll
+    for (int k = 0; k < NR; k++) {
+        const int i = lane + WAVE * k;
+        if (i < n) xp[i] = x.e[k];
+    }
+    if (lane == 0) {
+        g_iter[sys] = iter;
+        g_conv[sys] = conv;
+        if (g_ires) g_ires[sys] = iter_res;
+        if (g_tres) g_tres[sys] = tres;
+    }
+}
+
+// factor-only kernel (segregated path and the factor ABI):
+// F <- ILU(0)(A) or ParILU sweeps from F_init (or A)
+__global__ __launch_bounds__(WAVE) void k_factor(DevPattern P, int batch, const double *__restrict__ gA,
+                                                const double *__restrict__ gInit, double *__restrict__ gF,
+                                                int kind, int sweeps, int *__restrict__ g_status)
+{
+    extern __shared__ double lds[];
+    const int sys = blockIdx.x;
+    if (sys >= batch) return;
+    const int lane = threadIdx.x, nnz = P.nnz;
+    double *sA = lds, *sF = lds + nnz, *sW = sF + nnz;
+    const double *Ap = gA + (size_t)sys * nnz;
+    const double *Ip = gInit ? gInit + (size_t)sys * nnz : Ap;
+    for (int e = lane; e < nnz; e += WAVE) { sA[e] = Ap[e]; sF[e] = Ip[e]; }
+    __syncthreads();
+    int bad = 0;
+    if (kind == FK_ILU) bad = ilu_factor(P, sF, lane);
+    else parilu_sweeps(P, sA, sF, sW, sweeps, lane);
+    __syncthreads();
+    double *Fp = gF + (size_t)sys * nnz;
+    for (int e = lane; e < nnz; e += WAVE) Fp[e] = sF[e];
+    if (g_status && lane == 0) g_status[sys] = bad ? BILU_ZERO_PIVOT : 0;
+}
+
+// ISAI of a batch of typed views; out positions outside the view untouched
+__global__ __launch_bounds__(WAVE) void k_isai(DevPattern P, int batch, const double *__restrict__ gV,
+                                              int type, double *__restrict__ gOut,
+                                              int *__restrict__ g_status)
+{
+    extern __shared__ double lds[];
+    const int sys = blockIdx.x;
+    if (sys >= batch) return;
+    const int lane = threadIdx.x, nnz = P.nnz;
+    double *sV = lds, *sO = lds + nnz;
+    const double *Vp = gV + (size_t)sys * nnz;
+    double *Op = gOut + (size_t)sys * nnz;
+    for (int e = lane; e < nnz; e += WAVE) { sV[e] = Vp[e]; sO[e] = Op[e]; }
+    __syncthreads();
+    int bad;
+    if (type == BILU_ISAI_GENERAL) bad = isai_general(P, sV, sO, lane);
+    else bad = isai_tri(P, type, sV, sO, nullptr, lane);
+    __syncthreads();
+    for (int e = lane; e < nnz; e += WAVE) Op[e] = sO[e];
+    if (g_status && lane == 0) g_status[sys] = bad ? BILU_ZERO_PIVOT : 0;
+}
+
+// ||(L U - A)|_P||_F per system, F in place (unit-diagonal L)
+__global__ __launch_bounds__(WAVE) void k_factor_residual(DevPattern P, int batch, const double *__restrict__ gA,
+                                                         const double *__restrict__ gF, double *__restrict__ gRes)
+{
+    extern __shared__ double lds[];
+    const int sys = blockIdx.x;
+    if (sys >= batch) return;
+    const int lane = threadIdx.x, nnz = P.nnz;
+    double *sF = lds;
+    const double *Fp = gF + (size_t)sys * nnz;
+    const double *Ap = gA + (size_t)sys * nnz;
+    for (int e = lane; e < nnz; e += WAVE) sF[e] = Fp[e];
+    __syncthreads();
+    double acc = 0.0;
+    for (int e = lane; e < nnz; e += WAVE) {
+        const int i = P.erow[e], j = P.ci[e];
+        double sum = 0.0;
+        const int de = P.dep_ptr[e + 1];
+        for (int d = P.dep_ptr[e]; d < de; d++) {
+            const int2 pq = P.dep[d];
+            sum = fma(sF[pq.x], sF[pq.y], sum);
+        }
+        const double lu = (i > j) ? fma(sF[e], sF[P.diag[j]], sum) : sum + sF[e];
+        const double d = lu - Ap[e];
+        acc = fma(d, d, acc);
+    }
+    acc = wave_sum(acc);
+    if (lane == 0) gRes[sys] = sqrt(acc);
+}
+
+__global__ void k_split_lu(DevPattern P, int batch, const double *__restrict__ gF,
+                           double *__restrict__ gL, double *__restrict__ gU)
+{
+    const size_t tot_l = (size_t)batch * P.l_nnz, tot_u = (size_t)batch * P.u_nnz;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < tot_l || q < tot_u; q += stride) {
+        if (q < tot_l && gL) {
+            const size_t s = q / P.l_nnz;
+            const int src = P.l_src[q % P.l_nnz];
+            gL[q] = src < 0 ? 1.0 : gF[s * P.nnz + src];
+        }
+        if (q < tot_u && gU) {
+            const size_t s = q / P.u_nnz;
+            gU[q] = gF[s * P.nnz + P.u_src[q % P.u_nnz]];
+        }
+    }
+}
+
+template <int NR>
+__global__ __launch_bounds__(WAVE) void k_true_residual(DevPattern P, int batch, const double *__restrict__ gA,
+                                                       const double *__restrict__ gb, const double *__restrict__ gx,
+                                                       double *__restrict__ gRes)
+{
+    extern __shared__ double lds[];
+    const int sys = blockIdx.x;
+    if (sys >= batch) return;
+    const int lane = threadIdx.x, n = P.n, nnz = P.nnz;
+    double *sA = lds, *sx = lds + nnz;
+    for (int e = lane; e < nnz; e += WAVE) sA[e] = gA[(size_t)sys * nnz + e];
+    for (int i = lane; i < n; i += WAVE) sx[i] = gx[(size_t)sys * n + i];
+    __syncthreads();
+    Vec<NR> w;
+    spmv(P, sA, sx, w, lane);
+#pragma unroll
+    for (int k = 0; k < NR; k++) {
+        const int i = lane + WAVE * k;
+        w.e[k] = (i < n) ? gb[(size_t)sys * n + i] - w.e[k] : 0.0;
+    }
+    const double r = sqrt(vdot(w, w, lane, n));
+    if (lane == 0) gRes[sys] = r;
+}
+
+}  // namespace
+
+// ====================================================================== host
+struct bilu_pattern_s {
+    bilu::PatternAnalysis pa;
+    DevPattern dev{};
+    int *dbuf = nullptr;
+    int device = 0;
+    size_t max_lds = 65536;
+};
+
+namespace {
+
+#define HIP_TRY(x)                                                                       \
+    do {                                                                                 \
+        hipError_t err_ = (x);                                                           \
+        if (err_ != hipSuccess) {                                                        \
+            std::fprintf(stderr, "bilu: %s failed: %s (%s:%d)\n", #x, hipGetErrorString(err_), \
+                         __FILE__, __LINE__);                                            \
+            return BILU_ERR_HIP;                                                         \
+        }                                                                                \
+    } while (0)
+
+int upload(bilu_pattern_s &h)
+{
+    const auto &pa = h.pa;
+    std::vector<int> buf;
+    auto put = [&](const std::vector<int> &v) {
+        size_t off = buf.size();
+        buf.insert(buf.end(), v.begin(), v.end());
+        while (buf.size() % 4) buf.push_back(0);  // keep int2 arrays 8-byte aligned
+        return off;
+    };
+    std::vector<int> erow(pa.nnz);
+    for (int i = 0; i < pa.n; i++)
+        for (int p = pa.rp[i]; p < pa.rp[i + 1]; p++) erow[p] = i;
+    size_t o_rp = put(pa.rp), o_ci = put(pa.ci), o_diag = put(pa.diag), o_erow = put(erow);
+    size_t o_llp = put(pa.lev_l_ptr), o_llr = put(pa.lev_l_rows), o_lup = put(pa.lev_u_ptr),
+           o_lur = put(pa.lev_u_rows);
+    size_t o_upp = put(pa.upd_ptr), o_upd = put(pa.upd.empty() ? std::vector<int>{0, 0} : pa.upd);
+    size_t o_dpp = put(pa.dep_ptr), o_dep = put(pa.dep.empty() ? std::vector<int>{0, 0} : pa.dep);
+    size_t o_ls = put(pa.l_src), o_us = put(pa.u_src);
+    size_t o_is[3][5];
+    for (int t = 0; t < 3; t++) {
+        const auto &M = pa.isai[t];
+        o_is[t][0] = put(M.beg); o_is[t][1] = put(M.m); o_is[t][2] = put(M.goff);
+        o_is[t][3] = put(M.gather.empty() ? std::vector<int>{0} : M.gather); o_is[t][4] = put(M.rhs_pos);
+    }
+    HIP_TRY(hipMalloc((void **)&h.dbuf, buf.size() * sizeof(int)));
+    HIP_TRY(hipMemcpy(h.dbuf, buf.data(), buf.size() * sizeof(int), hipMemcpyHostToDevice));
+    DevPattern &d = h.dev;
+    const int *B = h.dbuf;
+    d.n = pa.n; d.nnz = pa.nnz;
+    d.nlev_l = int(pa.lev_l_ptr.size()) - 1;
+    d.nlev_u = int(pa.lev_u_ptr.size()) - 1;
+    d.rp = B + o_rp; d.ci = B + o_ci; d.diag = B + o_diag; d.erow = B + o_erow;
+    d.lev_l_ptr = B + o_llp; d.lev_l_rows = B + o_llr; d.lev_u_ptr = B + o_lup; d.lev_u_rows = B + o_lur;
+    d.upd_ptr = B + o_upp; d.upd = reinterpret_cast<const int2 *>(B + o_upd);
+    d.dep_ptr = B + o_dpp; d.dep = reinterpret_cast<const int2 *>(B + o_dep);
+    d.l_src = B + o_ls; d.u_src = B + o_us;
+    d.l_nnz = int(pa.l_ci.size()); d.u_nnz = int(pa.u_ci.size());
+    for (int t = 0; t < 3; t++) {
+        d.isai[t].beg = B + o_is[t][0]; d.isai[t].m = B + o_is[t][1]; d.isai[t].goff = B + o_is[t][2];
+        d.isai[t].gather = B + o_is[t][3]; d.isai[t].rhs = B + o_is[t][4];
+    }
+    return BILU_OK;
+}
+
+template <int NR, int PC>
+int launch_solve_nr(bilu_pattern_s &h, int batch, const double *A, const double *b, double *x,
+                    const double *F, const SolveParams &sp, float *it, int *cv, double *ir,
+                    double *tr, size_t lds, hipStream_t st)
+{
+    auto kern = k_solve<NR, PC>;
+    if (lds > 65536) HIP_TRY(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(kern, dim3(batch), dim3(WAVE), lds, st, h.dev, batch, A, b, x, F, sp, it, cv, ir, tr);
+    HIP_TRY(hipGetLastError());
+    return BILU_OK;
+}
+
+template <int PC>
+int launch_solve(bilu_pattern_s &h, int batch, const double *A, const double *b, double *x,
+                 const double *F, const SolveParams &sp, float *it, int *cv, double *ir, double *tr,
+                 size_t lds, hipStream_t st)
+{
+    const int nr = (h.pa.n + WAVE - 1) / WAVE;
+    switch (nr) {
+    case 1: return launch_solve_nr<1, PC>(h, batch, A, b, x, F, sp, it, cv, ir, tr, lds, st);
+    case 2: return launch_solve_nr<2, PC>(h, batch, A, b, x, F, sp, it, cv, ir, tr, lds, st);
+    case 3: return launch_solve_nr<3, PC>(h, batch, A, b, x, F, sp, it, cv, ir, tr, lds, st);
+    case 4: return launch_solve_nr<4, PC>(h, batch, A, b, x, F, sp, it, cv, ir, tr, lds, st);
+    case 5: return launch_solve_nr<5, PC>(h, batch, A, b, x, F, sp, it, cv, ir, tr, lds, st);
+    default: return BILU_ERR_TOO_LARGE;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+void bilu_opts_default(bilu_opts *o)
+{
+    if (!o) return;
+    o->precond = BILU_PRECOND_ILU0;
+    o->parilu_sweeps = 5;
+    o->tol = 1e-11;
+    o->tol_mode = BILU_TOL_ABSOLUTE;
+    o->max_iter = 500;
+    o->merge = BILU_MERGED;
+    o->true_res_restart = 0;
+    o->max_restarts = 3;
+}
+
+const char *bilu_version(void) { return "bilu 0.1 gfx950"; }
+
+const char *bilu_status_string(int s)
+{
+    switch (s) {
+    case BILU_OK: return "ok";
+    case BILU_ERR_INVALID_ARG: return "invalid argument";
+    case BILU_ERR_INVALID_PATTERN: return "invalid pattern";
+    case BILU_ERR_MISSING_DIAGONAL: return "missing diagonal";
+    case BILU_ERR_TOO_LARGE: return "problem too large for the kernels";
+    case BILU_ERR_HIP: return "HIP runtime error";
+    case BILU_ERR_NO_DEVICE: return "no HIP device";
+    default: return "unknown status";
+    }
+}
+
+int bilu_pattern_create(int n, int nnz, const int *row_ptrs, const int *col_idx, bilu_pattern_t *out)
+{
+    if (!out) return BILU_ERR_INVALID_ARG;
+    *out = nullptr;
+    auto *h = new (std::nothrow) bilu_pattern_s();
+    if (!h) return BILU_ERR_INVALID_ARG;
+    std::string why;
+    int st = bilu::analyse_pattern(n, nnz, row_ptrs, col_idx, h->pa, why);
+    if (st != BILU_OK) {
+        std::fprintf(stderr, "bilu_pattern_create: %s\n", why.c_str());
+        delete h;
+        return st;
+    }
+    int dev_count = 0;
+    if (hipGetDeviceCount(&dev_count) != hipSuccess || dev_count == 0) { delete h; return BILU_ERR_NO_DEVICE; }
+    if (hipGetDevice(&h->device) != hipSuccess) { delete h; return BILU_ERR_HIP; }
+    int lds = 0;
+    if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, h->device) == hipSuccess && lds > 0)
+        h->max_lds = (size_t)lds;
+    st = upload(*h);
+    if (st != BILU_OK) { if (h->dbuf) (void)hipFree(h->dbuf); delete h; return st; }
+    *out = h;
+    return BILU_OK;
+}
+
+int bilu_pattern_destroy(bilu_pattern_t p)
+{
+    if (!p) return BILU_ERR_INVALID_ARG;
+    if (p->dbuf) (void)hipFree(p->dbuf);
+    delete p;
+    return BILU_OK;
+}
+
+int bilu_pattern_get_info(bilu_pattern_t p, bilu_pattern_info *info)
+{
+    if (!p || !info) return BILU_ERR_INVALID_ARG;
+    info->n = p->pa.n;
+    info->nnz = p->pa.nnz;
+    info->l_nnz = int(p->pa.l_ci.size());
+    info->u_nnz = int(p->pa.u_ci.size());
+    info->levels_lower = p->dev.nlev_l;
+    info->levels_upper = p->dev.nlev_u;
+    info->ilu_updates = int(p->pa.upd.size() / 2);
+    info->max_row_nnz = p->pa.max_row;
+    return BILU_OK;
+}
+
+int bilu_pattern_lu_split(bilu_pattern_t p, int *lrp, int *lci, int *urp, int *uci)
+{
+    if (!p) return BILU_ERR_INVALID_ARG;
+    const auto &pa = p->pa;
+    if (lrp) std::memcpy(lrp, pa.l_rp.data(), pa.l_rp.size() * sizeof(int));
+    if (lci) std::memcpy(lci, pa.l_ci.data(), pa.l_ci.size() * sizeof(int));
+    if (urp) std::memcpy(urp, pa.u_rp.data(), pa.u_rp.size() * sizeof(int));
+    if (uci) std::memcpy(uci, pa.u_ci.data(), pa.u_ci.size() * sizeof(int));
+    return BILU_OK;
+}
+
+int bilu_solve(bilu_pattern_t h, int batch, const double *d_A, const double *d_b, double *d_x,
+               const bilu_opts *opts, float *d_iter, int *d_conv, double *d_iter_res,
+               double *d_true_res, void *stream)
+{
+    if (!h || batch < 0 || (batch > 0 && (!d_A || !d_b || !d_x || !d_iter || !d_conv)))
+        return BILU_ERR_INVALID_ARG;
+    if (batch == 0) return BILU_OK;
+    bilu_opts o;
+    if (opts) o = *opts; else bilu_opts_default(&o);
+    if (o.precond < BILU_PRECOND_ILU0 || o.precond > BILU_PRECOND_NONE || o.max_iter < 0 ||
+        o.parilu_sweeps < 0 || !(o.tol >= 0.0))
+        return BILU_ERR_INVALID_ARG;
+    const auto &pa = h->pa;
+    const int n = pa.n, nnz = pa.nnz;
+    if ((n + WAVE - 1) / WAVE > MAX_NR) return BILU_ERR_TOO_LARGE;
+    int pc;
+    SolveParams sp{o.tol, o.tol_mode, o.max_iter, o.true_res_restart, o.max_restarts, FK_NONE, o.parilu_sweeps};
+    switch (o.precond) {
+    case BILU_PRECOND_ILU0: pc = PC_LU; sp.factor = FK_ILU; break;
+    case BILU_PRECOND_PARILU0: pc = PC_LU; sp.factor = FK_PARILU; break;
+    case BILU_PRECOND_ILU0_ISAI: pc = PC_ISAI_LU; sp.factor = FK_ILU; break;
+    case BILU_PRECOND_PARILU0_ISAI: pc = PC_ISAI_LU; sp.factor = FK_PARILU; break;
+    case BILU_PRECOND_GENERAL_ISAI: pc = PC_ISAI_GEN; sp.factor = FK_NONE; break;
+    default: pc = PC_NONE; sp.factor = FK_NONE; break;
+    }
+    if (pc == PC_ISAI_LU && (pa.isai[BILU_ISAI_LOWER].max_m > ISAI_GROUP || pa.isai[BILU_ISAI_UPPER].max_m > ISAI_GROUP))
+        return BILU_ERR_TOO_LARGE;
+    if (pc == PC_ISAI_GEN && pa.isai[BILU_ISAI_GENERAL].max_m > ISAI_GROUP) return BILU_ERR_TOO_LARGE;
+    hipStream_t st = (hipStream_t)stream;
+    HIP_TRY(hipSetDevice(h->device));
+    const double *F = nullptr;
+    double *Fbuf = nullptr;
+    if (o.merge == BILU_SEGREGATED && pc == PC_LU) {
+        // factor launch, factors through HBM, then the solve launch (segregated.cu:429-490)
+        HIP_TRY(hipMallocAsync((void **)&Fbuf, (size_t)batch * nnz * sizeof(double), st));
+        const size_t flds = (size_t)(sp.factor == FK_PARILU ? 3 : 2) * nnz * sizeof(double);
+        if (flds > h->max_lds) { (void)hipFreeAsync(Fbuf, st); return BILU_ERR_TOO_LARGE; }
+        if (flds > 65536) HIP_TRY(hipFuncSetAttribute((const void *)k_factor, hipFuncAttributeMaxDynamicSharedMemorySize, (int)flds));
+        hipLaunchKernelGGL(k_factor, dim3(batch), dim3(WAVE), flds, st, h->dev, batch, d_A,
+                           (const double *)nullptr, Fbuf, sp.factor, sp.sweeps, (int *)nullptr);
+        HIP_TRY(hipGetLastError());
+        F = Fbuf;
+        sp.factor = FK_LOAD;
+    }
+    const bool need_w = (sp.factor == FK_PARILU) || pc == PC_ISAI_LU || pc == PC_ISAI_GEN;
+    const size_t lds = sizeof(double) * ((size_t)nnz * (2 + (need_w ? 1 : 0)) + (size_t)n * (3 + (pc == PC_ISAI_LU ? 1 : 0)));
+    int rc;
+    if (lds > h->max_lds) rc = BILU_ERR_TOO_LARGE;
+    else if (pc == PC_LU) rc = launch_solve<PC_LU>(*h, batch, d_A, d_b, d_x, F, sp, d_iter, d_conv, d_iter_res, d_true_res, lds, st);
+    else if (pc == PC_ISAI_LU) rc = launch_solve<PC_ISAI_LU>(*h, batch, d_A, d_b, d_x, F, sp, d_iter, d_conv, d_iter_res, d_true_res, lds, st);
+    else if (pc == PC_ISAI_GEN) rc = launch_solve<PC_ISAI_GEN>(*h, batch, d_A, d_b, d_x, F, sp, d_iter, d_conv, d_iter_res, d_true_res, lds, st);
+    else rc = launch_solve<PC_NONE>(*h, batch, d_A, d_b, d_x, F, sp, d_iter, d_conv, d_iter_res, d_true_res, lds, st);
+    if (Fbuf) HIP_TRY(hipFreeAsync(Fbuf, st));
+    return rc;
+}
+
+int bilu_ilu0(bilu_pattern_t h, int batch, const double *d_A, double *d_F, int *d_status, void *stream)
+{
+    if (!h || batch < 0 || (batch > 0 && (!d_A || !d_F))) return BILU_ERR_INVALID_ARG;
+    if (batch == 0) return BILU_OK;
+    const size_t lds = 2 * (size_t)h->pa.nnz * sizeof(double);
+    if (lds > h->max_lds) return BILU_ERR_TOO_LARGE;
+    HIP_TRY(hipSetDevice(h->device));
+    if (lds > 65536) HIP_TRY(hipFuncSetAttribute((const void *)k_factor, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(k_factor, dim3(batch), dim3(WAVE), lds, (hipStream_t)stream, h->dev, batch, d_A,
+                       (const double *)nullptr, d_F, (int)FK_ILU, 0, d_status);
+    HIP_TRY(hipGetLastError());
+    return BILU_OK;
+}
+
+int bilu_parilu0(bilu_pattern_t h, int batch, const double *d_A, const double *d_F_init, double *d_F,
+                 int sweeps, void *stream)
+{
+    if (!h || batch < 0 || sweeps < 0 || (batch > 0 && (!d_A || !d_F))) return BILU_ERR_INVALID_ARG;
+    if (batch == 0) return BILU_OK;
+    const size_t lds = 3 * (size_t)h->pa.nnz * sizeof(double);
+    if (lds > h->max_lds) return BILU_ERR_TOO_LARGE;
+    HIP_TRY(hipSetDevice(h->device));
+    if (lds > 65536) HIP_TRY(hipFuncSetAttribute((const void *)k_factor, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(k_factor, dim3(batch), dim3(WAVE), lds, (hipStream_t)stream, h->dev, batch, d_A,
+                       d_F_init, d_F, (int)FK_PARILU, sweeps, (int *)nullptr);
+    HIP_TRY(hipGetLastError());
+    return BILU_OK;
+}
+
+int bilu_factor_residual(bilu_pattern_t h, int batch, const double *d_A, const double *d_F, double *d_res,
+                         void *stream)
+{
+    if (!h || batch < 0 || (batch > 0 && (!d_A || !d_F || !d_res))) return BILU_ERR_INVALID_ARG;
+    if (batch == 0) return BILU_OK;
+    const size_t lds = (size_t)h->pa.nnz * sizeof(double);
+    if (lds > h->max_lds) return BILU_ERR_TOO_LARGE;
+    HIP_TRY(hipSetDevice(h->device));
+    if (lds > 65536) HIP_TRY(hipFuncSetAttribute((const void *)k_factor_residual, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(k_factor_residual, dim3(batch), dim3(WAVE), lds, (hipStream_t)stream, h->dev, batch, d_A, d_F, d_res);
+    HIP_TRY(hipGetLastError());
+    return BILU_OK;
+}
+
+int bilu_split_lu(bilu_pattern_t h, int batch, const double *d_F, double *d_L, double *d_U, void *stream)
+{
+    if (!h || batch < 0 || (batch > 0 && !d_F)) return BILU_ERR_INVALID_ARG;
+    if (batch == 0) return BILU_OK;
+    HIP_TRY(hipSetDevice(h->device));
+    const size_t tot = (size_t)batch * (size_t)std::max(h->dev.l_nnz, h->dev.u_nnz);
+    const int grid = (int)std::min<size_t>((tot + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_split_lu, dim3(grid), dim3(256), 0, (hipStream_t)stream, h->dev, batch, d_F, d_L, d_U);
+    HIP_TRY(hipGetLastError());
+    return BILU_OK;
+}
+
+int bilu_isai(bilu_pattern_t h, int batch, const double *d_vals, int type, double *d_out, int *d_status,
+              void *stream)
+{
+    if (!h || batch < 0 || type < 0 || type > 2 || (batch > 0 && (!d_vals || !d_out))) return BILU_ERR_INVALID_ARG;
+    if (batch == 0) return BILU_OK;
+    if (h->pa.isai[type].max_m > ISAI_GROUP) return BILU_ERR_TOO_LARGE;
+    const size_t lds = 2 * (size_t)h->pa.nnz * sizeof(double);
+    if (lds > h->max_lds) return BILU_ERR_TOO_LARGE;
+    HIP_TRY(hipSetDevice(h->device));
+    if (lds > 65536) HIP_TRY(hipFuncSetAttribute((const void *)k_isai, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(k_isai, dim3(batch), dim3(WAVE), lds, (hipStream_t)stream, h->dev, batch, d_vals, type, d_out, d_status);
+    HIP_TRY(hipGetLastError());
+    return BILU_OK;
+}
+
+int bilu_true_residual(bilu_pattern_t h, int batch, const double *d_A, const double *d_b, const double *d_x,
+                       double *d_res, void *stream)
+{
+    if (!h || batch < 0 || (batch > 0 && (!d_A || !d_b || !d_x || !d_res))) return BILU_ERR_INVALID_ARG;
+    if (batch == 0) return BILU_OK;
+    const int nr = (h->pa.n + WAVE - 1) / WAVE;
+    const size_t lds = ((size_t)h->pa.nnz + h->pa.n) * sizeof(double);
+    if (lds > h->max_lds || nr > MAX_NR) return BILU_ERR_TOO_LARGE;
+    HIP_TRY(hipSetDevice(h->device));
+    hipStream_t st = (hipStream_t)stream;
+#define TR_CASE(R)                                                                                   \
+    case R:                                                                                          \
+        if (lds > 65536) HIP_TRY(hipFuncSetAttribute((const void *)k_true_residual<R>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds)); \
+        hipLaunchKernelGGL(k_true_residual<R>, dim3(batch), dim3(WAVE), lds, st, h->dev, batch, d_A, d_b, d_x, d_res); \
+        break;
+    switch (nr) {
+        TR_CASE(1) TR_CASE(2) TR_CASE(3) TR_CASE(4) TR_CASE(5)
+    default: return BILU_ERR_TOO_LARGE;
+    }
+#undef TR_CASE
+    HIP_TRY(hipGetLastError());
+    return BILU_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,493 @@
+/*
+ * oracle.c -- CPU restatement of the reference arithmetic (test
+ * infrastructure only; see oracle.h for who may load it).
+ *
+ * Reference paths cited relative to
+ * Batched-ILU0-merged/ILU0_ISAI_bicgstab/ of aggarwal2000/Merging-ILU0.
+ * Short names: merged.cu = PrecondBiCGSTAB_exact_ilu0_app1_merged_gpu.cu,
+ * common.hpp = Bicgstab_common.hpp.
+ *
+ * Build: see oracle/build_oracle.sh (gcc -O3 -mfma -ffp-contract=off).
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* position of column c in row r of a sorted CSR pattern, -1 if absent */
+static int find_pos(const int *rp, const int *ci, int r, int c)
+{
+    int lo = rp[r], hi = rp[r + 1] - 1;
+    while (lo <= hi) {
+        int mid = (lo + hi) >> 1;
+        if (ci[mid] == c) return mid;
+        if (ci[mid] < c) lo = mid + 1; else hi = mid - 1;
+    }
+    return -1;
+}
+
+/* factorization.cu:192-212 (find_locn_of_diag_elements_kernel) and
+ * :399-421 (Count_Missing_Diagonal_Elements). */
+int orc_find_diag(int n, const int *rp, const int *ci, int *diag)
+{
+    int missing = 0;
+    for (int i = 0; i < n; i++) {
+        diag[i] = find_pos(rp, ci, i, i);
+        if (diag[i] < 0) missing++;
+    }
+    return missing;
+}
+
+/* Strided partial sums followed by a shfl_down tree, the order of the
+ * reference inner_product (common.hpp:97-128: lane l of warp 0 sums
+ * idx = l, l+32, ...; then offsets 16,8,4,2,1).  width 1 = serial sum
+ * (common.hpp:82-95), width 64 = the wave64 kernels' order. */
+double orc_dot(int n, const double *a, const double *b, int width)
+{
+    double t[64];
+    if (width < 1) width = 1;
+    for (int l = 0; l < width; l++) {
+        double s = 0.0;
+        for (int idx = l; idx < n; idx += width) s = fma(a[idx], b[idx], s);
+        t[l] = s;
+    }
+    for (int off = width / 2; off > 0; off /= 2)
+        for (int l = 0; l < off; l++) t[l] = t[l] + t[l + off];
+    return t[0];
+}
+
+static double row_sum(const double *vals, const int *ci, int beg, int end,
+                      const double *x, int width)
+{
+    double t[64];
+    if (width <= 1) {
+        double s = 0.0;
+        for (int k = beg; k < end; k++) s = fma(vals[k], x[ci[k]], s);
+        return s;
+    }
+    for (int l = 0; l < width; l++) {
+        double s = 0.0;
+        for (int k = beg + l; k < end; k += width) s = fma(vals[k], x[ci[k]], s);
+        t[l] = s;
+    }
+    for (int off = width / 2; off > 0; off /= 2)
+        for (int l = 0; l < off; l++) t[l] = t[l] + t[l + off];
+    return t[0];
+}
+
+/* SpMV (common.hpp:254-290): width 32 = one warp per row with lane-strided
+ * partial sums and a shfl_down tree; width 1 = left-to-right row sum. */
+void orc_spmv(int n, const int *rp, const int *ci, const double *vals,
+              const double *x, double *y, int width)
+{
+    for (int i = 0; i < n; i++) y[i] = row_sum(vals, ci, rp[i], rp[i + 1], x, width);
+}
+
+/* Exact ILU(0) in place on the pattern.
+ * Reference: the right-looking KIJ loop of merged.cu:145-246
+ * (compute_exact_ilu_0_approach1; twin ILU_0_gpu.cu:105-171): for k, for rows
+ * i > k holding (i,k): l_ik = a_ik / a_kk, then a_ij -= l_ik * a_kj for
+ * j > k.  Written here row by row (IKJ); every entry receives the same
+ * sequence of fma updates in the same k order, so the result is bitwise the
+ * KIJ result.  The reference also "updates" entries whose (k,j) is not in
+ * the pattern with a zero; that is skipped here (it changes nothing but the
+ * sign of zero).  Returns ORC_ZERO_PIVOT if a pivot or a U diagonal is 0
+ * (the reference asserts, merged.cu:198). */
+int orc_ilu0(int n, const int *rp, const int *ci, const int *diag, double *F)
+{
+    int *map = (int *)malloc(sizeof(int) * (size_t)n);
+    int status = 0;
+    for (int j = 0; j < n; j++) map[j] = -1;
+    for (int i = 0; i < n && status == 0; i++) {
+        for (int p = rp[i]; p < rp[i + 1]; p++) map[ci[p]] = p;
+        for (int p = rp[i]; p < diag[i]; p++) {
+            const int k = ci[p];
+            const double ukk = F[diag[k]];
+            if (ukk == 0.0) { status = ORC_ZERO_PIVOT; break; }
+            const double l = F[p] / ukk;
+            F[p] = l;
+            for (int q = diag[k] + 1; q < rp[k + 1]; q++) {
+                const int dst = map[ci[q]];
+                if (dst >= 0) F[dst] = fma(-l, F[q], F[dst]);
+            }
+        }
+        for (int p = rp[i]; p < rp[i + 1]; p++) map[ci[p]] = -1;
+    }
+    if (status == 0)
+        for (int i = 0; i < n; i++)
+            if (F[diag[i]] == 0.0) { status = ORC_ZERO_PIVOT; break; }
+    free(map);
+    return status;
+}
+
+/* ParILU(0) fixed-point sweeps, Jacobi form (every sweep reads the previous
+ * sweep's factors).  Per entry (parILU_0_gpu.cu:18-57 with the dependency
+ * lists of :118-220): sum over k < min(i,j) with (i,k),(k,j) in the pattern,
+ * k increasing, of l_ik*u_kj; lower entries (i>j) become (a_ij - sum)/u_jj,
+ * upper entries a_ij - sum.  Initial guess = A itself, which on the in-place
+ * layout is the reference's L = strict-lower(A)+I, U = upper(A)
+ * (factorization.cu:288-344).  The reference sweeps asynchronously in place
+ * (results after finitely many sweeps depend on scheduling); Jacobi is the
+ * deterministic variant with the same fixed point. */
+int orc_parilu0_sweeps(int n, const int *rp, const int *ci, const int *diag,
+                       const double *A, double *F, int sweeps)
+{
+    const int nnz = rp[n];
+    double *old = (double *)malloc(sizeof(double) * (size_t)nnz);
+    for (int s = 0; s < sweeps; s++) {
+        memcpy(old, F, sizeof(double) * (size_t)nnz);
+        for (int i = 0; i < n; i++) {
+            for (int e = rp[i]; e < rp[i + 1]; e++) {
+                const int j = ci[e];
+                const int kmax = (i < j ? i : j) - 1;
+                double sum = 0.0;
+                for (int p = rp[i]; p < diag[i] && ci[p] <= kmax; p++) {
+                    const int k = ci[p];
+                    const int q = find_pos(rp, ci, k, j);
+                    if (q >= 0) sum = fma(old[p], old[q], sum);
+                }
+                if (i > j) F[e] = (A[e] - sum) / old[diag[j]];
+                else F[e] = A[e] - sum;
+            }
+        }
+    }
+    free(old);
+    return 0;
+}
+
+/* ||(L*U - A) restricted to the pattern||_F with unit-diagonal L. */
+double orc_factor_residual(int n, const int *rp, const int *ci, const int *diag,
+                           const double *A, const double *F)
+{
+    double acc = 0.0;
+    for (int i = 0; i < n; i++) {
+        for (int e = rp[i]; e < rp[i + 1]; e++) {
+            const int j = ci[e];
+            const int kmax = (i < j ? i : j) - 1;
+            double sum = 0.0;
+            for (int p = rp[i]; p < diag[i] && ci[p] <= kmax; p++) {
+                const int q = find_pos(rp, ci, ci[p], j);
+                if (q >= 0) sum = fma(F[p], F[q], sum);
+            }
+            /* (LU)_ij = sum + (i > j ? l_ij * u_jj : u_ij) */
+            const double lu = (i > j) ? fma(F[e], F[diag[j]], sum) : sum + F[e];
+            const double d = lu - A[e];
+            acc = fma(d, d, acc);
+        }
+    }
+    return sqrt(acc);
+}
+
+/* v_hat = U^{-1} L^{-1} v on the in-place factor.
+ * merged.cu:22-64: y_i = (v_i - sum_{j<i} l_ij y_j) / 1, the sum running
+ * left to right; merged.cu:67-108: x_i = (y_i - sum_{j>i} u_ij x_j) / u_ii,
+ * the sum running right to left. */
+void orc_lu_solve(int n, const int *rp, const int *ci, const int *diag,
+                  const double *F, const double *v, double *tmp, double *out)
+{
+    for (int i = 0; i < n; i++) {
+        double sum = 0.0;
+        for (int p = rp[i]; p < diag[i]; p++) sum = fma(F[p], tmp[ci[p]], sum);
+        tmp[i] = v[i] - sum;
+    }
+    for (int i = n - 1; i >= 0; i--) {
+        double sum = 0.0;
+        for (int p = rp[i + 1] - 1; p > diag[i]; p--) sum = fma(F[p], out[ci[p]], sum);
+        out[i] = (tmp[i] - sum) / F[diag[i]];
+    }
+}
+
+/* value of the typed view of `vals` at (r,c): lower = strict-lower part with
+ * an explicit unit diagonal (the L of factorization.cu:313-314), upper =
+ * diagonal + strict upper, general = everything. */
+static double view_at(const int *rp, const int *ci, const double *vals,
+                      int type, int r, int c)
+{
+    if (type == ORC_ISAI_LOWER) {
+        if (r == c) return 1.0;
+        if (r < c) return 0.0;
+    } else if (type == ORC_ISAI_UPPER) {
+        if (r > c) return 0.0;
+    }
+    const int p = find_pos(rp, ci, r, c);
+    return p >= 0 ? vals[p] : 0.0;
+}
+
+/* Reference pivot choice (isai_gpu.cu:417-451, choose_pivot_row_1): 32
+ * lanes; lane t outside [dp, m) stands in with the diagonal candidate; a
+ * shfl_down tree keeps the other lane's candidate only if strictly larger in
+ * magnitude. */
+static int ref_pivot(double Dt[][64], int m, int dp)
+{
+    double val[32];
+    int piv[32];
+    for (int t = 0; t < 32; t++) {
+        if (t >= m || t < dp) { val[t] = Dt[dp][dp]; piv[t] = dp; }
+        else { val[t] = Dt[t][dp]; piv[t] = t; }
+    }
+    for (int off = 16; off > 0; off /= 2)
+        for (int l = 0; l + off < 32; l++)
+            if (fabs(val[l + off]) > fabs(val[l])) { val[l] = val[l + off]; piv[l] = piv[l + off]; }
+    return piv[0];
+}
+
+/* ISAI with the sparsity pattern of the input (power 1).
+ * isai_gpu.cu:559-626: for row i with J = cols(i), the thread holding column
+ * t of the dense block A(J,J) ("implicit transpose") solves A(J,J)^T m = e,
+ * e at the position of i in J; row i of the ISAI is m^T.
+ * lower: upper-triangular column sweep, rhs at the last position
+ * (isai_gpu.cu:325-355); upper: lower-triangular sweep, rhs at 0
+ * (:359-389); general: Gauss-Jordan with partial pivoting (:417-554).
+ * Output on the pattern, masked to the triangle of `type`. */
+int orc_isai(int n, const int *rp, const int *ci, const int *diag,
+             const double *vals, int type, double *out)
+{
+    static __thread double Dt[64][64];
+    double sol[64];
+    int J[64];
+    for (int i = 0; i < n; i++) {
+        int beg, end;
+        if (type == ORC_ISAI_LOWER) { beg = rp[i]; end = diag[i] + 1; }
+        else if (type == ORC_ISAI_UPPER) { beg = diag[i]; end = rp[i + 1]; }
+        else { beg = rp[i]; end = rp[i + 1]; }
+        const int m = end - beg;
+        if (m > 64 || (type == ORC_ISAI_GENERAL && m > 32)) return -4;
+        int ipos = 0;
+        for (int q = 0; q < m; q++) { J[q] = ci[beg + q]; if (J[q] == i) ipos = q; }
+        /* Dt[t][k] = D[k][t] = A(J[k], J[t]) */
+        for (int k = 0; k < m; k++)
+            for (int t = 0; t < m; t++) Dt[t][k] = view_at(rp, ci, vals, type, J[k], J[t]);
+        if (type == ORC_ISAI_LOWER) {
+            for (int t = 0; t < m; t++) sol[t] = (t == m - 1) ? 1.0 : 0.0;
+            for (int c = m - 1; c >= 0; c--) {
+                if (Dt[c][c] == 0.0) return ORC_ZERO_PIVOT;
+                sol[c] = sol[c] / Dt[c][c];
+                for (int t = 0; t < c; t++) sol[t] = fma(-sol[c], Dt[t][c], sol[t]);
+            }
+        } else if (type == ORC_ISAI_UPPER) {
+            for (int t = 0; t < m; t++) sol[t] = (t == 0) ? 1.0 : 0.0;
+            for (int c = 0; c < m; c++) {
+                if (Dt[c][c] == 0.0) return ORC_ZERO_PIVOT;
+                sol[c] = sol[c] / Dt[c][c];
+                for (int t = c + 1; t < m; t++) sol[t] = fma(-sol[c], Dt[t][c], sol[t]);
+            }
+        } else {
+            for (int t = 0; t < m; t++) sol[t] = (t == ipos) ? 1.0 : 0.0;
+            for (int dp = 0; dp < m; dp++) {
+                const int pv = ref_pivot(Dt, m, dp);
+                if (pv != dp) {
+                    for (int c = 0; c < m; c++) { double tmp = Dt[dp][c]; Dt[dp][c] = Dt[pv][c]; Dt[pv][c] = tmp; }
+                    double tmp = sol[dp]; sol[dp] = sol[pv]; sol[pv] = tmp;
+                }
+                const double de = Dt[dp][dp];
+                if (de == 0.0) return ORC_ZERO_PIVOT;
+                for (int t = 0; t < m; t++) {
+                    if (t == dp) continue;
+                    const double mult = Dt[t][dp] / de;
+                    for (int c = 0; c < m; c++) Dt[t][c] = fma(-mult, Dt[dp][c], Dt[t][c]);
+                    sol[t] = fma(-mult, sol[dp], sol[t]);
+                }
+            }
+            for (int t = 0; t < m; t++) sol[t] = sol[t] / Dt[t][t];
+        }
+        for (int q = 0; q < m; q++) out[beg + q] = sol[q];
+    }
+    return 0;
+}
+
+/* y = M^{-1} v for the chosen preconditioner */
+static void apply_pc(int n, const int *rp, const int *ci, const int *diag,
+                     int kind, const double *P1, const double *P2, int spmv_w,
+                     const double *v, double *tmp, double *out)
+{
+    if (kind == ORC_PC_LU) {
+        orc_lu_solve(n, rp, ci, diag, P1, v, tmp, out);
+    } else if (kind == ORC_PC_ISAI_LU) {
+        /* ilu_isai_solve.cu:25-45: temp = aiL * v; out = aiU * temp */
+        for (int i = 0; i < n; i++) tmp[i] = row_sum(P1, ci, rp[i], diag[i] + 1, v, spmv_w);
+        for (int i = 0; i < n; i++) out[i] = row_sum(P2, ci, diag[i], rp[i + 1], tmp, spmv_w);
+    } else if (kind == ORC_PC_ISAI_GEN) {
+        orc_spmv(n, rp, ci, P1, v, out, spmv_w);  /* gisai_solve.cu:23-30 */
+    } else {
+        memcpy(out, v, sizeof(double) * (size_t)n);
+    }
+}
+
+/* Right-preconditioned BiCGSTAB with the reference control flow
+ * (merged.cu:296-524; common.hpp:151-248): x0 = 0, r = b - A x0, r_hat = r,
+ * p = v = 0, rho_old = omega_old = alpha = 1; iteration counted in half
+ * steps (float); convergence on the recursive residual norm < tol_abs
+ * after each half step; rho == 0 => break with flag -1; b == 0 => x = 0,
+ * iter 0, flag 1; ||r0|| < tol_abs => iter 0, flag 1.
+ * Additions (documented options): relative tolerance mode, a restart from
+ * the true residual when it misses the tolerance, and non-finite detection
+ * (flag -3) in place of the reference's hang/garbage.
+ * true_res = ||b - A x||_2 (absolute, as common.hpp:294-342). */
+int orc_bicgstab(int n, const int *rp, const int *ci, const int *diag,
+                 const double *A, const double *b, double *x,
+                 int pc_kind, const double *P1, const double *P2,
+                 const orc_opts *o, float *iter_out, int *conv_out,
+                 double *iter_res_out, double *true_res_out)
+{
+    const int dw = o->dot_width, sw = o->spmv_width;
+    double *buf = (double *)malloc(sizeof(double) * (size_t)n * 10);
+    double *r = buf, *rh = buf + n, *p = buf + 2 * n, *v = buf + 3 * n, *s = buf + 4 * n;
+    double *t = buf + 5 * n, *ph = buf + 6 * n, *sh = buf + 7 * n, *tmp = buf + 8 * n, *w = buf + 9 * n;
+    float iter = 0.0f;
+    int conv = ORC_NOT_CONVERGED;
+    double iter_res;
+
+    for (int i = 0; i < n; i++) { x[i] = 0.0; v[i] = 0.0; p[i] = 0.0; r[i] = b[i]; rh[i] = b[i]; }
+    const double res0 = sqrt(orc_dot(n, r, r, dw));
+    const double bnorm = sqrt(orc_dot(n, b, b, dw));
+    const double tol_abs = (o->tol_mode == ORC_TOL_RELATIVE) ? o->tol * bnorm : o->tol;
+    iter_res = res0;
+
+    if (!isfinite(bnorm)) {
+        conv = ORC_NONFINITE;
+    } else if (bnorm == 0.0) {
+        conv = ORC_CONVERGED; iter_res = 0.0;
+    } else if (res0 < tol_abs) {
+        conv = ORC_CONVERGED;
+    } else {
+        double rho_old = 1.0, omega_old = 1.0, alpha = 1.0;
+        int restarts = 0;
+        while (iter < (float)o->max_iter) {
+            const double rho_new = orc_dot(n, r, rh, dw);
+            if (rho_new == 0.0) break;
+            const double beta = (rho_new / rho_old) * (alpha / omega_old);
+            for (int i = 0; i < n; i++) p[i] = fma(beta, fma(-omega_old, v[i], p[i]), r[i]);
+            apply_pc(n, rp, ci, diag, pc_kind, P1, P2, sw, p, tmp, ph);
+            orc_spmv(n, rp, ci, A, ph, v, sw);
+            alpha = rho_new / orc_dot(n, rh, v, dw);
+            for (int i = 0; i < n; i++) s[i] = fma(-alpha, v[i], r[i]);
+            iter_res = sqrt(orc_dot(n, s, s, dw));
+            iter += 0.5f;
+            if (!isfinite(iter_res)) { conv = ORC_NONFINITE; break; }
+            int half_conv = iter_res < tol_abs;
+            double omega_new = 0.0;
+            if (half_conv) {
+                for (int i = 0; i < n; i++) x[i] = fma(alpha, ph[i], x[i]);
+            } else {
+                apply_pc(n, rp, ci, diag, pc_kind, P1, P2, sw, s, tmp, sh);
+                orc_spmv(n, rp, ci, A, sh, t, sw);
+                const double ts = orc_dot(n, t, s, dw);
+                const double tt = orc_dot(n, t, t, dw);
+                omega_new = ts / tt;
+                for (int i = 0; i < n; i++) x[i] = fma(omega_new, sh[i], fma(alpha, ph[i], x[i]));
+                iter += 0.5f;
+                for (int i = 0; i < n; i++) r[i] = fma(-omega_new, t[i], s[i]);
+                iter_res = sqrt(orc_dot(n, r, r, dw));
+                rho_old = rho_new;
+                omega_old = omega_new;
+                if (!isfinite(iter_res)) { conv = ORC_NONFINITE; break; }
+            }
+            if (half_conv || iter_res < tol_abs) {
+                conv = ORC_CONVERGED;
+                if (!o->true_res_restart || restarts >= o->max_restarts) break;
+                /* true-residual check: r_true = b - A x */
+                orc_spmv(n, rp, ci, A, x, w, sw);
+                for (int i = 0; i < n; i++) w[i] = b[i] - w[i];
+                const double tr = sqrt(orc_dot(n, w, w, dw));
+                if (tr < tol_abs) break;
+                restarts++;
+                conv = ORC_NOT_CONVERGED;
+                for (int i = 0; i < n; i++) { r[i] = w[i]; rh[i] = w[i]; p[i] = 0.0; v[i] = 0.0; }
+                rho_old = 1.0; omega_old = 1.0; alpha = 1.0;
+                iter_res = tr;
+            }
+        }
+    }
+    /* fused epilogue: absolute true residual */
+    orc_spmv(n, rp, ci, A, x, w, sw);
+    for (int i = 0; i < n; i++) w[i] = b[i] - w[i];
+    *true_res_out = sqrt(orc_dot(n, w, w, dw));
+    *iter_out = iter;
+    *conv_out = conv;
+    *iter_res_out = iter_res;
+    free(buf);
+    return 0;
+}
+
+/* ---- batch driver (also the timed CPU baseline) ---- */
+typedef struct {
+    int n, nnz, batch_beg, batch_end, precond, sweeps;
+    const int *rp, *ci, *diag;
+    const double *A, *b;
+    double *x;
+    const orc_opts *o;
+    float *iter; int *conv; double *iter_res, *true_res;
+} batch_job;
+
+static void *batch_worker(void *arg)
+{
+    batch_job *j = (batch_job *)arg;
+    const int n = j->n, nnz = j->nnz;
+    double *F = (double *)malloc(sizeof(double) * (size_t)nnz);
+    double *P2 = (double *)malloc(sizeof(double) * (size_t)nnz);
+    double *P1 = (double *)malloc(sizeof(double) * (size_t)nnz);
+    for (int s = j->batch_beg; s < j->batch_end; s++) {
+        const double *A = j->A + (size_t)s * nnz;
+        const double *b = j->b + (size_t)s * n;
+        double *x = j->x + (size_t)s * n;
+        int st = 0, kind = ORC_PC_LU;
+        const double *p1 = F, *p2 = NULL;
+        memcpy(F, A, sizeof(double) * (size_t)nnz);
+        switch (j->precond) {
+        case 0: st = orc_ilu0(n, j->rp, j->ci, j->diag, F); break;
+        case 1: st = orc_parilu0_sweeps(n, j->rp, j->ci, j->diag, A, F, j->sweeps); break;
+        case 2: case 3:
+            st = (j->precond == 2) ? orc_ilu0(n, j->rp, j->ci, j->diag, F)
+                                   : orc_parilu0_sweeps(n, j->rp, j->ci, j->diag, A, F, j->sweeps);
+            if (st == 0) st = orc_isai(n, j->rp, j->ci, j->diag, F, ORC_ISAI_LOWER, P1);
+            if (st == 0) st = orc_isai(n, j->rp, j->ci, j->diag, F, ORC_ISAI_UPPER, P2);
+            kind = ORC_PC_ISAI_LU; p1 = P1; p2 = P2;
+            break;
+        case 4:
+            st = orc_isai(n, j->rp, j->ci, j->diag, A, ORC_ISAI_GENERAL, P1);
+            kind = ORC_PC_ISAI_GEN; p1 = P1;
+            break;
+        default: kind = ORC_PC_NONE; break;
+        }
+        if (st != 0) {
+            for (int i = 0; i < n; i++) x[i] = 0.0;
+            j->iter[s] = 0.0f; j->conv[s] = st; j->iter_res[s] = 0.0;
+            j->true_res[s] = sqrt(orc_dot(n, b, b, j->o->dot_width));
+            continue;
+        }
+        orc_bicgstab(n, j->rp, j->ci, j->diag, A, b, x, kind, p1, p2, j->o,
+                     &j->iter[s], &j->conv[s], &j->iter_res[s], &j->true_res[s]);
+    }
+    free(F); free(P1); free(P2);
+    return NULL;
+}
+
+int orc_batch_solve(int n, int nnz, const int *rp, const int *ci, int batch,
+                    const double *A, const double *b, double *x,
+                    int precond, int sweeps, const orc_opts *o,
+                    float *iter, int *conv, double *iter_res, double *true_res,
+                    int threads)
+{
+    int *diag = (int *)malloc(sizeof(int) * (size_t)n);
+    if (orc_find_diag(n, rp, ci, diag) > 0) { free(diag); return -5; }
+    if (threads < 1) threads = 1;
+    if (threads > batch) threads = batch > 0 ? batch : 1;
+    pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * (size_t)threads);
+    batch_job *jobs = (batch_job *)malloc(sizeof(batch_job) * (size_t)threads);
+    for (int t = 0; t < threads; t++) {
+        batch_job *j = &jobs[t];
+        j->n = n; j->nnz = nnz; j->precond = precond; j->sweeps = sweeps;
+        j->batch_beg = (int)((long long)batch * t / threads);
+        j->batch_end = (int)((long long)batch * (t + 1) / threads);
+        j->rp = rp; j->ci = ci; j->diag = diag; j->A = A; j->b = b; j->x = x; j->o = o;
+        j->iter = iter; j->conv = conv; j->iter_res = iter_res; j->true_res = true_res;
+        if (threads == 1) batch_worker(j);
+        else pthread_create(&th[t], NULL, batch_worker, j);
+    }
+    if (threads > 1)
+        for (int t = 0; t < threads; t++) pthread_join(th[t], NULL);
+    free(th); free(jobs); free(diag);
+    return 0;
+}

@@ -1,0 +1,101 @@
+/*
+ * oracle.h -- CPU restatement of the reference's batched shared-pattern
+ * ILU(0) / ParILU(0) / ISAI-preconditioned BiCGSTAB arithmetic.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load this library, and only as the
+ * checker (or the timed CPU baseline).  The product (libbilu) never links,
+ * loads or calls it.
+ *
+ * Everything works on one CSR pattern (row_ptrs rp[n+1], sorted column
+ * indices ci[nnz]) shared by all systems of a batch; values are page-major
+ * (system s owns vals[s*nnz .. s*nnz+nnz)).  Factors are kept "in place" on
+ * that pattern: strictly-lower positions hold L (unit diagonal implicit),
+ * diagonal and strictly-upper positions hold U -- the layout the reference
+ * merged kernel factors into (PrecondBiCGSTAB_exact_ilu0_app1_merged_gpu.cu
+ * :145-246).  ISAI factors are stored on the same pattern, masked to their
+ * triangle (lower: rp[i]..diag[i], upper: diag[i]..rp[i+1]-1).
+ *
+ * Floating-point contract: this file is compiled with -ffp-contract=off and
+ * every fused multiply-add is an explicit fma(), written where the CUDA
+ * reference's `a += b*c` / `a -= b*c` is contracted by nvcc.  Reduction
+ * orders are parameterised (width w, see orc_dot) so one routine covers the
+ * reference order (w = 32: warp-strided partial sums + shfl_down tree,
+ * Bicgstab_common.hpp:97-128) and the wave64 order of the HIP kernels
+ * (w = 64).
+ */
+#ifndef BILU_ORACLE_H
+#define BILU_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* per-system status codes (conv flag); 1 / -1 keep the reference meaning
+ * (PrecondBiCGSTAB_exact_ilu0_app1_merged_gpu.cu:337,445,500) */
+#define ORC_CONVERGED 1
+#define ORC_NOT_CONVERGED (-1)
+#define ORC_ZERO_PIVOT (-2)
+#define ORC_NONFINITE (-3)
+
+/* preconditioner kinds for orc_bicgstab */
+#define ORC_PC_NONE 0       /* identity */
+#define ORC_PC_LU 1         /* in-place L\U factor: lower then upper trisolve */
+#define ORC_PC_ISAI_LU 2    /* aiU * (aiL * v), two SpMVs (ilu_isai_solve.cu:25-45) */
+#define ORC_PC_ISAI_GEN 3   /* aiA * v (gisai_solve.cu:23-30) */
+
+/* ISAI matrix types (isai.h:7 enum class mat_type) */
+#define ORC_ISAI_GENERAL 0
+#define ORC_ISAI_LOWER 1
+#define ORC_ISAI_UPPER 2
+
+#define ORC_TOL_ABSOLUTE 0  /* reference: ||r|| < ATOL (merged.cu:438,496) */
+#define ORC_TOL_RELATIVE 1  /* ||r|| < tol * ||b|| */
+
+typedef struct {
+    double tol;
+    int tol_mode;
+    int max_iter;            /* full iterations (reference MAX_ITER = 500) */
+    int dot_width;           /* 1 = serial, 32 = reference warp, 64 = wave64 */
+    int spmv_width;          /* 1 = sequential row sum, 32 = reference warp tree */
+    int true_res_restart;    /* restart from b - A x when the true residual misses tol */
+    int max_restarts;
+} orc_opts;
+
+int orc_find_diag(int n, const int *rp, const int *ci, int *diag);
+
+double orc_dot(int n, const double *a, const double *b, int width);
+void orc_spmv(int n, const int *rp, const int *ci, const double *vals,
+              const double *x, double *y, int width);
+
+int orc_ilu0(int n, const int *rp, const int *ci, const int *diag, double *F);
+int orc_parilu0_sweeps(int n, const int *rp, const int *ci, const int *diag,
+                       const double *A, double *F, int sweeps);
+double orc_factor_residual(int n, const int *rp, const int *ci, const int *diag,
+                           const double *A, const double *F);
+void orc_lu_solve(int n, const int *rp, const int *ci, const int *diag,
+                  const double *F, const double *v, double *tmp, double *out);
+int orc_isai(int n, const int *rp, const int *ci, const int *diag,
+             const double *vals, int type, double *out);
+
+int orc_bicgstab(int n, const int *rp, const int *ci, const int *diag,
+                 const double *A, const double *b, double *x,
+                 int pc_kind, const double *P1, const double *P2,
+                 const orc_opts *o, float *iter, int *conv,
+                 double *iter_res, double *true_res);
+
+/* whole-batch driver: precond setup + solve per system, optionally on
+ * `threads` POSIX threads (systems statically partitioned).
+ * precond: 0 = exact ILU(0), 1 = ParILU(0) with `sweeps` Jacobi sweeps,
+ * 2 = ILU(0)+ISAI(L,U), 3 = ParILU(0)+ISAI(L,U), 4 = general ISAI(A),
+ * 5 = none. */
+int orc_batch_solve(int n, int nnz, const int *rp, const int *ci, int batch,
+                    const double *A, const double *b, double *x,
+                    int precond, int sweeps, const orc_opts *o,
+                    float *iter, int *conv, double *iter_res, double *true_res,
+                    int threads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,167 @@
+"""ctypes wrapper of the CPU oracle (liboracle.so).
+
+Test infrastructure only: tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg load it, as the checker or the timed CPU baseline.  The
+solver product never imports this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liboracle.so")
+
+CONVERGED, NOT_CONVERGED, ZERO_PIVOT, NONFINITE = 1, -1, -2, -3
+PC_NONE, PC_LU, PC_ISAI_LU, PC_ISAI_GEN = 0, 1, 2, 3
+ISAI_GENERAL, ISAI_LOWER, ISAI_UPPER = 0, 1, 2
+TOL_ABSOLUTE, TOL_RELATIVE = 0, 1
+# batch precond codes of orc_batch_solve
+ILU0, PARILU0, ILU0_ISAI, PARILU0_ISAI, GENERAL_ISAI, NONE = 0, 1, 2, 3, 4, 5
+
+
+class Opts(C.Structure):
+    _fields_ = [
+        ("tol", C.c_double),
+        ("tol_mode", C.c_int),
+        ("max_iter", C.c_int),
+        ("dot_width", C.c_int),
+        ("spmv_width", C.c_int),
+        ("true_res_restart", C.c_int),
+        ("max_restarts", C.c_int),
+    ]
+
+
+def opts(tol=1e-11, tol_mode=TOL_ABSOLUTE, max_iter=500, dot_width=64, spmv_width=1,
+         true_res_restart=0, max_restarts=3) -> Opts:
+    """Defaults: reference tolerance semantics (ATOL=1e-11, MAX_ITER=500,
+    PrecondBiCGSTAB.h:10-12) with the wave64 reduction order."""
+    return Opts(tol, tol_mode, max_iter, dot_width, spmv_width, true_res_restart, max_restarts)
+
+
+def reference_opts(**kw) -> Opts:
+    """Reference reduction order: warp-32 dots and warp-32 SpMV rows."""
+    kw.setdefault("dot_width", 32)
+    kw.setdefault("spmv_width", 32)
+    return opts(**kw)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"oracle library missing: {LIB_PATH} (run __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        P = np.ctypeslib.ndpointer
+        i32 = P(np.int32, flags="C")
+        f64 = P(np.float64, flags="C")
+        f32 = P(np.float32, flags="C")
+        L.orc_find_diag.argtypes = [C.c_int, i32, i32, i32]
+        L.orc_dot.argtypes = [C.c_int, f64, f64, C.c_int]
+        L.orc_dot.restype = C.c_double
+        L.orc_spmv.argtypes = [C.c_int, i32, i32, f64, f64, f64, C.c_int]
+        L.orc_ilu0.argtypes = [C.c_int, i32, i32, i32, f64]
+        L.orc_parilu0_sweeps.argtypes = [C.c_int, i32, i32, i32, f64, f64, C.c_int]
+        L.orc_factor_residual.argtypes = [C.c_int, i32, i32, i32, f64, f64]
+        L.orc_factor_residual.restype = C.c_double
+        L.orc_lu_solve.argtypes = [C.c_int, i32, i32, i32, f64, f64, f64, f64]
+        L.orc_isai.argtypes = [C.c_int, i32, i32, i32, f64, C.c_int, f64]
+        L.orc_bicgstab.argtypes = [C.c_int, i32, i32, i32, f64, f64, f64, C.c_int,
+                                   C.c_void_p, C.c_void_p, C.POINTER(Opts),
+                                   C.POINTER(C.c_float), C.POINTER(C.c_int),
+                                   C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.orc_batch_solve.argtypes = [C.c_int, C.c_int, i32, i32, C.c_int, f64, f64, f64,
+                                      C.c_int, C.c_int, C.POINTER(Opts), f32,
+                                      P(np.int32, flags="C"), f64, f64, C.c_int]
+        _lib = L
+    return _lib
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def find_diag(n, rp, ci):
+    d = np.empty(n, dtype=np.int32)
+    missing = lib().orc_find_diag(n, _i32(rp), _i32(ci), d)
+    return d, missing
+
+
+def dot(a, b, width=64):
+    a = _f64(a)
+    return lib().orc_dot(a.size, a, _f64(b), width)
+
+
+def spmv(pat, vals, x, width=1):
+    y = np.empty(pat.n)
+    lib().orc_spmv(pat.n, pat.row_ptrs, pat.col_idx, _f64(vals), _f64(x), y, width)
+    return y
+
+
+def ilu0(pat, vals):
+    F = _f64(vals).copy()
+    st = lib().orc_ilu0(pat.n, pat.row_ptrs, pat.col_idx, pat.diag, F)
+    return F, st
+
+
+def parilu0(pat, vals, sweeps, init=None):
+    A = _f64(vals)
+    F = (A if init is None else _f64(init)).copy()
+    lib().orc_parilu0_sweeps(pat.n, pat.row_ptrs, pat.col_idx, pat.diag, A, F, sweeps)
+    return F
+
+
+def factor_residual(pat, A, F):
+    return lib().orc_factor_residual(pat.n, pat.row_ptrs, pat.col_idx, pat.diag, _f64(A), _f64(F))
+
+
+def lu_solve(pat, F, v):
+    tmp = np.empty(pat.n)
+    out = np.empty(pat.n)
+    lib().orc_lu_solve(pat.n, pat.row_ptrs, pat.col_idx, pat.diag, _f64(F), _f64(v), tmp, out)
+    return out
+
+
+def isai(pat, vals, typ):
+    out = np.zeros(pat.nnz)
+    st = lib().orc_isai(pat.n, pat.row_ptrs, pat.col_idx, pat.diag, _f64(vals), typ, out)
+    return out, st
+
+
+def bicgstab(pat, A, b, pc_kind=PC_NONE, P1=None, P2=None, o: Opts | None = None):
+    o = o or opts()
+    x = np.empty(pat.n)
+    it, cv, ir, tr = C.c_float(), C.c_int(), C.c_double(), C.c_double()
+    p1 = _f64(P1) if P1 is not None else None
+    p2 = _f64(P2) if P2 is not None else None
+    lib().orc_bicgstab(pat.n, pat.row_ptrs, pat.col_idx, pat.diag, _f64(A), _f64(b), x, pc_kind,
+                       p1.ctypes.data if p1 is not None else None,
+                       p2.ctypes.data if p2 is not None else None,
+                       C.byref(o), C.byref(it), C.byref(cv), C.byref(ir), C.byref(tr))
+    return x, it.value, cv.value, ir.value, tr.value
+
+
+def batch_solve(pat, vals, b, precond=ILU0, sweeps=5, o: Opts | None = None, threads=1):
+    o = o or opts()
+    vals = _f64(vals)
+    b = _f64(b)
+    B = b.shape[0]
+    x = np.empty((B, pat.n))
+    it = np.empty(B, dtype=np.float32)
+    cv = np.empty(B, dtype=np.int32)
+    ir = np.empty(B)
+    tr = np.empty(B)
+    st = lib().orc_batch_solve(pat.n, pat.nnz, pat.row_ptrs, pat.col_idx, B, vals, b, x, precond,
+                               sweeps, C.byref(o), it, cv, ir, tr, threads)
+    if st != 0:
+        raise RuntimeError(f"orc_batch_solve failed: {st}")
+    return x, it, cv, ir, tr

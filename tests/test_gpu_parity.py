@@ -1,0 +1,219 @@
+"""GPU parity: the HIP kernels (through the C ABI) against the CPU oracle.
+
+The kernels and the oracle use the same operation order (explicit fma,
+wave64 dot order, CSR-order row sums), so results are compared bit for bit:
+factors, solutions, iteration counts, flags and residual norms.  Sizes are
+chosen so the oracle finishes in seconds; the full-size batch is covered by
+size-independent properties (every system converged, true relres <= tol,
+spot checks against the oracle).
+"""
+import numpy as np
+import pytest
+
+import bilu
+import oracle as O
+import synth
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def dev():
+    assert torch.cuda.is_available(), "GPU tests need a HIP device"
+    return torch.device("cuda:0")
+
+
+def to_dev(a, dtype=torch.float64):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(dev(), dtype=dtype)
+
+
+def run_gpu(pat, A, b, **kw):
+    P = bilu.Pattern.from_synth(pat)
+    dA, db = to_dev(A), to_dev(b)
+    x = torch.empty_like(db)
+    it, cv, ir, tr = bilu.solve(P, dA, db, x, bilu.opts(**kw))
+    torch.cuda.synchronize()
+    return x.cpu().numpy(), it.cpu().numpy(), cv.cpu().numpy(), ir.cpu().numpy(), tr.cpu().numpy()
+
+
+ORACLE_PC = {bilu.ILU0: O.ILU0, bilu.PARILU0: O.PARILU0, bilu.ILU0_ISAI: O.ILU0_ISAI,
+             bilu.PARILU0_ISAI: O.PARILU0_ISAI, bilu.GENERAL_ISAI: O.GENERAL_ISAI, bilu.NONE: O.NONE}
+
+
+def run_oracle(pat, A, b, precond=bilu.ILU0, parilu_sweeps=5, tol=1e-11, tol_mode=0, max_iter=500,
+               true_res_restart=0, max_restarts=3, **_):
+    o = O.opts(tol=tol, tol_mode=tol_mode, max_iter=max_iter, true_res_restart=true_res_restart,
+               max_restarts=max_restarts)
+    return O.batch_solve(pat, A, b, ORACLE_PC[precond], sweeps=parilu_sweeps, o=o, threads=8)
+
+
+def assert_same(g, o, what=""):
+    names = ["x", "iters", "conv", "iter_res", "true_res"]
+    for name, u, v in zip(names, g, o):
+        assert np.array_equal(u, v), f"{what}: {name} differs (max |diff| {np.nanmax(np.abs(u - v))})"
+
+
+CASES = [("stencil5", 64, 100), ("random", 256, 144), ("random", 64, 32), ("random", 64, 90),
+         ("random", 32, 300)]
+
+
+@pytest.mark.parametrize("kind,B,n", CASES)
+def test_merged_ilu0_bitwise(kind, B, n):
+    print(f"case merged ilu0 {kind} B={B} n={n}", flush=True)
+    pat, A, b = synth.batch(kind, B, n=n)
+    g = run_gpu(pat, A, b)
+    o = run_oracle(pat, A, b)
+    assert (o[2] == 1).all()
+    assert_same(g, o, f"{kind} n={n}")
+
+
+@pytest.mark.parametrize("precond", [bilu.PARILU0, bilu.ILU0_ISAI, bilu.PARILU0_ISAI,
+                                     bilu.GENERAL_ISAI, bilu.NONE])
+def test_precond_variants_bitwise(precond):
+    print(f"case precond {precond}", flush=True)
+    pat, A, b = synth.batch("random", 128, n=144)
+    kw = dict(precond=precond, parilu_sweeps=5, tol=1e-10, tol_mode=bilu.TOL_RELATIVE)
+    g = run_gpu(pat, A, b, **kw)
+    o = run_oracle(pat, A, b, **kw)
+    assert_same(g, o, f"precond {precond}")
+
+
+def test_relative_tol_with_restart_meets_1e10():
+    pat, A, b = synth.batch("random", 512, n=144)
+    kw = dict(tol=1e-10, tol_mode=bilu.TOL_RELATIVE, true_res_restart=1)
+    g = run_gpu(pat, A, b, **kw)
+    o = run_oracle(pat, A, b, **kw)
+    assert_same(g, o, "relative+restart")
+    relres = g[4] / np.linalg.norm(b, axis=1)
+    assert (g[2] == 1).all() and relres.max() < 1e-10
+
+
+def test_segregated_equals_merged():
+    pat, A, b = synth.batch("random", 128, n=144)
+    for pc in (bilu.ILU0, bilu.PARILU0):
+        m = run_gpu(pat, A, b, precond=pc)
+        s = run_gpu(pat, A, b, precond=pc, merge=bilu.SEGREGATED)
+        assert_same(m, s, f"segregated pc={pc}")
+
+
+def test_factor_entry_points_bitwise():
+    pat, A, _ = synth.batch("random", 64, n=144)
+    P = bilu.Pattern.from_synth(pat)
+    dA = to_dev(A)
+    F = torch.empty_like(dA)
+    st = torch.empty(64, dtype=torch.int32, device=dev())
+    bilu.ilu0(P, dA, F, st)
+    torch.cuda.synchronize()
+    Fo = np.stack([O.ilu0(pat, A[s])[0] for s in range(64)])
+    assert np.array_equal(F.cpu().numpy(), Fo) and not st.cpu().numpy().any()
+    # factor residual (reduction order differs: tolerance)
+    res = torch.empty(64, dtype=torch.float64, device=dev())
+    bilu.factor_residual(P, dA, F, res)
+    ro = np.array([O.factor_residual(pat, A[s], Fo[s]) for s in range(64)])
+    assert np.allclose(res.cpu().numpy(), ro, rtol=1e-6, atol=1e-13 * np.abs(A).max())
+    # ParILU sweeps: Jacobi, bitwise
+    for sweeps in (0, 1, 5, 40):
+        Fp = torch.empty_like(dA)
+        bilu.parilu0(P, dA, Fp, sweeps)
+        torch.cuda.synchronize()
+        Fpo = np.stack([O.parilu0(pat, A[s], sweeps) for s in range(64)])
+        assert np.array_equal(Fp.cpu().numpy(), Fpo), f"parilu sweeps={sweeps}"
+    # reference L/U split
+    lrp, lci, urp, uci = P.lu_split()
+    inf = P.info()
+    L = torch.empty(64, inf.l_nnz, dtype=torch.float64, device=dev())
+    U = torch.empty(64, inf.u_nnz, dtype=torch.float64, device=dev())
+    bilu.split_lu(P, F, L, U)
+    torch.cuda.synchronize()
+    Ln, Un = L.cpu().numpy(), U.cpu().numpy()
+    for s in (0, 63):
+        Fd = pat.dense(Fo[s])
+        Ld = np.zeros((pat.n, pat.n))
+        Ld[np.repeat(np.arange(pat.n), np.diff(lrp)), lci] = Ln[s]
+        Ud = np.zeros((pat.n, pat.n))
+        Ud[np.repeat(np.arange(pat.n), np.diff(urp)), uci] = Un[s]
+        assert np.array_equal(Ld, np.tril(Fd, -1) + np.eye(pat.n))
+        assert np.array_equal(Ud, np.triu(Fd))
+
+
+@pytest.mark.parametrize("typ", [bilu.ISAI_LOWER, bilu.ISAI_UPPER, bilu.ISAI_GENERAL])
+def test_isai_bitwise(typ):
+    pat, A, _ = synth.batch("random", 32, n=144)
+    Fo = np.stack([O.ilu0(pat, A[s])[0] for s in range(32)])
+    src = A if typ == bilu.ISAI_GENERAL else Fo
+    P = bilu.Pattern.from_synth(pat)
+    out = torch.zeros(32, pat.nnz, dtype=torch.float64, device=dev())
+    st = torch.empty(32, dtype=torch.int32, device=dev())
+    bilu.isai(P, to_dev(src), typ, out, st)
+    torch.cuda.synchronize()
+    want = np.stack([O.isai(pat, src[s], typ)[0] for s in range(32)])
+    assert np.array_equal(out.cpu().numpy(), want) and not st.cpu().numpy().any()
+
+
+def test_true_residual_kernel():
+    pat, A, b = synth.batch("random", 16, n=144)
+    x = np.random.default_rng(0).standard_normal(b.shape)
+    P = bilu.Pattern.from_synth(pat)
+    res = torch.empty(16, dtype=torch.float64, device=dev())
+    bilu.true_residual(P, to_dev(A), to_dev(b), to_dev(x), res)
+    torch.cuda.synchronize()
+    want = np.array([np.sqrt(O.dot(b[s] - O.spmv(pat, A[s], x[s]), b[s] - O.spmv(pat, A[s], x[s])))
+                     for s in range(16)])
+    assert np.array_equal(res.cpu().numpy(), want)
+
+
+def test_edge_cases_no_hang():
+    pat, A, b = synth.batch("random", 8, n=144)
+    A = A.copy()
+    b = b.copy()
+    b[0] = 0.0                      # b = 0 -> x = 0, iter 0, flag 1
+    A[1, 7] = np.nan                # non-finite -> flag -3, no hang
+    A[2, pat.diag[0]] = 0.0         # zero pivot -> flag -2
+    b[3] = 1e-14                    # ||b|| < ATOL -> iter 0, flag 1
+    g = run_gpu(pat, A, b)
+    o = run_oracle(pat, A, b)
+    x, it, cv, ir, tr = g
+    assert cv[0] == 1 and it[0] == 0 and not x[0].any()
+    assert cv[1] == bilu.NONFINITE
+    assert cv[2] == bilu.ZERO_PIVOT and not x[2].any()
+    assert cv[3] == 1 and it[3] == 0
+    assert (cv[4:] == 1).all()
+    assert np.array_equal(cv, o[2]) and np.array_equal(it, o[1])
+    ok = np.array([0, 2, 3, 4, 5, 6, 7])
+    assert np.array_equal(x[ok], o[0][ok])
+    # max_iter = 1
+    g = run_gpu(pat, A[4:], b[4:], max_iter=1, precond=bilu.NONE)
+    assert (g[2] == -1).all() and (g[1] <= 1).all()
+
+
+def test_empty_batch_and_bad_args():
+    pat, A, b = synth.batch("random", 1, n=144)
+    P = bilu.Pattern.from_synth(pat)
+    e = torch.empty(0, pat.nnz, dtype=torch.float64, device=dev())
+    eb = torch.empty(0, pat.n, dtype=torch.float64, device=dev())
+    bilu.solve(P, e, eb, eb.clone())
+    with pytest.raises(bilu.BiluError):
+        bilu.Pattern(3, [0, 1, 2, 3], [0, 2, 1])        # missing diagonal in rows 1, 2
+    with pytest.raises(bilu.BiluError):
+        bilu.Pattern(2, [0, 2, 3], [1, 0, 1])           # unsorted row
+    big = synth.random_pattern(400, 0.01)
+    with pytest.raises(bilu.BiluError):
+        Pb = bilu.Pattern.from_synth(big)
+        bilu.solve(Pb, torch.zeros(1, big.nnz, dtype=torch.float64, device=dev()),
+                   torch.ones(1, big.n, dtype=torch.float64, device=dev()),
+                   torch.zeros(1, big.n, dtype=torch.float64, device=dev()))
+
+
+def test_full_size_batch_properties():
+    """65536 systems of the n=144 pattern: all converge to relres < 1e-10;
+    a sample is bitwise equal to the oracle."""
+    B = 65536
+    pat, A, b = synth.batch("random", B, n=144)
+    kw = dict(tol=1e-10, tol_mode=bilu.TOL_RELATIVE, true_res_restart=1)
+    x, it, cv, ir, tr = run_gpu(pat, A, b, **kw)
+    relres = tr / np.linalg.norm(b, axis=1)
+    assert (cv == 1).all() and relres.max() < 1e-10
+    idx = np.random.default_rng(1).choice(B, 256, replace=False)
+    o = run_oracle(pat, A[idx], b[idx], **kw)
+    assert_same((x[idx], it[idx], cv[idx], ir[idx], tr[idx]), o, "full-size sample")
