@@ -61,6 +61,10 @@ extern "C" {
 #define BILU_MERGED 0         /* factor and solve in one launch (merged.cu) */
 #define BILU_SEGREGATED 1     /* factor launch, factors through HBM, solve launch (segregated.cu) */
 
+#define BILU_KERNEL_AUTO 0        /* fastest kernel that supports the pattern and options */
+#define BILU_KERNEL_WAVE 1        /* one wave64 workgroup per system, schedules read from global */
+#define BILU_KERNEL_PERSISTENT 2  /* persistent multi-wave workgroups, schedules in LDS */
+
 #define BILU_ISAI_GENERAL 0
 #define BILU_ISAI_LOWER 1     /* strictly-lower values + unit diagonal */
 #define BILU_ISAI_UPPER 2     /* diagonal + strictly-upper values */
@@ -76,6 +80,7 @@ typedef struct {
     int merge;            /* BILU_MERGED / BILU_SEGREGATED */
     int true_res_restart; /* 1: restart from b - A x when the true residual misses tol */
     int max_restarts;     /* bound on such restarts (default 3) */
+    int kernel;           /* BILU_KERNEL_*: which kernel family runs the solve */
 } bilu_opts;
 
 typedef struct {

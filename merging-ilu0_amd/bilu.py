@@ -21,6 +21,7 @@ ILU0, PARILU0, ILU0_ISAI, PARILU0_ISAI, GENERAL_ISAI, NONE = 0, 1, 2, 3, 4, 5
 TOL_ABSOLUTE, TOL_RELATIVE = 0, 1
 MERGED, SEGREGATED = 0, 1
 ISAI_GENERAL, ISAI_LOWER, ISAI_UPPER = 0, 1, 2
+KERNEL_AUTO, KERNEL_WAVE, KERNEL_PERSISTENT = 0, 1, 2
 
 # every symbol include/bilu.h declares
 EXPORTS = [
@@ -45,6 +46,7 @@ class Opts(C.Structure):
         ("merge", C.c_int),
         ("true_res_restart", C.c_int),
         ("max_restarts", C.c_int),
+        ("kernel", C.c_int),
     ]
 
 
@@ -91,9 +93,9 @@ def _check(rc, what):
 
 
 def opts(precond=ILU0, parilu_sweeps=5, tol=1e-11, tol_mode=TOL_ABSOLUTE, max_iter=500,
-         merge=MERGED, true_res_restart=0, max_restarts=3) -> Opts:
+         merge=MERGED, true_res_restart=0, max_restarts=3, kernel=KERNEL_AUTO) -> Opts:
     return Opts(precond, parilu_sweeps, tol, tol_mode, max_iter, merge, true_res_restart,
-                max_restarts)
+                max_restarts, kernel)
 
 
 def _ptr(t):

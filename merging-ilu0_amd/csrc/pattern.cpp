@@ -148,7 +148,128 @@ int analyse_pattern(int n, int nnz, const int *rp, const int *ci, PatternAnalysi
         pa.u_rp[i + 1] = int(pa.u_ci.size());
     }
     for (int t = 0; t < 3; t++) build_isai(pa, t, pa.isai[t]);
+    build_fast_plan(pa);
     return BILU_OK;
+}
+
+namespace {
+
+struct BlobWriter {
+    std::vector<uint8_t> b;
+    int put16(const std::vector<int> &v)
+    {
+        while (b.size() % 16) b.push_back(0);
+        const int off = int(b.size());
+        for (int x : v) { uint16_t u = uint16_t(x); b.push_back(uint8_t(u & 0xFF)); b.push_back(uint8_t(u >> 8)); }
+        return off;
+    }
+    void finish() { while (b.size() % 16) b.push_back(0); }
+};
+
+}  // namespace
+
+void build_fast_plan(PatternAnalysis &pa)
+{
+    FastPlan &fp = pa.fast;
+    fp = FastPlan();
+    const int n = pa.n, nnz = pa.nnz;
+    if (n >= 0xFFFF || nnz >= 0xFFFF) { fp.why = "n or nnz beyond the u16 schedule tables"; return; }
+    // balanced SpMV slot layout: longest-processing-time assignment of whole
+    // rows to 64 lanes; each lane keeps its rows in increasing row order so
+    // every row sum runs left to right over the CSR entries
+    std::vector<int> order(n);
+    for (int i = 0; i < n; i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        return (pa.rp[a + 1] - pa.rp[a]) > (pa.rp[b + 1] - pa.rp[b]);
+    });
+    std::vector<int> load(64, 0);
+    std::vector<std::vector<int>> rows(64);
+    for (int i : order) {
+        int best = 0;
+        for (int l = 1; l < 64; l++) if (load[l] < load[best]) best = l;
+        load[best] += pa.rp[i + 1] - pa.rp[i];
+        rows[best].push_back(i);
+    }
+    // improve the LPT start by row swaps between the fullest lane and the others
+    auto rlen = [&](int i) { return pa.rp[i + 1] - pa.rp[i]; };
+    for (int iter = 0; iter < 4096; iter++) {
+        int hi = 0;
+        for (int l = 1; l < 64; l++) if (load[l] > load[hi]) hi = l;
+        bool improved = false;
+        for (int l = 0; l < 64 && !improved; l++) {
+            if (l == hi) continue;
+            for (size_t a = 0; a < rows[hi].size() && !improved; a++) {
+                const int ra = rows[hi][a];
+                // move
+                if (load[l] + rlen(ra) < load[hi]) {
+                    load[l] += rlen(ra); load[hi] -= rlen(ra);
+                    rows[l].push_back(ra); rows[hi].erase(rows[hi].begin() + a);
+                    improved = true;
+                    break;
+                }
+                for (size_t b = 0; b < rows[l].size(); b++) {
+                    const int rb = rows[l][b];
+                    const int d = rlen(ra) - rlen(rb);
+                    if (d > 0 && load[l] + d < load[hi]) {
+                        load[l] += d; load[hi] -= d;
+                        std::swap(rows[hi][a], rows[l][b]);
+                        improved = true;
+                        break;
+                    }
+                }
+            }
+        }
+        if (!improved) break;
+    }
+    int S = 0;
+    for (int l = 0; l < 64; l++) { std::sort(rows[l].begin(), rows[l].end()); S = std::max(S, load[l]); }
+    fp.S = S;
+    fp.slot_meta.assign(size_t(S) * 64, 0xFFFF0000u);
+    fp.slot_pos.assign(size_t(S) * 64, -1);
+    for (int l = 0; l < 64; l++) {
+        int s = 0;
+        for (int i : rows[l]) {
+            for (int p = pa.rp[i]; p < pa.rp[i + 1]; p++, s++) {
+                const uint32_t row = (p == pa.rp[i + 1] - 1) ? uint32_t(i) : 0xFFFFu;
+                fp.slot_meta[size_t(s) * 64 + l] = uint32_t(pa.ci[p]) | (row << 16);
+                fp.slot_pos[size_t(s) * 64 + l] = p;
+            }
+        }
+    }
+    // level descriptors: (i, rp[i], diag[i], rp[i+1]) per row in level order
+    auto descs = [&](const std::vector<int> &lrows) {
+        std::vector<int> d;
+        for (int i : lrows) { d.push_back(i); d.push_back(pa.rp[i]); d.push_back(pa.diag[i]); d.push_back(pa.rp[i + 1]); }
+        return d;
+    };
+    const std::vector<int> ld = descs(pa.lev_l_rows), ud = descs(pa.lev_u_rows);
+    for (int v : pa.upd) if (v >= 0xFFFF) { fp.why = "ILU update list beyond u16"; return; }
+    if (pa.upd.size() / 2 >= 0xFFFF || pa.dep.size() / 2 >= 0xFFFF) { fp.why = "update lists beyond u16"; return; }
+    std::vector<int> pdiv(nnz, 0xFFFF);
+    for (int i = 0; i < n; i++)
+        for (int p = pa.rp[i]; p < pa.diag[i]; p++) pdiv[p] = pa.diag[pa.ci[p]];
+    for (int variant = 0; variant < 2; variant++) {
+        BlobWriter w;
+        fp.o_ci = w.put16(pa.ci);
+        fp.o_diag = w.put16(pa.diag);
+        fp.o_ldesc = w.put16(ld);
+        fp.o_udesc = w.put16(ud);
+        fp.o_lptr = w.put16(pa.lev_l_ptr);
+        fp.o_uptr = w.put16(pa.lev_u_ptr);
+        if (variant == 0) {
+            fp.o_upd_ptr = w.put16(pa.upd_ptr);
+            fp.o_upd = w.put16(pa.upd.empty() ? std::vector<int>{0, 0} : pa.upd);
+            w.finish();
+            fp.blob_ilu = w.b;
+        } else {
+            fp.o_dep_ptr = w.put16(pa.dep_ptr);
+            fp.o_dep = w.put16(pa.dep.empty() ? std::vector<int>{0, 0} : pa.dep);
+            fp.o_pdiv = w.put16(pdiv);
+            w.finish();
+            fp.blob_par = w.b;
+        }
+    }
+    fp.ok = true;
 }
 
 }  // namespace bilu

@@ -18,6 +18,25 @@ struct IsaiMap {
     int max_m = 0;
 };
 
+// Schedule tables of the persistent multi-wave kernels: one byte blob
+// copied once into LDS per workgroup (u16 indices), plus the per-lane SpMV
+// slot layout that is held in VGPRs.
+struct FastPlan {
+    bool ok = false;          // pattern fits the u16 tables and the slot budget
+    std::string why;          // reason when !ok
+    int S = 0;                // SpMV slots per lane (balanced row -> lane assignment)
+    // slot tables [S][64]: meta = col | (row << 16), row = 0xFFFF if the slot
+    // does not end a row; pos = pattern position (-1 = padding)
+    std::vector<uint32_t> slot_meta;
+    std::vector<int32_t> slot_pos;
+    // LDS blobs (bytes) for the exact-ILU(0) and the ParILU(0) variants;
+    // offsets in bytes into the blob
+    std::vector<uint8_t> blob_ilu, blob_par;
+    int o_ci = 0, o_diag = 0, o_ldesc = 0, o_udesc = 0, o_lptr = 0, o_uptr = 0;
+    int o_upd_ptr = 0, o_upd = 0;              // ILU blob only
+    int o_dep_ptr = 0, o_dep = 0, o_pdiv = 0;  // ParILU blob only
+};
+
 struct PatternAnalysis {
     int n = 0, nnz = 0;
     std::vector<int> rp, ci, diag;
@@ -37,7 +56,10 @@ struct PatternAnalysis {
     std::vector<int> u_rp, u_ci, u_src;
     IsaiMap isai[3];  // indexed by BILU_ISAI_* type
     int max_row = 0;
+    FastPlan fast;
 };
+
+void build_fast_plan(PatternAnalysis &pa);
 
 // returns BILU_OK or an error code; `why` gets a message
 int analyse_pattern(int n, int nnz, const int *rp, const int *ci, PatternAnalysis &pa,

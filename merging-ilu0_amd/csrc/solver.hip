@@ -66,12 +66,49 @@ struct SolveParams {
 };
 
 // ---------------------------------------------------------------- wave ops
+// 64-bit lane exchanges without the LDS pipe: DPP within 16-lane rows,
+// gfx950 permlane swaps across rows / halves
+template <int CTRL>
+__device__ __forceinline__ double dpp_mov(double v)
+{
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+
+// value of lane (lane ^ 32)
+__device__ __forceinline__ double lane_xor32(double v)
+{
+    const unsigned lo = __double2loint(v), hi = __double2hiint(v);
+    const auto a = __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
+    const auto b = __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
+    const bool low = (__lane_id() & 32) == 0;
+    return __hiloint2double(low ? b[1] : b[0], low ? a[1] : a[0]);
+}
+
+// value of lane (lane ^ 16)
+__device__ __forceinline__ double lane_xor16(double v)
+{
+    const unsigned lo = __double2loint(v), hi = __double2hiint(v);
+    const auto a = __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
+    const auto b = __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
+    const bool even = (__lane_id() & 16) == 0;
+    return __hiloint2double(even ? b[1] : b[0], even ? a[1] : a[0]);
+}
+
+// Sum over the wave, identical in every lane.  The association is the xor
+// butterfly with masks 32,16,8,4,2,1, whose lane-0 value equals a shfl_down
+// tree (offsets 32..1).  Masks 8 and 4 use row rotations: after the earlier
+// stages the partial sums are periodic, so the rotated partner holds the
+// same bits as the xor partner.
 __device__ __forceinline__ double wave_sum(double v)
 {
-    // butterfly; lane 0 associates exactly like a shfl_down tree and every
-    // lane ends with the same bits (IEEE addition is commutative)
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v = v + __shfl_xor(v, off, WAVE);
+    v = v + lane_xor32(v);
+    v = v + lane_xor16(v);
+    v = v + dpp_mov<0x128>(v);  // row_ror:8
+    v = v + dpp_mov<0x124>(v);  // row_ror:4
+    v = v + dpp_mov<0x4E>(v);   // quad_perm [2,3,0,1]
+    v = v + dpp_mov<0xB1>(v);   // quad_perm [1,0,3,2]
     return v;
 }
 
@@ -659,6 +696,8 @@ __global__ __launch_bounds__(WAVE) void k_true_residual(DevPattern P, int batch,
     if (lane == 0) gRes[sys] = r;
 }
 
+#include "fast_kernels.hpp"
+
 }  // namespace
 
 // ====================================================================== host
@@ -667,7 +706,14 @@ struct bilu_pattern_s {
     DevPattern dev{};
     int *dbuf = nullptr;
     int device = 0;
-    size_t max_lds = 65536;
+    size_t max_lds = 65536;      // per workgroup
+    size_t lds_per_cu = 65536;
+    int num_cus = 1;
+    // persistent-kernel tables
+    uint8_t *d_blob_ilu = nullptr, *d_blob_par = nullptr;
+    uint32_t *d_slot_meta = nullptr;
+    int32_t *d_slot_pos = nullptr;
+    int *d_counter = nullptr;
 };
 
 namespace {
@@ -724,7 +770,89 @@ int upload(bilu_pattern_s &h)
         d.isai[t].beg = B + o_is[t][0]; d.isai[t].m = B + o_is[t][1]; d.isai[t].goff = B + o_is[t][2];
         d.isai[t].gather = B + o_is[t][3]; d.isai[t].rhs = B + o_is[t][4];
     }
+    HIP_TRY(hipMalloc((void **)&h.d_counter, 64));
+    HIP_TRY(hipMemset(h.d_counter, 0, 64));
+    const auto &fp = pa.fast;
+    if (fp.ok) {
+        HIP_TRY(hipMalloc((void **)&h.d_blob_ilu, fp.blob_ilu.size()));
+        HIP_TRY(hipMemcpy(h.d_blob_ilu, fp.blob_ilu.data(), fp.blob_ilu.size(), hipMemcpyHostToDevice));
+        HIP_TRY(hipMalloc((void **)&h.d_blob_par, fp.blob_par.size()));
+        HIP_TRY(hipMemcpy(h.d_blob_par, fp.blob_par.data(), fp.blob_par.size(), hipMemcpyHostToDevice));
+        HIP_TRY(hipMalloc((void **)&h.d_slot_meta, fp.slot_meta.size() * 4));
+        HIP_TRY(hipMemcpy(h.d_slot_meta, fp.slot_meta.data(), fp.slot_meta.size() * 4, hipMemcpyHostToDevice));
+        HIP_TRY(hipMalloc((void **)&h.d_slot_pos, fp.slot_pos.size() * 4));
+        HIP_TRY(hipMemcpy(h.d_slot_pos, fp.slot_pos.data(), fp.slot_pos.size() * 4, hipMemcpyHostToDevice));
+    }
     return BILU_OK;
+}
+
+void free_handle(bilu_pattern_s *h)
+{
+    if (!h) return;
+    if (h->dbuf) (void)hipFree(h->dbuf);
+    if (h->d_counter) (void)hipFree(h->d_counter);
+    if (h->d_blob_ilu) (void)hipFree(h->d_blob_ilu);
+    if (h->d_blob_par) (void)hipFree(h->d_blob_par);
+    if (h->d_slot_meta) (void)hipFree(h->d_slot_meta);
+    if (h->d_slot_pos) (void)hipFree(h->d_slot_pos);
+    delete h;
+}
+
+template <int NR, int SM>
+int launch_fast_t(bilu_pattern_s &h, const FastParams &fp, int grid, int threads, size_t lds, int batch,
+                  const double *A, const double *b, double *x, const double *F, const SolveParams &sp,
+                  float *it, int *cv, double *ir, double *tr, hipStream_t st)
+{
+    auto kern = k_fast<NR, SM>;
+    if (lds > 65536) HIP_TRY(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, st, fp.counter);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(threads), lds, st, h.dev, fp, batch, A, b, x, F, sp, it, cv, ir, tr);
+    HIP_TRY(hipGetLastError());
+    return BILU_OK;
+}
+
+// returns -1 when the persistent kernel cannot take this pattern/option set
+int launch_fast(bilu_pattern_s &h, int batch, const double *A, const double *b, double *x, const double *F,
+                const SolveParams &sp, float *it, int *cv, double *ir, double *tr, hipStream_t st)
+{
+    const auto &pl = h.pa.fast;
+    const int n = h.pa.n, nnz = h.pa.nnz;
+    const int nr = (n + WAVE - 1) / WAVE;
+    if (!pl.ok || nr > MAX_NR) return -1;
+    int sm;
+    if (pl.S <= 24) sm = 24;
+    else if (pl.S <= 36) sm = 36;
+    else if (pl.S <= 48) sm = 48;
+    else return -1;
+    const bool par = sp.factor == FK_PARILU;
+    FastParams fp{};
+    fp.blob = par ? h.d_blob_par : h.d_blob_ilu;
+    fp.blob_bytes = int(par ? pl.blob_par.size() : pl.blob_ilu.size());
+    fp.o_ci = pl.o_ci; fp.o_diag = pl.o_diag; fp.o_ldesc = pl.o_ldesc; fp.o_udesc = pl.o_udesc;
+    fp.o_lptr = pl.o_lptr; fp.o_uptr = pl.o_uptr; fp.o_upd_ptr = pl.o_upd_ptr; fp.o_upd = pl.o_upd;
+    fp.o_dep_ptr = pl.o_dep_ptr; fp.o_dep = pl.o_dep; fp.o_pdiv = pl.o_pdiv;
+    fp.nlev_l = h.dev.nlev_l; fp.nlev_u = h.dev.nlev_u;
+    fp.S = pl.S; fp.slot_meta = h.d_slot_meta; fp.slot_pos = h.d_slot_pos;
+    // per wave: F, (ParILU: the previous sweep's factors), e0, e1, e2
+    fp.wave_bytes = int(((size_t)((par ? 2 : 1) * nnz + 3 * n) * sizeof(double) + 15) / 16 * 16);
+    fp.par_w = par ? 1 : 0;
+    fp.counter = h.d_counter;
+    const size_t avail = h.max_lds > (size_t)fp.blob_bytes ? h.max_lds - fp.blob_bytes : 0;
+    int waves = (int)std::min<size_t>(8, avail / fp.wave_bytes);
+    if (waves < 1) return -1;
+    waves = std::min(waves, std::max(1, (batch + 0) / 1));
+    const size_t lds = fp.blob_bytes + (size_t)waves * fp.wave_bytes;
+    const int per_cu = std::max<int>(1, (int)(h.lds_per_cu / lds));
+    int grid = h.num_cus * per_cu;
+    grid = std::max(1, std::min(grid, (batch + waves - 1) / waves));
+    const int threads = waves * WAVE;
+#define FAST_CASE(R, M) \
+    if (nr == R && sm == M) return launch_fast_t<R, M>(h, fp, grid, threads, lds, batch, A, b, x, F, sp, it, cv, ir, tr, st);
+#define FAST_ROW(R) FAST_CASE(R, 24) FAST_CASE(R, 36) FAST_CASE(R, 48)
+    FAST_ROW(1) FAST_ROW(2) FAST_ROW(3) FAST_ROW(4) FAST_ROW(5)
+#undef FAST_ROW
+#undef FAST_CASE
+    return -1;
 }
 
 template <int NR, int PC>
@@ -770,6 +898,7 @@ void bilu_opts_default(bilu_opts *o)
     o->merge = BILU_MERGED;
     o->true_res_restart = 0;
     o->max_restarts = 3;
+    o->kernel = BILU_KERNEL_AUTO;
 }
 
 const char *bilu_version(void) { return "bilu 0.1 gfx950"; }
@@ -807,8 +936,14 @@ int bilu_pattern_create(int n, int nnz, const int *row_ptrs, const int *col_idx,
     int lds = 0;
     if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, h->device) == hipSuccess && lds > 0)
         h->max_lds = (size_t)lds;
+    h->lds_per_cu = h->max_lds;
+    if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, h->device) == hipSuccess && lds > 0)
+        h->lds_per_cu = (size_t)lds;
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->device) == hipSuccess && cus > 0)
+        h->num_cus = cus;
     st = upload(*h);
-    if (st != BILU_OK) { if (h->dbuf) (void)hipFree(h->dbuf); delete h; return st; }
+    if (st != BILU_OK) { free_handle(h); return st; }
     *out = h;
     return BILU_OK;
 }
@@ -816,8 +951,7 @@ int bilu_pattern_create(int n, int nnz, const int *row_ptrs, const int *col_idx,
 int bilu_pattern_destroy(bilu_pattern_t p)
 {
     if (!p) return BILU_ERR_INVALID_ARG;
-    if (p->dbuf) (void)hipFree(p->dbuf);
-    delete p;
+    free_handle(p);
     return BILU_OK;
 }
 
@@ -890,9 +1024,19 @@ int bilu_solve(bilu_pattern_t h, int batch, const double *d_A, const double *d_b
         F = Fbuf;
         sp.factor = FK_LOAD;
     }
+    int rc = -1;
+    if (pc == PC_LU && o.kernel != BILU_KERNEL_WAVE) {
+        rc = launch_fast(*h, batch, d_A, d_b, d_x, F, sp, d_iter, d_conv, d_iter_res, d_true_res, st);
+        if (rc == -1 && o.kernel == BILU_KERNEL_PERSISTENT) rc = BILU_ERR_TOO_LARGE;
+    } else if (o.kernel == BILU_KERNEL_PERSISTENT) {
+        rc = BILU_ERR_INVALID_ARG;   // the persistent kernel serves the LU preconditioners only
+    }
+    if (rc != -1) {
+        if (Fbuf) HIP_TRY(hipFreeAsync(Fbuf, st));
+        return rc;
+    }
     const bool need_w = (sp.factor == FK_PARILU) || pc == PC_ISAI_LU || pc == PC_ISAI_GEN;
     const size_t lds = sizeof(double) * ((size_t)nnz * (2 + (need_w ? 1 : 0)) + (size_t)n * (3 + (pc == PC_ISAI_LU ? 1 : 0)));
-    int rc;
     if (lds > h->max_lds) rc = BILU_ERR_TOO_LARGE;
     else if (pc == PC_LU) rc = launch_solve<PC_LU>(*h, batch, d_A, d_b, d_x, F, sp, d_iter, d_conv, d_iter_res, d_true_res, lds, st);
     else if (pc == PC_ISAI_LU) rc = launch_solve<PC_ISAI_LU>(*h, batch, d_A, d_b, d_x, F, sp, d_iter, d_conv, d_iter_res, d_true_res, lds, st);

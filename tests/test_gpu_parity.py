@@ -58,14 +58,32 @@ CASES = [("stencil5", 64, 100), ("random", 256, 144), ("random", 64, 32), ("rand
          ("random", 32, 300)]
 
 
+@pytest.mark.parametrize("kernel", [bilu.KERNEL_WAVE, bilu.KERNEL_PERSISTENT])
 @pytest.mark.parametrize("kind,B,n", CASES)
-def test_merged_ilu0_bitwise(kind, B, n):
-    print(f"case merged ilu0 {kind} B={B} n={n}", flush=True)
+def test_merged_ilu0_bitwise(kind, B, n, kernel):
+    print(f"case merged ilu0 {kind} B={B} n={n} kernel={kernel}", flush=True)
     pat, A, b = synth.batch(kind, B, n=n)
-    g = run_gpu(pat, A, b)
+    if kernel == bilu.KERNEL_PERSISTENT and n == 300:
+        with pytest.raises(bilu.BiluError):   # 65 SpMV slots per lane: beyond the register budget
+            run_gpu(pat, A, b, kernel=kernel)
+        return
+    g = run_gpu(pat, A, b, kernel=kernel)
     o = run_oracle(pat, A, b)
     assert (o[2] == 1).all()
     assert_same(g, o, f"{kind} n={n}")
+
+
+@pytest.mark.parametrize("kernel", [bilu.KERNEL_WAVE, bilu.KERNEL_PERSISTENT])
+def test_persistent_parilu_and_segregated_bitwise(kernel):
+    pat, A, b = synth.batch("random", 200, n=144)
+    for kw in (dict(precond=bilu.PARILU0, parilu_sweeps=3),
+               dict(precond=bilu.ILU0, merge=bilu.SEGREGATED),
+               dict(precond=bilu.PARILU0, parilu_sweeps=5, merge=bilu.SEGREGATED),
+               dict(precond=bilu.ILU0, tol=1e-10, tol_mode=bilu.TOL_RELATIVE, true_res_restart=1)):
+        print(f"case {kw} kernel={kernel}", flush=True)
+        g = run_gpu(pat, A, b, kernel=kernel, **kw)
+        o = run_oracle(pat, A, b, **kw)
+        assert_same(g, o, f"{kw} kernel={kernel}")
 
 
 @pytest.mark.parametrize("precond", [bilu.PARILU0, bilu.ILU0_ISAI, bilu.PARILU0_ISAI,
