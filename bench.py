@@ -30,6 +30,7 @@ def parse():
     ap.add_argument("--precond", default="ilu0",
                     choices=["ilu0", "parilu0", "ilu0_isai", "parilu0_isai", "general_isai"])
     ap.add_argument("--merge", default="merged", choices=["merged", "segregated"])
+    ap.add_argument("--kernel", default="auto", choices=["auto", "wave", "persistent"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run only a 1 GiB device copy (HBM counter calibration)")
@@ -112,8 +113,10 @@ def main():
     pc = {"ilu0": bilu.ILU0, "parilu0": bilu.PARILU0, "ilu0_isai": bilu.ILU0_ISAI,
           "parilu0_isai": bilu.PARILU0_ISAI, "general_isai": bilu.GENERAL_ISAI}[args.precond]
     merge = bilu.MERGED if args.merge == "merged" else bilu.SEGREGATED
+    kern = {"auto": bilu.KERNEL_AUTO, "wave": bilu.KERNEL_WAVE,
+            "persistent": bilu.KERNEL_PERSISTENT}[args.kernel]
     o = bilu.opts(precond=pc, parilu_sweeps=5, tol=1e-10, tol_mode=bilu.TOL_RELATIVE,
-                  true_res_restart=1, max_restarts=3, merge=merge)
+                  true_res_restart=1, max_restarts=3, merge=merge, kernel=kern)
     stream = torch.cuda.current_stream()
 
     def step():
@@ -144,7 +147,8 @@ def main():
     achieved = alg_bytes / (ms_per_step * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
-                "kernel": "k_solve<3,PC_LU>" if args.merge == "merged" else "k_factor+k_solve",
+                "kernel": ("k_solve (one wave per system)" if args.kernel == "wave" else
+                           "k_fast (persistent multi-wave)") if args.merge == "merged" else "k_factor + solve",
                 "alg_bytes_per_launch": alg_bytes}
 
     cpu = None
@@ -169,7 +173,8 @@ def main():
                                f"{args.precond} {args.merge})",
                    "batch": B, "n": n, "nnz": nnz, "precond": args.precond, "merge": args.merge,
                    "levels_lower": info.levels_lower, "levels_upper": info.levels_upper,
-                   "parallelism": "one wave64 workgroup per system"},
+                   "kernel": args.kernel,
+                   "parallelism": "one wave64 per system; batch sharded over CUs"},
         "all_converged_relres_le_1e-10": all_ok,
         "max_relres": float(relres.max()),
         "iterations": {"min": float(ith.min()), "mean": float(ith.mean()), "max": float(ith.max())},
