@@ -4,27 +4,39 @@
 // to_lds, from_lds, wave_sum).
 //
 // Layout of one workgroup (W waves, W <= 8, one system per wave at a time):
-//   LDS  [ schedule blob (u16 tables, shared by all waves) ]
-//        [ wave 0: F[nnz] fp64 | e0[n] | e1[n] | e2[n] ] [ wave 1 ... ]
+//   LDS  [ wave 0: F[nnz] fp64 | (ParILU: W[nnz]) | e0[n] | e1[n] ] [ wave 1 ... ]
+//   global (cache-resident, shared by every wave): solve and factorization
+//        schedules, SpMV slot tables
 //   VGPR A values in the balanced slot layout (lane l, slot s <-> one CSR
 //        position; whole rows per lane, so an SpMV row sum is a register-
 //        only left-to-right fma chain over x gathered from LDS), Krylov
 //        vectors in the row-cyclic layout, slot metadata.
 // Waves take systems from a global counter (dynamic balance: systems need
 // 5 to 14 iterations), so the grid is sized to the machine, not the batch.
-// Waves of one workgroup never synchronise with each other after the blob
-// copy: intra-wave ordering of LDS traffic is enough (a wave's LDS
-// instructions execute in issue order), enforced against compiler
-// reordering by wavefront-scope fences.
+// Waves of one workgroup never synchronise with each other: intra-wave
+// ordering of LDS traffic is enough (a wave's LDS instructions execute in
+// issue order), enforced against compiler reordering by wavefront-scope
+// fences.
+//
+// The kernel is bound by LDS and VALU instruction issue, so every phase is
+// laid out to make each LDS instruction carry useful lanes: triangular
+// solves pull entries as their columns become available (no per-level
+// row scans), Krylov vectors never round-trip through LDS except as the
+// gather source of an SpMV or solve.
 
 struct FastParams {
-    const uint8_t *blob;
-    int blob_bytes;
-    int o_ci, o_diag, o_ldesc, o_udesc, o_lptr, o_uptr, o_upd_ptr, o_upd, o_dep_ptr, o_dep, o_pdiv;
-    int nlev_l, nlev_u;
+    const uint32_t *trL, *trU;            // solve slot records [slots][64]
+    const uint16_t *sptrL, *sptrU;        // slot range per level (nlev + 1)
+    const uint16_t *rlevL, *rlevU;        // [NR][64] level of the lane's rows
+    const ushort4 *etab;                  // factorization entry records
+    const uint16_t *eptr;                 // per entry-DAG level
+    const ushort2 *dep;                   // (pos(i,k), pos(k,j)) pairs
+    const int *diag;                      // pattern diagonal positions
+    int nlev_l, nlev_u, nlev_e, ne;
     int S;
     const uint32_t *slot_meta;  // [S][64]
     const int32_t *slot_pos;    // [S][64]
+    int meta_bytes;  // shared LDS copy of slot_meta (16-byte aligned size)
     int wave_bytes;
     int par_w;      // 1: the wave region holds a second factor buffer (ParILU Jacobi)
     int *counter;
@@ -39,19 +51,18 @@ __device__ __forceinline__ void wave_sync()
 
 __global__ void k_zero_counter(int *c) { *c = 0; }
 
-// y (LDS, by row) = A x over the slot layout; x in LDS
+// y (LDS, by row) = A x over the slot layout; x in LDS; slot metadata
+// (col | row << 16, row = 0xFFFF when the slot does not end a row) is read
+// from the workgroup's shared LDS copy
 template <int SM>
-__device__ __forceinline__ void spmv_slots(const double (&a)[SM], const uint32_t (&meta)[SM], int S,
-                                           const double *x, double *y)
+__device__ __forceinline__ void spmv_slots(const double (&a)[SM], const uint32_t *__restrict__ meta, int S,
+                                           const double *x, double *y, int lane)
 {
     double acc = 0.0;
 #pragma unroll
     for (int s = 0; s < SM; s++) {
         if (s < S) {
-            uint32_t m = meta[s];
-            // opaque copy: keeps the index extraction here instead of letting
-            // it be hoisted into long-lived registers across the whole solve
-            asm volatile("" : "+v"(m));
+            const uint32_t m = meta[s * WAVE + lane];
             acc = fma(a[s], x[m & 0xFFFFu], acc);
             if ((m >> 16) != 0xFFFFu) {
                 y[m >> 16] = acc;
@@ -61,70 +72,141 @@ __device__ __forceinline__ void spmv_slots(const double (&a)[SM], const uint32_t
     }
 }
 
-// y = L^{-1} vin (unit diagonal), rows level by level
-__device__ __forceinline__ void lsolve_fast(const ushort4 *desc, const uint16_t *lptr, int nlev,
-                                            const uint16_t *ci, const double *F, const double *vin,
-                                            double *y, int lane)
+// Triangular solve by availability.  Lane l owns rows l + 64 k; their
+// partial sums live in acc[k].  Level ℓ: (1) every row of level ℓ is final:
+// L: y_r = vin_r - acc, U: x_r = (vin_r - acc) / u_rr, kept in `res` and
+// stored to `out` (LDS) for the gathers of later levels; (2) each lane folds
+// in its entries whose column lies in level ℓ, acc_k = fma(F_p, out_c, acc_k).
+// The next level's first records are fetched while this level computes.
+template <int NR, bool LOWER>
+__device__ __forceinline__ void trsv_avail(const uint32_t *__restrict__ rec, const uint16_t *__restrict__ sptr,
+                                           int nlev, const int (&rlev)[NR], const double *F,
+                                           const Vec<NR> &vin, const Vec<NR> &dg, double *out, Vec<NR> &res,
+                                           int lane)
 {
+    constexpr int CH = 4;
+    double acc[NR];
+#pragma unroll
+    for (int k = 0; k < NR; k++) acc[k] = 0.0;
+    int s = 0, se = sptr[1];
+    uint32_t pre[CH];
+#pragma unroll
+    for (int j = 0; j < CH; j++) pre[j] = (j < se) ? rec[j * WAVE + lane] : 0u;
     for (int l = 0; l < nlev; l++) {
-        const int b = lptr[l], e = lptr[l + 1];
-        for (int idx = b + lane; idx < e; idx += WAVE) {
-            const ushort4 d = desc[idx];
-            const int i = d.x, p0 = d.y, pd = d.z;
-            double acc = 0.0;
-            for (int p = p0; p < pd; p++) acc = fma(F[p], y[ci[p]], acc);
-            y[i] = vin[i] - acc;
-        }
-        wave_sync();
-    }
-}
-
-// out = U^{-1} y, rows level by level from the bottom; sums right to left
-__device__ __forceinline__ void usolve_fast(const ushort4 *desc, const uint16_t *uptr, int nlev,
-                                            const uint16_t *ci, const double *F, const double *y,
-                                            double *out, int lane)
-{
-    for (int l = 0; l < nlev; l++) {
-        const int b = uptr[l], e = uptr[l + 1];
-        for (int idx = b + lane; idx < e; idx += WAVE) {
-            const ushort4 d = desc[idx];
-            const int i = d.x, pd = d.z, pe = d.w;
-            double acc = 0.0;
-            for (int p = pe - 1; p > pd; p--) acc = fma(F[p], out[ci[p]], acc);
-            out[i] = (y[i] - acc) / F[pd];
-        }
-        wave_sync();
-    }
-}
-
-// exact ILU(0) in place, IKJ per row, rows level by level; nonzero on a zero pivot
-__device__ __forceinline__ int ilu_fast(const ushort4 *desc, const uint16_t *lptr, int nlev,
-                                        const uint16_t *ci, const uint16_t *diag,
-                                        const uint16_t *upd_ptr, const ushort2 *upd, int n,
-                                        double *F, int lane)
-{
-    int bad = 0;
-    for (int l = 0; l < nlev; l++) {
-        const int b = lptr[l], e = lptr[l + 1];
-        for (int idx = b + lane; idx < e; idx += WAVE) {
-            const ushort4 d = desc[idx];
-            const int p0 = d.y, pd = d.z;
-            for (int p = p0; p < pd; p++) {
-                const double ukk = F[diag[ci[p]]];
-                bad |= (ukk == 0.0);
-                const double lik = F[p] / ukk;
-                F[p] = lik;
-                const int ue = upd_ptr[p + 1];
-                for (int u = upd_ptr[p]; u < ue; u++) {
-                    const ushort2 du = upd[u];
-                    F[du.x] = fma(-lik, F[du.y], F[du.x]);
-                }
+#pragma unroll
+        for (int k = 0; k < NR; k++) {
+            if ((LOWER ? (rlev[k] & 0xFFFF) : (rlev[k] >> 16)) == l) {
+                const double v = LOWER ? vin.e[k] - acc[k] : (vin.e[k] - acc[k]) / dg.e[k];
+                res.e[k] = v;
+                out[lane + WAVE * k] = v;
             }
         }
         wave_sync();
+        const int nse = (l + 1 < nlev) ? (int)sptr[l + 2] : se;
+        uint32_t r[CH];
+#pragma unroll
+        for (int j = 0; j < CH; j++) r[j] = pre[j];
+#pragma unroll
+        for (int j = 0; j < CH; j++) pre[j] = (se + j < nse) ? rec[(se + j) * WAVE + lane] : 0u;
+        for (int c = s; c < se; c += CH) {
+            if (c != s) {
+#pragma unroll
+                for (int j = 0; j < CH; j++) r[j] = (c + j < se) ? rec[(c + j) * WAVE + lane] : 0u;
+            }
+            double f[CH], yv[CH];
+#pragma unroll
+            for (int j = 0; j < CH; j++) {
+                f[j] = F[r[j] & 0x3FFFu];
+                yv[j] = out[(r[j] >> 14) & 0x3FFu];
+            }
+#pragma unroll
+            for (int j = 0; j < CH; j++) {
+                if (r[j] >> 27) {
+                    const int k = (r[j] >> 24) & 7;
+#pragma unroll
+                    for (int kk = 0; kk < NR; kk++)
+                        if (kk == k) acc[kk] = fma(f[j], yv[j], acc[kk]);
+                }
+            }
+        }
+        s = se;
+        se = nse;
     }
-    for (int i = lane; i < n; i += WAVE) bad |= (F[diag[i]] == 0.0);
+}
+
+// exact ILU(0) in pull form: entry e = (i,j) folds its updates
+// a_ij <- fma(-l_ik, u_kj, a_ij) for k increasing (the reference KIJ order,
+// merged.cu:168-219), then l_ij = a_ij / u_jj for lower entries; entries are
+// processed by dependency level, so every operand is final when read.
+// Returns nonzero on a zero pivot.
+__device__ __forceinline__ int ilu_pull(const ushort4 *__restrict__ etab, const uint16_t *__restrict__ eptr,
+                                        int nlev, const ushort2 *__restrict__ dep, int n,
+                                        const int *__restrict__ diag, double *F, int lane)
+{
+    int bad = 0;
+    for (int l = 0; l < nlev; l++) {
+        const int b = eptr[l], e = eptr[l + 1];
+        for (int idx = b + lane; idx < e; idx += WAVE) {
+            const ushort4 r = etab[idx];
+            const int pos = r.x, db = r.y, de = r.z, pv = r.w;
+            double acc = F[pos];
+            const double dv = F[pv == 0xFFFF ? pos : pv];
+            for (int d0 = db; d0 < de; d0 += 8) {
+                ushort2 pq[8];
+#pragma unroll
+                for (int k = 0; k < 8; k++) pq[k] = dep[d0 + k < de ? d0 + k : de - 1];
+                double lv[8], uv[8];
+#pragma unroll
+                for (int k = 0; k < 8; k++) {
+                    lv[k] = F[pq[k].x];
+                    uv[k] = F[pq[k].y];
+                }
+#pragma unroll
+                for (int k = 0; k < 8; k++)
+                    if (d0 + k < de) acc = fma(-lv[k], uv[k], acc);
+            }
+            if (pv != 0xFFFF) {
+                bad |= (dv == 0.0);
+                acc = acc / dv;
+            }
+            F[pos] = acc;
+        }
+        wave_sync();
+    }
+    // every U diagonal must be nonzero for the solves
+    for (int k = lane; k < n; k += WAVE) bad |= (F[diag[k]] == 0.0);
     return __any(bad);
+}
+
+// Jacobi ParILU(0) sweep (parILU_0_gpu.cu:18-57 formula) over the entries
+// with work; entries without dependencies (upper, no updates) keep a_ij
+__device__ __forceinline__ void parilu_sweep(const ushort4 *__restrict__ etab, int ne,
+                                             const ushort2 *__restrict__ dep, const double *__restrict__ Ap,
+                                             double *F, double *W, int nnz, int lane)
+{
+    for (int e = lane; e < nnz; e += WAVE) W[e] = F[e];
+    wave_sync();
+    for (int idx = lane; idx < ne; idx += WAVE) {
+        const ushort4 r = etab[idx];
+        const int pos = r.x, db = r.y, de = r.z, pv = r.w;
+        const double a = Ap[pos];
+        const double dv = W[pv == 0xFFFF ? pos : pv];
+        double sum = 0.0;
+        for (int d0 = db; d0 < de; d0 += 8) {
+            double lv[8], uv[8];
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                const ushort2 pq = dep[d0 + k < de ? d0 + k : de - 1];
+                lv[k] = W[pq.x];
+                uv[k] = W[pq.y];
+            }
+#pragma unroll
+            for (int k = 0; k < 8; k++)
+                if (d0 + k < de) sum = fma(lv[k], uv[k], sum);
+        }
+        F[pos] = (pv != 0xFFFF) ? (a - sum) / dv : a - sum;
+    }
+    wave_sync();
 }
 
 template <int NR, int SM>
@@ -136,27 +218,21 @@ __global__ __launch_bounds__(512) void k_fast(DevPattern P, FastParams fp, int b
                                               double *__restrict__ g_tres)
 {
     extern __shared__ __align__(16) unsigned char smem[];
-    {
-        const uint4 *src = reinterpret_cast<const uint4 *>(fp.blob);
-        uint4 *dst = reinterpret_cast<uint4 *>(smem);
-        for (int i = threadIdx.x; i < fp.blob_bytes / 16; i += blockDim.x) dst[i] = src[i];
-    }
-    __syncthreads();
     const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
     const int n = P.n, nnz = P.nnz, S = fp.S;
-    double *F = reinterpret_cast<double *>(smem + fp.blob_bytes + (size_t)wave * fp.wave_bytes);
+    // shared: SpMV slot metadata [S][64]
+    uint32_t *meta = reinterpret_cast<uint32_t *>(smem);
+    for (int i = threadIdx.x; i < S * WAVE; i += blockDim.x) meta[i] = fp.slot_meta[i];
+    __syncthreads();
+    double *F = reinterpret_cast<double *>(smem + fp.meta_bytes + (size_t)wave * fp.wave_bytes);
     double *Wb = F + nnz;                       // ParILU only
-    double *e0 = Wb + (fp.par_w ? nnz : 0), *e1 = e0 + n, *e2 = e1 + n;
-    const uint16_t *ci = reinterpret_cast<const uint16_t *>(smem + fp.o_ci);
-    const uint16_t *diag = reinterpret_cast<const uint16_t *>(smem + fp.o_diag);
-    const ushort4 *ldesc = reinterpret_cast<const ushort4 *>(smem + fp.o_ldesc);
-    const ushort4 *udesc = reinterpret_cast<const ushort4 *>(smem + fp.o_udesc);
-    const uint16_t *lptr = reinterpret_cast<const uint16_t *>(smem + fp.o_lptr);
-    const uint16_t *uptr = reinterpret_cast<const uint16_t *>(smem + fp.o_uptr);
-
-    uint32_t meta[SM];
+    double *e0 = Wb + (fp.par_w ? nnz : 0), *e1 = e0 + n;
+    // level of the lane's rows in the lower (low half) and upper (high half) solve
+    int rlev[NR];
 #pragma unroll
-    for (int s = 0; s < SM; s++) meta[s] = (s < S) ? fp.slot_meta[s * WAVE + lane] : 0xFFFF0000u;
+    for (int k = 0; k < NR; k++)
+        rlev[k] = (int)fp.rlevL[k * WAVE + lane] | ((int)fp.rlevU[k * WAVE + lane] << 16);
+
     double a[SM];
 
     for (;;) {
@@ -167,11 +243,11 @@ __global__ __launch_bounds__(512) void k_fast(DevPattern P, FastParams fp, int b
 
         const double *Ap = gA + (size_t)sys * nnz;
         const double *bp = gb + (size_t)sys * n;
-        Vec<NR> b, x, r, rh, p, v, s, t, ph, sh;
+        Vec<NR> x, r, rh, p, v, s, t, ph, sh;
 #pragma unroll
         for (int k = 0; k < NR; k++) {
             const int i = lane + WAVE * k;
-            b.e[k] = (i < n) ? bp[i] : 0.0;
+            r.e[k] = (i < n) ? bp[i] : 0.0;
         }
         int bad = 0;
         if (sp.factor == FK_LOAD) {
@@ -194,35 +270,9 @@ __global__ __launch_bounds__(512) void k_fast(DevPattern P, FastParams fp, int b
                 if (e >= 0) a[q] = F[e];
             }
             if (sp.factor == FK_ILU) {
-                bad = ilu_fast(ldesc, lptr, fp.nlev_l, ci, diag,
-                               reinterpret_cast<const uint16_t *>(smem + fp.o_upd_ptr),
-                               reinterpret_cast<const ushort2 *>(smem + fp.o_upd), n, F, lane);
+                bad = ilu_pull(fp.etab, fp.eptr, fp.nlev_e, fp.dep, n, fp.diag, F, lane);
             } else if (sp.factor == FK_PARILU) {
-                // Jacobi sweeps (parILU_0_gpu.cu:18-57 formula): Wb holds the
-                // previous sweep, A comes from the slot registers
-                const uint16_t *dptr = reinterpret_cast<const uint16_t *>(smem + fp.o_dep_ptr);
-                const ushort2 *dep = reinterpret_cast<const ushort2 *>(smem + fp.o_dep);
-                const uint16_t *pdiv = reinterpret_cast<const uint16_t *>(smem + fp.o_pdiv);
-                for (int sw = 0; sw < sp.sweeps; sw++) {
-                    for (int e = lane; e < nnz; e += WAVE) Wb[e] = F[e];
-                    wave_sync();
-#pragma unroll
-                    for (int q = 0; q < SM; q++) {
-                        const int e = (q < S) ? fp.slot_pos[q * WAVE + lane] : -1;
-                        if (e >= 0) {
-                            double sum = 0.0;
-                            const int de = dptr[e + 1];
-                            for (int dd = dptr[e]; dd < de; dd++) {
-                                const ushort2 pq = dep[dd];
-                                sum = fma(Wb[pq.x], Wb[pq.y], sum);
-                            }
-                            const int pv = pdiv[e];
-                            if (pv != 0xFFFF) F[e] = (a[q] - sum) / Wb[pv];
-                            else F[e] = a[q] - sum;
-                        }
-                    }
-                    wave_sync();
-                }
+                for (int sw = 0; sw < sp.sweeps; sw++) parilu_sweep(fp.etab, fp.ne, fp.dep, Ap, F, Wb, nnz, lane);
             }
         }
 
@@ -230,23 +280,30 @@ __global__ __launch_bounds__(512) void k_fast(DevPattern P, FastParams fp, int b
         int conv = BILU_NOT_CONVERGED;
 #pragma unroll
         for (int k = 0; k < NR; k++) {
-            x.e[k] = 0.0; v.e[k] = 0.0; p.e[k] = 0.0; r.e[k] = b.e[k]; rh.e[k] = b.e[k];
+            x.e[k] = 0.0; v.e[k] = 0.0; p.e[k] = 0.0; rh.e[k] = r.e[k];
             ph.e[k] = 0.0; sh.e[k] = 0.0; s.e[k] = 0.0; t.e[k] = 0.0;
         }
-        const double bnorm = sqrt(vdot(b, b, lane, n));
+        const double bnorm = sqrt(vdot(r, r, lane, n));   // r0 = b
         const double res0 = bnorm;
         const double tol_abs = (sp.tol_mode == BILU_TOL_RELATIVE) ? sp.tol * bnorm : sp.tol;
         double iter_res = res0;
 
+        // U diagonal of the lane's rows (divisors of the upper solve)
+        Vec<NR> udg;
+#pragma unroll
+        for (int k = 0; k < NR; k++) {
+            const int i = lane + WAVE * k;
+            udg.e[k] = (i < n) ? F[fp.diag[i]] : 1.0;
+        }
+        // out = U^{-1} L^{-1} in, in registers; also left in e0 for the following SpMV
         auto apply = [&](const Vec<NR> &in, Vec<NR> &out) {
-            to_lds(in, e0, lane, n);
+            Vec<NR> y;
+            trsv_avail<NR, true>(fp.trL, fp.sptrL, fp.nlev_l, rlev, F, in, udg, e1, y, lane);
+            trsv_avail<NR, false>(fp.trU, fp.sptrU, fp.nlev_u, rlev, F, y, udg, e0, out, lane);
             wave_sync();
-            lsolve_fast(ldesc, lptr, fp.nlev_l, ci, F, e0, e1, lane);
-            usolve_fast(udesc, uptr, fp.nlev_u, ci, F, e1, e2, lane);
-            from_lds(out, e2, lane, n);
         };
-        auto matvec = [&](Vec<NR> &out) {  // out = A e2
-            spmv_slots<SM>(a, meta, S, e2, e1);
+        auto matvec = [&](Vec<NR> &out) {  // out = A e0
+            spmv_slots<SM>(a, meta, S, e0, e1, lane);
             wave_sync();
             from_lds(out, e1, lane, n);
             wave_sync();
@@ -274,23 +331,23 @@ __global__ __launch_bounds__(512) void k_fast(DevPattern P, FastParams fp, int b
                 apply(p, ph);
                 matvec(v);
                 alpha = rho_new / vdot(rh, v, lane, n);
+                // x + alpha*ph now: the same fma the combined update performs first
+#pragma unroll
+                for (int k = 0; k < NR; k++) x.e[k] = fma(alpha, ph.e[k], x.e[k]);
 #pragma unroll
                 for (int k = 0; k < NR; k++) s.e[k] = fma(-alpha, v.e[k], r.e[k]);
                 iter_res = sqrt(vdot(s, s, lane, n));
                 iter += 0.5f;
                 if (!isfinite(iter_res)) { conv = BILU_NONFINITE; break; }
                 const bool half_conv = iter_res < tol_abs;
-                if (half_conv) {
-#pragma unroll
-                    for (int k = 0; k < NR; k++) x.e[k] = fma(alpha, ph.e[k], x.e[k]);
-                } else {
+                if (!half_conv) {
                     apply(s, sh);
                     matvec(t);
                     const double ts = vdot(t, s, lane, n);
                     const double tt = vdot(t, t, lane, n);
                     const double omega_new = ts / tt;
 #pragma unroll
-                    for (int k = 0; k < NR; k++) x.e[k] = fma(omega_new, sh.e[k], fma(alpha, ph.e[k], x.e[k]));
+                    for (int k = 0; k < NR; k++) x.e[k] = fma(omega_new, sh.e[k], x.e[k]);
                     iter += 0.5f;
 #pragma unroll
                     for (int k = 0; k < NR; k++) r.e[k] = fma(-omega_new, t.e[k], s.e[k]);
@@ -302,12 +359,15 @@ __global__ __launch_bounds__(512) void k_fast(DevPattern P, FastParams fp, int b
                 if (half_conv || iter_res < tol_abs) {
                     conv = BILU_CONVERGED;
                     if (!sp.restart || restarts >= sp.max_restarts) break;
-                    to_lds(x, e2, lane, n);
+                    to_lds(x, e0, lane, n);
                     wave_sync();
                     Vec<NR> w;
                     matvec(w);
 #pragma unroll
-                    for (int k = 0; k < NR; k++) w.e[k] = b.e[k] - w.e[k];
+                    for (int k = 0; k < NR; k++) {
+                        const int i = lane + WAVE * k;
+                        w.e[k] = ((i < n) ? bp[i] : 0.0) - w.e[k];
+                    }
                     const double tr = sqrt(vdot(w, w, lane, n));
                     if (tr < tol_abs) break;
                     restarts++;
@@ -325,12 +385,15 @@ __global__ __launch_bounds__(512) void k_fast(DevPattern P, FastParams fp, int b
         }
         // fused epilogue: true residual from the register-resident A
         wave_sync();
-        to_lds(x, e2, lane, n);
+        to_lds(x, e0, lane, n);
         wave_sync();
         Vec<NR> w;
         matvec(w);
 #pragma unroll
-        for (int k = 0; k < NR; k++) w.e[k] = b.e[k] - w.e[k];
+        for (int k = 0; k < NR; k++) {
+            const int i = lane + WAVE * k;
+            w.e[k] = ((i < n) ? bp[i] : 0.0) - w.e[k];
+        }
         const double tres = sqrt(vdot(w, w, lane, n));
         double *xp = gx + (size_t)sys * n;
 #pragma unroll

@@ -9,6 +9,7 @@
 #include "pattern.hpp"
 
 #include <algorithm>
+#include <array>
 
 #include "../../include/bilu.h"
 
@@ -100,6 +101,7 @@ int analyse_pattern(int n, int nnz, const int *rp, const int *ci, PatternAnalysi
         lev[i] = l;
     }
     build_levels(n, lev, pa.lev_l_ptr, pa.lev_l_rows);
+    pa.lev_l = lev;
     std::fill(lev.begin(), lev.end(), 0);
     for (int i = n - 1; i >= 0; i--) {
         int l = 0;
@@ -107,6 +109,7 @@ int analyse_pattern(int n, int nnz, const int *rp, const int *ci, PatternAnalysi
         lev[i] = l;
     }
     build_levels(n, lev, pa.lev_u_ptr, pa.lev_u_rows);
+    pa.lev_u = lev;
     // ILU(0) update lists
     pa.upd_ptr.assign(nnz + 1, 0);
     pa.upd.clear();
@@ -152,21 +155,6 @@ int analyse_pattern(int n, int nnz, const int *rp, const int *ci, PatternAnalysi
     return BILU_OK;
 }
 
-namespace {
-
-struct BlobWriter {
-    std::vector<uint8_t> b;
-    int put16(const std::vector<int> &v)
-    {
-        while (b.size() % 16) b.push_back(0);
-        const int off = int(b.size());
-        for (int x : v) { uint16_t u = uint16_t(x); b.push_back(uint8_t(u & 0xFF)); b.push_back(uint8_t(u >> 8)); }
-        return off;
-    }
-    void finish() { while (b.size() % 16) b.push_back(0); }
-};
-
-}  // namespace
 
 void build_fast_plan(PatternAnalysis &pa)
 {
@@ -236,39 +224,78 @@ void build_fast_plan(PatternAnalysis &pa)
             }
         }
     }
-    // level descriptors: (i, rp[i], diag[i], rp[i+1]) per row in level order
-    auto descs = [&](const std::vector<int> &lrows) {
-        std::vector<int> d;
-        for (int i : lrows) { d.push_back(i); d.push_back(pa.rp[i]); d.push_back(pa.diag[i]); d.push_back(pa.rp[i + 1]); }
-        return d;
+    // pull-by-availability triangular-solve schedules
+    const int NR = (n + 63) / 64;
+    fp.nr = NR;
+    if (nnz >= (1 << 14) || n > 1023 || NR > 7) { fp.why = "pattern beyond the packed solve records"; return; }
+    auto trsv_sched = [&](bool lower, std::vector<uint32_t> &rec, std::vector<uint16_t> &sptr,
+                          std::vector<uint16_t> &rlev) {
+        const std::vector<int> &lv = lower ? pa.lev_l : pa.lev_u;
+        int nlev = 0;
+        for (int i = 0; i < n; i++) nlev = std::max(nlev, lv[i] + 1);
+        // per level, per lane: entries (col, pos, k) whose column sits in that level
+        std::vector<std::vector<std::vector<std::array<int, 3>>>> E(nlev, std::vector<std::vector<std::array<int, 3>>>(64));
+        for (int i = 0; i < n; i++) {
+            const int b = lower ? pa.rp[i] : pa.diag[i] + 1, e = lower ? pa.diag[i] : pa.rp[i + 1];
+            for (int p = b; p < e; p++) E[lv[pa.ci[p]]][i % 64].push_back({pa.ci[p], p, i / 64});
+        }
+        sptr.assign(1, 0);
+        rec.clear();
+        for (int l = 0; l < nlev; l++) {
+            size_t S = 0;
+            for (int t = 0; t < 64; t++) {
+                std::sort(E[l][t].begin(), E[l][t].end());
+                S = std::max(S, E[l][t].size());
+            }
+            for (size_t sl = 0; sl < S; sl++)
+                for (int t = 0; t < 64; t++) {
+                    if (sl < E[l][t].size()) {
+                        const auto &x = E[l][t][sl];
+                        rec.push_back(uint32_t(x[1]) | (uint32_t(x[0]) << 14) | (uint32_t(x[2]) << 24) | (1u << 27));
+                    } else {
+                        rec.push_back(0);
+                    }
+                }
+            sptr.push_back(uint16_t(rec.size() / 64));
+        }
+        rlev.assign(size_t(NR) * 64, 0xFFFF);
+        for (int i = 0; i < n; i++) rlev[size_t(i / 64) * 64 + i % 64] = uint16_t(lv[i]);
+        if (rec.empty()) rec.assign(64, 0);
     };
-    const std::vector<int> ld = descs(pa.lev_l_rows), ud = descs(pa.lev_u_rows);
-    for (int v : pa.upd) if (v >= 0xFFFF) { fp.why = "ILU update list beyond u16"; return; }
-    if (pa.upd.size() / 2 >= 0xFFFF || pa.dep.size() / 2 >= 0xFFFF) { fp.why = "update lists beyond u16"; return; }
-    std::vector<int> pdiv(nnz, 0xFFFF);
-    for (int i = 0; i < n; i++)
-        for (int p = pa.rp[i]; p < pa.diag[i]; p++) pdiv[p] = pa.diag[pa.ci[p]];
-    for (int variant = 0; variant < 2; variant++) {
-        BlobWriter w;
-        fp.o_ci = w.put16(pa.ci);
-        fp.o_diag = w.put16(pa.diag);
-        fp.o_ldesc = w.put16(ld);
-        fp.o_udesc = w.put16(ud);
-        fp.o_lptr = w.put16(pa.lev_l_ptr);
-        fp.o_uptr = w.put16(pa.lev_u_ptr);
-        if (variant == 0) {
-            fp.o_upd_ptr = w.put16(pa.upd_ptr);
-            fp.o_upd = w.put16(pa.upd.empty() ? std::vector<int>{0, 0} : pa.upd);
-            w.finish();
-            fp.blob_ilu = w.b;
-        } else {
-            fp.o_dep_ptr = w.put16(pa.dep_ptr);
-            fp.o_dep = w.put16(pa.dep.empty() ? std::vector<int>{0, 0} : pa.dep);
-            fp.o_pdiv = w.put16(pdiv);
-            w.finish();
-            fp.blob_par = w.b;
+    trsv_sched(true, fp.trL, fp.sptrL, fp.rlevL);
+    trsv_sched(false, fp.trU, fp.sptrU, fp.rlevU);
+    // entry-DAG levels of the factorization (pull form of the KIJ updates)
+    std::vector<int> elev(nnz, 0), pdiv(nnz, 0xFFFF);
+    int maxlev = 0;
+    for (int i = 0; i < n; i++) {
+        for (int e = pa.rp[i]; e < pa.rp[i + 1]; e++) {
+            int l = 0;
+            for (int d = pa.dep_ptr[e]; d < pa.dep_ptr[e + 1]; d++)
+                l = std::max(l, std::max(elev[pa.dep[2 * d]], elev[pa.dep[2 * d + 1]]) + 1);
+            if (pa.ci[e] < i) { pdiv[e] = pa.diag[pa.ci[e]]; l = std::max(l, elev[pdiv[e]] + 1); }
+            elev[e] = l;
+            maxlev = std::max(maxlev, l);
         }
     }
+    fp.etab.clear(); fp.dep.clear(); fp.eptr.assign(1, 0);
+    for (int l = 1; l <= maxlev; l++) {
+        for (int e = 0; e < nnz; e++) {
+            if (elev[e] != l) continue;
+            const int db = int(fp.dep.size() / 2);
+            for (int d = pa.dep_ptr[e]; d < pa.dep_ptr[e + 1]; d++) {
+                fp.dep.push_back(uint16_t(pa.dep[2 * d])); fp.dep.push_back(uint16_t(pa.dep[2 * d + 1]));
+            }
+            const int de = int(fp.dep.size() / 2);
+            if (de >= 0xFFFF) { fp.why = "factorization dependency list beyond u16"; return; }
+            fp.etab.push_back(uint16_t(e)); fp.etab.push_back(uint16_t(db));
+            fp.etab.push_back(uint16_t(de)); fp.etab.push_back(uint16_t(pdiv[e]));
+        }
+        fp.eptr.push_back(uint16_t(fp.etab.size() / 4));
+    }
+    fp.nlev_e = maxlev;
+    fp.ne = int(fp.etab.size() / 4);
+    if (fp.dep.empty()) { fp.dep.assign(2, 0); }
+    if (fp.etab.empty()) { fp.etab.assign(4, 0); }
     fp.ok = true;
 }
 

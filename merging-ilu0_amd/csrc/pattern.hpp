@@ -18,23 +18,34 @@ struct IsaiMap {
     int max_m = 0;
 };
 
-// Schedule tables of the persistent multi-wave kernels: one byte blob
-// copied once into LDS per workgroup (u16 indices), plus the per-lane SpMV
-// slot layout that is held in VGPRs.
+// Schedule tables of the persistent multi-wave kernels (global memory; the
+// waves read them through the caches), plus the per-lane SpMV slot layout
+// that is held in VGPRs.
+//   Triangular solves, "pull by availability": lane l owns rows l + 64 k
+//   (k < NR) and keeps their partial sums in registers.  After level ℓ
+//   finalises its rows, every lane folds in the entries of its rows whose
+//   column lies in level ℓ.  tr{L,U}[s][64]: one u32 record per slot and
+//   lane = fpos | col << 14 | k << 24 | valid << 27; slots of level ℓ are
+//   s in [sptr[ℓ], sptr[ℓ+1]); a lane's entries of one row run in increasing
+//   column order within a level, so each row sum is an fma chain in
+//   (level of column, column) order.
+//   rlev{L,U}[k][64]: level of row l + 64 k (0xFFFF: no such row).
+//   Factorization: etab records (pos, dep_begin, dep_end, diag position of
+//   the column or 0xFFFF for upper entries) grouped by dependency level of
+//   the entry DAG (eptr); dep holds (pos(i,k), pos(k,j)) pairs, k increasing.
 struct FastPlan {
-    bool ok = false;          // pattern fits the u16 tables and the slot budget
+    bool ok = false;          // pattern fits the packed tables
     std::string why;          // reason when !ok
     int S = 0;                // SpMV slots per lane (balanced row -> lane assignment)
     // slot tables [S][64]: meta = col | (row << 16), row = 0xFFFF if the slot
     // does not end a row; pos = pattern position (-1 = padding)
     std::vector<uint32_t> slot_meta;
     std::vector<int32_t> slot_pos;
-    // LDS blobs (bytes) for the exact-ILU(0) and the ParILU(0) variants;
-    // offsets in bytes into the blob
-    std::vector<uint8_t> blob_ilu, blob_par;
-    int o_ci = 0, o_diag = 0, o_ldesc = 0, o_udesc = 0, o_lptr = 0, o_uptr = 0;
-    int o_upd_ptr = 0, o_upd = 0;              // ILU blob only
-    int o_dep_ptr = 0, o_dep = 0, o_pdiv = 0;  // ParILU blob only
+    int nr = 0;
+    std::vector<uint32_t> trL, trU;
+    std::vector<uint16_t> sptrL, sptrU, rlevL, rlevU;
+    std::vector<uint16_t> etab, eptr, dep;   // etab: 4 u16 per record, dep: 2 u16 per pair
+    int nlev_e = 0, ne = 0;
 };
 
 struct PatternAnalysis {
@@ -43,6 +54,7 @@ struct PatternAnalysis {
     // level schedules: rows grouped by dependency level
     std::vector<int> lev_l_ptr, lev_l_rows;  // lower solve and ILU(0) rows
     std::vector<int> lev_u_ptr, lev_u_rows;  // upper solve rows (bottom-up)
+    std::vector<int> lev_l, lev_u;           // level of each row
     // ILU(0) update lists, per strictly-lower position p=(i,k):
     // pairs (dst=(i,j), src=(k,j)) for j>k, (i,j) and (k,j) in the pattern
     std::vector<int> upd_ptr;   // nnz+1

@@ -710,9 +710,11 @@ struct bilu_pattern_s {
     size_t lds_per_cu = 65536;
     int num_cus = 1;
     // persistent-kernel tables
-    uint8_t *d_blob_ilu = nullptr, *d_blob_par = nullptr;
-    uint32_t *d_slot_meta = nullptr;
+    uint8_t *d_fast = nullptr;     // all persistent-kernel tables, one allocation
+    uint32_t *d_slot_meta = nullptr, *d_trL = nullptr, *d_trU = nullptr;
     int32_t *d_slot_pos = nullptr;
+    uint16_t *d_sptrL = nullptr, *d_sptrU = nullptr, *d_rlevL = nullptr, *d_rlevU = nullptr;
+    uint16_t *d_etab = nullptr, *d_eptr = nullptr, *d_dep = nullptr;
     int *d_counter = nullptr;
 };
 
@@ -774,14 +776,32 @@ int upload(bilu_pattern_s &h)
     HIP_TRY(hipMemset(h.d_counter, 0, 64));
     const auto &fp = pa.fast;
     if (fp.ok) {
-        HIP_TRY(hipMalloc((void **)&h.d_blob_ilu, fp.blob_ilu.size()));
-        HIP_TRY(hipMemcpy(h.d_blob_ilu, fp.blob_ilu.data(), fp.blob_ilu.size(), hipMemcpyHostToDevice));
-        HIP_TRY(hipMalloc((void **)&h.d_blob_par, fp.blob_par.size()));
-        HIP_TRY(hipMemcpy(h.d_blob_par, fp.blob_par.data(), fp.blob_par.size(), hipMemcpyHostToDevice));
-        HIP_TRY(hipMalloc((void **)&h.d_slot_meta, fp.slot_meta.size() * 4));
-        HIP_TRY(hipMemcpy(h.d_slot_meta, fp.slot_meta.data(), fp.slot_meta.size() * 4, hipMemcpyHostToDevice));
-        HIP_TRY(hipMalloc((void **)&h.d_slot_pos, fp.slot_pos.size() * 4));
-        HIP_TRY(hipMemcpy(h.d_slot_pos, fp.slot_pos.data(), fp.slot_pos.size() * 4, hipMemcpyHostToDevice));
+        std::vector<uint8_t> blob;
+        auto put = [&](const void *src, size_t bytes) {
+            while (blob.size() % 256) blob.push_back(0);
+            const size_t off = blob.size();
+            blob.insert(blob.end(), (const uint8_t *)src, (const uint8_t *)src + bytes);
+            return off;
+        };
+        const size_t o_meta = put(fp.slot_meta.data(), fp.slot_meta.size() * 4);
+        const size_t o_pos = put(fp.slot_pos.data(), fp.slot_pos.size() * 4);
+        const size_t o_trL = put(fp.trL.data(), fp.trL.size() * 4);
+        const size_t o_trU = put(fp.trU.data(), fp.trU.size() * 4);
+        const size_t o_sL = put(fp.sptrL.data(), fp.sptrL.size() * 2);
+        const size_t o_sU = put(fp.sptrU.data(), fp.sptrU.size() * 2);
+        const size_t o_rL = put(fp.rlevL.data(), fp.rlevL.size() * 2);
+        const size_t o_rU = put(fp.rlevU.data(), fp.rlevU.size() * 2);
+        const size_t o_et = put(fp.etab.data(), fp.etab.size() * 2);
+        const size_t o_ep = put(fp.eptr.data(), fp.eptr.size() * 2);
+        const size_t o_dp = put(fp.dep.data(), fp.dep.size() * 2);
+        HIP_TRY(hipMalloc((void **)&h.d_fast, blob.size()));
+        HIP_TRY(hipMemcpy(h.d_fast, blob.data(), blob.size(), hipMemcpyHostToDevice));
+        uint8_t *b = h.d_fast;
+        h.d_slot_meta = (uint32_t *)(b + o_meta); h.d_slot_pos = (int32_t *)(b + o_pos);
+        h.d_trL = (uint32_t *)(b + o_trL); h.d_trU = (uint32_t *)(b + o_trU);
+        h.d_sptrL = (uint16_t *)(b + o_sL); h.d_sptrU = (uint16_t *)(b + o_sU);
+        h.d_rlevL = (uint16_t *)(b + o_rL); h.d_rlevU = (uint16_t *)(b + o_rU);
+        h.d_etab = (uint16_t *)(b + o_et); h.d_eptr = (uint16_t *)(b + o_ep); h.d_dep = (uint16_t *)(b + o_dp);
     }
     return BILU_OK;
 }
@@ -791,10 +811,7 @@ void free_handle(bilu_pattern_s *h)
     if (!h) return;
     if (h->dbuf) (void)hipFree(h->dbuf);
     if (h->d_counter) (void)hipFree(h->d_counter);
-    if (h->d_blob_ilu) (void)hipFree(h->d_blob_ilu);
-    if (h->d_blob_par) (void)hipFree(h->d_blob_par);
-    if (h->d_slot_meta) (void)hipFree(h->d_slot_meta);
-    if (h->d_slot_pos) (void)hipFree(h->d_slot_pos);
+    if (h->d_fast) (void)hipFree(h->d_fast);
     delete h;
 }
 
@@ -825,23 +842,25 @@ int launch_fast(bilu_pattern_s &h, int batch, const double *A, const double *b, 
     else if (pl.S <= 48) sm = 48;
     else return -1;
     const bool par = sp.factor == FK_PARILU;
+    if (pl.nr != nr) return -1;
     FastParams fp{};
-    fp.blob = par ? h.d_blob_par : h.d_blob_ilu;
-    fp.blob_bytes = int(par ? pl.blob_par.size() : pl.blob_ilu.size());
-    fp.o_ci = pl.o_ci; fp.o_diag = pl.o_diag; fp.o_ldesc = pl.o_ldesc; fp.o_udesc = pl.o_udesc;
-    fp.o_lptr = pl.o_lptr; fp.o_uptr = pl.o_uptr; fp.o_upd_ptr = pl.o_upd_ptr; fp.o_upd = pl.o_upd;
-    fp.o_dep_ptr = pl.o_dep_ptr; fp.o_dep = pl.o_dep; fp.o_pdiv = pl.o_pdiv;
-    fp.nlev_l = h.dev.nlev_l; fp.nlev_u = h.dev.nlev_u;
+    fp.trL = h.d_trL; fp.trU = h.d_trU; fp.sptrL = h.d_sptrL; fp.sptrU = h.d_sptrU;
+    fp.rlevL = h.d_rlevL; fp.rlevU = h.d_rlevU;
+    fp.etab = reinterpret_cast<const ushort4 *>(h.d_etab); fp.eptr = h.d_eptr;
+    fp.dep = reinterpret_cast<const ushort2 *>(h.d_dep);
+    fp.diag = h.dev.diag;
+    fp.nlev_l = h.dev.nlev_l; fp.nlev_u = h.dev.nlev_u; fp.nlev_e = pl.nlev_e; fp.ne = pl.ne;
     fp.S = pl.S; fp.slot_meta = h.d_slot_meta; fp.slot_pos = h.d_slot_pos;
-    // per wave: F, (ParILU: the previous sweep's factors), e0, e1, e2
-    fp.wave_bytes = int(((size_t)((par ? 2 : 1) * nnz + 3 * n) * sizeof(double) + 15) / 16 * 16);
+    // per wave: F, (ParILU: the previous sweep's factors), e0, e1
+    fp.wave_bytes = int(((size_t)((par ? 2 : 1) * nnz + 2 * n) * sizeof(double) + 15) / 16 * 16);
     fp.par_w = par ? 1 : 0;
     fp.counter = h.d_counter;
-    const size_t avail = h.max_lds > (size_t)fp.blob_bytes ? h.max_lds - fp.blob_bytes : 0;
-    int waves = (int)std::min<size_t>(8, avail / fp.wave_bytes);
+    fp.meta_bytes = int(((size_t)pl.S * WAVE * 4 + 15) / 16 * 16);
+    if ((size_t)fp.meta_bytes >= h.max_lds) return -1;
+    int waves = (int)std::min<size_t>(8, (h.max_lds - fp.meta_bytes) / fp.wave_bytes);
     if (waves < 1) return -1;
-    waves = std::min(waves, std::max(1, (batch + 0) / 1));
-    const size_t lds = fp.blob_bytes + (size_t)waves * fp.wave_bytes;
+    waves = std::min(waves, std::max(1, batch));
+    const size_t lds = fp.meta_bytes + (size_t)waves * fp.wave_bytes;
     const int per_cu = std::max<int>(1, (int)(h.lds_per_cu / lds));
     int grid = h.num_cus * per_cu;
     grid = std::max(1, std::min(grid, (batch + waves - 1) / waves));

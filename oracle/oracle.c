@@ -7,7 +7,7 @@
  * Short names: merged.cu = PrecondBiCGSTAB_exact_ilu0_app1_merged_gpu.cu,
  * common.hpp = Bicgstab_common.hpp.
  *
- * Build: see oracle/build_oracle.sh (gcc -O3 -mfma -ffp-contract=off).
+ * Build: oracle/build.py (gcc -O3 -march=x86-64-v3 -ffp-contract=off).
  */
 #include "oracle.h"
 
@@ -199,6 +199,66 @@ void orc_lu_solve(int n, const int *rp, const int *ci, const int *diag,
     }
 }
 
+/* Dependency levels of the triangular solves: lower level(i) = 1 + max
+ * level(j) over strictly-lower columns j (0 if none); upper from the bottom
+ * over strictly-upper columns. */
+void orc_trsv_levels(int n, const int *rp, const int *ci, const int *diag, int *lev_l, int *lev_u)
+{
+    for (int i = 0; i < n; i++) {
+        int l = 0;
+        for (int p = rp[i]; p < diag[i]; p++) if (lev_l[ci[p]] + 1 > l) l = lev_l[ci[p]] + 1;
+        lev_l[i] = l;
+    }
+    for (int i = n - 1; i >= 0; i--) {
+        int l = 0;
+        for (int p = diag[i] + 1; p < rp[i + 1]; p++) if (lev_u[ci[p]] + 1 > l) l = lev_u[ci[p]] + 1;
+        lev_u[i] = l;
+    }
+}
+
+/* Same solves as orc_lu_solve, each row sum an fma chain in "availability"
+ * order: the row's entries sorted by (level of the column, column).  This is
+ * the order in which the persistent wave64 kernel folds entries in, as
+ * each level of rows becomes final. */
+static void sort_avail(int *pairs, int m, const int *lev)
+{
+    /* insertion sort of (col, pos) pairs keyed on (lev[col], col) */
+    for (int a = 1; a < m; a++) {
+        const int c = pairs[2 * a], p = pairs[2 * a + 1];
+        int b = a - 1;
+        while (b >= 0 && (lev[pairs[2 * b]] > lev[c] || (lev[pairs[2 * b]] == lev[c] && pairs[2 * b] > c))) {
+            pairs[2 * b + 2] = pairs[2 * b];
+            pairs[2 * b + 3] = pairs[2 * b + 1];
+            b--;
+        }
+        pairs[2 * b + 2] = c;
+        pairs[2 * b + 3] = p;
+    }
+}
+
+void orc_lu_solve_avail(int n, const int *rp, const int *ci, const int *diag,
+                        const int *lev_l, const int *lev_u, const double *F,
+                        const double *v, double *tmp, double *out)
+{
+    int pairs[2 * 1024];
+    for (int i = 0; i < n; i++) {
+        int m = 0;
+        for (int p = rp[i]; p < diag[i] && m < 1024; p++, m++) { pairs[2 * m] = ci[p]; pairs[2 * m + 1] = p; }
+        sort_avail(pairs, m, lev_l);
+        double sum = 0.0;
+        for (int q = 0; q < m; q++) sum = fma(F[pairs[2 * q + 1]], tmp[pairs[2 * q]], sum);
+        tmp[i] = v[i] - sum;
+    }
+    for (int i = n - 1; i >= 0; i--) {
+        int m = 0;
+        for (int p = diag[i] + 1; p < rp[i + 1] && m < 1024; p++, m++) { pairs[2 * m] = ci[p]; pairs[2 * m + 1] = p; }
+        sort_avail(pairs, m, lev_u);
+        double sum = 0.0;
+        for (int q = 0; q < m; q++) sum = fma(F[pairs[2 * q + 1]], out[pairs[2 * q]], sum);
+        out[i] = (tmp[i] - sum) / F[diag[i]];
+    }
+}
+
 /* value of the typed view of `vals` at (r,c): lower = strict-lower part with
  * an explicit unit diagonal (the L of factorization.cu:313-314), upper =
  * diagonal + strict upper, general = everything. */
@@ -300,9 +360,11 @@ int orc_isai(int n, const int *rp, const int *ci, const int *diag,
 /* y = M^{-1} v for the chosen preconditioner */
 static void apply_pc(int n, const int *rp, const int *ci, const int *diag,
                      int kind, const double *P1, const double *P2, int spmv_w,
-                     const double *v, double *tmp, double *out)
+                     const int *lev, const double *v, double *tmp, double *out)
 {
-    if (kind == ORC_PC_LU) {
+    if (kind == ORC_PC_LU && lev) {
+        orc_lu_solve_avail(n, rp, ci, diag, lev, lev + n, P1, v, tmp, out);
+    } else if (kind == ORC_PC_LU) {
         orc_lu_solve(n, rp, ci, diag, P1, v, tmp, out);
     } else if (kind == ORC_PC_ISAI_LU) {
         /* ilu_isai_solve.cu:25-45: temp = aiL * v; out = aiU * temp */
@@ -333,6 +395,11 @@ int orc_bicgstab(int n, const int *rp, const int *ci, const int *diag,
 {
     const int dw = o->dot_width, sw = o->spmv_width;
     double *buf = (double *)malloc(sizeof(double) * (size_t)n * 10);
+    int *lev = NULL;
+    if (o->trsv_order == 1 && pc_kind == ORC_PC_LU) {
+        lev = (int *)malloc(sizeof(int) * 2 * (size_t)n);
+        orc_trsv_levels(n, rp, ci, diag, lev, lev + n);
+    }
     double *r = buf, *rh = buf + n, *p = buf + 2 * n, *v = buf + 3 * n, *s = buf + 4 * n;
     double *t = buf + 5 * n, *ph = buf + 6 * n, *sh = buf + 7 * n, *tmp = buf + 8 * n, *w = buf + 9 * n;
     float iter = 0.0f;
@@ -359,7 +426,7 @@ int orc_bicgstab(int n, const int *rp, const int *ci, const int *diag,
             if (rho_new == 0.0) break;
             const double beta = (rho_new / rho_old) * (alpha / omega_old);
             for (int i = 0; i < n; i++) p[i] = fma(beta, fma(-omega_old, v[i], p[i]), r[i]);
-            apply_pc(n, rp, ci, diag, pc_kind, P1, P2, sw, p, tmp, ph);
+            apply_pc(n, rp, ci, diag, pc_kind, P1, P2, sw, lev, p, tmp, ph);
             orc_spmv(n, rp, ci, A, ph, v, sw);
             alpha = rho_new / orc_dot(n, rh, v, dw);
             for (int i = 0; i < n; i++) s[i] = fma(-alpha, v[i], r[i]);
@@ -371,7 +438,7 @@ int orc_bicgstab(int n, const int *rp, const int *ci, const int *diag,
             if (half_conv) {
                 for (int i = 0; i < n; i++) x[i] = fma(alpha, ph[i], x[i]);
             } else {
-                apply_pc(n, rp, ci, diag, pc_kind, P1, P2, sw, s, tmp, sh);
+                apply_pc(n, rp, ci, diag, pc_kind, P1, P2, sw, lev, s, tmp, sh);
                 orc_spmv(n, rp, ci, A, sh, t, sw);
                 const double ts = orc_dot(n, t, s, dw);
                 const double tt = orc_dot(n, t, t, dw);
@@ -408,6 +475,7 @@ int orc_bicgstab(int n, const int *rp, const int *ci, const int *diag,
     *conv_out = conv;
     *iter_res_out = iter_res;
     free(buf);
+    free(lev);
     return 0;
 }
 

@@ -60,6 +60,7 @@ typedef struct {
     int spmv_width;          /* 1 = sequential row sum, 32 = reference warp tree */
     int true_res_restart;    /* restart from b - A x when the true residual misses tol */
     int max_restarts;
+    int trsv_order;          /* 0: CSR order (reference), 1: availability order (see orc_lu_solve_avail) */
 } orc_opts;
 
 int orc_find_diag(int n, const int *rp, const int *ci, int *diag);
@@ -75,6 +76,10 @@ double orc_factor_residual(int n, const int *rp, const int *ci, const int *diag,
                            const double *A, const double *F);
 void orc_lu_solve(int n, const int *rp, const int *ci, const int *diag,
                   const double *F, const double *v, double *tmp, double *out);
+void orc_trsv_levels(int n, const int *rp, const int *ci, const int *diag, int *lev_l, int *lev_u);
+void orc_lu_solve_avail(int n, const int *rp, const int *ci, const int *diag,
+                        const int *lev_l, const int *lev_u, const double *F,
+                        const double *v, double *tmp, double *out);
 int orc_isai(int n, const int *rp, const int *ci, const int *diag,
              const double *vals, int type, double *out);
 

@@ -31,14 +31,20 @@ class Opts(C.Structure):
         ("spmv_width", C.c_int),
         ("true_res_restart", C.c_int),
         ("max_restarts", C.c_int),
+        ("trsv_order", C.c_int),
     ]
 
 
+TRSV_CSR, TRSV_AVAIL = 0, 1
+
+
 def opts(tol=1e-11, tol_mode=TOL_ABSOLUTE, max_iter=500, dot_width=64, spmv_width=1,
-         true_res_restart=0, max_restarts=3) -> Opts:
+         true_res_restart=0, max_restarts=3, trsv_order=TRSV_CSR) -> Opts:
     """Defaults: reference tolerance semantics (ATOL=1e-11, MAX_ITER=500,
-    PrecondBiCGSTAB.h:10-12) with the wave64 reduction order."""
-    return Opts(tol, tol_mode, max_iter, dot_width, spmv_width, true_res_restart, max_restarts)
+    PrecondBiCGSTAB.h:10-12) with the wave64 reduction order.  trsv_order 1
+    is the row-sum order of the persistent kernel."""
+    return Opts(tol, tol_mode, max_iter, dot_width, spmv_width, true_res_restart, max_restarts,
+                trsv_order)
 
 
 def reference_opts(**kw) -> Opts:
@@ -71,6 +77,8 @@ def lib():
         L.orc_factor_residual.restype = C.c_double
         L.orc_lu_solve.argtypes = [C.c_int, i32, i32, i32, f64, f64, f64, f64]
         L.orc_isai.argtypes = [C.c_int, i32, i32, i32, f64, C.c_int, f64]
+        L.orc_trsv_levels.argtypes = [C.c_int, i32, i32, i32, i32, i32]
+        L.orc_lu_solve_avail.argtypes = [C.c_int, i32, i32, i32, i32, i32, f64, f64, f64, f64]
         L.orc_bicgstab.argtypes = [C.c_int, i32, i32, i32, f64, f64, f64, C.c_int,
                                    C.c_void_p, C.c_void_p, C.POINTER(Opts),
                                    C.POINTER(C.c_float), C.POINTER(C.c_int),
@@ -128,6 +136,22 @@ def lu_solve(pat, F, v):
     tmp = np.empty(pat.n)
     out = np.empty(pat.n)
     lib().orc_lu_solve(pat.n, pat.row_ptrs, pat.col_idx, pat.diag, _f64(F), _f64(v), tmp, out)
+    return out
+
+
+def trsv_levels(pat):
+    ll = np.empty(pat.n, np.int32)
+    lu = np.empty(pat.n, np.int32)
+    lib().orc_trsv_levels(pat.n, pat.row_ptrs, pat.col_idx, pat.diag, ll, lu)
+    return ll, lu
+
+
+def lu_solve_avail(pat, F, v):
+    ll, lu = trsv_levels(pat)
+    tmp = np.empty(pat.n)
+    out = np.empty(pat.n)
+    lib().orc_lu_solve_avail(pat.n, pat.row_ptrs, pat.col_idx, pat.diag, ll, lu, _f64(F), _f64(v),
+                             tmp, out)
     return out
 
 

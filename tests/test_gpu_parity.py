@@ -41,10 +41,20 @@ ORACLE_PC = {bilu.ILU0: O.ILU0, bilu.PARILU0: O.PARILU0, bilu.ILU0_ISAI: O.ILU0_
              bilu.PARILU0_ISAI: O.PARILU0_ISAI, bilu.GENERAL_ISAI: O.GENERAL_ISAI, bilu.NONE: O.NONE}
 
 
+def uses_persistent(pat, precond=bilu.ILU0, kernel=bilu.KERNEL_AUTO, **_):
+    """Whether bilu_solve runs the persistent kernel (LU preconditioners, pattern
+    within its register budget), whose triangular-solve row sums run in
+    availability order."""
+    if precond not in (bilu.ILU0, bilu.PARILU0) or kernel == bilu.KERNEL_WAVE:
+        return False
+    return pat.n <= 192 and pat.nnz <= 48 * 64
+
+
 def run_oracle(pat, A, b, precond=bilu.ILU0, parilu_sweeps=5, tol=1e-11, tol_mode=0, max_iter=500,
-               true_res_restart=0, max_restarts=3, **_):
+               true_res_restart=0, max_restarts=3, kernel=bilu.KERNEL_AUTO, **_):
+    order = O.TRSV_AVAIL if uses_persistent(pat, precond, kernel) else O.TRSV_CSR
     o = O.opts(tol=tol, tol_mode=tol_mode, max_iter=max_iter, true_res_restart=true_res_restart,
-               max_restarts=max_restarts)
+               max_restarts=max_restarts, trsv_order=order)
     return O.batch_solve(pat, A, b, ORACLE_PC[precond], sweeps=parilu_sweeps, o=o, threads=8)
 
 
@@ -68,7 +78,7 @@ def test_merged_ilu0_bitwise(kind, B, n, kernel):
             run_gpu(pat, A, b, kernel=kernel)
         return
     g = run_gpu(pat, A, b, kernel=kernel)
-    o = run_oracle(pat, A, b)
+    o = run_oracle(pat, A, b, kernel=kernel)
     assert (o[2] == 1).all()
     assert_same(g, o, f"{kind} n={n}")
 
@@ -82,7 +92,7 @@ def test_persistent_parilu_and_segregated_bitwise(kernel):
                dict(precond=bilu.ILU0, tol=1e-10, tol_mode=bilu.TOL_RELATIVE, true_res_restart=1)):
         print(f"case {kw} kernel={kernel}", flush=True)
         g = run_gpu(pat, A, b, kernel=kernel, **kw)
-        o = run_oracle(pat, A, b, **kw)
+        o = run_oracle(pat, A, b, kernel=kernel, **kw)
         assert_same(g, o, f"{kw} kernel={kernel}")
 
 

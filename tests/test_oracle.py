@@ -344,3 +344,30 @@ def test_threads_do_not_change_bits():
     r4 = O.batch_solve(pat, A, b, O.ILU0, threads=4)
     for u, v in zip(r1, r4):
         assert np.array_equal(u, v)
+
+
+def test_availability_order_solve_is_the_same_solve():
+    """The persistent kernel's row-sum order (entries by level of their
+    column) is another evaluation order of the same exact triangular solves."""
+    pat, A, b = synth.batch("random", 4, n=144)
+    for s in range(4):
+        F, _ = O.ilu0(pat, A[s])
+        x1 = O.lu_solve(pat, F, b[s])
+        x2 = O.lu_solve_avail(pat, F, b[s])
+        L, U = lu_dense(pat, F)
+        xd = np.linalg.solve(U, np.linalg.solve(L, b[s]))
+        assert np.allclose(x1, xd, rtol=1e-9, atol=0) and np.allclose(x2, xd, rtol=1e-9, atol=0)
+        assert np.abs(x1 - x2).max() <= 1e-12 * np.abs(x1).max()
+    ll, lu = O.trsv_levels(pat)
+    for i in range(pat.n):
+        cols = pat.col_idx[pat.row_ptrs[i]:pat.diag[i]]
+        assert ll[i] == (1 + ll[cols].max() if cols.size else 0)
+
+
+def test_availability_order_iterations_within_one():
+    for kind in ("stencil5", "random"):
+        pat, A, b = synth.batch(kind, 64, n=144)
+        _, it0, cv0, _, _ = O.batch_solve(pat, A, b, O.ILU0, o=O.opts())
+        _, it1, cv1, _, _ = O.batch_solve(pat, A, b, O.ILU0, o=O.opts(trsv_order=O.TRSV_AVAIL))
+        assert (cv0 == 1).all() and (cv1 == 1).all()
+        assert np.abs(it0 - it1).max() <= 1.0
