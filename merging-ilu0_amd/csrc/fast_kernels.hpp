@@ -25,8 +25,9 @@
 // gather source of an SpMV or solve.
 
 struct FastParams {
-    const uint32_t *trL, *trU;            // solve slot records [slots][64]
-    const uint16_t *sptrL, *sptrU;        // slot range per level (nlev + 1)
+    const uint32_t *trL, *trU;            // solve chunk records [chunk][CH][64], tail-padded
+    int trL_words, trU_words;
+    const uint8_t *cntL, *cntU;           // chunks per level, [64]
     const uint16_t *rlevL, *rlevU;        // [NR][64] level of the lane's rows
     const ushort4 *etab;                  // factorization entry records
     const uint16_t *eptr;                 // per entry-DAG level
@@ -36,7 +37,8 @@ struct FastParams {
     int S;
     const uint32_t *slot_meta;  // [S][64]
     const int32_t *slot_pos;    // [S][64]
-    int meta_bytes;  // shared LDS copy of slot_meta (16-byte aligned size)
+    int shared_bytes;           // shared LDS: SpMV meta [SM][64], trL, trU (16-byte aligned)
+    int o_trL, o_trU;
     int wave_bytes;
     int par_w;      // 1: the wave region holds a second factor buffer (ParILU Jacobi)
     int *counter;
@@ -51,86 +53,92 @@ __device__ __forceinline__ void wave_sync()
 
 __global__ void k_zero_counter(int *c) { *c = 0; }
 
-// y (LDS, by row) = A x over the slot layout; x in LDS; slot metadata
-// (col | row << 16, row = 0xFFFF when the slot does not end a row) is read
-// from the workgroup's shared LDS copy
+// y (LDS, by row) = A x over the slot layout; x in LDS.  Slot metadata
+// (8*col | 8*row << 16, high half 0xFFFF when the slot does not end a row)
+// is read from the workgroup's shared LDS copy, padded to SM slots.  The
+// body is straight-line: every slot stores its running sum, to the row's
+// cell when the slot ends a row and to a scratch cell otherwise, so the
+// loads of all slots can be in flight together.
 template <int SM>
-__device__ __forceinline__ void spmv_slots(const double (&a)[SM], const uint32_t *__restrict__ meta, int S,
-                                           const double *x, double *y, int lane)
+__device__ __forceinline__ void spmv_slots(const double (&a)[SM], const uint32_t *__restrict__ meta,
+                                           const double *__restrict__ x, double *__restrict__ y,
+                                           double *__restrict__ scratch, int lane)
 {
+    const char *xb = reinterpret_cast<const char *>(x);
+    char *yb = reinterpret_cast<char *>(y);
     double acc = 0.0;
 #pragma unroll
     for (int s = 0; s < SM; s++) {
-        if (s < S) {
-            const uint32_t m = meta[s * WAVE + lane];
-            acc = fma(a[s], x[m & 0xFFFFu], acc);
-            if ((m >> 16) != 0xFFFFu) {
-                y[m >> 16] = acc;
-                acc = 0.0;
-            }
-        }
+        const uint32_t m = meta[s * WAVE + lane];
+        acc = fma(a[s], *reinterpret_cast<const double *>(xb + (m & 0xFFFFu)), acc);
+        const bool end = (m >> 16) != 0xFFFFu;
+        double *dst = end ? reinterpret_cast<double *>(yb + (m >> 16)) : scratch;
+        *dst = acc;
+        acc = end ? 0.0 : acc;
     }
 }
 
 // Triangular solve by availability.  Lane l owns rows l + 64 k; their
-// partial sums live in acc[k].  Level ℓ: (1) every row of level ℓ is final:
-// L: y_r = vin_r - acc, U: x_r = (vin_r - acc) / u_rr, kept in `res` and
-// stored to `out` (LDS) for the gathers of later levels; (2) each lane folds
-// in its entries whose column lies in level ℓ, acc_k = fma(F_p, out_c, acc_k).
-// The next level's first records are fetched while this level computes.
+// partial sums live in LDS cells acc[row] (per wave).  Level ℓ: (1) every
+// row of level ℓ is final: L: y_r = vin_r - acc_r, U: x_r = (vin_r - acc_r)
+// * (1 / u_rr), kept in `res` and stored to `out` (LDS) for the gathers of
+// later levels; (2) each lane adds the products of its entries whose column
+// lies in level ℓ into its rows' cells, acc_r += F_p * out_c, with the
+// LDS floating-point atomic (fire and forget; a lane only touches its own
+// rows, so the additions of one row land in program order).  Records (LDS
+// byte offsets + row slot k) come in chunks of CH per level; padding records
+// point at zero cells, so every chunk body is straight-line code.  The chunk
+// count of level ℓ is held by lane ℓ of `vcnt` and read with readlane (no
+// memory access on the loop bound); the next chunk's records are loaded one
+// chunk ahead.
 template <int NR, bool LOWER>
-__device__ __forceinline__ void trsv_avail(const uint32_t *__restrict__ rec, const uint16_t *__restrict__ sptr,
-                                           int nlev, const int (&rlev)[NR], const double *F,
-                                           const Vec<NR> &vin, const Vec<NR> &dg, double *out, Vec<NR> &res,
-                                           int lane)
+__device__ __forceinline__ void trsv_avail(const uint32_t *__restrict__ rec, int vcnt, int nlev,
+                                           const int (&rlev)[NR], const double *__restrict__ F,
+                                           const Vec<NR> &vin, const Vec<NR> &rinv, double *out,
+                                           double *acc, double *scratch, Vec<NR> &res, int lane)
 {
     constexpr int CH = 4;
-    double acc[NR];
+    const char *Fb = reinterpret_cast<const char *>(F);
+    const char *ob = reinterpret_cast<const char *>(out);
+    char *ab = reinterpret_cast<char *>(acc) + lane * 8;   // cell of row lane + 64 k at k * 512
 #pragma unroll
-    for (int k = 0; k < NR; k++) acc[k] = 0.0;
-    int s = 0, se = sptr[1];
-    uint32_t pre[CH];
+    for (int k = 0; k < NR; k++) acc[lane + WAVE * k] = 0.0;
+    uint32_t r[CH];
 #pragma unroll
-    for (int j = 0; j < CH; j++) pre[j] = (j < se) ? rec[j * WAVE + lane] : 0u;
+    for (int j = 0; j < CH; j++) r[j] = rec[j * WAVE + lane];
+    int chunk = 0;
+    wave_sync();
     for (int l = 0; l < nlev; l++) {
+        const int nch = __builtin_amdgcn_readlane(vcnt, l);
 #pragma unroll
         for (int k = 0; k < NR; k++) {
-            if ((LOWER ? (rlev[k] & 0xFFFF) : (rlev[k] >> 16)) == l) {
-                const double v = LOWER ? vin.e[k] - acc[k] : (vin.e[k] - acc[k]) / dg.e[k];
-                res.e[k] = v;
-                out[lane + WAVE * k] = v;
-            }
+            const bool mine = (LOWER ? (rlev[k] & 0xFFFF) : (rlev[k] >> 16)) == l;
+            const double a = acc[lane + WAVE * k];
+            const double v = LOWER ? vin.e[k] - a : (vin.e[k] - a) * rinv.e[k];
+            res.e[k] = mine ? v : res.e[k];
+            double *dst = mine ? out + (lane + WAVE * k) : scratch;
+            *dst = v;
         }
         wave_sync();
-        const int nse = (l + 1 < nlev) ? (int)sptr[l + 2] : se;
-        uint32_t r[CH];
+        for (int c = 0; c < nch; c++, chunk++) {
+            uint32_t rn[CH];
 #pragma unroll
-        for (int j = 0; j < CH; j++) r[j] = pre[j];
-#pragma unroll
-        for (int j = 0; j < CH; j++) pre[j] = (se + j < nse) ? rec[(se + j) * WAVE + lane] : 0u;
-        for (int c = s; c < se; c += CH) {
-            if (c != s) {
-#pragma unroll
-                for (int j = 0; j < CH; j++) r[j] = (c + j < se) ? rec[(c + j) * WAVE + lane] : 0u;
-            }
+            for (int j = 0; j < CH; j++) rn[j] = rec[((chunk + 1) * CH + j) * WAVE + lane];
             double f[CH], yv[CH];
 #pragma unroll
             for (int j = 0; j < CH; j++) {
-                f[j] = F[r[j] & 0x3FFFu];
-                yv[j] = out[(r[j] >> 14) & 0x3FFu];
+                f[j] = *reinterpret_cast<const double *>(Fb + (r[j] & 0xFFFFu));
+                yv[j] = *reinterpret_cast<const double *>(ob + ((r[j] >> 16) & 0xFFF8u));
             }
 #pragma unroll
             for (int j = 0; j < CH; j++) {
-                if (r[j] >> 27) {
-                    const int k = (r[j] >> 24) & 7;
-#pragma unroll
-                    for (int kk = 0; kk < NR; kk++)
-                        if (kk == k) acc[kk] = fma(f[j], yv[j], acc[kk]);
-                }
+                double *cell = reinterpret_cast<double *>(ab + (((r[j] >> 16) & 7u) << 9));
+                __hip_atomic_fetch_add(cell, f[j] * yv[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
             }
+#pragma unroll
+            for (int j = 0; j < CH; j++) r[j] = rn[j];
         }
-        s = se;
-        se = nse;
+        wave_sync();
     }
 }
 
@@ -220,18 +228,28 @@ __global__ __launch_bounds__(512) void k_fast(DevPattern P, FastParams fp, int b
     extern __shared__ __align__(16) unsigned char smem[];
     const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
     const int n = P.n, nnz = P.nnz, S = fp.S;
-    // shared: SpMV slot metadata [S][64]
+    // shared: SpMV slot metadata [SM][64] (padded); solve chunk records are read
+    // from global memory (cache-resident tables shared by every wave)
     uint32_t *meta = reinterpret_cast<uint32_t *>(smem);
-    for (int i = threadIdx.x; i < S * WAVE; i += blockDim.x) meta[i] = fp.slot_meta[i];
+    const uint32_t *trL = fp.trL;
+    const uint32_t *trU = fp.trU;
+    for (int i = threadIdx.x; i < SM * WAVE; i += blockDim.x) meta[i] = (i < S * WAVE) ? fp.slot_meta[i] : 0xFFFF0000u;
+    // per wave: F[nnz + 2] (F[nnz] = 0), (ParILU: W[nnz + 2]), e0[n + 2], e1[n + 2],
+    // acc[64 NR] (e[n] = 0 is the zero cell of padding records, e1[n + 1] a scratch cell)
+    double *F = reinterpret_cast<double *>(smem + fp.shared_bytes + (size_t)wave * fp.wave_bytes);
+    double *Wb = F + (nnz + 2);                       // ParILU only
+    double *e0 = Wb + (fp.par_w ? nnz + 2 : 0), *e1 = e0 + (n + 2);
+    double *scratch = e1 + (n + 1);
+    double *accb = e1 + (n + 2);
+    if (lane == 0) { F[nnz] = 0.0; e0[n] = 0.0; e1[n] = 0.0; }
     __syncthreads();
-    double *F = reinterpret_cast<double *>(smem + fp.meta_bytes + (size_t)wave * fp.wave_bytes);
-    double *Wb = F + nnz;                       // ParILU only
-    double *e0 = Wb + (fp.par_w ? nnz : 0), *e1 = e0 + n;
     // level of the lane's rows in the lower (low half) and upper (high half) solve
     int rlev[NR];
 #pragma unroll
     for (int k = 0; k < NR; k++)
         rlev[k] = (int)fp.rlevL[k * WAVE + lane] | ((int)fp.rlevU[k * WAVE + lane] << 16);
+    // chunks per level: lane l holds the count of level l
+    const int vcntL = fp.cntL[lane], vcntU = fp.cntU[lane];
 
     double a[SM];
 
@@ -288,22 +306,22 @@ __global__ __launch_bounds__(512) void k_fast(DevPattern P, FastParams fp, int b
         const double tol_abs = (sp.tol_mode == BILU_TOL_RELATIVE) ? sp.tol * bnorm : sp.tol;
         double iter_res = res0;
 
-        // U diagonal of the lane's rows (divisors of the upper solve)
-        Vec<NR> udg;
+        // reciprocal U diagonal of the lane's rows (multipliers of the upper solve)
+        Vec<NR> rinv;
 #pragma unroll
         for (int k = 0; k < NR; k++) {
             const int i = lane + WAVE * k;
-            udg.e[k] = (i < n) ? F[fp.diag[i]] : 1.0;
+            rinv.e[k] = (i < n) ? 1.0 / F[fp.diag[i]] : 1.0;
         }
         // out = U^{-1} L^{-1} in, in registers; also left in e0 for the following SpMV
         auto apply = [&](const Vec<NR> &in, Vec<NR> &out) {
             Vec<NR> y;
-            trsv_avail<NR, true>(fp.trL, fp.sptrL, fp.nlev_l, rlev, F, in, udg, e1, y, lane);
-            trsv_avail<NR, false>(fp.trU, fp.sptrU, fp.nlev_u, rlev, F, y, udg, e0, out, lane);
+            trsv_avail<NR, true>(trL, vcntL, fp.nlev_l, rlev, F, in, rinv, e1, accb, scratch, y, lane);
+            trsv_avail<NR, false>(trU, vcntU, fp.nlev_u, rlev, F, y, rinv, e0, accb, scratch, out, lane);
             wave_sync();
         };
         auto matvec = [&](Vec<NR> &out) {  // out = A e0
-            spmv_slots<SM>(a, meta, S, e0, e1, lane);
+            spmv_slots<SM>(a, meta, e0, e1, scratch, lane);
             wave_sync();
             from_lds(out, e1, lane, n);
             wave_sync();

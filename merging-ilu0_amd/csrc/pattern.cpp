@@ -227,43 +227,56 @@ void build_fast_plan(PatternAnalysis &pa)
     // pull-by-availability triangular-solve schedules
     const int NR = (n + 63) / 64;
     fp.nr = NR;
-    if (nnz >= (1 << 14) || n > 1023 || NR > 7) { fp.why = "pattern beyond the packed solve records"; return; }
-    auto trsv_sched = [&](bool lower, std::vector<uint32_t> &rec, std::vector<uint16_t> &sptr,
-                          std::vector<uint16_t> &rlev) {
+    if (nnz > 8190 || n > 4094 || NR > 7) { fp.why = "pattern beyond the packed solve records"; return; }
+    // padding records point at the zero cells (factor index nnz, vector index n), row slot 0
+    const uint32_t pad_rec = uint32_t(nnz * 8) | (uint32_t(n * 8) << 16);
+    auto trsv_sched = [&](bool lower, std::vector<uint32_t> &rec, std::vector<uint8_t> &cnt,
+                          std::vector<uint16_t> &rlev) -> bool {
         const std::vector<int> &lv = lower ? pa.lev_l : pa.lev_u;
         int nlev = 0;
         for (int i = 0; i < n; i++) nlev = std::max(nlev, lv[i] + 1);
-        // per level, per lane: entries (col, pos, k) whose column sits in that level
+        if (nlev > 64) return false;
+        // E[level][lane]: (col, pos, k) of the entries of the lane's rows whose column sits in that level
         std::vector<std::vector<std::vector<std::array<int, 3>>>> E(nlev, std::vector<std::vector<std::array<int, 3>>>(64));
         for (int i = 0; i < n; i++) {
             const int b = lower ? pa.rp[i] : pa.diag[i] + 1, e = lower ? pa.diag[i] : pa.rp[i + 1];
             for (int p = b; p < e; p++) E[lv[pa.ci[p]]][i % 64].push_back({pa.ci[p], p, i / 64});
         }
-        sptr.assign(1, 0);
         rec.clear();
+        cnt.assign(64, 0);
         for (int l = 0; l < nlev; l++) {
             size_t S = 0;
             for (int t = 0; t < 64; t++) {
                 std::sort(E[l][t].begin(), E[l][t].end());
                 S = std::max(S, E[l][t].size());
             }
-            for (size_t sl = 0; sl < S; sl++)
+            const size_t chunks = (S + FastPlan::CH - 1) / FastPlan::CH;
+            if (chunks > 255) return false;
+            cnt[l] = uint8_t(chunks);
+            for (size_t sl = 0; sl < chunks * FastPlan::CH; sl++)
                 for (int t = 0; t < 64; t++) {
                     if (sl < E[l][t].size()) {
                         const auto &x = E[l][t][sl];
-                        rec.push_back(uint32_t(x[1]) | (uint32_t(x[0]) << 14) | (uint32_t(x[2]) << 24) | (1u << 27));
+                        rec.push_back(uint32_t(x[1] * 8) | (uint32_t(x[0] * 8 + x[2]) << 16));
                     } else {
-                        rec.push_back(0);
+                        rec.push_back(pad_rec);
                     }
                 }
-            sptr.push_back(uint16_t(rec.size() / 64));
         }
+        rec.insert(rec.end(), size_t(64) * FastPlan::CH, pad_rec);   // one chunk of tail padding (prefetch)
         rlev.assign(size_t(NR) * 64, 0xFFFF);
         for (int i = 0; i < n; i++) rlev[size_t(i / 64) * 64 + i % 64] = uint16_t(lv[i]);
-        if (rec.empty()) rec.assign(64, 0);
+        return true;
     };
-    trsv_sched(true, fp.trL, fp.sptrL, fp.rlevL);
-    trsv_sched(false, fp.trU, fp.sptrU, fp.rlevU);
+    if (!trsv_sched(true, fp.trL, fp.cntL, fp.rlevL) || !trsv_sched(false, fp.trU, fp.cntU, fp.rlevU)) {
+        fp.why = "more than 64 solve levels";
+        return;
+    }
+    // SpMV slot metadata as LDS byte offsets: x offset | (row offset or 0xFFFF) << 16
+    for (auto &m : fp.slot_meta) {
+        const uint32_t col = m & 0xFFFFu, row = m >> 16;
+        m = (col * 8) | ((row == 0xFFFFu ? 0xFFFFu : row * 8) << 16);
+    }
     // entry-DAG levels of the factorization (pull form of the KIJ updates)
     std::vector<int> elev(nnz, 0), pdiv(nnz, 0xFFFF);
     int maxlev = 0;

@@ -25,10 +25,12 @@ struct IsaiMap {
 //   (k < NR) and keeps their partial sums in registers.  After level ℓ
 //   finalises its rows, every lane folds in the entries of its rows whose
 //   column lies in level ℓ.  tr{L,U}[s][64]: one u32 record per slot and
-//   lane = fpos | col << 14 | k << 24 | valid << 27; slots of level ℓ are
-//   s in [sptr[ℓ], sptr[ℓ+1]); a lane's entries of one row run in increasing
-//   column order within a level, so each row sum is an fma chain in
-//   (level of column, column) order.
+//   lane = 8*pos | (8*col + k) << 16 (LDS byte offsets into the factor and
+//   the vector, k = the lane's row slot); padding records point at zero
+//   cells (factor index nnz, vector index n).  Each level owns cnt[ℓ]
+//   chunks of CH consecutive slots; a lane's entries of one row run in
+//   increasing column order within a level, so each row sum is an fma chain
+//   in (level of column, column) order.
 //   rlev{L,U}[k][64]: level of row l + 64 k (0xFFFF: no such row).
 //   Factorization: etab records (pos, dep_begin, dep_end, diag position of
 //   the column or 0xFFFF for upper entries) grouped by dependency level of
@@ -37,13 +39,16 @@ struct FastPlan {
     bool ok = false;          // pattern fits the packed tables
     std::string why;          // reason when !ok
     int S = 0;                // SpMV slots per lane (balanced row -> lane assignment)
-    // slot tables [S][64]: meta = col | (row << 16), row = 0xFFFF if the slot
-    // does not end a row; pos = pattern position (-1 = padding)
+    // slot tables [S][64]: meta = 8*col | (8*row << 16) as LDS byte offsets,
+    // 0xFFFF in the high half if the slot does not end a row; pos = pattern
+    // position (-1 = padding)
     std::vector<uint32_t> slot_meta;
     std::vector<int32_t> slot_pos;
+    static constexpr int CH = 4;   // solve slots per chunk
     int nr = 0;
     std::vector<uint32_t> trL, trU;
-    std::vector<uint16_t> sptrL, sptrU, rlevL, rlevU;
+    std::vector<uint8_t> cntL, cntU;           // chunks per level (index = level, < 64)
+    std::vector<uint16_t> rlevL, rlevU;
     std::vector<uint16_t> etab, eptr, dep;   // etab: 4 u16 per record, dep: 2 u16 per pair
     int nlev_e = 0, ne = 0;
 };

@@ -713,7 +713,8 @@ struct bilu_pattern_s {
     uint8_t *d_fast = nullptr;     // all persistent-kernel tables, one allocation
     uint32_t *d_slot_meta = nullptr, *d_trL = nullptr, *d_trU = nullptr;
     int32_t *d_slot_pos = nullptr;
-    uint16_t *d_sptrL = nullptr, *d_sptrU = nullptr, *d_rlevL = nullptr, *d_rlevU = nullptr;
+    uint8_t *d_cntL = nullptr, *d_cntU = nullptr;
+    uint16_t *d_rlevL = nullptr, *d_rlevU = nullptr;
     uint16_t *d_etab = nullptr, *d_eptr = nullptr, *d_dep = nullptr;
     int *d_counter = nullptr;
 };
@@ -787,8 +788,8 @@ int upload(bilu_pattern_s &h)
         const size_t o_pos = put(fp.slot_pos.data(), fp.slot_pos.size() * 4);
         const size_t o_trL = put(fp.trL.data(), fp.trL.size() * 4);
         const size_t o_trU = put(fp.trU.data(), fp.trU.size() * 4);
-        const size_t o_sL = put(fp.sptrL.data(), fp.sptrL.size() * 2);
-        const size_t o_sU = put(fp.sptrU.data(), fp.sptrU.size() * 2);
+        const size_t o_sL = put(fp.cntL.data(), fp.cntL.size());
+        const size_t o_sU = put(fp.cntU.data(), fp.cntU.size());
         const size_t o_rL = put(fp.rlevL.data(), fp.rlevL.size() * 2);
         const size_t o_rU = put(fp.rlevU.data(), fp.rlevU.size() * 2);
         const size_t o_et = put(fp.etab.data(), fp.etab.size() * 2);
@@ -799,7 +800,7 @@ int upload(bilu_pattern_s &h)
         uint8_t *b = h.d_fast;
         h.d_slot_meta = (uint32_t *)(b + o_meta); h.d_slot_pos = (int32_t *)(b + o_pos);
         h.d_trL = (uint32_t *)(b + o_trL); h.d_trU = (uint32_t *)(b + o_trU);
-        h.d_sptrL = (uint16_t *)(b + o_sL); h.d_sptrU = (uint16_t *)(b + o_sU);
+        h.d_cntL = b + o_sL; h.d_cntU = b + o_sU;
         h.d_rlevL = (uint16_t *)(b + o_rL); h.d_rlevU = (uint16_t *)(b + o_rU);
         h.d_etab = (uint16_t *)(b + o_et); h.d_eptr = (uint16_t *)(b + o_ep); h.d_dep = (uint16_t *)(b + o_dp);
     }
@@ -844,23 +845,28 @@ int launch_fast(bilu_pattern_s &h, int batch, const double *A, const double *b, 
     const bool par = sp.factor == FK_PARILU;
     if (pl.nr != nr) return -1;
     FastParams fp{};
-    fp.trL = h.d_trL; fp.trU = h.d_trU; fp.sptrL = h.d_sptrL; fp.sptrU = h.d_sptrU;
+    fp.trL = h.d_trL; fp.trU = h.d_trU;
+    fp.trL_words = int(pl.trL.size()); fp.trU_words = int(pl.trU.size());
+    fp.cntL = h.d_cntL; fp.cntU = h.d_cntU;
     fp.rlevL = h.d_rlevL; fp.rlevU = h.d_rlevU;
     fp.etab = reinterpret_cast<const ushort4 *>(h.d_etab); fp.eptr = h.d_eptr;
     fp.dep = reinterpret_cast<const ushort2 *>(h.d_dep);
     fp.diag = h.dev.diag;
     fp.nlev_l = h.dev.nlev_l; fp.nlev_u = h.dev.nlev_u; fp.nlev_e = pl.nlev_e; fp.ne = pl.ne;
     fp.S = pl.S; fp.slot_meta = h.d_slot_meta; fp.slot_pos = h.d_slot_pos;
-    // per wave: F, (ParILU: the previous sweep's factors), e0, e1
-    fp.wave_bytes = int(((size_t)((par ? 2 : 1) * nnz + 2 * n) * sizeof(double) + 15) / 16 * 16);
+    // per wave: F (+2 cells), (ParILU: the previous sweep's factors), e0, e1 (+2 cells each),
+    // row accumulators of the solves (64 NR)
+    fp.wave_bytes = int(((size_t)((par ? 2 : 1) * (nnz + 2) + 2 * (n + 2) + 64 * nr) * sizeof(double) + 15) / 16 * 16);
     fp.par_w = par ? 1 : 0;
     fp.counter = h.d_counter;
-    fp.meta_bytes = int(((size_t)pl.S * WAVE * 4 + 15) / 16 * 16);
-    if ((size_t)fp.meta_bytes >= h.max_lds) return -1;
-    int waves = (int)std::min<size_t>(8, (h.max_lds - fp.meta_bytes) / fp.wave_bytes);
+    auto a16 = [](size_t x) { return (x + 15) / 16 * 16; };
+    fp.o_trL = fp.o_trU = 0;
+    fp.shared_bytes = int(a16((size_t)sm * WAVE * 4));
+    if ((size_t)fp.shared_bytes >= h.max_lds) return -1;
+    int waves = (int)std::min<size_t>(8, (h.max_lds - fp.shared_bytes) / fp.wave_bytes);
     if (waves < 1) return -1;
     waves = std::min(waves, std::max(1, batch));
-    const size_t lds = fp.meta_bytes + (size_t)waves * fp.wave_bytes;
+    const size_t lds = fp.shared_bytes + (size_t)waves * fp.wave_bytes;
     const int per_cu = std::max<int>(1, (int)(h.lds_per_cu / lds));
     int grid = h.num_cus * per_cu;
     grid = std::max(1, std::min(grid, (batch + waves - 1) / waves));

@@ -216,10 +216,12 @@ void orc_trsv_levels(int n, const int *rp, const int *ci, const int *diag, int *
     }
 }
 
-/* Same solves as orc_lu_solve, each row sum an fma chain in "availability"
- * order: the row's entries sorted by (level of the column, column).  This is
- * the order in which the persistent wave64 kernel folds entries in, as
- * each level of rows becomes final. */
+/* Same solves as orc_lu_solve in the persistent wave64 kernel's order: each
+ * row sum adds rounded products (no fma) in "availability" order, the row's
+ * entries sorted by (level of the column, column) -- the order in which the
+ * kernel adds entries into the row's accumulator as each level of rows
+ * becomes final -- and the upper solve multiplies by the reciprocal of u_ii
+ * (computed once per factor) instead of dividing. */
 static void sort_avail(int *pairs, int m, const int *lev)
 {
     /* insertion sort of (col, pos) pairs keyed on (lev[col], col) */
@@ -246,7 +248,7 @@ void orc_lu_solve_avail(int n, const int *rp, const int *ci, const int *diag,
         for (int p = rp[i]; p < diag[i] && m < 1024; p++, m++) { pairs[2 * m] = ci[p]; pairs[2 * m + 1] = p; }
         sort_avail(pairs, m, lev_l);
         double sum = 0.0;
-        for (int q = 0; q < m; q++) sum = fma(F[pairs[2 * q + 1]], tmp[pairs[2 * q]], sum);
+        for (int q = 0; q < m; q++) sum = sum + F[pairs[2 * q + 1]] * tmp[pairs[2 * q]];
         tmp[i] = v[i] - sum;
     }
     for (int i = n - 1; i >= 0; i--) {
@@ -254,8 +256,8 @@ void orc_lu_solve_avail(int n, const int *rp, const int *ci, const int *diag,
         for (int p = diag[i] + 1; p < rp[i + 1] && m < 1024; p++, m++) { pairs[2 * m] = ci[p]; pairs[2 * m + 1] = p; }
         sort_avail(pairs, m, lev_u);
         double sum = 0.0;
-        for (int q = 0; q < m; q++) sum = fma(F[pairs[2 * q + 1]], out[pairs[2 * q]], sum);
-        out[i] = (tmp[i] - sum) / F[diag[i]];
+        for (int q = 0; q < m; q++) sum = sum + F[pairs[2 * q + 1]] * out[pairs[2 * q]];
+        out[i] = (tmp[i] - sum) * (1.0 / F[diag[i]]);
     }
 }
 
