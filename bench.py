@@ -37,6 +37,19 @@ def parse():
     return ap.parse_args()
 
 
+def measured_traffic(key):
+    """HBM bytes per launch of the dominant kernel from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE runs (profiles/traffic.json), each
+    scaled by what the counter reads for a 1 GiB device copy; None when that
+    workload has not been profiled."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
+            rec = json.load(f).get(key)
+    except (OSError, ValueError):
+        return None
+    return None if rec is None else rec["traffic_bytes_per_launch"]
+
+
 def cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
@@ -57,17 +70,17 @@ def cpu_baseline(pat, A, b, precond):
             "parilu0_isai": O.PARILU0_ISAI, "general_isai": O.GENERAL_ISAI}[precond]
     oo = O.opts(tol=1e-10, tol_mode=O.TOL_RELATIVE, true_res_restart=1)
     threads = max(1, min(os.cpu_count() or 1, 16))
-    s_all = min(A.shape[0], 32768)
+    s_all = A.shape[0]
     t = time.perf_counter()
     O.batch_solve(pat, A[:s_all], b[:s_all], code, sweeps=5, o=oo, threads=threads)
     dt_all = time.perf_counter() - t
-    s_one = min(A.shape[0], 2048)
+    s_one = min(A.shape[0], 16384)
     t = time.perf_counter()
     O.batch_solve(pat, A[:s_one], b[:s_one], code, sweeps=5, o=oo, threads=1)
     dt_one = time.perf_counter() - t
     return {"value": round(s_all / dt_all, 1), "unit": "systems/s", "cores": threads,
             "kind": "port",
-            "sample": f"first {s_all} systems of the batch, {threads} threads, {dt_all:.2f} s wall",
+            "sample": f"{s_all} systems (the whole batch), {threads} threads, {dt_all:.2f} s wall",
             "single_core_value": round(s_one / dt_one, 1),
             "single_core_sample": f"first {s_one} systems, 1 thread, {dt_one:.2f} s",
             "cpu_model": cpu_model()}
@@ -145,8 +158,10 @@ def main():
     alg_bytes_per_system = 8 * nnz + 16 * n + 24
     alg_bytes = alg_bytes_per_system * B + 4 * (n + 1) + 4 * nnz
     achieved = alg_bytes / (ms_per_step * 1e-3) / 1e9
+    kname = "k_fast" if (args.merge == "merged" and args.kernel != "wave") else None
+    traffic = measured_traffic(f"{kname} {args.precond} {args.merge} n{n} b{B}") if kname else None
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                 "kernel": ("k_solve (one wave per system)" if args.kernel == "wave" else
                            "k_fast (persistent multi-wave)") if args.merge == "merged" else "k_factor + solve",
                 "alg_bytes_per_launch": alg_bytes}
