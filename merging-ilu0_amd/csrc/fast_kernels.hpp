@@ -3,7 +3,7 @@
 // inside its anonymous namespace (uses DevPattern, SolveParams, Vec, vdot,
 // to_lds, from_lds, wave_sum).
 //
-// Layout of one workgroup (W waves, W <= 8, one system per wave at a time):
+// Layout of one workgroup (W <= KF_MAX_WAVES waves, one system per wave at a time):
 //   LDS  [ wave 0: F[nnz] fp64 | (ParILU: W[nnz]) | e0[n] | e1[n] ] [ wave 1 ... ]
 //   global (cache-resident, shared by every wave): solve and factorization
 //        schedules, SpMV slot tables
@@ -27,7 +27,7 @@
 struct FastParams {
     const uint32_t *trL, *trU;            // solve chunk records [chunk][CH][64], tail-padded
     int trL_words, trU_words;
-    const uint8_t *cntL, *cntU;           // chunks per level, [64]
+    const uint16_t *cntL, *cntU;          // per level: chunks | row-slot mask << 8, [64]
     const uint16_t *rlevL, *rlevU;        // [NR][64] level of the lane's rows
     const ushort4 *etab;                  // factorization entry records
     const uint16_t *eptr;                 // per entry-DAG level
@@ -52,6 +52,13 @@ __device__ __forceinline__ void wave_sync()
 }
 
 __global__ void k_zero_counter(int *c) { *c = 0; }
+
+// waves per workgroup of the persistent kernel (its launch bound; the
+// register budget per lane is 512 / (waves / 4) rounded to the allocation
+// granule)
+#ifndef KF_MAX_WAVES
+#define KF_MAX_WAVES 8
+#endif
 
 // y (LDS, by row) = A x over the slot layout; x in LDS.  Slot metadata
 // (8*col | 8*row << 16, high half 0xFFFF when the slot does not end a row)
@@ -87,17 +94,18 @@ __device__ __forceinline__ void spmv_slots(const double (&a)[SM], const uint32_t
 // LDS floating-point atomic (fire and forget; a lane only touches its own
 // rows, so the additions of one row land in program order).  Records (LDS
 // byte offsets + row slot k) come in chunks of CH per level; padding records
-// point at zero cells, so every chunk body is straight-line code.  The chunk
-// count of level ℓ is held by lane ℓ of `vcnt` and read with readlane (no
-// memory access on the loop bound); the next chunk's records are loaded one
-// chunk ahead.
+// point at zero cells, so every chunk body is straight-line code.  Lane ℓ of
+// `vcnt` holds level ℓ's chunk count (low byte) and the mask of the row slots
+// k that hold a row of level ℓ (bits 8..); it is read with readlane (no
+// memory access on the loop bound), and row slots outside the mask skip the
+// finalisation.  The next chunk's records are loaded one chunk ahead.
 template <int NR, bool LOWER>
 __device__ __forceinline__ void trsv_avail(const uint32_t *__restrict__ rec, int vcnt, int nlev,
                                            const int (&rlev)[NR], const double *__restrict__ F,
                                            const Vec<NR> &vin, const Vec<NR> &rinv, double *out,
                                            double *acc, double *scratch, Vec<NR> &res, int lane)
 {
-    constexpr int CH = 4;
+    constexpr int CH = BILU_SOLVE_CH;
     const char *Fb = reinterpret_cast<const char *>(F);
     const char *ob = reinterpret_cast<const char *>(out);
     char *ab = reinterpret_cast<char *>(acc) + lane * 8;   // cell of row lane + 64 k at k * 512
@@ -109,9 +117,11 @@ __device__ __forceinline__ void trsv_avail(const uint32_t *__restrict__ rec, int
     int chunk = 0;
     wave_sync();
     for (int l = 0; l < nlev; l++) {
-        const int nch = __builtin_amdgcn_readlane(vcnt, l);
+        const int cm = __builtin_amdgcn_readlane(vcnt, l);
+        const int nch = cm & 0xFF;
 #pragma unroll
         for (int k = 0; k < NR; k++) {
+            if (!(cm & (0x100 << k))) continue;   // no row of this level in slot k
             const bool mine = (LOWER ? (rlev[k] & 0xFFFF) : (rlev[k] >> 16)) == l;
             const double a = acc[lane + WAVE * k];
             const double v = LOWER ? vin.e[k] - a : (vin.e[k] - a) * rinv.e[k];
@@ -218,7 +228,7 @@ __device__ __forceinline__ void parilu_sweep(const ushort4 *__restrict__ etab, i
 }
 
 template <int NR, int SM>
-__global__ __launch_bounds__(512) void k_fast(DevPattern P, FastParams fp, int batch,
+__global__ __launch_bounds__(KF_MAX_WAVES * WAVE) void k_fast(DevPattern P, FastParams fp, int batch,
                                               const double *__restrict__ gA, const double *__restrict__ gb,
                                               double *__restrict__ gx, const double *__restrict__ gF,
                                               SolveParams sp, float *__restrict__ g_iter,

@@ -230,7 +230,7 @@ void build_fast_plan(PatternAnalysis &pa)
     if (nnz > 8190 || n > 4094 || NR > 7) { fp.why = "pattern beyond the packed solve records"; return; }
     // padding records point at the zero cells (factor index nnz, vector index n), row slot 0
     const uint32_t pad_rec = uint32_t(nnz * 8) | (uint32_t(n * 8) << 16);
-    auto trsv_sched = [&](bool lower, std::vector<uint32_t> &rec, std::vector<uint8_t> &cnt,
+    auto trsv_sched = [&](bool lower, std::vector<uint32_t> &rec, std::vector<uint16_t> &cnt,
                           std::vector<uint16_t> &rlev) -> bool {
         const std::vector<int> &lv = lower ? pa.lev_l : pa.lev_u;
         int nlev = 0;
@@ -252,7 +252,7 @@ void build_fast_plan(PatternAnalysis &pa)
             }
             const size_t chunks = (S + FastPlan::CH - 1) / FastPlan::CH;
             if (chunks > 255) return false;
-            cnt[l] = uint8_t(chunks);
+            cnt[l] = uint16_t(chunks);
             for (size_t sl = 0; sl < chunks * FastPlan::CH; sl++)
                 for (int t = 0; t < 64; t++) {
                     if (sl < E[l][t].size()) {
@@ -265,7 +265,10 @@ void build_fast_plan(PatternAnalysis &pa)
         }
         rec.insert(rec.end(), size_t(64) * FastPlan::CH, pad_rec);   // one chunk of tail padding (prefetch)
         rlev.assign(size_t(NR) * 64, 0xFFFF);
-        for (int i = 0; i < n; i++) rlev[size_t(i / 64) * 64 + i % 64] = uint16_t(lv[i]);
+        for (int i = 0; i < n; i++) {
+            rlev[size_t(i / 64) * 64 + i % 64] = uint16_t(lv[i]);
+            cnt[lv[i]] |= uint16_t(1u << (8 + i / 64));
+        }
         return true;
     };
     if (!trsv_sched(true, fp.trL, fp.cntL, fp.rlevL) || !trsv_sched(false, fp.trU, fp.cntU, fp.rlevU)) {

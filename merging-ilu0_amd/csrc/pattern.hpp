@@ -32,9 +32,15 @@ struct IsaiMap {
 //   increasing column order within a level, so each row sum is an fma chain
 //   in (level of column, column) order.
 //   rlev{L,U}[k][64]: level of row l + 64 k (0xFFFF: no such row).
+//   A level's rows finalise only in the row slots k named by its mask.
 //   Factorization: etab records (pos, dep_begin, dep_end, diag position of
 //   the column or 0xFFFF for upper entries) grouped by dependency level of
 //   the entry DAG (eptr); dep holds (pos(i,k), pos(k,j)) pairs, k increasing.
+// solve records per lane and chunk (one chunk = CH instruction slots)
+#ifndef BILU_SOLVE_CH
+#define BILU_SOLVE_CH 4
+#endif
+
 struct FastPlan {
     bool ok = false;          // pattern fits the packed tables
     std::string why;          // reason when !ok
@@ -44,10 +50,12 @@ struct FastPlan {
     // position (-1 = padding)
     std::vector<uint32_t> slot_meta;
     std::vector<int32_t> slot_pos;
-    static constexpr int CH = 4;   // solve slots per chunk
+    static constexpr int CH = BILU_SOLVE_CH;   // solve slots per chunk
     int nr = 0;
     std::vector<uint32_t> trL, trU;
-    std::vector<uint8_t> cntL, cntU;           // chunks per level (index = level, < 64)
+    // per level (index < 64): chunk count | (mask of the row slots k holding a
+    // row of the level) << 8
+    std::vector<uint16_t> cntL, cntU;
     std::vector<uint16_t> rlevL, rlevU;
     std::vector<uint16_t> etab, eptr, dep;   // etab: 4 u16 per record, dep: 2 u16 per pair
     int nlev_e = 0, ne = 0;

@@ -713,7 +713,7 @@ struct bilu_pattern_s {
     uint8_t *d_fast = nullptr;     // all persistent-kernel tables, one allocation
     uint32_t *d_slot_meta = nullptr, *d_trL = nullptr, *d_trU = nullptr;
     int32_t *d_slot_pos = nullptr;
-    uint8_t *d_cntL = nullptr, *d_cntU = nullptr;
+    uint16_t *d_cntL = nullptr, *d_cntU = nullptr;
     uint16_t *d_rlevL = nullptr, *d_rlevU = nullptr;
     uint16_t *d_etab = nullptr, *d_eptr = nullptr, *d_dep = nullptr;
     int *d_counter = nullptr;
@@ -788,8 +788,8 @@ int upload(bilu_pattern_s &h)
         const size_t o_pos = put(fp.slot_pos.data(), fp.slot_pos.size() * 4);
         const size_t o_trL = put(fp.trL.data(), fp.trL.size() * 4);
         const size_t o_trU = put(fp.trU.data(), fp.trU.size() * 4);
-        const size_t o_sL = put(fp.cntL.data(), fp.cntL.size());
-        const size_t o_sU = put(fp.cntU.data(), fp.cntU.size());
+        const size_t o_sL = put(fp.cntL.data(), fp.cntL.size() * 2);
+        const size_t o_sU = put(fp.cntU.data(), fp.cntU.size() * 2);
         const size_t o_rL = put(fp.rlevL.data(), fp.rlevL.size() * 2);
         const size_t o_rU = put(fp.rlevU.data(), fp.rlevU.size() * 2);
         const size_t o_et = put(fp.etab.data(), fp.etab.size() * 2);
@@ -800,7 +800,7 @@ int upload(bilu_pattern_s &h)
         uint8_t *b = h.d_fast;
         h.d_slot_meta = (uint32_t *)(b + o_meta); h.d_slot_pos = (int32_t *)(b + o_pos);
         h.d_trL = (uint32_t *)(b + o_trL); h.d_trU = (uint32_t *)(b + o_trU);
-        h.d_cntL = b + o_sL; h.d_cntU = b + o_sU;
+        h.d_cntL = (uint16_t *)(b + o_sL); h.d_cntU = (uint16_t *)(b + o_sU);
         h.d_rlevL = (uint16_t *)(b + o_rL); h.d_rlevU = (uint16_t *)(b + o_rU);
         h.d_etab = (uint16_t *)(b + o_et); h.d_eptr = (uint16_t *)(b + o_ep); h.d_dep = (uint16_t *)(b + o_dp);
     }
@@ -863,7 +863,7 @@ int launch_fast(bilu_pattern_s &h, int batch, const double *A, const double *b, 
     fp.o_trL = fp.o_trU = 0;
     fp.shared_bytes = int(a16((size_t)sm * WAVE * 4));
     if ((size_t)fp.shared_bytes >= h.max_lds) return -1;
-    int waves = (int)std::min<size_t>(8, (h.max_lds - fp.shared_bytes) / fp.wave_bytes);
+    int waves = (int)std::min<size_t>(KF_MAX_WAVES, (h.max_lds - fp.shared_bytes) / fp.wave_bytes);
     if (waves < 1) return -1;
     waves = std::min(waves, std::max(1, batch));
     const size_t lds = fp.shared_bytes + (size_t)waves * fp.wave_bytes;
