@@ -31,6 +31,9 @@ def parse():
                     choices=["ilu0", "parilu0", "ilu0_isai", "parilu0_isai", "general_isai"])
     ap.add_argument("--merge", default="merged", choices=["merged", "segregated"])
     ap.add_argument("--kernel", default="auto", choices=["auto", "wave", "persistent"])
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32_ir"],
+                    help="fp32_ir: FP32 preconditioner and inner BiCGSTAB inside FP64 refinement")
+    ap.add_argument("--inner-tol", type=float, default=1e-5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run only a 1 GiB device copy (HBM counter calibration)")
@@ -128,12 +131,20 @@ def main():
     merge = bilu.MERGED if args.merge == "merged" else bilu.SEGREGATED
     kern = {"auto": bilu.KERNEL_AUTO, "wave": bilu.KERNEL_WAVE,
             "persistent": bilu.KERNEL_PERSISTENT}[args.kernel]
+    mixed = args.precision == "fp32_ir"
     o = bilu.opts(precond=pc, parilu_sweeps=5, tol=1e-10, tol_mode=bilu.TOL_RELATIVE,
-                  true_res_restart=1, max_restarts=3, merge=merge, kernel=kern)
+                  true_res_restart=1, max_restarts=3, merge=merge, kernel=kern,
+                  precision=bilu.PRECISION_FP32_IR if mixed else bilu.PRECISION_FP64,
+                  inner_tol=args.inner_tol)
     stream = torch.cuda.current_stream()
+    refine = torch.zeros(B, dtype=torch.int32, device=dev)
+    fstat = torch.zeros(B, dtype=torch.int32, device=dev)
 
     def step():
-        bilu.solve(P, dA, db, x, o, it, cv, ir, tr, stream=stream)
+        if mixed:
+            bilu.solve_ex(P, dA, db, x, o, None, it, cv, ir, tr, refine, fstat, stream=stream)
+        else:
+            bilu.solve(P, dA, db, x, o, it, cv, ir, tr, stream=stream)
 
     for _ in range(args.warmup):
         step()
@@ -158,12 +169,14 @@ def main():
     alg_bytes_per_system = 8 * nnz + 16 * n + 24
     alg_bytes = alg_bytes_per_system * B + 4 * (n + 1) + 4 * nnz
     achieved = alg_bytes / (ms_per_step * 1e-3) / 1e9
-    kname = "k_fast" if (args.merge == "merged" and args.kernel != "wave") else None
+    kname = ("k_mixed" if mixed else
+             "k_fast" if (args.merge == "merged" and args.kernel != "wave") else None)
     traffic = measured_traffic(f"{kname} {args.precond} {args.merge} n{n} b{B}") if kname else None
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
-                "kernel": ("k_solve (one wave per system)" if args.kernel == "wave" else
-                           "k_fast (persistent multi-wave)") if args.merge == "merged" else "k_factor + solve",
+                "kernel": ("k_factor_fast + k_mixed (FP32 inner, FP64 refinement)" if mixed else
+                           ("k_solve (one wave per system)" if args.kernel == "wave" else
+                            "k_fast (persistent multi-wave)") if args.merge == "merged" else "k_factor + solve"),
                 "alg_bytes_per_launch": alg_bytes}
 
     cpu = None
@@ -181,18 +194,24 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32+f64" if mixed else "f64",
         "data": "synthetic (counter-based generator, pattern seed 144, value seed 20261203)",
-        "config": {"workload": f"random pattern n={n} nnz={nnz}, batch {B}, exact ILU(0) merged "
-                               f"factor+BiCGSTAB, FP64, relres<=1e-10 (variant: plain FP64 "
-                               f"{args.precond} {args.merge})",
+        "config": {"workload": (f"random pattern n={n} nnz={nnz}, batch {B}, {args.precond} factor "
+                                f"(FP64, rounded to FP32) + FP32 BiCGSTAB (inner tol {args.inner_tol:g}) "
+                                f"inside FP64 iterative refinement, relres<=1e-10" if mixed else
+                                f"random pattern n={n} nnz={nnz}, batch {B}, exact ILU(0) merged "
+                                f"factor+BiCGSTAB, FP64, relres<=1e-10 (variant: plain FP64 "
+                                f"{args.precond} {args.merge})"),
                    "batch": B, "n": n, "nnz": nnz, "precond": args.precond, "merge": args.merge,
                    "levels_lower": info.levels_lower, "levels_upper": info.levels_upper,
-                   "kernel": args.kernel,
+                   "kernel": args.kernel, "precision": args.precision,
+                   "inner_tol": args.inner_tol if mixed else None,
                    "parallelism": "one wave64 per system; batch sharded over CUs"},
         "all_converged_relres_le_1e-10": all_ok,
         "max_relres": float(relres.max()),
         "iterations": {"min": float(ith.min()), "mean": float(ith.mean()), "max": float(ith.max())},
+        "refine_steps": ({"mean": float(refine.float().mean()), "max": int(refine.max())}
+                         if mixed else None),
         "roofline": roofline,
         "cpu_baseline": cpu,
         "gen_seconds": round(gen_s, 2),

@@ -65,6 +65,10 @@ extern "C" {
 #define BILU_KERNEL_WAVE 1        /* one wave64 workgroup per system, schedules read from global */
 #define BILU_KERNEL_PERSISTENT 2  /* persistent multi-wave workgroups, schedules in LDS */
 
+#define BILU_PRECISION_FP64 0     /* everything in FP64 (the reference arithmetic) */
+#define BILU_PRECISION_FP32_IR 1  /* FP32 preconditioner and inner BiCGSTAB inside FP64
+                                     iterative refinement (outer residual and its dot in FP64) */
+
 #define BILU_ISAI_GENERAL 0
 #define BILU_ISAI_LOWER 1     /* strictly-lower values + unit diagonal */
 #define BILU_ISAI_UPPER 2     /* diagonal + strictly-upper values */
@@ -81,6 +85,13 @@ typedef struct {
     int true_res_restart; /* 1: restart from b - A x when the true residual misses tol */
     int max_restarts;     /* bound on such restarts (default 3) */
     int kernel;           /* BILU_KERNEL_*: which kernel family runs the solve */
+    int precision;        /* BILU_PRECISION_* (default FP64) */
+    double inner_tol;     /* FP32_IR: relative tolerance of each inner solve (default 1e-5) */
+    int max_refine;       /* FP32_IR: refinement steps (default 10) */
+    double parilu_stop_tol; /* ParILU: > 0 stops the sweeps once ||(LU - A) on the pattern||_F
+                               <= parilu_stop_tol * ||A||_F (parilu_sweeps is then the cap)
+                               and falls back to exact ILU(0) when the cap is reached
+                               without it; 0 = fixed sweep count (default) */
 } bilu_opts;
 
 typedef struct {
@@ -91,7 +102,8 @@ typedef struct {
     int max_row_nnz;
 } bilu_pattern_info;
 
-/* defaults: ILU(0), 5 sweeps, tol 1e-11 absolute, 500 iterations, merged, no restart */
+/* defaults: ILU(0), 5 sweeps, tol 1e-11 absolute, 500 iterations, merged, no restart,
+ * FP64, inner_tol 1e-5, 10 refinement steps, fixed ParILU sweeps */
 void bilu_opts_default(bilu_opts *o);
 const char *bilu_version(void);
 const char *bilu_status_string(int status);
@@ -114,6 +126,21 @@ int bilu_pattern_lu_split(bilu_pattern_t p, int *l_row_ptrs, int *l_col_idx,
 int bilu_solve(bilu_pattern_t p, int batch, const double *d_A, const double *d_b,
                double *d_x, const bilu_opts *opts, float *d_iter, int *d_conv,
                double *d_iter_res, double *d_true_res, void *stream);
+
+/* bilu_solve with the extensions of the mixed-precision and warm-started
+ * paths: d_F_init[batch*nnz] (may be NULL) is the initial guess of the
+ * ParILU(0) sweeps (in-place factor layout, FP64); d_refine[batch] (may be
+ * NULL) receives the FP64 refinement steps of BILU_PRECISION_FP32_IR (0 for
+ * FP64); d_factor_status[batch] (may be NULL) receives the ParILU sweeps
+ * used, -1 when the residual stop fell back to exact ILU(0), 0 for exact
+ * ILU(0).  With BILU_PRECISION_FP32_IR, d_iter counts the inner FP32 half
+ * steps summed over the refinement steps and d_iter_res = d_true_res = the
+ * last FP64 residual norm.  No counterpart in the reference (its solvers are
+ * FP64 only and start ParILU from A, parILU_0_gpu.cu:293-358). */
+int bilu_solve_ex(bilu_pattern_t p, int batch, const double *d_A, const double *d_b,
+                  double *d_x, const bilu_opts *opts, const double *d_F_init, float *d_iter,
+                  int *d_conv, double *d_iter_res, double *d_true_res, int *d_refine,
+                  int *d_factor_status, void *stream);
 
 /* Exact ILU(0) in place on the pattern: d_F[batch*nnz] <- factors of d_A
  * (strict lower = L, unit diagonal implicit; upper = U).  d_status[batch]

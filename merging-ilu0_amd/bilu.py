@@ -22,12 +22,13 @@ TOL_ABSOLUTE, TOL_RELATIVE = 0, 1
 MERGED, SEGREGATED = 0, 1
 ISAI_GENERAL, ISAI_LOWER, ISAI_UPPER = 0, 1, 2
 KERNEL_AUTO, KERNEL_WAVE, KERNEL_PERSISTENT = 0, 1, 2
+PRECISION_FP64, PRECISION_FP32_IR = 0, 1
 
 # every symbol include/bilu.h declares
 EXPORTS = [
     "bilu_opts_default", "bilu_version", "bilu_status_string", "bilu_pattern_create",
     "bilu_pattern_destroy", "bilu_pattern_get_info", "bilu_pattern_lu_split", "bilu_solve",
-    "bilu_ilu0", "bilu_parilu0", "bilu_factor_residual", "bilu_split_lu", "bilu_isai",
+    "bilu_solve_ex", "bilu_ilu0", "bilu_parilu0", "bilu_factor_residual", "bilu_split_lu", "bilu_isai",
     "bilu_true_residual",
 ]
 
@@ -47,6 +48,10 @@ class Opts(C.Structure):
         ("true_res_restart", C.c_int),
         ("max_restarts", C.c_int),
         ("kernel", C.c_int),
+        ("precision", C.c_int),
+        ("inner_tol", C.c_double),
+        ("max_refine", C.c_int),
+        ("parilu_stop_tol", C.c_double),
     ]
 
 
@@ -76,6 +81,8 @@ def lib():
         L.bilu_pattern_get_info.argtypes = [vp, C.POINTER(PatternInfo)]
         L.bilu_pattern_lu_split.argtypes = [vp, i32p, i32p, i32p, i32p]
         L.bilu_solve.argtypes = [vp, C.c_int, vp, vp, vp, C.POINTER(Opts), vp, vp, vp, vp, vp]
+        L.bilu_solve_ex.argtypes = [vp, C.c_int, vp, vp, vp, C.POINTER(Opts), vp, vp, vp, vp, vp,
+                                    vp, vp, vp]
         L.bilu_ilu0.argtypes = [vp, C.c_int, vp, vp, vp, vp]
         L.bilu_parilu0.argtypes = [vp, C.c_int, vp, vp, vp, C.c_int, vp]
         L.bilu_factor_residual.argtypes = [vp, C.c_int, vp, vp, vp, vp]
@@ -93,9 +100,10 @@ def _check(rc, what):
 
 
 def opts(precond=ILU0, parilu_sweeps=5, tol=1e-11, tol_mode=TOL_ABSOLUTE, max_iter=500,
-         merge=MERGED, true_res_restart=0, max_restarts=3, kernel=KERNEL_AUTO) -> Opts:
+         merge=MERGED, true_res_restart=0, max_restarts=3, kernel=KERNEL_AUTO,
+         precision=PRECISION_FP64, inner_tol=1e-5, max_refine=10, parilu_stop_tol=0.0) -> Opts:
     return Opts(precond, parilu_sweeps, tol, tol_mode, max_iter, merge, true_res_restart,
-                max_restarts, kernel)
+                max_restarts, kernel, precision, inner_tol, max_refine, parilu_stop_tol)
 
 
 def _ptr(t):
@@ -178,6 +186,25 @@ def solve(pat: Pattern, A, b, x, o: Opts | None = None, iters=None, conv=None, i
                             _ptr(conv), _ptr(iter_res), _ptr(true_res), _stream(stream)),
            "bilu_solve")
     return iters, conv, iter_res, true_res
+
+
+def solve_ex(pat: Pattern, A, b, x, o: Opts | None = None, F_init=None, iters=None, conv=None,
+             iter_res=None, true_res=None, refine=None, factor_status=None, stream=None):
+    """bilu_solve_ex: mixed precision / warm-started ParILU.  Returns
+    (iters, conv, iter_res, true_res, refine, factor_status)."""
+    import torch
+    B = b.shape[0]
+    dev = b.device
+    mk = lambda t, dt: t if t is not None else torch.empty(B, dtype=dt, device=dev)
+    iters, conv = mk(iters, torch.float32), mk(conv, torch.int32)
+    iter_res, true_res = mk(iter_res, torch.float64), mk(true_res, torch.float64)
+    refine, factor_status = mk(refine, torch.int32), mk(factor_status, torch.int32)
+    o = o or opts()
+    _check(lib().bilu_solve_ex(pat.h, B, _ptr(A), _ptr(b), _ptr(x), C.byref(o), _ptr(F_init),
+                               _ptr(iters), _ptr(conv), _ptr(iter_res), _ptr(true_res),
+                               _ptr(refine), _ptr(factor_status), _stream(stream)),
+           "bilu_solve_ex")
+    return iters, conv, iter_res, true_res, refine, factor_status
 
 
 def ilu0(pat: Pattern, A, F, status=None, stream=None):

@@ -66,27 +66,32 @@ __global__ void k_zero_counter(int *c) { *c = 0; }
 // body is straight-line: every slot stores its running sum, to the row's
 // cell when the slot ends a row and to a scratch cell otherwise, so the
 // loads of all slots can be in flight together.
-template <int SM>
-__device__ __forceinline__ void spmv_slots(const double (&a)[SM], const uint32_t *__restrict__ meta,
-                                           const double *__restrict__ x, double *__restrict__ y,
-                                           double *__restrict__ scratch, int lane)
+template <int SM, typename T = double>
+__device__ __forceinline__ void spmv_slots(const T (&a)[SM], const uint32_t *__restrict__ meta,
+                                           const T *__restrict__ x, T *__restrict__ y,
+                                           T *__restrict__ scratch, int lane)
 {
+    // offsets in the tables are for 8-byte elements
+    constexpr int SH = sizeof(T) == 4 ? 1 : 0;
     const char *xb = reinterpret_cast<const char *>(x);
     char *yb = reinterpret_cast<char *>(y);
-    double acc = 0.0;
+    T acc = T(0);
 #pragma unroll
     for (int s = 0; s < SM; s++) {
         const uint32_t m = meta[s * WAVE + lane];
-        acc = fma(a[s], *reinterpret_cast<const double *>(xb + (m & 0xFFFFu)), acc);
+        acc = fmaT(a[s], *reinterpret_cast<const T *>(xb + ((m & 0xFFFFu) >> SH)), acc);
         const bool end = (m >> 16) != 0xFFFFu;
-        double *dst = end ? reinterpret_cast<double *>(yb + (m >> 16)) : scratch;
+        T *dst = end ? reinterpret_cast<T *>(yb + ((m >> 16) >> SH)) : scratch;
         *dst = acc;
-        acc = end ? 0.0 : acc;
+        acc = end ? T(0) : acc;
     }
 }
 
 // Triangular solve by availability.  Lane l owns rows l + 64 k; their
-// partial sums live in LDS cells acc[row] (per wave).  Level ℓ: (1) every
+// partial sums live in FP64 LDS cells acc[row] (per wave; for FP32 solves
+// too: the FP32 products are added into FP64 cells, rounded to FP32 once
+// when the row is final -- the LDS FP32 atomic add runs about 20x slower
+// than the FP64 one on gfx950).  Level ℓ: (1) every
 // row of level ℓ is final: L: y_r = vin_r - acc_r, U: x_r = (vin_r - acc_r)
 // * (1 / u_rr), kept in `res` and stored to `out` (LDS) for the gathers of
 // later levels; (2) each lane adds the products of its entries whose column
@@ -99,16 +104,17 @@ __device__ __forceinline__ void spmv_slots(const double (&a)[SM], const uint32_t
 // k that hold a row of level ℓ (bits 8..); it is read with readlane (no
 // memory access on the loop bound), and row slots outside the mask skip the
 // finalisation.  The next chunk's records are loaded one chunk ahead.
-template <int NR, bool LOWER>
+template <int NR, bool LOWER, typename T = double>
 __device__ __forceinline__ void trsv_avail(const uint32_t *__restrict__ rec, int vcnt, int nlev,
-                                           const int (&rlev)[NR], const double *__restrict__ F,
-                                           const Vec<NR> &vin, const Vec<NR> &rinv, double *out,
-                                           double *acc, double *scratch, Vec<NR> &res, int lane)
+                                           const int (&rlev)[NR], const T *__restrict__ F,
+                                           const Vec<NR, T> &vin, const Vec<NR, T> &rinv, T *out,
+                                           double *acc, T *scratch, Vec<NR, T> &res, int lane)
 {
     constexpr int CH = BILU_SOLVE_CH;
+    constexpr int SH = sizeof(T) == 4 ? 1 : 0;   // record offsets are for 8-byte elements
     const char *Fb = reinterpret_cast<const char *>(F);
     const char *ob = reinterpret_cast<const char *>(out);
-    char *ab = reinterpret_cast<char *>(acc) + lane * 8;   // cell of row lane + 64 k at k * 512
+    char *ab = reinterpret_cast<char *>(acc) + lane * 8;   // row lane + 64 k at k * 512
 #pragma unroll
     for (int k = 0; k < NR; k++) acc[lane + WAVE * k] = 0.0;
     uint32_t r[CH];
@@ -123,10 +129,10 @@ __device__ __forceinline__ void trsv_avail(const uint32_t *__restrict__ rec, int
         for (int k = 0; k < NR; k++) {
             if (!(cm & (0x100 << k))) continue;   // no row of this level in slot k
             const bool mine = (LOWER ? (rlev[k] & 0xFFFF) : (rlev[k] >> 16)) == l;
-            const double a = acc[lane + WAVE * k];
-            const double v = LOWER ? vin.e[k] - a : (vin.e[k] - a) * rinv.e[k];
+            const T a = (T)acc[lane + WAVE * k];
+            const T v = LOWER ? vin.e[k] - a : (vin.e[k] - a) * rinv.e[k];
             res.e[k] = mine ? v : res.e[k];
-            double *dst = mine ? out + (lane + WAVE * k) : scratch;
+            T *dst = mine ? out + (lane + WAVE * k) : scratch;
             *dst = v;
         }
         wave_sync();
@@ -134,16 +140,16 @@ __device__ __forceinline__ void trsv_avail(const uint32_t *__restrict__ rec, int
             uint32_t rn[CH];
 #pragma unroll
             for (int j = 0; j < CH; j++) rn[j] = rec[((chunk + 1) * CH + j) * WAVE + lane];
-            double f[CH], yv[CH];
+            T f[CH], yv[CH];
 #pragma unroll
             for (int j = 0; j < CH; j++) {
-                f[j] = *reinterpret_cast<const double *>(Fb + (r[j] & 0xFFFFu));
-                yv[j] = *reinterpret_cast<const double *>(ob + ((r[j] >> 16) & 0xFFF8u));
+                f[j] = *reinterpret_cast<const T *>(Fb + ((r[j] & 0xFFFFu) >> SH));
+                yv[j] = *reinterpret_cast<const T *>(ob + (((r[j] >> 16) & 0xFFF8u) >> SH));
             }
 #pragma unroll
             for (int j = 0; j < CH; j++) {
                 double *cell = reinterpret_cast<double *>(ab + (((r[j] >> 16) & 7u) << 9));
-                __hip_atomic_fetch_add(cell, f[j] * yv[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+                __hip_atomic_fetch_add(cell, (double)(f[j] * yv[j]), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
             }
 #pragma unroll
             for (int j = 0; j < CH; j++) r[j] = rn[j];

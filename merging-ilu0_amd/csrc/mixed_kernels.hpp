@@ -1,0 +1,376 @@
+// Mixed-precision path and the stand-alone persistent factorization kernel.
+// Included by solver.hip after fast_kernels.hpp (same anonymous namespace).
+//
+//   k_factor_fast<TO>  one wave per system at a time (persistent): exact
+//                      ILU(0) (pull form by entry levels) or ParILU(0) Jacobi
+//                      sweeps, optionally from a caller-supplied initial guess
+//                      and stopped on the factor residual, with exact ILU(0)
+//                      as the fallback; factors in FP64 in LDS, written out as
+//                      TO (float for the mixed solve, double otherwise).
+//   k_mixed<NR, SM>    FP64 iterative refinement around an FP32 BiCGSTAB:
+//                      x += |r| d with A d = r / |r| solved in FP32 (FP32 A in
+//                      VGPRs, FP32 factors in LDS, FP32 vectors and dots),
+//                      then r = b - A x in FP64 (A re-read from global memory,
+//                      FP64 dot), until |r| < tol.
+
+constexpr int FS_ILU = 0, FS_PARILU = 1;
+
+// Factor residual of the iterate W for the entries with work: for lower
+// (i, j): a_ij - sum_k l_ik u_kj - l_ij u_jj, for upper: a_ij - sum - u_ij.
+// Computed inside the Jacobi sweep (the sum is the sweep's own), so it costs
+// a multiply and a subtract per entry.  Returns the lane's partial sum of
+// squares.
+__device__ __forceinline__ double parilu_sweep_res(const ushort4 *__restrict__ etab, int ne,
+                                                   const ushort2 *__restrict__ dep,
+                                                   const double *__restrict__ Ap, double *F,
+                                                   double *W, int nnz, int lane)
+{
+    for (int e = lane; e < nnz; e += WAVE) W[e] = F[e];
+    wave_sync();
+    double rr = 0.0;
+    for (int idx = lane; idx < ne; idx += WAVE) {
+        const ushort4 r = etab[idx];
+        const int pos = r.x, db = r.y, de = r.z, pv = r.w;
+        const double a = Ap[pos];
+        const double dv = W[pv == 0xFFFF ? pos : pv];
+        double sum = 0.0;
+        for (int d0 = db; d0 < de; d0 += 8) {
+            double lv[8], uv[8];
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                const ushort2 pq = dep[d0 + k < de ? d0 + k : de - 1];
+                lv[k] = W[pq.x];
+                uv[k] = W[pq.y];
+            }
+#pragma unroll
+            for (int k = 0; k < 8; k++)
+                if (d0 + k < de) sum = fma(lv[k], uv[k], sum);
+        }
+        const double t = a - sum;
+        const double res = (pv != 0xFFFF) ? t - W[pos] * dv : t - W[pos];
+        rr = fma(res, res, rr);
+        F[pos] = (pv != 0xFFFF) ? t / dv : t;
+    }
+    wave_sync();
+    return rr;
+}
+
+template <typename TO>
+__global__ __launch_bounds__(KF_MAX_WAVES * WAVE) void k_factor_fast(
+    FastParams fp, int n, int nnz, int batch, const double *__restrict__ gA,
+    const double *__restrict__ gFinit, TO *__restrict__ gF, int mode, int sweeps, double stop_tol,
+    int *__restrict__ g_status)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    double *F = reinterpret_cast<double *>(smem + (size_t)wave * fp.wave_bytes);
+    double *W = F + (nnz + 2);
+    for (;;) {
+        int sys = 0;
+        if (lane == 0) sys = atomicAdd(fp.counter, 1);
+        sys = __builtin_amdgcn_readfirstlane(sys);
+        if (sys >= batch) break;
+        const double *Ap = gA + (size_t)sys * nnz;
+        int status = 0;
+        for (int e = lane; e < nnz; e += WAVE) F[e] = Ap[e];
+        if (mode == FS_PARILU) {
+            if (gFinit) {
+                // entries with work start from the guess; the others are exact (= a)
+                const double *Ip = gFinit + (size_t)sys * nnz;
+                for (int idx = lane; idx < fp.ne; idx += WAVE) {
+                    const int pos = fp.etab[idx].x;
+                    F[pos] = Ip[pos];
+                }
+            }
+            wave_sync();
+            double anorm2 = 0.0;
+            if (stop_tol > 0.0) {
+                for (int e = lane; e < nnz; e += WAVE) anorm2 = fma(Ap[e], Ap[e], anorm2);
+                anorm2 = wave_sum(anorm2);
+            }
+            const double lim2 = stop_tol * stop_tol * anorm2;
+            int used = 0;
+            bool met = stop_tol <= 0.0;
+            while (used < sweeps) {
+                const double rr = wave_sum(parilu_sweep_res(fp.etab, fp.ne, fp.dep, Ap, F, W, nnz, lane));
+                used++;
+                if (stop_tol > 0.0 && rr <= lim2) {
+                    // the iterate this sweep started from meets the tolerance: keep it
+                    for (int e = lane; e < nnz; e += WAVE) F[e] = W[e];
+                    wave_sync();
+                    used--;
+                    met = true;
+                    break;
+                }
+            }
+            if (!met) {
+                // fallback: exact ILU(0)
+                for (int e = lane; e < nnz; e += WAVE) F[e] = Ap[e];
+                wave_sync();
+                ilu_pull(fp.etab, fp.eptr, fp.nlev_e, fp.dep, n, fp.diag, F, lane);
+                status = -1;
+            } else {
+                status = used;
+            }
+        } else {
+            wave_sync();
+            ilu_pull(fp.etab, fp.eptr, fp.nlev_e, fp.dep, n, fp.diag, F, lane);
+        }
+        TO *Fo = gF + (size_t)sys * nnz;
+        for (int e = lane; e < nnz; e += WAVE) Fo[e] = (TO)F[e];
+        if (g_status && lane == 0) g_status[sys] = status;
+        wave_sync();
+    }
+}
+
+struct MixedParams {
+    double tol;          // outer tolerance (relative or absolute, tol_mode)
+    int tol_mode;
+    float inner_tol;     // inner relative tolerance (FP32 BiCGSTAB)
+    int max_iter;        // inner iterations per refinement step
+    int max_refine;      // refinement steps
+};
+
+// y (LDS, FP64, by row) = A x with the FP64 values read from global memory
+// slot by slot; x in LDS (FP64).  Same fma chain as spmv_slots.
+template <int SM>
+__device__ __forceinline__ void spmv_slots_global(const double *__restrict__ Ap,
+                                                  const int32_t *__restrict__ slot_pos, int S,
+                                                  const uint32_t *__restrict__ meta,
+                                                  const double *__restrict__ x, double *__restrict__ y,
+                                                  double *__restrict__ scratch, int lane)
+{
+    // groups of G slots: the loads of one group are in flight together, the
+    // compiler barrier keeps it from hoisting every group's loads (VGPRs)
+    constexpr int G = 12;
+    const char *xb = reinterpret_cast<const char *>(x);
+    char *yb = reinterpret_cast<char *>(y);
+    double acc = 0.0;
+#pragma unroll
+    for (int g = 0; g < SM; g += G) {
+        double av[G];
+#pragma unroll
+        for (int j = 0; j < G; j++) {
+            const int s = g + j;
+            const int e = (s < S && s < SM) ? slot_pos[s * WAVE + lane] : -1;
+            av[j] = (e >= 0) ? Ap[e] : 0.0;
+        }
+#pragma unroll
+        for (int j = 0; j < G; j++) {
+            const int s = g + j;
+            if (s >= SM) break;
+            const uint32_t m = meta[s * WAVE + lane];
+            acc = fma(av[j], *reinterpret_cast<const double *>(xb + (m & 0xFFFFu)), acc);
+            const bool end = (m >> 16) != 0xFFFFu;
+            double *dst = end ? reinterpret_cast<double *>(yb + (m >> 16)) : scratch;
+            *dst = acc;
+            acc = end ? 0.0 : acc;
+        }
+        asm volatile("" ::: "memory");
+    }
+}
+
+#ifndef KM_MAX_WAVES
+#define KM_MAX_WAVES 12
+#endif
+
+template <int NR, int SM>
+__global__ __launch_bounds__(KM_MAX_WAVES * WAVE) void k_mixed(
+    DevPattern P, FastParams fp, int batch, const double *__restrict__ gA,
+    const double *__restrict__ gb, double *__restrict__ gx, const float *__restrict__ gF,
+    MixedParams mp, float *__restrict__ g_iter, int *__restrict__ g_conv,
+    double *__restrict__ g_ires, double *__restrict__ g_tres, int *__restrict__ g_refine)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    const int n = P.n, nnz = P.nnz, S = fp.S;
+    uint32_t *meta = reinterpret_cast<uint32_t *>(smem);
+    const uint32_t *trL = fp.trL;
+    const uint32_t *trU = fp.trU;
+    for (int i = threadIdx.x; i < SM * WAVE; i += blockDim.x) meta[i] = (i < S * WAVE) ? fp.slot_meta[i] : 0xFFFF0000u;
+    // per wave: F[nnz + 2] FP32 (F[nnz] = 0), then two vector regions of
+    // (n + 2) doubles each, used as FP32 vectors by the inner solve (zero
+    // cell at [n], scratch at [n + 1]) and as FP64 vectors by the outer
+    // residual, then the FP64 row accumulators of the solves acc[64 NR]
+    unsigned char *wb = smem + fp.shared_bytes + (size_t)wave * fp.wave_bytes;
+    float *F = reinterpret_cast<float *>(wb);
+    double *d0 = reinterpret_cast<double *>(wb + (((size_t)(nnz + 2) * 4 + 15) / 16) * 16);
+    double *d1 = d0 + (n + 2);
+    float *e0 = reinterpret_cast<float *>(d0), *e1 = reinterpret_cast<float *>(d1);
+    float *scratch = e1 + (n + 1);
+    double *dscratch = d1 + (n + 1);
+    double *accb = d1 + (n + 2);
+    if (lane == 0) F[nnz] = 0.0f;
+    __syncthreads();
+    int rlev[NR];
+#pragma unroll
+    for (int k = 0; k < NR; k++)
+        rlev[k] = (int)fp.rlevL[k * WAVE + lane] | ((int)fp.rlevU[k * WAVE + lane] << 16);
+    const int vcntL = fp.cntL[lane], vcntU = fp.cntU[lane];
+
+    float a[SM];
+
+    for (;;) {
+        int sys = 0;
+        if (lane == 0) sys = atomicAdd(fp.counter, 1);
+        sys = __builtin_amdgcn_readfirstlane(sys);
+        if (sys >= batch) break;
+
+        const double *Ap = gA + (size_t)sys * nnz;
+        const double *bp = gb + (size_t)sys * n;
+        const float *Fp = gF + (size_t)sys * nnz;
+        for (int e = lane; e < nnz; e += WAVE) F[e] = Fp[e];
+#pragma unroll
+        for (int q = 0; q < SM; q++) {
+            const int e = (q < S) ? fp.slot_pos[q * WAVE + lane] : -1;
+            a[q] = (e >= 0) ? (float)Ap[e] : 0.0f;
+        }
+        wave_sync();
+        // zero cells of the FP32 views (the FP64 residual overwrites them)
+        auto zero_cells = [&]() {
+            if (lane == 0) { e0[n] = 0.0f; e1[n] = 0.0f; }
+            wave_sync();
+        };
+        zero_cells();
+        int bad = 0;
+        Vec<NR, float> rinv;
+#pragma unroll
+        for (int k = 0; k < NR; k++) {
+            const int i = lane + WAVE * k;
+            const float u = (i < n) ? F[fp.diag[i]] : 1.0f;
+            bad |= (u == 0.0f);
+            rinv.e[k] = 1.0f / u;
+        }
+        bad = __any(bad);
+
+        Vec<NR, double> x, r;
+#pragma unroll
+        for (int k = 0; k < NR; k++) {
+            const int i = lane + WAVE * k;
+            x.e[k] = 0.0;
+            r.e[k] = (i < n) ? bp[i] : 0.0;
+        }
+        const double bnorm = sqrt(vdot(r, r, lane, n));
+        const double tol_abs = (mp.tol_mode == BILU_TOL_RELATIVE) ? mp.tol * bnorm : mp.tol;
+        double rn = bnorm;
+        float iter = 0.0f;
+        int steps = 0;
+        int conv = BILU_NOT_CONVERGED;
+
+        auto apply = [&](const Vec<NR, float> &in, Vec<NR, float> &out) {
+            Vec<NR, float> y;
+            trsv_avail<NR, true, float>(trL, vcntL, fp.nlev_l, rlev, F, in, rinv, e1, accb, scratch, y, lane);
+            trsv_avail<NR, false, float>(trU, vcntU, fp.nlev_u, rlev, F, y, rinv, e0, accb, scratch, out, lane);
+            wave_sync();
+        };
+        auto matvec = [&](Vec<NR, float> &out) {   // out = A32 e0
+            spmv_slots<SM, float>(a, meta, e0, e1, scratch, lane);
+            wave_sync();
+            from_lds(out, e1, lane, n);
+            wave_sync();
+        };
+
+        if (bad) {
+            conv = BILU_ZERO_PIVOT;
+        } else if (!isfinite(bnorm)) {
+            conv = BILU_NONFINITE;
+        } else if (bnorm == 0.0) {
+            conv = BILU_CONVERGED;
+            rn = 0.0;
+        } else if (rn < tol_abs) {
+            conv = BILU_CONVERGED;
+        } else {
+            while (steps < mp.max_refine) {
+                // inner FP32 BiCGSTAB on A d = r / |r|, d0 = 0
+                const double sc = 1.0 / rn;
+                Vec<NR, float> d, ri, rh, p, v, s, t, ph, sh;
+#pragma unroll
+                for (int k = 0; k < NR; k++) {
+                    ri.e[k] = (float)(r.e[k] * sc);
+                    rh.e[k] = ri.e[k];
+                    d.e[k] = 0.0f; p.e[k] = 0.0f; v.e[k] = 0.0f;
+                }
+                const float r0 = sqrtf(vdot(ri, ri, lane, n));
+                const float itol = mp.inner_tol * r0;
+                float ires = r0;
+                float it = 0.0f;
+                float rho_old = 1.0f, omega_old = 1.0f, alpha = 1.0f;
+                bool inner_bad = false;
+                while (it < (float)mp.max_iter && ires >= itol) {
+                    const float rho_new = vdot(ri, rh, lane, n);
+                    if (rho_new == 0.0f) break;
+                    const float beta = (rho_new / rho_old) * (alpha / omega_old);
+#pragma unroll
+                    for (int k = 0; k < NR; k++) p.e[k] = fmaT(beta, fmaT(-omega_old, v.e[k], p.e[k]), ri.e[k]);
+                    apply(p, ph);
+                    matvec(v);
+                    alpha = rho_new / vdot(rh, v, lane, n);
+#pragma unroll
+                    for (int k = 0; k < NR; k++) d.e[k] = fmaT(alpha, ph.e[k], d.e[k]);
+#pragma unroll
+                    for (int k = 0; k < NR; k++) s.e[k] = fmaT(-alpha, v.e[k], ri.e[k]);
+                    ires = sqrtf(vdot(s, s, lane, n));
+                    it += 0.5f;
+                    if (!isfinite(ires)) { inner_bad = true; break; }
+                    if (ires < itol) break;
+                    apply(s, sh);
+                    matvec(t);
+                    const float ts = vdot(t, s, lane, n);
+                    const float tt = vdot(t, t, lane, n);
+                    const float omega_new = ts / tt;
+#pragma unroll
+                    for (int k = 0; k < NR; k++) d.e[k] = fmaT(omega_new, sh.e[k], d.e[k]);
+                    it += 0.5f;
+#pragma unroll
+                    for (int k = 0; k < NR; k++) ri.e[k] = fmaT(-omega_new, t.e[k], s.e[k]);
+                    ires = sqrtf(vdot(ri, ri, lane, n));
+                    rho_old = rho_new;
+                    omega_old = omega_new;
+                    if (!isfinite(ires)) { inner_bad = true; break; }
+                }
+                iter += it;
+                steps++;
+                if (inner_bad) { conv = BILU_NONFINITE; break; }
+                // x += |r| d; r = b - A x in FP64
+#pragma unroll
+                for (int k = 0; k < NR; k++) x.e[k] = fma((double)d.e[k], rn, x.e[k]);
+                to_lds(x, d0, lane, n);
+                wave_sync();
+                spmv_slots_global<SM>(Ap, fp.slot_pos, S, meta, d0, d1, dscratch, lane);
+                wave_sync();
+                Vec<NR, double> w;
+                from_lds(w, d1, lane, n);
+#pragma unroll
+                for (int k = 0; k < NR; k++) {
+                    const int i = lane + WAVE * k;
+                    r.e[k] = ((i < n) ? bp[i] : 0.0) - w.e[k];
+                }
+                wave_sync();
+                zero_cells();
+                rn = sqrt(vdot(r, r, lane, n));
+                if (!isfinite(rn)) { conv = BILU_NONFINITE; break; }
+                if (rn < tol_abs) { conv = BILU_CONVERGED; break; }
+                if (it == 0.0f) break;   // the inner solve could not move (rho = 0)
+            }
+        }
+        if (conv == BILU_ZERO_PIVOT) {
+#pragma unroll
+            for (int k = 0; k < NR; k++) x.e[k] = 0.0;
+            rn = bnorm;
+        }
+        double *xp = gx + (size_t)sys * n;
+#pragma unroll
+        for (int k = 0; k < NR; k++) {
+            const int i = lane + WAVE * k;
+            if (i < n) xp[i] = x.e[k];
+        }
+        if (lane == 0) {
+            g_iter[sys] = iter;
+            g_conv[sys] = conv;
+            if (g_ires) g_ires[sys] = (conv == BILU_ZERO_PIVOT) ? 0.0 : rn;
+            if (g_tres) g_tres[sys] = rn;
+            if (g_refine) g_refine[sys] = steps;
+        }
+        wave_sync();
+    }
+}

@@ -112,34 +112,70 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
-template <int NR>
+// FP32 versions of the lane exchanges and the wave sum (same association)
+template <int CTRL>
+__device__ __forceinline__ float dpp_mov(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+
+__device__ __forceinline__ float lane_xor32(float v)
+{
+    const unsigned u = __float_as_uint(v);
+    const auto a = __builtin_amdgcn_permlane32_swap(u, u, false, false);
+    return __uint_as_float((__lane_id() & 32) == 0 ? a[1] : a[0]);
+}
+
+__device__ __forceinline__ float lane_xor16(float v)
+{
+    const unsigned u = __float_as_uint(v);
+    const auto a = __builtin_amdgcn_permlane16_swap(u, u, false, false);
+    return __uint_as_float((__lane_id() & 16) == 0 ? a[1] : a[0]);
+}
+
+__device__ __forceinline__ float wave_sum(float v)
+{
+    v = v + lane_xor32(v);
+    v = v + lane_xor16(v);
+    v = v + dpp_mov<0x128>(v);
+    v = v + dpp_mov<0x124>(v);
+    v = v + dpp_mov<0x4E>(v);
+    v = v + dpp_mov<0xB1>(v);
+    return v;
+}
+
+__device__ __forceinline__ double fmaT(double a, double b, double c) { return fma(a, b, c); }
+__device__ __forceinline__ float fmaT(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
+// a lane's NR rows (lane + 64 k) of a vector, in registers
+template <int NR, typename T = double>
 struct Vec {
-    double e[NR];
+    T e[NR];
 };
 
-template <int NR>
-__device__ __forceinline__ double vdot(const Vec<NR> &a, const Vec<NR> &b, int lane, int n)
+template <int NR, typename T>
+__device__ __forceinline__ T vdot(const Vec<NR, T> &a, const Vec<NR, T> &b, int lane, int n)
 {
-    double s = 0.0;
+    T s = T(0);
 #pragma unroll
     for (int k = 0; k < NR; k++)
-        if (lane + WAVE * k < n) s = fma(a.e[k], b.e[k], s);
+        if (lane + WAVE * k < n) s = fmaT(a.e[k], b.e[k], s);
     return wave_sum(s);
 }
 
-template <int NR>
-__device__ __forceinline__ void to_lds(const Vec<NR> &v, double *dst, int lane, int n)
+template <int NR, typename T>
+__device__ __forceinline__ void to_lds(const Vec<NR, T> &v, T *dst, int lane, int n)
 {
 #pragma unroll
     for (int k = 0; k < NR; k++)
         if (lane + WAVE * k < n) dst[lane + WAVE * k] = v.e[k];
 }
 
-template <int NR>
-__device__ __forceinline__ void from_lds(Vec<NR> &v, const double *src, int lane, int n)
+template <int NR, typename T>
+__device__ __forceinline__ void from_lds(Vec<NR, T> &v, const T *src, int lane, int n)
 {
 #pragma unroll
-    for (int k = 0; k < NR; k++) v.e[k] = (lane + WAVE * k < n) ? src[lane + WAVE * k] : 0.0;
+    for (int k = 0; k < NR; k++) v.e[k] = (lane + WAVE * k < n) ? src[lane + WAVE * k] : T(0);
 }
 
 // y = A x, rows in the register layout; A over positions [rp[i], rp[i+1])
@@ -697,6 +733,7 @@ __global__ __launch_bounds__(WAVE) void k_true_residual(DevPattern P, int batch,
 }
 
 #include "fast_kernels.hpp"
+#include "mixed_kernels.hpp"
 
 }  // namespace
 
@@ -716,7 +753,10 @@ struct bilu_pattern_s {
     uint16_t *d_cntL = nullptr, *d_cntU = nullptr;
     uint16_t *d_rlevL = nullptr, *d_rlevU = nullptr;
     uint16_t *d_etab = nullptr, *d_eptr = nullptr, *d_dep = nullptr;
-    int *d_counter = nullptr;
+    int *d_counter = nullptr;      // 256 bytes of work counters: [0] solve kernels, [32] factor kernel
+    // solve workspace (factors between the factor and solve launches), grown on demand
+    void *ws = nullptr;
+    size_t ws_bytes = 0;
 };
 
 namespace {
@@ -773,8 +813,8 @@ int upload(bilu_pattern_s &h)
         d.isai[t].beg = B + o_is[t][0]; d.isai[t].m = B + o_is[t][1]; d.isai[t].goff = B + o_is[t][2];
         d.isai[t].gather = B + o_is[t][3]; d.isai[t].rhs = B + o_is[t][4];
     }
-    HIP_TRY(hipMalloc((void **)&h.d_counter, 64));
-    HIP_TRY(hipMemset(h.d_counter, 0, 64));
+    HIP_TRY(hipMalloc((void **)&h.d_counter, 256));
+    HIP_TRY(hipMemset(h.d_counter, 0, 256));
     const auto &fp = pa.fast;
     if (fp.ok) {
         std::vector<uint8_t> blob;
@@ -812,6 +852,7 @@ void free_handle(bilu_pattern_s *h)
     if (!h) return;
     if (h->dbuf) (void)hipFree(h->dbuf);
     if (h->d_counter) (void)hipFree(h->d_counter);
+    if (h->ws) (void)hipFree(h->ws);
     if (h->d_fast) (void)hipFree(h->d_fast);
     delete h;
 }
@@ -880,6 +921,130 @@ int launch_fast(bilu_pattern_s &h, int batch, const double *A, const double *b, 
     return -1;
 }
 
+__global__ void k_fill_int(int *p, int v, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// table pointers of the persistent kernels (no per-launch sizing)
+void fast_tables(const bilu_pattern_s &h, FastParams &fp)
+{
+    const auto &pl = h.pa.fast;
+    fp.trL = h.d_trL; fp.trU = h.d_trU;
+    fp.trL_words = int(pl.trL.size()); fp.trU_words = int(pl.trU.size());
+    fp.cntL = h.d_cntL; fp.cntU = h.d_cntU;
+    fp.rlevL = h.d_rlevL; fp.rlevU = h.d_rlevU;
+    fp.etab = reinterpret_cast<const ushort4 *>(h.d_etab); fp.eptr = h.d_eptr;
+    fp.dep = reinterpret_cast<const ushort2 *>(h.d_dep);
+    fp.diag = h.dev.diag;
+    fp.nlev_l = h.dev.nlev_l; fp.nlev_u = h.dev.nlev_u; fp.nlev_e = pl.nlev_e; fp.ne = pl.ne;
+    fp.S = pl.S; fp.slot_meta = h.d_slot_meta; fp.slot_pos = h.d_slot_pos;
+    fp.counter = h.d_counter;
+}
+
+int slot_template(int S)
+{
+    return S <= 24 ? 24 : S <= 36 ? 36 : S <= 48 ? 48 : -1;
+}
+
+// grid of a persistent kernel: every CU filled with as many workgroups as
+// its LDS holds, never more waves than systems
+int persistent_grid(const bilu_pattern_s &h, size_t lds, int waves, int batch)
+{
+    const int per_cu = std::max<int>(1, (int)(h.lds_per_cu / lds));
+    return std::max(1, std::min(h.num_cus * per_cu, (batch + waves - 1) / waves));
+}
+
+// workspace of at least `bytes` (grown by reallocation; the handle is used by
+// one host thread at a time, so growing waits for the device first)
+int workspace(bilu_pattern_s &h, size_t bytes, void **out)
+{
+    if (h.ws_bytes < bytes) {
+        if (h.ws) {
+            HIP_TRY(hipDeviceSynchronize());
+            HIP_TRY(hipFree(h.ws));
+            h.ws = nullptr;
+            h.ws_bytes = 0;
+        }
+        HIP_TRY(hipMalloc(&h.ws, bytes));
+        h.ws_bytes = bytes;
+    }
+    *out = h.ws;
+    return BILU_OK;
+}
+
+// persistent factorization: exact ILU(0) or ParILU(0) sweeps (optionally
+// from F_init, optionally stopped on the factor residual) -> F (TO)
+template <typename TO>
+int launch_factor_fast(bilu_pattern_s &h, int batch, const double *A, const double *F_init, TO *F,
+                       int mode, int sweeps, double stop_tol, int *status, hipStream_t st)
+{
+    const auto &pl = h.pa.fast;
+    if (!pl.ok) return BILU_ERR_TOO_LARGE;
+    const int nnz = h.pa.nnz;
+    FastParams fp{};
+    fast_tables(h, fp);
+    fp.counter = h.d_counter + 32;
+    fp.wave_bytes = int(((size_t)(mode == FS_PARILU ? 2 : 1) * (nnz + 2) * sizeof(double) + 15) / 16 * 16);
+    int waves = (int)std::min<size_t>(KF_MAX_WAVES, h.max_lds / fp.wave_bytes);
+    if (waves < 1) return BILU_ERR_TOO_LARGE;
+    waves = std::min(waves, std::max(1, batch));
+    const size_t lds = (size_t)waves * fp.wave_bytes;
+    const int grid = persistent_grid(h, lds, waves, batch);
+    auto kern = k_factor_fast<TO>;
+    if (lds > 65536) HIP_TRY(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, st, fp.counter);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(waves * WAVE), lds, st, fp, h.pa.n, nnz, batch, A, F_init, F,
+                       mode, sweeps, stop_tol, status);
+    HIP_TRY(hipGetLastError());
+    return BILU_OK;
+}
+
+template <int NR, int SM>
+int launch_mixed_t(bilu_pattern_s &h, const FastParams &fp, int grid, int waves, size_t lds, int batch,
+                   const double *A, const double *b, double *x, const float *F, const MixedParams &mp,
+                   float *it, int *cv, double *ir, double *tr, int *refine, hipStream_t st)
+{
+    auto kern = k_mixed<NR, SM>;
+    if (lds > 65536) HIP_TRY(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, st, fp.counter);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(waves * WAVE), lds, st, h.dev, fp, batch, A, b, x, F, mp, it, cv, ir, tr, refine);
+    HIP_TRY(hipGetLastError());
+    return BILU_OK;
+}
+
+// FP32 inner / FP64 refinement solve over FP32 factors F
+int launch_mixed(bilu_pattern_s &h, int batch, const double *A, const double *b, double *x, const float *F,
+                 const MixedParams &mp, float *it, int *cv, double *ir, double *tr, int *refine, hipStream_t st)
+{
+    const auto &pl = h.pa.fast;
+    const int n = h.pa.n, nnz = h.pa.nnz;
+    const int nr = (n + WAVE - 1) / WAVE;
+    const int sm = slot_template(pl.S);
+    if (!pl.ok || nr > MAX_NR || pl.nr != nr || sm < 0) return BILU_ERR_TOO_LARGE;
+    FastParams fp{};
+    fast_tables(h, fp);
+    // per wave: FP32 factors (+2 cells, 16-aligned), two (n + 2)-double vector regions,
+    // FP64 row accumulators of the solves (64 NR)
+    fp.wave_bytes = int(((size_t)(nnz + 2) * 4 + 15) / 16 * 16 + (size_t)2 * (n + 2) * 8 + (size_t)64 * nr * 8);
+    fp.wave_bytes = (fp.wave_bytes + 15) / 16 * 16;
+    fp.shared_bytes = sm * WAVE * 4;
+    if ((size_t)fp.shared_bytes >= h.max_lds) return BILU_ERR_TOO_LARGE;
+    int waves = (int)std::min<size_t>(KM_MAX_WAVES, (h.max_lds - fp.shared_bytes) / fp.wave_bytes);
+    if (waves < 1) return BILU_ERR_TOO_LARGE;
+    waves = std::min(waves, std::max(1, batch));
+    const size_t lds = fp.shared_bytes + (size_t)waves * fp.wave_bytes;
+    const int grid = persistent_grid(h, lds, waves, batch);
+#define MIXED_CASE(R, M) \
+    if (nr == R && sm == M) return launch_mixed_t<R, M>(h, fp, grid, waves, lds, batch, A, b, x, F, mp, it, cv, ir, tr, refine, st);
+#define MIXED_ROW(R) MIXED_CASE(R, 24) MIXED_CASE(R, 36) MIXED_CASE(R, 48)
+    MIXED_ROW(1) MIXED_ROW(2) MIXED_ROW(3) MIXED_ROW(4) MIXED_ROW(5)
+#undef MIXED_ROW
+#undef MIXED_CASE
+    return BILU_ERR_TOO_LARGE;
+}
+
 template <int NR, int PC>
 int launch_solve_nr(bilu_pattern_s &h, int batch, const double *A, const double *b, double *x,
                     const double *F, const SolveParams &sp, float *it, int *cv, double *ir,
@@ -924,6 +1089,10 @@ void bilu_opts_default(bilu_opts *o)
     o->true_res_restart = 0;
     o->max_restarts = 3;
     o->kernel = BILU_KERNEL_AUTO;
+    o->precision = BILU_PRECISION_FP64;
+    o->inner_tol = 1e-5;
+    o->max_refine = 10;
+    o->parilu_stop_tol = 0.0;
 }
 
 const char *bilu_version(void) { return "bilu 0.1 gfx950"; }
@@ -1005,18 +1174,10 @@ int bilu_pattern_lu_split(bilu_pattern_t p, int *lrp, int *lci, int *urp, int *u
     return BILU_OK;
 }
 
-int bilu_solve(bilu_pattern_t h, int batch, const double *d_A, const double *d_b, double *d_x,
-               const bilu_opts *opts, float *d_iter, int *d_conv, double *d_iter_res,
-               double *d_true_res, void *stream)
+static int solve_fp64(bilu_pattern_t h, int batch, const double *d_A, const double *d_b, double *d_x,
+                      const bilu_opts &o, float *d_iter, int *d_conv, double *d_iter_res,
+                      double *d_true_res, void *stream)
 {
-    if (!h || batch < 0 || (batch > 0 && (!d_A || !d_b || !d_x || !d_iter || !d_conv)))
-        return BILU_ERR_INVALID_ARG;
-    if (batch == 0) return BILU_OK;
-    bilu_opts o;
-    if (opts) o = *opts; else bilu_opts_default(&o);
-    if (o.precond < BILU_PRECOND_ILU0 || o.precond > BILU_PRECOND_NONE || o.max_iter < 0 ||
-        o.parilu_sweeps < 0 || !(o.tol >= 0.0))
-        return BILU_ERR_INVALID_ARG;
     const auto &pa = h->pa;
     const int n = pa.n, nnz = pa.nnz;
     if ((n + WAVE - 1) / WAVE > MAX_NR) return BILU_ERR_TOO_LARGE;
@@ -1069,6 +1230,86 @@ int bilu_solve(bilu_pattern_t h, int batch, const double *d_A, const double *d_b
     else rc = launch_solve<PC_NONE>(*h, batch, d_A, d_b, d_x, F, sp, d_iter, d_conv, d_iter_res, d_true_res, lds, st);
     if (Fbuf) HIP_TRY(hipFreeAsync(Fbuf, st));
     return rc;
+}
+
+int bilu_solve_ex(bilu_pattern_t h, int batch, const double *d_A, const double *d_b, double *d_x,
+                  const bilu_opts *opts, const double *d_F_init, float *d_iter, int *d_conv,
+                  double *d_iter_res, double *d_true_res, int *d_refine, int *d_factor_status,
+                  void *stream)
+{
+    if (!h || batch < 0 || (batch > 0 && (!d_A || !d_b || !d_x || !d_iter || !d_conv)))
+        return BILU_ERR_INVALID_ARG;
+    if (batch == 0) return BILU_OK;
+    bilu_opts o;
+    if (opts) o = *opts; else bilu_opts_default(&o);
+    if (o.precond < BILU_PRECOND_ILU0 || o.precond > BILU_PRECOND_NONE || o.max_iter < 0 ||
+        o.parilu_sweeps < 0 || !(o.tol >= 0.0) ||
+        (o.precision != BILU_PRECISION_FP64 && o.precision != BILU_PRECISION_FP32_IR) ||
+        !(o.parilu_stop_tol >= 0.0))
+        return BILU_ERR_INVALID_ARG;
+    if (o.precision == BILU_PRECISION_FP32_IR && (!(o.inner_tol > 0.0 && o.inner_tol < 1.0) || o.max_refine < 1))
+        return BILU_ERR_INVALID_ARG;
+    hipStream_t st = (hipStream_t)stream;
+    HIP_TRY(hipSetDevice(h->device));
+    const bool lu = o.precond == BILU_PRECOND_ILU0 || o.precond == BILU_PRECOND_PARILU0;
+    const bool warm = o.precond == BILU_PRECOND_PARILU0 && (d_F_init || o.parilu_stop_tol > 0.0);
+    const bool split = o.precision == BILU_PRECISION_FP32_IR || warm || d_factor_status;
+    if (!split) {
+        const int rc = solve_fp64(h, batch, d_A, d_b, d_x, o, d_iter, d_conv, d_iter_res, d_true_res, stream);
+        if (rc == BILU_OK && d_refine) {
+            hipLaunchKernelGGL(k_fill_int, dim3((batch + 255) / 256), dim3(256), 0, st, d_refine, 0, batch);
+            HIP_TRY(hipGetLastError());
+        }
+        return rc;
+    }
+    // factor launch (persistent), factors through the handle's workspace, then the solve
+    if (!lu) return BILU_ERR_INVALID_ARG;   // these paths serve the ILU(0) / ParILU(0) preconditioners
+    if (!h->pa.fast.ok) return BILU_ERR_TOO_LARGE;
+    const int nnz = h->pa.nnz;
+    const int mode = o.precond == BILU_PRECOND_ILU0 ? FS_ILU : FS_PARILU;
+    const double stop = mode == FS_PARILU ? o.parilu_stop_tol : 0.0;
+    void *ws = nullptr;
+    int rc;
+    if (o.precision == BILU_PRECISION_FP32_IR) {
+        const int nr = (h->pa.n + WAVE - 1) / WAVE;
+        if (slot_template(h->pa.fast.S) < 0 || nr > MAX_NR || h->pa.fast.nr != nr) return BILU_ERR_TOO_LARGE;
+        rc = workspace(*h, (size_t)batch * nnz * sizeof(float), &ws);
+        if (rc != BILU_OK) return rc;
+        float *F = static_cast<float *>(ws);
+        rc = launch_factor_fast<float>(*h, batch, d_A, mode == FS_PARILU ? d_F_init : nullptr, F, mode,
+                                       o.parilu_sweeps, stop, d_factor_status, st);
+        if (rc != BILU_OK) return rc;
+        MixedParams mp{o.tol, o.tol_mode, (float)o.inner_tol, o.max_iter, o.max_refine};
+        return launch_mixed(*h, batch, d_A, d_b, d_x, F, mp, d_iter, d_conv, d_iter_res, d_true_res, d_refine, st);
+    }
+    rc = workspace(*h, (size_t)batch * nnz * sizeof(double), &ws);
+    if (rc != BILU_OK) return rc;
+    double *F = static_cast<double *>(ws);
+    rc = launch_factor_fast<double>(*h, batch, d_A, mode == FS_PARILU ? d_F_init : nullptr, F, mode,
+                                    o.parilu_sweeps, stop, d_factor_status, st);
+    if (rc != BILU_OK) return rc;
+    SolveParams sp{o.tol, o.tol_mode, o.max_iter, o.true_res_restart, o.max_restarts, FK_LOAD, 0};
+    rc = -1;
+    if (o.kernel != BILU_KERNEL_WAVE) rc = launch_fast(*h, batch, d_A, d_b, d_x, F, sp, d_iter, d_conv, d_iter_res, d_true_res, st);
+    if (rc == -1) {
+        if (o.kernel == BILU_KERNEL_PERSISTENT) return BILU_ERR_TOO_LARGE;
+        const size_t lds = sizeof(double) * ((size_t)nnz * 2 + (size_t)h->pa.n * 3);
+        if (lds > h->max_lds) return BILU_ERR_TOO_LARGE;
+        rc = launch_solve<PC_LU>(*h, batch, d_A, d_b, d_x, F, sp, d_iter, d_conv, d_iter_res, d_true_res, lds, st);
+    }
+    if (rc == BILU_OK && d_refine) {
+        hipLaunchKernelGGL(k_fill_int, dim3((batch + 255) / 256), dim3(256), 0, st, d_refine, 0, batch);
+        HIP_TRY(hipGetLastError());
+    }
+    return rc;
+}
+
+int bilu_solve(bilu_pattern_t h, int batch, const double *d_A, const double *d_b, double *d_x,
+               const bilu_opts *opts, float *d_iter, int *d_conv, double *d_iter_res,
+               double *d_true_res, void *stream)
+{
+    return bilu_solve_ex(h, batch, d_A, d_b, d_x, opts, nullptr, d_iter, d_conv, d_iter_res, d_true_res,
+                         nullptr, nullptr, stream);
 }
 
 int bilu_ilu0(bilu_pattern_t h, int batch, const double *d_A, double *d_F, int *d_status, void *stream)

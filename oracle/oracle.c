@@ -561,3 +561,341 @@ int orc_batch_solve(int n, int nnz, const int *rp, const int *ci, int batch,
     free(th); free(jobs); free(diag);
     return 0;
 }
+
+/* ---------------------------------------------------------------------
+ * Mixed precision (FP32 inner BiCGSTAB inside FP64 iterative refinement)
+ * and ParILU(0) with a warm start and a residual stop.  These paths have no
+ * counterpart in the reference; they restate the build's own persistent
+ * kernels operation for operation, on the reference's building blocks
+ * (BiCGSTAB control flow of merged.cu:296-524, ParILU formula of
+ * parILU_0_gpu.cu:18-57).
+ * ------------------------------------------------------------------- */
+
+/* FP32 dot in the wave64 order (lane-strided fmaf partial sums, then the
+ * pairwise tree with offsets 32..1) */
+static float dotf64w(int n, const float *a, const float *b)
+{
+    float t[64];
+    for (int l = 0; l < 64; l++) {
+        float s = 0.0f;
+        for (int idx = l; idx < n; idx += 64) s = fmaf(a[idx], b[idx], s);
+        t[l] = s;
+    }
+    for (int off = 32; off > 0; off /= 2)
+        for (int l = 0; l < off; l++) t[l] = t[l] + t[l + off];
+    return t[0];
+}
+
+/* FP32 SpMV, each row a left-to-right fmaf chain */
+static void spmvf(int n, const int *rp, const int *ci, const float *vals, const float *x, float *y)
+{
+    for (int i = 0; i < n; i++) {
+        float s = 0.0f;
+        for (int p = rp[i]; p < rp[i + 1]; p++) s = fmaf(vals[p], x[ci[p]], s);
+        y[i] = s;
+    }
+}
+
+/* FP32 twin of orc_lu_solve_avail (availability order, reciprocal of
+ * u_ii): each product is formed in FP32 and added into an FP64 row sum,
+ * which is rounded to FP32 once before the row is finalised (the kernel's
+ * accumulators are FP64 LDS cells). */
+static void lu_solve_avail_f(int n, const int *rp, const int *ci, const int *diag,
+                             const int *lev_l, const int *lev_u, const float *F,
+                             const float *v, float *tmp, float *out)
+{
+    int pairs[2 * 1024];
+    for (int i = 0; i < n; i++) {
+        int m = 0;
+        for (int p = rp[i]; p < diag[i] && m < 1024; p++, m++) { pairs[2 * m] = ci[p]; pairs[2 * m + 1] = p; }
+        sort_avail(pairs, m, lev_l);
+        double sum = 0.0;
+        for (int q = 0; q < m; q++) sum = sum + (double)(F[pairs[2 * q + 1]] * tmp[pairs[2 * q]]);
+        tmp[i] = v[i] - (float)sum;
+    }
+    for (int i = n - 1; i >= 0; i--) {
+        int m = 0;
+        for (int p = diag[i] + 1; p < rp[i + 1] && m < 1024; p++, m++) { pairs[2 * m] = ci[p]; pairs[2 * m + 1] = p; }
+        sort_avail(pairs, m, lev_u);
+        double sum = 0.0;
+        for (int q = 0; q < m; q++) sum = sum + (double)(F[pairs[2 * q + 1]] * out[pairs[2 * q]]);
+        out[i] = (tmp[i] - (float)sum) * (1.0f / F[diag[i]]);
+    }
+}
+
+int orc_bicgstab_mixed(int n, const int *rp, const int *ci, const int *diag,
+                       const int *lev_l, const int *lev_u, const double *A, const double *b,
+                       const float *F32, const orc_opts *o, double inner_tol, int max_refine,
+                       double *x, float *iter_out, int *conv_out, double *iter_res_out,
+                       double *true_res_out, int *refine_out)
+{
+    const int nnz = rp[n];
+    float *Af = (float *)malloc(sizeof(float) * (size_t)nnz);
+    float *fb = (float *)malloc(sizeof(float) * (size_t)n * 10);
+    double *r = (double *)malloc(sizeof(double) * (size_t)n * 2), *w = r + n;
+    float *ri = fb, *rh = fb + n, *p = fb + 2 * n, *v = fb + 3 * n, *s = fb + 4 * n;
+    float *t = fb + 5 * n, *ph = fb + 6 * n, *sh = fb + 7 * n, *d = fb + 8 * n, *tmp = fb + 9 * n;
+    for (int q = 0; q < nnz; q++) Af[q] = (float)A[q];
+    int bad = 0;
+    for (int i = 0; i < n; i++) bad |= (F32[diag[i]] == 0.0f);
+    for (int i = 0; i < n; i++) { x[i] = 0.0; r[i] = b[i]; }
+    const double bnorm = sqrt(orc_dot(n, r, r, 64));
+    const double tol_abs = (o->tol_mode == ORC_TOL_RELATIVE) ? o->tol * bnorm : o->tol;
+    double rn = bnorm;
+    float iter = 0.0f;
+    int steps = 0, conv = ORC_NOT_CONVERGED;
+    if (bad) {
+        conv = ORC_ZERO_PIVOT;
+    } else if (!isfinite(bnorm)) {
+        conv = ORC_NONFINITE;
+    } else if (bnorm == 0.0) {
+        conv = ORC_CONVERGED;
+        rn = 0.0;
+    } else if (rn < tol_abs) {
+        conv = ORC_CONVERGED;
+    } else {
+        while (steps < max_refine) {
+            const double sc = 1.0 / rn;
+            for (int i = 0; i < n; i++) {
+                ri[i] = (float)(r[i] * sc);
+                rh[i] = ri[i];
+                d[i] = 0.0f; p[i] = 0.0f; v[i] = 0.0f;
+            }
+            const float r0 = sqrtf(dotf64w(n, ri, ri));
+            const float itol = (float)inner_tol * r0;
+            float ires = r0, it = 0.0f;
+            float rho_old = 1.0f, omega_old = 1.0f, alpha = 1.0f;
+            int inner_bad = 0;
+            while (it < (float)o->max_iter && ires >= itol) {
+                const float rho_new = dotf64w(n, ri, rh);
+                if (rho_new == 0.0f) break;
+                const float beta = (rho_new / rho_old) * (alpha / omega_old);
+                for (int i = 0; i < n; i++) p[i] = fmaf(beta, fmaf(-omega_old, v[i], p[i]), ri[i]);
+                lu_solve_avail_f(n, rp, ci, diag, lev_l, lev_u, F32, p, tmp, ph);
+                spmvf(n, rp, ci, Af, ph, v);
+                alpha = rho_new / dotf64w(n, rh, v);
+                for (int i = 0; i < n; i++) d[i] = fmaf(alpha, ph[i], d[i]);
+                for (int i = 0; i < n; i++) s[i] = fmaf(-alpha, v[i], ri[i]);
+                ires = sqrtf(dotf64w(n, s, s));
+                it += 0.5f;
+                if (!isfinite(ires)) { inner_bad = 1; break; }
+                if (ires < itol) break;
+                lu_solve_avail_f(n, rp, ci, diag, lev_l, lev_u, F32, s, tmp, sh);
+                spmvf(n, rp, ci, Af, sh, t);
+                const float ts = dotf64w(n, t, s);
+                const float tt = dotf64w(n, t, t);
+                const float omega_new = ts / tt;
+                for (int i = 0; i < n; i++) d[i] = fmaf(omega_new, sh[i], d[i]);
+                it += 0.5f;
+                for (int i = 0; i < n; i++) ri[i] = fmaf(-omega_new, t[i], s[i]);
+                ires = sqrtf(dotf64w(n, ri, ri));
+                rho_old = rho_new;
+                omega_old = omega_new;
+                if (!isfinite(ires)) { inner_bad = 1; break; }
+            }
+            iter += it;
+            steps++;
+            if (inner_bad) { conv = ORC_NONFINITE; break; }
+            for (int i = 0; i < n; i++) x[i] = fma((double)d[i], rn, x[i]);
+            orc_spmv(n, rp, ci, A, x, w, 1);
+            for (int i = 0; i < n; i++) r[i] = b[i] - w[i];
+            rn = sqrt(orc_dot(n, r, r, 64));
+            if (!isfinite(rn)) { conv = ORC_NONFINITE; break; }
+            if (rn < tol_abs) { conv = ORC_CONVERGED; break; }
+            if (it == 0.0f) break;
+        }
+    }
+    if (conv == ORC_ZERO_PIVOT) {
+        for (int i = 0; i < n; i++) x[i] = 0.0;
+        rn = bnorm;
+    }
+    *iter_out = iter;
+    *conv_out = conv;
+    *iter_res_out = (conv == ORC_ZERO_PIVOT) ? 0.0 : rn;
+    *true_res_out = rn;
+    *refine_out = steps;
+    free(Af); free(fb); free(r);
+    return 0;
+}
+
+/* Entries with work of the persistent factor kernel (a dependency pair or
+ * a pivot division), ordered by level in the entry dependency DAG, then by
+ * position.  Returns their number. */
+int orc_entry_order(int n, const int *rp, const int *ci, const int *diag, int *order)
+{
+    const int nnz = rp[n];
+    int *elev = (int *)calloc((size_t)nnz, sizeof(int));
+    int maxlev = 0;
+    for (int i = 0; i < n; i++) {
+        for (int e = rp[i]; e < rp[i + 1]; e++) {
+            const int j = ci[e];
+            const int kmax = (i < j ? i : j) - 1;
+            int l = 0;
+            for (int p = rp[i]; p < diag[i] && ci[p] <= kmax; p++) {
+                const int q = find_pos(rp, ci, ci[p], j);
+                if (q < 0) continue;
+                const int m = (elev[p] > elev[q] ? elev[p] : elev[q]) + 1;
+                if (m > l) l = m;
+            }
+            if (j < i && elev[diag[j]] + 1 > l) l = elev[diag[j]] + 1;
+            elev[e] = l;
+            if (l > maxlev) maxlev = l;
+        }
+    }
+    int ne = 0;
+    for (int l = 1; l <= maxlev; l++)
+        for (int e = 0; e < nnz; e++)
+            if (elev[e] == l) order[ne++] = e;
+    free(elev);
+    return ne;
+}
+
+/* ParILU(0) Jacobi sweeps from `init` (NULL: from A; only the entries with
+ * work take the guess, the others are exact = a_ij), stopped as soon as the
+ * iterate a sweep starts from has ||(LU - A) on the pattern||_F <=
+ * stop_tol ||A||_F (that iterate is kept), at most `sweeps` sweeps; when
+ * the cap is reached without meeting the tolerance, exact ILU(0) replaces
+ * the factors.  stop_tol <= 0: exactly `sweeps` sweeps.  The residual of an
+ * iterate is summed in the kernel's order: entry q of the work order goes to
+ * lane q % 64, then the wave tree.  *status = sweeps used, or -1 for the
+ * fallback.  F[nnz] out. */
+int orc_parilu0_stop(int n, const int *rp, const int *ci, const int *diag, const double *A,
+                     const double *init, double *F, int sweeps, double stop_tol, int *status)
+{
+    const int nnz = rp[n];
+    int *order = (int *)malloc(sizeof(int) * (size_t)(nnz > 0 ? nnz : 1));
+    double *W = (double *)malloc(sizeof(double) * (size_t)(nnz > 0 ? nnz : 1));
+    const int ne = orc_entry_order(n, rp, ci, diag, order);
+    memcpy(F, A, sizeof(double) * (size_t)nnz);
+    if (init)
+        for (int q = 0; q < ne; q++) F[order[q]] = init[order[q]];
+    const double anorm2 = stop_tol > 0.0 ? orc_dot(nnz, A, A, 64) : 0.0;
+    const double lim2 = stop_tol * stop_tol * anorm2;
+    int used = 0, met = stop_tol <= 0.0;
+    int *rowof = (int *)malloc(sizeof(int) * (size_t)(nnz > 0 ? nnz : 1));
+    for (int i = 0; i < n; i++)
+        for (int e = rp[i]; e < rp[i + 1]; e++) rowof[e] = i;
+    while (used < sweeps) {
+        memcpy(W, F, sizeof(double) * (size_t)nnz);
+        double t[64];
+        for (int l = 0; l < 64; l++) t[l] = 0.0;
+        for (int q = 0; q < ne; q++) {
+            const int e = order[q], i = rowof[e], j = ci[e];
+            const int kmax = (i < j ? i : j) - 1;
+            double sum = 0.0;
+            for (int p = rp[i]; p < diag[i] && ci[p] <= kmax; p++) {
+                const int qq = find_pos(rp, ci, ci[p], j);
+                if (qq >= 0) sum = fma(W[p], W[qq], sum);
+            }
+            const double tt = A[e] - sum;
+            double res;
+            if (j < i) {
+                const double dv = W[diag[j]];
+                res = tt - W[e] * dv;
+                F[e] = tt / dv;
+            } else {
+                res = tt - W[e];
+                F[e] = tt;
+            }
+            t[q % 64] = fma(res, res, t[q % 64]);
+        }
+        for (int off = 32; off > 0; off /= 2)
+            for (int l = 0; l < off; l++) t[l] = t[l] + t[l + off];
+        used++;
+        if (stop_tol > 0.0 && t[0] <= lim2) {
+            memcpy(F, W, sizeof(double) * (size_t)nnz);
+            used--;
+            met = 1;
+            break;
+        }
+    }
+    if (!met) {
+        memcpy(F, A, sizeof(double) * (size_t)nnz);
+        orc_ilu0(n, rp, ci, diag, F);
+        *status = -1;
+    } else {
+        *status = used;
+    }
+    free(order); free(W); free(rowof);
+    return 0;
+}
+
+/* whole-batch driver of the split factor + solve paths: precond 0 = exact
+ * ILU(0), 1 = ParILU(0) (with init / stop_tol as orc_parilu0_stop);
+ * precision 0 = FP64 BiCGSTAB (availability-order solves), 1 = FP32 inner /
+ * FP64 refinement over the FP32-rounded factors. */
+typedef struct {
+    int n, nnz, b0, b1, precond, sweeps, precision, max_refine;
+    double stop_tol, inner_tol;
+    const int *rp, *ci, *diag, *lev;
+    const double *A, *b, *init;
+    double *x;
+    const orc_opts *o;
+    float *iter; int *conv; double *iter_res, *true_res; int *refine, *fstat;
+} split_job;
+
+static void *split_worker(void *arg)
+{
+    split_job *j = (split_job *)arg;
+    const int n = j->n, nnz = j->nnz;
+    double *F = (double *)malloc(sizeof(double) * (size_t)nnz);
+    float *F32 = (float *)malloc(sizeof(float) * (size_t)nnz);
+    for (int s = j->b0; s < j->b1; s++) {
+        const double *A = j->A + (size_t)s * nnz;
+        const double *b = j->b + (size_t)s * n;
+        double *x = j->x + (size_t)s * n;
+        int st = 0;
+        if (j->precond == 1) {
+            orc_parilu0_stop(n, j->rp, j->ci, j->diag, A, j->init ? j->init + (size_t)s * nnz : NULL, F,
+                             j->sweeps, j->stop_tol, &st);
+        } else {
+            memcpy(F, A, sizeof(double) * (size_t)nnz);
+            orc_ilu0(n, j->rp, j->ci, j->diag, F);
+        }
+        if (j->fstat) j->fstat[s] = st;
+        if (j->precision == 1) {
+            for (int q = 0; q < nnz; q++) F32[q] = (float)F[q];
+            orc_bicgstab_mixed(n, j->rp, j->ci, j->diag, j->lev, j->lev + n, A, b, F32, j->o,
+                               j->inner_tol, j->max_refine, x, &j->iter[s], &j->conv[s],
+                               &j->iter_res[s], &j->true_res[s], &j->refine[s]);
+        } else {
+            orc_bicgstab(n, j->rp, j->ci, j->diag, A, b, x, ORC_PC_LU, F, NULL, j->o,
+                         &j->iter[s], &j->conv[s], &j->iter_res[s], &j->true_res[s]);
+            j->refine[s] = 0;
+        }
+    }
+    free(F); free(F32);
+    return NULL;
+}
+
+int orc_batch_solve_split(int n, int nnz, const int *rp, const int *ci, int batch,
+                          const double *A, const double *b, double *x, int precond, int sweeps,
+                          const double *init, double stop_tol, int precision, double inner_tol,
+                          int max_refine, const orc_opts *o, float *iter, int *conv,
+                          double *iter_res, double *true_res, int *refine, int *fstat, int threads)
+{
+    int *diag = (int *)malloc(sizeof(int) * (size_t)n);
+    if (orc_find_diag(n, rp, ci, diag) > 0) { free(diag); return -5; }
+    int *lev = (int *)malloc(sizeof(int) * 2 * (size_t)n);
+    orc_trsv_levels(n, rp, ci, diag, lev, lev + n);
+    if (threads < 1) threads = 1;
+    if (threads > batch) threads = batch > 0 ? batch : 1;
+    pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * (size_t)threads);
+    split_job *jobs = (split_job *)malloc(sizeof(split_job) * (size_t)threads);
+    for (int t = 0; t < threads; t++) {
+        split_job *j = &jobs[t];
+        j->n = n; j->nnz = nnz; j->precond = precond; j->sweeps = sweeps; j->precision = precision;
+        j->max_refine = max_refine; j->stop_tol = stop_tol; j->inner_tol = inner_tol;
+        j->b0 = (int)((long long)batch * t / threads);
+        j->b1 = (int)((long long)batch * (t + 1) / threads);
+        j->rp = rp; j->ci = ci; j->diag = diag; j->lev = lev; j->A = A; j->b = b; j->init = init;
+        j->x = x; j->o = o; j->iter = iter; j->conv = conv; j->iter_res = iter_res;
+        j->true_res = true_res; j->refine = refine; j->fstat = fstat;
+        if (threads == 1) split_worker(j);
+        else pthread_create(&th[t], NULL, split_worker, j);
+    }
+    if (threads > 1)
+        for (int t = 0; t < threads; t++) pthread_join(th[t], NULL);
+    free(th); free(jobs); free(diag); free(lev);
+    return 0;
+}

@@ -100,6 +100,23 @@ int orc_batch_solve(int n, int nnz, const int *rp, const int *ci, int batch,
                     float *iter, int *conv, double *iter_res, double *true_res,
                     int threads);
 
+/* mixed precision: FP32 inner BiCGSTAB over FP32 factors F32 inside FP64
+ * iterative refinement (tol / tol_mode / max_iter from o; the solves in
+ * availability order with lev_l / lev_u from orc_trsv_levels) */
+int orc_bicgstab_mixed(int n, const int *rp, const int *ci, const int *diag,
+                       const int *lev_l, const int *lev_u, const double *A, const double *b,
+                       const float *F32, const orc_opts *o, double inner_tol, int max_refine,
+                       double *x, float *iter, int *conv, double *iter_res,
+                       double *true_res, int *refine);
+int orc_entry_order(int n, const int *rp, const int *ci, const int *diag, int *order);
+int orc_parilu0_stop(int n, const int *rp, const int *ci, const int *diag, const double *A,
+                     const double *init, double *F, int sweeps, double stop_tol, int *status);
+int orc_batch_solve_split(int n, int nnz, const int *rp, const int *ci, int batch,
+                          const double *A, const double *b, double *x, int precond, int sweeps,
+                          const double *init, double stop_tol, int precision, double inner_tol,
+                          int max_refine, const orc_opts *o, float *iter, int *conv,
+                          double *iter_res, double *true_res, int *refine, int *fstat, int threads);
+
 #ifdef __cplusplus
 }
 #endif

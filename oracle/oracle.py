@@ -86,6 +86,13 @@ def lib():
         L.orc_batch_solve.argtypes = [C.c_int, C.c_int, i32, i32, C.c_int, f64, f64, f64,
                                       C.c_int, C.c_int, C.POINTER(Opts), f32,
                                       P(np.int32, flags="C"), f64, f64, C.c_int]
+        L.orc_entry_order.argtypes = [C.c_int, i32, i32, i32, i32]
+        L.orc_parilu0_stop.argtypes = [C.c_int, i32, i32, i32, f64, C.c_void_p, f64, C.c_int,
+                                       C.c_double, C.POINTER(C.c_int)]
+        L.orc_batch_solve_split.argtypes = [C.c_int, C.c_int, i32, i32, C.c_int, f64, f64, f64,
+                                            C.c_int, C.c_int, C.c_void_p, C.c_double, C.c_int,
+                                            C.c_double, C.c_int, C.POINTER(Opts), f32, i32, f64,
+                                            f64, i32, i32, C.c_int]
         _lib = L
     return _lib
 
@@ -189,3 +196,54 @@ def batch_solve(pat, vals, b, precond=ILU0, sweeps=5, o: Opts | None = None, thr
     if st != 0:
         raise RuntimeError(f"orc_batch_solve failed: {st}")
     return x, it, cv, ir, tr
+
+
+def entry_order(pat):
+    """Entries with work of the persistent factor kernel, in its order."""
+    order = np.empty(max(pat.nnz, 1), np.int32)
+    ne = lib().orc_entry_order(pat.n, pat.row_ptrs, pat.col_idx, pat.diag, order)
+    return order[:ne]
+
+
+def parilu0_stop(pat, vals, sweeps, stop_tol=0.0, init=None):
+    """ParILU(0) sweeps with warm start / residual stop / exact fallback.
+    Returns (F, status): status = sweeps used or -1 (fallback)."""
+    A = _f64(vals)
+    F = np.empty(pat.nnz)
+    st = C.c_int()
+    ini = _f64(init) if init is not None else None
+    lib().orc_parilu0_stop(pat.n, pat.row_ptrs, pat.col_idx, pat.diag, A,
+                           ini.ctypes.data if ini is not None else None, F, sweeps, stop_tol,
+                           C.byref(st))
+    return F, st.value
+
+
+PREC_FP64, PREC_FP32_IR = 0, 1
+
+
+def batch_solve_split(pat, vals, b, precond=ILU0, sweeps=5, init=None, stop_tol=0.0,
+                      precision=PREC_FP64, inner_tol=1e-5, max_refine=10, o: Opts | None = None,
+                      threads=1):
+    """Split factor + solve paths (persistent factor kernel, then the FP64
+    solve in availability order or the FP32-inner / FP64-refinement solve).
+    Returns x, iter, conv, iter_res, true_res, refine, factor_status."""
+    o = o or opts(trsv_order=TRSV_AVAIL)
+    vals = _f64(vals)
+    b = _f64(b)
+    B = b.shape[0]
+    x = np.empty((B, pat.n))
+    it = np.empty(B, dtype=np.float32)
+    cv = np.empty(B, dtype=np.int32)
+    ir = np.empty(B)
+    tr = np.empty(B)
+    rf = np.empty(B, dtype=np.int32)
+    fs = np.empty(B, dtype=np.int32)
+    ini = _f64(init) if init is not None else None
+    st = lib().orc_batch_solve_split(pat.n, pat.nnz, pat.row_ptrs, pat.col_idx, B, vals, b, x,
+                                     1 if precond == PARILU0 else 0, sweeps,
+                                     ini.ctypes.data if ini is not None else None, stop_tol,
+                                     precision, inner_tol, max_refine, C.byref(o), it, cv, ir, tr,
+                                     rf, fs, threads)
+    if st != 0:
+        raise RuntimeError(f"orc_batch_solve_split failed: {st}")
+    return x, it, cv, ir, tr, rf, fs

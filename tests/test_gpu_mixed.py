@@ -1,0 +1,152 @@
+"""GPU parity of the split factor + solve paths against the oracle:
+FP32-inner / FP64-refinement solves and warm-started, residual-stopped
+ParILU(0).  Same operation order on both sides, so results are compared bit
+for bit (solutions, iteration and refinement counts, flags, residuals,
+factor status); the full-size batch is covered by size-independent
+properties.
+"""
+import numpy as np
+import pytest
+
+import bilu
+import oracle as O
+import synth
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def dev():
+    assert torch.cuda.is_available(), "GPU tests need a HIP device"
+    return torch.device("cuda:0")
+
+
+def to_dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(dev())
+
+
+def run_gpu(pat, A, b, F_init=None, **kw):
+    P = bilu.Pattern.from_synth(pat)
+    dA, db = to_dev(A), to_dev(b)
+    x = torch.empty_like(db)
+    out = bilu.solve_ex(P, dA, db, x, bilu.opts(**kw), F_init=None if F_init is None else to_dev(F_init))
+    torch.cuda.synchronize()
+    return (x.cpu().numpy(),) + tuple(t.cpu().numpy() for t in out)
+
+
+def run_oracle(pat, A, b, precond=bilu.ILU0, parilu_sweeps=5, tol=1e-10, tol_mode=1,
+               precision=0, inner_tol=1e-5, max_refine=10, parilu_stop_tol=0.0, init=None,
+               true_res_restart=0, max_iter=500, **_):
+    o = O.opts(tol=tol, tol_mode=tol_mode, max_iter=max_iter, true_res_restart=true_res_restart,
+               trsv_order=O.TRSV_AVAIL)
+    return O.batch_solve_split(pat, A, b, precond=O.PARILU0 if precond == bilu.PARILU0 else O.ILU0,
+                               sweeps=parilu_sweeps, init=init, stop_tol=parilu_stop_tol,
+                               precision=precision, inner_tol=inner_tol, max_refine=max_refine,
+                               o=o, threads=8)
+
+
+NAMES = ["x", "iters", "conv", "iter_res", "true_res", "refine", "factor_status"]
+
+
+def assert_same(g, o, what):
+    for name, u, v in zip(NAMES, g, o):
+        assert np.array_equal(u, v), f"{what}: {name} differs (max |diff| {np.nanmax(np.abs(u - v))})"
+
+
+@pytest.mark.parametrize("kind,B,n", [("random", 256, 144), ("stencil5", 64, 100), ("random", 64, 32),
+                                      ("random", 64, 90)])
+@pytest.mark.parametrize("inner_tol", [1e-5, 1e-3])
+def test_mixed_bitwise(kind, B, n, inner_tol):
+    print(f"case mixed {kind} B={B} n={n} inner_tol={inner_tol}", flush=True)
+    pat, A, b = synth.batch(kind, B, n=n)
+    kw = dict(tol=1e-10, tol_mode=bilu.TOL_RELATIVE, precision=bilu.PRECISION_FP32_IR, inner_tol=inner_tol)
+    g = run_gpu(pat, A, b, **kw)
+    o = run_oracle(pat, A, b, **kw)
+    assert_same(g, o, f"mixed {kind} n={n}")
+    assert (o[2] == 1).all()
+
+
+def test_mixed_parilu_bitwise():
+    print("case mixed parilu 5 sweeps", flush=True)
+    pat, A, b = synth.batch("random", 128, n=144)
+    kw = dict(precond=bilu.PARILU0, parilu_sweeps=5, tol=1e-10, tol_mode=bilu.TOL_RELATIVE,
+              precision=bilu.PRECISION_FP32_IR)
+    assert_same(run_gpu(pat, A, b, **kw), run_oracle(pat, A, b, **kw), "mixed parilu")
+
+
+@pytest.mark.parametrize("precision", [bilu.PRECISION_FP64, bilu.PRECISION_FP32_IR])
+def test_warm_parilu_stop_bitwise(precision):
+    print(f"case warm parilu precision={precision}", flush=True)
+    pat, A, b = synth.batch("random", 128, n=144)
+    # warm start: exact factors with a relative perturbation (a stand-in for a learned guess)
+    F = np.stack([O.ilu0(pat, A[s])[0] for s in range(A.shape[0])])
+    rng = np.random.default_rng(5)
+    init = F * (1.0 + 1e-3 * rng.standard_normal(F.shape))
+    kw = dict(precond=bilu.PARILU0, parilu_sweeps=40, parilu_stop_tol=1e-8, tol=1e-10,
+              tol_mode=bilu.TOL_RELATIVE, precision=precision, true_res_restart=1)
+    g = run_gpu(pat, A, b, F_init=init, **kw)
+    o = run_oracle(pat, A, b, init=init, **kw)
+    assert_same(g, o, "warm parilu")
+    assert (o[6] > 0).all()           # sweeps were needed and the tolerance was met
+
+
+def test_parilu_stop_fallback_bitwise():
+    print("case parilu fallback", flush=True)
+    pat, A, b = synth.batch("random", 64, n=144)
+    kw = dict(precond=bilu.PARILU0, parilu_sweeps=2, parilu_stop_tol=1e-12, tol=1e-10,
+              tol_mode=bilu.TOL_RELATIVE, precision=bilu.PRECISION_FP32_IR)
+    g = run_gpu(pat, A, b, **kw)
+    o = run_oracle(pat, A, b, **kw)
+    assert_same(g, o, "parilu fallback")
+    assert (g[6] == -1).all()
+
+
+def test_mixed_special_cases():
+    print("case mixed special", flush=True)
+    pat, A, b = synth.batch("random", 3, n=64)
+    b = b.copy(); A = A.copy()
+    b[0] = 0.0
+    A[2][pat.diag[0]] = 0.0
+    kw = dict(tol=1e-10, tol_mode=bilu.TOL_RELATIVE, precision=bilu.PRECISION_FP32_IR)
+    g = run_gpu(pat, A, b, **kw)
+    assert_same(g, run_oracle(pat, A, b, **kw), "mixed special")
+    assert g[2][2] == bilu.ZERO_PIVOT
+
+
+def test_mixed_rejects_unsupported():
+    pat, A, b = synth.batch("random", 4, n=300)    # beyond the persistent kernels' slot budget
+    with pytest.raises(bilu.BiluError):
+        run_gpu(pat, A, b, precision=bilu.PRECISION_FP32_IR, tol=1e-10, tol_mode=1)
+    pat, A, b = synth.batch("random", 4, n=144)
+    with pytest.raises(bilu.BiluError):
+        run_gpu(pat, A, b, precond=bilu.ILU0_ISAI, precision=bilu.PRECISION_FP32_IR)
+
+
+def test_mixed_full_batch_properties():
+    print("case mixed full batch 65536", flush=True)
+    B = 65536
+    pat, A, b = synth.batch("random", B, n=144)
+    P = bilu.Pattern.from_synth(pat)
+    dA, db = to_dev(A), to_dev(b)
+    x = torch.empty_like(db)
+    kw = dict(tol=1e-10, tol_mode=bilu.TOL_RELATIVE)
+    it, cv, ir, tr, rf, fs = bilu.solve_ex(P, dA, db, x, bilu.opts(precision=bilu.PRECISION_FP32_IR, **kw))
+    x64 = torch.empty_like(db)
+    bilu.solve(P, dA, db, x64, bilu.opts(true_res_restart=1, **kw))
+    res = torch.empty(B, dtype=torch.float64, device=dev())
+    bilu.true_residual(P, dA, db, x, res)      # independent FP64 kernel
+    torch.cuda.synchronize()
+    bn = np.linalg.norm(b, axis=1)
+    rel = res.cpu().numpy() / bn
+    assert (cv.cpu().numpy() == 1).all()
+    assert rel.max() <= 1e-10
+    assert np.allclose(tr.cpu().numpy() / bn, rel, rtol=1e-3, atol=1e-12)
+    err = (torch.linalg.norm(x - x64, dim=1) / torch.linalg.norm(x64, dim=1)).cpu().numpy()
+    print(f"mixed vs FP64 solution: max rel error {err.max():.3e}, refine steps "
+          f"{rf.cpu().numpy().mean():.2f} mean / {rf.cpu().numpy().max()} max", flush=True)
+    assert err.max() <= 1e-6
+    # spot check against the oracle
+    idx = np.random.default_rng(0).choice(B, 64, replace=False)
+    o = run_oracle(pat, A[idx], b[idx], precision=1, **kw)
+    assert np.array_equal(x.cpu().numpy()[idx], o[0])

@@ -1,0 +1,153 @@
+"""Oracle checks of the mixed-precision and warm-started ParILU(0) paths.
+
+Each function is pinned by something other than its own formula: the
+Jacobi ParILU(0) routine it extends, exact ILU(0) (the fixed point and the
+fallback), the factor residual definition, numpy's dense solve and a CPU
+recomputation of the true residual.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+import synth
+
+
+def relres(pat, A, b, x):
+    D = pat.dense(A)
+    return np.linalg.norm(b - D @ x) / np.linalg.norm(b)
+
+
+def test_parilu_stop_fixed_sweeps_equals_jacobi():
+    pat, A, _ = synth.batch("random", 4, n=144)
+    for s in range(4):
+        for sw in (0, 1, 3):
+            F, st = O.parilu0_stop(pat, A[s], sw)
+            Fj = O.parilu0(pat, A[s], sw)
+            assert st == sw
+            assert np.array_equal(F, Fj)
+
+
+def test_parilu_stop_reaches_exact_ilu():
+    pat, A, _ = synth.batch("random", 3, n=90)
+    for s in range(3):
+        Fe, _ = O.ilu0(pat, A[s])
+        F, st = O.parilu0_stop(pat, A[s], 200, stop_tol=1e-14)
+        assert 0 < st < 200
+        assert np.allclose(F, Fe, rtol=1e-12, atol=0.0)
+        # the kept iterate meets the tolerance by the residual's definition
+        assert O.factor_residual(pat, A[s], F) <= 1e-14 * np.linalg.norm(A[s]) * 1.01
+
+
+def test_parilu_stop_warm_start_at_fixed_point():
+    pat, A, _ = synth.batch("random", 2, n=144)
+    Fe, _ = O.ilu0(pat, A[0])
+    F, st = O.parilu0_stop(pat, A[0], 10, stop_tol=1e-12, init=Fe)
+    assert st == 0 and np.array_equal(F, Fe)
+
+
+def test_parilu_stop_warm_start_only_entries_with_work():
+    # entries without work keep a_ij whatever the guess says
+    pat, A, _ = synth.batch("random", 1, n=64)
+    junk = np.full(pat.nnz, 7.0)
+    F, st = O.parilu0_stop(pat, A[0], 0, init=junk)
+    order = O.entry_order(pat)
+    rest = np.setdiff1d(np.arange(pat.nnz), order)
+    assert np.array_equal(F[rest], A[0][rest])
+    assert np.all(F[order] == 7.0)
+
+
+def test_parilu_stop_fallback_is_exact_ilu():
+    pat, A, _ = synth.batch("random", 2, n=144)
+    Fe, _ = O.ilu0(pat, A[1])
+    F, st = O.parilu0_stop(pat, A[1], 1, stop_tol=1e-14)
+    assert st == -1 and np.array_equal(F, Fe)
+
+
+def test_entry_order_is_topological():
+    # every entry's dependencies (ParILU pairs and the pivot) come at a lower
+    # DAG level, so one pass in this order with level barriers is exact ILU(0)
+    pat, _, _ = synth.batch("random", 1, n=90)
+    order = O.entry_order(pat)
+    rp, ci, diag = pat.row_ptrs, pat.col_idx, pat.diag
+    pos = {(i, ci[p]): p for i in range(pat.n) for p in range(rp[i], rp[i + 1])}
+    rank = {e: q for q, e in enumerate(order)}
+    for e in order:
+        i = int(np.searchsorted(rp, e, side="right") - 1)
+        j = ci[e]
+        for p in range(rp[i], diag[i]):
+            k = ci[p]
+            if k >= min(i, j) or (k, j) not in pos:
+                continue
+            for d in (p, pos[(k, j)]):
+                assert d not in rank or rank[d] < rank[e]
+        if j < i and diag[j] in rank:
+            assert rank[diag[j]] < rank[e]
+
+
+@pytest.mark.parametrize("kind,B,n", [("random", 24, 144), ("stencil5", 16, 100), ("random", 12, 300)])
+def test_mixed_converges_to_fp64_accuracy(kind, B, n):
+    pat, A, b = synth.batch(kind, B, n=n)
+    o = O.opts(tol=1e-10, tol_mode=O.TOL_RELATIVE, trsv_order=O.TRSV_AVAIL)
+    x, it, cv, ir, tr, rf, fs = O.batch_solve_split(pat, A, b, precision=O.PREC_FP32_IR, o=o,
+                                                    threads=8)
+    assert (cv == 1).all()
+    for s in range(B):
+        # independent CPU recomputation of the relative true residual
+        rr = relres(pat, A[s], b[s], x[s])
+        assert rr <= 1e-10
+        # both recomputations carry rounding noise ~ eps * cond(A) ~ 1e-13
+        assert tr[s] / np.linalg.norm(b[s]) == pytest.approx(rr, rel=1e-3, abs=1e-12)
+        xd = np.linalg.solve(pat.dense(A[s]), b[s])
+        assert np.linalg.norm(x[s] - xd) <= 1e-6 * np.linalg.norm(xd)
+    # FP32 inner solves at 1e-5 cannot reach 1e-10 in one refinement step
+    assert rf.min() >= 2 and rf.max() <= 6
+
+
+def test_mixed_split_fp64_equals_merged_oracle():
+    pat, A, b = synth.batch("random", 16, n=144)
+    o = O.opts(tol=1e-10, tol_mode=O.TOL_RELATIVE, true_res_restart=1, trsv_order=O.TRSV_AVAIL)
+    g = O.batch_solve_split(pat, A, b, precision=O.PREC_FP64, o=o, threads=4)
+    m = O.batch_solve(pat, A, b, O.ILU0, o=o, threads=4)
+    for u, v in zip(g[:5], m):
+        assert np.array_equal(u, v)
+    assert (g[5] == 0).all() and (g[6] == 0).all()
+
+
+def test_mixed_special_cases():
+    pat, A, b = synth.batch("random", 3, n=64)
+    b = b.copy()
+    b[0] = 0.0
+    A = A.copy()
+    A[2][pat.diag[5]] = 0.0       # zero pivot somewhere in the factorization
+    A[2][pat.diag[0]] = 0.0
+    o = O.opts(tol=1e-10, tol_mode=O.TOL_RELATIVE)
+    x, it, cv, ir, tr, rf, fs = O.batch_solve_split(pat, A, b, precision=O.PREC_FP32_IR, o=o)
+    assert cv[0] == 1 and it[0] == 0 and rf[0] == 0 and (x[0] == 0).all()
+    assert cv[1] == 1
+    # a zero leading pivot makes u_00 = 0: the FP32 factor check reports it
+    assert cv[2] == O.ZERO_PIVOT and (x[2] == 0).all() and ir[2] == 0.0
+    assert tr[2] == pytest.approx(np.linalg.norm(b[2]), rel=1e-15)
+
+
+def test_mixed_exact_preconditioner_one_inner_step():
+    # tridiagonal: ILU(0) is the exact LU, so each inner solve ends at its
+    # first half step and refinement only fixes FP32 rounding
+    n = 40
+    rp, ci = [0], []
+    for i in range(n):
+        ci += [j for j in (i - 1, i, i + 1) if 0 <= j < n]
+        rp.append(len(ci))
+    pat = synth.Pattern(n, np.array(rp), np.array(ci))
+    rng = np.random.default_rng(3)
+    A = np.empty((4, pat.nnz))
+    for s in range(4):
+        v = -rng.uniform(0.5, 1.5, pat.nnz)
+        for i in range(n):
+            off = [p for p in range(pat.row_ptrs[i], pat.row_ptrs[i + 1]) if p != pat.diag[i]]
+            v[pat.diag[i]] = np.abs(v[off]).sum() * 1.5
+        A[s] = v
+    b = rng.uniform(-1, 1, (4, n))
+    o = O.opts(tol=1e-12, tol_mode=O.TOL_RELATIVE)
+    x, it, cv, ir, tr, rf, fs = O.batch_solve_split(pat, A, b, precision=O.PREC_FP32_IR, o=o)
+    assert (cv == 1).all()
+    assert (it == 0.5 * rf).all()
