@@ -42,6 +42,10 @@ struct FastParams {
     int wave_bytes;
     int par_w;      // 1: the wave region holds a second factor buffer (ParILU Jacobi)
     int *counter;
+    // ISAI generation streams [step][64] and the slot triangles (ISAI applies)
+    const uint32_t *isL0, *isL1, *isU0, *isU1;
+    int isL_steps, isU_steps;
+    const uint8_t *slot_kind;
 };
 
 __device__ __forceinline__ void wave_sync()

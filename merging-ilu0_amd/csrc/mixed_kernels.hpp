@@ -55,16 +55,57 @@ __device__ __forceinline__ double parilu_sweep_res(const ushort4 *__restrict__ e
     return rr;
 }
 
+// ISAI of the L (or U) factor in F (FP64, in-place factor layout) into M
+// (FP32 on the pattern: strictly lower positions = ISAI(L), diagonal and
+// upper = ISAI(U)), by the lane streams of the pattern plan: each step is
+// acc = fma(-m_c, f_q, acc) in FP64; a finishing step writes the target
+// (divided by its diagonal for U) rounded to FP32 and restarts acc.  A
+// target's m_c were written by earlier steps of the same lane, and a wave's
+// LDS accesses execute in program order, so no synchronisation is needed
+// inside the stream.  Returns nonzero when a divisor (a U diagonal) is 0.
+__device__ __forceinline__ int isai_gen(const uint32_t *__restrict__ r0p, const uint32_t *__restrict__ r1p,
+                                        int steps, const double *F, float *M, float *mscratch, int lane)
+{
+    const char *Fb = reinterpret_cast<const char *>(F);
+    char *Mb = reinterpret_cast<char *>(M);
+    double acc = 0.0;
+    int bad = 0;
+    uint32_t a = r0p[lane], b = r1p[lane];
+    for (int st = 0; st < steps; st++) {
+        const int nx = (st + 1 < steps) ? st + 1 : st;
+        const uint32_t an = r0p[nx * WAVE + lane], bn = r1p[nx * WAVE + lane];
+        const double mv = (double)*reinterpret_cast<const float *>(Mb + ((a >> 16) >> 1));
+        const double fv = *reinterpret_cast<const double *>(Fb + (a & 0xFFFFu));
+        acc = fma(-mv, fv, acc);
+        const uint32_t p = b & 0xFFFFu, d = b >> 16;
+        const bool fin = p != 0xFFFFu, div = d != 0xFFFFu;
+        const double dv = *reinterpret_cast<const double *>(Fb + (div ? d : 0u));
+        bad |= (fin && div && dv == 0.0);
+        const double val = div ? acc / dv : acc;
+        float *dst = fin ? reinterpret_cast<float *>(Mb + (p >> 1)) : mscratch;
+        *dst = (float)val;
+        acc = fin ? 0.0 : acc;
+        a = an;
+        b = bn;
+    }
+    return bad;
+}
+
 template <typename TO>
 __global__ __launch_bounds__(KF_MAX_WAVES * WAVE) void k_factor_fast(
     FastParams fp, int n, int nnz, int batch, const double *__restrict__ gA,
     const double *__restrict__ gFinit, TO *__restrict__ gF, int mode, int sweeps, double stop_tol,
-    int *__restrict__ g_status)
+    int *__restrict__ g_status, int isai, float *__restrict__ gM, int *__restrict__ gZ)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    // per wave: F[nnz + 2] FP64 (F[nnz] = 0, F[nnz + 1] = 1), then the ParILU
+    // buffer W[nnz + 2] FP64 and/or the ISAI M[nnz + 4] FP32 (M[nnz] = 0,
+    // M[nnz + 1] = 1, M[nnz + 2] = -1, M[nnz + 3] scratch) sharing one region
     double *F = reinterpret_cast<double *>(smem + (size_t)wave * fp.wave_bytes);
     double *W = F + (nnz + 2);
+    float *M = reinterpret_cast<float *>(W);
+    if (lane == 0) F[nnz + 1] = 1.0;
     for (;;) {
         int sys = 0;
         if (lane == 0) sys = atomicAdd(fp.counter, 1);
@@ -116,8 +157,20 @@ __global__ __launch_bounds__(KF_MAX_WAVES * WAVE) void k_factor_fast(
             wave_sync();
             ilu_pull(fp.etab, fp.eptr, fp.nlev_e, fp.dep, n, fp.diag, F, lane);
         }
-        TO *Fo = gF + (size_t)sys * nnz;
-        for (int e = lane; e < nnz; e += WAVE) Fo[e] = (TO)F[e];
+        if (isai) {
+            if (lane == 0) { F[nnz] = 0.0; M[nnz] = 0.0f; M[nnz + 1] = 1.0f; M[nnz + 2] = -1.0f; }
+            wave_sync();
+            int bad = isai_gen(fp.isL0, fp.isL1, fp.isL_steps, F, M, M + nnz + 3, lane);
+            bad |= isai_gen(fp.isU0, fp.isU1, fp.isU_steps, F, M, M + nnz + 3, lane);
+            wave_sync();
+            float *Mo = gM + (size_t)sys * nnz;
+            for (int e = lane; e < nnz; e += WAVE) Mo[e] = M[e];
+            bad = __any(bad);
+            if (lane == 0) gZ[sys] = bad;
+        } else {
+            TO *Fo = gF + (size_t)sys * nnz;
+            for (int e = lane; e < nnz; e += WAVE) Fo[e] = (TO)F[e];
+        }
         if (g_status && lane == 0) g_status[sys] = status;
         wave_sync();
     }
@@ -170,15 +223,46 @@ __device__ __forceinline__ void spmv_slots_global(const double *__restrict__ Ap,
     }
 }
 
+// y (LDS, by row) = C x over the slot layout with the slot coefficients C
+// given per slot: in registers (c[s]) or in the wave's LDS, slot-major
+// (cl[s * 64 + lane]).  Same CSR-order fma chain as spmv_slots.
+template <int SM, bool FROM_LDS>
+__device__ __forceinline__ void spmv_coef(const float (&c)[SM], const float *__restrict__ cl,
+                                          const uint32_t *__restrict__ meta, const float *__restrict__ x,
+                                          float *__restrict__ y, float *__restrict__ scratch, int lane)
+{
+    const char *xb = reinterpret_cast<const char *>(x);
+    char *yb = reinterpret_cast<char *>(y);
+    float acc = 0.0f;
+#pragma unroll
+    for (int s = 0; s < SM; s++) {
+        const float cs = FROM_LDS ? cl[s * WAVE + lane] : c[s];
+        const uint32_t mm = meta[s * WAVE + lane];
+        acc = __builtin_fmaf(cs, *reinterpret_cast<const float *>(xb + ((mm & 0xFFFFu) >> 1)), acc);
+        const bool end = (mm >> 16) != 0xFFFFu;
+        float *dst = end ? reinterpret_cast<float *>(yb + ((mm >> 16) >> 1)) : scratch;
+        *dst = acc;
+        acc = end ? 0.0f : acc;
+    }
+}
+
 #ifndef KM_MAX_WAVES
-#define KM_MAX_WAVES 12
+#define KM_MAX_WAVES 12      // LU variant
+#endif
+#ifndef KMI_MAX_WAVES
+#define KMI_MAX_WAVES 8      // ISAI variant (two slot coefficient sets in VGPRs)
 #endif
 
-template <int NR, int SM>
-__global__ __launch_bounds__(KM_MAX_WAVES * WAVE) void k_mixed(
+constexpr int PCK_LU = 0, PCK_ISAI = 1;
+
+// PCK_LU: gF = FP32 in-place factors (triangular solves); PCK_ISAI: gF =
+// FP32 ISAI(L) / ISAI(U) values on the pattern, gZ = zero-pivot flags of
+// their generation (two SpMVs per apply)
+template <int NR, int SM, int PCK>
+__global__ __launch_bounds__((PCK == PCK_LU ? KM_MAX_WAVES : KMI_MAX_WAVES) * WAVE) void k_mixed(
     DevPattern P, FastParams fp, int batch, const double *__restrict__ gA,
     const double *__restrict__ gb, double *__restrict__ gx, const float *__restrict__ gF,
-    MixedParams mp, float *__restrict__ g_iter, int *__restrict__ g_conv,
+    const int *__restrict__ gZ, MixedParams mp, float *__restrict__ g_iter, int *__restrict__ g_conv,
     double *__restrict__ g_ires, double *__restrict__ g_tres, int *__restrict__ g_refine)
 {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -192,23 +276,32 @@ __global__ __launch_bounds__(KM_MAX_WAVES * WAVE) void k_mixed(
     // (n + 2) doubles each, used as FP32 vectors by the inner solve (zero
     // cell at [n], scratch at [n + 1]) and as FP64 vectors by the outer
     // residual, then the FP64 row accumulators of the solves acc[64 NR]
+    // (PCK_ISAI: instead of the factors, A (FP32) slot-major [SM][64]; no accumulators)
     unsigned char *wb = smem + fp.shared_bytes + (size_t)wave * fp.wave_bytes;
     float *F = reinterpret_cast<float *>(wb);
-    double *d0 = reinterpret_cast<double *>(wb + (((size_t)(nnz + 2) * 4 + 15) / 16) * 16);
+    float *Asl = F;
+    double *d0 = reinterpret_cast<double *>(wb + (PCK == PCK_LU ? (((size_t)(nnz + 2) * 4 + 15) / 16) * 16
+                                                                : (size_t)SM * WAVE * 4));
     double *d1 = d0 + (n + 2);
     float *e0 = reinterpret_cast<float *>(d0), *e1 = reinterpret_cast<float *>(d1);
     float *scratch = e1 + (n + 1);
     double *dscratch = d1 + (n + 1);
     double *accb = d1 + (n + 2);
-    if (lane == 0) F[nnz] = 0.0f;
+    if (PCK == PCK_LU && lane == 0) F[nnz] = 0.0f;
     __syncthreads();
+
     int rlev[NR];
 #pragma unroll
     for (int k = 0; k < NR; k++)
         rlev[k] = (int)fp.rlevL[k * WAVE + lane] | ((int)fp.rlevU[k * WAVE + lane] << 16);
     const int vcntL = fp.cntL[lane], vcntU = fp.cntU[lane];
 
-    float a[SM];
+    // LU: A (FP32) in VGPRs.  ISAI: the slot coefficients of ISAI(L) (strict
+    // lower values, 1 on the diagonal, 0 above) and ISAI(U) (0 below, values
+    // on and above the diagonal) in VGPRs, A in LDS.
+    constexpr int SA = PCK == PCK_LU ? SM : 1, SC = PCK == PCK_ISAI ? SM : 1;
+    float a[SA];
+    float cL[SC], cU[SC];
 
     for (;;) {
         int sys = 0;
@@ -219,11 +312,21 @@ __global__ __launch_bounds__(KM_MAX_WAVES * WAVE) void k_mixed(
         const double *Ap = gA + (size_t)sys * nnz;
         const double *bp = gb + (size_t)sys * n;
         const float *Fp = gF + (size_t)sys * nnz;
-        for (int e = lane; e < nnz; e += WAVE) F[e] = Fp[e];
+        if (PCK == PCK_LU)
+            for (int e = lane; e < nnz; e += WAVE) F[e] = Fp[e];
 #pragma unroll
         for (int q = 0; q < SM; q++) {
             const int e = (q < S) ? fp.slot_pos[q * WAVE + lane] : -1;
-            a[q] = (e >= 0) ? (float)Ap[e] : 0.0f;
+            const float av = (e >= 0) ? (float)Ap[e] : 0.0f;
+            if constexpr (PCK == PCK_LU) {
+                a[q] = av;
+            } else {
+                Asl[q * WAVE + lane] = av;
+                const int kd = (q < S) ? fp.slot_kind[q * WAVE + lane] : 0;
+                const float m = (e >= 0) ? Fp[e] : 0.0f;
+                cL[q] = kd == 1 ? m : (kd == 2 ? 1.0f : 0.0f);
+                cU[q] = kd == 1 ? 0.0f : m;
+            }
         }
         wave_sync();
         // zero cells of the FP32 views (the FP64 residual overwrites them)
@@ -234,14 +337,18 @@ __global__ __launch_bounds__(KM_MAX_WAVES * WAVE) void k_mixed(
         zero_cells();
         int bad = 0;
         Vec<NR, float> rinv;
+        if constexpr (PCK == PCK_LU) {
 #pragma unroll
-        for (int k = 0; k < NR; k++) {
-            const int i = lane + WAVE * k;
-            const float u = (i < n) ? F[fp.diag[i]] : 1.0f;
-            bad |= (u == 0.0f);
-            rinv.e[k] = 1.0f / u;
+            for (int k = 0; k < NR; k++) {
+                const int i = lane + WAVE * k;
+                const float u = (i < n) ? F[fp.diag[i]] : 1.0f;
+                bad |= (u == 0.0f);
+                rinv.e[k] = 1.0f / u;
+            }
+            bad = __any(bad);
+        } else {
+            bad = gZ[sys];
         }
-        bad = __any(bad);
 
         Vec<NR, double> x, r;
 #pragma unroll
@@ -257,14 +364,26 @@ __global__ __launch_bounds__(KM_MAX_WAVES * WAVE) void k_mixed(
         int steps = 0;
         int conv = BILU_NOT_CONVERGED;
 
+        // out = M^{-1} in, also left in e0 for the following SpMV
         auto apply = [&](const Vec<NR, float> &in, Vec<NR, float> &out) {
-            Vec<NR, float> y;
-            trsv_avail<NR, true, float>(trL, vcntL, fp.nlev_l, rlev, F, in, rinv, e1, accb, scratch, y, lane);
-            trsv_avail<NR, false, float>(trU, vcntU, fp.nlev_u, rlev, F, y, rinv, e0, accb, scratch, out, lane);
-            wave_sync();
+            if constexpr (PCK == PCK_LU) {
+                Vec<NR, float> y;
+                trsv_avail<NR, true, float>(trL, vcntL, fp.nlev_l, rlev, F, in, rinv, e1, accb, scratch, y, lane);
+                trsv_avail<NR, false, float>(trU, vcntU, fp.nlev_u, rlev, F, y, rinv, e0, accb, scratch, out, lane);
+                wave_sync();
+            } else {
+                to_lds(in, e0, lane, n);
+                wave_sync();
+                spmv_coef<SM, false>(cL, nullptr, meta, e0, e1, scratch, lane);
+                wave_sync();
+                spmv_coef<SM, false>(cU, nullptr, meta, e1, e0, e0 + (n + 1), lane);
+                wave_sync();
+                from_lds(out, e0, lane, n);
+            }
         };
         auto matvec = [&](Vec<NR, float> &out) {   // out = A32 e0
-            spmv_slots<SM, float>(a, meta, e0, e1, scratch, lane);
+            if constexpr (PCK == PCK_LU) spmv_slots<SM, float>(a, meta, e0, e1, scratch, lane);
+            else spmv_coef<SM, true>(cL, Asl, meta, e0, e1, scratch, lane);
             wave_sync();
             from_lds(out, e1, lane, n);
             wave_sync();

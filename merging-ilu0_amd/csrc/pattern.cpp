@@ -227,7 +227,7 @@ void build_fast_plan(PatternAnalysis &pa)
     // pull-by-availability triangular-solve schedules
     const int NR = (n + 63) / 64;
     fp.nr = NR;
-    if (nnz > 8190 || n > 4094 || NR > 7) { fp.why = "pattern beyond the packed solve records"; return; }
+    if (nnz > 8188 || n > 4094 || NR > 7) { fp.why = "pattern beyond the packed solve records"; return; }
     // padding records point at the zero cells (factor index nnz, vector index n), row slot 0
     const uint32_t pad_rec = uint32_t(nnz * 8) | (uint32_t(n * 8) << 16);
     auto trsv_sched = [&](bool lower, std::vector<uint32_t> &rec, std::vector<uint16_t> &cnt,
@@ -312,6 +312,82 @@ void build_fast_plan(PatternAnalysis &pa)
     fp.ne = int(fp.etab.size() / 4);
     if (fp.dep.empty()) { fp.dep.assign(2, 0); }
     if (fp.etab.empty()) { fp.etab.assign(4, 0); }
+
+    // triangle of every SpMV slot (ISAI applies): 1 strictly lower, 2 diagonal, 0 upper / padding
+    fp.slot_kind.assign(fp.slot_pos.size(), 0);
+    for (size_t q = 0; q < fp.slot_pos.size(); q++) {
+        const int e = fp.slot_pos[q];
+        if (e < 0) continue;
+        const int row = int(std::upper_bound(pa.rp.begin(), pa.rp.end(), e) - pa.rp.begin()) - 1;
+        fp.slot_kind[q] = uint8_t(e < pa.diag[row] ? 1 : e == pa.diag[row] ? 2 : 0);
+    }
+    // ISAI(L), ISAI(U) generation streams (power-1 pattern).  Row i of the
+    // ISAI solves A(J,J)^T m = e by the column sweep of the reference
+    // (isai_gpu.cu:325-389); in pull form every target m_t is an fma chain
+    // acc = fma(-m_c, A(J_c, J_t), acc) over the already final m_c in the
+    // sweep's order (L: c = |J|-1 down to t+1, U: c = 0 up to t-1), then
+    // (U only) divided by A(J_t, J_t).  Lane l runs the targets of its rows
+    // l + 64 k one after the other; records are LDS byte offsets for 8-byte
+    // elements: rec0 = 8 q | 8 mp << 16 (q: factor position, mp: ISAI
+    // position of m_c), rec1 = 8 p | 8 d << 16 (p: target written after this
+    // step, d: divisor position; 0xFFFF = none).  Cells past nnz: factor
+    // zero (nnz) and one (nnz + 1); ISAI zero (nnz), one (nnz + 1), minus one
+    // (nnz + 2).
+    const uint32_t FZ = uint32_t(nnz), FONE = uint32_t(nnz + 1);
+    const uint32_t MZ = uint32_t(nnz), MONE = uint32_t(nnz + 1), MNEG = uint32_t(nnz + 2);
+    auto posof = [&](int r, int c) -> int {
+        const auto b = pa.ci.begin() + pa.rp[r], e = pa.ci.begin() + pa.rp[r + 1];
+        const auto it = std::lower_bound(b, e, c);
+        return (it != e && *it == c) ? int(it - pa.ci.begin()) : -1;
+    };
+    auto rec0 = [](uint32_t q, uint32_t mp) { return (q * 8) | ((mp * 8) << 16); };
+    auto rec1 = [](uint32_t p, uint32_t d) { return (p == 0xFFFFu ? 0xFFFFu : p * 8) | ((d == 0xFFFFu ? 0xFFFFu : d * 8) << 16); };
+    auto isai_stream = [&](bool lower, std::vector<uint32_t> &r0, std::vector<uint32_t> &r1, int &steps) {
+        std::vector<std::vector<std::pair<uint32_t, uint32_t>>> L(64);
+        for (int i = 0; i < n; i++) {
+            auto &v = L[i % 64];
+            if (lower) {
+                const int b = pa.rp[i], m = pa.diag[i] - b + 1;
+                for (int t = m - 2; t >= 0; t--) {
+                    const int jt = pa.ci[b + t];
+                    size_t first = v.size();
+                    for (int c = m - 1; c > t; c--) {
+                        const int q = posof(pa.ci[b + c], jt);
+                        if (q < 0) continue;
+                        v.push_back({rec0(uint32_t(q), c == m - 1 ? MONE : uint32_t(b + c)), rec1(0xFFFFu, 0xFFFFu)});
+                    }
+                    if (v.size() == first) v.push_back({rec0(FZ, MZ), 0});   // (no pair: cannot happen, c = m-1 exists)
+                    v.back().second = rec1(uint32_t(b + t), 0xFFFFu);
+                }
+            } else {
+                const int b = pa.diag[i], m = pa.rp[i + 1] - b;
+                v.push_back({rec0(FONE, MNEG), rec1(uint32_t(b), uint32_t(pa.diag[i]))});
+                for (int t = 1; t < m; t++) {
+                    const int jt = pa.ci[b + t];
+                    size_t first = v.size();
+                    for (int c = 0; c < t; c++) {
+                        const int q = posof(pa.ci[b + c], jt);
+                        if (q < 0) continue;
+                        v.push_back({rec0(uint32_t(q), uint32_t(b + c)), rec1(0xFFFFu, 0xFFFFu)});
+                    }
+                    if (v.size() == first) v.push_back({rec0(FZ, MZ), 0});
+                    v.back().second = rec1(uint32_t(b + t), uint32_t(pa.diag[jt]));
+                }
+            }
+        }
+        size_t S = 1;
+        for (auto &v : L) S = std::max(S, v.size());
+        steps = int(S);
+        r0.assign(S * 64, rec0(FZ, MZ));
+        r1.assign(S * 64, rec1(0xFFFFu, 0xFFFFu));
+        for (int l = 0; l < 64; l++)
+            for (size_t k = 0; k < L[l].size(); k++) {
+                r0[k * 64 + l] = L[l][k].first;
+                r1[k * 64 + l] = L[l][k].second;
+            }
+    };
+    isai_stream(true, fp.isL0, fp.isL1, fp.isL_steps);
+    isai_stream(false, fp.isU0, fp.isU1, fp.isU_steps);
     fp.ok = true;
 }
 

@@ -59,6 +59,11 @@ struct FastPlan {
     std::vector<uint16_t> rlevL, rlevU;
     std::vector<uint16_t> etab, eptr, dep;   // etab: 4 u16 per record, dep: 2 u16 per pair
     int nlev_e = 0, ne = 0;
+    // ISAI(L) / ISAI(U) generation streams [step][64] (two u32 records per step)
+    // and the triangle of each SpMV slot (1 lower, 2 diagonal, 0 upper / padding)
+    std::vector<uint32_t> isL0, isL1, isU0, isU1;
+    int isL_steps = 0, isU_steps = 0;
+    std::vector<uint8_t> slot_kind;
 };
 
 struct PatternAnalysis {

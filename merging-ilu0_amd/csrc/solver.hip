@@ -753,6 +753,8 @@ struct bilu_pattern_s {
     uint16_t *d_cntL = nullptr, *d_cntU = nullptr;
     uint16_t *d_rlevL = nullptr, *d_rlevU = nullptr;
     uint16_t *d_etab = nullptr, *d_eptr = nullptr, *d_dep = nullptr;
+    uint32_t *d_isL0 = nullptr, *d_isL1 = nullptr, *d_isU0 = nullptr, *d_isU1 = nullptr;
+    uint8_t *d_slot_kind = nullptr;
     int *d_counter = nullptr;      // 256 bytes of work counters: [0] solve kernels, [32] factor kernel
     // solve workspace (factors between the factor and solve launches), grown on demand
     void *ws = nullptr;
@@ -835,6 +837,11 @@ int upload(bilu_pattern_s &h)
         const size_t o_et = put(fp.etab.data(), fp.etab.size() * 2);
         const size_t o_ep = put(fp.eptr.data(), fp.eptr.size() * 2);
         const size_t o_dp = put(fp.dep.data(), fp.dep.size() * 2);
+        const size_t o_iL0 = put(fp.isL0.data(), fp.isL0.size() * 4);
+        const size_t o_iL1 = put(fp.isL1.data(), fp.isL1.size() * 4);
+        const size_t o_iU0 = put(fp.isU0.data(), fp.isU0.size() * 4);
+        const size_t o_iU1 = put(fp.isU1.data(), fp.isU1.size() * 4);
+        const size_t o_sk = put(fp.slot_kind.data(), fp.slot_kind.size());
         HIP_TRY(hipMalloc((void **)&h.d_fast, blob.size()));
         HIP_TRY(hipMemcpy(h.d_fast, blob.data(), blob.size(), hipMemcpyHostToDevice));
         uint8_t *b = h.d_fast;
@@ -843,6 +850,9 @@ int upload(bilu_pattern_s &h)
         h.d_cntL = (uint16_t *)(b + o_sL); h.d_cntU = (uint16_t *)(b + o_sU);
         h.d_rlevL = (uint16_t *)(b + o_rL); h.d_rlevU = (uint16_t *)(b + o_rU);
         h.d_etab = (uint16_t *)(b + o_et); h.d_eptr = (uint16_t *)(b + o_ep); h.d_dep = (uint16_t *)(b + o_dp);
+        h.d_isL0 = (uint32_t *)(b + o_iL0); h.d_isL1 = (uint32_t *)(b + o_iL1);
+        h.d_isU0 = (uint32_t *)(b + o_iU0); h.d_isU1 = (uint32_t *)(b + o_iU1);
+        h.d_slot_kind = b + o_sk;
     }
     return BILU_OK;
 }
@@ -941,6 +951,9 @@ void fast_tables(const bilu_pattern_s &h, FastParams &fp)
     fp.nlev_l = h.dev.nlev_l; fp.nlev_u = h.dev.nlev_u; fp.nlev_e = pl.nlev_e; fp.ne = pl.ne;
     fp.S = pl.S; fp.slot_meta = h.d_slot_meta; fp.slot_pos = h.d_slot_pos;
     fp.counter = h.d_counter;
+    fp.isL0 = h.d_isL0; fp.isL1 = h.d_isL1; fp.isU0 = h.d_isU0; fp.isU1 = h.d_isU1;
+    fp.isL_steps = pl.isL_steps; fp.isU_steps = pl.isU_steps;
+    fp.slot_kind = h.d_slot_kind;
 }
 
 int slot_template(int S)
@@ -976,9 +989,12 @@ int workspace(bilu_pattern_s &h, size_t bytes, void **out)
 
 // persistent factorization: exact ILU(0) or ParILU(0) sweeps (optionally
 // from F_init, optionally stopped on the factor residual) -> F (TO)
+// isai: write ISAI(L) / ISAI(U) of the factors (FP32) to M and zero-pivot
+// flags to Z instead of the factors
 template <typename TO>
 int launch_factor_fast(bilu_pattern_s &h, int batch, const double *A, const double *F_init, TO *F,
-                       int mode, int sweeps, double stop_tol, int *status, hipStream_t st)
+                       int mode, int sweeps, double stop_tol, int *status, hipStream_t st,
+                       int isai = 0, float *M = nullptr, int *Z = nullptr)
 {
     const auto &pl = h.pa.fast;
     if (!pl.ok) return BILU_ERR_TOO_LARGE;
@@ -986,7 +1002,10 @@ int launch_factor_fast(bilu_pattern_s &h, int batch, const double *A, const doub
     FastParams fp{};
     fast_tables(h, fp);
     fp.counter = h.d_counter + 32;
-    fp.wave_bytes = int(((size_t)(mode == FS_PARILU ? 2 : 1) * (nnz + 2) * sizeof(double) + 15) / 16 * 16);
+    // F (FP64), then the ParILU buffer (FP64) or / and the ISAI values (FP32) in one region
+    const size_t second = std::max<size_t>(mode == FS_PARILU ? (size_t)(nnz + 2) * sizeof(double) : 0,
+                                           isai ? (size_t)(nnz + 4) * sizeof(float) : 0);
+    fp.wave_bytes = int(((size_t)(nnz + 2) * sizeof(double) + second + 15) / 16 * 16);
     int waves = (int)std::min<size_t>(KF_MAX_WAVES, h.max_lds / fp.wave_bytes);
     if (waves < 1) return BILU_ERR_TOO_LARGE;
     waves = std::min(waves, std::max(1, batch));
@@ -996,27 +1015,30 @@ int launch_factor_fast(bilu_pattern_s &h, int batch, const double *A, const doub
     if (lds > 65536) HIP_TRY(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, st, fp.counter);
     hipLaunchKernelGGL(kern, dim3(grid), dim3(waves * WAVE), lds, st, fp, h.pa.n, nnz, batch, A, F_init, F,
-                       mode, sweeps, stop_tol, status);
+                       mode, sweeps, stop_tol, status, isai, M, Z);
     HIP_TRY(hipGetLastError());
     return BILU_OK;
 }
 
-template <int NR, int SM>
+template <int NR, int SM, int PCK>
 int launch_mixed_t(bilu_pattern_s &h, const FastParams &fp, int grid, int waves, size_t lds, int batch,
-                   const double *A, const double *b, double *x, const float *F, const MixedParams &mp,
-                   float *it, int *cv, double *ir, double *tr, int *refine, hipStream_t st)
+                   const double *A, const double *b, double *x, const float *F, const int *Z,
+                   const MixedParams &mp, float *it, int *cv, double *ir, double *tr, int *refine,
+                   hipStream_t st)
 {
-    auto kern = k_mixed<NR, SM>;
+    auto kern = k_mixed<NR, SM, PCK>;
     if (lds > 65536) HIP_TRY(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, st, fp.counter);
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(waves * WAVE), lds, st, h.dev, fp, batch, A, b, x, F, mp, it, cv, ir, tr, refine);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(waves * WAVE), lds, st, h.dev, fp, batch, A, b, x, F, Z, mp, it, cv, ir, tr, refine);
     HIP_TRY(hipGetLastError());
     return BILU_OK;
 }
 
-// FP32 inner / FP64 refinement solve over FP32 factors F
+// FP32 inner / FP64 refinement solve over FP32 factors F (pck = PCK_LU) or
+// FP32 ISAI values F with zero-pivot flags Z (pck = PCK_ISAI)
 int launch_mixed(bilu_pattern_s &h, int batch, const double *A, const double *b, double *x, const float *F,
-                 const MixedParams &mp, float *it, int *cv, double *ir, double *tr, int *refine, hipStream_t st)
+                 const int *Z, int pck, const MixedParams &mp, float *it, int *cv, double *ir, double *tr,
+                 int *refine, hipStream_t st)
 {
     const auto &pl = h.pa.fast;
     const int n = h.pa.n, nnz = h.pa.nnz;
@@ -1027,17 +1049,23 @@ int launch_mixed(bilu_pattern_s &h, int batch, const double *A, const double *b,
     fast_tables(h, fp);
     // per wave: FP32 factors (+2 cells, 16-aligned), two (n + 2)-double vector regions,
     // FP64 row accumulators of the solves (64 NR)
-    fp.wave_bytes = int(((size_t)(nnz + 2) * 4 + 15) / 16 * 16 + (size_t)2 * (n + 2) * 8 + (size_t)64 * nr * 8);
+    fp.wave_bytes = pck == PCK_LU
+        ? int(((size_t)(nnz + 2) * 4 + 15) / 16 * 16 + (size_t)2 * (n + 2) * 8 + (size_t)64 * nr * 8)
+        : int((size_t)sm * WAVE * 4 + (size_t)2 * (n + 2) * 8);
     fp.wave_bytes = (fp.wave_bytes + 15) / 16 * 16;
     fp.shared_bytes = sm * WAVE * 4;
     if ((size_t)fp.shared_bytes >= h.max_lds) return BILU_ERR_TOO_LARGE;
-    int waves = (int)std::min<size_t>(KM_MAX_WAVES, (h.max_lds - fp.shared_bytes) / fp.wave_bytes);
+    int waves = (int)std::min<size_t>(pck == PCK_LU ? KM_MAX_WAVES : KMI_MAX_WAVES,
+                                      (h.max_lds - fp.shared_bytes) / fp.wave_bytes);
     if (waves < 1) return BILU_ERR_TOO_LARGE;
     waves = std::min(waves, std::max(1, batch));
     const size_t lds = fp.shared_bytes + (size_t)waves * fp.wave_bytes;
     const int grid = persistent_grid(h, lds, waves, batch);
 #define MIXED_CASE(R, M) \
-    if (nr == R && sm == M) return launch_mixed_t<R, M>(h, fp, grid, waves, lds, batch, A, b, x, F, mp, it, cv, ir, tr, refine, st);
+    if (nr == R && sm == M) \
+        return pck == PCK_LU \
+            ? launch_mixed_t<R, M, PCK_LU>(h, fp, grid, waves, lds, batch, A, b, x, F, Z, mp, it, cv, ir, tr, refine, st) \
+            : launch_mixed_t<R, M, PCK_ISAI>(h, fp, grid, waves, lds, batch, A, b, x, F, Z, mp, it, cv, ir, tr, refine, st);
 #define MIXED_ROW(R) MIXED_CASE(R, 24) MIXED_CASE(R, 36) MIXED_CASE(R, 48)
     MIXED_ROW(1) MIXED_ROW(2) MIXED_ROW(3) MIXED_ROW(4) MIXED_ROW(5)
 #undef MIXED_ROW
@@ -1252,7 +1280,9 @@ int bilu_solve_ex(bilu_pattern_t h, int batch, const double *d_A, const double *
     hipStream_t st = (hipStream_t)stream;
     HIP_TRY(hipSetDevice(h->device));
     const bool lu = o.precond == BILU_PRECOND_ILU0 || o.precond == BILU_PRECOND_PARILU0;
-    const bool warm = o.precond == BILU_PRECOND_PARILU0 && (d_F_init || o.parilu_stop_tol > 0.0);
+    const bool isai = o.precond == BILU_PRECOND_ILU0_ISAI || o.precond == BILU_PRECOND_PARILU0_ISAI;
+    const bool par = o.precond == BILU_PRECOND_PARILU0 || o.precond == BILU_PRECOND_PARILU0_ISAI;
+    const bool warm = par && (d_F_init || o.parilu_stop_tol > 0.0);
     const bool split = o.precision == BILU_PRECISION_FP32_IR || warm || d_factor_status;
     if (!split) {
         const int rc = solve_fp64(h, batch, d_A, d_b, d_x, o, d_iter, d_conv, d_iter_res, d_true_res, stream);
@@ -1262,25 +1292,29 @@ int bilu_solve_ex(bilu_pattern_t h, int batch, const double *d_A, const double *
         }
         return rc;
     }
-    // factor launch (persistent), factors through the handle's workspace, then the solve
-    if (!lu) return BILU_ERR_INVALID_ARG;   // these paths serve the ILU(0) / ParILU(0) preconditioners
+    // factor launch (persistent), factors through the handle's workspace, then the solve;
+    // these paths serve ILU(0) / ParILU(0), and their ISAI(L, U) in FP32_IR
+    if (!lu && !(isai && o.precision == BILU_PRECISION_FP32_IR)) return BILU_ERR_INVALID_ARG;
     if (!h->pa.fast.ok) return BILU_ERR_TOO_LARGE;
     const int nnz = h->pa.nnz;
-    const int mode = o.precond == BILU_PRECOND_ILU0 ? FS_ILU : FS_PARILU;
+    const int mode = par ? FS_PARILU : FS_ILU;
     const double stop = mode == FS_PARILU ? o.parilu_stop_tol : 0.0;
     void *ws = nullptr;
     int rc;
     if (o.precision == BILU_PRECISION_FP32_IR) {
         const int nr = (h->pa.n + WAVE - 1) / WAVE;
         if (slot_template(h->pa.fast.S) < 0 || nr > MAX_NR || h->pa.fast.nr != nr) return BILU_ERR_TOO_LARGE;
-        rc = workspace(*h, (size_t)batch * nnz * sizeof(float), &ws);
+        const size_t fbytes = ((size_t)batch * nnz * sizeof(float) + 255) / 256 * 256;
+        rc = workspace(*h, fbytes + (size_t)batch * sizeof(int), &ws);
         if (rc != BILU_OK) return rc;
         float *F = static_cast<float *>(ws);
+        int *Z = reinterpret_cast<int *>(static_cast<char *>(ws) + fbytes);
         rc = launch_factor_fast<float>(*h, batch, d_A, mode == FS_PARILU ? d_F_init : nullptr, F, mode,
-                                       o.parilu_sweeps, stop, d_factor_status, st);
+                                       o.parilu_sweeps, stop, d_factor_status, st, isai ? 1 : 0, F, Z);
         if (rc != BILU_OK) return rc;
         MixedParams mp{o.tol, o.tol_mode, (float)o.inner_tol, o.max_iter, o.max_refine};
-        return launch_mixed(*h, batch, d_A, d_b, d_x, F, mp, d_iter, d_conv, d_iter_res, d_true_res, d_refine, st);
+        return launch_mixed(*h, batch, d_A, d_b, d_x, F, Z, isai ? PCK_ISAI : PCK_LU, mp, d_iter, d_conv,
+                            d_iter_res, d_true_res, d_refine, st);
     }
     rc = workspace(*h, (size_t)batch * nnz * sizeof(double), &ws);
     if (rc != BILU_OK) return rc;

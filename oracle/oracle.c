@@ -571,6 +571,12 @@ int orc_batch_solve(int n, int nnz, const int *rp, const int *ci, int batch,
  * parILU_0_gpu.cu:18-57).
  * ------------------------------------------------------------------- */
 
+int orc_bicgstab_mixed_pc(int n, const int *rp, const int *ci, const int *diag,
+                          const int *lev_l, const int *lev_u, const double *A, const double *b,
+                          const float *F32, int isai, int zp, const orc_opts *o, double inner_tol,
+                          int max_refine, double *x, float *iter_out, int *conv_out,
+                          double *iter_res_out, double *true_res_out, int *refine_out);
+
 /* FP32 dot in the wave64 order (lane-strided fmaf partial sums, then the
  * pairwise tree with offsets 32..1) */
 static float dotf64w(int n, const float *a, const float *b)
@@ -623,11 +629,86 @@ static void lu_solve_avail_f(int n, const int *rp, const int *ci, const int *dia
     }
 }
 
+/* ISAI(L) and ISAI(U) of an in-place factor F (power-1 patterns) for the
+ * FP32 preconditioner: the column sweeps of orc_isai (isai_gpu.cu:325-389)
+ * with every finished m_c rounded to FP32 before it is used, and updates
+ * only from entries of the pattern.  M[rp[i] .. diag[i]) = ISAI(L) (its unit
+ * diagonal implicit), M[diag[i] .. rp[i+1]) = ISAI(U).  Returns
+ * ORC_ZERO_PIVOT if a U diagonal is 0. */
+int orc_isai_lu_f32(int n, const int *rp, const int *ci, const int *diag, const double *F, float *M)
+{
+    double sol[64];
+    int bad = 0;
+    for (int i = 0; i < n; i++) {
+        /* lower: J = cols of row i up to the diagonal, rhs 1 at the last position */
+        int b = rp[i], m = diag[i] - rp[i] + 1;
+        if (m > 64) return -4;
+        for (int t = 0; t < m; t++) sol[t] = (t == m - 1) ? 1.0 : 0.0;
+        for (int c = m - 1; c >= 0; c--) {
+            sol[c] = (double)(float)sol[c];   /* L(J_c, J_c) = 1 */
+            for (int t = c - 1; t >= 0; t--) {
+                const int q = find_pos(rp, ci, ci[b + c], ci[b + t]);
+                if (q >= 0) sol[t] = fma(-sol[c], F[q], sol[t]);
+            }
+        }
+        for (int t = 0; t < m - 1; t++) M[b + t] = (float)sol[t];
+        /* upper: J = cols of row i from the diagonal, rhs 1 at position 0 */
+        b = diag[i]; m = rp[i + 1] - diag[i];
+        if (m > 64) return -4;
+        for (int t = 0; t < m; t++) sol[t] = (t == 0) ? 1.0 : 0.0;
+        for (int c = 0; c < m; c++) {
+            const double dv = F[diag[ci[b + c]]];
+            if (dv == 0.0) bad = 1;
+            sol[c] = (double)(float)(sol[c] / dv);
+            for (int t = c + 1; t < m; t++) {
+                const int q = find_pos(rp, ci, ci[b + c], ci[b + t]);
+                if (q >= 0) sol[t] = fma(-sol[c], F[q], sol[t]);
+            }
+        }
+        for (int t = 0; t < m; t++) M[b + t] = (float)sol[t];
+    }
+    return bad ? ORC_ZERO_PIVOT : 0;
+}
+
+/* FP32 ISAI apply: out = ISAI(U) (ISAI(L) v), each row a CSR-order fmaf
+ * chain over the whole row with the coefficients of the kernel's slots
+ * (0 outside the factor's triangle, 1 on the ISAI(L) diagonal) */
+static void isai_apply_f(int n, const int *rp, const int *ci, const int *diag, const float *M,
+                         const float *v, float *tmp, float *out)
+{
+    for (int i = 0; i < n; i++) {
+        float s = 0.0f;
+        for (int p = rp[i]; p < rp[i + 1]; p++) {
+            const float c = p < diag[i] ? M[p] : (p == diag[i] ? 1.0f : 0.0f);
+            s = fmaf(c, v[ci[p]], s);
+        }
+        tmp[i] = s;
+    }
+    for (int i = 0; i < n; i++) {
+        float s = 0.0f;
+        for (int p = rp[i]; p < rp[i + 1]; p++) s = fmaf(p < diag[i] ? 0.0f : M[p], tmp[ci[p]], s);
+        out[i] = s;
+    }
+}
+
 int orc_bicgstab_mixed(int n, const int *rp, const int *ci, const int *diag,
                        const int *lev_l, const int *lev_u, const double *A, const double *b,
                        const float *F32, const orc_opts *o, double inner_tol, int max_refine,
                        double *x, float *iter_out, int *conv_out, double *iter_res_out,
                        double *true_res_out, int *refine_out)
+{
+    return orc_bicgstab_mixed_pc(n, rp, ci, diag, lev_l, lev_u, A, b, F32, 0, 0, o, inner_tol,
+                                 max_refine, x, iter_out, conv_out, iter_res_out, true_res_out,
+                                 refine_out);
+}
+
+/* isai = 0: P32 is the FP32 in-place LU factor (availability-order solves);
+ * isai = 1: P32 is the FP32 ISAI(L)/ISAI(U) array, zp its zero-pivot flag */
+int orc_bicgstab_mixed_pc(int n, const int *rp, const int *ci, const int *diag,
+                          const int *lev_l, const int *lev_u, const double *A, const double *b,
+                          const float *F32, int isai, int zp, const orc_opts *o, double inner_tol,
+                          int max_refine, double *x, float *iter_out, int *conv_out,
+                          double *iter_res_out, double *true_res_out, int *refine_out)
 {
     const int nnz = rp[n];
     float *Af = (float *)malloc(sizeof(float) * (size_t)nnz);
@@ -637,7 +718,8 @@ int orc_bicgstab_mixed(int n, const int *rp, const int *ci, const int *diag,
     float *t = fb + 5 * n, *ph = fb + 6 * n, *sh = fb + 7 * n, *d = fb + 8 * n, *tmp = fb + 9 * n;
     for (int q = 0; q < nnz; q++) Af[q] = (float)A[q];
     int bad = 0;
-    for (int i = 0; i < n; i++) bad |= (F32[diag[i]] == 0.0f);
+    if (isai) bad = zp;
+    else for (int i = 0; i < n; i++) bad |= (F32[diag[i]] == 0.0f);
     for (int i = 0; i < n; i++) { x[i] = 0.0; r[i] = b[i]; }
     const double bnorm = sqrt(orc_dot(n, r, r, 64));
     const double tol_abs = (o->tol_mode == ORC_TOL_RELATIVE) ? o->tol * bnorm : o->tol;
@@ -671,7 +753,8 @@ int orc_bicgstab_mixed(int n, const int *rp, const int *ci, const int *diag,
                 if (rho_new == 0.0f) break;
                 const float beta = (rho_new / rho_old) * (alpha / omega_old);
                 for (int i = 0; i < n; i++) p[i] = fmaf(beta, fmaf(-omega_old, v[i], p[i]), ri[i]);
-                lu_solve_avail_f(n, rp, ci, diag, lev_l, lev_u, F32, p, tmp, ph);
+                if (isai) isai_apply_f(n, rp, ci, diag, F32, p, tmp, ph);
+                else lu_solve_avail_f(n, rp, ci, diag, lev_l, lev_u, F32, p, tmp, ph);
                 spmvf(n, rp, ci, Af, ph, v);
                 alpha = rho_new / dotf64w(n, rh, v);
                 for (int i = 0; i < n; i++) d[i] = fmaf(alpha, ph[i], d[i]);
@@ -680,7 +763,8 @@ int orc_bicgstab_mixed(int n, const int *rp, const int *ci, const int *diag,
                 it += 0.5f;
                 if (!isfinite(ires)) { inner_bad = 1; break; }
                 if (ires < itol) break;
-                lu_solve_avail_f(n, rp, ci, diag, lev_l, lev_u, F32, s, tmp, sh);
+                if (isai) isai_apply_f(n, rp, ci, diag, F32, s, tmp, sh);
+                else lu_solve_avail_f(n, rp, ci, diag, lev_l, lev_u, F32, s, tmp, sh);
                 spmvf(n, rp, ci, Af, sh, t);
                 const float ts = dotf64w(n, t, s);
                 const float tt = dotf64w(n, t, t);
@@ -821,9 +905,10 @@ int orc_parilu0_stop(int n, const int *rp, const int *ci, const int *diag, const
 }
 
 /* whole-batch driver of the split factor + solve paths: precond 0 = exact
- * ILU(0), 1 = ParILU(0) (with init / stop_tol as orc_parilu0_stop);
- * precision 0 = FP64 BiCGSTAB (availability-order solves), 1 = FP32 inner /
- * FP64 refinement over the FP32-rounded factors. */
+ * ILU(0), 1 = ParILU(0) (with init / stop_tol as orc_parilu0_stop), 2 / 3 =
+ * the same followed by ISAI(L), ISAI(U) (precision 1 only); precision 0 =
+ * FP64 BiCGSTAB (availability-order solves), 1 = FP32 inner / FP64
+ * refinement over the FP32-rounded factors or the FP32 ISAI. */
 typedef struct {
     int n, nnz, b0, b1, precond, sweeps, precision, max_refine;
     double stop_tol, inner_tol;
@@ -845,7 +930,7 @@ static void *split_worker(void *arg)
         const double *b = j->b + (size_t)s * n;
         double *x = j->x + (size_t)s * n;
         int st = 0;
-        if (j->precond == 1) {
+        if (j->precond == 1 || j->precond == 3) {
             orc_parilu0_stop(n, j->rp, j->ci, j->diag, A, j->init ? j->init + (size_t)s * nnz : NULL, F,
                              j->sweeps, j->stop_tol, &st);
         } else {
@@ -853,7 +938,12 @@ static void *split_worker(void *arg)
             orc_ilu0(n, j->rp, j->ci, j->diag, F);
         }
         if (j->fstat) j->fstat[s] = st;
-        if (j->precision == 1) {
+        if (j->precision == 1 && j->precond >= 2) {
+            const int zp = orc_isai_lu_f32(n, j->rp, j->ci, j->diag, F, F32) != 0;
+            orc_bicgstab_mixed_pc(n, j->rp, j->ci, j->diag, j->lev, j->lev + n, A, b, F32, 1, zp, j->o,
+                                  j->inner_tol, j->max_refine, x, &j->iter[s], &j->conv[s],
+                                  &j->iter_res[s], &j->true_res[s], &j->refine[s]);
+        } else if (j->precision == 1) {
             for (int q = 0; q < nnz; q++) F32[q] = (float)F[q];
             orc_bicgstab_mixed(n, j->rp, j->ci, j->diag, j->lev, j->lev + n, A, b, F32, j->o,
                                j->inner_tol, j->max_refine, x, &j->iter[s], &j->conv[s],

@@ -108,6 +108,12 @@ int orc_bicgstab_mixed(int n, const int *rp, const int *ci, const int *diag,
                        const float *F32, const orc_opts *o, double inner_tol, int max_refine,
                        double *x, float *iter, int *conv, double *iter_res,
                        double *true_res, int *refine);
+int orc_bicgstab_mixed_pc(int n, const int *rp, const int *ci, const int *diag,
+                          const int *lev_l, const int *lev_u, const double *A, const double *b,
+                          const float *F32, int isai, int zp, const orc_opts *o, double inner_tol,
+                          int max_refine, double *x, float *iter, int *conv, double *iter_res,
+                          double *true_res, int *refine);
+int orc_isai_lu_f32(int n, const int *rp, const int *ci, const int *diag, const double *F, float *M);
 int orc_entry_order(int n, const int *rp, const int *ci, const int *diag, int *order);
 int orc_parilu0_stop(int n, const int *rp, const int *ci, const int *diag, const double *A,
                      const double *init, double *F, int sweeps, double stop_tol, int *status);

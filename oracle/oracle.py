@@ -87,6 +87,7 @@ def lib():
                                       C.c_int, C.c_int, C.POINTER(Opts), f32,
                                       P(np.int32, flags="C"), f64, f64, C.c_int]
         L.orc_entry_order.argtypes = [C.c_int, i32, i32, i32, i32]
+        L.orc_isai_lu_f32.argtypes = [C.c_int, i32, i32, i32, f64, f32]
         L.orc_parilu0_stop.argtypes = [C.c_int, i32, i32, i32, f64, C.c_void_p, f64, C.c_int,
                                        C.c_double, C.POINTER(C.c_int)]
         L.orc_batch_solve_split.argtypes = [C.c_int, C.c_int, i32, i32, C.c_int, f64, f64, f64,
@@ -240,10 +241,18 @@ def batch_solve_split(pat, vals, b, precond=ILU0, sweeps=5, init=None, stop_tol=
     fs = np.empty(B, dtype=np.int32)
     ini = _f64(init) if init is not None else None
     st = lib().orc_batch_solve_split(pat.n, pat.nnz, pat.row_ptrs, pat.col_idx, B, vals, b, x,
-                                     1 if precond == PARILU0 else 0, sweeps,
+                                     {ILU0: 0, PARILU0: 1, ILU0_ISAI: 2, PARILU0_ISAI: 3}[precond], sweeps,
                                      ini.ctypes.data if ini is not None else None, stop_tol,
                                      precision, inner_tol, max_refine, C.byref(o), it, cv, ir, tr,
                                      rf, fs, threads)
     if st != 0:
         raise RuntimeError(f"orc_batch_solve_split failed: {st}")
     return x, it, cv, ir, tr, rf, fs
+
+
+def isai_lu_f32(pat, F):
+    """FP32 ISAI(L) / ISAI(U) of the in-place factor F on one array; returns
+    (M, status)."""
+    M = np.empty(pat.nnz, np.float32)
+    st = lib().orc_isai_lu_f32(pat.n, pat.row_ptrs, pat.col_idx, pat.diag, _f64(F), M)
+    return M, st

@@ -40,7 +40,9 @@ def run_oracle(pat, A, b, precond=bilu.ILU0, parilu_sweeps=5, tol=1e-10, tol_mod
                true_res_restart=0, max_iter=500, **_):
     o = O.opts(tol=tol, tol_mode=tol_mode, max_iter=max_iter, true_res_restart=true_res_restart,
                trsv_order=O.TRSV_AVAIL)
-    return O.batch_solve_split(pat, A, b, precond=O.PARILU0 if precond == bilu.PARILU0 else O.ILU0,
+    pc = {bilu.ILU0: O.ILU0, bilu.PARILU0: O.PARILU0, bilu.ILU0_ISAI: O.ILU0_ISAI,
+          bilu.PARILU0_ISAI: O.PARILU0_ISAI}[precond]
+    return O.batch_solve_split(pat, A, b, precond=pc,
                                sweeps=parilu_sweeps, init=init, stop_tol=parilu_stop_tol,
                                precision=precision, inner_tol=inner_tol, max_refine=max_refine,
                                o=o, threads=8)
@@ -65,6 +67,41 @@ def test_mixed_bitwise(kind, B, n, inner_tol):
     o = run_oracle(pat, A, b, **kw)
     assert_same(g, o, f"mixed {kind} n={n}")
     assert (o[2] == 1).all()
+
+
+@pytest.mark.parametrize("kind,B,n", [("random", 256, 144), ("stencil5", 64, 100), ("random", 64, 32),
+                                      ("random", 64, 90)])
+@pytest.mark.parametrize("precond", [bilu.ILU0_ISAI, bilu.PARILU0_ISAI])
+def test_mixed_isai_bitwise(kind, B, n, precond):
+    print(f"case mixed isai {kind} B={B} n={n} precond={precond}", flush=True)
+    pat, A, b = synth.batch(kind, B, n=n)
+    kw = dict(precond=precond, tol=1e-10, tol_mode=bilu.TOL_RELATIVE, precision=bilu.PRECISION_FP32_IR)
+    g = run_gpu(pat, A, b, **kw)
+    o = run_oracle(pat, A, b, **kw)
+    assert_same(g, o, f"mixed isai {kind} n={n}")
+    assert (o[2] == 1).all()
+
+
+def test_mixed_isai_warm_parilu_bitwise():
+    print("case mixed isai warm parilu", flush=True)
+    pat, A, b = synth.batch("random", 128, n=144)
+    F = np.stack([O.ilu0(pat, A[s])[0] for s in range(A.shape[0])])
+    init = F * (1.0 + 1e-3 * np.random.default_rng(7).standard_normal(F.shape))
+    kw = dict(precond=bilu.PARILU0_ISAI, parilu_sweeps=40, parilu_stop_tol=1e-8, tol=1e-10,
+              tol_mode=bilu.TOL_RELATIVE, precision=bilu.PRECISION_FP32_IR)
+    assert_same(run_gpu(pat, A, b, F_init=init, **kw), run_oracle(pat, A, b, init=init, **kw),
+                "mixed isai warm parilu")
+
+
+def test_mixed_isai_zero_pivot():
+    print("case mixed isai zero pivot", flush=True)
+    pat, A, b = synth.batch("random", 4, n=64)
+    A = A.copy()
+    A[1][pat.diag[0]] = 0.0
+    kw = dict(precond=bilu.ILU0_ISAI, tol=1e-10, tol_mode=bilu.TOL_RELATIVE, precision=bilu.PRECISION_FP32_IR)
+    g = run_gpu(pat, A, b, **kw)
+    assert_same(g, run_oracle(pat, A, b, **kw), "mixed isai zero pivot")
+    assert g[2][1] == bilu.ZERO_PIVOT and (g[2][[0, 2, 3]] == 1).all()
 
 
 def test_mixed_parilu_bitwise():
@@ -120,7 +157,7 @@ def test_mixed_rejects_unsupported():
         run_gpu(pat, A, b, precision=bilu.PRECISION_FP32_IR, tol=1e-10, tol_mode=1)
     pat, A, b = synth.batch("random", 4, n=144)
     with pytest.raises(bilu.BiluError):
-        run_gpu(pat, A, b, precond=bilu.ILU0_ISAI, precision=bilu.PRECISION_FP32_IR)
+        run_gpu(pat, A, b, precond=bilu.GENERAL_ISAI, precision=bilu.PRECISION_FP32_IR)
 
 
 def test_mixed_full_batch_properties():

@@ -151,3 +151,70 @@ def test_mixed_exact_preconditioner_one_inner_step():
     x, it, cv, ir, tr, rf, fs = O.batch_solve_split(pat, A, b, precision=O.PREC_FP32_IR, o=o)
     assert (cv == 1).all()
     assert (it == 0.5 * rf).all()
+
+
+def test_isai_f32_matches_fp64_isai_and_identity():
+    # the FP32-storage sweep is the reference-order FP64 sweep (orc_isai) up
+    # to FP32 rounding, and satisfies the defining identity (M L)(i, j) =
+    # delta_ij on the pattern to FP32 accuracy
+    pat, A, _ = synth.batch("random", 2, n=144)
+    F, _ = O.ilu0(pat, A[0])
+    M, st = O.isai_lu_f32(pat, F)
+    assert st == 0
+    ML, _ = O.isai(pat, F, O.ISAI_LOWER)
+    MU, _ = O.isai(pat, F, O.ISAI_UPPER)
+    rp, diag = pat.row_ptrs, pat.diag
+    low = np.concatenate([np.arange(rp[i], diag[i]) for i in range(pat.n)])
+    up = np.concatenate([np.arange(diag[i], rp[i + 1]) for i in range(pat.n)])
+    assert np.allclose(M[low], ML[low], rtol=1e-4, atol=1e-6 * np.abs(ML[low]).max())
+    assert np.allclose(M[up], MU[up], rtol=1e-4, atol=1e-6 * np.abs(MU[up]).max())
+    Ld = np.eye(pat.n) + np.tril(pat.dense(F), -1)
+    Ud = np.triu(pat.dense(F))
+    MLd = np.eye(pat.n) + np.tril(pat.dense(M.astype(np.float64)), -1)
+    MUd = np.triu(pat.dense(M.astype(np.float64)))
+    mask = pat.dense(np.ones(pat.nnz)) != 0
+    EL = (MLd @ Ld - np.eye(pat.n))[mask & np.tri(pat.n, dtype=bool)]
+    EU = (MUd @ Ud - np.eye(pat.n))[mask & np.tri(pat.n, dtype=bool).T]
+    assert np.abs(EL).max() < 1e-5 and np.abs(EU).max() < 1e-5
+
+
+def test_isai_f32_exact_for_closed_patterns():
+    # tridiagonal: L is bidiagonal and the power-1 ISAI of a bidiagonal
+    # factor on its pattern reproduces the inverse's entries there
+    n = 12
+    rp, ci = [0], []
+    for i in range(n):
+        ci += [j for j in (i - 1, i, i + 1) if 0 <= j < n]
+        rp.append(len(ci))
+    pat = synth.Pattern(n, np.array(rp), np.array(ci))
+    rng = np.random.default_rng(1)
+    A = -rng.uniform(0.5, 1.5, pat.nnz)
+    A[pat.diag] = 4.0
+    F, _ = O.ilu0(pat, A)
+    M, st = O.isai_lu_f32(pat, F)
+    Linv = np.linalg.inv(np.eye(n) + np.tril(pat.dense(F), -1))
+    Uinv = np.linalg.inv(np.triu(pat.dense(F)))
+    for i in range(n):
+        for p in range(rp[i], rp[i + 1]):
+            j = ci[p]
+            want = Linv[i, j] if j < i else Uinv[i, j]
+            assert M[p] == pytest.approx(want, rel=2e-7, abs=1e-7)
+
+
+def test_isai_f32_zero_pivot():
+    pat, A, _ = synth.batch("random", 1, n=40)
+    F, _ = O.ilu0(pat, A[0])
+    F[pat.diag[7]] = 0.0
+    _, st = O.isai_lu_f32(pat, F)
+    assert st == O.ZERO_PIVOT
+
+
+def test_mixed_isai_converges():
+    pat, A, b = synth.batch("random", 32, n=144)
+    o = O.opts(tol=1e-10, tol_mode=O.TOL_RELATIVE)
+    for pc in (O.ILU0_ISAI, O.PARILU0_ISAI):
+        x, it, cv, ir, tr, rf, fs = O.batch_solve_split(pat, A, b, precond=pc, precision=O.PREC_FP32_IR,
+                                                        o=o, threads=8)
+        assert (cv == 1).all()
+        for s in range(0, 32, 5):
+            assert relres(pat, A[s], b[s], x[s]) <= 1e-10
