@@ -164,6 +164,32 @@ int bilu_isai(bilu_pattern_t p, int batch, const double *d_vals, int type, doubl
 int bilu_true_residual(bilu_pattern_t p, int batch, const double *d_A, const double *d_b,
                        const double *d_x, double *d_res, void *stream);
 
+/* ---- learned warm start of ParILU(0) (no counterpart in the reference) ----
+ * An MLP (features asinh(a/1e-8)/24 of the nnz values of A, two hidden
+ * layers of `hidden` GELU units, nnz outputs) predicts the in-place ILU(0)
+ * factor; its output feeds bilu_solve_ex's d_F_init.  BF16 MFMA GEMMs with
+ * FP32 accumulation; FP32 master weights trained with Adam on exact ILU(0)
+ * factors (e.g. from bilu_ilu0).  Weights are exchanged in the packed FP32
+ * layout W1[hidden][nnz], b1[hidden], W2[hidden][hidden], b2[hidden],
+ * W3[nnz][hidden], b3[nnz]. */
+typedef struct bilu_mlp_s *bilu_mlp_t;
+
+int bilu_mlp_create(bilu_pattern_t p, int hidden, unsigned long long seed, bilu_mlp_t *out);
+int bilu_mlp_destroy(bilu_mlp_t m);
+int bilu_mlp_num_params(bilu_mlp_t m, long long *count);
+int bilu_mlp_get_weights(bilu_mlp_t m, float *d_w, void *stream);
+int bilu_mlp_set_weights(bilu_mlp_t m, const float *d_w, void *stream);
+/* mean squared error of the scaled factors over the minibatch (d_loss, one
+ * device float, may be NULL) and its gradient (d_grad[num_params], packed
+ * layout, may be NULL), without updating the weights */
+int bilu_mlp_gradient(bilu_mlp_t m, int batch, const double *d_A, const double *d_F, float *d_loss,
+                      float *d_grad, void *stream);
+/* one Adam step (lr) on the minibatch d_A[batch*nnz] with exact factors d_F */
+int bilu_mlp_train_step(bilu_mlp_t m, int batch, const double *d_A, const double *d_F, float lr,
+                        float *d_loss, void *stream);
+/* predicted factors d_F_init[batch*nnz] (FP64, in-place factor layout) */
+int bilu_mlp_infer(bilu_mlp_t m, int batch, const double *d_A, double *d_F_init, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

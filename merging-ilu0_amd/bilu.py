@@ -29,7 +29,9 @@ EXPORTS = [
     "bilu_opts_default", "bilu_version", "bilu_status_string", "bilu_pattern_create",
     "bilu_pattern_destroy", "bilu_pattern_get_info", "bilu_pattern_lu_split", "bilu_solve",
     "bilu_solve_ex", "bilu_ilu0", "bilu_parilu0", "bilu_factor_residual", "bilu_split_lu", "bilu_isai",
-    "bilu_true_residual",
+    "bilu_true_residual", "bilu_mlp_create", "bilu_mlp_destroy", "bilu_mlp_num_params",
+    "bilu_mlp_get_weights", "bilu_mlp_set_weights", "bilu_mlp_gradient", "bilu_mlp_train_step",
+    "bilu_mlp_infer",
 ]
 
 
@@ -89,6 +91,14 @@ def lib():
         L.bilu_split_lu.argtypes = [vp, C.c_int, vp, vp, vp, vp]
         L.bilu_isai.argtypes = [vp, C.c_int, vp, C.c_int, vp, vp, vp]
         L.bilu_true_residual.argtypes = [vp, C.c_int, vp, vp, vp, vp, vp]
+        L.bilu_mlp_create.argtypes = [vp, C.c_int, C.c_ulonglong, C.POINTER(vp)]
+        L.bilu_mlp_destroy.argtypes = [vp]
+        L.bilu_mlp_num_params.argtypes = [vp, C.POINTER(C.c_longlong)]
+        L.bilu_mlp_get_weights.argtypes = [vp, vp, vp]
+        L.bilu_mlp_set_weights.argtypes = [vp, vp, vp]
+        L.bilu_mlp_gradient.argtypes = [vp, C.c_int, vp, vp, vp, vp, vp]
+        L.bilu_mlp_train_step.argtypes = [vp, C.c_int, vp, vp, C.c_float, vp, vp]
+        L.bilu_mlp_infer.argtypes = [vp, C.c_int, vp, vp, vp]
         _lib = L
     return _lib
 
@@ -235,3 +245,59 @@ def isai(pat: Pattern, vals, typ, out, status=None, stream=None):
 def true_residual(pat: Pattern, A, b, x, res, stream=None):
     _check(lib().bilu_true_residual(pat.h, A.shape[0], _ptr(A), _ptr(b), _ptr(x), _ptr(res),
                                     _stream(stream)), "bilu_true_residual")
+
+
+class Mlp:
+    """Learned warm start of ParILU(0) (bilu_mlp_*): predicts in-place ILU(0)
+    factors from A; trained on exact factors with the library's BF16 MFMA
+    kernels."""
+
+    def __init__(self, pat: Pattern, hidden=512, seed=1):
+        h = C.c_void_p()
+        _check(lib().bilu_mlp_create(pat.h, hidden, seed, C.byref(h)), "bilu_mlp_create")
+        self.h = h
+        self.nnz = pat.nnz
+        self.hidden = hidden
+        n = C.c_longlong()
+        _check(lib().bilu_mlp_num_params(self.h, C.byref(n)), "bilu_mlp_num_params")
+        self.num_params = int(n.value)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().bilu_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def get_weights(self, stream=None):
+        import torch
+        w = torch.empty(self.num_params, dtype=torch.float32, device="cuda")
+        _check(lib().bilu_mlp_get_weights(self.h, _ptr(w), _stream(stream)), "bilu_mlp_get_weights")
+        return w
+
+    def set_weights(self, w, stream=None):
+        _check(lib().bilu_mlp_set_weights(self.h, _ptr(w), _stream(stream)), "bilu_mlp_set_weights")
+
+    def gradient(self, A, F, stream=None):
+        import torch
+        loss = torch.empty(1, dtype=torch.float32, device=A.device)
+        g = torch.empty(self.num_params, dtype=torch.float32, device=A.device)
+        _check(lib().bilu_mlp_gradient(self.h, A.shape[0], _ptr(A), _ptr(F), _ptr(loss), _ptr(g),
+                                       _stream(stream)), "bilu_mlp_gradient")
+        return loss, g
+
+    def train_step(self, A, F, lr, loss=None, stream=None):
+        _check(lib().bilu_mlp_train_step(self.h, A.shape[0], _ptr(A), _ptr(F), float(lr), _ptr(loss),
+                                         _stream(stream)), "bilu_mlp_train_step")
+
+    def infer(self, A, out=None, stream=None):
+        import torch
+        if out is None:
+            out = torch.empty_like(A)
+        _check(lib().bilu_mlp_infer(self.h, A.shape[0], _ptr(A), _ptr(out), _stream(stream)),
+               "bilu_mlp_infer")
+        return out

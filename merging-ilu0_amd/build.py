@@ -20,7 +20,7 @@ HIP_FLAGS = ["-O3", "-std=c++17", "--offload-arch=gfx950", "-fPIC", "-ffp-contra
 
 
 def build(verbose: bool = False, extra=()) -> str:
-    srcs = [os.path.join(CSRC, "solver.hip"), os.path.join(CSRC, "pattern.cpp")]
+    srcs = [os.path.join(CSRC, "solver.hip"), os.path.join(CSRC, "mlp.hip"), os.path.join(CSRC, "pattern.cpp")]
     cmd = [HIPCC, *HIP_FLAGS, *extra, "-shared", *srcs, "-o", LIB + ".tmp"]
     if verbose:
         print(" ".join(cmd), flush=True)

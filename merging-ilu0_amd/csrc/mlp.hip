@@ -1,0 +1,613 @@
+// mlp.hip -- the learned warm start of ParILU(0): an MLP maps the scaled
+// values of A (one system = one row of features on the pattern) to the
+// values of its in-place ILU(0) factor, with BF16 MFMA GEMMs (FP32
+// accumulation) for training and inference on gfx950.
+//
+//   x_e = asinh(a_e / SIGMA) / YS           (features, BF16)
+//   h1 = gelu(x W1^T + b1), h2 = gelu(h1 W2^T + b2)   (hidden 512, BF16)
+//   y = h2 W3^T + b3,  F_guess_e = SIGMA sinh(YS (base_e + y_e))  (FP64 out)
+// with the base guess base_e = asinh(s_e / SIGMA) / YS, s_e = a_ij for
+// (i, j) on or above the diagonal and a_ij / a_jj below it (the first
+// Jacobi guess of the factor): the network learns the correction, which
+// carries far less information than the factor itself.  Trained on exact
+// ILU(0) factors F with the mean squared error of y against t_e - base_e,
+// t_e = asinh(F_e / SIGMA) / YS (sign-preserving log scale: errors in y are
+// relative errors in F).  Weights are kept in FP32 (master, Adam moments)
+// and copied to BF16 for the GEMMs.
+//
+// GEMM: C[m][n] = sum_k A(m, k) B(k, n), A(m, k) = TA ? A[k lda + m] :
+// A[m lda + k], B(k, n) = TB ? B[n ldb + k] : B[k ldb + n], all dimensions
+// multiples of the tile (buffers are padded).  256 threads = 4 waves, tile
+// 128 x 128 x 64: each wave owns a 64 x 64 quarter as 4 x 4 MFMA
+// 16x16x32 blocks; the next k tile's global loads are issued before the
+// MFMAs of the current one.  Operands are staged in LDS as As[m][k], Bs[n][k] (k
+// contiguous, so each lane's MFMA fragment -- 8 consecutive k -- is one
+// 16-byte read); a transposed global layout is transposed on the way in.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include "../../include/bilu.h"
+
+namespace {
+
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(4 * sizeof(float)))) float floatx4;
+
+constexpr int TM = 128, TN = 128, TK = 64, KP = TK + 8;   // LDS row pitch (bf16), padded
+constexpr double SIGMA = 1e-8;
+constexpr float YS = 24.0f;
+
+enum Epi { EPI_F32 = 0, EPI_BIAS_GELU = 1, EPI_BIAS = 2, EPI_GELU_BWD = 3 };
+
+__device__ __forceinline__ float gelu(float z)
+{
+    const float u = 0.7978845608028654f * (z + 0.044715f * z * z * z);
+    return 0.5f * z * (1.0f + tanhf(u));
+}
+
+__device__ __forceinline__ float gelu_grad(float z)
+{
+    const float u = 0.7978845608028654f * (z + 0.044715f * z * z * z);
+    const float t = tanhf(u);
+    const float du = 0.7978845608028654f * (1.0f + 3.0f * 0.044715f * z * z);
+    return 0.5f * (1.0f + t) + 0.5f * z * (1.0f - t * t) * du;
+}
+
+// EPI_F32: C = acc; EPI_BIAS: C = acc + bias[n]; EPI_BIAS_GELU: Z = acc +
+// bias[n] (FP32, kept for the backward pass), H = gelu(Z) (BF16);
+// EPI_GELU_BWD: H = acc * gelu'(Z[m][n]) (BF16)
+template <bool TA, bool TB, int EPI>
+__global__ __launch_bounds__(256) void k_gemm(const __bf16 *__restrict__ A, int lda, const __bf16 *__restrict__ B,
+                                              int ldb, int K, float *__restrict__ C, int ldc,
+                                              const float *__restrict__ bias, float *__restrict__ Z,
+                                              __bf16 *__restrict__ H)
+{
+    __shared__ __align__(16) __bf16 As[TM * KP];
+    __shared__ __align__(16) __bf16 Bs[TN * KP];
+    const int t = threadIdx.x, lane = t % 64, wave = t / 64;
+    const int m0 = blockIdx.y * TM, n0 = blockIdx.x * TN;
+    const int wm = (wave / 2) * 64, wn = (wave % 2) * 64;
+    floatx4 acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+
+    // a tile of one operand is 128 x TK bf16 = TK * 16 vectors of 8, VPT per thread
+    constexpr int VPT = TK * 16 / 256;
+    constexpr int KV = TK / 8;   // vectors along k in a row
+    auto gload = [&](const __bf16 *P, int ld, bool trans, int r0, int k0, bf16x8 (&g)[VPT]) {
+#pragma unroll
+        for (int v = 0; v < VPT; v++) {
+            const int id = t + 256 * v;
+            if (!trans) {   // rows r (128), KV vectors of 8 k each
+                const int r = id / KV, c = (id % KV) * 8;
+                g[v] = *reinterpret_cast<const bf16x8 *>(&P[(size_t)(r0 + r) * ld + k0 + c]);
+            } else {        // rows k (TK), 16 vectors of 8 r each
+                const int kk = id / 16, c = (id % 16) * 8;
+                g[v] = *reinterpret_cast<const bf16x8 *>(&P[(size_t)(k0 + kk) * ld + r0 + c]);
+            }
+        }
+    };
+    auto sstore = [&](__bf16 *S, bool trans, const bf16x8 (&g)[VPT]) {
+#pragma unroll
+        for (int v = 0; v < VPT; v++) {
+            const int id = t + 256 * v;
+            if (!trans) {
+                const int r = id / KV, c = (id % KV) * 8;
+                *reinterpret_cast<bf16x8 *>(&S[r * KP + c]) = g[v];
+            } else {        // transpose into S[r][k]
+                const int kk = id / 16, c = (id % 16) * 8;
+#pragma unroll
+                for (int j = 0; j < 8; j++) S[(c + j) * KP + kk] = g[v][j];
+            }
+        }
+    };
+    bf16x8 ga[VPT], gb[VPT];
+    gload(A, lda, TA, m0, 0, ga);
+    gload(B, ldb, !TB, n0, 0, gb);
+    for (int k0 = 0; k0 < K; k0 += TK) {
+        sstore(As, TA, ga);
+        sstore(Bs, !TB, gb);
+        __syncthreads();
+        // the next tile's global loads are in flight while this one is multiplied
+        if (k0 + TK < K) {
+            gload(A, lda, TA, m0, k0 + TK, ga);
+            gload(B, ldb, !TB, n0, k0 + TK, gb);
+        }
+#pragma unroll
+        for (int kq = 0; kq < TK; kq += 32) {
+            bf16x8 af[4], bfr[4];
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+                af[i] = *reinterpret_cast<const bf16x8 *>(&As[(wm + 16 * i + lane % 16) * KP + kq + 8 * (lane / 16)]);
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+                bfr[j] = *reinterpret_cast<const bf16x8 *>(&Bs[(wn + 16 * j + lane % 16) * KP + kq + 8 * (lane / 16)]);
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+#pragma unroll
+                for (int j = 0; j < 4; j++)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    // lane holds D[m = 4 (lane / 16) + r][n = lane % 16] of each 16 x 16 block
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int m = m0 + wm + 16 * i + 4 * (lane / 16) + r;
+                const int n = n0 + wn + 16 * j + lane % 16;
+                const float v = acc[i][j][r];
+                const size_t o = (size_t)m * ldc + n;
+                if (EPI == EPI_F32) {
+                    C[o] = v;
+                } else if (EPI == EPI_BIAS) {
+                    C[o] = v + bias[n];
+                } else if (EPI == EPI_BIAS_GELU) {
+                    const float z = v + bias[n];
+                    Z[o] = z;
+                    H[o] = (__bf16)gelu(z);
+                } else {
+                    H[o] = (__bf16)(v * gelu_grad(Z[o]));
+                }
+            }
+}
+
+// features: X[s][e] = asinh(a / SIGMA) / YS (BF16), padding columns 0
+__global__ void k_features(const double *__restrict__ A, int nnz, int P0, int batch, __bf16 *__restrict__ X)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)batch * P0) return;
+    const int s = int(i / P0), e = int(i % P0);
+    const float v = (e < nnz) ? asinhf((float)(A[(size_t)s * nnz + e] / SIGMA)) / YS : 0.0f;
+    X[i] = (__bf16)v;
+}
+
+// base guess of entry e of system s (piv[e] = position of a_jj for a lower
+// entry (i, j), -1 otherwise), in the scaled space (FP32: the warm start is
+// a guess; scaled values carry relative accuracy ~1e-7)
+__device__ __forceinline__ float base_guess(const double *__restrict__ As, const int *__restrict__ piv, int e)
+{
+    const int pv = piv[e];
+    const double sv = pv >= 0 ? As[e] / As[pv] : As[e];
+    return asinhf((float)(sv / SIGMA)) / YS;
+}
+
+// loss gradient dY = 2 (y - (t - base)) / (batch nnz) (BF16), t from the exact
+// factors; per-block partial sums of the squared error into part[]
+__global__ void k_loss_grad(const float *__restrict__ Y, const double *__restrict__ F, const double *__restrict__ A,
+                            const int *__restrict__ piv, int nnz, int P3,
+                            int batch, __bf16 *__restrict__ dY, float *__restrict__ part)
+{
+    __shared__ float red[256];
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    float sq = 0.0f;
+    if (i < (size_t)batch * P3) {
+        const int s = int(i / P3), e = int(i % P3);
+        float g = 0.0f;
+        if (e < nnz) {
+            const float tv = asinhf((float)(F[(size_t)s * nnz + e] / SIGMA)) / YS -
+                             base_guess(A + (size_t)s * nnz, piv, e);
+            const float d = Y[i] - tv;
+            sq = d * d;
+            g = 2.0f * d / ((float)batch * (float)nnz);
+        }
+        dY[i] = (__bf16)g;
+    }
+    red[threadIdx.x] = sq;
+    __syncthreads();
+    for (int o = 128; o > 0; o /= 2) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+__global__ void k_loss_final(const float *__restrict__ part, int nparts, float scale, float *__restrict__ loss)
+{
+    __shared__ float red[256];
+    float s = 0.0f;
+    for (int i = threadIdx.x; i < nparts; i += 256) s += part[i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o /= 2) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *loss = red[0] * scale;
+}
+
+// bias gradient g[n] = sum_m D[m][n] over BF16 rows, in two passes: block
+// (x, y) sums rows y, y + CS, ... of 256 columns into part[y][n], then the CS
+// partial sums of each column are added in order (deterministic)
+constexpr int CS = 32;
+__global__ void k_colsum_part(const __bf16 *__restrict__ D, int M, int N, float *__restrict__ part)
+{
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    float s = 0.0f;
+    for (int m = blockIdx.y; m < M; m += CS) s += (float)D[(size_t)m * N + n];
+    part[(size_t)blockIdx.y * N + n] = s;
+}
+
+__global__ void k_colsum_final(const float *__restrict__ part, int N, float *__restrict__ g)
+{
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    float s = 0.0f;
+    for (int y = 0; y < CS; y++) s += part[(size_t)y * N + n];
+    g[n] = s;
+}
+
+// output: F_guess = SIGMA sinh(YS (base + y)) (FP64, unpadded)
+__global__ void k_outputs(const float *__restrict__ Y, const double *__restrict__ A, const int *__restrict__ piv,
+                          int nnz, int P3, int batch, double *__restrict__ Fo)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)batch * nnz) return;
+    const int s = int(i / nnz), e = int(i % nnz);
+    const float yb = base_guess(A + (size_t)s * nnz, piv, e) + Y[(size_t)s * P3 + e];
+    Fo[i] = SIGMA * (double)sinhf(YS * yb);
+}
+
+// Adam step on FP32 master weights w (moments m, v), BF16 copy wb
+__global__ void k_adam(float *__restrict__ w, const float *__restrict__ g, float *__restrict__ m,
+                       float *__restrict__ v, __bf16 *__restrict__ wb, size_t n, float lr, float b1t, float b2t)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.0f - b1) * gi;
+    const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    const float mh = mi / (1.0f - b1t), vh = vi / (1.0f - b2t);
+    const float wi = w[i] - lr * mh / (sqrtf(vh) + eps);
+    w[i] = wi;
+    if (wb) wb[i] = (__bf16)wi;
+}
+
+__global__ void k_to_bf16(const float *__restrict__ w, __bf16 *__restrict__ wb, size_t n)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) wb[i] = (__bf16)w[i];
+}
+
+// [rows][cols] FP32 block copy between a packed and a padded (pitch) layout
+__global__ void k_copy2d(const float *__restrict__ src, int sp, float *__restrict__ dst, int dp, int rows, int cols)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)rows * cols) return;
+    const int r = int(i / cols), c = int(i % cols);
+    dst[(size_t)r * dp + c] = src[(size_t)r * sp + c];
+}
+
+int pad_to(int x, int q) { return (x + q - 1) / q * q; }
+
+unsigned blocks(size_t n, unsigned b = 256) { return unsigned((n + b - 1) / b); }
+
+}  // namespace
+
+// ---------------------------------------------------------------- handle
+struct bilu_mlp_s {
+    int nnz = 0, P0 = 0, H = 0, P3 = 0;
+    int device = 0;
+    long step = 0;
+    // parameters, FP32, one allocation: W1[H][P0], b1[H], W2[H][H], b2[H], W3[P3][H], b3[P3]
+    size_t off[7] = {};
+    float *w = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;
+    __bf16 *wb = nullptr;
+    // activations for a minibatch of up to cap rows
+    int cap = 0;
+    __bf16 *X = nullptr, *H1 = nullptr, *H2 = nullptr, *dY = nullptr, *dZ2 = nullptr, *dZ1 = nullptr;
+    float *Z1 = nullptr, *Z2 = nullptr, *Y = nullptr, *part = nullptr, *cpart = nullptr;
+    int *piv = nullptr;   // [nnz]: position of a_jj for lower entries (i, j), else -1
+};
+
+namespace {
+
+#define MLP_TRY(x)                                  \
+    do {                                            \
+        if ((x) != hipSuccess) return BILU_ERR_HIP; \
+    } while (0)
+
+size_t nparams(const bilu_mlp_s &q) { return q.off[6]; }
+
+void free_acts(bilu_mlp_s &q)
+{
+    for (void *p : {(void *)q.X, (void *)q.H1, (void *)q.H2, (void *)q.dY, (void *)q.dZ2, (void *)q.dZ1,
+                    (void *)q.Z1, (void *)q.Z2, (void *)q.Y, (void *)q.part, (void *)q.cpart})
+        if (p) (void)hipFree(p);
+    q.X = q.H1 = q.H2 = q.dY = q.dZ2 = q.dZ1 = nullptr;
+    q.Z1 = q.Z2 = q.Y = q.part = q.cpart = nullptr;
+    q.cap = 0;
+}
+
+// activations for `rows` (a multiple of the tile) rows
+int reserve(bilu_mlp_s &q, int rows)
+{
+    if (rows <= q.cap) return BILU_OK;
+    MLP_TRY(hipDeviceSynchronize());
+    free_acts(q);
+    const size_t r = (size_t)rows;
+    MLP_TRY(hipMalloc((void **)&q.X, r * q.P0 * 2));
+    MLP_TRY(hipMalloc((void **)&q.H1, r * q.H * 2));
+    MLP_TRY(hipMalloc((void **)&q.H2, r * q.H * 2));
+    MLP_TRY(hipMalloc((void **)&q.dY, r * q.P3 * 2));
+    MLP_TRY(hipMalloc((void **)&q.dZ2, r * q.H * 2));
+    MLP_TRY(hipMalloc((void **)&q.dZ1, r * q.H * 2));
+    MLP_TRY(hipMalloc((void **)&q.Z1, r * q.H * 4));
+    MLP_TRY(hipMalloc((void **)&q.Z2, r * q.H * 4));
+    MLP_TRY(hipMalloc((void **)&q.Y, r * q.P3 * 4));
+    MLP_TRY(hipMalloc((void **)&q.part, (r * q.P3 / 256 + 1) * 4));
+    MLP_TRY(hipMalloc((void **)&q.cpart, (size_t)CS * std::max(q.P3, q.H) * 4));
+    q.cap = rows;
+    return BILU_OK;
+}
+
+template <bool TA, bool TB, int EPI>
+void gemm(hipStream_t st, int M, int N, int K, const __bf16 *A, int lda, const __bf16 *B, int ldb, float *C,
+          int ldc, const float *bias = nullptr, float *Z = nullptr, __bf16 *H = nullptr)
+{
+    hipLaunchKernelGGL((k_gemm<TA, TB, EPI>), dim3(N / TN, M / TM), dim3(256), 0, st, A, lda, B, ldb, K, C, ldc,
+                       bias, Z, H);
+}
+
+void colsum(hipStream_t st, const __bf16 *D, int M, int N, float *part, float *g)
+{
+    hipLaunchKernelGGL(k_colsum_part, dim3(blocks(N), CS), dim3(256), 0, st, D, M, N, part);
+    hipLaunchKernelGGL(k_colsum_final, dim3(blocks(N)), dim3(256), 0, st, part, N, g);
+}
+
+// forward of `rows` padded rows (features already in X): Y
+void forward(bilu_mlp_s &q, int rows, hipStream_t st)
+{
+    const __bf16 *W1 = q.wb + q.off[0], *W2 = q.wb + q.off[2], *W3 = q.wb + q.off[4];
+    const float *b1 = q.w + q.off[1], *b2 = q.w + q.off[3], *b3 = q.w + q.off[5];
+    gemm<false, true, EPI_BIAS_GELU>(st, rows, q.H, q.P0, q.X, q.P0, W1, q.P0, nullptr, q.H, b1, q.Z1, q.H1);
+    gemm<false, true, EPI_BIAS_GELU>(st, rows, q.H, q.H, q.H1, q.H, W2, q.H, nullptr, q.H, b2, q.Z2, q.H2);
+    gemm<false, true, EPI_BIAS>(st, rows, q.P3, q.H, q.H2, q.H, W3, q.H, q.Y, q.P3, b3);
+}
+
+// loss and gradient into q.g for `batch` systems (rows padded with zero features
+// and zero loss gradient)
+int loss_and_grad(bilu_mlp_s &q, int batch, const double *A, const double *F, float *d_loss, hipStream_t st)
+{
+    const int rows = pad_to(batch, TM);
+    int rc = reserve(q, rows);
+    if (rc != BILU_OK) return rc;
+    MLP_TRY(hipMemsetAsync(q.X, 0, (size_t)rows * q.P0 * 2, st));
+    hipLaunchKernelGGL(k_features, dim3(blocks((size_t)batch * q.P0)), dim3(256), 0, st, A, q.nnz, q.P0, batch, q.X);
+    forward(q, rows, st);
+    MLP_TRY(hipMemsetAsync(q.dY, 0, (size_t)rows * q.P3 * 2, st));
+    const unsigned nb = blocks((size_t)batch * q.P3);
+    hipLaunchKernelGGL(k_loss_grad, dim3(nb), dim3(256), 0, st, q.Y, F, A, q.piv, q.nnz, q.P3, batch, q.dY, q.part);
+    if (d_loss)
+        hipLaunchKernelGGL(k_loss_final, dim3(1), dim3(256), 0, st, q.part, (int)nb,
+                           1.0f / ((float)batch * (float)q.nnz), d_loss);
+    float *gW1 = q.g + q.off[0], *gb1 = q.g + q.off[1], *gW2 = q.g + q.off[2], *gb2 = q.g + q.off[3];
+    float *gW3 = q.g + q.off[4], *gb3 = q.g + q.off[5];
+    const __bf16 *W2 = q.wb + q.off[2], *W3 = q.wb + q.off[4];
+    // layer 3: gW3[n][k] = sum_m dY[m][n] H2[m][k]; gb3 = colsum dY; dZ2 = (dY W3) * gelu'(Z2)
+    gemm<true, false, EPI_F32>(st, q.P3, q.H, rows, q.dY, q.P3, q.H2, q.H, gW3, q.H);
+    colsum(st, q.dY, rows, q.P3, q.cpart, gb3);
+    gemm<false, false, EPI_GELU_BWD>(st, rows, q.H, q.P3, q.dY, q.P3, W3, q.H, nullptr, q.H, nullptr, q.Z2, q.dZ2);
+    // layer 2
+    gemm<true, false, EPI_F32>(st, q.H, q.H, rows, q.dZ2, q.H, q.H1, q.H, gW2, q.H);
+    colsum(st, q.dZ2, rows, q.H, q.cpart, gb2);
+    gemm<false, false, EPI_GELU_BWD>(st, rows, q.H, q.H, q.dZ2, q.H, W2, q.H, nullptr, q.H, nullptr, q.Z1, q.dZ1);
+    // layer 1
+    gemm<true, false, EPI_F32>(st, q.H, q.P0, rows, q.dZ1, q.H, q.X, q.P0, gW1, q.P0);
+    colsum(st, q.dZ1, rows, q.H, q.cpart, gb1);
+    MLP_TRY(hipGetLastError());
+    return BILU_OK;
+}
+
+// packed (exported) layout: W1[H][nnz], b1[H], W2[H][H], b2[H], W3[nnz][H], b3[nnz]
+size_t packed_count(const bilu_mlp_s &q)
+{
+    return (size_t)q.H * q.nnz + q.H + (size_t)q.H * q.H + q.H + (size_t)q.nnz * q.H + q.nnz;
+}
+
+// copy between the packed layout and the padded internal one
+void copy_params(const bilu_mlp_s &q, const float *src, float *dst, bool to_packed, hipStream_t st)
+{
+    struct Part { size_t ioff, poff; int rows, cols, ipitch; };
+    const size_t p1 = 0, p2 = p1 + (size_t)q.H * q.nnz, p3 = p2 + q.H, p4 = p3 + (size_t)q.H * q.H, p5 = p4 + q.H,
+                 p6 = p5 + (size_t)q.nnz * q.H;
+    const Part parts[6] = {{q.off[0], p1, q.H, q.nnz, q.P0}, {q.off[1], p2, 1, q.H, q.H},
+                           {q.off[2], p3, q.H, q.H, q.H},    {q.off[3], p4, 1, q.H, q.H},
+                           {q.off[4], p5, q.nnz, q.H, q.H},  {q.off[5], p6, 1, q.nnz, q.P3}};
+    for (const auto &pt : parts) {
+        const size_t n = (size_t)pt.rows * pt.cols;
+        if (to_packed)
+            hipLaunchKernelGGL(k_copy2d, dim3(blocks(n)), dim3(256), 0, st, src + pt.ioff, pt.ipitch,
+                               dst + pt.poff, pt.cols, pt.rows, pt.cols);
+        else
+            hipLaunchKernelGGL(k_copy2d, dim3(blocks(n)), dim3(256), 0, st, src + pt.poff, pt.cols,
+                               dst + pt.ioff, pt.ipitch, pt.rows, pt.cols);
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int bilu_mlp_create(bilu_pattern_t p, int hidden, unsigned long long seed, bilu_mlp_t *out)
+{
+    if (!p || !out || hidden <= 0 || hidden % TN) return BILU_ERR_INVALID_ARG;
+    bilu_pattern_info info;
+    int rc = bilu_pattern_get_info(p, &info);
+    if (rc != BILU_OK) return rc;
+    auto *q = new bilu_mlp_s();
+    (void)hipGetDevice(&q->device);
+    // pivot positions of the lower entries (base guess), from the reference L/U split
+    std::vector<int> lrp(info.n + 1), lci(info.l_nnz), urp(info.n + 1), uci(info.u_nnz);
+    rc = bilu_pattern_lu_split(p, lrp.data(), lci.data(), urp.data(), uci.data());
+    if (rc != BILU_OK) { delete q; return rc; }
+    std::vector<int> piv(info.nnz, -1), dpos(info.n, -1);
+    {
+        // pattern position of (i, j): row i's entries are L's strict part then U's
+        int e = 0;
+        std::vector<int> rowof(info.nnz), colof(info.nnz);
+        for (int i = 0; i < info.n; i++) {
+            for (int k = lrp[i]; k < lrp[i + 1]; k++)
+                if (lci[k] != i) { rowof[e] = i; colof[e] = lci[k]; e++; }
+            for (int k = urp[i]; k < urp[i + 1]; k++) {
+                if (uci[k] == i) dpos[i] = e;
+                rowof[e] = i; colof[e] = uci[k]; e++;
+            }
+        }
+        for (int x = 0; x < info.nnz; x++)
+            if (colof[x] < rowof[x]) piv[x] = dpos[colof[x]];
+    }
+    q->nnz = info.nnz;
+    q->P0 = pad_to(info.nnz, TN);
+    q->P3 = pad_to(info.nnz, TN);
+    q->H = hidden;
+    size_t o = 0;
+    const size_t sizes[6] = {(size_t)q->H * q->P0, (size_t)q->H, (size_t)q->H * q->H, (size_t)q->H,
+                             (size_t)q->P3 * q->H, (size_t)q->P3};
+    for (int i = 0; i < 6; i++) { q->off[i] = o; o += (sizes[i] + 63) / 64 * 64; }
+    q->off[6] = o;
+    // Glorot-uniform hidden weights, zero biases and a zero output layer (the
+    // untrained network returns the base guess); splitmix64 stream from the seed
+    std::vector<float> hw(o, 0.0f);
+    uint64_t z = seed;
+    auto next = [&]() {
+        z += 0x9E3779B97F4A7C15ull;
+        uint64_t r = z;
+        r = (r ^ (r >> 30)) * 0xBF58476D1CE4E5B9ull;
+        r = (r ^ (r >> 27)) * 0x94D049BB133111EBull;
+        r ^= r >> 31;
+        return (double)(r >> 11) * (1.0 / 9007199254740992.0);
+    };
+    const int fan[3][2] = {{q->nnz, q->H}, {q->H, q->H}, {q->H, q->nnz}};
+    const int pitch[3] = {q->P0, q->H, q->H}, rowsn[3] = {q->H, q->H, q->nnz};
+    for (int l = 0; l < 2; l++) {
+        const double lim = std::sqrt(6.0 / (fan[l][0] + fan[l][1]));
+        for (int r = 0; r < rowsn[l]; r++)
+            for (int c = 0; c < fan[l][0]; c++) hw[q->off[2 * l] + (size_t)r * pitch[l] + c] = (float)((2.0 * next() - 1.0) * lim);
+    }
+    if (hipMalloc((void **)&q->w, o * 4) != hipSuccess || hipMalloc((void **)&q->g, o * 4) != hipSuccess ||
+        hipMalloc((void **)&q->m, o * 4) != hipSuccess || hipMalloc((void **)&q->v, o * 4) != hipSuccess ||
+        hipMalloc((void **)&q->wb, o * 2) != hipSuccess) {
+        bilu_mlp_destroy(q);
+        return BILU_ERR_HIP;
+    }
+    if (hipMemcpy(q->w, hw.data(), o * 4, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemset(q->g, 0, o * 4) != hipSuccess || hipMemset(q->m, 0, o * 4) != hipSuccess ||
+        hipMemset(q->v, 0, o * 4) != hipSuccess) {
+        bilu_mlp_destroy(q);
+        return BILU_ERR_HIP;
+    }
+    if (hipMalloc((void **)&q->piv, (size_t)info.nnz * sizeof(int)) != hipSuccess ||
+        hipMemcpy(q->piv, piv.data(), (size_t)info.nnz * sizeof(int), hipMemcpyHostToDevice) != hipSuccess) {
+        bilu_mlp_destroy(q);
+        return BILU_ERR_HIP;
+    }
+    hipLaunchKernelGGL(k_to_bf16, dim3(blocks(o)), dim3(256), 0, 0, q->w, q->wb, o);
+    if (hipDeviceSynchronize() != hipSuccess) {
+        bilu_mlp_destroy(q);
+        return BILU_ERR_HIP;
+    }
+    *out = q;
+    return BILU_OK;
+}
+
+int bilu_mlp_destroy(bilu_mlp_t q)
+{
+    if (!q) return BILU_ERR_INVALID_ARG;
+    free_acts(*q);
+    for (void *p : {(void *)q->w, (void *)q->g, (void *)q->m, (void *)q->v, (void *)q->wb, (void *)q->piv})
+        if (p) (void)hipFree(p);
+    delete q;
+    return BILU_OK;
+}
+
+int bilu_mlp_num_params(bilu_mlp_t q, long long *count)
+{
+    if (!q || !count) return BILU_ERR_INVALID_ARG;
+    *count = (long long)packed_count(*q);
+    return BILU_OK;
+}
+
+int bilu_mlp_get_weights(bilu_mlp_t q, float *d_w, void *stream)
+{
+    if (!q || !d_w) return BILU_ERR_INVALID_ARG;
+    copy_params(*q, q->w, d_w, true, (hipStream_t)stream);
+    MLP_TRY(hipGetLastError());
+    return BILU_OK;
+}
+
+int bilu_mlp_set_weights(bilu_mlp_t q, const float *d_w, void *stream)
+{
+    if (!q || !d_w) return BILU_ERR_INVALID_ARG;
+    hipStream_t st = (hipStream_t)stream;
+    MLP_TRY(hipMemsetAsync(q->w, 0, nparams(*q) * 4, st));
+    copy_params(*q, d_w, q->w, false, st);
+    hipLaunchKernelGGL(k_to_bf16, dim3(blocks(nparams(*q))), dim3(256), 0, st, q->w, q->wb, nparams(*q));
+    MLP_TRY(hipGetLastError());
+    return BILU_OK;
+}
+
+int bilu_mlp_gradient(bilu_mlp_t q, int batch, const double *d_A, const double *d_F, float *d_loss,
+                      float *d_grad, void *stream)
+{
+    if (!q || batch <= 0 || !d_A || !d_F) return BILU_ERR_INVALID_ARG;
+    hipStream_t st = (hipStream_t)stream;
+    int rc = loss_and_grad(*q, batch, d_A, d_F, d_loss, st);
+    if (rc != BILU_OK) return rc;
+    if (d_grad) copy_params(*q, q->g, d_grad, true, st);
+    MLP_TRY(hipGetLastError());
+    return BILU_OK;
+}
+
+int bilu_mlp_train_step(bilu_mlp_t q, int batch, const double *d_A, const double *d_F, float lr,
+                        float *d_loss, void *stream)
+{
+    if (!q || batch <= 0 || !d_A || !d_F || !(lr > 0.0f)) return BILU_ERR_INVALID_ARG;
+    hipStream_t st = (hipStream_t)stream;
+    int rc = loss_and_grad(*q, batch, d_A, d_F, d_loss, st);
+    if (rc != BILU_OK) return rc;
+    q->step++;
+    const float b1t = std::pow(0.9f, (float)q->step), b2t = std::pow(0.999f, (float)q->step);
+    hipLaunchKernelGGL(k_adam, dim3(blocks(nparams(*q))), dim3(256), 0, st, q->w, q->g, q->m, q->v, q->wb,
+                       nparams(*q), lr, b1t, b2t);
+    MLP_TRY(hipGetLastError());
+    return BILU_OK;
+}
+
+int bilu_mlp_infer(bilu_mlp_t q, int batch, const double *d_A, double *d_F_init, void *stream)
+{
+    if (!q || batch < 0 || (batch > 0 && (!d_A || !d_F_init))) return BILU_ERR_INVALID_ARG;
+    if (batch == 0) return BILU_OK;
+    hipStream_t st = (hipStream_t)stream;
+    // in chunks, so the activations stay bounded
+    const int chunk = 16384;
+    int rc = reserve(*q, std::min(pad_to(batch, TM), chunk));
+    if (rc != BILU_OK) return rc;
+    for (int s0 = 0; s0 < batch; s0 += chunk) {
+        const int nb = std::min(chunk, batch - s0), rows = pad_to(nb, TM);
+        hipLaunchKernelGGL(k_features, dim3(blocks((size_t)rows * q->P0)), dim3(256), 0, st,
+                           d_A + (size_t)s0 * q->nnz, q->nnz, q->P0, nb, q->X);
+        if (rows > nb) MLP_TRY(hipMemsetAsync(q->X + (size_t)nb * q->P0, 0, (size_t)(rows - nb) * q->P0 * 2, st));
+        forward(*q, rows, st);
+        hipLaunchKernelGGL(k_outputs, dim3(blocks((size_t)nb * q->nnz)), dim3(256), 0, st, q->Y,
+                           d_A + (size_t)s0 * q->nnz, q->piv, q->nnz, q->P3, nb, d_F_init + (size_t)s0 * q->nnz);
+    }
+    MLP_TRY(hipGetLastError());
+    return BILU_OK;
+}
+
+}  // extern "C"
