@@ -1,16 +1,24 @@
 """Benchmark: systems solved per second to relres <= 1e-10 on a batch of
 65536 sparse systems (n=144, one shared random pattern, ~2000 nnz), one GPU.
 
-One step = one batched solve (preconditioner setup + BiCGSTAB + true-residual
-epilogue) over the whole batch, inputs resident in HBM.  Prints one JSON
-line (see the repository's bench contract).
+One step = one batched solve over the whole batch, inputs (A, b) resident
+in HBM: preconditioner setup (and, on the learned path, the network's
+inference) + BiCGSTAB + FP64 true residual.  Prints one JSON line (see the
+repository's bench contract).
+
+Default (headline): the learned + mixed-precision solve -- MLP warm start,
+ParILU(0) sweeps to a factor residual of 1e-8 (exact ILU(0) fallback),
+ISAI(L, U) of the factors applied in FP32, FP32 BiCGSTAB inside FP64
+iterative refinement -- timed beside the plain FP64 merged exact-ILU(0)
+BiCGSTAB kernel and the mixed-precision solve without the network.  The
+network is trained before the timed region (reported under "training").
+--mode single times one variant chosen by the flags.
 """
 from __future__ import annotations
 
 import argparse
 import json
 import os
-import subprocess
 import sys
 import time
 
@@ -18,6 +26,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "merging-ilu0_amd"))
 
 HBM_PEAK_GBS = 8000.0   # MI355X published HBM3E bandwidth
+METRIC = "systems solved/s at relres<=1e-10, n=144 batch 65536: learned+mixed vs plain FP64"
 
 
 def parse():
@@ -27,6 +36,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--batch", type=int, default=65536)
     ap.add_argument("--n", type=int, default=144)
+    ap.add_argument("--mode", default="headline", choices=["headline", "single"])
     ap.add_argument("--precond", default="ilu0",
                     choices=["ilu0", "parilu0", "ilu0_isai", "parilu0_isai", "general_isai"])
     ap.add_argument("--merge", default="merged", choices=["merged", "segregated"])
@@ -34,6 +44,8 @@ def parse():
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32_ir"],
                     help="fp32_ir: FP32 preconditioner and inner BiCGSTAB inside FP64 refinement")
     ap.add_argument("--inner-tol", type=float, default=1e-5)
+    ap.add_argument("--train-systems", type=int, default=262144)
+    ap.add_argument("--train-epochs", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run only a 1 GiB device copy (HBM counter calibration)")
@@ -65,8 +77,8 @@ def cpu_model():
 
 def cpu_baseline(pat, A, b, precond):
     """The oracle (CPU restatement of the reference path, same algorithm and
-    stopping rule) timed on this host: a bounded prefix of the same batch on
-    all threads, plus a single-core figure on a smaller prefix."""
+    stopping rule) timed on this host: the whole batch on all threads, plus a
+    single-core figure on a prefix."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     code = {"ilu0": O.ILU0, "parilu0": O.PARILU0, "ilu0_isai": O.ILU0_ISAI,
@@ -83,10 +95,27 @@ def cpu_baseline(pat, A, b, precond):
     dt_one = time.perf_counter() - t
     return {"value": round(s_all / dt_all, 1), "unit": "systems/s", "cores": threads,
             "kind": "port",
-            "sample": f"{s_all} systems (the whole batch), {threads} threads, {dt_all:.2f} s wall",
+            "sample": f"{s_all} systems (the whole batch), plain FP64 {precond} + BiCGSTAB, "
+                      f"{threads} threads, {dt_all:.2f} s wall",
             "single_core_value": round(s_one / dt_one, 1),
             "single_core_sample": f"first {s_one} systems, 1 thread, {dt_one:.2f} s",
             "cpu_model": cpu_model()}
+
+
+def time_steps(step, warmup, steps, stream):
+    """HIP events on the launch stream around exactly `steps` steps."""
+    import torch
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(steps):
+        step()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / max(steps, 1)
 
 
 def main():
@@ -122,100 +151,161 @@ def main():
     db = torch.from_numpy(b).to(dev)
     x = torch.empty_like(db)
     B, n, nnz = args.batch, pat.n, pat.nnz
-    it = torch.empty(B, dtype=torch.float32, device=dev)
-    cv = torch.empty(B, dtype=torch.int32, device=dev)
-    ir = torch.empty(B, dtype=torch.float64, device=dev)
-    tr = torch.empty(B, dtype=torch.float64, device=dev)
-    pc = {"ilu0": bilu.ILU0, "parilu0": bilu.PARILU0, "ilu0_isai": bilu.ILU0_ISAI,
-          "parilu0_isai": bilu.PARILU0_ISAI, "general_isai": bilu.GENERAL_ISAI}[args.precond]
-    merge = bilu.MERGED if args.merge == "merged" else bilu.SEGREGATED
-    kern = {"auto": bilu.KERNEL_AUTO, "wave": bilu.KERNEL_WAVE,
-            "persistent": bilu.KERNEL_PERSISTENT}[args.kernel]
-    mixed = args.precision == "fp32_ir"
-    o = bilu.opts(precond=pc, parilu_sweeps=5, tol=1e-10, tol_mode=bilu.TOL_RELATIVE,
-                  true_res_restart=1, max_restarts=3, merge=merge, kernel=kern,
-                  precision=bilu.PRECISION_FP32_IR if mixed else bilu.PRECISION_FP64,
-                  inner_tol=args.inner_tol)
+    bnorm = np.linalg.norm(b, axis=1)
     stream = torch.cuda.current_stream()
-    refine = torch.zeros(B, dtype=torch.int32, device=dev)
-    fstat = torch.zeros(B, dtype=torch.int32, device=dev)
-
-    def step():
-        if mixed:
-            bilu.solve_ex(P, dA, db, x, o, None, it, cv, ir, tr, refine, fstat, stream=stream)
-        else:
-            bilu.solve(P, dA, db, x, o, it, cv, ir, tr, stream=stream)
-
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    e0.record(stream)
-    for _ in range(args.steps):
-        step()
-    e1.record(stream)
-    torch.cuda.synchronize()
-    ms_total = e0.elapsed_time(e1)
-    ms_per_step = ms_total / max(args.steps, 1)
-    value = B / (ms_per_step * 1e-3)
-
-    # acceptance: every system converged with true relres <= 1e-10
-    cvh, trh, ith = cv.cpu().numpy(), tr.cpu().numpy(), it.cpu().numpy()
-    relres = trh / np.linalg.norm(b, axis=1)
-    all_ok = bool((cvh == 1).all() and (relres <= 1e-10).all())
-
-    # roofline of the dominant kernel (the merged solve is one launch per step)
+    out = dict(it=torch.empty(B, dtype=torch.float32, device=dev),
+               cv=torch.empty(B, dtype=torch.int32, device=dev),
+               ir=torch.empty(B, dtype=torch.float64, device=dev),
+               tr=torch.empty(B, dtype=torch.float64, device=dev),
+               rf=torch.zeros(B, dtype=torch.int32, device=dev),
+               fs=torch.zeros(B, dtype=torch.int32, device=dev))
+    PC = {"ilu0": bilu.ILU0, "parilu0": bilu.PARILU0, "ilu0_isai": bilu.ILU0_ISAI,
+          "parilu0_isai": bilu.PARILU0_ISAI, "general_isai": bilu.GENERAL_ISAI}
     alg_bytes_per_system = 8 * nnz + 16 * n + 24
     alg_bytes = alg_bytes_per_system * B + 4 * (n + 1) + 4 * nnz
-    achieved = alg_bytes / (ms_per_step * 1e-3) / 1e9
-    kname = ("k_mixed" if mixed else
-             "k_fast" if (args.merge == "merged" and args.kernel != "wave") else None)
-    traffic = measured_traffic(f"{kname} {args.precond} {args.merge} n{n} b{B}") if kname else None
-    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
-                "kernel": ("k_factor_fast + k_mixed (FP32 inner, FP64 refinement)" if mixed else
-                           ("k_solve (one wave per system)" if args.kernel == "wave" else
-                            "k_fast (persistent multi-wave)") if args.merge == "merged" else "k_factor + solve"),
-                "alg_bytes_per_launch": alg_bytes}
 
-    cpu = None
-    if not args.no_cpu:
-        cpu = cpu_baseline(pat, A, b, args.precond)
+    def summary(ms):
+        cvh, trh, ith = out["cv"].cpu().numpy(), out["tr"].cpu().numpy(), out["it"].cpu().numpy()
+        relres = trh / bnorm
+        rf, fs = out["rf"].cpu().numpy(), out["fs"].cpu().numpy()
+        return {"systems_per_s": round(B / (ms * 1e-3), 1), "ms_per_step": round(ms, 4),
+                "all_converged_relres_le_1e-10": bool((cvh == 1).all() and (relres <= 1e-10).all()),
+                "max_relres": float(relres.max()),
+                "iterations_mean": float(ith.mean()), "iterations_max": float(ith.max()),
+                "refine_steps_mean": float(rf.mean()), "refine_steps_max": int(rf.max()),
+                "parilu_sweeps_mean": float(fs[fs >= 0].mean()) if (fs >= 0).any() else None,
+                "parilu_fallbacks": int((fs < 0).sum())}
 
-    rec = {
-        "metric": "systems solved/s at relres<=1e-10, n=144 batch 65536: learned+mixed vs plain FP64",
-        "value": round(value, 1),
-        "unit": "systems/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 4),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32+f64" if mixed else "f64",
-        "data": "synthetic (counter-based generator, pattern seed 144, value seed 20261203)",
-        "config": {"workload": (f"random pattern n={n} nnz={nnz}, batch {B}, {args.precond} factor "
-                                f"(FP64, rounded to FP32) + FP32 BiCGSTAB (inner tol {args.inner_tol:g}) "
-                                f"inside FP64 iterative refinement, relres<=1e-10" if mixed else
-                                f"random pattern n={n} nnz={nnz}, batch {B}, exact ILU(0) merged "
-                                f"factor+BiCGSTAB, FP64, relres<=1e-10 (variant: plain FP64 "
-                                f"{args.precond} {args.merge})"),
-                   "batch": B, "n": n, "nnz": nnz, "precond": args.precond, "merge": args.merge,
-                   "levels_lower": info.levels_lower, "levels_upper": info.levels_upper,
-                   "kernel": args.kernel, "precision": args.precision,
-                   "inner_tol": args.inner_tol if mixed else None,
-                   "parallelism": "one wave64 per system; batch sharded over CUs"},
-        "all_converged_relres_le_1e-10": all_ok,
-        "max_relres": float(relres.max()),
-        "iterations": {"min": float(ith.min()), "mean": float(ith.mean()), "max": float(ith.max())},
-        "refine_steps": ({"mean": float(refine.float().mean()), "max": int(refine.max())}
-                         if mixed else None),
-        "roofline": roofline,
-        "cpu_baseline": cpu,
+    def run(o, F_init=None, pre=None):
+        # the factor status is requested only from the ParILU paths (asking for
+        # it routes a plain solve through the split factor + solve launches)
+        par = o.precond in (bilu.PARILU0, bilu.PARILU0_ISAI)
+        out["fs"].zero_()
+        out["rf"].zero_()
+
+        def step():
+            if pre is not None:
+                pre()
+            bilu.solve_ex(P, dA, db, x, o, F_init, out["it"], out["cv"], out["ir"], out["tr"],
+                          out["rf"], out["fs"] if par else False, stream=stream)
+        return time_steps(step, args.warmup, args.steps, stream)
+
+    rec = {"metric": METRIC, "unit": "systems/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "data": "synthetic (counter-based generator, pattern seed 144, value seed 20261203)"}
+    base_cfg = {"batch": B, "n": n, "nnz": nnz, "levels_lower": info.levels_lower,
+                "levels_upper": info.levels_upper,
+                "parallelism": "one wave64 per system; persistent workgroups over the CUs"}
+
+    if args.mode == "single":
+        mixed = args.precision == "fp32_ir"
+        o = bilu.opts(precond=PC[args.precond], parilu_sweeps=5, tol=1e-10, tol_mode=bilu.TOL_RELATIVE,
+                      true_res_restart=1, max_restarts=3,
+                      merge=bilu.MERGED if args.merge == "merged" else bilu.SEGREGATED,
+                      kernel={"auto": bilu.KERNEL_AUTO, "wave": bilu.KERNEL_WAVE,
+                              "persistent": bilu.KERNEL_PERSISTENT}[args.kernel],
+                      precision=bilu.PRECISION_FP32_IR if mixed else bilu.PRECISION_FP64,
+                      inner_tol=args.inner_tol)
+        ms = run(o)
+        s = summary(ms)
+        kname = ("k_mixed" if mixed else
+                 "k_fast" if (args.merge == "merged" and args.kernel != "wave") else None)
+        achieved = alg_bytes / (ms * 1e-3) / 1e9
+        rec.update({"value": s["systems_per_s"], "ms_per_step": s["ms_per_step"],
+                    "dtype": "f32+f64" if mixed else "f64",
+                    "config": dict(base_cfg, workload=f"random pattern n={n} nnz={nnz}, batch {B}, "
+                                                      f"{args.precond} {args.merge} {args.precision}",
+                                   precond=args.precond, merge=args.merge, kernel=args.kernel,
+                                   precision=args.precision),
+                    "result": s,
+                    "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+                                 "traffic": measured_traffic(f"{kname} {args.precond} {args.merge} n{n} b{B}")
+                                 if kname else None,
+                                 "kernel": kname or "k_solve", "alg_bytes_per_launch": alg_bytes}})
+        rec["cpu_baseline"] = None if args.no_cpu else cpu_baseline(pat, A, b, args.precond)
+        print(json.dumps(rec), flush=True)
+        return
+
+    # ---------------------------------------------------------------- headline
+    common = dict(tol=1e-10, tol_mode=bilu.TOL_RELATIVE)
+    # (1) plain FP64 merged exact ILU(0) + BiCGSTAB (the persistent k_fast kernel)
+    o_plain = bilu.opts(precond=bilu.ILU0, true_res_restart=1, **common)
+    ms_plain = run(o_plain)
+    plain = summary(ms_plain)
+    # (2) mixed precision without the network: exact ILU(0), ISAI(L, U) in FP32, FP32 inner / FP64 IR
+    o_mixed = bilu.opts(precond=bilu.ILU0_ISAI, precision=bilu.PRECISION_FP32_IR,
+                        inner_tol=args.inner_tol, **common)
+    ms_mixed = run(o_mixed)
+    mixed = summary(ms_mixed)
+    # (3) learned + mixed: train the warm start (untimed), then time inference + solve
+    t0 = time.time()
+    net = bilu.Mlp(P, hidden=512, seed=1)
+    chunk = 65536
+    loss = torch.zeros(1, dtype=torch.float32, device=dev)
+    tr_A, tr_F = [], []
+    for c0 in range(0, args.train_systems, chunk):
+        _, At, _ = synth.batch("random", min(chunk, args.train_systems - c0), first=(1 << 24) + c0, n=n)
+        dAt = torch.from_numpy(At).to(dev)
+        dFt = torch.empty_like(dAt)
+        bilu.ilu0(P, dAt, dFt)           # exact FP64 ILU(0) labels from the library itself
+        tr_A.append(dAt)
+        tr_F.append(dFt)
+    torch.cuda.synchronize()
+    gen_train_s = time.time() - t0
+    t1 = time.time()
+    mb, steps_done, losses = 4096, 0, []
+    total_steps = max(1, args.train_epochs * args.train_systems // mb)
+    for ep in range(args.train_epochs):
+        for ci in range(len(tr_A)):
+            for s0 in range(0, tr_A[ci].shape[0], mb):
+                lr = 3e-4 if steps_done < 0.7 * total_steps else 1e-4
+                net.train_step(tr_A[ci][s0:s0 + mb], tr_F[ci][s0:s0 + mb], lr, loss)
+                steps_done += 1
+        losses.append(round(loss.item(), 6))
+    torch.cuda.synchronize()
+    train_s = time.time() - t1
+    del tr_A, tr_F
+    guess = torch.empty_like(dA)
+    o_learn = bilu.opts(precond=bilu.PARILU0_ISAI, parilu_sweeps=40, parilu_stop_tol=1e-8,
+                        precision=bilu.PRECISION_FP32_IR, inner_tol=args.inner_tol, **common)
+    ms_learn = run(o_learn, F_init=guess, pre=lambda: net.infer(dA, guess, stream=stream))
+    learned = summary(ms_learn)
+    # (3b) the same ParILU path from the standard start (no network), for the gain
+    ms_nolearn = run(o_learn)
+    nolearn = summary(ms_nolearn)
+    ms_infer = time_steps(lambda: net.infer(dA, guess, stream=stream), 1, args.steps, stream)
+
+    achieved = alg_bytes / (ms_learn * 1e-3) / 1e9
+    achieved_plain = alg_bytes / (ms_plain * 1e-3) / 1e9
+    rec.update({
+        "value": learned["systems_per_s"], "ms_per_step": learned["ms_per_step"], "dtype": "f32+f64",
+        "config": dict(base_cfg,
+                       workload=(f"random pattern n={n} nnz={nnz}, batch {B}: MLP warm start (BF16) -> "
+                                 f"ParILU(0) to factor residual 1e-8 (exact ILU(0) fallback) -> ISAI(L,U) "
+                                 f"FP32 -> FP32 BiCGSTAB (inner tol {args.inner_tol:g}) in FP64 refinement "
+                                 f"to relres 1e-10"),
+                       variant="learned+mixed"),
+        "speedup_vs_plain_fp64": round(ms_plain / ms_learn, 3),
+        "variants": {"plain_fp64_ilu0_merged": plain, "mixed_ilu0_isai": mixed,
+                     "learned_mixed_parilu0_isai": learned, "mixed_parilu0_isai_no_warm_start": nolearn},
+        "mlp_inference_ms": round(ms_infer, 4),
+        "training": {"systems": args.train_systems, "epochs": args.train_epochs, "minibatch": mb,
+                     "label_generation_s": round(gen_train_s, 2), "train_s": round(train_s, 2),
+                     "loss_per_epoch": losses},
+        "all_converged_relres_le_1e-10": learned["all_converged_relres_le_1e-10"],
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+                     "traffic": measured_traffic(f"learned_mixed n{n} b{B}"),
+                     "kernel": "whole learned+mixed step (MLP inference, k_factor_fast, k_mixed)",
+                     "alg_bytes_per_launch": alg_bytes},
+        "roofline_plain_fp64": {"bound": "hbm", "achieved": round(achieved_plain, 2), "peak": HBM_PEAK_GBS,
+                                "unit": "GB/s", "frac": round(achieved_plain / HBM_PEAK_GBS, 5),
+                                "traffic": measured_traffic(f"k_fast ilu0 merged n{n} b{B}"),
+                                "kernel": "k_fast (one launch per step)", "alg_bytes_per_launch": alg_bytes},
         "gen_seconds": round(gen_s, 2),
-    }
+    })
+    rec["cpu_baseline"] = None if args.no_cpu else cpu_baseline(pat, A, b, "ilu0")
     print(json.dumps(rec), flush=True)
 
 

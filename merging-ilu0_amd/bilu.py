@@ -208,7 +208,9 @@ def solve_ex(pat: Pattern, A, b, x, o: Opts | None = None, F_init=None, iters=No
     mk = lambda t, dt: t if t is not None else torch.empty(B, dtype=dt, device=dev)
     iters, conv = mk(iters, torch.float32), mk(conv, torch.int32)
     iter_res, true_res = mk(iter_res, torch.float64), mk(true_res, torch.float64)
-    refine, factor_status = mk(refine, torch.int32), mk(factor_status, torch.int32)
+    refine = mk(refine, torch.int32)
+    # factor_status=False: not requested (a plain solve then keeps its merged kernel)
+    factor_status = None if factor_status is False else mk(factor_status, torch.int32)
     o = o or opts()
     _check(lib().bilu_solve_ex(pat.h, B, _ptr(A), _ptr(b), _ptr(x), C.byref(o), _ptr(F_init),
                                _ptr(iters), _ptr(conv), _ptr(iter_res), _ptr(true_res),
