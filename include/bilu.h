@@ -54,6 +54,8 @@ extern "C" {
 #define BILU_PRECOND_PARILU0_ISAI 3   /* ParILU(0), then ISAI(L), ISAI(U) */
 #define BILU_PRECOND_GENERAL_ISAI 4   /* ISAI(A), Gauss-Jordan per row */
 #define BILU_PRECOND_NONE 5           /* identity (testing) */
+#define BILU_PRECOND_FACTORS 6        /* in-place ILU factors supplied by the caller in bilu_solve_ex's
+                                         d_F_init (the factor-then-solve flow of ilu_solve.cu:443-509) */
 
 #define BILU_TOL_ABSOLUTE 0   /* ||r|| < tol (reference, ATOL = 1e-11) */
 #define BILU_TOL_RELATIVE 1   /* ||r|| < tol * ||b|| */

@@ -17,7 +17,7 @@ LIB_PATH = os.path.join(HERE, "libbilu.so")
 
 OK = 0
 CONVERGED, NOT_CONVERGED, ZERO_PIVOT, NONFINITE = 1, -1, -2, -3
-ILU0, PARILU0, ILU0_ISAI, PARILU0_ISAI, GENERAL_ISAI, NONE = 0, 1, 2, 3, 4, 5
+ILU0, PARILU0, ILU0_ISAI, PARILU0_ISAI, GENERAL_ISAI, NONE, FACTORS = 0, 1, 2, 3, 4, 5, 6
 TOL_ABSOLUTE, TOL_RELATIVE = 0, 1
 MERGED, SEGREGATED = 0, 1
 ISAI_GENERAL, ISAI_LOWER, ISAI_UPPER = 0, 1, 2

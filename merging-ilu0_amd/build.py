@@ -19,13 +19,30 @@ HIP_FLAGS = ["-O3", "-std=c++17", "--offload-arch=gfx950", "-fPIC", "-ffp-contra
              "-Wno-unused-command-line-argument"]
 
 
-def build(verbose: bool = False, extra=()) -> str:
-    srcs = [os.path.join(CSRC, "solver.hip"), os.path.join(CSRC, "mlp.hip"), os.path.join(CSRC, "pattern.cpp")]
-    cmd = [HIPCC, *HIP_FLAGS, *extra, "-shared", *srcs, "-o", LIB + ".tmp"]
+CPP = os.path.join(HERE, "cpp")
+LIB_CPP = os.path.join(HERE, "libbilu_cpp.so")
+DRIVER = os.path.join(HERE, "bicgstab_driver")
+
+
+def _run(cmd, verbose):
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
+
+
+def build(verbose: bool = False, extra=()) -> str:
+    srcs = [os.path.join(CSRC, "solver.hip"), os.path.join(CSRC, "mlp.hip"), os.path.join(CSRC, "pattern.cpp")]
+    _run([HIPCC, *HIP_FLAGS, *extra, "-shared", *srcs, "-o", LIB + ".tmp"], verbose)
     os.replace(LIB + ".tmp", LIB)
+    # drop-in C++ interface (host code over the C ABI) and its driver program
+    host = ["-O2", "-std=c++17", "-fPIC", "-Wno-unused-command-line-argument"]
+    link = ["-L" + HERE, "-lbilu", "-Wl,-rpath,$ORIGIN"]
+    _run([HIPCC, *host, "-shared", os.path.join(CPP, "containers.cpp"), os.path.join(CPP, "drop_in.cpp"),
+          *link, "-o", LIB_CPP + ".tmp"], verbose)
+    os.replace(LIB_CPP + ".tmp", LIB_CPP)
+    _run([HIPCC, *host, os.path.join(CPP, "bicgstab_driver.cpp"), "-L" + HERE, "-lbilu_cpp", "-lbilu",
+          "-Wl,-rpath,$ORIGIN", "-o", DRIVER + ".tmp"], verbose)
+    os.replace(DRIVER + ".tmp", DRIVER)
     return LIB
 
 

@@ -1,0 +1,16 @@
+// Drop-in factor-level entry points (reference ILU_0.h:7-11).
+#pragma once
+
+class PagedCSRMatrices;
+
+// exact ILU(0) of every page; L (explicit unit diagonal) and U in the
+// reference split layout (factorization.cu:216-344), device arrays
+// allocated here (host pattern arrays too).  `approach_num` (1-4) selects
+// among the reference's equivalent algorithms and is not needed here.
+void ILU_0_Factorization_Gpu(const PagedCSRMatrices &A_pages, PagedCSRMatrices &L_pages, PagedCSRMatrices &U_pages,
+                             const int approach_num);
+
+// in place on the device values: strictly lower = L (unit diagonal
+// implicit), diagonal + upper = U; diagonal positions are found from the
+// pattern (`diag_ptrs` may be null)
+void ComputeILU0Approach1(PagedCSRMatrices &Factored_Pages, const int *const diag_ptrs);

@@ -1,0 +1,91 @@
+// Driver of the drop-in C++ interface, in the manner of the reference's
+// main_bicgstab_*.cpp programs: loads a batch from a raw binary file, calls
+// one of the reference-named entry points and writes x and the per-page
+// results.
+//
+//   bicgstab_driver <merged|segregated|ilu|parilu|ilu_isai|parilu_isai|general_isai> in.bin out.bin
+//
+// in.bin:  int32 n, nnz, pages; int32 row_ptrs[n+1], col_ind[nnz];
+//          float64 values[pages*nnz], b[pages*n]
+// out.bin: float64 x[pages*n]; float32 iter[pages]; int32 conv[pages];
+//          float64 iter_res[pages], true_res[pages]; float64 PST, IST, OET
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "PrecondBiCGSTAB.h"
+#include "SolverResults.h"
+#include "matrix.h"
+
+template <typename T>
+static bool rd(FILE *f, T *p, size_t n) { return std::fread(p, sizeof(T), n, f) == n; }
+template <typename T>
+static bool wr(FILE *f, const T *p, size_t n) { return std::fwrite(p, sizeof(T), n, f) == n; }
+
+int main(int argc, char **argv)
+{
+    if (argc != 4) {
+        std::fprintf(stderr, "usage: %s <variant> in.bin out.bin\n", argv[0]);
+        return 2;
+    }
+    const std::string variant = argv[1];
+    FILE *f = std::fopen(argv[2], "rb");
+    if (!f) { std::perror("in.bin"); return 2; }
+    int hdr[3];
+    if (!rd(f, hdr, 3)) { std::fprintf(stderr, "bad header\n"); return 2; }
+    const int n = hdr[0], nnz = hdr[1], pages = hdr[2];
+    PagedCSRMatrices A;
+    A.SetNumPages(pages); A.SetNumRows(n); A.SetNumCols(n); A.SetNumNz(nnz);
+    A.AllocateMemory(LOCATION::CPU);
+    PagedVectors b;
+    b.SetNumPages(pages); b.SetNumElements(n);
+    b.AllocateMemory(LOCATION::CPU);
+    if (!rd(f, A.GetPtrToCpuRowPtrs(), n + 1) || !rd(f, A.GetPtrToCpuColInd(), nnz) ||
+        !rd(f, A.GetPtrToCpuValues(), (size_t)pages * nnz) || !rd(f, b.GetPtrToCpuValues(), (size_t)pages * n)) {
+        std::fprintf(stderr, "short input\n");
+        return 2;
+    }
+    std::fclose(f);
+    A.AllocateMemory(LOCATION::GPU);
+    A.CopyFromCpuToGpu();
+    b.AllocateMemory(LOCATION::GPU);
+    b.CopyFromCpuToGpu();
+    PagedVectors x;
+    x.SetNumPages(pages); x.SetNumElements(n);
+    x.AllocateMemory(LOCATION::CPU);
+    SolverResults res(pages, ATOL, CPU_EXISTENCE::EXISTENT, GPU_EXISTENCE::EXISTENT);
+    const std::vector<std::string> subdir;
+    double PST = 0, IST = 0, OET = 0;
+    try {
+        if (variant == "merged")
+            Batched_conv_ILU_app1_Preconditioned_BiCGSTAB_merged_Gpu(subdir, A, b, x, false, res);
+        else if (variant == "segregated")
+            Batched_conv_ILU_app1_Preconditioned_BiCGSTAB_segregated_Gpu(subdir, A, b, x, false, res);
+        else if (variant == "ilu" || variant == "parilu")
+            Batched_ILU_Preconditioned_BiCGSTAB_Gpu(subdir, A, b, x, false, res, variant == "parilu", 1, 5, PST, IST, OET);
+        else if (variant == "ilu_isai" || variant == "parilu_isai")
+            Batched_ILU_ISAI_Preconditioned_BiCGSTAB_Gpu(subdir, A, b, x, false, res, variant == "parilu_isai", 5, PST,
+                                                         IST, OET);
+        else if (variant == "general_isai")
+            Batched_General_ISAI_Preconditioned_BiCGSTAB_Gpu(subdir, A, b, x, false, res, PST, IST, OET);
+        else {
+            std::fprintf(stderr, "unknown variant %s\n", variant.c_str());
+            return 2;
+        }
+    } catch (const std::exception &e) {
+        std::fprintf(stderr, "solve failed: %s\n", e.what());
+        return 1;
+    }
+    if (OET == 0) OET = IST = res.GetTimeTaken();
+    FILE *o = std::fopen(argv[3], "wb");
+    if (!o) { std::perror("out.bin"); return 2; }
+    const double t[3] = {PST, IST, OET};
+    const bool ok = wr(o, x.GetPtrToCpuValues(), (size_t)pages * n) && wr(o, res.GetPtrToCpuIterCount(), pages) &&
+                    wr(o, res.GetPtrToCpuConvFlag(), pages) && wr(o, res.GetPtrToCpuIterResNorm(), pages) &&
+                    wr(o, res.GetPtrToCpuTrueResNorm(), pages) && wr(o, t, 3);
+    std::fclose(o);
+    std::printf("%s: %d pages n=%d nnz=%d, PST %.3f ms IST %.3f ms OET %.3f ms\n", variant.c_str(), pages, n, nnz,
+                PST, IST, OET);
+    return ok ? 0 : 1;
+}

@@ -1,0 +1,310 @@
+// Drop-in C++ interface over the C ABI of libbilu (include/bilu.h): the
+// reference's solver, factorization and ISAI entry points on HIP-backed
+// paged containers.
+#include <hip/hip_runtime.h>
+
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/bilu.h"
+#include "ILU_0.h"
+#include "PrecondBiCGSTAB.h"
+#include "SolverResults.h"
+#include "isai.h"
+#include "matrix.h"
+#include "parILU_0.h"
+
+namespace {
+
+void check(int rc, const char *what)
+{
+    if (rc != BILU_OK) throw std::runtime_error(std::string(what) + ": " + bilu_status_string(rc));
+}
+
+void hip_check(hipError_t e, const char *what)
+{
+    if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// analysed pattern of a page batch (host pattern arrays, copied from the
+// device when the batch lives there only); destroyed with the object
+struct PatternHandle {
+    bilu_pattern_t h = nullptr;
+    explicit PatternHandle(const PagedCSRMatrices &A)
+    {
+        const int n = A.GetNumRows(), nnz = A.GetNumNz();
+        if (A.GetNumCols() != n) throw std::invalid_argument("batched solves need square matrices");
+        std::vector<int> rp(n + 1), ci(nnz);
+        if (A.ExistsCPU() && A.GetPtrToCpuRowPtrs()) {
+            std::copy(A.GetPtrToCpuRowPtrs(), A.GetPtrToCpuRowPtrs() + n + 1, rp.begin());
+            std::copy(A.GetPtrToCpuColInd(), A.GetPtrToCpuColInd() + nnz, ci.begin());
+        } else if (A.ExistsGPU()) {
+            hip_check(hipMemcpy(rp.data(), A.GetPtrToGpuRowPtrs(), sizeof(int) * (n + 1), hipMemcpyDeviceToHost), "pattern D2H");
+            hip_check(hipMemcpy(ci.data(), A.GetPtrToGpuColInd(), sizeof(int) * nnz, hipMemcpyDeviceToHost), "pattern D2H");
+        } else {
+            throw std::invalid_argument("matrix batch has no arrays");
+        }
+        check(bilu_pattern_create(n, nnz, rp.data(), ci.data(), &h), "bilu_pattern_create");
+    }
+    ~PatternHandle() { if (h) bilu_pattern_destroy(h); }
+    PatternHandle(const PatternHandle &) = delete;
+    PatternHandle &operator=(const PatternHandle &) = delete;
+};
+
+// device arrays of A / b (uploaded when only the host copies exist), x and
+// the results (allocated when absent)
+void prepare(const PagedCSRMatrices &A, const PagedVectors &b, PagedVectors &x, SolverResults &r)
+{
+    if (!A.ExistsGPU()) const_cast<PagedCSRMatrices &>(A).CopyFromCpuToGpu();
+    if (!b.ExistsGPU()) const_cast<PagedVectors &>(b).CopyFromCpuToGpu();
+    if (b.GetNumPages() != A.GetNumPages() || b.GetNumElements() != A.GetNumRows())
+        throw std::invalid_argument("b does not match A's pages / rows");
+    if (x.GetNumPages() != A.GetNumPages() || x.GetNumElements() != A.GetNumCols() || !x.ExistsGPU()) {
+        x.SetNumPages(A.GetNumPages());
+        x.SetNumElements(A.GetNumCols());
+        x.AllocateMemory(LOCATION::GPU);
+    }
+    if (r.GetNumPages() != A.GetNumPages() || !r.ExistsGPU()) {
+        r.SetNumPages(A.GetNumPages());
+        r.AllocateMemory(LOCATION::GPU);
+    }
+}
+
+void copy_back(PagedVectors &x, SolverResults &r)
+{
+    x.CopyFromGpuToCpu();
+    r.CopyFromGpuToCpu();
+}
+
+struct Timer {
+    hipEvent_t e0{}, e1{};
+    hipStream_t st;
+    explicit Timer(hipStream_t s) : st(s)
+    {
+        hip_check(hipEventCreate(&e0), "hipEventCreate");
+        hip_check(hipEventCreate(&e1), "hipEventCreate");
+    }
+    ~Timer() { (void)hipEventDestroy(e0); (void)hipEventDestroy(e1); }
+    void start() { hip_check(hipEventRecord(e0, st), "hipEventRecord"); }
+    double stop()
+    {
+        hip_check(hipEventRecord(e1, st), "hipEventRecord");
+        hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
+        float ms = 0.0f;
+        hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+        return ms;
+    }
+};
+
+bilu_opts to_opts(const BiluSolveOptions &s)
+{
+    bilu_opts o;
+    bilu_opts_default(&o);
+    o.precond = s.precond;
+    o.parilu_sweeps = s.parilu_sweeps;
+    o.tol = s.tol;
+    o.tol_mode = s.tol_mode;
+    o.max_iter = s.max_iter;
+    o.merge = s.merge;
+    o.true_res_restart = s.true_res_restart;
+    o.precision = s.precision;
+    o.inner_tol = s.inner_tol;
+    o.max_refine = s.max_refine;
+    o.parilu_stop_tol = s.parilu_stop_tol;
+    return o;
+}
+
+double run_solve(const PatternHandle &P, const PagedCSRMatrices &A, const PagedVectors &b, PagedVectors &x,
+                 SolverResults &r, const bilu_opts &o, const double *F_init, void *stream)
+{
+    Timer t((hipStream_t)stream);
+    t.start();
+    check(bilu_solve_ex(P.h, A.GetNumPages(), A.GetPtrToGpuValues(), b.GetPtrToGpuValues(), x.GetPtrToGpuValues(), &o,
+                        F_init, r.GetPtrToGpuIterCount(), r.GetPtrToGpuConvFlag(), r.GetPtrToGpuIterResNorm(),
+                        r.GetPtrToGpuTrueResNorm(), nullptr, nullptr, stream),
+          "bilu_solve_ex");
+    return t.stop();
+}
+
+// the reference's L / U split layouts of the pattern, filled from in-place factors F
+void split_into(const PatternHandle &P, const PagedCSRMatrices &A, const double *F, PagedCSRMatrices &L,
+                PagedCSRMatrices &U)
+{
+    bilu_pattern_info info;
+    check(bilu_pattern_get_info(P.h, &info), "bilu_pattern_get_info");
+    for (PagedCSRMatrices *M : {&L, &U}) {
+        M->DeallocateMemory(LOCATION::CPU);
+        M->DeallocateMemory(LOCATION::GPU);
+        M->SetNumPages(A.GetNumPages());
+        M->SetNumRows(info.n);
+        M->SetNumCols(info.n);
+        M->SetNumNz(M == &L ? info.l_nnz : info.u_nnz);
+        M->AllocateMemory(LOCATION::CPU);
+        M->AllocateMemory(LOCATION::GPU);
+    }
+    check(bilu_pattern_lu_split(P.h, L.GetPtrToCpuRowPtrs(), L.GetPtrToCpuColInd(), U.GetPtrToCpuRowPtrs(),
+                                U.GetPtrToCpuColInd()),
+          "bilu_pattern_lu_split");
+    const int n = info.n;
+    hip_check(hipMemcpy(L.GetPtrToGpuRowPtrs(), L.GetPtrToCpuRowPtrs(), sizeof(int) * (n + 1), hipMemcpyHostToDevice), "H2D");
+    hip_check(hipMemcpy(L.GetPtrToGpuColInd(), L.GetPtrToCpuColInd(), sizeof(int) * info.l_nnz, hipMemcpyHostToDevice), "H2D");
+    hip_check(hipMemcpy(U.GetPtrToGpuRowPtrs(), U.GetPtrToCpuRowPtrs(), sizeof(int) * (n + 1), hipMemcpyHostToDevice), "H2D");
+    hip_check(hipMemcpy(U.GetPtrToGpuColInd(), U.GetPtrToCpuColInd(), sizeof(int) * info.u_nnz, hipMemcpyHostToDevice), "H2D");
+    check(bilu_split_lu(P.h, A.GetNumPages(), F, L.GetPtrToGpuValues(), U.GetPtrToGpuValues(), nullptr), "bilu_split_lu");
+    hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+}
+
+struct DeviceBuffer {
+    double *p = nullptr;
+    explicit DeviceBuffer(size_t n) { hip_check(hipMalloc((void **)&p, (n ? n : 1) * sizeof(double)), "hipMalloc"); }
+    ~DeviceBuffer() { if (p) (void)hipFree(p); }
+};
+
+}  // namespace
+
+double Batched_Preconditioned_BiCGSTAB_Gpu(const PagedCSRMatrices &A_pages, const PagedVectors &b_pages,
+                                           PagedVectors &x_pages, SolverResults &solver_results,
+                                           const BiluSolveOptions &opts)
+{
+    prepare(A_pages, b_pages, x_pages, solver_results);
+    PatternHandle P(A_pages);
+    const double ms = run_solve(P, A_pages, b_pages, x_pages, solver_results, to_opts(opts), opts.d_F_init, opts.stream);
+    solver_results.SetATOL(opts.tol);
+    solver_results.SetTimeTaken(ms);
+    if (opts.copy_back) copy_back(x_pages, solver_results);
+    return ms;
+}
+
+void Batched_ILU_Preconditioned_BiCGSTAB_Gpu(const std::vector<std::string> &, const PagedCSRMatrices &A_pages,
+                                             const PagedVectors &b_pages, PagedVectors &x_pages, const bool,
+                                             SolverResults &solver_results, const bool is_parilu, const int,
+                                             const int parilu_sweeps, double &PST, double &IST, double &OET)
+{
+    prepare(A_pages, b_pages, x_pages, solver_results);
+    PatternHandle P(A_pages);
+    const int B = A_pages.GetNumPages();
+    DeviceBuffer F((size_t)B * A_pages.GetNumNz());
+    Timer t(nullptr);
+    t.start();
+    if (is_parilu)
+        check(bilu_parilu0(P.h, B, A_pages.GetPtrToGpuValues(), nullptr, F.p, parilu_sweeps, nullptr), "bilu_parilu0");
+    else
+        check(bilu_ilu0(P.h, B, A_pages.GetPtrToGpuValues(), F.p, nullptr, nullptr), "bilu_ilu0");
+    PST = t.stop();
+    bilu_opts o;
+    bilu_opts_default(&o);
+    o.precond = BILU_PRECOND_FACTORS;
+    o.tol = solver_results.GetATOL() > 0 ? solver_results.GetATOL() : ATOL;
+    o.max_iter = MAX_ITER;
+    IST = run_solve(P, A_pages, b_pages, x_pages, solver_results, o, F.p, nullptr);
+    OET = PST + IST;
+    solver_results.SetTimeTaken(OET);
+    copy_back(x_pages, solver_results);
+}
+
+void Batched_ILU_ISAI_Preconditioned_BiCGSTAB_Gpu(const std::vector<std::string> &, const PagedCSRMatrices &A_pages,
+                                                  const PagedVectors &b_pages, PagedVectors &x_pages, const bool,
+                                                  SolverResults &solver_results, const bool is_parilu,
+                                                  const int parilu_sweeps, double &PST, double &IST, double &OET)
+{
+    BiluSolveOptions s;
+    s.precond = is_parilu ? BILU_PRECOND_PARILU0_ISAI : BILU_PRECOND_ILU0_ISAI;
+    s.parilu_sweeps = parilu_sweeps;
+    s.tol = solver_results.GetATOL() > 0 ? solver_results.GetATOL() : ATOL;
+    IST = OET = Batched_Preconditioned_BiCGSTAB_Gpu(A_pages, b_pages, x_pages, solver_results, s);
+    PST = 0.0;
+}
+
+void Batched_General_ISAI_Preconditioned_BiCGSTAB_Gpu(const std::vector<std::string> &,
+                                                      const PagedCSRMatrices &A_pages, const PagedVectors &b_pages,
+                                                      PagedVectors &x_pages, const bool, SolverResults &solver_results,
+                                                      double &PST, double &IST, double &OET)
+{
+    BiluSolveOptions s;
+    s.precond = BILU_PRECOND_GENERAL_ISAI;
+    s.tol = solver_results.GetATOL() > 0 ? solver_results.GetATOL() : ATOL;
+    IST = OET = Batched_Preconditioned_BiCGSTAB_Gpu(A_pages, b_pages, x_pages, solver_results, s);
+    PST = 0.0;
+}
+
+void Batched_conv_ILU_app1_Preconditioned_BiCGSTAB_merged_Gpu(const std::vector<std::string> &,
+                                                              const PagedCSRMatrices &A_pages,
+                                                              const PagedVectors &b_pages, PagedVectors &x_pages,
+                                                              const bool, SolverResults &solver_results)
+{
+    BiluSolveOptions s;
+    s.precond = BILU_PRECOND_ILU0;
+    s.merge = BILU_MERGED;
+    s.tol = solver_results.GetATOL() > 0 ? solver_results.GetATOL() : ATOL;
+    Batched_Preconditioned_BiCGSTAB_Gpu(A_pages, b_pages, x_pages, solver_results, s);
+}
+
+void Batched_conv_ILU_app1_Preconditioned_BiCGSTAB_segregated_Gpu(const std::vector<std::string> &,
+                                                                  const PagedCSRMatrices &A_pages,
+                                                                  const PagedVectors &b_pages, PagedVectors &x_pages,
+                                                                  const bool, SolverResults &solver_results)
+{
+    BiluSolveOptions s;
+    s.precond = BILU_PRECOND_ILU0;
+    s.merge = BILU_SEGREGATED;
+    s.tol = solver_results.GetATOL() > 0 ? solver_results.GetATOL() : ATOL;
+    Batched_Preconditioned_BiCGSTAB_Gpu(A_pages, b_pages, x_pages, solver_results, s);
+}
+
+void ILU_0_Factorization_Gpu(const PagedCSRMatrices &A_pages, PagedCSRMatrices &L_pages, PagedCSRMatrices &U_pages,
+                             const int)
+{
+    if (!A_pages.ExistsGPU()) const_cast<PagedCSRMatrices &>(A_pages).CopyFromCpuToGpu();
+    PatternHandle P(A_pages);
+    DeviceBuffer F((size_t)A_pages.GetNumPages() * A_pages.GetNumNz());
+    check(bilu_ilu0(P.h, A_pages.GetNumPages(), A_pages.GetPtrToGpuValues(), F.p, nullptr, nullptr), "bilu_ilu0");
+    split_into(P, A_pages, F.p, L_pages, U_pages);
+}
+
+void ComputeILU0Approach1(PagedCSRMatrices &Factored_Pages, const int *const)
+{
+    if (!Factored_Pages.ExistsGPU()) Factored_Pages.CopyFromCpuToGpu();
+    PatternHandle P(Factored_Pages);
+    double *v = Factored_Pages.GetPtrToGpuValues();
+    check(bilu_ilu0(P.h, Factored_Pages.GetNumPages(), v, v, nullptr, nullptr), "bilu_ilu0");
+    hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+}
+
+void ParILU_0_Factorization_Gpu(const PagedCSRMatrices &A_pages, PagedCSRMatrices &L_pages, PagedCSRMatrices &U_pages,
+                                const int num_iterations)
+{
+    if (!A_pages.ExistsGPU()) const_cast<PagedCSRMatrices &>(A_pages).CopyFromCpuToGpu();
+    PatternHandle P(A_pages);
+    DeviceBuffer F((size_t)A_pages.GetNumPages() * A_pages.GetNumNz());
+    check(bilu_parilu0(P.h, A_pages.GetNumPages(), A_pages.GetPtrToGpuValues(), nullptr, F.p, num_iterations, nullptr),
+          "bilu_parilu0");
+    split_into(P, A_pages, F.p, L_pages, U_pages);
+}
+
+void GenerateISAI_gpu(PagedCSRMatrices &aiA_pages, const PagedCSRMatrices &A_pages, const mat_type A_type,
+                      const int power)
+{
+    if (power != 1) throw std::invalid_argument("GenerateISAI_gpu: only power 1 (the input's pattern) is supported");
+    if (!A_pages.ExistsGPU()) const_cast<PagedCSRMatrices &>(A_pages).CopyFromCpuToGpu();
+    PatternHandle P(A_pages);
+    const int type = A_type == mat_type::lower_tri ? BILU_ISAI_LOWER
+                   : A_type == mat_type::upper_tri ? BILU_ISAI_UPPER : BILU_ISAI_GENERAL;
+    const int n = A_pages.GetNumRows(), nnz = A_pages.GetNumNz();
+    aiA_pages.DeallocateMemory(LOCATION::CPU);
+    aiA_pages.DeallocateMemory(LOCATION::GPU);
+    aiA_pages.SetNumPages(A_pages.GetNumPages());
+    aiA_pages.SetNumRows(n);
+    aiA_pages.SetNumCols(n);
+    aiA_pages.SetNumNz(nnz);
+    aiA_pages.AllocateMemory(LOCATION::GPU);
+    hip_check(hipMemcpy(aiA_pages.GetPtrToGpuRowPtrs(), A_pages.GetPtrToGpuRowPtrs(), sizeof(int) * (n + 1),
+                        hipMemcpyDeviceToDevice), "D2D");
+    hip_check(hipMemcpy(aiA_pages.GetPtrToGpuColInd(), A_pages.GetPtrToGpuColInd(), sizeof(int) * nnz,
+                        hipMemcpyDeviceToDevice), "D2D");
+    hip_check(hipMemset(aiA_pages.GetPtrToGpuValues(), 0, sizeof(double) * (size_t)A_pages.GetNumPages() * nnz), "memset");
+    check(bilu_isai(P.h, A_pages.GetNumPages(), A_pages.GetPtrToGpuValues(), type, aiA_pages.GetPtrToGpuValues(),
+                    nullptr, nullptr),
+          "bilu_isai");
+    hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+}

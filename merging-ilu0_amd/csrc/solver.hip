@@ -1270,8 +1270,9 @@ int bilu_solve_ex(bilu_pattern_t h, int batch, const double *d_A, const double *
     if (batch == 0) return BILU_OK;
     bilu_opts o;
     if (opts) o = *opts; else bilu_opts_default(&o);
-    if (o.precond < BILU_PRECOND_ILU0 || o.precond > BILU_PRECOND_NONE || o.max_iter < 0 ||
+    if (o.precond < BILU_PRECOND_ILU0 || o.precond > BILU_PRECOND_FACTORS || o.max_iter < 0 ||
         o.parilu_sweeps < 0 || !(o.tol >= 0.0) ||
+        (o.precond == BILU_PRECOND_FACTORS && (!d_F_init || o.precision != BILU_PRECISION_FP64)) ||
         (o.precision != BILU_PRECISION_FP64 && o.precision != BILU_PRECISION_FP32_IR) ||
         !(o.parilu_stop_tol >= 0.0))
         return BILU_ERR_INVALID_ARG;
@@ -1284,6 +1285,26 @@ int bilu_solve_ex(bilu_pattern_t h, int batch, const double *d_A, const double *
     const bool par = o.precond == BILU_PRECOND_PARILU0 || o.precond == BILU_PRECOND_PARILU0_ISAI;
     const bool warm = par && (d_F_init || o.parilu_stop_tol > 0.0);
     const bool split = o.precision == BILU_PRECISION_FP32_IR || warm || d_factor_status;
+    if (o.precond == BILU_PRECOND_FACTORS) {
+        // caller's factors: the solve launch only (persistent kernel when the pattern fits)
+        SolveParams sp{o.tol, o.tol_mode, o.max_iter, o.true_res_restart, o.max_restarts, FK_LOAD, 0};
+        int rc = -1;
+        if (o.kernel != BILU_KERNEL_WAVE)
+            rc = launch_fast(*h, batch, d_A, d_b, d_x, d_F_init, sp, d_iter, d_conv, d_iter_res, d_true_res, st);
+        if (rc == -1) {
+            if (o.kernel == BILU_KERNEL_PERSISTENT) return BILU_ERR_TOO_LARGE;
+            const size_t lds = sizeof(double) * ((size_t)h->pa.nnz * 2 + (size_t)h->pa.n * 3);
+            if (lds > h->max_lds) return BILU_ERR_TOO_LARGE;
+            rc = launch_solve<PC_LU>(*h, batch, d_A, d_b, d_x, d_F_init, sp, d_iter, d_conv, d_iter_res,
+                                     d_true_res, lds, st);
+        }
+        if (rc == BILU_OK && (d_refine || d_factor_status)) {
+            if (d_refine) hipLaunchKernelGGL(k_fill_int, dim3((batch + 255) / 256), dim3(256), 0, st, d_refine, 0, batch);
+            if (d_factor_status) hipLaunchKernelGGL(k_fill_int, dim3((batch + 255) / 256), dim3(256), 0, st, d_factor_status, 0, batch);
+            HIP_TRY(hipGetLastError());
+        }
+        return rc;
+    }
     if (!split) {
         const int rc = solve_fp64(h, batch, d_A, d_b, d_x, o, d_iter, d_conv, d_iter_res, d_true_res, stream);
         if (rc == BILU_OK && d_refine) {
