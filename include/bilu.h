@@ -166,6 +166,34 @@ int bilu_isai(bilu_pattern_t p, int batch, const double *d_vals, int type, doubl
 int bilu_true_residual(bilu_pattern_t p, int batch, const double *d_A, const double *d_b,
                        const double *d_x, double *d_res, void *stream);
 
+/* ---- uneven batches (no counterpart in the reference: its batches share one
+ * pattern).  A request list over several patterns, solved by the
+ * mixed-precision path (opts: precision BILU_PRECISION_FP32_IR, precond
+ * ILU0 / PARILU0 / ILU0_ISAI / PARILU0_ISAI).  Request r solves system
+ * req_index[r] of pattern req_pattern[r] (arrays d_A[p] [*, nnz_p], d_b[p],
+ * d_x[p] per pattern); per-request results d_iter[r], d_conv[r],
+ * d_iter_res[r], d_true_res[r], d_refine[r] (last three may be NULL).  One
+ * persistent factor + solve kernel chain per pattern, as concurrent branches
+ * of one hipGraph, captured by the first solve and replayed while the
+ * arguments are the same.  Patterns must be distinct handles; a plan is used
+ * from one host thread at a time. */
+typedef struct bilu_uneven_s *bilu_uneven_t;
+
+int bilu_uneven_create(int npat, const bilu_pattern_t *patterns, int nreq, const int *req_pattern,
+                       const int *req_index, const bilu_opts *opts, bilu_uneven_t *out);
+int bilu_uneven_destroy(bilu_uneven_t u);
+/* autotuner: times the candidate workgroup sizes (waves per workgroup of the
+ * factor and of the solve kernel) of every pattern on its own requests with
+ * these arrays and keeps the fastest (results do not depend on the choice) */
+int bilu_uneven_tune(bilu_uneven_t u, const double *const *d_A, const double *const *d_b, double *const *d_x,
+                     float *d_iter, int *d_conv, double *d_iter_res, double *d_true_res, int *d_refine,
+                     void *stream);
+/* chosen configuration of pattern p and its number of requests */
+int bilu_uneven_config(bilu_uneven_t u, int p, int *factor_waves, int *solve_waves, int *jobs);
+int bilu_uneven_solve(bilu_uneven_t u, const double *const *d_A, const double *const *d_b, double *const *d_x,
+                      float *d_iter, int *d_conv, double *d_iter_res, double *d_true_res, int *d_refine,
+                      void *stream);
+
 /* ---- learned warm start of ParILU(0) (no counterpart in the reference) ----
  * An MLP (features asinh(a/1e-8)/24 of the nnz values of A, two hidden
  * layers of `hidden` GELU units, nnz outputs) predicts the in-place ILU(0)

@@ -31,7 +31,8 @@ EXPORTS = [
     "bilu_solve_ex", "bilu_ilu0", "bilu_parilu0", "bilu_factor_residual", "bilu_split_lu", "bilu_isai",
     "bilu_true_residual", "bilu_mlp_create", "bilu_mlp_destroy", "bilu_mlp_num_params",
     "bilu_mlp_get_weights", "bilu_mlp_set_weights", "bilu_mlp_gradient", "bilu_mlp_train_step",
-    "bilu_mlp_infer",
+    "bilu_mlp_infer", "bilu_uneven_create", "bilu_uneven_destroy", "bilu_uneven_tune", "bilu_uneven_config",
+    "bilu_uneven_solve",
 ]
 
 
@@ -91,6 +92,12 @@ def lib():
         L.bilu_split_lu.argtypes = [vp, C.c_int, vp, vp, vp, vp]
         L.bilu_isai.argtypes = [vp, C.c_int, vp, C.c_int, vp, vp, vp]
         L.bilu_true_residual.argtypes = [vp, C.c_int, vp, vp, vp, vp, vp]
+        L.bilu_uneven_create.argtypes = [C.c_int, C.POINTER(vp), C.c_int, i32p, i32p, C.POINTER(Opts),
+                                         C.POINTER(vp)]
+        L.bilu_uneven_destroy.argtypes = [vp]
+        L.bilu_uneven_tune.argtypes = [vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp), vp, vp, vp, vp, vp, vp]
+        L.bilu_uneven_config.argtypes = [vp, C.c_int, i32p, i32p, i32p]
+        L.bilu_uneven_solve.argtypes = [vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp), vp, vp, vp, vp, vp, vp]
         L.bilu_mlp_create.argtypes = [vp, C.c_int, C.c_ulonglong, C.POINTER(vp)]
         L.bilu_mlp_destroy.argtypes = [vp]
         L.bilu_mlp_num_params.argtypes = [vp, C.POINTER(C.c_longlong)]
@@ -303,3 +310,58 @@ class Mlp:
         _check(lib().bilu_mlp_infer(self.h, A.shape[0], _ptr(A), _ptr(out), _stream(stream)),
                "bilu_mlp_infer")
         return out
+
+
+class Uneven:
+    """Plan of an uneven request list over several patterns (bilu_uneven_*):
+    request r solves system req_index[r] of pattern req_pattern[r]."""
+
+    def __init__(self, patterns, req_pattern, req_index, o: Opts):
+        self.patterns = list(patterns)
+        hs = (C.c_void_p * len(self.patterns))(*[p.h.value for p in self.patterns])
+        rp = np.ascontiguousarray(req_pattern, dtype=np.int32)
+        ri = np.ascontiguousarray(req_index, dtype=np.int32)
+        self.nreq = int(rp.size)
+        h = C.c_void_p()
+        _check(lib().bilu_uneven_create(len(self.patterns), hs, self.nreq,
+                                        rp.ctypes.data_as(C.POINTER(C.c_int)),
+                                        ri.ctypes.data_as(C.POINTER(C.c_int)), C.byref(o), C.byref(h)),
+               "bilu_uneven_create")
+        self.h = h
+
+    def _ptrs(self, ts):
+        return (C.c_void_p * len(ts))(*[t.data_ptr() for t in ts])
+
+    def _call(self, fn, A, b, x, out, stream):
+        it, cv, ir, tr, rf = out
+        _check(fn(self.h, self._ptrs(A), self._ptrs(b), self._ptrs(x), _ptr(it), _ptr(cv), _ptr(ir),
+                  _ptr(tr), _ptr(rf), _stream(stream)), fn.__name__)
+
+    def outputs(self, device="cuda"):
+        import torch
+        B = self.nreq
+        return (torch.empty(B, dtype=torch.float32, device=device), torch.empty(B, dtype=torch.int32, device=device),
+                torch.empty(B, dtype=torch.float64, device=device), torch.empty(B, dtype=torch.float64, device=device),
+                torch.empty(B, dtype=torch.int32, device=device))
+
+    def tune(self, A, b, x, out, stream=None):
+        self._call(lib().bilu_uneven_tune, A, b, x, out, stream)
+
+    def solve(self, A, b, x, out, stream=None):
+        self._call(lib().bilu_uneven_solve, A, b, x, out, stream)
+
+    def config(self, p):
+        fw, sw, jobs = C.c_int(), C.c_int(), C.c_int()
+        _check(lib().bilu_uneven_config(self.h, p, C.byref(fw), C.byref(sw), C.byref(jobs)), "bilu_uneven_config")
+        return fw.value, sw.value, jobs.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().bilu_uneven_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

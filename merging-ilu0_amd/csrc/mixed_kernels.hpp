@@ -95,8 +95,10 @@ template <typename TO>
 __global__ __launch_bounds__(KF_MAX_WAVES * WAVE) void k_factor_fast(
     FastParams fp, int n, int nnz, int batch, const double *__restrict__ gA,
     const double *__restrict__ gFinit, TO *__restrict__ gF, int mode, int sweeps, double stop_tol,
-    int *__restrict__ g_status, int isai, float *__restrict__ gM, int *__restrict__ gZ)
+    int *__restrict__ g_status, int isai, float *__restrict__ gM, int *__restrict__ gZ,
+    const int *__restrict__ sysidx)
 {
+    // job j factors system sysidx[j] (identity when sysidx is null); outputs are per job
     extern __shared__ __align__(16) unsigned char smem[];
     const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
     // per wave: F[nnz + 2] FP64 (F[nnz] = 0, F[nnz + 1] = 1), then the ParILU
@@ -111,13 +113,13 @@ __global__ __launch_bounds__(KF_MAX_WAVES * WAVE) void k_factor_fast(
         if (lane == 0) sys = atomicAdd(fp.counter, 1);
         sys = __builtin_amdgcn_readfirstlane(sys);
         if (sys >= batch) break;
-        const double *Ap = gA + (size_t)sys * nnz;
+        const double *Ap = gA + (size_t)(sysidx ? sysidx[sys] : sys) * nnz;
         int status = 0;
         for (int e = lane; e < nnz; e += WAVE) F[e] = Ap[e];
         if (mode == FS_PARILU) {
             if (gFinit) {
                 // entries with work start from the guess; the others are exact (= a)
-                const double *Ip = gFinit + (size_t)sys * nnz;
+                const double *Ip = gFinit + (size_t)(sysidx ? sysidx[sys] : sys) * nnz;
                 for (int idx = lane; idx < fp.ne; idx += WAVE) {
                     const int pos = fp.etab[idx].x;
                     F[pos] = Ip[pos];
@@ -258,13 +260,20 @@ constexpr int PCK_LU = 0, PCK_ISAI = 1;
 // PCK_LU: gF = FP32 in-place factors (triangular solves); PCK_ISAI: gF =
 // FP32 ISAI(L) / ISAI(U) values on the pattern, gZ = zero-pivot flags of
 // their generation (two SpMVs per apply)
+// waves per workgroup (launch bound) of a k_mixed instantiation: 72 slots
+// (patterns up to ~4600 nnz) need the register budget of 8 waves
+constexpr int km_waves(int SM, int PCK) { return PCK == PCK_LU ? (SM > 48 ? 8 : KM_MAX_WAVES) : KMI_MAX_WAVES; }
+
 template <int NR, int SM, int PCK>
-__global__ __launch_bounds__((PCK == PCK_LU ? KM_MAX_WAVES : KMI_MAX_WAVES) * WAVE) void k_mixed(
+__global__ __launch_bounds__(km_waves(SM, PCK) * WAVE) void k_mixed(
     DevPattern P, FastParams fp, int batch, const double *__restrict__ gA,
     const double *__restrict__ gb, double *__restrict__ gx, const float *__restrict__ gF,
     const int *__restrict__ gZ, MixedParams mp, float *__restrict__ g_iter, int *__restrict__ g_conv,
-    double *__restrict__ g_ires, double *__restrict__ g_tres, int *__restrict__ g_refine)
+    double *__restrict__ g_ires, double *__restrict__ g_tres, int *__restrict__ g_refine,
+    const int *__restrict__ sysidx, const int *__restrict__ outidx)
 {
+    // job j solves system sysidx[j] (A, b, x) and writes its results at
+    // outidx[j] (identity when null); factors / ISAI values are per job
     extern __shared__ __align__(16) unsigned char smem[];
     const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
     const int n = P.n, nnz = P.nnz, S = fp.S;
@@ -308,9 +317,11 @@ __global__ __launch_bounds__((PCK == PCK_LU ? KM_MAX_WAVES : KMI_MAX_WAVES) * WA
         if (lane == 0) sys = atomicAdd(fp.counter, 1);
         sys = __builtin_amdgcn_readfirstlane(sys);
         if (sys >= batch) break;
+        const int si = sysidx ? sysidx[sys] : sys;
+        const int oi = outidx ? outidx[sys] : sys;
 
-        const double *Ap = gA + (size_t)sys * nnz;
-        const double *bp = gb + (size_t)sys * n;
+        const double *Ap = gA + (size_t)si * nnz;
+        const double *bp = gb + (size_t)si * n;
         const float *Fp = gF + (size_t)sys * nnz;
         if (PCK == PCK_LU)
             for (int e = lane; e < nnz; e += WAVE) F[e] = Fp[e];
@@ -477,18 +488,18 @@ __global__ __launch_bounds__((PCK == PCK_LU ? KM_MAX_WAVES : KMI_MAX_WAVES) * WA
             for (int k = 0; k < NR; k++) x.e[k] = 0.0;
             rn = bnorm;
         }
-        double *xp = gx + (size_t)sys * n;
+        double *xp = gx + (size_t)si * n;
 #pragma unroll
         for (int k = 0; k < NR; k++) {
             const int i = lane + WAVE * k;
             if (i < n) xp[i] = x.e[k];
         }
         if (lane == 0) {
-            g_iter[sys] = iter;
-            g_conv[sys] = conv;
-            if (g_ires) g_ires[sys] = (conv == BILU_ZERO_PIVOT) ? 0.0 : rn;
-            if (g_tres) g_tres[sys] = rn;
-            if (g_refine) g_refine[sys] = steps;
+            g_iter[oi] = iter;
+            g_conv[oi] = conv;
+            if (g_ires) g_ires[oi] = (conv == BILU_ZERO_PIVOT) ? 0.0 : rn;
+            if (g_tres) g_tres[oi] = rn;
+            if (g_refine) g_refine[oi] = steps;
         }
         wave_sync();
     }

@@ -994,7 +994,8 @@ int workspace(bilu_pattern_s &h, size_t bytes, void **out)
 template <typename TO>
 int launch_factor_fast(bilu_pattern_s &h, int batch, const double *A, const double *F_init, TO *F,
                        int mode, int sweeps, double stop_tol, int *status, hipStream_t st,
-                       int isai = 0, float *M = nullptr, int *Z = nullptr)
+                       int isai = 0, float *M = nullptr, int *Z = nullptr, const int *sysidx = nullptr,
+                       int max_waves = KF_MAX_WAVES)
 {
     const auto &pl = h.pa.fast;
     if (!pl.ok) return BILU_ERR_TOO_LARGE;
@@ -1006,7 +1007,7 @@ int launch_factor_fast(bilu_pattern_s &h, int batch, const double *A, const doub
     const size_t second = std::max<size_t>(mode == FS_PARILU ? (size_t)(nnz + 2) * sizeof(double) : 0,
                                            isai ? (size_t)(nnz + 4) * sizeof(float) : 0);
     fp.wave_bytes = int(((size_t)(nnz + 2) * sizeof(double) + second + 15) / 16 * 16);
-    int waves = (int)std::min<size_t>(KF_MAX_WAVES, h.max_lds / fp.wave_bytes);
+    int waves = (int)std::min<size_t>(std::min(max_waves, KF_MAX_WAVES), h.max_lds / fp.wave_bytes);
     if (waves < 1) return BILU_ERR_TOO_LARGE;
     waves = std::min(waves, std::max(1, batch));
     const size_t lds = (size_t)waves * fp.wave_bytes;
@@ -1015,35 +1016,49 @@ int launch_factor_fast(bilu_pattern_s &h, int batch, const double *A, const doub
     if (lds > 65536) HIP_TRY(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, st, fp.counter);
     hipLaunchKernelGGL(kern, dim3(grid), dim3(waves * WAVE), lds, st, fp, h.pa.n, nnz, batch, A, F_init, F,
-                       mode, sweeps, stop_tol, status, isai, M, Z);
+                       mode, sweeps, stop_tol, status, isai, M, Z, sysidx);
     HIP_TRY(hipGetLastError());
     return BILU_OK;
 }
+
+// indirection of the jobs of one launch (uneven request lists): job j uses
+// system sysidx[j] and writes results at outidx[j]; null = identity
+struct Jobs {
+    const int *sysidx = nullptr, *outidx = nullptr;
+};
 
 template <int NR, int SM, int PCK>
 int launch_mixed_t(bilu_pattern_s &h, const FastParams &fp, int grid, int waves, size_t lds, int batch,
                    const double *A, const double *b, double *x, const float *F, const int *Z,
                    const MixedParams &mp, float *it, int *cv, double *ir, double *tr, int *refine,
-                   hipStream_t st)
+                   hipStream_t st, Jobs jobs)
 {
     auto kern = k_mixed<NR, SM, PCK>;
     if (lds > 65536) HIP_TRY(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, st, fp.counter);
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(waves * WAVE), lds, st, h.dev, fp, batch, A, b, x, F, Z, mp, it, cv, ir, tr, refine);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(waves * WAVE), lds, st, h.dev, fp, batch, A, b, x, F, Z, mp, it, cv,
+                       ir, tr, refine, jobs.sysidx, jobs.outidx);
     HIP_TRY(hipGetLastError());
     return BILU_OK;
 }
 
+// SpMV slot templates of the mixed kernel (FP32 A: up to 72 slots per lane)
+int slot_template_mixed(int S)
+{
+    return S <= 24 ? 24 : S <= 36 ? 36 : S <= 48 ? 48 : S <= 72 ? 72 : -1;
+}
+
 // FP32 inner / FP64 refinement solve over FP32 factors F (pck = PCK_LU) or
-// FP32 ISAI values F with zero-pivot flags Z (pck = PCK_ISAI)
+// FP32 ISAI values F with zero-pivot flags Z (pck = PCK_ISAI); at most
+// max_waves waves per workgroup
 int launch_mixed(bilu_pattern_s &h, int batch, const double *A, const double *b, double *x, const float *F,
                  const int *Z, int pck, const MixedParams &mp, float *it, int *cv, double *ir, double *tr,
-                 int *refine, hipStream_t st)
+                 int *refine, hipStream_t st, Jobs jobs = Jobs(), int max_waves = 64)
 {
     const auto &pl = h.pa.fast;
     const int n = h.pa.n, nnz = h.pa.nnz;
     const int nr = (n + WAVE - 1) / WAVE;
-    const int sm = slot_template(pl.S);
+    const int sm = slot_template_mixed(pl.S);
     if (!pl.ok || nr > MAX_NR || pl.nr != nr || sm < 0) return BILU_ERR_TOO_LARGE;
     FastParams fp{};
     fast_tables(h, fp);
@@ -1055,7 +1070,8 @@ int launch_mixed(bilu_pattern_s &h, int batch, const double *A, const double *b,
     fp.wave_bytes = (fp.wave_bytes + 15) / 16 * 16;
     fp.shared_bytes = sm * WAVE * 4;
     if ((size_t)fp.shared_bytes >= h.max_lds) return BILU_ERR_TOO_LARGE;
-    int waves = (int)std::min<size_t>(pck == PCK_LU ? KM_MAX_WAVES : KMI_MAX_WAVES,
+    if (pck == PCK_ISAI && sm > 48) return BILU_ERR_TOO_LARGE;   // ISAI coefficients beyond the VGPR budget
+    int waves = (int)std::min<size_t>(std::min(max_waves, km_waves(sm, pck)),
                                       (h.max_lds - fp.shared_bytes) / fp.wave_bytes);
     if (waves < 1) return BILU_ERR_TOO_LARGE;
     waves = std::min(waves, std::max(1, batch));
@@ -1064,11 +1080,15 @@ int launch_mixed(bilu_pattern_s &h, int batch, const double *A, const double *b,
 #define MIXED_CASE(R, M) \
     if (nr == R && sm == M) \
         return pck == PCK_LU \
-            ? launch_mixed_t<R, M, PCK_LU>(h, fp, grid, waves, lds, batch, A, b, x, F, Z, mp, it, cv, ir, tr, refine, st) \
-            : launch_mixed_t<R, M, PCK_ISAI>(h, fp, grid, waves, lds, batch, A, b, x, F, Z, mp, it, cv, ir, tr, refine, st);
-#define MIXED_ROW(R) MIXED_CASE(R, 24) MIXED_CASE(R, 36) MIXED_CASE(R, 48)
+            ? launch_mixed_t<R, M, PCK_LU>(h, fp, grid, waves, lds, batch, A, b, x, F, Z, mp, it, cv, ir, tr, refine, st, jobs) \
+            : launch_mixed_t<R, M, PCK_ISAI>(h, fp, grid, waves, lds, batch, A, b, x, F, Z, mp, it, cv, ir, tr, refine, st, jobs);
+#define MIXED_CASE_LU(R, M) \
+    if (nr == R && sm == M) \
+        return launch_mixed_t<R, M, PCK_LU>(h, fp, grid, waves, lds, batch, A, b, x, F, Z, mp, it, cv, ir, tr, refine, st, jobs);
+#define MIXED_ROW(R) MIXED_CASE(R, 24) MIXED_CASE(R, 36) MIXED_CASE(R, 48) MIXED_CASE_LU(R, 72)
     MIXED_ROW(1) MIXED_ROW(2) MIXED_ROW(3) MIXED_ROW(4) MIXED_ROW(5)
 #undef MIXED_ROW
+#undef MIXED_CASE_LU
 #undef MIXED_CASE
     return BILU_ERR_TOO_LARGE;
 }
@@ -1324,7 +1344,7 @@ int bilu_solve_ex(bilu_pattern_t h, int batch, const double *d_A, const double *
     int rc;
     if (o.precision == BILU_PRECISION_FP32_IR) {
         const int nr = (h->pa.n + WAVE - 1) / WAVE;
-        if (slot_template(h->pa.fast.S) < 0 || nr > MAX_NR || h->pa.fast.nr != nr) return BILU_ERR_TOO_LARGE;
+        if (slot_template_mixed(h->pa.fast.S) < 0 || nr > MAX_NR || h->pa.fast.nr != nr) return BILU_ERR_TOO_LARGE;
         const size_t fbytes = ((size_t)batch * nnz * sizeof(float) + 255) / 256 * 256;
         rc = workspace(*h, fbytes + (size_t)batch * sizeof(int), &ws);
         if (rc != BILU_OK) return rc;
@@ -1462,3 +1482,5 @@ int bilu_true_residual(bilu_pattern_t h, int batch, const double *d_A, const dou
 }
 
 }  // extern "C"
+
+#include "uneven.hpp"

@@ -1,0 +1,270 @@
+// Uneven batches: one request list over several shared patterns, solved by
+// the mixed-precision path (FP32 inner, FP64 refinement).  Included at the
+// end of solver.hip (uses its internals).
+//
+// A plan groups the requests by pattern once (host), uploads the per-pattern
+// job lists (system index in the pattern's arrays, request index for the
+// results), allocates the per-pattern workspaces and one stream per pattern.
+// A solve launches, per pattern, the persistent factor kernel and the
+// persistent mixed solve kernel over that pattern's jobs; the per-pattern
+// chains are independent branches (forked from and joined back into the
+// caller's stream), so the persistent kernels of all patterns share the
+// machine, each taking work from its own counter until its jobs run out.
+// The whole launch sequence is captured into a hipGraph on the first solve
+// and replayed while the arguments stay the same.  The autotuner times the
+// candidate workgroup sizes (waves per workgroup of the factor and of the
+// solve kernel, which trade occupancy against LDS per CU) of each pattern on
+// its own jobs and keeps the fastest; results do not depend on the choice
+// (every system is processed by one wave with the same arithmetic).
+
+struct bilu_uneven_s {
+    int npat = 0, nreq = 0, device = 0;
+    std::vector<bilu_pattern_s *> pats;
+    std::vector<int> count;          // jobs per pattern
+    int *d_jobs = nullptr;           // per pattern: sysidx[count] then outidx[count]
+    std::vector<size_t> job_off;
+    bilu_opts o{};
+    std::vector<int> fwaves, swaves; // tuned waves per workgroup (factor, solve)
+    std::vector<void *> ws;          // per pattern: FP32 factors / ISAI [count * nnz], then flags [count]
+    std::vector<size_t> fbytes;
+    std::vector<hipStream_t> streams;
+    std::vector<hipEvent_t> events;  // [0] fork, [1 + p] join of branch p, [npat + 1 / + 2] caller links
+    hipStream_t cap = nullptr;       // origin stream of the graph (capture is not allowed on the null stream)
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<const void *> captured;   // arguments of the captured graph
+};
+
+namespace {
+
+void free_uneven(bilu_uneven_s *u)
+{
+    if (!u) return;
+    if (u->exec) (void)hipGraphExecDestroy(u->exec);
+    if (u->graph) (void)hipGraphDestroy(u->graph);
+    for (auto s : u->streams) (void)hipStreamDestroy(s);
+    if (u->cap) (void)hipStreamDestroy(u->cap);
+    for (auto e : u->events) (void)hipEventDestroy(e);
+    for (auto w : u->ws)
+        if (w) (void)hipFree(w);
+    if (u->d_jobs) (void)hipFree(u->d_jobs);
+    delete u;
+}
+
+// the factor + solve chain of pattern p on stream st
+int uneven_branch(bilu_uneven_s &u, int p, const double *A, const double *b, double *x, float *it, int *cv,
+                  double *ir, double *tr, int *rf, hipStream_t st, int fw, int sw)
+{
+    bilu_pattern_s &h = *u.pats[p];
+    const int cnt = u.count[p];
+    if (cnt == 0) return BILU_OK;
+    const bool isai = u.o.precond == BILU_PRECOND_ILU0_ISAI || u.o.precond == BILU_PRECOND_PARILU0_ISAI;
+    const bool par = u.o.precond == BILU_PRECOND_PARILU0 || u.o.precond == BILU_PRECOND_PARILU0_ISAI;
+    Jobs jobs;
+    jobs.sysidx = u.d_jobs + u.job_off[p];
+    jobs.outidx = jobs.sysidx + cnt;
+    float *F = static_cast<float *>(u.ws[p]);
+    int *Z = reinterpret_cast<int *>(static_cast<char *>(u.ws[p]) + u.fbytes[p]);
+    int rc = launch_factor_fast<float>(h, cnt, A, nullptr, F, par ? FS_PARILU : FS_ILU, u.o.parilu_sweeps,
+                                       par ? u.o.parilu_stop_tol : 0.0, nullptr, st, isai ? 1 : 0, F, Z,
+                                       jobs.sysidx, fw);
+    if (rc != BILU_OK) return rc;
+    MixedParams mp{u.o.tol, u.o.tol_mode, (float)u.o.inner_tol, u.o.max_iter, u.o.max_refine};
+    return launch_mixed(h, cnt, A, b, x, F, Z, isai ? PCK_ISAI : PCK_LU, mp, it, cv, ir, tr, rf, st, jobs, sw);
+}
+
+// all branches: fork from st, one chain per pattern on its stream, join into st
+int uneven_launch(bilu_uneven_s &u, const double *const *A, const double *const *b, double *const *x, float *it,
+                  int *cv, double *ir, double *tr, int *rf, hipStream_t st)
+{
+    HIP_TRY(hipEventRecord(u.events[0], st));
+    for (int p = 0; p < u.npat; p++) {
+        HIP_TRY(hipStreamWaitEvent(u.streams[p], u.events[0], 0));
+        const int rc = uneven_branch(u, p, A[p], b[p], x[p], it, cv, ir, tr, rf, u.streams[p], u.fwaves[p], u.swaves[p]);
+        if (rc != BILU_OK) return rc;
+        HIP_TRY(hipEventRecord(u.events[1 + p], u.streams[p]));
+        HIP_TRY(hipStreamWaitEvent(st, u.events[1 + p], 0));
+    }
+    return BILU_OK;
+}
+
+std::vector<const void *> uneven_args(const bilu_uneven_s &u, const double *const *A, const double *const *b,
+                                      double *const *x, float *it, int *cv, double *ir, double *tr, int *rf,
+                                      hipStream_t st)
+{
+    std::vector<const void *> a;
+    for (int p = 0; p < u.npat; p++) { a.push_back(A[p]); a.push_back(b[p]); a.push_back(x[p]); }
+    for (const void *q : {(const void *)it, (const void *)cv, (const void *)ir, (const void *)tr, (const void *)rf})
+        a.push_back(q);
+    (void)st;
+    for (int p = 0; p < u.npat; p++) { a.push_back((const void *)(intptr_t)u.fwaves[p]); a.push_back((const void *)(intptr_t)u.swaves[p]); }
+    return a;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bilu_uneven_create(int npat, const bilu_pattern_t *patterns, int nreq, const int *req_pattern,
+                       const int *req_index, const bilu_opts *opts, bilu_uneven_t *out)
+{
+    if (npat <= 0 || !patterns || nreq < 0 || (nreq > 0 && (!req_pattern || !req_index)) || !out)
+        return BILU_ERR_INVALID_ARG;
+    bilu_opts o;
+    if (opts) o = *opts; else bilu_opts_default(&o);
+    if (o.precision != BILU_PRECISION_FP32_IR ||
+        !(o.precond == BILU_PRECOND_ILU0 || o.precond == BILU_PRECOND_PARILU0 ||
+          o.precond == BILU_PRECOND_ILU0_ISAI || o.precond == BILU_PRECOND_PARILU0_ISAI) ||
+        !(o.inner_tol > 0.0 && o.inner_tol < 1.0) || o.max_refine < 1 || !(o.tol >= 0.0) || o.max_iter < 0)
+        return BILU_ERR_INVALID_ARG;
+    for (int p = 0; p < npat; p++) {
+        if (!patterns[p]) return BILU_ERR_INVALID_ARG;
+        for (int q = 0; q < p; q++)
+            if (patterns[q] == patterns[p]) return BILU_ERR_INVALID_ARG;   // a handle's counters serve one branch
+        const auto &pa = patterns[p]->pa;
+        const int nr = (pa.n + WAVE - 1) / WAVE;
+        if (!pa.fast.ok || nr > MAX_NR || slot_template_mixed(pa.fast.S) < 0) return BILU_ERR_TOO_LARGE;
+    }
+    std::vector<std::vector<int>> sys(npat), outi(npat);
+    for (int r = 0; r < nreq; r++) {
+        const int p = req_pattern[r];
+        if (p < 0 || p >= npat || req_index[r] < 0) return BILU_ERR_INVALID_ARG;
+        sys[p].push_back(req_index[r]);
+        outi[p].push_back(r);
+    }
+    auto *u = new bilu_uneven_s();
+    u->npat = npat;
+    u->nreq = nreq;
+    u->o = o;
+    u->device = patterns[0]->device;
+    if (hipSetDevice(u->device) != hipSuccess) { free_uneven(u); return BILU_ERR_HIP; }
+    std::vector<int> flat;
+    for (int p = 0; p < npat; p++) {
+        u->pats.push_back(patterns[p]);
+        u->count.push_back(int(sys[p].size()));
+        u->job_off.push_back(flat.size());
+        flat.insert(flat.end(), sys[p].begin(), sys[p].end());
+        flat.insert(flat.end(), outi[p].begin(), outi[p].end());
+        u->fwaves.push_back(KF_MAX_WAVES);
+        u->swaves.push_back(64);
+    }
+    if (hipMalloc((void **)&u->d_jobs, std::max<size_t>(1, flat.size()) * sizeof(int)) != hipSuccess ||
+        (!flat.empty() && hipMemcpy(u->d_jobs, flat.data(), flat.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess)) {
+        free_uneven(u);
+        return BILU_ERR_HIP;
+    }
+    for (int p = 0; p < npat; p++) {
+        const size_t fb = ((size_t)std::max(1, u->count[p]) * patterns[p]->pa.nnz * sizeof(float) + 255) / 256 * 256;
+        void *w = nullptr;
+        if (hipMalloc(&w, fb + (size_t)std::max(1, u->count[p]) * sizeof(int)) != hipSuccess) { free_uneven(u); return BILU_ERR_HIP; }
+        u->ws.push_back(w);
+        u->fbytes.push_back(fb);
+        hipStream_t s;
+        if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) { free_uneven(u); return BILU_ERR_HIP; }
+        u->streams.push_back(s);
+    }
+    if (hipStreamCreateWithFlags(&u->cap, hipStreamNonBlocking) != hipSuccess) { free_uneven(u); return BILU_ERR_HIP; }
+    for (int e = 0; e <= npat + 2; e++) {
+        hipEvent_t ev;
+        if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) { free_uneven(u); return BILU_ERR_HIP; }
+        u->events.push_back(ev);
+    }
+    *out = u;
+    return BILU_OK;
+}
+
+int bilu_uneven_destroy(bilu_uneven_t u)
+{
+    if (!u) return BILU_ERR_INVALID_ARG;
+    (void)hipSetDevice(u->device);
+    (void)hipDeviceSynchronize();
+    free_uneven(u);
+    return BILU_OK;
+}
+
+int bilu_uneven_tune(bilu_uneven_t u, const double *const *d_A, const double *const *d_b, double *const *d_x,
+                     float *d_iter, int *d_conv, double *d_iter_res, double *d_true_res, int *d_refine, void *stream)
+{
+    if (!u || !d_A || !d_b || !d_x || !d_iter || !d_conv) return BILU_ERR_INVALID_ARG;
+    HIP_TRY(hipSetDevice(u->device));
+    hipStream_t st = (hipStream_t)stream;
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    const int fcand[] = {2, 4, 8}, scand[] = {4, 8, 12};
+    int rc = BILU_OK;
+    for (int p = 0; p < u->npat && rc == BILU_OK; p++) {
+        if (u->count[p] == 0) continue;
+        float best = 1e30f;
+        int bf = u->fwaves[p], bs = u->swaves[p];
+        for (int fw : fcand)
+            for (int sw : scand) {
+                // one untimed run (first-use costs), then the timed one
+                for (int rep = 0; rep < 2 && rc == BILU_OK; rep++) {
+                    if (rep == 1) { if (hipEventRecord(e0, st) != hipSuccess) rc = BILU_ERR_HIP; }
+                    if (rc == BILU_OK)
+                        rc = uneven_branch(*u, p, d_A[p], d_b[p], d_x[p], d_iter, d_conv, d_iter_res, d_true_res,
+                                           d_refine, st, fw, sw);
+                }
+                if (rc != BILU_OK) break;
+                float ms = 0.0f;
+                if (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                    hipEventElapsedTime(&ms, e0, e1) != hipSuccess) { rc = BILU_ERR_HIP; break; }
+                if (ms < best) { best = ms; bf = fw; bs = sw; }
+            }
+        u->fwaves[p] = bf;
+        u->swaves[p] = bs;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return rc;
+}
+
+int bilu_uneven_config(bilu_uneven_t u, int p, int *factor_waves, int *solve_waves, int *jobs)
+{
+    if (!u || p < 0 || p >= u->npat) return BILU_ERR_INVALID_ARG;
+    if (factor_waves) *factor_waves = u->fwaves[p];
+    if (solve_waves) *solve_waves = u->swaves[p];
+    if (jobs) *jobs = u->count[p];
+    return BILU_OK;
+}
+
+int bilu_uneven_solve(bilu_uneven_t u, const double *const *d_A, const double *const *d_b, double *const *d_x,
+                      float *d_iter, int *d_conv, double *d_iter_res, double *d_true_res, int *d_refine, void *stream)
+{
+    if (!u || !d_A || !d_b || !d_x || (u->nreq > 0 && (!d_iter || !d_conv))) return BILU_ERR_INVALID_ARG;
+    for (int p = 0; p < u->npat; p++)
+        if (u->count[p] > 0 && (!d_A[p] || !d_b[p] || !d_x[p])) return BILU_ERR_INVALID_ARG;
+    if (u->nreq == 0) return BILU_OK;
+    HIP_TRY(hipSetDevice(u->device));
+    hipStream_t st = (hipStream_t)stream;
+    const auto args = uneven_args(*u, d_A, d_b, d_x, d_iter, d_conv, d_iter_res, d_true_res, d_refine, st);
+    if (!u->exec || args != u->captured) {
+        if (u->exec) { (void)hipGraphExecDestroy(u->exec); u->exec = nullptr; }
+        if (u->graph) { (void)hipGraphDestroy(u->graph); u->graph = nullptr; }
+        HIP_TRY(hipStreamSynchronize(u->cap));
+        HIP_TRY(hipStreamBeginCapture(u->cap, hipStreamCaptureModeRelaxed));
+        const int rc = uneven_launch(*u, d_A, d_b, d_x, d_iter, d_conv, d_iter_res, d_true_res, d_refine, u->cap);
+        hipGraph_t g = nullptr;
+        const hipError_t ec = hipStreamEndCapture(u->cap, &g);
+        if (rc != BILU_OK) { if (g) (void)hipGraphDestroy(g); return rc; }
+        if (ec != hipSuccess || !g) return BILU_ERR_HIP;
+        u->graph = g;
+        if (hipGraphInstantiate(&u->exec, u->graph, nullptr, nullptr, 0) != hipSuccess) {
+            (void)hipGraphDestroy(u->graph);
+            u->graph = nullptr;
+            u->exec = nullptr;
+            return BILU_ERR_HIP;
+        }
+        u->captured = args;
+    }
+    // ordered after the caller's earlier work on st, and st after the graph
+    HIP_TRY(hipEventRecord(u->events[u->npat + 1], st));
+    HIP_TRY(hipStreamWaitEvent(u->cap, u->events[u->npat + 1], 0));
+    HIP_TRY(hipGraphLaunch(u->exec, u->cap));
+    HIP_TRY(hipEventRecord(u->events[u->npat + 2], u->cap));
+    HIP_TRY(hipStreamWaitEvent(st, u->events[u->npat + 2], 0));
+    return BILU_OK;
+}
+
+}  // extern "C"

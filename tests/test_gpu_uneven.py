@@ -1,0 +1,154 @@
+"""Uneven batches (several patterns, one interleaved request list) through
+the hipGraph-captured plan: bitwise the same as solving each pattern's
+systems on their own, unchanged by the autotuner's choice, stable over graph
+replays, and the oracle's results for the 72-slot (n = 300) kernels."""
+import numpy as np
+import pytest
+
+import bilu
+import oracle as O
+import synth
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+NS = (32, 90, 144, 300)
+
+
+def dev():
+    assert torch.cuda.is_available(), "GPU tests need a HIP device"
+    return torch.device("cuda:0")
+
+
+def c3(B_each, seed=0):
+    """4 patterns with n in {32, 90, 144, 300}, nnz ~ 14 n, and a random
+    interleaving of their systems in one request list."""
+    pats, As, bs = [], [], []
+    for n in NS:
+        pat, A, b = synth.batch("random", B_each, n=n, density=13.0 / (n - 1))
+        pats.append(pat); As.append(A); bs.append(b)
+    req_p = np.repeat(np.arange(len(NS)), B_each).astype(np.int32)
+    req_i = np.tile(np.arange(B_each), len(NS)).astype(np.int32)
+    perm = np.random.default_rng(seed).permutation(req_p.size)
+    return pats, As, bs, req_p[perm], req_i[perm]
+
+
+OPTS = dict(tol=1e-10, tol_mode=bilu.TOL_RELATIVE, precision=bilu.PRECISION_FP32_IR)
+
+
+def per_pattern(P, A, b, **kw):
+    x = torch.empty_like(b)
+    out = bilu.solve_ex(P, A, b, x, bilu.opts(**kw))
+    torch.cuda.synchronize()
+    return x.cpu().numpy(), [t.cpu().numpy() for t in out[:5]]
+
+
+@pytest.mark.parametrize("precond", [bilu.ILU0, bilu.ILU0_ISAI])
+def test_uneven_matches_per_pattern_and_tuning(precond):
+    print(f"case uneven precond={precond}", flush=True)
+    B_each = 384
+    pats, As, bs, req_p, req_i = c3(B_each)
+    kw = dict(OPTS, precond=precond)
+    if precond == bilu.ILU0_ISAI:   # the ISAI coefficients of n = 300 exceed the kernel's register budget
+        keep = [0, 1, 2]
+        pats, As, bs = [pats[k] for k in keep], [As[k] for k in keep], [bs[k] for k in keep]
+        sel = req_p < 3
+        req_p, req_i = req_p[sel], req_i[sel]
+    P = [bilu.Pattern.from_synth(p) for p in pats]
+    dA = [torch.from_numpy(A).to(dev()) for A in As]
+    db = [torch.from_numpy(b).to(dev()) for b in bs]
+    dx = [torch.empty_like(b) for b in db]
+    plan = bilu.Uneven(P, req_p, req_i, bilu.opts(**kw))
+    out = plan.outputs()
+    plan.solve(dA, db, dx, out)
+    torch.cuda.synchronize()
+    first = [t.cpu().numpy().copy() for t in out]
+    xs = [x.cpu().numpy().copy() for x in dx]
+    for p in range(len(P)):
+        xr, res = per_pattern(P[p], dA[p], db[p], **kw)
+        assert np.array_equal(xs[p], xr), f"pattern {p}: x differs"
+        rows = np.nonzero(req_p == p)[0]
+        for name, u, v in zip(["iters", "conv", "iter_res", "true_res", "refine"], first, res):
+            assert np.array_equal(u[rows], v[req_i[rows]]), f"pattern {p}: {name} differs"
+    assert (first[1] == 1).all()
+    # the autotuner picks workgroup sizes; the results stay bit for bit
+    plan.tune(dA, db, dx, out)
+    for p in range(len(P)):
+        fw, sw, jobs = plan.config(p)
+        assert fw in (2, 4, 8) and sw in (4, 8, 12) and jobs == B_each
+    for x in dx:
+        x.zero_()
+    plan.solve(dA, db, dx, out)
+    torch.cuda.synchronize()
+    for p in range(len(P)):
+        assert np.array_equal(dx[p].cpu().numpy(), xs[p])
+    for u, v in zip(first, out):
+        assert np.array_equal(u, v.cpu().numpy())
+    plan.close()
+
+
+def test_uneven_graph_replays_and_recapture():
+    print("case uneven graph replays", flush=True)
+    pats, As, bs, req_p, req_i = c3(256, seed=3)
+    P = [bilu.Pattern.from_synth(p) for p in pats]
+    dA = [torch.from_numpy(A).to(dev()) for A in As]
+    db = [torch.from_numpy(b).to(dev()) for b in bs]
+    dx = [torch.empty_like(b) for b in db]
+    plan = bilu.Uneven(P, req_p, req_i, bilu.opts(precond=bilu.ILU0, **OPTS))
+    out = plan.outputs()
+    ref = None
+    for rep in range(4):
+        for x in dx:
+            x.fill_(float("nan"))          # eager work between the replays
+        for t in out:
+            t.zero_()
+        torch.cuda.synchronize()
+        plan.solve(dA, db, dx, out)
+        torch.cuda.synchronize()
+        got = [x.cpu().numpy().copy() for x in dx] + [t.cpu().numpy().copy() for t in out]
+        if ref is None:
+            ref = got
+        else:
+            for u, v in zip(ref, got):
+                assert np.array_equal(u, v), f"replay {rep} differs"
+    # new output buffers: the plan captures a new graph
+    out2 = plan.outputs()
+    dx2 = [torch.empty_like(b) for b in db]
+    plan.solve(dA, db, dx2, out2)
+    torch.cuda.synchronize()
+    for u, v in zip(ref[:4], dx2):
+        assert np.array_equal(u, v.cpu().numpy())
+    assert np.array_equal(ref[4 + 1], out2[1].cpu().numpy())
+    plan.close()
+
+
+def test_uneven_n300_against_oracle():
+    print("case uneven n300 oracle", flush=True)
+    pat, A, b = synth.batch("random", 48, n=300, density=13.0 / 299)
+    P = bilu.Pattern.from_synth(pat)
+    dA, db = torch.from_numpy(A).to(dev()), torch.from_numpy(b).to(dev())
+    dx = torch.empty_like(db)
+    plan = bilu.Uneven([P], np.zeros(48, np.int32), np.arange(48, dtype=np.int32),
+                       bilu.opts(precond=bilu.ILU0, **OPTS))
+    out = plan.outputs()
+    plan.solve([dA], [db], [dx], out)
+    torch.cuda.synchronize()
+    o = O.batch_solve_split(pat, A, b, precision=O.PREC_FP32_IR,
+                            o=O.opts(tol=1e-10, tol_mode=O.TOL_RELATIVE, trsv_order=O.TRSV_AVAIL), threads=8)
+    assert np.array_equal(dx.cpu().numpy(), o[0])
+    for u, v in zip(out, (o[1], o[2], o[3], o[4], o[5])):
+        assert np.array_equal(u.cpu().numpy(), v)
+    assert (o[2] == 1).all()
+    plan.close()
+
+
+def test_uneven_rejects_bad_plans():
+    pats, As, bs, req_p, req_i = c3(4)
+    P = [bilu.Pattern.from_synth(p) for p in pats]
+    with pytest.raises(bilu.BiluError):     # the same handle twice
+        bilu.Uneven([P[0], P[0]], np.zeros(2, np.int32), np.arange(2, dtype=np.int32), bilu.opts(**OPTS))
+    with pytest.raises(bilu.BiluError):     # FP64 is served by bilu_solve
+        bilu.Uneven(P, req_p, req_i, bilu.opts(tol=1e-10, tol_mode=1))
+    with pytest.raises(bilu.BiluError):     # request of an unknown pattern
+        bilu.Uneven(P, np.full(2, 7, np.int32), np.arange(2, dtype=np.int32), bilu.opts(**OPTS))
