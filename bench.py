@@ -36,7 +36,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--batch", type=int, default=65536)
     ap.add_argument("--n", type=int, default=144)
-    ap.add_argument("--mode", default="headline", choices=["headline", "single"])
+    ap.add_argument("--mode", default="headline", choices=["headline", "single", "uneven"])
+    ap.add_argument("--uneven-each", type=int, default=16384)
     ap.add_argument("--precond", default="ilu0",
                     choices=["ilu0", "parilu0", "ilu0_isai", "parilu0_isai", "general_isai"])
     ap.add_argument("--merge", default="merged", choices=["merged", "segregated"])
@@ -118,6 +119,60 @@ def time_steps(step, warmup, steps, stream):
     return e0.elapsed_time(e1) / max(steps, 1)
 
 
+def uneven_bench(args, dev):
+    """Uneven batch: 4 patterns (n = 32, 90, 144, 300, nnz ~ 14 n), `uneven_each`
+    systems each, interleaved in one request list, FP32 inner / FP64
+    refinement; the plan is autotuned, then its hipGraph is replayed."""
+    import numpy as np
+    import torch
+
+    import bilu
+    import synth
+    ns = (32, 90, 144, 300)
+    E = args.uneven_each
+    pats, dA, db, dx, nnzs = [], [], [], [], []
+    for n in ns:
+        pat, A, b = synth.batch("random", E, n=n, density=13.0 / (n - 1))
+        pats.append(bilu.Pattern.from_synth(pat))
+        dA.append(torch.from_numpy(A).to(dev)); db.append(torch.from_numpy(b).to(dev))
+        dx.append(torch.empty_like(db[-1]))
+        nnzs.append(pat.nnz)
+    req_p = np.repeat(np.arange(len(ns)), E).astype(np.int32)
+    req_i = np.tile(np.arange(E), len(ns)).astype(np.int32)
+    perm = np.random.default_rng(0).permutation(req_p.size)
+    req_p, req_i = req_p[perm], req_i[perm]
+    plan = bilu.Uneven(pats, req_p, req_i, bilu.opts(precond=bilu.ILU0, tol=1e-10, tol_mode=bilu.TOL_RELATIVE,
+                                                     precision=bilu.PRECISION_FP32_IR, inner_tol=args.inner_tol))
+    out = plan.outputs()
+    t0 = time.time()
+    plan.tune(dA, db, dx, out)
+    torch.cuda.synchronize()
+    tune_s = time.time() - t0
+    stream = torch.cuda.current_stream()
+    ms = time_steps(lambda: plan.solve(dA, db, dx, out, stream=stream), args.warmup, args.steps, stream)
+    cv, tr = out[1].cpu().numpy(), out[3].cpu().numpy()
+    bn = np.concatenate([torch.linalg.norm(b, dim=1).cpu().numpy() for b in db])
+    # per-request relative residual: request r is system req_i[r] of pattern req_p[r]
+    offs = np.cumsum([0] + [E] * len(ns))
+    relres = tr / bn[offs[req_p] + req_i]
+    B = req_p.size
+    alg = sum(E * (8 * nz + 16 * n + 24) for n, nz in zip(ns, nnzs)) + 4 * B
+    achieved = alg / (ms * 1e-3) / 1e9
+    return {"metric": "systems solved/s, uneven batch (4 patterns, n = 32/90/144/300), relres<=1e-10",
+            "value": round(B / (ms * 1e-3), 1), "unit": "systems/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
+            "config": {"workload": f"{len(ns)} patterns x {E} systems interleaved, exact ILU(0) FP32 + FP32 "
+                                   f"BiCGSTAB in FP64 refinement, one hipGraph", "n": list(ns), "nnz": nnzs},
+            "all_converged_relres_le_1e-10": bool((cv == 1).all() and (relres <= 1e-10).all()),
+            "tuned": {str(n): dict(zip(("factor_waves", "solve_waves", "jobs"), plan.config(p)))
+                      for p, n in enumerate(ns)},
+            "tune_seconds": round(tune_s, 2),
+            "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                         "kernel": "whole graph (per-pattern k_factor_fast + k_mixed)", "alg_bytes_per_launch": alg}}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -140,6 +195,10 @@ def main():
             c.copy_(a)
         torch.cuda.synchronize()
         print(json.dumps({"copy_probe_bytes_read": a.numel() * 8, "copy_probe_bytes_written": a.numel() * 8}))
+        return
+
+    if args.mode == "uneven":
+        print(json.dumps(uneven_bench(args, dev)), flush=True)
         return
 
     t0 = time.time()
