@@ -46,6 +46,9 @@ struct FastParams {
     const uint32_t *isL0, *isL1, *isU0, *isU1;
     int isL_steps, isU_steps;
     const uint8_t *slot_kind;
+    // exact ILU(0) lane streams [step][64]
+    const uint32_t *ilu0, *ilu1;
+    int ilu_steps;
 };
 
 __device__ __forceinline__ void wave_sync()
@@ -162,45 +165,60 @@ __device__ __forceinline__ void trsv_avail(const uint32_t *__restrict__ rec, int
     }
 }
 
-// exact ILU(0) in pull form: entry e = (i,j) folds its updates
-// a_ij <- fma(-l_ik, u_kj, a_ij) for k increasing (the reference KIJ order,
-// merged.cu:168-219), then l_ij = a_ij / u_jj for lower entries; entries are
-// processed by dependency level, so every operand is final when read.
+// Exact ILU(0) in place by the lane streams of the pattern plan: each step
+// folds one dependency pair acc = fma(-l_ik, u_kj, acc) of the lane's
+// current entry (acc starts at a_ij on the entry's first step; the k order
+// is the reference KIJ order, merged.cu:168-219), and an entry's last step
+// writes it (lower entries divided by u_jj).  Every value is read at least
+// ILU_GAP steps after its write, so a step's LDS operands are loaded one
+// step ahead, before the previous step's store; records arrive in batches of
+// ILU_BATCH steps, the next batch in flight during the current one.
 // Returns nonzero on a zero pivot.
-__device__ __forceinline__ int ilu_pull(const ushort4 *__restrict__ etab, const uint16_t *__restrict__ eptr,
-                                        int nlev, const ushort2 *__restrict__ dep, int n,
-                                        const int *__restrict__ diag, double *F, int lane)
+__device__ __forceinline__ int ilu_stream(const uint32_t *__restrict__ r0p, const uint32_t *__restrict__ r1p,
+                                          int steps, double *F, double *scratch, const int *__restrict__ diag,
+                                          int n, int lane)
 {
+    constexpr int K = bilu::ILU_BATCH;
+    char *Fb = reinterpret_cast<char *>(F);
+    auto ld = [&](uint32_t off) { return *reinterpret_cast<const double *>(Fb + off); };
+    uint32_t ra[K], rb[K];
+#pragma unroll
+    for (int j = 0; j < K; j++) { ra[j] = r0p[j * WAVE + lane]; rb[j] = r1p[j * WAVE + lane]; }
+    double acc = 0.0;
     int bad = 0;
-    for (int l = 0; l < nlev; l++) {
-        const int b = eptr[l], e = eptr[l + 1];
-        for (int idx = b + lane; idx < e; idx += WAVE) {
-            const ushort4 r = etab[idx];
-            const int pos = r.x, db = r.y, de = r.z, pv = r.w;
-            double acc = F[pos];
-            const double dv = F[pv == 0xFFFF ? pos : pv];
-            for (int d0 = db; d0 < de; d0 += 8) {
-                ushort2 pq[8];
+    for (int s0 = 0; s0 < steps; s0 += K) {
+        const int nx = (s0 + K < steps) ? s0 + K : s0;
+        uint32_t na[K], nb[K];
 #pragma unroll
-                for (int k = 0; k < 8; k++) pq[k] = dep[d0 + k < de ? d0 + k : de - 1];
-                double lv[8], uv[8];
+        for (int j = 0; j < K; j++) { na[j] = r0p[(nx + j) * WAVE + lane]; nb[j] = r1p[(nx + j) * WAVE + lane]; }
+        double l = ld(ra[0] & 0xFFFFu), u = ld(ra[0] >> 16), in = ld(rb[0] & 0xFFF8u);
+        uint32_t pv = rb[0] >> 16;
+        double dv = ld(pv == 0xFFFFu ? 0u : pv);
 #pragma unroll
-                for (int k = 0; k < 8; k++) {
-                    lv[k] = F[pq[k].x];
-                    uv[k] = F[pq[k].y];
-                }
-#pragma unroll
-                for (int k = 0; k < 8; k++)
-                    if (d0 + k < de) acc = fma(-lv[k], uv[k], acc);
+        for (int j = 0; j < K; j++) {
+            double l1 = 0.0, u1 = 0.0, in1 = 0.0, dv1 = 0.0;
+            uint32_t pv1 = 0xFFFFu;
+            if (j + 1 < K) {
+                l1 = ld(ra[j + 1] & 0xFFFFu);
+                u1 = ld(ra[j + 1] >> 16);
+                in1 = ld(rb[j + 1] & 0xFFF8u);
+                pv1 = rb[j + 1] >> 16;
+                dv1 = ld(pv1 == 0xFFFFu ? 0u : pv1);
             }
-            if (pv != 0xFFFF) {
-                bad |= (dv == 0.0);
-                acc = acc / dv;
-            }
-            F[pos] = acc;
+            const uint32_t b = rb[j];
+            acc = (b & 1u) ? in : acc;
+            acc = fma(-l, u, acc);
+            const bool div = pv != 0xFFFFu, end = (b & 2u) != 0;
+            const double val = div ? acc / dv : acc;
+            bad |= (end && div && dv == 0.0);
+            double *dst = end ? reinterpret_cast<double *>(Fb + (b & 0xFFF8u)) : scratch;
+            *dst = val;
+            l = l1; u = u1; in = in1; pv = pv1; dv = dv1;
         }
-        wave_sync();
+#pragma unroll
+        for (int j = 0; j < K; j++) { ra[j] = na[j]; rb[j] = nb[j]; }
     }
+    wave_sync();
     // every U diagonal must be nonzero for the solves
     for (int k = lane; k < n; k += WAVE) bad |= (F[diag[k]] == 0.0);
     return __any(bad);
@@ -308,7 +326,7 @@ __global__ __launch_bounds__(KF_MAX_WAVES * WAVE) void k_fast(DevPattern P, Fast
                 if (e >= 0) a[q] = F[e];
             }
             if (sp.factor == FK_ILU) {
-                bad = ilu_pull(fp.etab, fp.eptr, fp.nlev_e, fp.dep, n, fp.diag, F, lane);
+                bad = ilu_stream(fp.ilu0, fp.ilu1, fp.ilu_steps, F, F + nnz + 1, fp.diag, n, lane);
             } else if (sp.factor == FK_PARILU) {
                 for (int sw = 0; sw < sp.sweeps; sw++) parilu_sweep(fp.etab, fp.ne, fp.dep, Ap, F, Wb, nnz, lane);
             }

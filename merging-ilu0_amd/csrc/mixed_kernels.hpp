@@ -101,13 +101,13 @@ __global__ __launch_bounds__(KF_MAX_WAVES * WAVE) void k_factor_fast(
     // job j factors system sysidx[j] (identity when sysidx is null); outputs are per job
     extern __shared__ __align__(16) unsigned char smem[];
     const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
-    // per wave: F[nnz + 2] FP64 (F[nnz] = 0, F[nnz + 1] = 1), then the ParILU
-    // buffer W[nnz + 2] FP64 and/or the ISAI M[nnz + 4] FP32 (M[nnz] = 0,
-    // M[nnz + 1] = 1, M[nnz + 2] = -1, M[nnz + 3] scratch) sharing one region
+    // per wave: F[nnz + 4] FP64 (F[nnz] = 0, F[nnz + 1] = 1, F[nnz + 2] scratch),
+    // then the ParILU buffer W[nnz + 2] FP64 and/or the ISAI M[nnz + 4] FP32
+    // (M[nnz] = 0, M[nnz + 1] = 1, M[nnz + 2] = -1, M[nnz + 3] scratch) sharing one region
     double *F = reinterpret_cast<double *>(smem + (size_t)wave * fp.wave_bytes);
-    double *W = F + (nnz + 2);
+    double *W = F + (nnz + 4);
     float *M = reinterpret_cast<float *>(W);
-    if (lane == 0) F[nnz + 1] = 1.0;
+    if (lane == 0) { F[nnz] = 0.0; F[nnz + 1] = 1.0; }
     for (;;) {
         int sys = 0;
         if (lane == 0) sys = atomicAdd(fp.counter, 1);
@@ -150,17 +150,17 @@ __global__ __launch_bounds__(KF_MAX_WAVES * WAVE) void k_factor_fast(
                 // fallback: exact ILU(0)
                 for (int e = lane; e < nnz; e += WAVE) F[e] = Ap[e];
                 wave_sync();
-                ilu_pull(fp.etab, fp.eptr, fp.nlev_e, fp.dep, n, fp.diag, F, lane);
+                ilu_stream(fp.ilu0, fp.ilu1, fp.ilu_steps, F, F + nnz + 2, fp.diag, n, lane);
                 status = -1;
             } else {
                 status = used;
             }
         } else {
             wave_sync();
-            ilu_pull(fp.etab, fp.eptr, fp.nlev_e, fp.dep, n, fp.diag, F, lane);
+            ilu_stream(fp.ilu0, fp.ilu1, fp.ilu_steps, F, F + nnz + 2, fp.diag, n, lane);
         }
         if (isai) {
-            if (lane == 0) { F[nnz] = 0.0; M[nnz] = 0.0f; M[nnz + 1] = 1.0f; M[nnz + 2] = -1.0f; }
+            if (lane == 0) { M[nnz] = 0.0f; M[nnz + 1] = 1.0f; M[nnz + 2] = -1.0f; }
             wave_sync();
             int bad = isai_gen(fp.isL0, fp.isL1, fp.isL_steps, F, M, M + nnz + 3, lane);
             bad |= isai_gen(fp.isU0, fp.isU1, fp.isU_steps, F, M, M + nnz + 3, lane);

@@ -388,6 +388,61 @@ void build_fast_plan(PatternAnalysis &pa)
     };
     isai_stream(true, fp.isL0, fp.isL1, fp.isL_steps);
     isai_stream(false, fp.isU0, fp.isU1, fp.isU_steps);
+
+    // Exact ILU(0) as per-lane step streams.  Entry e = (i, j) with work
+    // folds its dependency pairs acc = fma(-l_ik, u_kj, acc) from acc = a_ij
+    // in k order (the reference KIJ order, merged.cu:168-219), one pair per
+    // step, then (lower entries) divides by u_jj and writes the entry.  A
+    // lane runs its entries one after another; entries are list-scheduled in
+    // entry-level order onto the lane where they can start first, starting no
+    // earlier than ILU_GAP steps after every value they read was written (a
+    // wave's LDS accesses execute in program order, and the kernel loads a
+    // step's operands one step ahead).  Records: r0 = 8 pL | 8 pU << 16 (the
+    // pair; zero cell for entries without pairs), r1 = 8 pos | begin (bit 0)
+    // | end (bit 1) | (8 pdiv or 0xFFFF) << 16.  Idle steps: zero cells, no
+    // flags.
+    {
+        const uint32_t ZC = uint32_t(nnz) * 8u;
+        std::vector<int> fin(nnz, -1000000);   // step writing each entry (entries without work: initial)
+        std::vector<int> freel(64, 0);
+        std::vector<std::vector<std::pair<uint32_t, uint32_t>>> L(64);
+        const int ne = int(fp.etab.size() / 4);
+        for (int q = 0; q < ne; q++) {
+            const int e = fp.etab[4 * q], db = fp.etab[4 * q + 1], de = fp.etab[4 * q + 2], pv = fp.etab[4 * q + 3];
+            int ready = 0;
+            for (int d = db; d < de; d++)
+                ready = std::max(ready, std::max(fin[fp.dep[2 * d]], fin[fp.dep[2 * d + 1]]) + ILU_GAP);
+            if (pv != 0xFFFF) ready = std::max(ready, fin[pv] + ILU_GAP);
+            const int k = std::max(1, de - db);
+            int best = 0, bstart = 1 << 30, bidle = 1 << 30;
+            for (int l = 0; l < 64; l++) {
+                const int st = std::max(freel[l], ready), idle = st - freel[l];
+                if (st < bstart || (st == bstart && idle < bidle)) { best = l; bstart = st; bidle = idle; }
+            }
+            auto &v = L[best];
+            while ((int)v.size() < bstart) v.push_back({ZC | (ZC << 16), ZC | (0xFFFFu << 16)});
+            for (int t = 0; t < k; t++) {
+                const uint32_t r0 = (de > db) ? (uint32_t(fp.dep[2 * (db + t)]) * 8u) | ((uint32_t(fp.dep[2 * (db + t) + 1]) * 8u) << 16)
+                                              : (ZC | (ZC << 16));
+                uint32_t r1 = uint32_t(e) * 8u | (t == 0 ? 1u : 0u) | (t == k - 1 ? 2u : 0u);
+                r1 |= (pv != 0xFFFF ? uint32_t(pv) * 8u : 0xFFFFu) << 16;
+                v.push_back({r0, r1});
+            }
+            fin[e] = bstart + k - 1;
+            freel[best] = bstart + k;
+        }
+        int S = 1;
+        for (auto &v : L) S = std::max(S, (int)v.size());
+        S = (S + ILU_BATCH - 1) / ILU_BATCH * ILU_BATCH;
+        fp.ilu_steps = S;
+        fp.ilu0.assign(size_t(S) * 64, ZC | (ZC << 16));
+        fp.ilu1.assign(size_t(S) * 64, ZC | (0xFFFFu << 16));
+        for (int l = 0; l < 64; l++)
+            for (size_t t = 0; t < L[l].size(); t++) {
+                fp.ilu0[t * 64 + l] = L[l][t].first;
+                fp.ilu1[t * 64 + l] = L[l][t].second;
+            }
+    }
     fp.ok = true;
 }
 

@@ -64,7 +64,18 @@ struct FastPlan {
     std::vector<uint32_t> isL0, isL1, isU0, isU1;
     int isL_steps = 0, isU_steps = 0;
     std::vector<uint8_t> slot_kind;
+    // exact ILU(0) as per-lane step streams [step][64] (two u32 records per
+    // step; list-scheduled over the entry DAG, padded to a multiple of
+    // ILU_BATCH steps)
+    std::vector<uint32_t> ilu0, ilu1;
+    int ilu_steps = 0;
 };
+
+// steps per record batch of the ILU(0) streams (records of a batch are
+// loaded together, one batch ahead)
+constexpr int ILU_BATCH = 8;
+// steps between the write of a factor value and its first read
+constexpr int ILU_GAP = 2;
 
 struct PatternAnalysis {
     int n = 0, nnz = 0;

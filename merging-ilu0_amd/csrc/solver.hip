@@ -755,6 +755,7 @@ struct bilu_pattern_s {
     uint16_t *d_etab = nullptr, *d_eptr = nullptr, *d_dep = nullptr;
     uint32_t *d_isL0 = nullptr, *d_isL1 = nullptr, *d_isU0 = nullptr, *d_isU1 = nullptr;
     uint8_t *d_slot_kind = nullptr;
+    uint32_t *d_ilu0 = nullptr, *d_ilu1 = nullptr;
     int *d_counter = nullptr;      // 256 bytes of work counters: [0] solve kernels, [32] factor kernel
     // solve workspace (factors between the factor and solve launches), grown on demand
     void *ws = nullptr;
@@ -842,6 +843,8 @@ int upload(bilu_pattern_s &h)
         const size_t o_iU0 = put(fp.isU0.data(), fp.isU0.size() * 4);
         const size_t o_iU1 = put(fp.isU1.data(), fp.isU1.size() * 4);
         const size_t o_sk = put(fp.slot_kind.data(), fp.slot_kind.size());
+        const size_t o_i0 = put(fp.ilu0.data(), fp.ilu0.size() * 4);
+        const size_t o_i1 = put(fp.ilu1.data(), fp.ilu1.size() * 4);
         HIP_TRY(hipMalloc((void **)&h.d_fast, blob.size()));
         HIP_TRY(hipMemcpy(h.d_fast, blob.data(), blob.size(), hipMemcpyHostToDevice));
         uint8_t *b = h.d_fast;
@@ -853,6 +856,7 @@ int upload(bilu_pattern_s &h)
         h.d_isL0 = (uint32_t *)(b + o_iL0); h.d_isL1 = (uint32_t *)(b + o_iL1);
         h.d_isU0 = (uint32_t *)(b + o_iU0); h.d_isU1 = (uint32_t *)(b + o_iU1);
         h.d_slot_kind = b + o_sk;
+        h.d_ilu0 = (uint32_t *)(b + o_i0); h.d_ilu1 = (uint32_t *)(b + o_i1);
     }
     return BILU_OK;
 }
@@ -865,6 +869,26 @@ void free_handle(bilu_pattern_s *h)
     if (h->ws) (void)hipFree(h->ws);
     if (h->d_fast) (void)hipFree(h->d_fast);
     delete h;
+}
+
+// table pointers of the persistent kernels (no per-launch sizing)
+void fast_tables(const bilu_pattern_s &h, FastParams &fp)
+{
+    const auto &pl = h.pa.fast;
+    fp.trL = h.d_trL; fp.trU = h.d_trU;
+    fp.trL_words = int(pl.trL.size()); fp.trU_words = int(pl.trU.size());
+    fp.cntL = h.d_cntL; fp.cntU = h.d_cntU;
+    fp.rlevL = h.d_rlevL; fp.rlevU = h.d_rlevU;
+    fp.etab = reinterpret_cast<const ushort4 *>(h.d_etab); fp.eptr = h.d_eptr;
+    fp.dep = reinterpret_cast<const ushort2 *>(h.d_dep);
+    fp.diag = h.dev.diag;
+    fp.nlev_l = h.dev.nlev_l; fp.nlev_u = h.dev.nlev_u; fp.nlev_e = pl.nlev_e; fp.ne = pl.ne;
+    fp.S = pl.S; fp.slot_meta = h.d_slot_meta; fp.slot_pos = h.d_slot_pos;
+    fp.counter = h.d_counter;
+    fp.isL0 = h.d_isL0; fp.isL1 = h.d_isL1; fp.isU0 = h.d_isU0; fp.isU1 = h.d_isU1;
+    fp.isL_steps = pl.isL_steps; fp.isU_steps = pl.isU_steps;
+    fp.slot_kind = h.d_slot_kind;
+    fp.ilu0 = h.d_ilu0; fp.ilu1 = h.d_ilu1; fp.ilu_steps = pl.ilu_steps;
 }
 
 template <int NR, int SM>
@@ -896,15 +920,7 @@ int launch_fast(bilu_pattern_s &h, int batch, const double *A, const double *b, 
     const bool par = sp.factor == FK_PARILU;
     if (pl.nr != nr) return -1;
     FastParams fp{};
-    fp.trL = h.d_trL; fp.trU = h.d_trU;
-    fp.trL_words = int(pl.trL.size()); fp.trU_words = int(pl.trU.size());
-    fp.cntL = h.d_cntL; fp.cntU = h.d_cntU;
-    fp.rlevL = h.d_rlevL; fp.rlevU = h.d_rlevU;
-    fp.etab = reinterpret_cast<const ushort4 *>(h.d_etab); fp.eptr = h.d_eptr;
-    fp.dep = reinterpret_cast<const ushort2 *>(h.d_dep);
-    fp.diag = h.dev.diag;
-    fp.nlev_l = h.dev.nlev_l; fp.nlev_u = h.dev.nlev_u; fp.nlev_e = pl.nlev_e; fp.ne = pl.ne;
-    fp.S = pl.S; fp.slot_meta = h.d_slot_meta; fp.slot_pos = h.d_slot_pos;
+    fast_tables(h, fp);
     // per wave: F (+2 cells), (ParILU: the previous sweep's factors), e0, e1 (+2 cells each),
     // row accumulators of the solves (64 NR)
     fp.wave_bytes = int(((size_t)((par ? 2 : 1) * (nnz + 2) + 2 * (n + 2) + 64 * nr) * sizeof(double) + 15) / 16 * 16);
@@ -935,25 +951,6 @@ __global__ void k_fill_int(int *p, int v, int n)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
-}
-
-// table pointers of the persistent kernels (no per-launch sizing)
-void fast_tables(const bilu_pattern_s &h, FastParams &fp)
-{
-    const auto &pl = h.pa.fast;
-    fp.trL = h.d_trL; fp.trU = h.d_trU;
-    fp.trL_words = int(pl.trL.size()); fp.trU_words = int(pl.trU.size());
-    fp.cntL = h.d_cntL; fp.cntU = h.d_cntU;
-    fp.rlevL = h.d_rlevL; fp.rlevU = h.d_rlevU;
-    fp.etab = reinterpret_cast<const ushort4 *>(h.d_etab); fp.eptr = h.d_eptr;
-    fp.dep = reinterpret_cast<const ushort2 *>(h.d_dep);
-    fp.diag = h.dev.diag;
-    fp.nlev_l = h.dev.nlev_l; fp.nlev_u = h.dev.nlev_u; fp.nlev_e = pl.nlev_e; fp.ne = pl.ne;
-    fp.S = pl.S; fp.slot_meta = h.d_slot_meta; fp.slot_pos = h.d_slot_pos;
-    fp.counter = h.d_counter;
-    fp.isL0 = h.d_isL0; fp.isL1 = h.d_isL1; fp.isU0 = h.d_isU0; fp.isU1 = h.d_isU1;
-    fp.isL_steps = pl.isL_steps; fp.isU_steps = pl.isU_steps;
-    fp.slot_kind = h.d_slot_kind;
 }
 
 int slot_template(int S)
@@ -1006,7 +1003,7 @@ int launch_factor_fast(bilu_pattern_s &h, int batch, const double *A, const doub
     // F (FP64), then the ParILU buffer (FP64) or / and the ISAI values (FP32) in one region
     const size_t second = std::max<size_t>(mode == FS_PARILU ? (size_t)(nnz + 2) * sizeof(double) : 0,
                                            isai ? (size_t)(nnz + 4) * sizeof(float) : 0);
-    fp.wave_bytes = int(((size_t)(nnz + 2) * sizeof(double) + second + 15) / 16 * 16);
+    fp.wave_bytes = int(((size_t)(nnz + 4) * sizeof(double) + second + 15) / 16 * 16);
     int waves = (int)std::min<size_t>(std::min(max_waves, KF_MAX_WAVES), h.max_lds / fp.wave_bytes);
     if (waves < 1) return BILU_ERR_TOO_LARGE;
     waves = std::min(waves, std::max(1, batch));

@@ -152,7 +152,7 @@ def test_mixed_special_cases():
 
 
 def test_mixed_rejects_unsupported():
-    pat, A, b = synth.batch("random", 4, n=300)    # beyond the persistent kernels' slot budget
+    pat, A, b = synth.batch("random", 4, n=300, density=0.09)   # ~8400 nnz: beyond the persistent kernels' tables
     with pytest.raises(bilu.BiluError):
         run_gpu(pat, A, b, precision=bilu.PRECISION_FP32_IR, tol=1e-10, tol_mode=1)
     pat, A, b = synth.batch("random", 4, n=144)
