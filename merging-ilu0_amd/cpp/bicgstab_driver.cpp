@@ -4,19 +4,28 @@
 // results.
 //
 //   bicgstab_driver <merged|segregated|ilu|parilu|ilu_isai|parilu_isai|general_isai> in.bin out.bin
+//   bicgstab_driver <ilu0|parilu0|isai_general|isai_lower|isai_upper> in.bin out.bin
 //
 // in.bin:  int32 n, nnz, pages; int32 row_ptrs[n+1], col_ind[nnz];
 //          float64 values[pages*nnz], b[pages*n]
-// out.bin: float64 x[pages*n]; float32 iter[pages]; int32 conv[pages];
+// out.bin (solvers): float64 x[pages*n]; float32 iter[pages]; int32 conv[pages];
 //          float64 iter_res[pages], true_res[pages]; float64 PST, IST, OET
+// out.bin (ilu0 / parilu0, ILU_0_Factorization_Gpu / ParILU_0_Factorization_Gpu
+//          with 5 sweeps): int32 l_nnz, u_nnz; int32 L row_ptrs[n+1], col_ind;
+//          U row_ptrs, col_ind; float64 L values[pages*l_nnz], U values
+// out.bin (isai_*, GenerateISAI_gpu power 1 of the input's typed view):
+//          float64 values[pages*nnz]
 #include <cstdio>
 #include <cstring>
 #include <string>
 #include <vector>
 
+#include "ILU_0.h"
 #include "PrecondBiCGSTAB.h"
 #include "SolverResults.h"
+#include "isai.h"
 #include "matrix.h"
+#include "parILU_0.h"
 
 template <typename T>
 static bool rd(FILE *f, T *p, size_t n) { return std::fread(p, sizeof(T), n, f) == n; }
@@ -51,6 +60,39 @@ int main(int argc, char **argv)
     A.CopyFromCpuToGpu();
     b.AllocateMemory(LOCATION::GPU);
     b.CopyFromCpuToGpu();
+    if (variant == "ilu0" || variant == "parilu0" || variant.rfind("isai_", 0) == 0) {
+        FILE *o = std::fopen(argv[3], "wb");
+        if (!o) { std::perror("out.bin"); return 2; }
+        bool ok = true;
+        try {
+            if (variant.rfind("isai_", 0) == 0) {
+                PagedCSRMatrices M;
+                const mat_type t = variant == "isai_lower" ? mat_type::lower_tri
+                                 : variant == "isai_upper" ? mat_type::upper_tri : mat_type::general;
+                GenerateISAI_gpu(M, A, t, 1);
+                M.CopyFromGpuToCpu();
+                ok = wr(o, M.GetPtrToCpuValues(), (size_t)pages * nnz);
+            } else {
+                PagedCSRMatrices L, U;
+                if (variant == "ilu0") ILU_0_Factorization_Gpu(A, L, U, 1);
+                else ParILU_0_Factorization_Gpu(A, L, U, 5);
+                L.CopyFromGpuToCpu();
+                U.CopyFromGpuToCpu();
+                const int sz[2] = {L.GetNumNz(), U.GetNumNz()};
+                ok = wr(o, sz, 2) && wr(o, L.GetPtrToCpuRowPtrs(), n + 1) && wr(o, L.GetPtrToCpuColInd(), sz[0]) &&
+                     wr(o, U.GetPtrToCpuRowPtrs(), n + 1) && wr(o, U.GetPtrToCpuColInd(), sz[1]) &&
+                     wr(o, L.GetPtrToCpuValues(), (size_t)pages * sz[0]) &&
+                     wr(o, U.GetPtrToCpuValues(), (size_t)pages * sz[1]);
+            }
+        } catch (const std::exception &e) {
+            std::fprintf(stderr, "%s failed: %s\n", variant.c_str(), e.what());
+            std::fclose(o);
+            return 1;
+        }
+        std::fclose(o);
+        std::printf("%s: %d pages n=%d nnz=%d\n", variant.c_str(), pages, n, nnz);
+        return ok ? 0 : 1;
+    }
     PagedVectors x;
     x.SetNumPages(pages); x.SetNumElements(n);
     x.AllocateMemory(LOCATION::CPU);

@@ -14,7 +14,8 @@ HERE = os.path.dirname(os.path.abspath(bilu.__file__))
 LIB_CPP = os.path.join(HERE, "libbilu_cpp.so")
 DRIVER = os.path.join(HERE, "bicgstab_driver")
 
-ENTRY_POINTS = [
+ENTRY_POINTS = [  # C++ symbols the reference's callers link against
+
     "Batched_ILU_Preconditioned_BiCGSTAB_Gpu", "Batched_ILU_ISAI_Preconditioned_BiCGSTAB_Gpu",
     "Batched_General_ISAI_Preconditioned_BiCGSTAB_Gpu", "Batched_conv_ILU_app1_Preconditioned_BiCGSTAB_merged_Gpu",
     "Batched_conv_ILU_app1_Preconditioned_BiCGSTAB_segregated_Gpu", "ILU_0_Factorization_Gpu", "ComputeILU0Approach1",
@@ -83,3 +84,70 @@ def test_driver_matches_oracle(tmp_path, kind, B, n, variant):
     assert OET > 0 and IST > 0 and OET >= IST
     if variant in ("ilu", "parilu"):
         assert PST > 0 and OET == pytest.approx(PST + IST)
+
+
+def read_factors(path, B, n):
+    raw = open(path, "rb").read()
+    o = 0
+
+    def take(dt, cnt):
+        nonlocal o
+        a = np.frombuffer(raw, dtype=dt, count=cnt, offset=o)
+        o += a.nbytes
+        return a
+    ln, un = take(np.int32, 2)
+    lrp, lci = take(np.int32, n + 1), take(np.int32, ln)
+    urp, uci = take(np.int32, n + 1), take(np.int32, un)
+    return lrp, lci, urp, uci, take(np.float64, B * ln).reshape(B, ln), take(np.float64, B * un).reshape(B, un)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("variant", ["ilu0", "parilu0"])
+def test_driver_factorizations_match_oracle(tmp_path, variant):
+    import oracle as O
+    print(f"case drop-in factor {variant}", flush=True)
+    pat, A, b = synth.batch("random", 64, n=144)
+    fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
+    write_problem(fin, pat, A, b)
+    r = subprocess.run([DRIVER, variant, str(fin), str(fout)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    lrp, lci, urp, uci, Lv, Uv = read_factors(fout, 64, pat.n)
+    rp, ci, diag = pat.row_ptrs, pat.col_idx, pat.diag
+    for s in range(64):
+        F = O.ilu0(pat, A[s])[0] if variant == "ilu0" else O.parilu0(pat, A[s], 5)
+        # reference split: L = strict lower + explicit 1 on the diagonal, U = diagonal + upper
+        Lw, Uw, lc, uc = [], [], [], []
+        for i in range(pat.n):
+            for p in range(rp[i], diag[i]):
+                Lw.append(F[p]); lc.append(ci[p])
+            Lw.append(1.0); lc.append(i)
+            for p in range(diag[i], rp[i + 1]):
+                Uw.append(F[p]); uc.append(ci[p])
+        assert np.array_equal(lci, lc) and np.array_equal(uci, uc)
+        assert np.array_equal(Lv[s], Lw) and np.array_equal(Uv[s], Uw)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("variant,typ", [("isai_general", 0), ("isai_lower", 1), ("isai_upper", 2)])
+def test_driver_isai_matches_oracle(tmp_path, variant, typ):
+    import oracle as O
+    print(f"case drop-in {variant}", flush=True)
+    pat, A, b = synth.batch("random", 32, n=90)
+    fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
+    write_problem(fin, pat, A, b)
+    r = subprocess.run([DRIVER, variant, str(fin), str(fout)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    M = np.fromfile(fout, dtype=np.float64).reshape(32, pat.nnz)
+    rp, diag = pat.row_ptrs, pat.diag
+    for s in range(32):
+        want, st = O.isai(pat, A[s], typ)
+        assert st == 0
+        if typ == 0:
+            assert np.array_equal(M[s], want)
+        else:
+            mask = np.zeros(pat.nnz, bool)
+            for i in range(pat.n):
+                rng = range(rp[i], diag[i] + 1) if typ == 1 else range(diag[i], rp[i + 1])
+                mask[list(rng)] = True
+            assert np.array_equal(M[s][mask], want[mask])
+            assert np.all(M[s][~mask] == 0.0)

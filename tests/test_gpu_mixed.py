@@ -187,3 +187,21 @@ def test_mixed_full_batch_properties():
     idx = np.random.default_rng(0).choice(B, 64, replace=False)
     o = run_oracle(pat, A[idx], b[idx], precision=1, **kw)
     assert np.array_equal(x.cpu().numpy()[idx], o[0])
+
+
+@pytest.mark.parametrize("kw", [
+    dict(tol=1e-11, tol_mode=bilu.TOL_ABSOLUTE),                     # the reference's ATOL semantics
+    dict(tol=1e-10, tol_mode=bilu.TOL_RELATIVE, max_iter=2),         # inner solves cut short
+    dict(tol=1e-10, tol_mode=bilu.TOL_RELATIVE, max_refine=1),       # not converged after one step
+    dict(tol=1e-10, tol_mode=bilu.TOL_RELATIVE, inner_tol=0.5),      # loose inner solves, many steps
+], ids=["absolute", "max_iter2", "max_refine1", "inner_tol_loose"])
+@pytest.mark.parametrize("precond", [bilu.ILU0, bilu.ILU0_ISAI])
+def test_mixed_options_bitwise(kw, precond):
+    print(f"case mixed options {kw} precond={precond}", flush=True)
+    pat, A, b = synth.batch("random", 128, n=144)
+    full = dict(kw, precond=precond, precision=bilu.PRECISION_FP32_IR)
+    g = run_gpu(pat, A, b, **full)
+    o = run_oracle(pat, A, b, **full)
+    assert_same(g, o, f"mixed options {kw}")
+    if kw.get("max_refine") == 1:
+        assert (o[2] == bilu.NOT_CONVERGED).all() and (o[5] == 1).all()

@@ -152,3 +152,35 @@ def test_uneven_rejects_bad_plans():
         bilu.Uneven(P, req_p, req_i, bilu.opts(tol=1e-10, tol_mode=1))
     with pytest.raises(bilu.BiluError):     # request of an unknown pattern
         bilu.Uneven(P, np.full(2, 7, np.int32), np.arange(2, dtype=np.int32), bilu.opts(**OPTS))
+
+
+def test_uneven_parilu_stop_and_empty_pattern():
+    """ParILU with the residual stop through the plan (bitwise equal to the
+    per-pattern solve), with one pattern that has no requests."""
+    print("case uneven parilu", flush=True)
+    pats, As, bs, req_p, req_i = c3(128, seed=5)
+    keep = req_p != 1                      # pattern n = 90 gets no requests
+    req_p, req_i = req_p[keep], req_i[keep]
+    kw = dict(OPTS, precond=bilu.PARILU0, parilu_sweeps=40, parilu_stop_tol=1e-8)
+    P = [bilu.Pattern.from_synth(p) for p in pats]
+    dA = [torch.from_numpy(A).to(dev()) for A in As]
+    db = [torch.from_numpy(b).to(dev()) for b in bs]
+    dx = [torch.zeros_like(b) for b in db]
+    plan = bilu.Uneven(P, req_p, req_i, bilu.opts(**kw))
+    assert plan.config(1)[2] == 0
+    out = plan.outputs()
+    plan.solve(dA, db, dx, out)
+    torch.cuda.synchronize()
+    for p in (0, 2, 3):
+        xr, res = per_pattern(P[p], dA[p], db[p], **kw)
+        assert np.array_equal(dx[p].cpu().numpy(), xr)
+        rows = np.nonzero(req_p == p)[0]
+        assert np.array_equal(out[0].cpu().numpy()[rows], res[0][req_i[rows]])
+        assert np.array_equal(out[1].cpu().numpy()[rows], res[1][req_i[rows]])
+    assert (out[1].cpu().numpy() == 1).all()
+    assert (dx[1].cpu().numpy() == 0).all()     # untouched
+    plan.close()
+    # an empty request list is a no-op
+    plan0 = bilu.Uneven(P, np.zeros(0, np.int32), np.zeros(0, np.int32), bilu.opts(**kw))
+    plan0.solve(dA, db, dx, plan0.outputs())
+    plan0.close()
