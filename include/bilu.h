@@ -22,6 +22,9 @@
  *   bilu_pattern_create   <- Count_Missing_Diagonal_Elements + Find_locations_of_diagonal_elements
  *                            (factorization.h:10,17), done once per pattern on the host
  *   bilu_true_residual    <- KernelFillTrueResNorms (Bicgstab_common.hpp:294-342)
+ *   bilu_pattern_add_diagonal, bilu_expand_values
+ *                         <- Count_Missing_Diagonal_Elements + Add_Missing_Diagonal_Elements
+ *                            (factorization.h:10-15, factorization.cu:399-467)
  */
 #ifndef BILU_H
 #define BILU_H
@@ -117,6 +120,21 @@ int bilu_pattern_create(int n, int nnz, const int *row_ptrs, const int *col_idx,
                         bilu_pattern_t *out);
 int bilu_pattern_destroy(bilu_pattern_t p);
 int bilu_pattern_get_info(bilu_pattern_t p, bilu_pattern_info *info);
+/* Missing-diagonal insertion (factorization.cu:399-467): the pattern with an
+ * explicit entry (i,i) inserted at its sorted place in every row that lacks
+ * one.  Host arrays, columns sorted within each row (checked as in
+ * bilu_pattern_create).  *n_missing receives the number of inserted entries;
+ * with row_ptrs_out == NULL only counts.  Otherwise row_ptrs_out[n+1],
+ * col_idx_out[nnz + n_missing] and src_out[nnz + n_missing] (position of each
+ * new entry in the input pattern, -1 for an inserted diagonal) are written. */
+int bilu_pattern_add_diagonal(int n, int nnz, const int *row_ptrs, const int *col_idx,
+                              int *n_missing, int *row_ptrs_out, int *col_idx_out, int *src_out);
+/* Values of a batch on the augmented pattern (update_col_idxs_and_values_diagonal_add,
+ * factorization.cu:169-189): d_out[s*nnz_out + k] = d_src[k] < 0 ? 0 :
+ * d_vals[s*nnz + d_src[k]].  d_src[nnz_out] is the device copy of src_out
+ * above (entries < nnz). */
+int bilu_expand_values(int batch, int nnz, int nnz_out, const int *d_src, const double *d_vals,
+                       double *d_out, void *stream);
 /* host copies of the reference L/U split patterns (sizes from get_info) */
 int bilu_pattern_lu_split(bilu_pattern_t p, int *l_row_ptrs, int *l_col_idx,
                           int *u_row_ptrs, int *u_col_idx);

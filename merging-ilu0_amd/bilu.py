@@ -16,6 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libbilu.so")
 
 OK = 0
+ERR_INVALID_ARG, ERR_INVALID_PATTERN, ERR_MISSING_DIAGONAL, ERR_TOO_LARGE, ERR_HIP, ERR_NO_DEVICE = 1, 2, 3, 4, 5, 6
 CONVERGED, NOT_CONVERGED, ZERO_PIVOT, NONFINITE = 1, -1, -2, -3
 ILU0, PARILU0, ILU0_ISAI, PARILU0_ISAI, GENERAL_ISAI, NONE, FACTORS = 0, 1, 2, 3, 4, 5, 6
 TOL_ABSOLUTE, TOL_RELATIVE = 0, 1
@@ -32,7 +33,7 @@ EXPORTS = [
     "bilu_true_residual", "bilu_mlp_create", "bilu_mlp_destroy", "bilu_mlp_num_params",
     "bilu_mlp_get_weights", "bilu_mlp_set_weights", "bilu_mlp_gradient", "bilu_mlp_train_step",
     "bilu_mlp_infer", "bilu_uneven_create", "bilu_uneven_destroy", "bilu_uneven_tune", "bilu_uneven_config",
-    "bilu_uneven_solve",
+    "bilu_uneven_solve", "bilu_pattern_add_diagonal", "bilu_expand_values",
 ]
 
 
@@ -83,6 +84,8 @@ def lib():
         L.bilu_pattern_destroy.argtypes = [vp]
         L.bilu_pattern_get_info.argtypes = [vp, C.POINTER(PatternInfo)]
         L.bilu_pattern_lu_split.argtypes = [vp, i32p, i32p, i32p, i32p]
+        L.bilu_pattern_add_diagonal.argtypes = [C.c_int, C.c_int, i32p, i32p, i32p, i32p, i32p, i32p]
+        L.bilu_expand_values.argtypes = [C.c_int, C.c_int, C.c_int, vp, vp, vp, vp]
         L.bilu_solve.argtypes = [vp, C.c_int, vp, vp, vp, C.POINTER(Opts), vp, vp, vp, vp, vp]
         L.bilu_solve_ex.argtypes = [vp, C.c_int, vp, vp, vp, C.POINTER(Opts), vp, vp, vp, vp, vp,
                                     vp, vp, vp]
@@ -182,6 +185,31 @@ class Pattern:
             self.close()
         except Exception:
             pass
+
+
+def add_diagonal(n, row_ptrs, col_idx):
+    """Pattern with every missing diagonal entry inserted (host, no device
+    needed): returns (row_ptrs, col_idx, src), src[k] = position in the input
+    pattern or -1 for an inserted (i,i)."""
+    rp = np.ascontiguousarray(row_ptrs, dtype=np.int32)
+    ci = np.ascontiguousarray(col_idx, dtype=np.int32)
+    P = C.POINTER(C.c_int)
+    miss = C.c_int()
+    _check(lib().bilu_pattern_add_diagonal(n, len(ci), rp.ctypes.data_as(P), ci.ctypes.data_as(P), C.byref(miss),
+                                           None, None, None), "bilu_pattern_add_diagonal")
+    rpo = np.empty(n + 1, np.int32)
+    cio = np.empty(len(ci) + miss.value, np.int32)
+    src = np.empty(len(ci) + miss.value, np.int32)
+    _check(lib().bilu_pattern_add_diagonal(n, len(ci), rp.ctypes.data_as(P), ci.ctypes.data_as(P), C.byref(miss),
+                                           rpo.ctypes.data_as(P), cio.ctypes.data_as(P), src.ctypes.data_as(P)),
+           "bilu_pattern_add_diagonal")
+    return rpo, cio, src
+
+
+def expand_values(src, vals, out, stream=None):
+    """out[s, k] = 0 if src[k] < 0 else vals[s, src[k]] (device tensors)"""
+    _check(lib().bilu_expand_values(vals.shape[0], vals.shape[1], out.shape[1], _ptr(src), _ptr(vals), _ptr(out),
+                                    _stream(stream)), "bilu_expand_values")
 
 
 def solve(pat: Pattern, A, b, x, o: Opts | None = None, iters=None, conv=None, iter_res=None,

@@ -38,7 +38,7 @@ def build(verbose: bool = False, extra=()) -> str:
     host = ["-O2", "-std=c++17", "-fPIC", "-Wno-unused-command-line-argument"]
     link = ["-L" + HERE, "-lbilu", "-Wl,-rpath,$ORIGIN"]
     _run([HIPCC, *host, "-shared", os.path.join(CPP, "containers.cpp"), os.path.join(CPP, "drop_in.cpp"),
-          *link, "-o", LIB_CPP + ".tmp"], verbose)
+          os.path.join(CPP, "ReadWriteData.cpp"), *link, "-o", LIB_CPP + ".tmp"], verbose)
     os.replace(LIB_CPP + ".tmp", LIB_CPP)
     _run([HIPCC, *host, os.path.join(CPP, "bicgstab_driver.cpp"), "-L" + HERE, "-lbilu_cpp", "-lbilu",
           "-Wl,-rpath,$ORIGIN", "-o", DRIVER + ".tmp"], verbose)

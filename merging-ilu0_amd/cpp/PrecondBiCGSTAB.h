@@ -7,10 +7,15 @@
 // residual, MAX_ITER full iterations, x_0 = 0, per-page iteration count in
 // half steps and flags 1 / -1 (library flags -2 zero pivot, -3 non-finite
 // extend them).  After the solve, x and the results are copied back to the
-// host arrays (allocated if absent), as the reference's helpers do.
-// subdir / is_scaled name the reference's output files; writing solution
-// files is not part of this library (the arrays are returned instead).
-// Errors raise std::runtime_error (never exit / abort).
+// host arrays (allocated if absent), as the reference's helpers do, and
+// when subdir is not empty the first subdir.size() pages of x are written
+// to subdir[p]/x_gpu_<solver>.mtx (x_scaled_gpu_... when is_scaled), the
+// reference's answer files.  Patterns without a full diagonal: the ILU and
+// ILU-ISAI solvers insert explicit zero diagonals first, as the reference's
+// factorizations do, and then solve on that pattern; the merged and
+// segregated solvers reject them, as the reference asserts, and so does the
+// general ISAI solver.  Errors raise std::runtime_error or
+// std::invalid_argument (never exit / abort).
 #pragma once
 
 #include <string>

@@ -1,10 +1,19 @@
 // Driver of the drop-in C++ interface, in the manner of the reference's
-// main_bicgstab_*.cpp programs: loads a batch from a raw binary file, calls
-// one of the reference-named entry points and writes x and the per-page
-// results.
+// main_bicgstab_*.cpp programs: loads a batch from a raw binary file or a
+// Matrix Market tree, calls one of the reference-named entry points and
+// writes x and the per-page results.
 //
 //   bicgstab_driver <merged|segregated|ilu|parilu|ilu_isai|parilu_isai|general_isai> in.bin out.bin
 //   bicgstab_driver <ilu0|parilu0|isai_general|isai_lower|isai_upper> in.bin out.bin
+//   bicgstab_driver <variant> --mtx <category_dir/> <problem_size> <1 for scaled> [out.bin]
+//   bicgstab_driver convert --mtx <category_dir/> <problem_size> <1 for scaled> out.bin
+//
+// --mtx reads the batch from a tree of Matrix Market problems (FillSubdir /
+// ReadData: Problem_Size copies of every subdirectory's A.mtx / b.mtx); the
+// solvers then write x_gpu_*.mtx into each problem directory and
+// Results_<variant>_bicgstab_<category>.txt (convergence flag, iterations per
+// problem) into the working directory.  convert writes the batch read from
+// the tree in the in.bin layout and touches no GPU.
 //
 // in.bin:  int32 n, nnz, pages; int32 row_ptrs[n+1], col_ind[nnz];
 //          float64 values[pages*nnz], b[pages*n]
@@ -17,11 +26,13 @@
 //          float64 values[pages*nnz]
 #include <cstdio>
 #include <cstring>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
 #include "ILU_0.h"
 #include "PrecondBiCGSTAB.h"
+#include "ReadWriteData.h"
 #include "SolverResults.h"
 #include "isai.h"
 #include "matrix.h"
@@ -32,36 +43,82 @@ static bool rd(FILE *f, T *p, size_t n) { return std::fread(p, sizeof(T), n, f) 
 template <typename T>
 static bool wr(FILE *f, const T *p, size_t n) { return std::fwrite(p, sizeof(T), n, f) == n; }
 
+static void load_bin(const char *path, PagedCSRMatrices &A, PagedVectors &b)
+{
+    FILE *f = std::fopen(path, "rb");
+    if (!f) throw std::runtime_error(std::string("cannot open ") + path);
+    int hdr[3];
+    bool ok = rd(f, hdr, 3);
+    const int n = hdr[0], nnz = hdr[1], pages = hdr[2];
+    if (ok) {
+        A.SetNumPages(pages); A.SetNumRows(n); A.SetNumCols(n); A.SetNumNz(nnz);
+        A.AllocateMemory(LOCATION::CPU);
+        b.SetNumPages(pages); b.SetNumElements(n);
+        b.AllocateMemory(LOCATION::CPU);
+        ok = rd(f, A.GetPtrToCpuRowPtrs(), n + 1) && rd(f, A.GetPtrToCpuColInd(), nnz) &&
+             rd(f, A.GetPtrToCpuValues(), (size_t)pages * nnz) && rd(f, b.GetPtrToCpuValues(), (size_t)pages * n);
+    }
+    std::fclose(f);
+    if (!ok) throw std::runtime_error(std::string("short input ") + path);
+}
+
+static bool write_bin(const char *path, const PagedCSRMatrices &A, const PagedVectors &b)
+{
+    FILE *o = std::fopen(path, "wb");
+    if (!o) return false;
+    const int n = A.GetNumRows(), nnz = A.GetNumNz(), pages = A.GetNumPages();
+    const int hdr[3] = {n, nnz, pages};
+    const bool ok = wr(o, hdr, 3) && wr(o, A.GetPtrToCpuRowPtrs(), n + 1) && wr(o, A.GetPtrToCpuColInd(), nnz) &&
+                    wr(o, A.GetPtrToCpuValues(), (size_t)pages * nnz) && wr(o, b.GetPtrToCpuValues(), (size_t)pages * n);
+    std::fclose(o);
+    return ok;
+}
+
 int main(int argc, char **argv)
 {
-    if (argc != 4) {
-        std::fprintf(stderr, "usage: %s <variant> in.bin out.bin\n", argv[0]);
+    const bool mtx = argc >= 6 && std::strcmp(argv[2], "--mtx") == 0;
+    if (!(argc == 4 || (mtx && argc <= 7))) {
+        std::fprintf(stderr, "usage: %s <variant> in.bin out.bin\n"
+                             "       %s <variant> --mtx <category_dir/> <problem_size> <1 for scaled> [out.bin]\n",
+                     argv[0], argv[0]);
         return 2;
     }
     const std::string variant = argv[1];
-    FILE *f = std::fopen(argv[2], "rb");
-    if (!f) { std::perror("in.bin"); return 2; }
-    int hdr[3];
-    if (!rd(f, hdr, 3)) { std::fprintf(stderr, "bad header\n"); return 2; }
-    const int n = hdr[0], nnz = hdr[1], pages = hdr[2];
+    const char *out_path = mtx ? (argc == 7 ? argv[6] : nullptr) : argv[3];
     PagedCSRMatrices A;
-    A.SetNumPages(pages); A.SetNumRows(n); A.SetNumCols(n); A.SetNumNz(nnz);
-    A.AllocateMemory(LOCATION::CPU);
     PagedVectors b;
-    b.SetNumPages(pages); b.SetNumElements(n);
-    b.AllocateMemory(LOCATION::CPU);
-    if (!rd(f, A.GetPtrToCpuRowPtrs(), n + 1) || !rd(f, A.GetPtrToCpuColInd(), nnz) ||
-        !rd(f, A.GetPtrToCpuValues(), (size_t)pages * nnz) || !rd(f, b.GetPtrToCpuValues(), (size_t)pages * n)) {
-        std::fprintf(stderr, "short input\n");
+    std::vector<std::string> subdir;
+    std::string category;
+    try {
+        if (mtx) {
+            std::string dir = argv[3];
+            if (dir.empty() || dir.back() != '/') dir += '/';
+            const size_t cut = dir.find_last_of('/', dir.size() - 2);
+            category = dir.substr(cut == std::string::npos ? 0 : cut + 1);
+            category.pop_back();
+            FillSubdir(dir, subdir);
+            ReadData(std::stoi(argv[4]), subdir, A, b, std::stoi(argv[5]) == 1);
+        } else {
+            load_bin(argv[2], A, b);
+        }
+    } catch (const std::exception &e) {
+        std::fprintf(stderr, "reading the batch failed: %s\n", e.what());
         return 2;
     }
-    std::fclose(f);
+    const int n = A.GetNumRows(), nnz = A.GetNumNz(), pages = A.GetNumPages();
+    const bool is_scaled = mtx && std::stoi(argv[5]) == 1;
+    if (variant == "convert") {
+        if (!out_path || !write_bin(out_path, A, b)) { std::fprintf(stderr, "convert: cannot write\n"); return 2; }
+        std::printf("convert: %zu problems x %s = %d pages n=%d nnz=%d\n", subdir.size(), mtx ? argv[4] : "1", pages,
+                    n, nnz);
+        return 0;
+    }
     A.AllocateMemory(LOCATION::GPU);
     A.CopyFromCpuToGpu();
     b.AllocateMemory(LOCATION::GPU);
     b.CopyFromCpuToGpu();
     if (variant == "ilu0" || variant == "parilu0" || variant.rfind("isai_", 0) == 0) {
-        FILE *o = std::fopen(argv[3], "wb");
+        FILE *o = out_path ? std::fopen(out_path, "wb") : nullptr;
         if (!o) { std::perror("out.bin"); return 2; }
         bool ok = true;
         try {
@@ -97,20 +154,19 @@ int main(int argc, char **argv)
     x.SetNumPages(pages); x.SetNumElements(n);
     x.AllocateMemory(LOCATION::CPU);
     SolverResults res(pages, ATOL, CPU_EXISTENCE::EXISTENT, GPU_EXISTENCE::EXISTENT);
-    const std::vector<std::string> subdir;
     double PST = 0, IST = 0, OET = 0;
     try {
         if (variant == "merged")
-            Batched_conv_ILU_app1_Preconditioned_BiCGSTAB_merged_Gpu(subdir, A, b, x, false, res);
+            Batched_conv_ILU_app1_Preconditioned_BiCGSTAB_merged_Gpu(subdir, A, b, x, is_scaled, res);
         else if (variant == "segregated")
-            Batched_conv_ILU_app1_Preconditioned_BiCGSTAB_segregated_Gpu(subdir, A, b, x, false, res);
+            Batched_conv_ILU_app1_Preconditioned_BiCGSTAB_segregated_Gpu(subdir, A, b, x, is_scaled, res);
         else if (variant == "ilu" || variant == "parilu")
-            Batched_ILU_Preconditioned_BiCGSTAB_Gpu(subdir, A, b, x, false, res, variant == "parilu", 1, 5, PST, IST, OET);
+            Batched_ILU_Preconditioned_BiCGSTAB_Gpu(subdir, A, b, x, is_scaled, res, variant == "parilu", 1, 5, PST, IST, OET);
         else if (variant == "ilu_isai" || variant == "parilu_isai")
-            Batched_ILU_ISAI_Preconditioned_BiCGSTAB_Gpu(subdir, A, b, x, false, res, variant == "parilu_isai", 5, PST,
+            Batched_ILU_ISAI_Preconditioned_BiCGSTAB_Gpu(subdir, A, b, x, is_scaled, res, variant == "parilu_isai", 5, PST,
                                                          IST, OET);
         else if (variant == "general_isai")
-            Batched_General_ISAI_Preconditioned_BiCGSTAB_Gpu(subdir, A, b, x, false, res, PST, IST, OET);
+            Batched_General_ISAI_Preconditioned_BiCGSTAB_Gpu(subdir, A, b, x, is_scaled, res, PST, IST, OET);
         else {
             std::fprintf(stderr, "unknown variant %s\n", variant.c_str());
             return 2;
@@ -120,13 +176,20 @@ int main(int argc, char **argv)
         return 1;
     }
     if (OET == 0) OET = IST = res.GetTimeTaken();
-    FILE *o = std::fopen(argv[3], "wb");
-    if (!o) { std::perror("out.bin"); return 2; }
+    try {
+        if (mtx) PrintSolverResults(subdir, res, "Results_" + variant + "_bicgstab_" + category + ".txt");
+    } catch (const std::exception &e) {
+        std::fprintf(stderr, "writing the results failed: %s\n", e.what());
+        return 1;
+    }
+    bool ok = true;
     const double t[3] = {PST, IST, OET};
-    const bool ok = wr(o, x.GetPtrToCpuValues(), (size_t)pages * n) && wr(o, res.GetPtrToCpuIterCount(), pages) &&
+    FILE *o = out_path ? std::fopen(out_path, "wb") : nullptr;
+    if (out_path && !o) { std::perror("out.bin"); return 2; }
+    if (o) ok = wr(o, x.GetPtrToCpuValues(), (size_t)pages * n) && wr(o, res.GetPtrToCpuIterCount(), pages) &&
                     wr(o, res.GetPtrToCpuConvFlag(), pages) && wr(o, res.GetPtrToCpuIterResNorm(), pages) &&
                     wr(o, res.GetPtrToCpuTrueResNorm(), pages) && wr(o, t, 3);
-    std::fclose(o);
+    if (o) std::fclose(o);
     std::printf("%s: %d pages n=%d nnz=%d, PST %.3f ms IST %.3f ms OET %.3f ms\n", variant.c_str(), pages, n, nnz,
                 PST, IST, OET);
     return ok ? 0 : 1;

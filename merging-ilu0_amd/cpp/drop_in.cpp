@@ -10,6 +10,7 @@
 #include "../../include/bilu.h"
 #include "ILU_0.h"
 #include "PrecondBiCGSTAB.h"
+#include "ReadWriteData.h"
 #include "SolverResults.h"
 #include "isai.h"
 #include "matrix.h"
@@ -27,25 +28,33 @@ void hip_check(hipError_t e, const char *what)
     if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
 }
 
-// analysed pattern of a page batch (host pattern arrays, copied from the
-// device when the batch lives there only); destroyed with the object
+// host copies of a page batch's pattern arrays (from the device when the
+// batch lives there only)
+void host_pattern(const PagedCSRMatrices &A, std::vector<int> &rp, std::vector<int> &ci)
+{
+    const int n = A.GetNumRows(), nnz = A.GetNumNz();
+    if (A.GetNumCols() != n) throw std::invalid_argument("batched solves need square matrices");
+    rp.resize(n + 1);
+    ci.resize(nnz);
+    if (A.ExistsCPU() && A.GetPtrToCpuRowPtrs()) {
+        std::copy(A.GetPtrToCpuRowPtrs(), A.GetPtrToCpuRowPtrs() + n + 1, rp.begin());
+        std::copy(A.GetPtrToCpuColInd(), A.GetPtrToCpuColInd() + nnz, ci.begin());
+    } else if (A.ExistsGPU()) {
+        hip_check(hipMemcpy(rp.data(), A.GetPtrToGpuRowPtrs(), sizeof(int) * (n + 1), hipMemcpyDeviceToHost), "pattern D2H");
+        hip_check(hipMemcpy(ci.data(), A.GetPtrToGpuColInd(), sizeof(int) * nnz, hipMemcpyDeviceToHost), "pattern D2H");
+    } else {
+        throw std::invalid_argument("matrix batch has no arrays");
+    }
+}
+
+// analysed pattern of a page batch; destroyed with the object
 struct PatternHandle {
     bilu_pattern_t h = nullptr;
     explicit PatternHandle(const PagedCSRMatrices &A)
     {
-        const int n = A.GetNumRows(), nnz = A.GetNumNz();
-        if (A.GetNumCols() != n) throw std::invalid_argument("batched solves need square matrices");
-        std::vector<int> rp(n + 1), ci(nnz);
-        if (A.ExistsCPU() && A.GetPtrToCpuRowPtrs()) {
-            std::copy(A.GetPtrToCpuRowPtrs(), A.GetPtrToCpuRowPtrs() + n + 1, rp.begin());
-            std::copy(A.GetPtrToCpuColInd(), A.GetPtrToCpuColInd() + nnz, ci.begin());
-        } else if (A.ExistsGPU()) {
-            hip_check(hipMemcpy(rp.data(), A.GetPtrToGpuRowPtrs(), sizeof(int) * (n + 1), hipMemcpyDeviceToHost), "pattern D2H");
-            hip_check(hipMemcpy(ci.data(), A.GetPtrToGpuColInd(), sizeof(int) * nnz, hipMemcpyDeviceToHost), "pattern D2H");
-        } else {
-            throw std::invalid_argument("matrix batch has no arrays");
-        }
-        check(bilu_pattern_create(n, nnz, rp.data(), ci.data(), &h), "bilu_pattern_create");
+        std::vector<int> rp, ci;
+        host_pattern(A, rp, ci);
+        check(bilu_pattern_create(A.GetNumRows(), A.GetNumNz(), rp.data(), ci.data(), &h), "bilu_pattern_create");
     }
     ~PatternHandle() { if (h) bilu_pattern_destroy(h); }
     PatternHandle(const PatternHandle &) = delete;
@@ -155,10 +164,64 @@ void split_into(const PatternHandle &P, const PagedCSRMatrices &A, const double 
     hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
 }
 
+// the solvers' answer files: one Matrix Market array per problem directory,
+// named after the solver (x_gpu_<stem>.mtx, x_scaled_gpu_<stem>.mtx), written
+// only when the caller passed problem directories
+void write_solution(const std::vector<std::string> &subdir, const PagedVectors &x, bool is_scaled,
+                    const std::string &stem)
+{
+    if (subdir.empty()) return;
+    Print_ans(subdir, x, (is_scaled ? "x_scaled_gpu_" : "x_gpu_") + stem + ".mtx");
+}
+
+template <typename T = double>
 struct DeviceBuffer {
-    double *p = nullptr;
-    explicit DeviceBuffer(size_t n) { hip_check(hipMalloc((void **)&p, (n ? n : 1) * sizeof(double)), "hipMalloc"); }
+    T *p = nullptr;
+    explicit DeviceBuffer(size_t n) { hip_check(hipMalloc((void **)&p, (n ? n : 1) * sizeof(T)), "hipMalloc"); }
     ~DeviceBuffer() { if (p) (void)hipFree(p); }
+    DeviceBuffer(const DeviceBuffer &) = delete;
+    DeviceBuffer &operator=(const DeviceBuffer &) = delete;
+};
+
+// The batch to factor: the caller's, or a device copy on the pattern with
+// every missing diagonal entry inserted as an explicit zero, as the
+// reference's factorizations do before factoring (ILU_0_gpu.cu:1262-1272,
+// parILU_0_gpu.cu:309-319).
+class WithDiagonal {
+    PagedCSRMatrices aug_;
+    bool used_ = false;
+
+public:
+    explicit WithDiagonal(const PagedCSRMatrices &A)
+    {
+        if (!A.ExistsGPU()) const_cast<PagedCSRMatrices &>(A).CopyFromCpuToGpu();
+        std::vector<int> rp, ci;
+        host_pattern(A, rp, ci);
+        const int n = A.GetNumRows(), nnz = A.GetNumNz();
+        int missing = 0;
+        check(bilu_pattern_add_diagonal(n, nnz, rp.data(), ci.data(), &missing, nullptr, nullptr, nullptr),
+              "bilu_pattern_add_diagonal");
+        if (missing == 0) return;
+        const int nnz2 = nnz + missing;
+        std::vector<int> rp2(n + 1), ci2(nnz2), src(nnz2);
+        check(bilu_pattern_add_diagonal(n, nnz, rp.data(), ci.data(), &missing, rp2.data(), ci2.data(), src.data()),
+              "bilu_pattern_add_diagonal");
+        aug_.SetNumPages(A.GetNumPages());
+        aug_.SetNumRows(n);
+        aug_.SetNumCols(n);
+        aug_.SetNumNz(nnz2);
+        aug_.AllocateMemory(LOCATION::GPU);
+        hip_check(hipMemcpy(aug_.GetPtrToGpuRowPtrs(), rp2.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice), "H2D");
+        hip_check(hipMemcpy(aug_.GetPtrToGpuColInd(), ci2.data(), sizeof(int) * nnz2, hipMemcpyHostToDevice), "H2D");
+        DeviceBuffer<int> d_src(nnz2);
+        hip_check(hipMemcpy(d_src.p, src.data(), sizeof(int) * nnz2, hipMemcpyHostToDevice), "H2D");
+        check(bilu_expand_values(A.GetNumPages(), nnz, nnz2, d_src.p, A.GetPtrToGpuValues(), aug_.GetPtrToGpuValues(),
+                                 nullptr),
+              "bilu_expand_values");
+        hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+        used_ = true;
+    }
+    const PagedCSRMatrices &get(const PagedCSRMatrices &A) const { return used_ ? aug_ : A; }
 };
 
 }  // namespace
@@ -176,35 +239,40 @@ double Batched_Preconditioned_BiCGSTAB_Gpu(const PagedCSRMatrices &A_pages, cons
     return ms;
 }
 
-void Batched_ILU_Preconditioned_BiCGSTAB_Gpu(const std::vector<std::string> &, const PagedCSRMatrices &A_pages,
-                                             const PagedVectors &b_pages, PagedVectors &x_pages, const bool,
-                                             SolverResults &solver_results, const bool is_parilu, const int,
+void Batched_ILU_Preconditioned_BiCGSTAB_Gpu(const std::vector<std::string> &subdir, const PagedCSRMatrices &A_pages,
+                                             const PagedVectors &b_pages, PagedVectors &x_pages, const bool is_scaled,
+                                             SolverResults &solver_results, const bool is_parilu, const int approach,
                                              const int parilu_sweeps, double &PST, double &IST, double &OET)
 {
-    prepare(A_pages, b_pages, x_pages, solver_results);
-    PatternHandle P(A_pages);
-    const int B = A_pages.GetNumPages();
-    DeviceBuffer F((size_t)B * A_pages.GetNumNz());
+    const WithDiagonal D(A_pages);
+    const PagedCSRMatrices &A = D.get(A_pages);
+    prepare(A, b_pages, x_pages, solver_results);
+    PatternHandle P(A);
+    const int B = A.GetNumPages();
+    DeviceBuffer<> F((size_t)B * A.GetNumNz());
     Timer t(nullptr);
     t.start();
     if (is_parilu)
-        check(bilu_parilu0(P.h, B, A_pages.GetPtrToGpuValues(), nullptr, F.p, parilu_sweeps, nullptr), "bilu_parilu0");
+        check(bilu_parilu0(P.h, B, A.GetPtrToGpuValues(), nullptr, F.p, parilu_sweeps, nullptr), "bilu_parilu0");
     else
-        check(bilu_ilu0(P.h, B, A_pages.GetPtrToGpuValues(), F.p, nullptr, nullptr), "bilu_ilu0");
+        check(bilu_ilu0(P.h, B, A.GetPtrToGpuValues(), F.p, nullptr, nullptr), "bilu_ilu0");
     PST = t.stop();
     bilu_opts o;
     bilu_opts_default(&o);
     o.precond = BILU_PRECOND_FACTORS;
     o.tol = solver_results.GetATOL() > 0 ? solver_results.GetATOL() : ATOL;
     o.max_iter = MAX_ITER;
-    IST = run_solve(P, A_pages, b_pages, x_pages, solver_results, o, F.p, nullptr);
+    IST = run_solve(P, A, b_pages, x_pages, solver_results, o, F.p, nullptr);
     OET = PST + IST;
     solver_results.SetTimeTaken(OET);
     copy_back(x_pages, solver_results);
+    write_solution(subdir, x_pages, is_scaled,
+                   is_parilu ? "parILU_" + std::to_string(parilu_sweeps) + "_bicgstab"
+                             : "conv_ILU_app_" + std::to_string(approach) + "_bicgstab");
 }
 
-void Batched_ILU_ISAI_Preconditioned_BiCGSTAB_Gpu(const std::vector<std::string> &, const PagedCSRMatrices &A_pages,
-                                                  const PagedVectors &b_pages, PagedVectors &x_pages, const bool,
+void Batched_ILU_ISAI_Preconditioned_BiCGSTAB_Gpu(const std::vector<std::string> &subdir, const PagedCSRMatrices &A_pages,
+                                                  const PagedVectors &b_pages, PagedVectors &x_pages, const bool is_scaled,
                                                   SolverResults &solver_results, const bool is_parilu,
                                                   const int parilu_sweeps, double &PST, double &IST, double &OET)
 {
@@ -212,13 +280,15 @@ void Batched_ILU_ISAI_Preconditioned_BiCGSTAB_Gpu(const std::vector<std::string>
     s.precond = is_parilu ? BILU_PRECOND_PARILU0_ISAI : BILU_PRECOND_ILU0_ISAI;
     s.parilu_sweeps = parilu_sweeps;
     s.tol = solver_results.GetATOL() > 0 ? solver_results.GetATOL() : ATOL;
-    IST = OET = Batched_Preconditioned_BiCGSTAB_Gpu(A_pages, b_pages, x_pages, solver_results, s);
+    const WithDiagonal D(A_pages);
+    IST = OET = Batched_Preconditioned_BiCGSTAB_Gpu(D.get(A_pages), b_pages, x_pages, solver_results, s);
     PST = 0.0;
+    write_solution(subdir, x_pages, is_scaled, "ilu_isai_bicgstab");
 }
 
-void Batched_General_ISAI_Preconditioned_BiCGSTAB_Gpu(const std::vector<std::string> &,
+void Batched_General_ISAI_Preconditioned_BiCGSTAB_Gpu(const std::vector<std::string> &subdir,
                                                       const PagedCSRMatrices &A_pages, const PagedVectors &b_pages,
-                                                      PagedVectors &x_pages, const bool, SolverResults &solver_results,
+                                                      PagedVectors &x_pages, const bool is_scaled, SolverResults &solver_results,
                                                       double &PST, double &IST, double &OET)
 {
     BiluSolveOptions s;
@@ -226,40 +296,44 @@ void Batched_General_ISAI_Preconditioned_BiCGSTAB_Gpu(const std::vector<std::str
     s.tol = solver_results.GetATOL() > 0 ? solver_results.GetATOL() : ATOL;
     IST = OET = Batched_Preconditioned_BiCGSTAB_Gpu(A_pages, b_pages, x_pages, solver_results, s);
     PST = 0.0;
+    write_solution(subdir, x_pages, is_scaled, "general_isai_bicgstab");
 }
 
-void Batched_conv_ILU_app1_Preconditioned_BiCGSTAB_merged_Gpu(const std::vector<std::string> &,
+void Batched_conv_ILU_app1_Preconditioned_BiCGSTAB_merged_Gpu(const std::vector<std::string> &subdir,
                                                               const PagedCSRMatrices &A_pages,
                                                               const PagedVectors &b_pages, PagedVectors &x_pages,
-                                                              const bool, SolverResults &solver_results)
+                                                              const bool is_scaled, SolverResults &solver_results)
 {
     BiluSolveOptions s;
     s.precond = BILU_PRECOND_ILU0;
     s.merge = BILU_MERGED;
     s.tol = solver_results.GetATOL() > 0 ? solver_results.GetATOL() : ATOL;
     Batched_Preconditioned_BiCGSTAB_Gpu(A_pages, b_pages, x_pages, solver_results, s);
+    write_solution(subdir, x_pages, is_scaled, "conv_ilu_merged_app1_bicgstab");
 }
 
-void Batched_conv_ILU_app1_Preconditioned_BiCGSTAB_segregated_Gpu(const std::vector<std::string> &,
+void Batched_conv_ILU_app1_Preconditioned_BiCGSTAB_segregated_Gpu(const std::vector<std::string> &subdir,
                                                                   const PagedCSRMatrices &A_pages,
                                                                   const PagedVectors &b_pages, PagedVectors &x_pages,
-                                                                  const bool, SolverResults &solver_results)
+                                                                  const bool is_scaled, SolverResults &solver_results)
 {
     BiluSolveOptions s;
     s.precond = BILU_PRECOND_ILU0;
     s.merge = BILU_SEGREGATED;
     s.tol = solver_results.GetATOL() > 0 ? solver_results.GetATOL() : ATOL;
     Batched_Preconditioned_BiCGSTAB_Gpu(A_pages, b_pages, x_pages, solver_results, s);
+    write_solution(subdir, x_pages, is_scaled, "conv_ilu_segregated_app1_bicgstab");
 }
 
 void ILU_0_Factorization_Gpu(const PagedCSRMatrices &A_pages, PagedCSRMatrices &L_pages, PagedCSRMatrices &U_pages,
                              const int)
 {
-    if (!A_pages.ExistsGPU()) const_cast<PagedCSRMatrices &>(A_pages).CopyFromCpuToGpu();
-    PatternHandle P(A_pages);
-    DeviceBuffer F((size_t)A_pages.GetNumPages() * A_pages.GetNumNz());
-    check(bilu_ilu0(P.h, A_pages.GetNumPages(), A_pages.GetPtrToGpuValues(), F.p, nullptr, nullptr), "bilu_ilu0");
-    split_into(P, A_pages, F.p, L_pages, U_pages);
+    const WithDiagonal D(A_pages);
+    const PagedCSRMatrices &A = D.get(A_pages);
+    PatternHandle P(A);
+    DeviceBuffer<> F((size_t)A.GetNumPages() * A.GetNumNz());
+    check(bilu_ilu0(P.h, A.GetNumPages(), A.GetPtrToGpuValues(), F.p, nullptr, nullptr), "bilu_ilu0");
+    split_into(P, A, F.p, L_pages, U_pages);
 }
 
 void ComputeILU0Approach1(PagedCSRMatrices &Factored_Pages, const int *const)
@@ -274,12 +348,13 @@ void ComputeILU0Approach1(PagedCSRMatrices &Factored_Pages, const int *const)
 void ParILU_0_Factorization_Gpu(const PagedCSRMatrices &A_pages, PagedCSRMatrices &L_pages, PagedCSRMatrices &U_pages,
                                 const int num_iterations)
 {
-    if (!A_pages.ExistsGPU()) const_cast<PagedCSRMatrices &>(A_pages).CopyFromCpuToGpu();
-    PatternHandle P(A_pages);
-    DeviceBuffer F((size_t)A_pages.GetNumPages() * A_pages.GetNumNz());
-    check(bilu_parilu0(P.h, A_pages.GetNumPages(), A_pages.GetPtrToGpuValues(), nullptr, F.p, num_iterations, nullptr),
+    const WithDiagonal D(A_pages);
+    const PagedCSRMatrices &A = D.get(A_pages);
+    PatternHandle P(A);
+    DeviceBuffer<> F((size_t)A.GetNumPages() * A.GetNumNz());
+    check(bilu_parilu0(P.h, A.GetNumPages(), A.GetPtrToGpuValues(), nullptr, F.p, num_iterations, nullptr),
           "bilu_parilu0");
-    split_into(P, A_pages, F.p, L_pages, U_pages);
+    split_into(P, A, F.p, L_pages, U_pages);
 }
 
 void GenerateISAI_gpu(PagedCSRMatrices &aiA_pages, const PagedCSRMatrices &A_pages, const mat_type A_type,

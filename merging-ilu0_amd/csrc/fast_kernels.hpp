@@ -331,6 +331,13 @@ __global__ __launch_bounds__(KF_MAX_WAVES * WAVE) void k_fast(DevPattern P, Fast
                 for (int sw = 0; sw < sp.sweeps; sw++) parilu_sweep(fp.etab, fp.ne, fp.dep, Ap, F, Wb, nnz, lane);
             }
         }
+        if (sp.factor != FK_ILU) {
+            // a zero on U's diagonal is a zero pivot, whatever produced the factors
+            // (exact ILU(0) checks its pivots as it divides)
+            wave_sync();
+            for (int k = lane; k < n; k += WAVE) bad |= (F[fp.diag[k]] == 0.0);
+            bad = __any(bad);
+        }
 
         float iter = 0.0f;
         int conv = BILU_NOT_CONVERGED;
@@ -387,14 +394,15 @@ __global__ __launch_bounds__(KF_MAX_WAVES * WAVE) void k_fast(DevPattern P, Fast
                 apply(p, ph);
                 matvec(v);
                 alpha = rho_new / vdot(rh, v, lane, n);
-                // x + alpha*ph now: the same fma the combined update performs first
-#pragma unroll
-                for (int k = 0; k < NR; k++) x.e[k] = fma(alpha, ph.e[k], x.e[k]);
 #pragma unroll
                 for (int k = 0; k < NR; k++) s.e[k] = fma(-alpha, v.e[k], r.e[k]);
                 iter_res = sqrt(vdot(s, s, lane, n));
                 iter += 0.5f;
                 if (!isfinite(iter_res)) { conv = BILU_NONFINITE; break; }
+                // x + alpha*ph now (x keeps the last finite iterate on a break
+                // above): the same fma the combined update performs first
+#pragma unroll
+                for (int k = 0; k < NR; k++) x.e[k] = fma(alpha, ph.e[k], x.e[k]);
                 const bool half_conv = iter_res < tol_abs;
                 if (!half_conv) {
                     apply(s, sh);

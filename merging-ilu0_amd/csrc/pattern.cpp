@@ -447,3 +447,41 @@ void build_fast_plan(PatternAnalysis &pa)
 }
 
 }  // namespace bilu
+
+// Missing-diagonal insertion on the host: one pass over the rows, the new
+// entry (i,i) going before the first column greater than i.
+extern "C" int bilu_pattern_add_diagonal(int n, int nnz, const int *rp, const int *ci, int *n_missing, int *rp_out,
+                                         int *ci_out, int *src_out)
+{
+    if (n <= 0 || nnz < 0 || !rp || (nnz > 0 && !ci) || !n_missing) return BILU_ERR_INVALID_ARG;
+    if (rp_out && (!ci_out || !src_out)) return BILU_ERR_INVALID_ARG;
+    if (rp[0] != 0 || rp[n] != nnz) return BILU_ERR_INVALID_PATTERN;
+    int missing = 0;
+    for (int i = 0; i < n; i++) {
+        if (rp[i + 1] < rp[i]) return BILU_ERR_INVALID_PATTERN;
+        bool has = false;
+        for (int p = rp[i]; p < rp[i + 1]; p++) {
+            if (ci[p] < 0 || ci[p] >= n || (p > rp[i] && ci[p] <= ci[p - 1])) return BILU_ERR_INVALID_PATTERN;
+            has |= ci[p] == i;
+        }
+        missing += has ? 0 : 1;
+    }
+    *n_missing = missing;
+    if (!rp_out) return BILU_OK;
+    int q = 0;
+    rp_out[0] = 0;
+    for (int i = 0; i < n; i++) {
+        bool placed = false;
+        for (int p = rp[i]; p < rp[i + 1]; p++) {
+            if (!placed && ci[p] >= i) {
+                if (ci[p] > i) { ci_out[q] = i; src_out[q++] = -1; }
+                placed = true;
+            }
+            ci_out[q] = ci[p];
+            src_out[q++] = p;
+        }
+        if (!placed) { ci_out[q] = i; src_out[q++] = -1; }
+        rp_out[i + 1] = q;
+    }
+    return BILU_OK;
+}

@@ -477,8 +477,15 @@ __global__ __launch_bounds__(WAVE) void k_solve(DevPattern P, int batch, const d
     // ---- preconditioner generation (fused: no HBM round trip) ----
     int bad = 0;
     if (PC == PC_LU || PC == PC_ISAI_LU) {
-        if (sp.factor == FK_ILU) bad = ilu_factor(P, sF, lane);
-        else if (sp.factor == FK_PARILU) parilu_sweeps(P, sA, sF, sW, sp.sweeps, lane);
+        if (sp.factor == FK_ILU) {
+            bad = ilu_factor(P, sF, lane);
+        } else {
+            if (sp.factor == FK_PARILU) parilu_sweeps(P, sA, sF, sW, sp.sweeps, lane);
+            // a zero on U's diagonal is a zero pivot, whatever produced the factors
+            __syncthreads();
+            for (int i = lane; i < P.n; i += WAVE) bad |= (sF[P.diag[i]] == 0.0);
+            bad = __any(bad);
+        }
     }
     if (PC == PC_ISAI_LU && !bad) {
         bad |= isai_tri(P, BILU_ISAI_LOWER, sF, sW, nullptr, lane);
@@ -688,6 +695,20 @@ __global__ __launch_bounds__(WAVE) void k_factor_residual(DevPattern P, int batc
     }
     acc = wave_sum(acc);
     if (lane == 0) gRes[sys] = sqrt(acc);
+}
+
+// values of a batch on the diagonal-augmented pattern: a gather with zero
+// fill, HBM-bound (8 bytes read + 8 written per output entry; src is read
+// from L2).  Grid-stride over batch*nnz_out entries, 64-bit indices.
+__global__ void k_expand_values(int batch, int nnz, int nnz_out, const int *__restrict__ src,
+                                const double *__restrict__ v, double *__restrict__ out)
+{
+    const size_t tot = (size_t)batch * (size_t)nnz_out;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < tot; i += (size_t)gridDim.x * blockDim.x) {
+        const size_t s = i / (size_t)nnz_out;
+        const int j = src[i - s * (size_t)nnz_out];
+        out[i] = j < 0 ? 0.0 : v[s * (size_t)nnz + (size_t)j];
+    }
 }
 
 __global__ void k_split_lu(DevPattern P, int batch, const double *__restrict__ gF,
@@ -1435,6 +1456,20 @@ int bilu_split_lu(bilu_pattern_t h, int batch, const double *d_F, double *d_L, d
     const size_t tot = (size_t)batch * (size_t)std::max(h->dev.l_nnz, h->dev.u_nnz);
     const int grid = (int)std::min<size_t>((tot + 255) / 256, 65536);
     hipLaunchKernelGGL(k_split_lu, dim3(grid), dim3(256), 0, (hipStream_t)stream, h->dev, batch, d_F, d_L, d_U);
+    HIP_TRY(hipGetLastError());
+    return BILU_OK;
+}
+
+int bilu_expand_values(int batch, int nnz, int nnz_out, const int *d_src, const double *d_vals, double *d_out,
+                       void *stream)
+{
+    if (batch < 0 || nnz < 0 || nnz_out < nnz || (batch > 0 && nnz_out > 0 && (!d_src || !d_out || (nnz > 0 && !d_vals))))
+        return BILU_ERR_INVALID_ARG;
+    const size_t tot = (size_t)batch * (size_t)nnz_out;
+    if (tot == 0) return BILU_OK;
+    const int grid = (int)std::min<size_t>((tot + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_expand_values, dim3(grid), dim3(256), 0, (hipStream_t)stream, batch, nnz, nnz_out, d_src,
+                       d_vals, d_out);
     HIP_TRY(hipGetLastError());
     return BILU_OK;
 }

@@ -40,6 +40,60 @@ int orc_find_diag(int n, const int *rp, const int *ci, int *diag)
     return missing;
 }
 
+/* Missing-diagonal insertion, in the steps of factorization.cu:31-65
+ * (diag_info[i] = 1 for a row without (i,i), then an inclusive prefix sum),
+ * :70-78 (new row pointers = old + diag_info), :82-165 (per new position the
+ * holder = old position, or -(i+1) for the inserted (i,i): entries left of
+ * the diagonal keep their offset, entries right of it move one further, the
+ * diagonal takes the first position whose column exceeds i, or the row's
+ * end) and :169-189 (column = old column or i, value = old value or 0).
+ * Returns the number of inserted entries; rp_out == NULL only counts.
+ * The reference's warp code also marks lane 0 of every later 32-entry chunk
+ * that lies right of the diagonal (factorization.cu:134-136), which in a row
+ * of more than 32 entries overwrites a moved entry; this restatement keeps
+ * the one diagonal per row that the rest of the routine assumes, so such
+ * rows follow the brute-force union instead (pinned against it). */
+int orc_add_diagonal(int n, const int *rp, const int *ci, int *rp_out, int *ci_out, int *holder)
+{
+    int *info = (int *)malloc(sizeof(int) * (size_t)n);
+    for (int i = 0; i < n; i++) info[i] = find_pos(rp, ci, i, i) < 0 ? 1 : 0;
+    for (int i = 1; i < n; i++) info[i] += info[i - 1];
+    const int missing = info[n - 1];
+    if (rp_out) {
+        rp_out[0] = 0;
+        for (int i = 0; i < n; i++) rp_out[i + 1] = rp[i + 1] + info[i];
+        for (int i = 0; i < n; i++) {
+            const int ob = rp[i], oe = rp[i + 1], nb = rp_out[i], ne = rp_out[i + 1], disp = nb - ob;
+            if (ne - nb == oe - ob) {
+                for (int p = ob; p < oe; p++) holder[p + disp] = p;
+                continue;
+            }
+            int prev_flag = 0;
+            for (int p = ob; p < oe; p++) {
+                const int flag = ci[p] > i;
+                if (ci[p] < i) holder[p + disp] = p;
+                else holder[p + disp + 1] = p;
+                if (flag && (p == ob || !prev_flag)) holder[p + disp] = -i - 1;
+                if (p == oe - 1 && !flag) holder[p + disp + 1] = -i - 1;
+                prev_flag = flag;
+            }
+            if (ob == oe) holder[nb] = -i - 1;
+        }
+        for (int k = 0; k < rp_out[n]; k++) ci_out[k] = holder[k] < 0 ? -(holder[k] + 1) : ci[holder[k]];
+    }
+    free(info);
+    return missing;
+}
+
+void orc_expand_values(int batch, int nnz, int nnz_out, const int *holder, const double *vals, double *out)
+{
+    for (long long i = 0; i < (long long)batch * nnz_out; i++) {
+        const long long s = i / nnz_out;
+        const int h = holder[i % nnz_out];
+        out[i] = h < 0 ? 0.0 : vals[h + s * nnz];
+    }
+}
+
 /* Strided partial sums followed by a shfl_down tree, the order of the
  * reference inner_product (common.hpp:97-128: lane l of warp 0 sums
  * idx = l, l+32, ...; then offsets 16,8,4,2,1).  width 1 = serial sum
@@ -414,7 +468,14 @@ int orc_bicgstab(int n, const int *rp, const int *ci, const int *diag,
     const double tol_abs = (o->tol_mode == ORC_TOL_RELATIVE) ? o->tol * bnorm : o->tol;
     iter_res = res0;
 
-    if (!isfinite(bnorm)) {
+    /* a zero on U's diagonal is a zero pivot, whatever produced the factors
+     * (ParILU sweeps, the caller): x = 0, as after a failed factorization */
+    int bad = 0;
+    if (pc_kind == ORC_PC_LU)
+        for (int i = 0; i < n; i++) bad |= (P1[diag[i]] == 0.0);
+    if (bad) {
+        conv = ORC_ZERO_PIVOT; iter_res = 0.0;
+    } else if (!isfinite(bnorm)) {
         conv = ORC_NONFINITE;
     } else if (bnorm == 0.0) {
         conv = ORC_CONVERGED; iter_res = 0.0;

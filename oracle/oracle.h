@@ -64,6 +64,8 @@ typedef struct {
 } orc_opts;
 
 int orc_find_diag(int n, const int *rp, const int *ci, int *diag);
+int orc_add_diagonal(int n, const int *rp, const int *ci, int *rp_out, int *ci_out, int *holder);
+void orc_expand_values(int batch, int nnz, int nnz_out, const int *holder, const double *vals, double *out);
 
 double orc_dot(int n, const double *a, const double *b, int width);
 void orc_spmv(int n, const int *rp, const int *ci, const double *vals,

@@ -68,6 +68,8 @@ def lib():
         f64 = P(np.float64, flags="C")
         f32 = P(np.float32, flags="C")
         L.orc_find_diag.argtypes = [C.c_int, i32, i32, i32]
+        L.orc_add_diagonal.argtypes = [C.c_int, i32, i32, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_expand_values.argtypes = [C.c_int, C.c_int, C.c_int, i32, f64, f64]
         L.orc_dot.argtypes = [C.c_int, f64, f64, C.c_int]
         L.orc_dot.restype = C.c_double
         L.orc_spmv.argtypes = [C.c_int, i32, i32, f64, f64, f64, C.c_int]
@@ -110,6 +112,26 @@ def find_diag(n, rp, ci):
     d = np.empty(n, dtype=np.int32)
     missing = lib().orc_find_diag(n, _i32(rp), _i32(ci), d)
     return d, missing
+
+
+def add_diagonal(n, rp, ci):
+    """(row_ptrs, col_idx, holder) of the pattern with every missing (i,i)
+    inserted; holder[k] = old position or -(i+1) for an inserted diagonal"""
+    rp, ci = _i32(rp), _i32(ci)
+    missing = lib().orc_add_diagonal(n, rp, ci, None, None, None)
+    rpo = np.empty(n + 1, np.int32)
+    cio = np.empty(len(ci) + missing, np.int32)
+    h = np.empty(len(ci) + missing, np.int32)
+    lib().orc_add_diagonal(n, rp, ci, rpo.ctypes.data, cio.ctypes.data, h.ctypes.data)
+    return rpo, cio, h
+
+
+def expand_values(holder, vals):
+    """[batch, nnz] values on the augmented pattern (0 at inserted diagonals)"""
+    vals = _f64(np.atleast_2d(vals))
+    out = np.empty((vals.shape[0], len(holder)))
+    lib().orc_expand_values(vals.shape[0], vals.shape[1], len(holder), _i32(holder), vals, out)
+    return out
 
 
 def dot(a, b, width=64):

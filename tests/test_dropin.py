@@ -21,6 +21,7 @@ ENTRY_POINTS = [  # C++ symbols the reference's callers link against
     "Batched_conv_ILU_app1_Preconditioned_BiCGSTAB_segregated_Gpu", "ILU_0_Factorization_Gpu", "ComputeILU0Approach1",
     "ParILU_0_Factorization_Gpu", "GenerateISAI_gpu", "PagedCSRMatrices::CopyFromCpuToGpu",
     "PagedVectors::CopyFromGpuToCpu", "SolverResults::SolverResults", "Batched_Preconditioned_BiCGSTAB_Gpu",
+    "FillSubdir", "ReadData", "Print_ans", "PrintSolverResults",
 ]
 
 
