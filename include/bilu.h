@@ -97,6 +97,11 @@ typedef struct {
                                <= parilu_stop_tol * ||A||_F (parilu_sweeps is then the cap)
                                and falls back to exact ILU(0) when the cap is reached
                                without it; 0 = fixed sweep count (default) */
+    int isai_power;       /* ISAI preconditioners: sparsity of |L|^k, |U|^k (|A|^k for the
+                             general ISAI); 1 = the factors' / matrix's own pattern (default).
+                             The reference's ILU-ISAI solver uses 2 (PrecondBiCGSTAB_ilu_isai_gpu.cu:
+                             297-299).  k > 1: FP64, one wave per system, ISAI rows of at most 32
+                             entries (the reference's row_size_limit), no warm start */
 } bilu_opts;
 
 typedef struct {
@@ -180,6 +185,16 @@ int bilu_split_lu(bilu_pattern_t p, int batch, const double *d_F, double *d_L, d
  * to the type's triangle (other positions are left untouched). */
 int bilu_isai(bilu_pattern_t p, int batch, const double *d_vals, int type, double *d_out,
               int *d_status, void *stream);
+/* ISAI with the sparsity of |P|^power, P = the typed view of the pattern
+ * (LOWER: columns < i plus the unit diagonal, UPPER: columns >= i, GENERAL:
+ * all) -- the reference's chosen-pattern ISAI (GenerateISAI_gpu(aiA, A, type,
+ * power), isai.h:9, pattern isai_gpu.cu:40-178, taken structurally).
+ * bilu_isai_pattern: host copy of that pattern (row_ptrs == NULL: *nnz only).
+ * bilu_isai_power: values d_out[batch * nnz] on it; rows of more than 32
+ * entries give BILU_ERR_TOO_LARGE (the reference's row_size_limit). */
+int bilu_isai_pattern(bilu_pattern_t p, int type, int power, int *nnz, int *row_ptrs, int *col_idx);
+int bilu_isai_power(bilu_pattern_t p, int batch, const double *d_vals, int type, int power,
+                    double *d_out, int *d_status, void *stream);
 /* ||b - A x||_2 per system */
 int bilu_true_residual(bilu_pattern_t p, int batch, const double *d_A, const double *d_b,
                        const double *d_x, double *d_res, void *stream);

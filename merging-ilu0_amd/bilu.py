@@ -33,7 +33,8 @@ EXPORTS = [
     "bilu_true_residual", "bilu_mlp_create", "bilu_mlp_destroy", "bilu_mlp_num_params",
     "bilu_mlp_get_weights", "bilu_mlp_set_weights", "bilu_mlp_gradient", "bilu_mlp_train_step",
     "bilu_mlp_infer", "bilu_uneven_create", "bilu_uneven_destroy", "bilu_uneven_tune", "bilu_uneven_config",
-    "bilu_uneven_solve", "bilu_pattern_add_diagonal", "bilu_expand_values",
+    "bilu_uneven_solve", "bilu_pattern_add_diagonal", "bilu_expand_values", "bilu_isai_pattern",
+    "bilu_isai_power",
 ]
 
 
@@ -56,6 +57,7 @@ class Opts(C.Structure):
         ("inner_tol", C.c_double),
         ("max_refine", C.c_int),
         ("parilu_stop_tol", C.c_double),
+        ("isai_power", C.c_int),
     ]
 
 
@@ -94,6 +96,8 @@ def lib():
         L.bilu_factor_residual.argtypes = [vp, C.c_int, vp, vp, vp, vp]
         L.bilu_split_lu.argtypes = [vp, C.c_int, vp, vp, vp, vp]
         L.bilu_isai.argtypes = [vp, C.c_int, vp, C.c_int, vp, vp, vp]
+        L.bilu_isai_pattern.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_int), i32p, i32p]
+        L.bilu_isai_power.argtypes = [vp, C.c_int, vp, C.c_int, C.c_int, vp, vp, vp]
         L.bilu_true_residual.argtypes = [vp, C.c_int, vp, vp, vp, vp, vp]
         L.bilu_uneven_create.argtypes = [C.c_int, C.POINTER(vp), C.c_int, i32p, i32p, C.POINTER(Opts),
                                          C.POINTER(vp)]
@@ -121,9 +125,10 @@ def _check(rc, what):
 
 def opts(precond=ILU0, parilu_sweeps=5, tol=1e-11, tol_mode=TOL_ABSOLUTE, max_iter=500,
          merge=MERGED, true_res_restart=0, max_restarts=3, kernel=KERNEL_AUTO,
-         precision=PRECISION_FP64, inner_tol=1e-5, max_refine=10, parilu_stop_tol=0.0) -> Opts:
+         precision=PRECISION_FP64, inner_tol=1e-5, max_refine=10, parilu_stop_tol=0.0,
+         isai_power=1) -> Opts:
     return Opts(precond, parilu_sweeps, tol, tol_mode, max_iter, merge, true_res_restart,
-                max_restarts, kernel, precision, inner_tol, max_refine, parilu_stop_tol)
+                max_restarts, kernel, precision, inner_tol, max_refine, parilu_stop_tol, isai_power)
 
 
 def _ptr(t):
@@ -277,6 +282,23 @@ def split_lu(pat: Pattern, F, L, U, stream=None):
 def isai(pat: Pattern, vals, typ, out, status=None, stream=None):
     _check(lib().bilu_isai(pat.h, vals.shape[0], _ptr(vals), typ, _ptr(out), _ptr(status),
                            _stream(stream)), "bilu_isai")
+
+
+def isai_pattern(pat: Pattern, typ, power):
+    """host (row_ptrs, col_idx) of the ISAI pattern |P|^power of the typed view"""
+    P = C.POINTER(C.c_int)
+    nnz = C.c_int()
+    _check(lib().bilu_isai_pattern(pat.h, typ, power, C.byref(nnz), None, None), "bilu_isai_pattern")
+    rp, ci = np.empty(pat.n + 1, np.int32), np.empty(nnz.value, np.int32)
+    _check(lib().bilu_isai_pattern(pat.h, typ, power, C.byref(nnz), rp.ctypes.data_as(P), ci.ctypes.data_as(P)),
+           "bilu_isai_pattern")
+    return rp, ci
+
+
+def isai_power(pat: Pattern, vals, typ, power, out, status=None, stream=None):
+    """ISAI of the typed view of vals [B, nnz] on |P|^power: out [B, nnz_power]"""
+    _check(lib().bilu_isai_power(pat.h, vals.shape[0], _ptr(vals), typ, power, _ptr(out), _ptr(status),
+                                 _stream(stream)), "bilu_isai_power")
 
 
 def true_residual(pat: Pattern, A, b, x, res, stream=None):

@@ -79,6 +79,7 @@ struct BiluSolveOptions {
     double inner_tol = 1e-5;
     int max_refine = 10;
     double parilu_stop_tol = 0.0;
+    int isai_power = 1;           // ISAI sparsity |L|^k, |U|^k (|A|^k); the reference's ILU-ISAI uses 2
     const double *d_F_init = nullptr;   // ParILU warm start (device, in-place factor layout)
     void *stream = nullptr;             // hipStream_t; nullptr = default stream
     bool copy_back = true;              // copy x and the results to the host arrays

@@ -24,6 +24,8 @@
 //          U row_ptrs, col_ind; float64 L values[pages*l_nnz], U values
 // out.bin (isai_*, GenerateISAI_gpu power 1 of the input's typed view):
 //          float64 values[pages*nnz]
+// out.bin (isai2_general|isai2_lower|isai2_upper, power 2): int32 nnz_M;
+//          int32 row_ptrs[n+1], col_ind[nnz_M]; float64 values[pages*nnz_M]
 #include <cstdio>
 #include <cstring>
 #include <stdexcept>
@@ -117,18 +119,24 @@ int main(int argc, char **argv)
     A.CopyFromCpuToGpu();
     b.AllocateMemory(LOCATION::GPU);
     b.CopyFromCpuToGpu();
-    if (variant == "ilu0" || variant == "parilu0" || variant.rfind("isai_", 0) == 0) {
+    if (variant == "ilu0" || variant == "parilu0" || variant.rfind("isai_", 0) == 0 ||
+        variant.rfind("isai2_", 0) == 0) {
         FILE *o = out_path ? std::fopen(out_path, "wb") : nullptr;
         if (!o) { std::perror("out.bin"); return 2; }
         bool ok = true;
         try {
-            if (variant.rfind("isai_", 0) == 0) {
+            if (variant.rfind("isai", 0) == 0) {
                 PagedCSRMatrices M;
-                const mat_type t = variant == "isai_lower" ? mat_type::lower_tri
-                                 : variant == "isai_upper" ? mat_type::upper_tri : mat_type::general;
-                GenerateISAI_gpu(M, A, t, 1);
+                const bool p2 = variant.rfind("isai2_", 0) == 0;
+                const std::string ty = variant.substr(p2 ? 6 : 5);
+                const mat_type t = ty == "lower" ? mat_type::lower_tri
+                                 : ty == "upper" ? mat_type::upper_tri : mat_type::general;
+                GenerateISAI_gpu(M, A, t, p2 ? 2 : 1);
                 M.CopyFromGpuToCpu();
-                ok = wr(o, M.GetPtrToCpuValues(), (size_t)pages * nnz);
+                const int mnnz = M.GetNumNz();
+                if (p2)
+                    ok = wr(o, &mnnz, 1) && wr(o, M.GetPtrToCpuRowPtrs(), n + 1) && wr(o, M.GetPtrToCpuColInd(), mnnz);
+                ok = ok && wr(o, M.GetPtrToCpuValues(), (size_t)pages * mnnz);
             } else {
                 PagedCSRMatrices L, U;
                 if (variant == "ilu0") ILU_0_Factorization_Gpu(A, L, U, 1);

@@ -121,6 +121,7 @@ bilu_opts to_opts(const BiluSolveOptions &s)
     o.inner_tol = s.inner_tol;
     o.max_refine = s.max_refine;
     o.parilu_stop_tol = s.parilu_stop_tol;
+    o.isai_power = s.isai_power;
     return o;
 }
 
@@ -172,6 +173,24 @@ void write_solution(const std::vector<std::string> &subdir, const PagedVectors &
 {
     if (subdir.empty()) return;
     Print_ans(subdir, x, (is_scaled ? "x_scaled_gpu_" : "x_gpu_") + stem + ".mtx");
+}
+
+// the ISAI power of the reference's ILU-ISAI solver (2, PrecondBiCGSTAB_ilu_isai_gpu.cu:297-299) when
+// |L|^2 and |U|^2 rows fit its 32-entry limit; beyond it (where the reference stops on an assert) the
+// factors' own patterns (power 1)
+int ilu_isai_power(const PagedCSRMatrices &A)
+{
+    PatternHandle P(A);
+    const int n = A.GetNumRows();
+    for (int type : {BILU_ISAI_LOWER, BILU_ISAI_UPPER}) {
+        int nnz = 0;
+        check(bilu_isai_pattern(P.h, type, 2, &nnz, nullptr, nullptr), "bilu_isai_pattern");
+        std::vector<int> rp(n + 1), ci(nnz);
+        check(bilu_isai_pattern(P.h, type, 2, &nnz, rp.data(), ci.data()), "bilu_isai_pattern");
+        for (int i = 0; i < n; i++)
+            if (rp[i + 1] - rp[i] > 32) return 1;
+    }
+    return 2;
 }
 
 template <typename T = double>
@@ -281,6 +300,7 @@ void Batched_ILU_ISAI_Preconditioned_BiCGSTAB_Gpu(const std::vector<std::string>
     s.parilu_sweeps = parilu_sweeps;
     s.tol = solver_results.GetATOL() > 0 ? solver_results.GetATOL() : ATOL;
     const WithDiagonal D(A_pages);
+    s.isai_power = ilu_isai_power(D.get(A_pages));
     IST = OET = Batched_Preconditioned_BiCGSTAB_Gpu(D.get(A_pages), b_pages, x_pages, solver_results, s);
     PST = 0.0;
     write_solution(subdir, x_pages, is_scaled, "ilu_isai_bicgstab");
@@ -360,12 +380,35 @@ void ParILU_0_Factorization_Gpu(const PagedCSRMatrices &A_pages, PagedCSRMatrice
 void GenerateISAI_gpu(PagedCSRMatrices &aiA_pages, const PagedCSRMatrices &A_pages, const mat_type A_type,
                       const int power)
 {
-    if (power != 1) throw std::invalid_argument("GenerateISAI_gpu: only power 1 (the input's pattern) is supported");
+    if (power < 1) throw std::invalid_argument("GenerateISAI_gpu: power must be >= 1");
     if (!A_pages.ExistsGPU()) const_cast<PagedCSRMatrices &>(A_pages).CopyFromCpuToGpu();
     PatternHandle P(A_pages);
     const int type = A_type == mat_type::lower_tri ? BILU_ISAI_LOWER
                    : A_type == mat_type::upper_tri ? BILU_ISAI_UPPER : BILU_ISAI_GENERAL;
     const int n = A_pages.GetNumRows(), nnz = A_pages.GetNumNz();
+    if (power > 1) {
+        int mnnz = 0;
+        check(bilu_isai_pattern(P.h, type, power, &mnnz, nullptr, nullptr), "bilu_isai_pattern");
+        aiA_pages.DeallocateMemory(LOCATION::CPU);
+        aiA_pages.DeallocateMemory(LOCATION::GPU);
+        aiA_pages.SetNumPages(A_pages.GetNumPages());
+        aiA_pages.SetNumRows(n);
+        aiA_pages.SetNumCols(n);
+        aiA_pages.SetNumNz(mnnz);
+        aiA_pages.AllocateMemory(LOCATION::CPU);
+        aiA_pages.AllocateMemory(LOCATION::GPU);
+        check(bilu_isai_pattern(P.h, type, power, &mnnz, aiA_pages.GetPtrToCpuRowPtrs(), aiA_pages.GetPtrToCpuColInd()),
+              "bilu_isai_pattern");
+        hip_check(hipMemcpy(aiA_pages.GetPtrToGpuRowPtrs(), aiA_pages.GetPtrToCpuRowPtrs(), sizeof(int) * (n + 1),
+                            hipMemcpyHostToDevice), "H2D");
+        hip_check(hipMemcpy(aiA_pages.GetPtrToGpuColInd(), aiA_pages.GetPtrToCpuColInd(), sizeof(int) * mnnz,
+                            hipMemcpyHostToDevice), "H2D");
+        check(bilu_isai_power(P.h, A_pages.GetNumPages(), A_pages.GetPtrToGpuValues(), type, power,
+                              aiA_pages.GetPtrToGpuValues(), nullptr, nullptr),
+              "bilu_isai_power");
+        hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+        return;
+    }
     aiA_pages.DeallocateMemory(LOCATION::CPU);
     aiA_pages.DeallocateMemory(LOCATION::GPU);
     aiA_pages.SetNumPages(A_pages.GetNumPages());

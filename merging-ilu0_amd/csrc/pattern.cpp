@@ -70,6 +70,78 @@ static void build_isai(const PatternAnalysis &pa, int type, IsaiMap &M)
     }
 }
 
+// Rows of P^k by unions: row i of P^k = the union over j in row i of P^(k-1)
+// of row j of P (a marker array, then a scan for the sorted order).  The
+// reference takes the nonzeros of the first page's A^power computed densely
+// (isai_gpu.cu:40-178); the structural pattern is that set without exact
+// numeric cancellations.
+void isai_power_pattern(const PatternAnalysis &pa, int type, int power, std::vector<int> &mrp,
+                        std::vector<int> &mci)
+{
+    const int n = pa.n;
+    std::vector<int> brp(n + 1, 0), bci;
+    for (int i = 0; i < n; i++) {
+        for (int p = pa.rp[i]; p < pa.rp[i + 1]; p++) {
+            const int c = pa.ci[p];
+            if ((type == BILU_ISAI_LOWER && c < i) || (type == BILU_ISAI_UPPER && c >= i) || type == BILU_ISAI_GENERAL)
+                bci.push_back(c);
+        }
+        if (type == BILU_ISAI_LOWER) bci.push_back(i);
+        brp[i + 1] = int(bci.size());
+    }
+    mrp = brp;
+    mci = bci;
+    std::vector<char> mark(n, 0);
+    for (int k = 2; k <= power; k++) {
+        std::vector<int> nrp(n + 1, 0), nci;
+        for (int i = 0; i < n; i++) {
+            int len = 0;
+            for (int p = mrp[i]; p < mrp[i + 1]; p++) {
+                const int j = mci[p];
+                for (int q = brp[j]; q < brp[j + 1]; q++)
+                    if (!mark[bci[q]]) { mark[bci[q]] = 1; len++; }
+            }
+            for (int c = 0, w = 0; c < n && w < len; c++)
+                if (mark[c]) { mark[c] = 0; nci.push_back(c); w++; }
+            nrp[i + 1] = int(nci.size());
+        }
+        mrp.swap(nrp);
+        mci.swap(nci);
+    }
+}
+
+void build_isai_on(const PatternAnalysis &pa, int type, const std::vector<int> &mrp, const std::vector<int> &mci,
+                   IsaiMap &M)
+{
+    const int n = pa.n;
+    M.beg.assign(n, 0);
+    M.m.assign(n, 0);
+    M.goff.assign(n + 1, 0);
+    M.rhs_pos.assign(n, 0);
+    M.gather.clear();
+    M.max_m = 0;
+    for (int i = 0; i < n; i++) {
+        const int b = mrp[i], m = mrp[i + 1] - b;
+        M.beg[i] = b;
+        M.m[i] = m;
+        M.max_m = std::max(M.max_m, m);
+        for (int q = 0; q < m; q++)
+            if (mci[b + q] == i) M.rhs_pos[i] = q;
+        for (int t = 0; t < m; t++) {
+            for (int k = 0; k < m; k++) {
+                const int r = mci[b + k], c = mci[b + t];  // entry (J[k], J[t]) of the typed view
+                int g;
+                if (type == BILU_ISAI_LOWER && r == c) g = -2;
+                else if (type == BILU_ISAI_LOWER && r < c) g = -1;
+                else if (type == BILU_ISAI_UPPER && r > c) g = -1;
+                else g = find_pos(pa.rp, pa.ci, r, c);
+                M.gather.push_back(g);
+            }
+        }
+        M.goff[i + 1] = int(M.gather.size());
+    }
+}
+
 int analyse_pattern(int n, int nnz, const int *rp, const int *ci, PatternAnalysis &pa,
                     std::string &why)
 {

@@ -102,6 +102,15 @@ struct PatternAnalysis {
 
 void build_fast_plan(PatternAnalysis &pa);
 
+// ISAI with a chosen sparsity: the pattern of |P|^power for the typed view P
+// (LOWER: columns < i plus the unit diagonal, UPPER: columns >= i, GENERAL:
+// all), and the gather map of its rows (positions beg[i] = mrp[i] of that
+// pattern) from the pattern positions of the typed view.
+void isai_power_pattern(const PatternAnalysis &pa, int type, int power, std::vector<int> &mrp,
+                        std::vector<int> &mci);
+void build_isai_on(const PatternAnalysis &pa, int type, const std::vector<int> &mrp, const std::vector<int> &mci,
+                   IsaiMap &M);
+
 // returns BILU_OK or an error code; `why` gets a message
 int analyse_pattern(int n, int nnz, const int *rp, const int *ci, PatternAnalysis &pa,
                     std::string &why);

@@ -25,6 +25,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <deque>
 #include <mutex>
 #include <new>
 #include <string>
@@ -39,11 +40,18 @@ constexpr int WAVE = 64;
 constexpr int MAX_NR = 5;        // rows per lane in the register layout: n <= 320
 constexpr int ISAI_GROUP = 32;   // lanes per ISAI row (reference row_size_limit 32)
 
-enum PcKind { PC_LU = 0, PC_ISAI_LU = 1, PC_ISAI_GEN = 2, PC_NONE = 3 };
+enum PcKind { PC_LU = 0, PC_ISAI_LU = 1, PC_ISAI_GEN = 2, PC_NONE = 3, PC_ISAI_POW = 4, PC_ISAI_GEN_POW = 5 };
 enum FactorKind { FK_ILU = 0, FK_PARILU = 1, FK_LOAD = 2, FK_NONE = 3 };
 
 struct DevIsai {
     const int *beg, *m, *goff, *gather, *rhs;
+};
+
+// an ISAI on a chosen pattern (powers > 1): the pattern and its gather map
+struct DevIsaiPow {
+    int nnz = 0;
+    const int *rp = nullptr, *ci = nullptr;
+    DevIsai map{};
 };
 
 struct DevPattern {
@@ -258,6 +266,37 @@ __device__ void spmv_lds(const DevPattern &P, const double *M, const double *vin
     __syncthreads();
 }
 
+// out = aiU (aiL vin) with aiL, aiU on their own patterns, rows left to right
+__device__ void isai_pow_apply(const DevIsaiPow &QL, const double *isL, const DevIsaiPow &QU, const double *isU,
+                               int n, const double *vin, double *tmp, double *out, int lane)
+{
+    for (int i = lane; i < n; i += WAVE) {
+        double acc = 0.0;
+        const int e = QL.rp[i + 1];
+        for (int p = QL.rp[i]; p < e; p++) acc = fma(isL[p], vin[QL.ci[p]], acc);
+        tmp[i] = acc;
+    }
+    __syncthreads();
+    for (int i = lane; i < n; i += WAVE) {
+        double acc = 0.0;
+        const int e = QU.rp[i + 1];
+        for (int p = QU.rp[i]; p < e; p++) acc = fma(isU[p], tmp[QU.ci[p]], acc);
+        out[i] = acc;
+    }
+    __syncthreads();
+}
+
+__device__ void spmv_pow(const DevIsaiPow &Q, const double *M, int n, const double *vin, double *out, int lane)
+{
+    for (int i = lane; i < n; i += WAVE) {
+        double acc = 0.0;
+        const int e = Q.rp[i + 1];
+        for (int p = Q.rp[i]; p < e; p++) acc = fma(M[p], vin[Q.ci[p]], acc);
+        out[i] = acc;
+    }
+    __syncthreads();
+}
+
 // exact ILU(0) in place, rows level by level (IKJ order == the reference KIJ
 // order per entry, merged.cu:145-246); returns nonzero on a zero pivot
 __device__ int ilu_factor(const DevPattern &P, double *F, int lane)
@@ -319,15 +358,14 @@ __device__ __forceinline__ double view_val(const double *vals, int g)
     return g >= 0 ? vals[g] : (g == -2 ? 1.0 : 0.0);
 }
 
-__device__ int isai_tri(const DevPattern &P, int type, const double *vals, double *out,
+__device__ int isai_tri(const DevIsai &M, int n, int type, const double *vals, double *out,
                         double *udiag, int lane)
 {
-    const DevIsai &M = P.isai[type];
     const int grp = lane / ISAI_GROUP, t = lane % ISAI_GROUP;
     int bad = 0;
-    for (int i0 = 0; i0 < P.n; i0 += 2) {
+    for (int i0 = 0; i0 < n; i0 += 2) {
         const int i = i0 + grp;
-        const bool act = i < P.n;
+        const bool act = i < n;
         const int m = act ? M.m[i] : 0;
         const int mo = __shfl_xor(m, ISAI_GROUP, WAVE);
         const int mmax = m > mo ? m : mo;
@@ -368,14 +406,13 @@ __device__ int isai_tri(const DevPattern &P, int type, const double *vals, doubl
     return __any(bad);
 }
 
-__device__ int isai_general(const DevPattern &P, const double *vals, double *out, int lane)
+__device__ int isai_general(const DevIsai &M, int n, const double *vals, double *out, int lane)
 {
-    const DevIsai &M = P.isai[BILU_ISAI_GENERAL];
     const int grp = lane / ISAI_GROUP, t = lane % ISAI_GROUP;
     int bad = 0;
-    for (int i0 = 0; i0 < P.n; i0 += 2) {
+    for (int i0 = 0; i0 < n; i0 += 2) {
         const int i = i0 + grp;
-        const bool act = i < P.n;
+        const bool act = i < n;
         const int m = act ? M.m[i] : 0;
         const int *g = act ? M.gather + M.goff[i] : M.gather;
         double row[ISAI_GROUP];
@@ -436,12 +473,14 @@ __device__ int isai_general(const DevPattern &P, const double *vals, double *out
 
 // ------------------------------------------------------------- the solver
 // LDS layout (doubles): sA[nnz] | sF[nnz] | sW[nnz or 0] | sU[n or 0] | e0[n] e1[n] e2[n]
+// (powers > 1: ... | sML[QL.nnz] | sMU[QU.nnz] | e0 e1 e2, the ISAI arrays on their own patterns)
 template <int NR, int PC>
 __global__ __launch_bounds__(WAVE) void k_solve(DevPattern P, int batch, const double *__restrict__ gA,
                                                const double *__restrict__ gb, double *__restrict__ gx,
                                                const double *__restrict__ gF, SolveParams sp,
                                                float *__restrict__ g_iter, int *__restrict__ g_conv,
-                                               double *__restrict__ g_ires, double *__restrict__ g_tres)
+                                               double *__restrict__ g_ires, double *__restrict__ g_tres,
+                                               DevIsaiPow QL, DevIsaiPow QU)
 {
     extern __shared__ double lds[];
     const int sys = blockIdx.x;
@@ -453,7 +492,9 @@ __global__ __launch_bounds__(WAVE) void k_solve(DevPattern P, int batch, const d
     double *sF = sA + nnz;
     double *sW = sF + nnz;
     double *sU = sW + (need_w ? nnz : 0);
-    double *e0 = sU + (PC == PC_ISAI_LU ? n : 0);
+    double *sML = sU + (PC == PC_ISAI_LU ? n : 0);
+    double *sMU = sML + ((PC == PC_ISAI_POW || PC == PC_ISAI_GEN_POW) ? QL.nnz : 0);
+    double *e0 = sMU + (PC == PC_ISAI_POW ? QU.nnz : 0);
     double *e1 = e0 + n;
     double *e2 = e1 + n;
 
@@ -476,7 +517,7 @@ __global__ __launch_bounds__(WAVE) void k_solve(DevPattern P, int batch, const d
 
     // ---- preconditioner generation (fused: no HBM round trip) ----
     int bad = 0;
-    if (PC == PC_LU || PC == PC_ISAI_LU) {
+    if (PC == PC_LU || PC == PC_ISAI_LU || PC == PC_ISAI_POW) {
         if (sp.factor == FK_ILU) {
             bad = ilu_factor(P, sF, lane);
         } else {
@@ -488,10 +529,15 @@ __global__ __launch_bounds__(WAVE) void k_solve(DevPattern P, int batch, const d
         }
     }
     if (PC == PC_ISAI_LU && !bad) {
-        bad |= isai_tri(P, BILU_ISAI_LOWER, sF, sW, nullptr, lane);
-        bad |= isai_tri(P, BILU_ISAI_UPPER, sF, sW, sU, lane);
+        bad |= isai_tri(P.isai[BILU_ISAI_LOWER], P.n, BILU_ISAI_LOWER, sF, sW, nullptr, lane);
+        bad |= isai_tri(P.isai[BILU_ISAI_UPPER], P.n, BILU_ISAI_UPPER, sF, sW, sU, lane);
     } else if (PC == PC_ISAI_GEN) {
-        bad |= isai_general(P, sA, sW, lane);
+        bad |= isai_general(P.isai[BILU_ISAI_GENERAL], P.n, sA, sW, lane);
+    } else if (PC == PC_ISAI_POW && !bad) {
+        bad |= isai_tri(QL.map, n, BILU_ISAI_LOWER, sF, sML, nullptr, lane);
+        bad |= isai_tri(QU.map, n, BILU_ISAI_UPPER, sF, sMU, nullptr, lane);
+    } else if (PC == PC_ISAI_GEN_POW) {
+        bad |= isai_general(QL.map, n, sA, sML, lane);
     }
 
     float iter = 0.0f;
@@ -519,6 +565,8 @@ __global__ __launch_bounds__(WAVE) void k_solve(DevPattern P, int batch, const d
         __syncthreads();
         if (PC == PC_LU) lu_apply(P, sF, e0, e1, e2, lane);
         else if (PC == PC_ISAI_LU) isai_lu_apply(P, sW, sW, sU, e0, e1, e2, lane);
+        else if (PC == PC_ISAI_POW) isai_pow_apply(QL, sML, QU, sMU, n, e0, e1, e2, lane);
+        else if (PC == PC_ISAI_GEN_POW) spmv_pow(QL, sML, n, e0, e2, lane);
         else spmv_lds(P, sW, e0, e2, lane);
         from_lds(out, e2, lane, n);
     };
@@ -660,10 +708,32 @@ __global__ __launch_bounds__(WAVE) void k_isai(DevPattern P, int batch, const do
     for (int e = lane; e < nnz; e += WAVE) { sV[e] = Vp[e]; sO[e] = Op[e]; }
     __syncthreads();
     int bad;
-    if (type == BILU_ISAI_GENERAL) bad = isai_general(P, sV, sO, lane);
-    else bad = isai_tri(P, type, sV, sO, nullptr, lane);
+    if (type == BILU_ISAI_GENERAL) bad = isai_general(P.isai[type], P.n, sV, sO, lane);
+    else bad = isai_tri(P.isai[type], P.n, type, sV, sO, nullptr, lane);
     __syncthreads();
     for (int e = lane; e < nnz; e += WAVE) Op[e] = sO[e];
+    if (g_status && lane == 0) g_status[sys] = bad ? BILU_ZERO_PIVOT : 0;
+}
+
+// ISAI on a chosen pattern: sV[nnz] | sO[Q.nnz]
+__global__ __launch_bounds__(WAVE) void k_isai_pow(DevPattern P, DevIsaiPow Q, int batch,
+                                                  const double *__restrict__ gV, int type,
+                                                  double *__restrict__ gOut, int *__restrict__ g_status)
+{
+    extern __shared__ double lds[];
+    const int sys = blockIdx.x;
+    if (sys >= batch) return;
+    const int lane = threadIdx.x, nnz = P.nnz;
+    double *sV = lds, *sO = lds + nnz;
+    const double *Vp = gV + (size_t)sys * nnz;
+    double *Op = gOut + (size_t)sys * Q.nnz;
+    for (int e = lane; e < nnz; e += WAVE) sV[e] = Vp[e];
+    __syncthreads();
+    int bad;
+    if (type == BILU_ISAI_GENERAL) bad = isai_general(Q.map, P.n, sV, sO, lane);
+    else bad = isai_tri(Q.map, P.n, type, sV, sO, nullptr, lane);
+    __syncthreads();
+    for (int e = lane; e < Q.nnz; e += WAVE) Op[e] = sO[e];
     if (g_status && lane == 0) g_status[sys] = bad ? BILU_ZERO_PIVOT : 0;
 }
 
@@ -781,6 +851,14 @@ struct bilu_pattern_s {
     // solve workspace (factors between the factor and solve launches), grown on demand
     void *ws = nullptr;
     size_t ws_bytes = 0;
+    // ISAI maps of chosen patterns (powers > 1), built and uploaded on first use
+    struct PowIsai {
+        int type, power, max_m;
+        std::vector<int> rp, ci;
+        int *dbuf = nullptr;
+        DevIsaiPow dev;
+    };
+    std::deque<PowIsai> pow_isai;    // stable addresses: callers keep pointers
 };
 
 namespace {
@@ -889,7 +967,38 @@ void free_handle(bilu_pattern_s *h)
     if (h->d_counter) (void)hipFree(h->d_counter);
     if (h->ws) (void)hipFree(h->ws);
     if (h->d_fast) (void)hipFree(h->d_fast);
+    for (auto &q : h->pow_isai)
+        if (q.dbuf) (void)hipFree(q.dbuf);
     delete h;
+}
+
+// the ISAI map of |P|^power for `type`, built and uploaded at its first use
+int get_pow_isai(bilu_pattern_s &h, int type, int power, const bilu_pattern_s::PowIsai *&out)
+{
+    for (auto &q : h.pow_isai)
+        if (q.type == type && q.power == power) { out = &q; return BILU_OK; }
+    bilu_pattern_s::PowIsai q;
+    q.type = type;
+    q.power = power;
+    bilu::isai_power_pattern(h.pa, type, power, q.rp, q.ci);
+    bilu::IsaiMap M;
+    bilu::build_isai_on(h.pa, type, q.rp, q.ci, M);
+    q.max_m = M.max_m;
+    const int n = h.pa.n;
+    std::vector<int> buf;
+    auto put = [&](const std::vector<int> &v) { const size_t o = buf.size(); buf.insert(buf.end(), v.begin(), v.end()); return o; };
+    const size_t o_rp = put(q.rp), o_ci = put(q.ci), o_beg = put(M.beg), o_m = put(M.m), o_goff = put(M.goff),
+                 o_g = put(M.gather), o_rhs = put(M.rhs_pos);
+    HIP_TRY(hipSetDevice(h.device));
+    HIP_TRY(hipMalloc((void **)&q.dbuf, std::max<size_t>(buf.size(), 1) * sizeof(int)));
+    HIP_TRY(hipMemcpy(q.dbuf, buf.data(), buf.size() * sizeof(int), hipMemcpyHostToDevice));
+    q.dev.nnz = q.rp[n];
+    q.dev.rp = q.dbuf + o_rp; q.dev.ci = q.dbuf + o_ci;
+    q.dev.map.beg = q.dbuf + o_beg; q.dev.map.m = q.dbuf + o_m; q.dev.map.goff = q.dbuf + o_goff;
+    q.dev.map.gather = q.dbuf + o_g; q.dev.map.rhs = q.dbuf + o_rhs;
+    h.pow_isai.push_back(std::move(q));
+    out = &h.pow_isai.back();
+    return BILU_OK;
 }
 
 // table pointers of the persistent kernels (no per-launch sizing)
@@ -1114,11 +1223,11 @@ int launch_mixed(bilu_pattern_s &h, int batch, const double *A, const double *b,
 template <int NR, int PC>
 int launch_solve_nr(bilu_pattern_s &h, int batch, const double *A, const double *b, double *x,
                     const double *F, const SolveParams &sp, float *it, int *cv, double *ir,
-                    double *tr, size_t lds, hipStream_t st)
+                    double *tr, size_t lds, hipStream_t st, const DevIsaiPow &QL, const DevIsaiPow &QU)
 {
     auto kern = k_solve<NR, PC>;
     if (lds > 65536) HIP_TRY(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL(kern, dim3(batch), dim3(WAVE), lds, st, h.dev, batch, A, b, x, F, sp, it, cv, ir, tr);
+    hipLaunchKernelGGL(kern, dim3(batch), dim3(WAVE), lds, st, h.dev, batch, A, b, x, F, sp, it, cv, ir, tr, QL, QU);
     HIP_TRY(hipGetLastError());
     return BILU_OK;
 }
@@ -1126,15 +1235,15 @@ int launch_solve_nr(bilu_pattern_s &h, int batch, const double *A, const double 
 template <int PC>
 int launch_solve(bilu_pattern_s &h, int batch, const double *A, const double *b, double *x,
                  const double *F, const SolveParams &sp, float *it, int *cv, double *ir, double *tr,
-                 size_t lds, hipStream_t st)
+                 size_t lds, hipStream_t st, const DevIsaiPow &QL = DevIsaiPow{}, const DevIsaiPow &QU = DevIsaiPow{})
 {
     const int nr = (h.pa.n + WAVE - 1) / WAVE;
     switch (nr) {
-    case 1: return launch_solve_nr<1, PC>(h, batch, A, b, x, F, sp, it, cv, ir, tr, lds, st);
-    case 2: return launch_solve_nr<2, PC>(h, batch, A, b, x, F, sp, it, cv, ir, tr, lds, st);
-    case 3: return launch_solve_nr<3, PC>(h, batch, A, b, x, F, sp, it, cv, ir, tr, lds, st);
-    case 4: return launch_solve_nr<4, PC>(h, batch, A, b, x, F, sp, it, cv, ir, tr, lds, st);
-    case 5: return launch_solve_nr<5, PC>(h, batch, A, b, x, F, sp, it, cv, ir, tr, lds, st);
+    case 1: return launch_solve_nr<1, PC>(h, batch, A, b, x, F, sp, it, cv, ir, tr, lds, st, QL, QU);
+    case 2: return launch_solve_nr<2, PC>(h, batch, A, b, x, F, sp, it, cv, ir, tr, lds, st, QL, QU);
+    case 3: return launch_solve_nr<3, PC>(h, batch, A, b, x, F, sp, it, cv, ir, tr, lds, st, QL, QU);
+    case 4: return launch_solve_nr<4, PC>(h, batch, A, b, x, F, sp, it, cv, ir, tr, lds, st, QL, QU);
+    case 5: return launch_solve_nr<5, PC>(h, batch, A, b, x, F, sp, it, cv, ir, tr, lds, st, QL, QU);
     default: return BILU_ERR_TOO_LARGE;
     }
 }
@@ -1159,6 +1268,7 @@ void bilu_opts_default(bilu_opts *o)
     o->inner_tol = 1e-5;
     o->max_refine = 10;
     o->parilu_stop_tol = 0.0;
+    o->isai_power = 1;
 }
 
 const char *bilu_version(void) { return "bilu 0.1 gfx950"; }
@@ -1257,11 +1367,27 @@ static int solve_fp64(bilu_pattern_t h, int batch, const double *d_A, const doub
     case BILU_PRECOND_GENERAL_ISAI: pc = PC_ISAI_GEN; sp.factor = FK_NONE; break;
     default: pc = PC_NONE; sp.factor = FK_NONE; break;
     }
+    hipStream_t st = (hipStream_t)stream;
+    HIP_TRY(hipSetDevice(h->device));
+    if (o.isai_power > 1 && (pc == PC_ISAI_LU || pc == PC_ISAI_GEN)) {
+        // ISAI on |L|^k, |U|^k (|A|^k): the wave-per-system kernel with both maps
+        const bilu_pattern_s::PowIsai *q1 = nullptr, *q2 = nullptr;
+        const bool gen = pc == PC_ISAI_GEN;
+        int rc = get_pow_isai(*h, gen ? BILU_ISAI_GENERAL : BILU_ISAI_LOWER, o.isai_power, q1);
+        if (rc == BILU_OK && !gen) rc = get_pow_isai(*h, BILU_ISAI_UPPER, o.isai_power, q2);
+        if (rc != BILU_OK) return rc;
+        if (q1->max_m > ISAI_GROUP || (q2 && q2->max_m > ISAI_GROUP)) return BILU_ERR_TOO_LARGE;
+        const size_t lds = sizeof(double) * ((size_t)nnz * (2 + (sp.factor == FK_PARILU ? 1 : 0)) +
+                                             (size_t)q1->dev.nnz + (q2 ? (size_t)q2->dev.nnz : 0) + (size_t)n * 3);
+        if (lds > h->max_lds) return BILU_ERR_TOO_LARGE;
+        if (gen) return launch_solve<PC_ISAI_GEN_POW>(*h, batch, d_A, d_b, d_x, nullptr, sp, d_iter, d_conv,
+                                                      d_iter_res, d_true_res, lds, st, q1->dev, DevIsaiPow{});
+        return launch_solve<PC_ISAI_POW>(*h, batch, d_A, d_b, d_x, nullptr, sp, d_iter, d_conv, d_iter_res,
+                                         d_true_res, lds, st, q1->dev, q2->dev);
+    }
     if (pc == PC_ISAI_LU && (pa.isai[BILU_ISAI_LOWER].max_m > ISAI_GROUP || pa.isai[BILU_ISAI_UPPER].max_m > ISAI_GROUP))
         return BILU_ERR_TOO_LARGE;
     if (pc == PC_ISAI_GEN && pa.isai[BILU_ISAI_GENERAL].max_m > ISAI_GROUP) return BILU_ERR_TOO_LARGE;
-    hipStream_t st = (hipStream_t)stream;
-    HIP_TRY(hipSetDevice(h->device));
     const double *F = nullptr;
     double *Fbuf = nullptr;
     if (o.merge == BILU_SEGREGATED && pc == PC_LU) {
@@ -1312,10 +1438,27 @@ int bilu_solve_ex(bilu_pattern_t h, int batch, const double *d_A, const double *
         o.parilu_sweeps < 0 || !(o.tol >= 0.0) ||
         (o.precond == BILU_PRECOND_FACTORS && (!d_F_init || o.precision != BILU_PRECISION_FP64)) ||
         (o.precision != BILU_PRECISION_FP64 && o.precision != BILU_PRECISION_FP32_IR) ||
-        !(o.parilu_stop_tol >= 0.0))
+        !(o.parilu_stop_tol >= 0.0) || o.isai_power < 1 || o.isai_power > 16)
         return BILU_ERR_INVALID_ARG;
     if (o.precision == BILU_PRECISION_FP32_IR && (!(o.inner_tol > 0.0 && o.inner_tol < 1.0) || o.max_refine < 1))
         return BILU_ERR_INVALID_ARG;
+    const bool isai_any = o.precond == BILU_PRECOND_ILU0_ISAI || o.precond == BILU_PRECOND_PARILU0_ISAI ||
+                          o.precond == BILU_PRECOND_GENERAL_ISAI;
+    if (o.isai_power > 1 && isai_any) {
+        // chosen-pattern ISAI: FP64, fixed sweeps, one wave per system
+        if (o.precision != BILU_PRECISION_FP64 || d_F_init || o.parilu_stop_tol > 0.0 ||
+            o.kernel == BILU_KERNEL_PERSISTENT)
+            return BILU_ERR_INVALID_ARG;
+        const int rc = solve_fp64(h, batch, d_A, d_b, d_x, o, d_iter, d_conv, d_iter_res, d_true_res, stream);
+        if (rc == BILU_OK && (d_refine || d_factor_status)) {
+            hipStream_t st0 = (hipStream_t)stream;
+            const int fs = o.precond == BILU_PRECOND_PARILU0_ISAI ? o.parilu_sweeps : 0;
+            if (d_refine) hipLaunchKernelGGL(k_fill_int, dim3((batch + 255) / 256), dim3(256), 0, st0, d_refine, 0, batch);
+            if (d_factor_status) hipLaunchKernelGGL(k_fill_int, dim3((batch + 255) / 256), dim3(256), 0, st0, d_factor_status, fs, batch);
+            HIP_TRY(hipGetLastError());
+        }
+        return rc;
+    }
     hipStream_t st = (hipStream_t)stream;
     HIP_TRY(hipSetDevice(h->device));
     const bool lu = o.precond == BILU_PRECOND_ILU0 || o.precond == BILU_PRECOND_PARILU0;
@@ -1485,6 +1628,40 @@ int bilu_isai(bilu_pattern_t h, int batch, const double *d_vals, int type, doubl
     HIP_TRY(hipSetDevice(h->device));
     if (lds > 65536) HIP_TRY(hipFuncSetAttribute((const void *)k_isai, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     hipLaunchKernelGGL(k_isai, dim3(batch), dim3(WAVE), lds, (hipStream_t)stream, h->dev, batch, d_vals, type, d_out, d_status);
+    HIP_TRY(hipGetLastError());
+    return BILU_OK;
+}
+
+int bilu_isai_pattern(bilu_pattern_t h, int type, int power, int *nnz, int *row_ptrs, int *col_idx)
+{
+    if (!h || type < 0 || type > 2 || power < 1 || power > 16 || !nnz || (row_ptrs && !col_idx))
+        return BILU_ERR_INVALID_ARG;
+    std::vector<int> rp, ci;
+    bilu::isai_power_pattern(h->pa, type, power, rp, ci);
+    *nnz = rp[h->pa.n];
+    if (row_ptrs) {
+        std::copy(rp.begin(), rp.end(), row_ptrs);
+        std::copy(ci.begin(), ci.end(), col_idx);
+    }
+    return BILU_OK;
+}
+
+int bilu_isai_power(bilu_pattern_t h, int batch, const double *d_vals, int type, int power, double *d_out,
+                    int *d_status, void *stream)
+{
+    if (!h || batch < 0 || type < 0 || type > 2 || power < 1 || power > 16 || (batch > 0 && (!d_vals || !d_out)))
+        return BILU_ERR_INVALID_ARG;
+    if (batch == 0) return BILU_OK;
+    const bilu_pattern_s::PowIsai *q = nullptr;
+    const int rc = get_pow_isai(*h, type, power, q);
+    if (rc != BILU_OK) return rc;
+    if (q->max_m > ISAI_GROUP) return BILU_ERR_TOO_LARGE;
+    const size_t lds = ((size_t)h->pa.nnz + (size_t)q->dev.nnz) * sizeof(double);
+    if (lds > h->max_lds) return BILU_ERR_TOO_LARGE;
+    HIP_TRY(hipSetDevice(h->device));
+    if (lds > 65536) HIP_TRY(hipFuncSetAttribute((const void *)k_isai_pow, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(k_isai_pow, dim3(batch), dim3(WAVE), lds, (hipStream_t)stream, h->dev, q->dev, batch, d_vals,
+                       type, d_out, d_status);
     HIP_TRY(hipGetLastError());
     return BILU_OK;
 }

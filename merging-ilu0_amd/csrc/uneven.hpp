@@ -115,7 +115,8 @@ int bilu_uneven_create(int npat, const bilu_pattern_t *patterns, int nreq, const
     if (o.precision != BILU_PRECISION_FP32_IR ||
         !(o.precond == BILU_PRECOND_ILU0 || o.precond == BILU_PRECOND_PARILU0 ||
           o.precond == BILU_PRECOND_ILU0_ISAI || o.precond == BILU_PRECOND_PARILU0_ISAI) ||
-        !(o.inner_tol > 0.0 && o.inner_tol < 1.0) || o.max_refine < 1 || !(o.tol >= 0.0) || o.max_iter < 0)
+        !(o.inner_tol > 0.0 && o.inner_tol < 1.0) || o.max_refine < 1 || !(o.tol >= 0.0) || o.max_iter < 0 ||
+        o.isai_power != 1)
         return BILU_ERR_INVALID_ARG;
     for (int p = 0; p < npat; p++) {
         if (!patterns[p]) return BILU_ERR_INVALID_ARG;

@@ -357,12 +357,81 @@ static int ref_pivot(double Dt[][64], int m, int dp)
  * (isai_gpu.cu:325-355); upper: lower-triangular sweep, rhs at 0
  * (:359-389); general: Gauss-Jordan with partial pivoting (:417-554).
  * Output on the pattern, masked to the triangle of `type`. */
+/* one ISAI row: solve Dt^T-form system for `sol` (Dt[t][k] = A(J[k], J[t])),
+ * rhs e_ipos; lower: upper-triangular column sweep from the last position
+ * (isai_gpu.cu:325-355), upper: lower-triangular sweep from 0 (:359-389),
+ * general: Gauss-Jordan with the reference pivot choice (:417-554). */
+static int isai_row_solve(double Dt[][64], int m, int type, int ipos, double *sol)
+{
+    if (type == ORC_ISAI_LOWER) {
+        for (int t = 0; t < m; t++) sol[t] = (t == m - 1) ? 1.0 : 0.0;
+        for (int c = m - 1; c >= 0; c--) {
+            if (Dt[c][c] == 0.0) return ORC_ZERO_PIVOT;
+            sol[c] = sol[c] / Dt[c][c];
+            for (int t = 0; t < c; t++) sol[t] = fma(-sol[c], Dt[t][c], sol[t]);
+        }
+    } else if (type == ORC_ISAI_UPPER) {
+        for (int t = 0; t < m; t++) sol[t] = (t == 0) ? 1.0 : 0.0;
+        for (int c = 0; c < m; c++) {
+            if (Dt[c][c] == 0.0) return ORC_ZERO_PIVOT;
+            sol[c] = sol[c] / Dt[c][c];
+            for (int t = c + 1; t < m; t++) sol[t] = fma(-sol[c], Dt[t][c], sol[t]);
+        }
+    } else {
+        for (int t = 0; t < m; t++) sol[t] = (t == ipos) ? 1.0 : 0.0;
+        for (int dp = 0; dp < m; dp++) {
+            const int pv = ref_pivot(Dt, m, dp);
+            if (pv != dp) {
+                for (int c = 0; c < m; c++) { double tmp = Dt[dp][c]; Dt[dp][c] = Dt[pv][c]; Dt[pv][c] = tmp; }
+                double tmp = sol[dp]; sol[dp] = sol[pv]; sol[pv] = tmp;
+            }
+            const double de = Dt[dp][dp];
+            if (de == 0.0) return ORC_ZERO_PIVOT;
+            for (int t = 0; t < m; t++) {
+                if (t == dp) continue;
+                const double mult = Dt[t][dp] / de;
+                for (int c = 0; c < m; c++) Dt[t][c] = fma(-mult, Dt[dp][c], Dt[t][c]);
+                sol[t] = fma(-mult, sol[dp], sol[t]);
+            }
+        }
+        for (int t = 0; t < m; t++) sol[t] = sol[t] / Dt[t][t];
+    }
+    return 0;
+}
+
+/* ISAI on the rows J_i = mci[mrp[i] .. mrp[i+1]) of a chosen pattern:
+ * Dt[t][k] = A(J[k], J[t]) from the typed view of (rp, ci, vals), row i of
+ * the ISAI = the solution, written to out[mrp[i] + q]. */
+static int isai_on_rows(int n, const int *rp, const int *ci, const double *vals, int type,
+                        const int *mrp, const int *mci, double *out)
+{
+    static __thread double Dt[64][64];
+    double sol[64];
+    for (int i = 0; i < n; i++) {
+        const int beg = mrp[i], m = mrp[i + 1] - beg;
+        if (m > 64 || (type == ORC_ISAI_GENERAL && m > 32)) return -4;
+        int ipos = 0;
+        for (int q = 0; q < m; q++) if (mci[beg + q] == i) ipos = q;
+        for (int k = 0; k < m; k++)
+            for (int t = 0; t < m; t++) Dt[t][k] = view_at(rp, ci, vals, type, mci[beg + k], mci[beg + t]);
+        const int st = isai_row_solve(Dt, m, type, ipos, sol);
+        if (st != 0) return st;
+        for (int q = 0; q < m; q++) out[beg + q] = sol[q];
+    }
+    return 0;
+}
+
+/* ISAI with the sparsity pattern of the input (power 1).
+ * isai_gpu.cu:559-626: for row i with J = cols(i), the thread holding column
+ * t of the dense block A(J,J) ("implicit transpose") solves A(J,J)^T m = e,
+ * e at the position of i in J; row i of the ISAI is m^T.
+ * Output on the pattern, masked to the triangle of `type` (rows start at
+ * rp[i] or diag[i], so the typed rows are windows of the pattern). */
 int orc_isai(int n, const int *rp, const int *ci, const int *diag,
              const double *vals, int type, double *out)
 {
     static __thread double Dt[64][64];
     double sol[64];
-    int J[64];
     for (int i = 0; i < n; i++) {
         int beg, end;
         if (type == ORC_ISAI_LOWER) { beg = rp[i]; end = diag[i] + 1; }
@@ -371,54 +440,103 @@ int orc_isai(int n, const int *rp, const int *ci, const int *diag,
         const int m = end - beg;
         if (m > 64 || (type == ORC_ISAI_GENERAL && m > 32)) return -4;
         int ipos = 0;
-        for (int q = 0; q < m; q++) { J[q] = ci[beg + q]; if (J[q] == i) ipos = q; }
+        for (int q = 0; q < m; q++) if (ci[beg + q] == i) ipos = q;
         /* Dt[t][k] = D[k][t] = A(J[k], J[t]) */
         for (int k = 0; k < m; k++)
-            for (int t = 0; t < m; t++) Dt[t][k] = view_at(rp, ci, vals, type, J[k], J[t]);
-        if (type == ORC_ISAI_LOWER) {
-            for (int t = 0; t < m; t++) sol[t] = (t == m - 1) ? 1.0 : 0.0;
-            for (int c = m - 1; c >= 0; c--) {
-                if (Dt[c][c] == 0.0) return ORC_ZERO_PIVOT;
-                sol[c] = sol[c] / Dt[c][c];
-                for (int t = 0; t < c; t++) sol[t] = fma(-sol[c], Dt[t][c], sol[t]);
-            }
-        } else if (type == ORC_ISAI_UPPER) {
-            for (int t = 0; t < m; t++) sol[t] = (t == 0) ? 1.0 : 0.0;
-            for (int c = 0; c < m; c++) {
-                if (Dt[c][c] == 0.0) return ORC_ZERO_PIVOT;
-                sol[c] = sol[c] / Dt[c][c];
-                for (int t = c + 1; t < m; t++) sol[t] = fma(-sol[c], Dt[t][c], sol[t]);
-            }
-        } else {
-            for (int t = 0; t < m; t++) sol[t] = (t == ipos) ? 1.0 : 0.0;
-            for (int dp = 0; dp < m; dp++) {
-                const int pv = ref_pivot(Dt, m, dp);
-                if (pv != dp) {
-                    for (int c = 0; c < m; c++) { double tmp = Dt[dp][c]; Dt[dp][c] = Dt[pv][c]; Dt[pv][c] = tmp; }
-                    double tmp = sol[dp]; sol[dp] = sol[pv]; sol[pv] = tmp;
-                }
-                const double de = Dt[dp][dp];
-                if (de == 0.0) return ORC_ZERO_PIVOT;
-                for (int t = 0; t < m; t++) {
-                    if (t == dp) continue;
-                    const double mult = Dt[t][dp] / de;
-                    for (int c = 0; c < m; c++) Dt[t][c] = fma(-mult, Dt[dp][c], Dt[t][c]);
-                    sol[t] = fma(-mult, sol[dp], sol[t]);
-                }
-            }
-            for (int t = 0; t < m; t++) sol[t] = sol[t] / Dt[t][t];
-        }
+            for (int t = 0; t < m; t++) Dt[t][k] = view_at(rp, ci, vals, type, ci[beg + k], ci[beg + t]);
+        const int st = isai_row_solve(Dt, m, type, ipos, sol);
+        if (st != 0) return st;
         for (int q = 0; q < m; q++) out[beg + q] = sol[q];
     }
     return 0;
 }
 
+/* Pattern of |P|^power for the typed view P of (rp, ci): LOWER = columns
+ * <= i plus the (unit) diagonal, UPPER = columns >= i, GENERAL = all.  The
+ * reference chooses the ISAI pattern as the nonzeros of A_0^power computed
+ * from the first page's values (isai_gpu.cu:40-178); this is the structural
+ * version (no entry lost to an exact numeric cancellation).  Row i of P^k =
+ * union over j in row i of P^(k-1) of row j of P, sorted.  Returns nnz;
+ * mrp == NULL only counts. */
+int orc_isai_pattern(int n, const int *rp, const int *ci, int type, int power, int *mrp, int *mci)
+{
+    int *brp = (int *)malloc(sizeof(int) * (size_t)(n + 1));
+    int *bci = (int *)malloc(sizeof(int) * (size_t)(rp[n] + n));
+    brp[0] = 0;
+    for (int i = 0; i < n; i++) {
+        int q = brp[i];
+        for (int p = rp[i]; p < rp[i + 1]; p++) {
+            const int c = ci[p];
+            if ((type == ORC_ISAI_LOWER && c < i) || (type == ORC_ISAI_UPPER && c >= i) ||
+                type == ORC_ISAI_GENERAL)
+                bci[q++] = c;
+        }
+        if (type == ORC_ISAI_LOWER) bci[q++] = i;
+        brp[i + 1] = q;
+    }
+    int *cur_rp = (int *)malloc(sizeof(int) * (size_t)(n + 1));
+    int cap = brp[n] > 0 ? brp[n] : 1;
+    int *cur_ci = (int *)malloc(sizeof(int) * (size_t)cap);
+    memcpy(cur_rp, brp, sizeof(int) * (size_t)(n + 1));
+    memcpy(cur_ci, bci, sizeof(int) * (size_t)brp[n]);
+    char *mark = (char *)calloc((size_t)n, 1);
+    int *row = (int *)malloc(sizeof(int) * (size_t)n);
+    for (int k = 2; k <= power; k++) {
+        int *nrp = (int *)malloc(sizeof(int) * (size_t)(n + 1));
+        int ncap = cur_rp[n] + n, nq = 0;
+        int *nci = (int *)malloc(sizeof(int) * (size_t)ncap);
+        nrp[0] = 0;
+        for (int i = 0; i < n; i++) {
+            int len = 0;
+            for (int p = cur_rp[i]; p < cur_rp[i + 1]; p++) {
+                const int j = cur_ci[p];
+                for (int q = brp[j]; q < brp[j + 1]; q++)
+                    if (!mark[bci[q]]) { mark[bci[q]] = 1; row[len++] = bci[q]; }
+            }
+            for (int c = 0, w = 0; c < n && w < len; c++)   /* sorted by scanning the marks */
+                if (mark[c]) { mark[c] = 0; row[w++] = c; }
+            if (nq + len > ncap) { ncap = 2 * (nq + len); nci = (int *)realloc(nci, sizeof(int) * (size_t)ncap); }
+            memcpy(nci + nq, row, sizeof(int) * (size_t)len);
+            nq += len;
+            nrp[i + 1] = nq;
+        }
+        free(cur_rp); free(cur_ci);
+        cur_rp = nrp; cur_ci = nci;
+    }
+    const int nnz = cur_rp[n];
+    if (mrp) {
+        memcpy(mrp, cur_rp, sizeof(int) * (size_t)(n + 1));
+        memcpy(mci, cur_ci, sizeof(int) * (size_t)nnz);
+    }
+    free(brp); free(bci); free(cur_rp); free(cur_ci); free(mark); free(row);
+    return nnz;
+}
+
+/* ISAI of the typed view of (rp, ci, vals) on the pattern (mrp, mci), as
+ * orc_isai row by row; out[mrp[n]]. */
+int orc_isai_on(int n, const int *rp, const int *ci, const double *vals, int type,
+                const int *mrp, const int *mci, double *out)
+{
+    return isai_on_rows(n, rp, ci, vals, type, mrp, mci, out);
+}
+
+/* patterns of the ISAI arrays when they are not the matrix's own (powers > 1) */
+typedef struct {
+    const int *lrp, *lci, *urp, *uci;
+} isai_pats;
+
 /* y = M^{-1} v for the chosen preconditioner */
 static void apply_pc(int n, const int *rp, const int *ci, const int *diag,
                      int kind, const double *P1, const double *P2, int spmv_w,
-                     const int *lev, const double *v, double *tmp, double *out)
+                     const int *lev, const isai_pats *pats, const double *v, double *tmp, double *out)
 {
-    if (kind == ORC_PC_LU && lev) {
+    if (kind == ORC_PC_ISAI_LU_PAT) {
+        /* aiL then aiU on their own patterns, rows left to right */
+        for (int i = 0; i < n; i++) tmp[i] = row_sum(P1, pats->lci, pats->lrp[i], pats->lrp[i + 1], v, spmv_w);
+        for (int i = 0; i < n; i++) out[i] = row_sum(P2, pats->uci, pats->urp[i], pats->urp[i + 1], tmp, spmv_w);
+    } else if (kind == ORC_PC_ISAI_GEN_PAT) {
+        for (int i = 0; i < n; i++) out[i] = row_sum(P1, pats->lci, pats->lrp[i], pats->lrp[i + 1], v, spmv_w);
+    } else if (kind == ORC_PC_LU && lev) {
         orc_lu_solve_avail(n, rp, ci, diag, lev, lev + n, P1, v, tmp, out);
     } else if (kind == ORC_PC_LU) {
         orc_lu_solve(n, rp, ci, diag, P1, v, tmp, out);
@@ -443,11 +561,27 @@ static void apply_pc(int n, const int *rp, const int *ci, const int *diag,
  * the true residual when it misses the tolerance, and non-finite detection
  * (flag -3) in place of the reference's hang/garbage.
  * true_res = ||b - A x||_2 (absolute, as common.hpp:294-342). */
+static int bicgstab_core(int n, const int *rp, const int *ci, const int *diag,
+                         const double *A, const double *b, double *x,
+                         int pc_kind, const double *P1, const double *P2, const isai_pats *pats,
+                         const orc_opts *o, float *iter_out, int *conv_out,
+                         double *iter_res_out, double *true_res_out);
+
 int orc_bicgstab(int n, const int *rp, const int *ci, const int *diag,
                  const double *A, const double *b, double *x,
                  int pc_kind, const double *P1, const double *P2,
                  const orc_opts *o, float *iter_out, int *conv_out,
                  double *iter_res_out, double *true_res_out)
+{
+    return bicgstab_core(n, rp, ci, diag, A, b, x, pc_kind, P1, P2, NULL, o, iter_out, conv_out,
+                         iter_res_out, true_res_out);
+}
+
+static int bicgstab_core(int n, const int *rp, const int *ci, const int *diag,
+                         const double *A, const double *b, double *x,
+                         int pc_kind, const double *P1, const double *P2, const isai_pats *pats,
+                         const orc_opts *o, float *iter_out, int *conv_out,
+                         double *iter_res_out, double *true_res_out)
 {
     const int dw = o->dot_width, sw = o->spmv_width;
     double *buf = (double *)malloc(sizeof(double) * (size_t)n * 10);
@@ -489,7 +623,7 @@ int orc_bicgstab(int n, const int *rp, const int *ci, const int *diag,
             if (rho_new == 0.0) break;
             const double beta = (rho_new / rho_old) * (alpha / omega_old);
             for (int i = 0; i < n; i++) p[i] = fma(beta, fma(-omega_old, v[i], p[i]), r[i]);
-            apply_pc(n, rp, ci, diag, pc_kind, P1, P2, sw, lev, p, tmp, ph);
+            apply_pc(n, rp, ci, diag, pc_kind, P1, P2, sw, lev, pats, p, tmp, ph);
             orc_spmv(n, rp, ci, A, ph, v, sw);
             alpha = rho_new / orc_dot(n, rh, v, dw);
             for (int i = 0; i < n; i++) s[i] = fma(-alpha, v[i], r[i]);
@@ -501,7 +635,7 @@ int orc_bicgstab(int n, const int *rp, const int *ci, const int *diag,
             if (half_conv) {
                 for (int i = 0; i < n; i++) x[i] = fma(alpha, ph[i], x[i]);
             } else {
-                apply_pc(n, rp, ci, diag, pc_kind, P1, P2, sw, lev, s, tmp, sh);
+                apply_pc(n, rp, ci, diag, pc_kind, P1, P2, sw, lev, pats, s, tmp, sh);
                 orc_spmv(n, rp, ci, A, sh, t, sw);
                 const double ts = orc_dot(n, t, s, dw);
                 const double tt = orc_dot(n, t, t, dw);
@@ -1049,4 +1183,49 @@ int orc_batch_solve_split(int n, int nnz, const int *rp, const int *ci, int batc
         for (int t = 0; t < threads; t++) pthread_join(th[t], NULL);
     free(th); free(jobs); free(diag); free(lev);
     return 0;
+}
+
+/* One system through the ISAI solvers with power-k patterns: precond 2 =
+ * exact ILU(0) then ISAI(L), ISAI(U) on |L|^power, |U|^power (the
+ * reference's ILU-ISAI solver uses power 2, PrecondBiCGSTAB_ilu_isai_gpu.cu:
+ * 297-299), 3 = the same after `sweeps` ParILU(0) sweeps, 4 = general ISAI
+ * of A on |A|^power; then BiCGSTAB applying them as SpMVs on their own
+ * patterns.  A factorization or ISAI zero pivot gives x = 0, flag -2. */
+int orc_solve_isai_pow(int n, const int *rp, const int *ci, const double *A, const double *b, double *x,
+                       int precond, int sweeps, int power, const orc_opts *o, float *iter, int *conv,
+                       double *iter_res, double *true_res)
+{
+    const int nnz = rp[n];
+    int *diag = (int *)malloc(sizeof(int) * (size_t)n);
+    if (orc_find_diag(n, rp, ci, diag) > 0) { free(diag); return -5; }
+    const int t1 = precond == 4 ? ORC_ISAI_GENERAL : ORC_ISAI_LOWER;
+    const int n1 = orc_isai_pattern(n, rp, ci, t1, power, NULL, NULL);
+    const int n2 = precond == 4 ? 0 : orc_isai_pattern(n, rp, ci, ORC_ISAI_UPPER, power, NULL, NULL);
+    int *lrp = (int *)malloc(sizeof(int) * (size_t)(n + 1)), *lci = (int *)malloc(sizeof(int) * (size_t)(n1 + 1));
+    int *urp = (int *)malloc(sizeof(int) * (size_t)(n + 1)), *uci = (int *)malloc(sizeof(int) * (size_t)(n2 + 1));
+    orc_isai_pattern(n, rp, ci, t1, power, lrp, lci);
+    if (precond != 4) orc_isai_pattern(n, rp, ci, ORC_ISAI_UPPER, power, urp, uci);
+    double *F = (double *)malloc(sizeof(double) * (size_t)nnz);
+    double *M1 = (double *)malloc(sizeof(double) * (size_t)(n1 + 1));
+    double *M2 = (double *)malloc(sizeof(double) * (size_t)(n2 + 1));
+    memcpy(F, A, sizeof(double) * (size_t)nnz);
+    int st = 0;
+    if (precond == 2) st = orc_ilu0(n, rp, ci, diag, F);
+    else if (precond == 3) st = orc_parilu0_sweeps(n, rp, ci, diag, A, F, sweeps);
+    if (st == 0) st = isai_on_rows(n, rp, ci, F, t1, lrp, lci, M1);
+    if (st == 0 && precond != 4) st = isai_on_rows(n, rp, ci, F, ORC_ISAI_UPPER, urp, uci, M2);
+    int rc = 0;
+    if (st == -4) {
+        rc = -4;
+    } else if (st != 0) {
+        for (int i = 0; i < n; i++) x[i] = 0.0;
+        *iter = 0.0f; *conv = st; *iter_res = 0.0;
+        *true_res = sqrt(orc_dot(n, b, b, o->dot_width));
+    } else {
+        const isai_pats pats = {lrp, lci, urp, uci};
+        bicgstab_core(n, rp, ci, diag, A, b, x, precond == 4 ? ORC_PC_ISAI_GEN_PAT : ORC_PC_ISAI_LU_PAT,
+                      M1, M2, &pats, o, iter, conv, iter_res, true_res);
+    }
+    free(diag); free(lrp); free(lci); free(urp); free(uci); free(F); free(M1); free(M2);
+    return rc;
 }

@@ -43,6 +43,8 @@ extern "C" {
 #define ORC_PC_LU 1         /* in-place L\U factor: lower then upper trisolve */
 #define ORC_PC_ISAI_LU 2    /* aiU * (aiL * v), two SpMVs (ilu_isai_solve.cu:25-45) */
 #define ORC_PC_ISAI_GEN 3   /* aiA * v (gisai_solve.cu:23-30) */
+#define ORC_PC_ISAI_LU_PAT 4   /* aiU * (aiL * v) with aiL, aiU on their own patterns */
+#define ORC_PC_ISAI_GEN_PAT 5  /* aiA * v with aiA on its own pattern */
 
 /* ISAI matrix types (isai.h:7 enum class mat_type) */
 #define ORC_ISAI_GENERAL 0
@@ -64,6 +66,12 @@ typedef struct {
 } orc_opts;
 
 int orc_find_diag(int n, const int *rp, const int *ci, int *diag);
+int orc_isai_pattern(int n, const int *rp, const int *ci, int type, int power, int *mrp, int *mci);
+int orc_isai_on(int n, const int *rp, const int *ci, const double *vals, int type,
+                const int *mrp, const int *mci, double *out);
+int orc_solve_isai_pow(int n, const int *rp, const int *ci, const double *A, const double *b, double *x,
+                       int precond, int sweeps, int power, const orc_opts *o, float *iter, int *conv,
+                       double *iter_res, double *true_res);
 int orc_add_diagonal(int n, const int *rp, const int *ci, int *rp_out, int *ci_out, int *holder);
 void orc_expand_values(int batch, int nnz, int nnz_out, const int *holder, const double *vals, double *out);
 

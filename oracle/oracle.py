@@ -68,6 +68,11 @@ def lib():
         f64 = P(np.float64, flags="C")
         f32 = P(np.float32, flags="C")
         L.orc_find_diag.argtypes = [C.c_int, i32, i32, i32]
+        L.orc_isai_pattern.argtypes = [C.c_int, i32, i32, C.c_int, C.c_int, C.c_void_p, C.c_void_p]
+        L.orc_isai_on.argtypes = [C.c_int, i32, i32, f64, C.c_int, i32, i32, f64]
+        L.orc_solve_isai_pow.argtypes = [C.c_int, i32, i32, f64, f64, f64, C.c_int, C.c_int, C.c_int,
+                                         C.POINTER(Opts), C.POINTER(C.c_float), C.POINTER(C.c_int),
+                                         C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.orc_add_diagonal.argtypes = [C.c_int, i32, i32, C.c_void_p, C.c_void_p, C.c_void_p]
         L.orc_expand_values.argtypes = [C.c_int, C.c_int, C.c_int, i32, f64, f64]
         L.orc_dot.argtypes = [C.c_int, f64, f64, C.c_int]
@@ -112,6 +117,44 @@ def find_diag(n, rp, ci):
     d = np.empty(n, dtype=np.int32)
     missing = lib().orc_find_diag(n, _i32(rp), _i32(ci), d)
     return d, missing
+
+
+def isai_pattern(n, rp, ci, typ, power):
+    """(row_ptrs, col_idx) of |P|^power, P the typed view's pattern
+    (LOWER: columns < i plus i; UPPER: columns >= i; GENERAL: all)"""
+    rp, ci = _i32(rp), _i32(ci)
+    nnz = lib().orc_isai_pattern(n, rp, ci, typ, power, None, None)
+    mrp, mci = np.empty(n + 1, np.int32), np.empty(nnz, np.int32)
+    lib().orc_isai_pattern(n, rp, ci, typ, power, mrp.ctypes.data, mci.ctypes.data)
+    return mrp, mci
+
+
+def isai_on(pat, vals, typ, mrp, mci):
+    """ISAI of the typed view of (pat, vals) on the pattern (mrp, mci): (values, status)"""
+    out = np.zeros(int(mrp[-1]))
+    st = lib().orc_isai_on(pat.n, _i32(pat.row_ptrs), _i32(pat.col_idx), _f64(vals), typ, _i32(mrp),
+                           _i32(mci), out)
+    return out, st
+
+
+def solve_isai_pow(pat, A, b, precond=ILU0_ISAI, sweeps=5, power=2, o: Opts | None = None):
+    """batch loop over orc_solve_isai_pow: (x, iter, conv, iter_res, true_res)"""
+    o = o or opts()
+    A, b = np.atleast_2d(_f64(A)), np.atleast_2d(_f64(b))
+    B = A.shape[0]
+    x = np.zeros((B, pat.n))
+    it, cv = np.zeros(B, np.float32), np.zeros(B, np.int32)
+    ir, tr = np.zeros(B), np.zeros(B)
+    rp, ci = _i32(pat.row_ptrs), _i32(pat.col_idx)
+    for s in range(B):
+        fi, ii, di, dt = C.c_float(), C.c_int(), C.c_double(), C.c_double()
+        xs = np.zeros(pat.n)
+        rc = lib().orc_solve_isai_pow(pat.n, rp, ci, _f64(A[s]), _f64(b[s]), xs, precond, sweeps, power,
+                                      C.byref(o), C.byref(fi), C.byref(ii), C.byref(di), C.byref(dt))
+        if rc != 0:
+            raise ValueError(f"orc_solve_isai_pow: {rc}")
+        x[s], it[s], cv[s], ir[s], tr[s] = xs, fi.value, ii.value, di.value, dt.value
+    return x, it, cv, ir, tr
 
 
 def add_diagonal(n, rp, ci):

@@ -201,8 +201,7 @@ def test_dropin_factorizations_insert_diagonals(tmp_path, variant):
 @pytest.mark.parametrize("kind,n", [("stencil5", 100), ("random", 144)])
 @pytest.mark.parametrize("variant", ["ilu", "parilu", "ilu_isai"])
 def test_dropin_ilu_solvers_insert_diagonals(tmp_path, kind, n, variant):
-    from test_dropin import VARIANTS, read_result
-    from test_gpu_parity import run_oracle
+    from test_dropin import dropin_oracle, read_result
     print(f"case missing-diagonal solve {variant} {kind}", flush=True)
     pat, A, b = synth.batch(kind, 48, n=n)
     rp, ci, Ak = remove_diagonals(pat, A, range(3, pat.n, 11))
@@ -210,8 +209,7 @@ def test_dropin_ilu_solvers_insert_diagonals(tmp_path, kind, n, variant):
     assert r.returncode == 0, r.stderr
     x, it, cv, ir, tr, _ = read_result(fout, 48, pat.n)
     Aa = O.expand_values(O.add_diagonal(pat.n, rp, ci)[2], Ak)
-    precond, kernel = VARIANTS[variant]
-    o = run_oracle(pat, Aa, b, precond=precond, parilu_sweeps=5, tol=1e-11, tol_mode=0, kernel=kernel)
+    o = dropin_oracle(pat, Aa, b, variant)
     for name, u, v in zip(["x", "iters", "conv", "iter_res", "true_res"], (x, it, cv, ir, tr), o):
         assert np.array_equal(u, v, equal_nan=True), f"{variant}: {name} differs"
     if kind == "stencil5" and variant != "parilu":

@@ -189,7 +189,6 @@ ANSWER_FILES = {"merged": "conv_ilu_merged_app1_bicgstab", "segregated": "conv_i
                                             ("parilu", False), ("ilu_isai", True), ("general_isai", False)])
 def test_solve_from_tree_writes_answers(tmp_path, variant, scaled):
     from test_dropin import read_result
-    from test_gpu_parity import run_oracle
     print(f"case mtx tree {variant} scaled={scaled}", flush=True)
     pat, A, b = synth.batch("random", 6, n=144)
     A, dirs = make_tree(tmp_path / "cat", pat, A, b, scaled=scaled)
@@ -200,11 +199,13 @@ def test_solve_from_tree_writes_answers(tmp_path, variant, scaled):
     assert r.returncode == 0, r.stderr
     x, it, cv, ir, tr, _ = read_result(work / "out.bin", 12, pat.n)
     # the batch read from the tree solves exactly like the same arrays given directly
-    from test_dropin import VARIANTS
-    precond, kernel = VARIANTS[variant]
+    from test_dropin import dropin_oracle
     Ab, bb = np.concatenate([A, A]), np.concatenate([b, b])
-    o = run_oracle(pat, Ab, bb, precond=precond, parilu_sweeps=5, tol=1e-11, tol_mode=0, kernel=kernel)
+    o = dropin_oracle(pat, Ab, bb, variant)
     assert np.array_equal(x, o[0]) and np.array_equal(it, o[1]) and np.array_equal(cv, o[2])
+    # Problem_Size copies of a problem are the same system: identical answers, so one
+    # answer file per problem directory loses nothing
+    assert np.array_equal(x[6:], x[:6]) and np.array_equal(it[6:], it[:6])
     # one answer file per problem directory, the first pages of x, %0.17lg round trips
     name = ("x_scaled_gpu_" if scaled else "x_gpu_") + ANSWER_FILES[variant] + ".mtx"
     for s, d in enumerate(dirs):
