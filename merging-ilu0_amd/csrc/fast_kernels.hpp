@@ -49,6 +49,9 @@ struct FastParams {
     // exact ILU(0) lane streams [step][64]
     const uint32_t *ilu0, *ilu1;
     int ilu_steps;
+    // Jacobi ParILU(0) sweep lane streams [step][64]
+    const uint32_t *par0, *par1;
+    int par_steps;
 };
 
 __device__ __forceinline__ void wave_sync()
@@ -224,8 +227,75 @@ __device__ __forceinline__ int ilu_stream(const uint32_t *__restrict__ r0p, cons
     return __any(bad);
 }
 
-// Jacobi ParILU(0) sweep (parILU_0_gpu.cu:18-57 formula) over the entries
-// with work; entries without dependencies (upper, no updates) keep a_ij
+// One Jacobi ParILU(0) sweep (parILU_0_gpu.cu:18-57 formula) as per-lane
+// step streams: src = the current iterate, dst = the next one (entries
+// without work hold a_ij in both).  Entry q of the work order is lane
+// q % 64's; its steps accumulate sum = fma(l_ik, u_kj, sum) over its pairs in
+// order and its last step finishes it: t = a_ij - sum, dst = t / src[pivot]
+// (lower) or t (upper), residual t - src*pivot (or t - src) squared into the
+// lane's sum.  Every operand comes from src, so steps are free of order
+// constraints.  Record batches are loaded two batches ahead and a_ij of the
+// next batch one batch ahead (global memory; L2 hits after the first sweep).
+// Returns the lane's sum of squared residuals (RES) or 0.
+template <bool RES, int K = bilu::PAR_BATCH>
+__device__ __forceinline__ double parilu_stream(const uint32_t *__restrict__ r0p, const uint32_t *__restrict__ r1p,
+                                                int steps, const double *__restrict__ Ap, const double *src,
+                                                double *dst, double *scratch, int lane)
+{
+    static_assert(bilu::PAR_BATCH % K == 0, "streams are padded to PAR_BATCH steps");
+    const char *Sb = reinterpret_cast<const char *>(src);
+    char *Db = reinterpret_cast<char *>(dst);
+    auto ld = [&](uint32_t off) { return *reinterpret_cast<const double *>(Sb + off); };
+    uint32_t ra[K], rb[K], na[K], nb[K];
+    double av[K];
+    const int s1 = (K < steps) ? K : 0;
+#pragma unroll
+    for (int j = 0; j < K; j++) {
+        ra[j] = r0p[j * WAVE + lane]; rb[j] = r1p[j * WAVE + lane];
+        na[j] = r0p[(s1 + j) * WAVE + lane]; nb[j] = r1p[(s1 + j) * WAVE + lane];
+    }
+#pragma unroll
+    for (int j = 0; j < K; j++) av[j] = Ap[(rb[j] & 0xFFF8u) >> 3];
+    double acc = 0.0, rr = 0.0;
+    for (int s0 = 0; s0 < steps; s0 += K) {
+        const int s2 = (s0 + 2 * K < steps) ? s0 + 2 * K : s0;
+        uint32_t qa[K], qb[K];
+        double nav[K];
+#pragma unroll
+        for (int j = 0; j < K; j++) { qa[j] = r0p[(s2 + j) * WAVE + lane]; qb[j] = r1p[(s2 + j) * WAVE + lane]; }
+#pragma unroll
+        for (int j = 0; j < K; j++) nav[j] = Ap[(nb[j] & 0xFFF8u) >> 3];
+#pragma unroll
+        for (int j = 0; j < K; j++) {
+            const uint32_t b = rb[j];
+            const double l = ld(ra[j] & 0xFFFFu), u = ld(ra[j] >> 16);
+            acc = (b & 1u) ? 0.0 : acc;
+            acc = fma(l, u, acc);
+            const uint32_t pv = b >> 16, pos = b & 0xFFF8u;
+            const bool div = pv != 0xFFFFu, end = (b & 2u) != 0;
+            const double old = ld(pos), dv = ld(div ? pv : pos);
+            const double t = av[j] - acc;
+            const double val = div ? t / dv : t;
+            if (RES) {
+                const double res = div ? t - old * dv : t - old;
+                rr = end ? fma(res, res, rr) : rr;
+            }
+            double *o = end ? reinterpret_cast<double *>(Db + pos) : scratch;
+            *o = val;
+        }
+#pragma unroll
+        for (int j = 0; j < K; j++) {
+            ra[j] = na[j]; rb[j] = nb[j]; av[j] = nav[j];
+            na[j] = qa[j]; nb[j] = qb[j];
+        }
+    }
+    return rr;
+}
+
+// Jacobi ParILU(0) sweep (parILU_0_gpu.cu:18-57 formula) entry by entry
+// (entry idx on lane idx % 64), W = the old iterate; entries without work
+// keep a_ij.  The register-lean form for the merged FP64 kernel, whose slot
+// values stay live in VGPRs across the factorization.
 __device__ __forceinline__ void parilu_sweep(const ushort4 *__restrict__ etab, int ne,
                                              const ushort2 *__restrict__ dep, const double *__restrict__ Ap,
                                              double *F, double *W, int nnz, int lane)
@@ -238,17 +308,9 @@ __device__ __forceinline__ void parilu_sweep(const ushort4 *__restrict__ etab, i
         const double a = Ap[pos];
         const double dv = W[pv == 0xFFFF ? pos : pv];
         double sum = 0.0;
-        for (int d0 = db; d0 < de; d0 += 8) {
-            double lv[8], uv[8];
-#pragma unroll
-            for (int k = 0; k < 8; k++) {
-                const ushort2 pq = dep[d0 + k < de ? d0 + k : de - 1];
-                lv[k] = W[pq.x];
-                uv[k] = W[pq.y];
-            }
-#pragma unroll
-            for (int k = 0; k < 8; k++)
-                if (d0 + k < de) sum = fma(lv[k], uv[k], sum);
+        for (int d = db; d < de; d++) {
+            const ushort2 pq = dep[d];
+            sum = fma(W[pq.x], W[pq.y], sum);
         }
         F[pos] = (pv != 0xFFFF) ? (a - sum) / dv : a - sum;
     }

@@ -15,46 +15,6 @@
 
 constexpr int FS_ILU = 0, FS_PARILU = 1;
 
-// Factor residual of the iterate W for the entries with work: for lower
-// (i, j): a_ij - sum_k l_ik u_kj - l_ij u_jj, for upper: a_ij - sum - u_ij.
-// Computed inside the Jacobi sweep (the sum is the sweep's own), so it costs
-// a multiply and a subtract per entry.  Returns the lane's partial sum of
-// squares.
-__device__ __forceinline__ double parilu_sweep_res(const ushort4 *__restrict__ etab, int ne,
-                                                   const ushort2 *__restrict__ dep,
-                                                   const double *__restrict__ Ap, double *F,
-                                                   double *W, int nnz, int lane)
-{
-    for (int e = lane; e < nnz; e += WAVE) W[e] = F[e];
-    wave_sync();
-    double rr = 0.0;
-    for (int idx = lane; idx < ne; idx += WAVE) {
-        const ushort4 r = etab[idx];
-        const int pos = r.x, db = r.y, de = r.z, pv = r.w;
-        const double a = Ap[pos];
-        const double dv = W[pv == 0xFFFF ? pos : pv];
-        double sum = 0.0;
-        for (int d0 = db; d0 < de; d0 += 8) {
-            double lv[8], uv[8];
-#pragma unroll
-            for (int k = 0; k < 8; k++) {
-                const ushort2 pq = dep[d0 + k < de ? d0 + k : de - 1];
-                lv[k] = W[pq.x];
-                uv[k] = W[pq.y];
-            }
-#pragma unroll
-            for (int k = 0; k < 8; k++)
-                if (d0 + k < de) sum = fma(lv[k], uv[k], sum);
-        }
-        const double t = a - sum;
-        const double res = (pv != 0xFFFF) ? t - W[pos] * dv : t - W[pos];
-        rr = fma(res, res, rr);
-        F[pos] = (pv != 0xFFFF) ? t / dv : t;
-    }
-    wave_sync();
-    return rr;
-}
-
 // ISAI of the L (or U) factor in F (FP64, in-place factor layout) into M
 // (FP32 on the pattern: strictly lower positions = ISAI(L), diagonal and
 // upper = ISAI(U)), by the lane streams of the pattern plan: each step is
@@ -134,17 +94,27 @@ __global__ __launch_bounds__(KF_MAX_WAVES * WAVE) void k_factor_fast(
             const double lim2 = stop_tol * stop_tol * anorm2;
             int used = 0;
             bool met = stop_tol <= 0.0;
+            // Jacobi sweeps between F and W (entries without work and the zero cell in both)
+            for (int e = lane; e < nnz; e += WAVE) W[e] = F[e];
+            if (lane == 0) W[nnz] = 0.0;
+            wave_sync();
+            double *src = F, *dst = W;
             while (used < sweeps) {
-                const double rr = wave_sum(parilu_sweep_res(fp.etab, fp.ne, fp.dep, Ap, F, W, nnz, lane));
+                const double rr = wave_sum(parilu_stream<true>(fp.par0, fp.par1, fp.par_steps, Ap, src, dst,
+                                                               F + nnz + 2, lane));
+                wave_sync();
                 used++;
                 if (stop_tol > 0.0 && rr <= lim2) {
-                    // the iterate this sweep started from meets the tolerance: keep it
-                    for (int e = lane; e < nnz; e += WAVE) F[e] = W[e];
-                    wave_sync();
+                    // the iterate this sweep started from (src) meets the tolerance: keep it
                     used--;
                     met = true;
                     break;
                 }
+                double *tmp = src; src = dst; dst = tmp;
+            }
+            if (src != F) {
+                for (int e = lane; e < nnz; e += WAVE) F[e] = src[e];
+                wave_sync();
             }
             if (!met) {
                 // fallback: exact ILU(0)
@@ -252,7 +222,7 @@ __device__ __forceinline__ void spmv_coef(const float (&c)[SM], const float *__r
 #define KM_MAX_WAVES 12      // LU variant
 #endif
 #ifndef KMI_MAX_WAVES
-#define KMI_MAX_WAVES 8      // ISAI variant (two slot coefficient sets in VGPRs)
+#define KMI_MAX_WAVES 12     // ISAI variant (two slot coefficient sets in VGPRs; 12 waves fill the LDS)
 #endif
 
 constexpr int PCK_LU = 0, PCK_ISAI = 1;

@@ -41,6 +41,8 @@ typedef __attribute__((__vector_size__(4 * sizeof(float)))) float floatx4;
 constexpr int TM = 128, TN = 128, TK = 64, KP = TK + 8;   // LDS row pitch (bf16), padded
 constexpr double SIGMA = 1e-8;
 constexpr float YS = 24.0f;
+constexpr float INV_SIGMA_F = 1e8f;          // 1 / SIGMA, exact in FP32
+constexpr float INV_YS = 1.0f / YS;
 
 enum Epi { EPI_F32 = 0, EPI_BIAS_GELU = 1, EPI_BIAS = 2, EPI_GELU_BWD = 3 };
 
@@ -162,14 +164,15 @@ __global__ __launch_bounds__(256) void k_gemm(const __bf16 *__restrict__ A, int 
             }
 }
 
-// features: X[s][e] = asinh(a / SIGMA) / YS (BF16), padding columns 0
+// features: X[s][e] = asinh(a / SIGMA) / YS (BF16) in FP32 arithmetic; padding
+// columns and rows s >= batch are 0.  Grid (ceil(P0 / 256), rows).
 __global__ void k_features(const double *__restrict__ A, int nnz, int P0, int batch, __bf16 *__restrict__ X)
 {
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (size_t)batch * P0) return;
-    const int s = int(i / P0), e = int(i % P0);
-    const float v = (e < nnz) ? asinhf((float)(A[(size_t)s * nnz + e] / SIGMA)) / YS : 0.0f;
-    X[i] = (__bf16)v;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x, s = blockIdx.y;
+    if (e >= P0) return;
+    float v = 0.0f;
+    if (e < nnz && s < batch) v = asinhf((float)A[(size_t)s * nnz + e] * INV_SIGMA_F) * INV_YS;
+    X[(size_t)s * P0 + e] = (__bf16)v;
 }
 
 // base guess of entry e of system s (piv[e] = position of a_jj for a lower
@@ -178,8 +181,9 @@ __global__ void k_features(const double *__restrict__ A, int nnz, int P0, int ba
 __device__ __forceinline__ float base_guess(const double *__restrict__ As, const int *__restrict__ piv, int e)
 {
     const int pv = piv[e];
-    const double sv = pv >= 0 ? As[e] / As[pv] : As[e];
-    return asinhf((float)(sv / SIGMA)) / YS;
+    const float a = (float)As[e];
+    const float sv = pv >= 0 ? a / (float)As[pv] : a;
+    return asinhf(sv * INV_SIGMA_F) * INV_YS;
 }
 
 // loss gradient dY = 2 (y - (t - base)) / (batch nnz) (BF16), t from the exact
@@ -248,15 +252,15 @@ __global__ void k_colsum_final(const float *__restrict__ part, int N, float *__r
     g[n] = s;
 }
 
-// output: F_guess = SIGMA sinh(YS (base + y)) (FP64, unpadded)
+// output: F_guess = SIGMA sinh(YS (base + y)) (FP64, unpadded).  Grid
+// (ceil(nnz / 256), batch).
 __global__ void k_outputs(const float *__restrict__ Y, const double *__restrict__ A, const int *__restrict__ piv,
                           int nnz, int P3, int batch, double *__restrict__ Fo)
 {
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (size_t)batch * nnz) return;
-    const int s = int(i / nnz), e = int(i % nnz);
+    const int e = blockIdx.x * blockDim.x + threadIdx.x, s = blockIdx.y;
+    if (e >= nnz || s >= batch) return;
     const float yb = base_guess(A + (size_t)s * nnz, piv, e) + Y[(size_t)s * P3 + e];
-    Fo[i] = SIGMA * (double)sinhf(YS * yb);
+    Fo[(size_t)s * nnz + e] = SIGMA * (double)sinhf(YS * yb);
 }
 
 // Adam step on FP32 master weights w (moments m, v), BF16 copy wb
@@ -384,10 +388,10 @@ void forward(bilu_mlp_s &q, int rows, hipStream_t st)
 int loss_and_grad(bilu_mlp_s &q, int batch, const double *A, const double *F, float *d_loss, hipStream_t st)
 {
     const int rows = pad_to(batch, TM);
+    if (rows > 65535) return BILU_ERR_TOO_LARGE;   // one grid row per system in the element kernels
     int rc = reserve(q, rows);
     if (rc != BILU_OK) return rc;
-    MLP_TRY(hipMemsetAsync(q.X, 0, (size_t)rows * q.P0 * 2, st));
-    hipLaunchKernelGGL(k_features, dim3(blocks((size_t)batch * q.P0)), dim3(256), 0, st, A, q.nnz, q.P0, batch, q.X);
+    hipLaunchKernelGGL(k_features, dim3((q.P0 + 255) / 256, rows), dim3(256), 0, st, A, q.nnz, q.P0, batch, q.X);
     forward(q, rows, st);
     MLP_TRY(hipMemsetAsync(q.dY, 0, (size_t)rows * q.P3 * 2, st));
     const unsigned nb = blocks((size_t)batch * q.P3);
@@ -599,11 +603,10 @@ int bilu_mlp_infer(bilu_mlp_t q, int batch, const double *d_A, double *d_F_init,
     if (rc != BILU_OK) return rc;
     for (int s0 = 0; s0 < batch; s0 += chunk) {
         const int nb = std::min(chunk, batch - s0), rows = pad_to(nb, TM);
-        hipLaunchKernelGGL(k_features, dim3(blocks((size_t)rows * q->P0)), dim3(256), 0, st,
+        hipLaunchKernelGGL(k_features, dim3((q->P0 + 255) / 256, rows), dim3(256), 0, st,
                            d_A + (size_t)s0 * q->nnz, q->nnz, q->P0, nb, q->X);
-        if (rows > nb) MLP_TRY(hipMemsetAsync(q->X + (size_t)nb * q->P0, 0, (size_t)(rows - nb) * q->P0 * 2, st));
         forward(*q, rows, st);
-        hipLaunchKernelGGL(k_outputs, dim3(blocks((size_t)nb * q->nnz)), dim3(256), 0, st, q->Y,
+        hipLaunchKernelGGL(k_outputs, dim3((q->nnz + 255) / 256, nb), dim3(256), 0, st, q->Y,
                            d_A + (size_t)s0 * q->nnz, q->piv, q->nnz, q->P3, nb, d_F_init + (size_t)s0 * q->nnz);
     }
     MLP_TRY(hipGetLastError());

@@ -69,7 +69,15 @@ struct FastPlan {
     // ILU_BATCH steps)
     std::vector<uint32_t> ilu0, ilu1;
     int ilu_steps = 0;
+    // one Jacobi ParILU(0) sweep as per-lane step streams [step][64]: entry q
+    // of the work order (etab) on lane q % 64, one step per product pair, the
+    // last step of an entry finishing it; padded to a multiple of PAR_BATCH
+    std::vector<uint32_t> par0, par1;
+    int par_steps = 0;
 };
+
+// steps per record batch of the ParILU sweep streams
+constexpr int PAR_BATCH = 8;
 
 // steps per record batch of the ILU(0) streams (records of a batch are
 // loaded together, one batch ahead)
