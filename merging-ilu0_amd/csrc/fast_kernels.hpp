@@ -49,9 +49,9 @@ struct FastParams {
     // exact ILU(0) lane streams [step][64]
     const uint32_t *ilu0, *ilu1;
     int ilu_steps;
-    // Jacobi ParILU(0) sweep lane streams [step][64]
-    const uint32_t *par0, *par1;
-    int par_steps;
+    // Jacobi ParILU(0) sweep rounds: products [step][64], finishes [round][64]
+    const uint32_t *parp, *parf;
+    int par_rounds, par_psteps;
 };
 
 __device__ __forceinline__ void wave_sync()
@@ -227,67 +227,71 @@ __device__ __forceinline__ int ilu_stream(const uint32_t *__restrict__ r0p, cons
     return __any(bad);
 }
 
-// One Jacobi ParILU(0) sweep (parILU_0_gpu.cu:18-57 formula) as per-lane
-// step streams: src = the current iterate, dst = the next one (entries
-// without work hold a_ij in both).  Entry q of the work order is lane
-// q % 64's; its steps accumulate sum = fma(l_ik, u_kj, sum) over its pairs in
-// order and its last step finishes it: t = a_ij - sum, dst = t / src[pivot]
-// (lower) or t (upper), residual t - src*pivot (or t - src) squared into the
-// lane's sum.  Every operand comes from src, so steps are free of order
-// constraints.  Record batches are loaded two batches ahead and a_ij of the
-// next batch one batch ahead (global memory; L2 hits after the first sweep).
+// One Jacobi ParILU(0) sweep (parILU_0_gpu.cu:18-57 formula) in
+// lane-synchronous rounds: src = the current iterate, dst = the next one
+// (entries without work hold a_ij in both).  In round r lane l owns entry
+// 64 r + l of the sweep order; the round's product steps accumulate
+// sum = fma(l_ik, u_kj, sum) (zero pairs pad the shorter entries) and its
+// last step finishes every lane's entry at once: t = a_ij - sum, dst = t /
+// src[pivot] (lower) or t (upper), residual t - src*pivot (or t - src)
+// squared into the lane's sum.  Every operand comes from src.  Product
+// records arrive in batches, one batch ahead; finish records and a_ij (global
+// memory, L2 hits after the first sweep) are queued PAR_AHEAD rounds ahead.
 // Returns the lane's sum of squared residuals (RES) or 0.
-template <bool RES, int K = bilu::PAR_BATCH>
-__device__ __forceinline__ double parilu_stream(const uint32_t *__restrict__ r0p, const uint32_t *__restrict__ r1p,
-                                                int steps, const double *__restrict__ Ap, const double *src,
-                                                double *dst, double *scratch, int lane)
+constexpr int PAR_AHEAD = 4;
+template <bool RES>
+__device__ __forceinline__ double parilu_rounds(const uint32_t *__restrict__ pp, const uint32_t *__restrict__ pf,
+                                                int R, int PS, const double *__restrict__ Ap, const double *src,
+                                                double *dst, int lane)
 {
-    static_assert(bilu::PAR_BATCH % K == 0, "streams are padded to PAR_BATCH steps");
+    constexpr int K = bilu::PAR_BATCH, D = PAR_AHEAD;
     const char *Sb = reinterpret_cast<const char *>(src);
     char *Db = reinterpret_cast<char *>(dst);
     auto ld = [&](uint32_t off) { return *reinterpret_cast<const double *>(Sb + off); };
-    uint32_t ra[K], rb[K], na[K], nb[K];
-    double av[K];
-    const int s1 = (K < steps) ? K : 0;
+    auto frec = [&](int r) { return pf[(r < R ? r : R - 1) * WAVE + lane]; };
+    uint32_t fq[D + 1];
+    double aq[D];
 #pragma unroll
-    for (int j = 0; j < K; j++) {
-        ra[j] = r0p[j * WAVE + lane]; rb[j] = r1p[j * WAVE + lane];
-        na[j] = r0p[(s1 + j) * WAVE + lane]; nb[j] = r1p[(s1 + j) * WAVE + lane];
-    }
+    for (int k = 0; k <= D; k++) fq[k] = frec(k);
 #pragma unroll
-    for (int j = 0; j < K; j++) av[j] = Ap[(rb[j] & 0xFFF8u) >> 3];
+    for (int k = 0; k < D; k++) aq[k] = Ap[(fq[k] & 0xFFF8u) >> 3];
+    int r = 0;
+    uint32_t ra[K];
+#pragma unroll
+    for (int j = 0; j < K; j++) ra[j] = pp[j * WAVE + lane];
     double acc = 0.0, rr = 0.0;
-    for (int s0 = 0; s0 < steps; s0 += K) {
-        const int s2 = (s0 + 2 * K < steps) ? s0 + 2 * K : s0;
-        uint32_t qa[K], qb[K];
-        double nav[K];
+    for (int s0 = 0; s0 < PS; s0 += K) {
+        const int nx = (s0 + K < PS) ? s0 + K : s0;
+        uint32_t na[K];
 #pragma unroll
-        for (int j = 0; j < K; j++) { qa[j] = r0p[(s2 + j) * WAVE + lane]; qb[j] = r1p[(s2 + j) * WAVE + lane]; }
-#pragma unroll
-        for (int j = 0; j < K; j++) nav[j] = Ap[(nb[j] & 0xFFF8u) >> 3];
+        for (int j = 0; j < K; j++) na[j] = pp[(nx + j) * WAVE + lane];
 #pragma unroll
         for (int j = 0; j < K; j++) {
-            const uint32_t b = rb[j];
-            const double l = ld(ra[j] & 0xFFFFu), u = ld(ra[j] >> 16);
-            acc = (b & 1u) ? 0.0 : acc;
-            acc = fma(l, u, acc);
-            const uint32_t pv = b >> 16, pos = b & 0xFFF8u;
-            const bool div = pv != 0xFFFFu, end = (b & 2u) != 0;
-            const double old = ld(pos), dv = ld(div ? pv : pos);
-            const double t = av[j] - acc;
-            const double val = div ? t / dv : t;
-            if (RES) {
-                const double res = div ? t - old * dv : t - old;
-                rr = end ? fma(res, res, rr) : rr;
+            const uint32_t rec = ra[j];
+            acc = fma(ld(rec & 0xFFF8u), ld(rec >> 16), acc);
+            if (__builtin_amdgcn_readfirstlane(rec) & 1u) {   // the round's last step (all lanes)
+                const uint32_t f = fq[0], pos = f & 0xFFF8u, pv = f >> 16;
+                const bool valid = (f & 1u) != 0, div = pv != 0xFFFFu;
+                const double old = ld(pos), dv = ld(div ? pv : pos);
+                const double t = aq[0] - acc;
+                const double val = div ? t / dv : t;
+                if (RES) {
+                    const double res = div ? t - old * dv : t - old;
+                    rr = valid ? fma(res, res, rr) : rr;
+                }
+                if (valid) *reinterpret_cast<double *>(Db + pos) = val;
+                acc = 0.0;
+                r++;
+#pragma unroll
+                for (int k = 0; k < D; k++) fq[k] = fq[k + 1];
+                fq[D] = frec(r + D);
+#pragma unroll
+                for (int k = 0; k + 1 < D; k++) aq[k] = aq[k + 1];
+                aq[D - 1] = Ap[(fq[D - 1] & 0xFFF8u) >> 3];
             }
-            double *o = end ? reinterpret_cast<double *>(Db + pos) : scratch;
-            *o = val;
         }
 #pragma unroll
-        for (int j = 0; j < K; j++) {
-            ra[j] = na[j]; rb[j] = nb[j]; av[j] = nav[j];
-            na[j] = qa[j]; nb[j] = qb[j];
-        }
+        for (int j = 0; j < K; j++) ra[j] = na[j];
     }
     return rr;
 }

@@ -100,8 +100,9 @@ __global__ __launch_bounds__(KF_MAX_WAVES * WAVE) void k_factor_fast(
             wave_sync();
             double *src = F, *dst = W;
             while (used < sweeps) {
-                const double rr = wave_sum(parilu_stream<true>(fp.par0, fp.par1, fp.par_steps, Ap, src, dst,
-                                                               F + nnz + 2, lane));
+                const double rr = wave_sum(fp.par_rounds > 0
+                    ? parilu_rounds<true>(fp.parp, fp.parf, fp.par_rounds, fp.par_psteps, Ap, src, dst, lane)
+                    : 0.0);
                 wave_sync();
                 used++;
                 if (stop_tol > 0.0 && rr <= lim2) {

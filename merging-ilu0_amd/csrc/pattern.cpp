@@ -383,37 +383,40 @@ void build_fast_plan(PatternAnalysis &pa)
     fp.nlev_e = maxlev;
     fp.ne = int(fp.etab.size() / 4);
 
-    // ParILU sweep streams: r0 = byte offsets of the pair (l, u) in the source
-    // iterate (the zero cell nnz for an entry without products); r1 = byte
-    // offset of the entry | 1 on its first step | 2 on its last | byte offset
-    // of its pivot << 16 (0xFFFF: no division).  Padding steps act on entry 0
-    // without finishing it.
+    // ParILU sweep rounds: a product record holds the byte offsets of the pair
+    // (l, u) in the source iterate (the zero cell nnz for padding) | 1 on the
+    // round's last step; a finish record the byte offset of the entry | 1 for
+    // a lane with an entry | byte offset of its pivot << 16 (0xFFFF: none).
     {
         const uint32_t ZC = uint32_t(nnz) * 8u;
-        std::vector<std::vector<std::pair<uint32_t, uint32_t>>> L(64);
-        for (int q = 0; q < fp.ne; q++) {
-            const int e = fp.etab[4 * q], db = fp.etab[4 * q + 1], de = fp.etab[4 * q + 2], pv = fp.etab[4 * q + 3];
-            const int k = std::max(1, de - db);
-            const uint32_t pvw = (pv != 0xFFFF ? uint32_t(pv) * 8u : 0xFFFFu) << 16;
-            for (int t = 0; t < k; t++) {
-                const uint32_t r0 = (de > db) ? (uint32_t(fp.dep[2 * (db + t)]) * 8u) |
-                                                    ((uint32_t(fp.dep[2 * (db + t) + 1]) * 8u) << 16)
-                                              : (ZC | (ZC << 16));
-                const uint32_t r1 = uint32_t(e) * 8u | (t == 0 ? 1u : 0u) | (t == k - 1 ? 2u : 0u) | pvw;
-                L[q % 64].push_back({r0, r1});
+        std::vector<int> ord(fp.ne);
+        for (int q = 0; q < fp.ne; q++) ord[q] = q;
+        auto cnt = [&](int q) { return int(fp.etab[4 * q + 2]) - int(fp.etab[4 * q + 1]); };
+        std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return cnt(x) > cnt(y); });
+        const int R = (fp.ne + 63) / 64;
+        std::vector<int> base(R + 1, 0);
+        for (int r = 0; r < R; r++) base[r + 1] = base[r] + std::max(1, cnt(ord[64 * r]));
+        const int PS = std::max(PAR_BATCH, (base[R] + PAR_BATCH - 1) / PAR_BATCH * PAR_BATCH);
+        fp.par_rounds = R;
+        fp.par_psteps = PS;
+        fp.parp.assign(size_t(PS) * 64, ZC | (ZC << 16));
+        fp.parf.assign(size_t(std::max(R, 1)) * 64, 0xFFFFu << 16);
+        for (int r = 0; r < R; r++) {
+            for (int l = 0; l < 64; l++) {
+                const int w = 64 * r + l;
+                if (w < fp.ne) {
+                    const int q = ord[w];
+                    const int e = fp.etab[4 * q], db = fp.etab[4 * q + 1], de = fp.etab[4 * q + 2];
+                    const int pv = fp.etab[4 * q + 3];
+                    for (int t = 0; t < de - db; t++)
+                        fp.parp[size_t(base[r] + t) * 64 + l] = (uint32_t(fp.dep[2 * (db + t)]) * 8u) |
+                                                                ((uint32_t(fp.dep[2 * (db + t) + 1]) * 8u) << 16);
+                    fp.parf[size_t(r) * 64 + l] = uint32_t(e) * 8u | 1u |
+                                                  ((pv != 0xFFFF ? uint32_t(pv) * 8u : 0xFFFFu) << 16);
+                }
+                fp.parp[size_t(base[r + 1] - 1) * 64 + l] |= 1u;
             }
         }
-        int S = PAR_BATCH;
-        for (auto &v : L) S = std::max(S, (int)v.size());
-        S = (S + PAR_BATCH - 1) / PAR_BATCH * PAR_BATCH;
-        fp.par_steps = S;
-        fp.par0.assign(size_t(S) * 64, ZC | (ZC << 16));
-        fp.par1.assign(size_t(S) * 64, 0xFFFFu << 16);
-        for (int l = 0; l < 64; l++)
-            for (size_t t = 0; t < L[l].size(); t++) {
-                fp.par0[t * 64 + l] = L[l][t].first;
-                fp.par1[t * 64 + l] = L[l][t].second;
-            }
     }
     if (fp.dep.empty()) { fp.dep.assign(2, 0); }
     if (fp.etab.empty()) { fp.etab.assign(4, 0); }

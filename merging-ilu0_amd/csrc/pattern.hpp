@@ -69,11 +69,13 @@ struct FastPlan {
     // ILU_BATCH steps)
     std::vector<uint32_t> ilu0, ilu1;
     int ilu_steps = 0;
-    // one Jacobi ParILU(0) sweep as per-lane step streams [step][64]: entry q
-    // of the work order (etab) on lane q % 64, one step per product pair, the
-    // last step of an entry finishing it; padded to a multiple of PAR_BATCH
-    std::vector<uint32_t> par0, par1;
-    int par_steps = 0;
+    // one Jacobi ParILU(0) sweep in lane-synchronous rounds: the entries with
+    // work sorted by product count (most first, stable), entry q of that
+    // order on lane q % 64 in round q / 64.  Products [step][64] (round r
+    // takes max(1, its largest count) steps; bit 0 marks a round's last
+    // step), padded to a multiple of PAR_BATCH; finish records [round][64].
+    std::vector<uint32_t> parp, parf;
+    int par_rounds = 0, par_psteps = 0;
 };
 
 // steps per record batch of the ParILU sweep streams

@@ -847,7 +847,7 @@ struct bilu_pattern_s {
     uint32_t *d_isL0 = nullptr, *d_isL1 = nullptr, *d_isU0 = nullptr, *d_isU1 = nullptr;
     uint8_t *d_slot_kind = nullptr;
     uint32_t *d_ilu0 = nullptr, *d_ilu1 = nullptr;
-    uint32_t *d_par0 = nullptr, *d_par1 = nullptr;
+    uint32_t *d_parp = nullptr, *d_parf = nullptr;
     int *d_counter = nullptr;      // 256 bytes of work counters: [0] solve kernels, [32] factor kernel
     // solve workspace (factors between the factor and solve launches), grown on demand
     void *ws = nullptr;
@@ -945,8 +945,8 @@ int upload(bilu_pattern_s &h)
         const size_t o_sk = put(fp.slot_kind.data(), fp.slot_kind.size());
         const size_t o_i0 = put(fp.ilu0.data(), fp.ilu0.size() * 4);
         const size_t o_i1 = put(fp.ilu1.data(), fp.ilu1.size() * 4);
-        const size_t o_p0 = put(fp.par0.data(), fp.par0.size() * 4);
-        const size_t o_p1 = put(fp.par1.data(), fp.par1.size() * 4);
+        const size_t o_p0 = put(fp.parp.data(), fp.parp.size() * 4);
+        const size_t o_p1 = put(fp.parf.data(), fp.parf.size() * 4);
         HIP_TRY(hipMalloc((void **)&h.d_fast, blob.size()));
         HIP_TRY(hipMemcpy(h.d_fast, blob.data(), blob.size(), hipMemcpyHostToDevice));
         uint8_t *b = h.d_fast;
@@ -959,7 +959,7 @@ int upload(bilu_pattern_s &h)
         h.d_isU0 = (uint32_t *)(b + o_iU0); h.d_isU1 = (uint32_t *)(b + o_iU1);
         h.d_slot_kind = b + o_sk;
         h.d_ilu0 = (uint32_t *)(b + o_i0); h.d_ilu1 = (uint32_t *)(b + o_i1);
-        h.d_par0 = (uint32_t *)(b + o_p0); h.d_par1 = (uint32_t *)(b + o_p1);
+        h.d_parp = (uint32_t *)(b + o_p0); h.d_parf = (uint32_t *)(b + o_p1);
     }
     return BILU_OK;
 }
@@ -1023,7 +1023,7 @@ void fast_tables(const bilu_pattern_s &h, FastParams &fp)
     fp.isL_steps = pl.isL_steps; fp.isU_steps = pl.isU_steps;
     fp.slot_kind = h.d_slot_kind;
     fp.ilu0 = h.d_ilu0; fp.ilu1 = h.d_ilu1; fp.ilu_steps = pl.ilu_steps;
-    fp.par0 = h.d_par0; fp.par1 = h.d_par1; fp.par_steps = pl.par_steps;
+    fp.parp = h.d_parp; fp.parf = h.d_parf; fp.par_rounds = pl.par_rounds; fp.par_psteps = pl.par_psteps;
 }
 
 template <int NR, int SM>

@@ -1029,22 +1029,51 @@ int orc_entry_order(int n, const int *rp, const int *ci, const int *diag, int *o
     return ne;
 }
 
+/* The sweep order of the factor kernel's ParILU rounds: the entries with
+ * work (orc_entry_order) stably sorted by their number of products
+ * l_ik u_kj, most first; entry q of it is lane q % 64's in round q / 64.
+ * Returns the entry count. */
+int orc_sweep_order(int n, const int *rp, const int *ci, const int *diag, int *order)
+{
+    const int nnz = rp[n];
+    int *eo = (int *)malloc(sizeof(int) * (size_t)(nnz > 0 ? nnz : 1));
+    int *cnt = (int *)calloc((size_t)(nnz > 0 ? nnz : 1), sizeof(int));
+    const int ne = orc_entry_order(n, rp, ci, diag, eo);
+    int maxc = 0;
+    for (int i = 0; i < n; i++)
+        for (int e = rp[i]; e < rp[i + 1]; e++) {
+            const int j = ci[e], kmax = (i < j ? i : j) - 1;
+            int c = 0;
+            for (int p = rp[i]; p < diag[i] && ci[p] <= kmax; p++)
+                if (find_pos(rp, ci, ci[p], j) >= 0) c++;
+            cnt[e] = c;
+            if (c > maxc) maxc = c;
+        }
+    int w = 0;   /* by count, descending; stable in the entry order */
+    for (int c = maxc; c >= 0; c--)
+        for (int q = 0; q < ne; q++)
+            if (cnt[eo[q]] == c) order[w++] = eo[q];
+    free(eo); free(cnt);
+    return ne;
+}
+
 /* ParILU(0) Jacobi sweeps from `init` (NULL: from A; only the entries with
  * work take the guess, the others are exact = a_ij), stopped as soon as the
  * iterate a sweep starts from has ||(LU - A) on the pattern||_F <=
  * stop_tol ||A||_F (that iterate is kept), at most `sweeps` sweeps; when
  * the cap is reached without meeting the tolerance, exact ILU(0) replaces
  * the factors.  stop_tol <= 0: exactly `sweeps` sweeps.  The residual of an
- * iterate is summed in the kernel's order: entry q of the work order goes to
- * lane q % 64, then the wave tree.  *status = sweeps used, or -1 for the
- * fallback.  F[nnz] out. */
+ * iterate is summed in the kernel's order: entry q of the work order
+ * (orc_sweep_order) goes to lane q % 64, lanes adding in increasing q, then
+ * the wave tree.  *status = sweeps used, or -1 for the fallback.  F[nnz]
+ * out. */
 int orc_parilu0_stop(int n, const int *rp, const int *ci, const int *diag, const double *A,
                      const double *init, double *F, int sweeps, double stop_tol, int *status)
 {
     const int nnz = rp[n];
     int *order = (int *)malloc(sizeof(int) * (size_t)(nnz > 0 ? nnz : 1));
     double *W = (double *)malloc(sizeof(double) * (size_t)(nnz > 0 ? nnz : 1));
-    const int ne = orc_entry_order(n, rp, ci, diag, order);
+    const int ne = orc_sweep_order(n, rp, ci, diag, order);
     memcpy(F, A, sizeof(double) * (size_t)nnz);
     if (init)
         for (int q = 0; q < ne; q++) F[order[q]] = init[order[q]];

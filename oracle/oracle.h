@@ -66,6 +66,7 @@ typedef struct {
 } orc_opts;
 
 int orc_find_diag(int n, const int *rp, const int *ci, int *diag);
+int orc_sweep_order(int n, const int *rp, const int *ci, const int *diag, int *order);
 int orc_isai_pattern(int n, const int *rp, const int *ci, int type, int power, int *mrp, int *mci);
 int orc_isai_on(int n, const int *rp, const int *ci, const double *vals, int type,
                 const int *mrp, const int *mci, double *out);

@@ -94,6 +94,7 @@ def lib():
                                       C.c_int, C.c_int, C.POINTER(Opts), f32,
                                       P(np.int32, flags="C"), f64, f64, C.c_int]
         L.orc_entry_order.argtypes = [C.c_int, i32, i32, i32, i32]
+        L.orc_sweep_order.argtypes = [C.c_int, i32, i32, i32, i32]
         L.orc_isai_lu_f32.argtypes = [C.c_int, i32, i32, i32, f64, f32]
         L.orc_parilu0_stop.argtypes = [C.c_int, i32, i32, i32, f64, C.c_void_p, f64, C.c_int,
                                        C.c_double, C.POINTER(C.c_int)]
@@ -270,6 +271,14 @@ def entry_order(pat):
     ne = lib().orc_entry_order(pat.n, pat.row_ptrs, pat.col_idx, pat.diag, order)
     return order[:ne]
 
+
+
+def sweep_order(pat):
+    """entries with work sorted by product count (most first), stable in
+    the entry order: the factor kernel's ParILU round order"""
+    order = np.empty(pat.nnz, np.int32)
+    ne = lib().orc_sweep_order(pat.n, _i32(pat.row_ptrs), _i32(pat.col_idx), _i32(pat.diag), order)
+    return order[:ne]
 
 def parilu0_stop(pat, vals, sweeps, stop_tol=0.0, init=None):
     """ParILU(0) sweeps with warm start / residual stop / exact fallback.
