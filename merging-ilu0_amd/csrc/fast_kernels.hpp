@@ -238,7 +238,10 @@ __device__ __forceinline__ int ilu_stream(const uint32_t *__restrict__ r0p, cons
 // records arrive in batches, one batch ahead; finish records and a_ij (global
 // memory, L2 hits after the first sweep) are queued PAR_AHEAD rounds ahead.
 // Returns the lane's sum of squared residuals (RES) or 0.
-constexpr int PAR_AHEAD = 4;
+#ifndef PAR_AHEAD_ROUNDS
+#define PAR_AHEAD_ROUNDS 2
+#endif
+constexpr int PAR_AHEAD = PAR_AHEAD_ROUNDS;
 template <bool RES>
 __device__ __forceinline__ double parilu_rounds(const uint32_t *__restrict__ pp, const uint32_t *__restrict__ pf,
                                                 int R, int PS, const double *__restrict__ Ap, const double *src,
@@ -265,10 +268,14 @@ __device__ __forceinline__ double parilu_rounds(const uint32_t *__restrict__ pp,
         uint32_t na[K];
 #pragma unroll
         for (int j = 0; j < K; j++) na[j] = pp[(nx + j) * WAVE + lane];
+        // the batch's operands first: src is never written during the sweep
+        double lv[K], uv[K];
+#pragma unroll
+        for (int j = 0; j < K; j++) { lv[j] = ld(ra[j] & 0xFFF8u); uv[j] = ld(ra[j] >> 16); }
 #pragma unroll
         for (int j = 0; j < K; j++) {
             const uint32_t rec = ra[j];
-            acc = fma(ld(rec & 0xFFF8u), ld(rec >> 16), acc);
+            acc = fma(lv[j], uv[j], acc);
             if (__builtin_amdgcn_readfirstlane(rec) & 1u) {   // the round's last step (all lanes)
                 const uint32_t f = fq[0], pos = f & 0xFFF8u, pv = f >> 16;
                 const bool valid = (f & 1u) != 0, div = pv != 0xFFFFu;
