@@ -165,14 +165,19 @@ __global__ __launch_bounds__(256) void k_gemm(const __bf16 *__restrict__ A, int 
 }
 
 // features: X[s][e] = asinh(a / SIGMA) / YS (BF16) in FP32 arithmetic; padding
-// columns and rows s >= batch are 0.  Grid (ceil(P0 / 256), rows).
-__global__ void k_features(const double *__restrict__ A, int nnz, int P0, int batch, __bf16 *__restrict__ X)
+// columns and rows s >= batch are 0.  One block per system row s (its reads
+// and writes stay in one XCD's L2).
+__global__ __launch_bounds__(256) void k_features(const double *__restrict__ A, int nnz, int P0, int batch,
+                                                 __bf16 *__restrict__ X)
 {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x, s = blockIdx.y;
-    if (e >= P0) return;
-    float v = 0.0f;
-    if (e < nnz && s < batch) v = asinhf((float)A[(size_t)s * nnz + e] * INV_SIGMA_F) * INV_YS;
-    X[(size_t)s * P0 + e] = (__bf16)v;
+    const int s = blockIdx.x;
+    const double *As = A + (size_t)s * nnz;
+    __bf16 *Xs = X + (size_t)s * P0;
+    for (int e = threadIdx.x; e < P0; e += blockDim.x) {
+        float v = 0.0f;
+        if (e < nnz && s < batch) v = asinhf((float)As[e] * INV_SIGMA_F) * INV_YS;
+        Xs[e] = (__bf16)v;
+    }
 }
 
 // base guess of entry e of system s (piv[e] = position of a_jj for a lower
@@ -252,15 +257,28 @@ __global__ void k_colsum_final(const float *__restrict__ part, int N, float *__r
     g[n] = s;
 }
 
-// output: F_guess = SIGMA sinh(YS (base + y)) (FP64, unpadded).  Grid
-// (ceil(nnz / 256), batch).
-__global__ void k_outputs(const float *__restrict__ Y, const double *__restrict__ A, const int *__restrict__ piv,
-                          int nnz, int P3, int batch, double *__restrict__ Fo)
+// output: F_guess = SIGMA sinh(YS (base + y)) (FP64, unpadded).  One block
+// per system: its A row (FP32, as base_guess rounds it) is staged in LDS, so
+// the pivot reads of the base guess stay on chip.
+__global__ __launch_bounds__(256) void k_outputs(const float *__restrict__ Y, const double *__restrict__ A,
+                                                const int *__restrict__ piv, int nnz, int P3, int batch,
+                                                double *__restrict__ Fo)
 {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x, s = blockIdx.y;
-    if (e >= nnz || s >= batch) return;
-    const float yb = base_guess(A + (size_t)s * nnz, piv, e) + Y[(size_t)s * P3 + e];
-    Fo[(size_t)s * nnz + e] = SIGMA * (double)sinhf(YS * yb);
+    extern __shared__ float sa[];
+    const int s = blockIdx.x;
+    if (s >= batch) return;
+    const double *As = A + (size_t)s * nnz;
+    for (int e = threadIdx.x; e < nnz; e += blockDim.x) sa[e] = (float)As[e];
+    __syncthreads();
+    const float *Ys = Y + (size_t)s * P3;
+    double *Fs = Fo + (size_t)s * nnz;
+    for (int e = threadIdx.x; e < nnz; e += blockDim.x) {
+        const int pv = piv[e];
+        const float a = sa[e];
+        const float sv = pv >= 0 ? a / sa[pv] : a;
+        const float yb = asinhf(sv * INV_SIGMA_F) * INV_YS + Ys[e];
+        Fs[e] = SIGMA * (double)sinhf(YS * yb);
+    }
 }
 
 // Adam step on FP32 master weights w (moments m, v), BF16 copy wb
@@ -388,10 +406,9 @@ void forward(bilu_mlp_s &q, int rows, hipStream_t st)
 int loss_and_grad(bilu_mlp_s &q, int batch, const double *A, const double *F, float *d_loss, hipStream_t st)
 {
     const int rows = pad_to(batch, TM);
-    if (rows > 65535) return BILU_ERR_TOO_LARGE;   // one grid row per system in the element kernels
     int rc = reserve(q, rows);
     if (rc != BILU_OK) return rc;
-    hipLaunchKernelGGL(k_features, dim3((q.P0 + 255) / 256, rows), dim3(256), 0, st, A, q.nnz, q.P0, batch, q.X);
+    hipLaunchKernelGGL(k_features, dim3(rows), dim3(256), 0, st, A, q.nnz, q.P0, batch, q.X);
     forward(q, rows, st);
     MLP_TRY(hipMemsetAsync(q.dY, 0, (size_t)rows * q.P3 * 2, st));
     const unsigned nb = blocks((size_t)batch * q.P3);
@@ -603,10 +620,10 @@ int bilu_mlp_infer(bilu_mlp_t q, int batch, const double *d_A, double *d_F_init,
     if (rc != BILU_OK) return rc;
     for (int s0 = 0; s0 < batch; s0 += chunk) {
         const int nb = std::min(chunk, batch - s0), rows = pad_to(nb, TM);
-        hipLaunchKernelGGL(k_features, dim3((q->P0 + 255) / 256, rows), dim3(256), 0, st,
+        hipLaunchKernelGGL(k_features, dim3(rows), dim3(256), 0, st,
                            d_A + (size_t)s0 * q->nnz, q->nnz, q->P0, nb, q->X);
         forward(*q, rows, st);
-        hipLaunchKernelGGL(k_outputs, dim3((q->nnz + 255) / 256, nb), dim3(256), 0, st, q->Y,
+        hipLaunchKernelGGL(k_outputs, dim3(nb), dim3(256), (size_t)q->nnz * sizeof(float), st, q->Y,
                            d_A + (size_t)s0 * q->nnz, q->piv, q->nnz, q->P3, nb, d_F_init + (size_t)s0 * q->nnz);
     }
     MLP_TRY(hipGetLastError());
