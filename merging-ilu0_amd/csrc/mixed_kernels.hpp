@@ -26,27 +26,39 @@ constexpr int FS_ILU = 0, FS_PARILU = 1;
 __device__ __forceinline__ int isai_gen(const uint32_t *__restrict__ r0p, const uint32_t *__restrict__ r1p,
                                         int steps, const double *F, float *M, float *mscratch, int lane)
 {
+    constexpr int K = bilu::ISAI_BATCH;
     const char *Fb = reinterpret_cast<const char *>(F);
     char *Mb = reinterpret_cast<char *>(M);
     double acc = 0.0;
     int bad = 0;
-    uint32_t a = r0p[lane], b = r1p[lane];
-    for (int st = 0; st < steps; st++) {
-        const int nx = (st + 1 < steps) ? st + 1 : st;
-        const uint32_t an = r0p[nx * WAVE + lane], bn = r1p[nx * WAVE + lane];
-        const double mv = (double)*reinterpret_cast<const float *>(Mb + ((a >> 16) >> 1));
-        const double fv = *reinterpret_cast<const double *>(Fb + (a & 0xFFFFu));
-        acc = fma(-mv, fv, acc);
-        const uint32_t p = b & 0xFFFFu, d = b >> 16;
-        const bool fin = p != 0xFFFFu, div = d != 0xFFFFu;
-        const double dv = *reinterpret_cast<const double *>(Fb + (div ? d : 0u));
-        bad |= (fin && div && dv == 0.0);
-        const double val = div ? acc / dv : acc;
-        float *dst = fin ? reinterpret_cast<float *>(Mb + (p >> 1)) : mscratch;
-        *dst = (float)val;
-        acc = fin ? 0.0 : acc;
-        a = an;
-        b = bn;
+    // records arrive in batches of K steps (streams are padded to whole
+    // batches), the next batch in flight during the current one; operands
+    // are read in the step (a step may read a value the lane wrote just before)
+    uint32_t ra[K], rb[K];
+#pragma unroll
+    for (int j = 0; j < K; j++) { ra[j] = r0p[j * WAVE + lane]; rb[j] = r1p[j * WAVE + lane]; }
+    for (int s0 = 0; s0 < steps; s0 += K) {
+        const int nx = (s0 + K < steps) ? s0 + K : s0;
+        uint32_t na[K], nb[K];
+#pragma unroll
+        for (int j = 0; j < K; j++) { na[j] = r0p[(nx + j) * WAVE + lane]; nb[j] = r1p[(nx + j) * WAVE + lane]; }
+#pragma unroll
+        for (int j = 0; j < K; j++) {
+            const uint32_t a = ra[j], b = rb[j];
+            const double mv = (double)*reinterpret_cast<const float *>(Mb + ((a >> 16) >> 1));
+            const double fv = *reinterpret_cast<const double *>(Fb + (a & 0xFFFFu));
+            acc = fma(-mv, fv, acc);
+            const uint32_t p = b & 0xFFFFu, d = b >> 16;
+            const bool fin = p != 0xFFFFu, div = d != 0xFFFFu;
+            const double dv = *reinterpret_cast<const double *>(Fb + (div ? d : 0u));
+            bad |= (fin && div && dv == 0.0);
+            const double val = div ? acc / dv : acc;
+            float *dst = fin ? reinterpret_cast<float *>(Mb + (p >> 1)) : mscratch;
+            *dst = (float)val;
+            acc = fin ? 0.0 : acc;
+        }
+#pragma unroll
+        for (int j = 0; j < K; j++) { ra[j] = na[j]; rb[j] = nb[j]; }
     }
     return bad;
 }

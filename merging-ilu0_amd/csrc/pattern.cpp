@@ -485,6 +485,7 @@ void build_fast_plan(PatternAnalysis &pa)
         }
         size_t S = 1;
         for (auto &v : L) S = std::max(S, v.size());
+        S = (S + ISAI_BATCH - 1) / ISAI_BATCH * ISAI_BATCH;   // whole record batches
         steps = int(S);
         r0.assign(S * 64, rec0(FZ, MZ));
         r1.assign(S * 64, rec1(0xFFFFu, 0xFFFFu));

@@ -80,6 +80,8 @@ struct FastPlan {
 
 // steps per record batch of the ParILU sweep streams
 constexpr int PAR_BATCH = 8;
+// steps per record batch of the ISAI generation streams (padded with no-op steps)
+constexpr int ISAI_BATCH = 8;
 
 // steps per record batch of the ILU(0) streams (records of a batch are
 // loaded together, one batch ahead)
