@@ -381,7 +381,7 @@ __global__ __launch_bounds__(KF_MAX_WAVES * WAVE) void k_fast(DevPattern P, Fast
         int bad = 0;
         if (sp.factor == FK_LOAD) {
             const double *Fp = gF + (size_t)sys * nnz;
-            for (int e = lane; e < nnz; e += WAVE) F[e] = Fp[e];
+            wave_copy(F, Fp, nnz, lane);
 #pragma unroll
             for (int q = 0; q < SM; q++) {
                 a[q] = 0.0;
@@ -390,7 +390,7 @@ __global__ __launch_bounds__(KF_MAX_WAVES * WAVE) void k_fast(DevPattern P, Fast
             }
             wave_sync();
         } else {
-            for (int e = lane; e < nnz; e += WAVE) F[e] = Ap[e];
+            wave_copy(F, Ap, nnz, lane);
             wave_sync();
 #pragma unroll
             for (int q = 0; q < SM; q++) {

@@ -87,22 +87,26 @@ __global__ __launch_bounds__(KF_MAX_WAVES * WAVE) void k_factor_fast(
         if (sys >= batch) break;
         const double *Ap = gA + (size_t)(sysidx ? sysidx[sys] : sys) * nnz;
         int status = 0;
-        for (int e = lane; e < nnz; e += WAVE) F[e] = Ap[e];
+        wave_copy(F, Ap, nnz, lane);
         if (mode == FS_PARILU) {
-            if (gFinit) {
-                // entries with work start from the guess; the others are exact (= a)
-                const double *Ip = gFinit + (size_t)(sysidx ? sysidx[sys] : sys) * nnz;
-                for (int idx = lane; idx < fp.ne; idx += WAVE) {
-                    const int pos = fp.etab[idx].x;
-                    F[pos] = Ip[pos];
-                }
-            }
             wave_sync();
             double anorm2 = 0.0;
             if (stop_tol > 0.0) {
-                for (int e = lane; e < nnz; e += WAVE) anorm2 = fma(Ap[e], Ap[e], anorm2);
+                for (int e = lane; e < nnz; e += WAVE) anorm2 = fma(F[e], F[e], anorm2);
                 anorm2 = wave_sum(anorm2);
             }
+            if (gFinit) {
+                // entries with work start from the guess; the others are exact
+                // (= a).  The guess row comes in through W with coalesced loads.
+                const double *Ip = gFinit + (size_t)(sysidx ? sysidx[sys] : sys) * nnz;
+                wave_copy(W, Ip, nnz, lane);
+                wave_sync();
+                for (int idx = lane; idx < fp.ne; idx += WAVE) {
+                    const int pos = fp.etab[idx].x;
+                    F[pos] = W[pos];
+                }
+            }
+            wave_sync();
             const double lim2 = stop_tol * stop_tol * anorm2;
             int used = 0;
             bool met = stop_tol <= 0.0;
@@ -131,7 +135,7 @@ __global__ __launch_bounds__(KF_MAX_WAVES * WAVE) void k_factor_fast(
             }
             if (!met) {
                 // fallback: exact ILU(0)
-                for (int e = lane; e < nnz; e += WAVE) F[e] = Ap[e];
+                wave_copy(F, Ap, nnz, lane);
                 wave_sync();
                 ilu_stream(fp.ilu0, fp.ilu1, fp.ilu_steps, F, F + nnz + 2, fp.diag, n, lane);
                 status = -1;
@@ -307,7 +311,7 @@ __global__ __launch_bounds__(km_waves(SM, PCK) * WAVE) void k_mixed(
         const double *bp = gb + (size_t)si * n;
         const float *Fp = gF + (size_t)sys * nnz;
         if (PCK == PCK_LU)
-            for (int e = lane; e < nnz; e += WAVE) F[e] = Fp[e];
+            wave_copy(F, Fp, nnz, lane);
 #pragma unroll
         for (int q = 0; q < SM; q++) {
             const int e = (q < S) ? fp.slot_pos[q * WAVE + lane] : -1;

@@ -73,6 +73,32 @@ struct SolveParams {
     int factor, sweeps;
 };
 
+// ---------------------------------------------------------------- copies
+// global -> LDS copy of cnt values by one wave (lane l takes l, l + 64, ...),
+// U loads in flight per lane: a plain copy loop waits for each load before
+// its store, which leaves a wave idle on one memory latency per 64 values
+template <typename TD, typename TS>
+__device__ __forceinline__ void wave_copy(TD *dst, const TS *__restrict__ src, int cnt, int lane)
+{
+    constexpr int U = 8;
+    int e0 = lane;
+    for (; e0 + (U - 1) * WAVE < cnt; e0 += U * WAVE) {
+        TS v[U];
+#pragma unroll
+        for (int j = 0; j < U; j++) v[j] = src[e0 + j * WAVE];
+#pragma unroll
+        for (int j = 0; j < U; j++) dst[e0 + j * WAVE] = (TD)v[j];
+    }
+    if (e0 < cnt) {
+        TS v[U];
+#pragma unroll
+        for (int j = 0; j < U; j++) v[j] = (e0 + j * WAVE < cnt) ? src[e0 + j * WAVE] : TS(0);
+#pragma unroll
+        for (int j = 0; j < U; j++)
+            if (e0 + j * WAVE < cnt) dst[e0 + j * WAVE] = (TD)v[j];
+    }
+}
+
 // ---------------------------------------------------------------- wave ops
 // 64-bit lane exchanges without the LDS pipe: DPP within 16-lane rows,
 // gfx950 permlane swaps across rows / halves
@@ -499,12 +525,12 @@ __global__ __launch_bounds__(WAVE) void k_solve(DevPattern P, int batch, const d
     double *e2 = e1 + n;
 
     const double *Ap = gA + (size_t)sys * nnz;
-    for (int e = lane; e < nnz; e += WAVE) sA[e] = Ap[e];
+    wave_copy(sA, Ap, nnz, lane);
     if (sp.factor == FK_LOAD) {
         const double *Fp = gF + (size_t)sys * nnz;
-        for (int e = lane; e < nnz; e += WAVE) sF[e] = Fp[e];
+        wave_copy(sF, Fp, nnz, lane);
     } else if (PC != PC_NONE) {
-        for (int e = lane; e < nnz; e += WAVE) sF[e] = Ap[e];
+        wave_copy(sF, Ap, nnz, lane);
     }
     Vec<NR> b, x, r, rh, p, v, s, t, ph, sh;
     const double *bp = gb + (size_t)sys * n;
@@ -682,7 +708,8 @@ __global__ __launch_bounds__(WAVE) void k_factor(DevPattern P, int batch, const 
     double *sA = lds, *sF = lds + nnz, *sW = sF + nnz;
     const double *Ap = gA + (size_t)sys * nnz;
     const double *Ip = gInit ? gInit + (size_t)sys * nnz : Ap;
-    for (int e = lane; e < nnz; e += WAVE) { sA[e] = Ap[e]; sF[e] = Ip[e]; }
+    wave_copy(sA, Ap, nnz, lane);
+    wave_copy(sF, Ip, nnz, lane);
     __syncthreads();
     int bad = 0;
     if (kind == FK_ILU) bad = ilu_factor(P, sF, lane);
@@ -705,7 +732,8 @@ __global__ __launch_bounds__(WAVE) void k_isai(DevPattern P, int batch, const do
     double *sV = lds, *sO = lds + nnz;
     const double *Vp = gV + (size_t)sys * nnz;
     double *Op = gOut + (size_t)sys * nnz;
-    for (int e = lane; e < nnz; e += WAVE) { sV[e] = Vp[e]; sO[e] = Op[e]; }
+    wave_copy(sV, Vp, nnz, lane);
+    wave_copy(sO, Op, nnz, lane);
     __syncthreads();
     int bad;
     if (type == BILU_ISAI_GENERAL) bad = isai_general(P.isai[type], P.n, sV, sO, lane);
@@ -727,7 +755,7 @@ __global__ __launch_bounds__(WAVE) void k_isai_pow(DevPattern P, DevIsaiPow Q, i
     double *sV = lds, *sO = lds + nnz;
     const double *Vp = gV + (size_t)sys * nnz;
     double *Op = gOut + (size_t)sys * Q.nnz;
-    for (int e = lane; e < nnz; e += WAVE) sV[e] = Vp[e];
+    wave_copy(sV, Vp, nnz, lane);
     __syncthreads();
     int bad;
     if (type == BILU_ISAI_GENERAL) bad = isai_general(Q.map, P.n, sV, sO, lane);
@@ -809,7 +837,7 @@ __global__ __launch_bounds__(WAVE) void k_true_residual(DevPattern P, int batch,
     if (sys >= batch) return;
     const int lane = threadIdx.x, n = P.n, nnz = P.nnz;
     double *sA = lds, *sx = lds + nnz;
-    for (int e = lane; e < nnz; e += WAVE) sA[e] = gA[(size_t)sys * nnz + e];
+    wave_copy(sA, gA + (size_t)sys * nnz, nnz, lane);
     for (int i = lane; i < n; i += WAVE) sx[i] = gx[(size_t)sys * n + i];
     __syncthreads();
     Vec<NR> w;
