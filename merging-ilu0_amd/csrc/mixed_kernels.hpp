@@ -101,9 +101,13 @@ __global__ __launch_bounds__(KF_MAX_WAVES * WAVE) void k_factor_fast(
                 const double *Ip = gFinit + (size_t)(sysidx ? sysidx[sys] : sys) * nnz;
                 wave_copy(W, Ip, nnz, lane);
                 wave_sync();
-                for (int idx = lane; idx < fp.ne; idx += WAVE) {
-                    const int pos = fp.etab[idx].x;
-                    F[pos] = W[pos];
+                for (int i0 = lane; i0 < fp.ne; i0 += 8 * WAVE) {
+                    int pos[8];
+#pragma unroll
+                    for (int j = 0; j < 8; j++) pos[j] = (i0 + j * WAVE < fp.ne) ? fp.etab[i0 + j * WAVE].x : -1;
+#pragma unroll
+                    for (int j = 0; j < 8; j++)
+                        if (pos[j] >= 0) F[pos[j]] = W[pos[j]];
                 }
             }
             wave_sync();
@@ -111,7 +115,7 @@ __global__ __launch_bounds__(KF_MAX_WAVES * WAVE) void k_factor_fast(
             int used = 0;
             bool met = stop_tol <= 0.0;
             // Jacobi sweeps between F and W (entries without work and the zero cell in both)
-            for (int e = lane; e < nnz; e += WAVE) W[e] = F[e];
+            wave_copy(W, F, nnz, lane);
             if (lane == 0) W[nnz] = 0.0;
             wave_sync();
             double *src = F, *dst = W;
@@ -130,7 +134,7 @@ __global__ __launch_bounds__(KF_MAX_WAVES * WAVE) void k_factor_fast(
                 double *tmp = src; src = dst; dst = tmp;
             }
             if (src != F) {
-                for (int e = lane; e < nnz; e += WAVE) F[e] = src[e];
+                wave_copy(F, src, nnz, lane);
                 wave_sync();
             }
             if (!met) {
@@ -153,12 +157,12 @@ __global__ __launch_bounds__(KF_MAX_WAVES * WAVE) void k_factor_fast(
             bad |= isai_gen(fp.isU0, fp.isU1, fp.isU_steps, F, M, M + nnz + 3, lane);
             wave_sync();
             float *Mo = gM + (size_t)sys * nnz;
-            for (int e = lane; e < nnz; e += WAVE) Mo[e] = M[e];
+            wave_copy(Mo, M, nnz, lane);
             bad = __any(bad);
             if (lane == 0) gZ[sys] = bad;
         } else {
             TO *Fo = gF + (size_t)sys * nnz;
-            for (int e = lane; e < nnz; e += WAVE) Fo[e] = (TO)F[e];
+            wave_copy(Fo, F, nnz, lane);
         }
         if (g_status && lane == 0) g_status[sys] = status;
         wave_sync();
