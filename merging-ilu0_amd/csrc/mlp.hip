@@ -615,7 +615,7 @@ int bilu_mlp_infer(bilu_mlp_t q, int batch, const double *d_A, double *d_F_init,
     if (batch == 0) return BILU_OK;
     hipStream_t st = (hipStream_t)stream;
     // in chunks, so the activations stay bounded
-    const int chunk = 16384;
+    const int chunk = 65536;
     int rc = reserve(*q, std::min(pad_to(batch, TM), chunk));
     if (rc != BILU_OK) return rc;
     for (int s0 = 0; s0 < batch; s0 += chunk) {
