@@ -206,7 +206,7 @@ __device__ __forceinline__ void spmv_slots_global(const double *__restrict__ Ap,
             const int s = g + j;
             if (s >= SM) break;
             const uint32_t m = meta[s * WAVE + lane];
-            acc = fma(av[j], *reinterpret_cast<const double *>(xb + (m & 0xFFFFu)), acc);
+            acc = fma(av[j], *reinterpret_cast<const double *>(xb + (m & 0xFFF8u)), acc);
             const bool end = (m >> 16) != 0xFFFFu;
             double *dst = end ? reinterpret_cast<double *>(yb + (m >> 16)) : scratch;
             *dst = acc;
@@ -216,10 +216,15 @@ __device__ __forceinline__ void spmv_slots_global(const double *__restrict__ Ap,
     }
 }
 
-// y (LDS, by row) = C x over the slot layout with the slot coefficients C
-// given per slot: in registers (c[s]) or in the wave's LDS, slot-major
-// (cl[s * 64 + lane]).  Same CSR-order fma chain as spmv_slots.
-template <int SM, bool FROM_LDS>
+// y (LDS, by row) = C x over the slot layout.  Slot coefficients: CM_LDS:
+// from the wave's LDS, slot-major (cl[s * 64 + lane]); CM_ISAI_L / CM_ISAI_U:
+// from the slot's ISAI value c[s] (registers) and the slot's triangle in
+// bits 0-1 of its meta record (1 strictly lower, 2 diagonal, 0 upper or
+// padding): ISAI(L) takes c below the diagonal, 1 on it, 0 above; ISAI(U)
+// takes c on and above the diagonal, 0 below.  Same CSR-order fma chain as
+// spmv_slots.
+constexpr int CM_LDS = 0, CM_ISAI_L = 1, CM_ISAI_U = 2;
+template <int SM, int CM>
 __device__ __forceinline__ void spmv_coef(const float (&c)[SM], const float *__restrict__ cl,
                                           const uint32_t *__restrict__ meta, const float *__restrict__ x,
                                           float *__restrict__ y, float *__restrict__ scratch, int lane)
@@ -229,9 +234,13 @@ __device__ __forceinline__ void spmv_coef(const float (&c)[SM], const float *__r
     float acc = 0.0f;
 #pragma unroll
     for (int s = 0; s < SM; s++) {
-        const float cs = FROM_LDS ? cl[s * WAVE + lane] : c[s];
         const uint32_t mm = meta[s * WAVE + lane];
-        acc = __builtin_fmaf(cs, *reinterpret_cast<const float *>(xb + ((mm & 0xFFFFu) >> 1)), acc);
+        const uint32_t kd = mm & 3u;
+        float cs;
+        if (CM == CM_LDS) cs = cl[s * WAVE + lane];
+        else if (CM == CM_ISAI_L) cs = kd == 1u ? c[s] : (kd == 2u ? 1.0f : 0.0f);
+        else cs = kd == 1u ? 0.0f : c[s];
+        acc = __builtin_fmaf(cs, *reinterpret_cast<const float *>(xb + ((mm & 0xFFF8u) >> 1)), acc);
         const bool end = (mm >> 16) != 0xFFFFu;
         float *dst = end ? reinterpret_cast<float *>(yb + ((mm >> 16) >> 1)) : scratch;
         *dst = acc;
@@ -271,7 +280,10 @@ __global__ __launch_bounds__(km_waves(SM, PCK) * WAVE) void k_mixed(
     uint32_t *meta = reinterpret_cast<uint32_t *>(smem);
     const uint32_t *trL = fp.trL;
     const uint32_t *trU = fp.trU;
-    for (int i = threadIdx.x; i < SM * WAVE; i += blockDim.x) meta[i] = (i < S * WAVE) ? fp.slot_meta[i] : 0xFFFF0000u;
+    // (ISAI: the slot's triangle in bits 0-1; x offsets are multiples of 8)
+    for (int i = threadIdx.x; i < SM * WAVE; i += blockDim.x)
+        meta[i] = (i < S * WAVE) ? (fp.slot_meta[i] | (PCK == PCK_ISAI ? (uint32_t)fp.slot_kind[i] : 0u))
+                                 : 0xFFFF0000u;
     // per wave: F[nnz + 2] FP32 (F[nnz] = 0), then two vector regions of
     // (n + 2) doubles each, used as FP32 vectors by the inner solve (zero
     // cell at [n], scratch at [n + 1]) and as FP64 vectors by the outer
@@ -296,12 +308,12 @@ __global__ __launch_bounds__(km_waves(SM, PCK) * WAVE) void k_mixed(
         rlev[k] = (int)fp.rlevL[k * WAVE + lane] | ((int)fp.rlevU[k * WAVE + lane] << 16);
     const int vcntL = fp.cntL[lane], vcntU = fp.cntU[lane];
 
-    // LU: A (FP32) in VGPRs.  ISAI: the slot coefficients of ISAI(L) (strict
-    // lower values, 1 on the diagonal, 0 above) and ISAI(U) (0 below, values
-    // on and above the diagonal) in VGPRs, A in LDS.
+    // LU: A (FP32) in VGPRs.  ISAI: one ISAI value per slot in VGPRs (the
+    // strictly lower slots hold ISAI(L), the others ISAI(U); spmv_coef derives
+    // each triangle's coefficients from the slot's meta bits), A in LDS.
     constexpr int SA = PCK == PCK_LU ? SM : 1, SC = PCK == PCK_ISAI ? SM : 1;
     float a[SA];
-    float cL[SC], cU[SC];
+    float cv[SC];
 
     for (;;) {
         int sys = 0;
@@ -324,10 +336,7 @@ __global__ __launch_bounds__(km_waves(SM, PCK) * WAVE) void k_mixed(
                 a[q] = av;
             } else {
                 Asl[q * WAVE + lane] = av;
-                const int kd = (q < S) ? fp.slot_kind[q * WAVE + lane] : 0;
-                const float m = (e >= 0) ? Fp[e] : 0.0f;
-                cL[q] = kd == 1 ? m : (kd == 2 ? 1.0f : 0.0f);
-                cU[q] = kd == 1 ? 0.0f : m;
+                cv[q] = (e >= 0) ? Fp[e] : 0.0f;
             }
         }
         wave_sync();
@@ -376,16 +385,16 @@ __global__ __launch_bounds__(km_waves(SM, PCK) * WAVE) void k_mixed(
             } else {
                 to_lds(in, e0, lane, n);
                 wave_sync();
-                spmv_coef<SM, false>(cL, nullptr, meta, e0, e1, scratch, lane);
+                spmv_coef<SM, CM_ISAI_L>(cv, nullptr, meta, e0, e1, scratch, lane);
                 wave_sync();
-                spmv_coef<SM, false>(cU, nullptr, meta, e1, e0, e0 + (n + 1), lane);
+                spmv_coef<SM, CM_ISAI_U>(cv, nullptr, meta, e1, e0, e0 + (n + 1), lane);
                 wave_sync();
                 from_lds(out, e0, lane, n);
             }
         };
         auto matvec = [&](Vec<NR, float> &out) {   // out = A32 e0
             if constexpr (PCK == PCK_LU) spmv_slots<SM, float>(a, meta, e0, e1, scratch, lane);
-            else spmv_coef<SM, true>(cL, Asl, meta, e0, e1, scratch, lane);
+            else spmv_coef<SM, CM_LDS>(cv, Asl, meta, e0, e1, scratch, lane);
             wave_sync();
             from_lds(out, e1, lane, n);
             wave_sync();
