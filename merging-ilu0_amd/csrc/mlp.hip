@@ -170,13 +170,20 @@ __global__ __launch_bounds__(256) void k_gemm(const __bf16 *__restrict__ A, int 
 __global__ __launch_bounds__(256) void k_features(const double *__restrict__ A, int nnz, int P0, int batch,
                                                  __bf16 *__restrict__ X)
 {
+    constexpr int U = 8;   // loads in flight per thread
     const int s = blockIdx.x;
     const double *As = A + (size_t)s * nnz;
     __bf16 *Xs = X + (size_t)s * P0;
-    for (int e = threadIdx.x; e < P0; e += blockDim.x) {
-        float v = 0.0f;
-        if (e < nnz && s < batch) v = asinhf((float)As[e] * INV_SIGMA_F) * INV_YS;
-        Xs[e] = (__bf16)v;
+    const int lim = s < batch ? nnz : 0;
+    for (int e0 = threadIdx.x; e0 < P0; e0 += U * 256) {
+        double a[U];
+#pragma unroll
+        for (int j = 0; j < U; j++) a[j] = (e0 + j * 256 < lim) ? As[e0 + j * 256] : 0.0;
+#pragma unroll
+        for (int j = 0; j < U; j++) {
+            const int e = e0 + j * 256;
+            if (e < P0) Xs[e] = (__bf16)(e < lim ? asinhf((float)a[j] * INV_SIGMA_F) * INV_YS : 0.0f);
+        }
     }
 }
 
@@ -264,20 +271,41 @@ __global__ __launch_bounds__(256) void k_outputs(const float *__restrict__ Y, co
                                                 const int *__restrict__ piv, int nnz, int P3, int batch,
                                                 double *__restrict__ Fo)
 {
+    constexpr int U = 8;   // loads in flight per thread
     extern __shared__ float sa[];
     const int s = blockIdx.x;
     if (s >= batch) return;
     const double *As = A + (size_t)s * nnz;
-    for (int e = threadIdx.x; e < nnz; e += blockDim.x) sa[e] = (float)As[e];
+    for (int e0 = threadIdx.x; e0 < nnz; e0 += U * 256) {
+        double a[U];
+#pragma unroll
+        for (int j = 0; j < U; j++) a[j] = (e0 + j * 256 < nnz) ? As[e0 + j * 256] : 0.0;
+#pragma unroll
+        for (int j = 0; j < U; j++)
+            if (e0 + j * 256 < nnz) sa[e0 + j * 256] = (float)a[j];
+    }
     __syncthreads();
     const float *Ys = Y + (size_t)s * P3;
     double *Fs = Fo + (size_t)s * nnz;
-    for (int e = threadIdx.x; e < nnz; e += blockDim.x) {
-        const int pv = piv[e];
-        const float a = sa[e];
-        const float sv = pv >= 0 ? a / sa[pv] : a;
-        const float yb = asinhf(sv * INV_SIGMA_F) * INV_YS + Ys[e];
-        Fs[e] = SIGMA * (double)sinhf(YS * yb);
+    for (int e0 = threadIdx.x; e0 < nnz; e0 += U * 256) {
+        float y[U];
+        int pv[U];
+#pragma unroll
+        for (int j = 0; j < U; j++) {
+            const bool in = e0 + j * 256 < nnz;
+            y[j] = in ? Ys[e0 + j * 256] : 0.0f;
+            pv[j] = in ? piv[e0 + j * 256] : -1;
+        }
+#pragma unroll
+        for (int j = 0; j < U; j++) {
+            const int e = e0 + j * 256;
+            if (e < nnz) {
+                const float a = sa[e];
+                const float sv = pv[j] >= 0 ? a / sa[pv[j]] : a;
+                const float yb = asinhf(sv * INV_SIGMA_F) * INV_YS + y[j];
+                Fs[e] = SIGMA * (double)sinhf(YS * yb);
+            }
+        }
     }
 }
 
