@@ -328,8 +328,21 @@ def main():
     guess = torch.empty_like(dA)
     o_learn = bilu.opts(precond=bilu.PARILU0_ISAI, parilu_sweeps=40, parilu_stop_tol=1e-8,
                         precision=bilu.PRECISION_FP32_IR, inner_tol=args.inner_tol, **common)
-    ms_learn = run(o_learn, F_init=guess, pre=lambda: net.infer(dA, guess, stream=stream))
+    # (3a) split: bilu_mlp_infer writes the FP64 guess, bilu_solve_ex reads it
+    ms_split = run(o_learn, F_init=guess, pre=lambda: net.infer(dA, guess, stream=stream))
+    split = summary(ms_split)
+    x_split = x.clone()
+    # (3) fused (bilu_mlp_solve): the factor kernel forms the guess from the
+    # network's raw outputs; the same results bit for bit
+    out["fs"].zero_()
+    out["rf"].zero_()
+
+    def fused_step():
+        net.solve(P, dA, db, x, o_learn, out["it"], out["cv"], out["ir"], out["tr"], out["rf"], out["fs"],
+                  stream=stream)
+    ms_learn = time_steps(fused_step, args.warmup, args.steps, stream)
     learned = summary(ms_learn)
+    fused_equal = bool(torch.equal(x, x_split))
     # (3b) the same ParILU path from the standard start (no network), for the gain
     ms_nolearn = run(o_learn)
     nolearn = summary(ms_nolearn)
@@ -347,7 +360,9 @@ def main():
                        variant="learned+mixed"),
         "speedup_vs_plain_fp64": round(ms_plain / ms_learn, 3),
         "variants": {"plain_fp64_ilu0_merged": plain, "mixed_ilu0_isai": mixed,
-                     "learned_mixed_parilu0_isai": learned, "mixed_parilu0_isai_no_warm_start": nolearn},
+                     "learned_mixed_parilu0_isai": learned, "learned_mixed_parilu0_isai_split": split,
+                     "mixed_parilu0_isai_no_warm_start": nolearn},
+        "fused_x_equals_split_x": fused_equal,
         "mlp_inference_ms": round(ms_infer, 4),
         "training": {"systems": args.train_systems, "epochs": args.train_epochs, "minibatch": mb,
                      "label_generation_s": round(gen_train_s, 2), "train_s": round(train_s, 2),
@@ -356,7 +371,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
                      "traffic": measured_traffic(f"learned_mixed n{n} b{B}"),
-                     "kernel": "whole learned+mixed step (MLP inference, k_factor_fast, k_mixed)",
+                     "kernel": "whole learned+mixed step (MLP features + GEMMs, k_factor_fast, k_mixed)",
                      "alg_bytes_per_launch": alg_bytes},
         "roofline_plain_fp64": {"bound": "hbm", "achieved": round(achieved_plain, 2), "peak": HBM_PEAK_GBS,
                                 "unit": "GB/s", "frac": round(achieved_plain / HBM_PEAK_GBS, 5),

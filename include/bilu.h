@@ -252,6 +252,16 @@ int bilu_mlp_train_step(bilu_mlp_t m, int batch, const double *d_A, const double
                         float *d_loss, void *stream);
 /* predicted factors d_F_init[batch*nnz] (FP64, in-place factor layout) */
 int bilu_mlp_infer(bilu_mlp_t m, int batch, const double *d_A, double *d_F_init, void *stream);
+/* learned + mixed-precision solve in one call: the same results, bit for bit,
+ * as bilu_mlp_infer followed by bilu_solve_ex with that d_F_init and the
+ * FP32-IR ParILU(0) or ParILU(0)+ISAI options, without the FP64 d_F_init
+ * round trip (the factor kernel turns the network's raw outputs into the warm
+ * start itself).  m must have been created on the handle p; opts->precond is
+ * BILU_PRECOND_PARILU0 or _PARILU0_ISAI, opts->precision BILU_PRECISION_FP32_IR,
+ * isai_power 1.  Outputs as bilu_solve_ex (d_factor_status per system). */
+int bilu_mlp_solve(bilu_mlp_t m, bilu_pattern_t p, int batch, const double *d_A, const double *d_b, double *d_x,
+                   const bilu_opts *opts, float *d_iter, int *d_conv, double *d_iter_res, double *d_true_res,
+                   int *d_refine, int *d_factor_status, void *stream);
 
 #ifdef __cplusplus
 }

@@ -34,7 +34,7 @@ EXPORTS = [
     "bilu_mlp_get_weights", "bilu_mlp_set_weights", "bilu_mlp_gradient", "bilu_mlp_train_step",
     "bilu_mlp_infer", "bilu_uneven_create", "bilu_uneven_destroy", "bilu_uneven_tune", "bilu_uneven_config",
     "bilu_uneven_solve", "bilu_pattern_add_diagonal", "bilu_expand_values", "bilu_isai_pattern",
-    "bilu_isai_power",
+    "bilu_isai_power", "bilu_mlp_solve",
 ]
 
 
@@ -113,6 +113,7 @@ def lib():
         L.bilu_mlp_gradient.argtypes = [vp, C.c_int, vp, vp, vp, vp, vp]
         L.bilu_mlp_train_step.argtypes = [vp, C.c_int, vp, vp, C.c_float, vp, vp]
         L.bilu_mlp_infer.argtypes = [vp, C.c_int, vp, vp, vp]
+        L.bilu_mlp_solve.argtypes = [vp, vp, C.c_int, vp, vp, vp, C.POINTER(Opts), vp, vp, vp, vp, vp, vp, vp]
         _lib = L
     return _lib
 
@@ -360,6 +361,24 @@ class Mlp:
         _check(lib().bilu_mlp_infer(self.h, A.shape[0], _ptr(A), _ptr(out), _stream(stream)),
                "bilu_mlp_infer")
         return out
+
+    def solve(self, pat: Pattern, A, b, x, o: Opts, iters=None, conv=None, iter_res=None, true_res=None,
+              refine=None, factor_status=None, stream=None):
+        """bilu_mlp_solve: infer + FP32-IR ParILU(0) solve in one call, the same
+        results as infer() followed by solve_ex(F_init=...).  Returns
+        (iters, conv, iter_res, true_res, refine, factor_status)."""
+        import torch
+        B = b.shape[0]
+        dev = b.device
+        mk = lambda t, dt: t if t is not None else torch.empty(B, dtype=dt, device=dev)
+        iters, conv = mk(iters, torch.float32), mk(conv, torch.int32)
+        iter_res, true_res = mk(iter_res, torch.float64), mk(true_res, torch.float64)
+        refine = mk(refine, torch.int32)
+        factor_status = None if factor_status is False else mk(factor_status, torch.int32)
+        _check(lib().bilu_mlp_solve(self.h, pat.h, B, _ptr(A), _ptr(b), _ptr(x), C.byref(o), _ptr(iters),
+                                    _ptr(conv), _ptr(iter_res), _ptr(true_res), _ptr(refine),
+                                    _ptr(factor_status), _stream(stream)), "bilu_mlp_solve")
+        return iters, conv, iter_res, true_res, refine, factor_status
 
 
 class Uneven:

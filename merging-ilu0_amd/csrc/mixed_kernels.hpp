@@ -68,7 +68,7 @@ __global__ __launch_bounds__(KF_MAX_WAVES * WAVE) void k_factor_fast(
     FastParams fp, int n, int nnz, int batch, const double *__restrict__ gA,
     const double *__restrict__ gFinit, TO *__restrict__ gF, int mode, int sweeps, double stop_tol,
     int *__restrict__ g_status, int isai, float *__restrict__ gM, int *__restrict__ gZ,
-    const int *__restrict__ sysidx)
+    const int *__restrict__ sysidx, const float *__restrict__ gY, int P3, const int *__restrict__ gpiv)
 {
     // job j factors system sysidx[j] (identity when sysidx is null); outputs are per job
     extern __shared__ __align__(16) unsigned char smem[];
@@ -95,7 +95,40 @@ __global__ __launch_bounds__(KF_MAX_WAVES * WAVE) void k_factor_fast(
                 for (int e = lane; e < nnz; e += WAVE) anorm2 = fma(F[e], F[e], anorm2);
                 anorm2 = wave_sum(anorm2);
             }
-            if (gFinit) {
+            if (gY) {
+                // the warm start from the network's raw outputs y (row sys of gY):
+                // guess = SIGMA sinh(YS (base + y)) for the entries with work, the
+                // same operations as bilu_mlp_infer; computed from A (in F) into W
+                // first, then picked, so every base reads an unmodified a_jj
+                const float *Yp = gY + (size_t)sys * P3;
+                for (int i0 = lane; i0 < fp.ne; i0 += 8 * WAVE) {
+                    int pos[8], pv[8];
+                    float y[8];
+#pragma unroll
+                    for (int j = 0; j < 8; j++) pos[j] = (i0 + j * WAVE < fp.ne) ? fp.etab[i0 + j * WAVE].x : -1;
+#pragma unroll
+                    for (int j = 0; j < 8; j++) {
+                        pv[j] = pos[j] >= 0 ? gpiv[pos[j]] : -1;
+                        y[j] = pos[j] >= 0 ? Yp[pos[j]] : 0.0f;
+                    }
+#pragma unroll
+                    for (int j = 0; j < 8; j++)
+                        if (pos[j] >= 0) {
+                            const bool lower = pv[j] >= 0;
+                            W[pos[j]] = bilu_mlp::guess(
+                                bilu_mlp::base_scaled((float)F[pos[j]], lower ? (float)F[pv[j]] : 1.0f, lower), y[j]);
+                        }
+                }
+                wave_sync();
+                for (int i0 = lane; i0 < fp.ne; i0 += 8 * WAVE) {
+                    int pos[8];
+#pragma unroll
+                    for (int j = 0; j < 8; j++) pos[j] = (i0 + j * WAVE < fp.ne) ? fp.etab[i0 + j * WAVE].x : -1;
+#pragma unroll
+                    for (int j = 0; j < 8; j++)
+                        if (pos[j] >= 0) F[pos[j]] = W[pos[j]];
+                }
+            } else if (gFinit) {
                 // entries with work start from the guess; the others are exact
                 // (= a).  The guess row comes in through W with coalesced loads.
                 const double *Ip = gFinit + (size_t)(sysidx ? sysidx[sys] : sys) * nnz;

@@ -32,6 +32,7 @@
 #include <vector>
 
 #include "../../include/bilu.h"
+#include "mlp_scaling.hpp"
 
 namespace {
 
@@ -39,10 +40,10 @@ typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
 typedef __attribute__((__vector_size__(4 * sizeof(float)))) float floatx4;
 
 constexpr int TM = 128, TN = 128, TK = 64, KP = TK + 8;   // LDS row pitch (bf16), padded
-constexpr double SIGMA = 1e-8;
-constexpr float YS = 24.0f;
-constexpr float INV_SIGMA_F = 1e8f;          // 1 / SIGMA, exact in FP32
-constexpr float INV_YS = 1.0f / YS;
+using bilu_mlp::SIGMA;
+using bilu_mlp::YS;
+using bilu_mlp::INV_SIGMA_F;
+using bilu_mlp::INV_YS;
 
 enum Epi { EPI_F32 = 0, EPI_BIAS_GELU = 1, EPI_BIAS = 2, EPI_GELU_BWD = 3 };
 
@@ -193,9 +194,7 @@ __global__ __launch_bounds__(256) void k_features(const double *__restrict__ A, 
 __device__ __forceinline__ float base_guess(const double *__restrict__ As, const int *__restrict__ piv, int e)
 {
     const int pv = piv[e];
-    const float a = (float)As[e];
-    const float sv = pv >= 0 ? a / (float)As[pv] : a;
-    return asinhf(sv * INV_SIGMA_F) * INV_YS;
+    return bilu_mlp::base_scaled((float)As[e], pv >= 0 ? (float)As[pv] : 1.0f, pv >= 0);
 }
 
 // loss gradient dY = 2 (y - (t - base)) / (batch nnz) (BF16), t from the exact
@@ -300,10 +299,8 @@ __global__ __launch_bounds__(256) void k_outputs(const float *__restrict__ Y, co
         for (int j = 0; j < U; j++) {
             const int e = e0 + j * 256;
             if (e < nnz) {
-                const float a = sa[e];
-                const float sv = pv[j] >= 0 ? a / sa[pv[j]] : a;
-                const float yb = asinhf(sv * INV_SIGMA_F) * INV_YS + y[j];
-                Fs[e] = SIGMA * (double)sinhf(YS * yb);
+                const bool lower = pv[j] >= 0;
+                Fs[e] = bilu_mlp::guess(bilu_mlp::base_scaled(sa[e], lower ? sa[pv[j]] : 1.0f, lower), y[j]);
             }
         }
     }
@@ -362,6 +359,7 @@ struct bilu_mlp_s {
     __bf16 *X = nullptr, *H1 = nullptr, *H2 = nullptr, *dY = nullptr, *dZ2 = nullptr, *dZ1 = nullptr;
     float *Z1 = nullptr, *Z2 = nullptr, *Y = nullptr, *part = nullptr, *cpart = nullptr;
     int *piv = nullptr;   // [nnz]: position of a_jj for lower entries (i, j), else -1
+    const void *pattern = nullptr;   // the pattern handle the network was created on
 };
 
 namespace {
@@ -490,6 +488,25 @@ void copy_params(const bilu_mlp_s &q, const float *src, float *dst, bool to_pack
 
 }  // namespace
 
+int bilu_mlp_forward_rows() { return 65536; }
+
+int bilu_mlp_forward_raw(bilu_mlp_s *q, const void *pattern, int batch, const double *d_A, hipStream_t st,
+                         const float **Y, int *P3, const int **piv)
+{
+    if (!q || q->pattern != pattern || batch <= 0 || batch > bilu_mlp_forward_rows() || !d_A)
+        return BILU_ERR_INVALID_ARG;
+    const int rows = pad_to(batch, TM);
+    const int rc = reserve(*q, rows);
+    if (rc != BILU_OK) return rc;
+    hipLaunchKernelGGL(k_features, dim3(rows), dim3(256), 0, st, d_A, q->nnz, q->P0, batch, q->X);
+    forward(*q, rows, st);
+    MLP_TRY(hipGetLastError());
+    *Y = q->Y;
+    *P3 = q->P3;
+    *piv = q->piv;
+    return BILU_OK;
+}
+
 extern "C" {
 
 int bilu_mlp_create(bilu_pattern_t p, int hidden, unsigned long long seed, bilu_mlp_t *out)
@@ -521,6 +538,7 @@ int bilu_mlp_create(bilu_pattern_t p, int hidden, unsigned long long seed, bilu_
             if (colof[x] < rowof[x]) piv[x] = dpos[colof[x]];
     }
     q->nnz = info.nnz;
+    q->pattern = p;
     q->P0 = pad_to(info.nnz, TN);
     q->P3 = pad_to(info.nnz, TN);
     q->H = hidden;
@@ -643,7 +661,7 @@ int bilu_mlp_infer(bilu_mlp_t q, int batch, const double *d_A, double *d_F_init,
     if (batch == 0) return BILU_OK;
     hipStream_t st = (hipStream_t)stream;
     // in chunks, so the activations stay bounded
-    const int chunk = 65536;
+    const int chunk = bilu_mlp_forward_rows();
     int rc = reserve(*q, std::min(pad_to(batch, TM), chunk));
     if (rc != BILU_OK) return rc;
     for (int s0 = 0; s0 < batch; s0 += chunk) {

@@ -32,6 +32,7 @@
 #include <vector>
 
 #include "../../include/bilu.h"
+#include "mlp_scaling.hpp"
 #include "pattern.hpp"
 
 namespace {
@@ -1155,7 +1156,8 @@ template <typename TO>
 int launch_factor_fast(bilu_pattern_s &h, int batch, const double *A, const double *F_init, TO *F,
                        int mode, int sweeps, double stop_tol, int *status, hipStream_t st,
                        int isai = 0, float *M = nullptr, int *Z = nullptr, const int *sysidx = nullptr,
-                       int max_waves = KF_MAX_WAVES)
+                       int max_waves = KF_MAX_WAVES, const float *Y = nullptr, int P3 = 0,
+                       const int *piv = nullptr)
 {
     const auto &pl = h.pa.fast;
     if (!pl.ok) return BILU_ERR_TOO_LARGE;
@@ -1176,7 +1178,7 @@ int launch_factor_fast(bilu_pattern_s &h, int batch, const double *A, const doub
     if (lds > 65536) HIP_TRY(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, st, fp.counter);
     hipLaunchKernelGGL(kern, dim3(grid), dim3(waves * WAVE), lds, st, fp, h.pa.n, nnz, batch, A, F_init, F,
-                       mode, sweeps, stop_tol, status, isai, M, Z, sysidx);
+                       mode, sweeps, stop_tol, status, isai, M, Z, sysidx, Y, P3, piv);
     HIP_TRY(hipGetLastError());
     return BILU_OK;
 }
@@ -1571,6 +1573,53 @@ int bilu_solve_ex(bilu_pattern_t h, int batch, const double *d_A, const double *
         HIP_TRY(hipGetLastError());
     }
     return rc;
+}
+
+int bilu_mlp_solve(bilu_mlp_t m, bilu_pattern_t h, int batch, const double *d_A, const double *d_b, double *d_x,
+                   const bilu_opts *opts, float *d_iter, int *d_conv, double *d_iter_res, double *d_true_res,
+                   int *d_refine, int *d_factor_status, void *stream)
+{
+    if (!m || !h || batch < 0 || (batch > 0 && (!d_A || !d_b || !d_x || !d_iter || !d_conv)))
+        return BILU_ERR_INVALID_ARG;
+    if (batch == 0) return BILU_OK;
+    bilu_opts o;
+    if (opts) o = *opts; else bilu_opts_default(&o);
+    const bool isai = o.precond == BILU_PRECOND_PARILU0_ISAI;
+    if ((o.precond != BILU_PRECOND_PARILU0 && !isai) || o.precision != BILU_PRECISION_FP32_IR ||
+        o.parilu_sweeps < 0 || !(o.tol >= 0.0) || o.max_iter < 0 || !(o.parilu_stop_tol >= 0.0) ||
+        !(o.inner_tol > 0.0 && o.inner_tol < 1.0) || o.max_refine < 1 || o.isai_power != 1)
+        return BILU_ERR_INVALID_ARG;
+    if (!h->pa.fast.ok) return BILU_ERR_TOO_LARGE;
+    const int n = h->pa.n, nnz = h->pa.nnz, nr = (n + WAVE - 1) / WAVE;
+    if (slot_template_mixed(h->pa.fast.S) < 0 || nr > MAX_NR || h->pa.fast.nr != nr) return BILU_ERR_TOO_LARGE;
+    hipStream_t st = (hipStream_t)stream;
+    HIP_TRY(hipSetDevice(h->device));
+    const int chunk = std::min(batch, bilu_mlp_forward_rows());
+    const size_t fbytes = ((size_t)chunk * nnz * sizeof(float) + 255) / 256 * 256;
+    void *ws = nullptr;
+    int rc = workspace(*h, fbytes + (size_t)chunk * sizeof(int), &ws);
+    if (rc != BILU_OK) return rc;
+    float *F = static_cast<float *>(ws);
+    int *Z = reinterpret_cast<int *>(static_cast<char *>(ws) + fbytes);
+    MixedParams mp{o.tol, o.tol_mode, (float)o.inner_tol, o.max_iter, o.max_refine};
+    for (int s0 = 0; s0 < batch; s0 += chunk) {
+        const int cb = std::min(chunk, batch - s0);
+        const double *A = d_A + (size_t)s0 * nnz;
+        const float *Y = nullptr;
+        const int *piv = nullptr;
+        int P3 = 0;
+        rc = bilu_mlp_forward_raw(m, h, cb, A, st, &Y, &P3, &piv);
+        if (rc != BILU_OK) return rc;
+        rc = launch_factor_fast<float>(*h, cb, A, nullptr, F, FS_PARILU, o.parilu_sweeps, o.parilu_stop_tol,
+                                       d_factor_status ? d_factor_status + s0 : nullptr, st, isai ? 1 : 0, F, Z,
+                                       nullptr, KF_MAX_WAVES, Y, P3, piv);
+        if (rc != BILU_OK) return rc;
+        rc = launch_mixed(*h, cb, A, d_b + (size_t)s0 * n, d_x + (size_t)s0 * n, F, Z, isai ? PCK_ISAI : PCK_LU, mp,
+                          d_iter + s0, d_conv + s0, d_iter_res ? d_iter_res + s0 : nullptr,
+                          d_true_res ? d_true_res + s0 : nullptr, d_refine ? d_refine + s0 : nullptr, st);
+        if (rc != BILU_OK) return rc;
+    }
+    return BILU_OK;
 }
 
 int bilu_solve(bilu_pattern_t h, int batch, const double *d_A, const double *d_b, double *d_x,

@@ -128,3 +128,66 @@ def test_mlp_training_gain():
     ea = np.abs(MO.targets(At) - MO.targets(Fe)).mean()
     print(f"mean scaled error: trained guess {eg:.4f}, base guess {eb:.4f}, A {ea:.4f}", flush=True)
     assert eg < ea
+
+
+def _fused_vs_split(P, net, dA, db, precond, sweeps=40, stop_tol=1e-8):
+    """bilu_mlp_solve against bilu_mlp_infer + bilu_solve_ex(F_init): every
+    output bit-equal (the fused warm start runs the same FP32 operations)."""
+    o = bilu.opts(precond=precond, parilu_sweeps=sweeps, parilu_stop_tol=stop_tol, tol=1e-10,
+                  tol_mode=bilu.TOL_RELATIVE, precision=bilu.PRECISION_FP32_IR)
+    x1, x2 = torch.empty_like(db), torch.empty_like(db)
+    r1 = net.solve(P, dA, db, x1, o)
+    r2 = bilu.solve_ex(P, dA, db, x2, o, F_init=net.infer(dA))
+    torch.cuda.synchronize()
+    assert torch.equal(x1, x2)
+    for name, a, b in zip(("iters", "conv", "iter_res", "true_res", "refine", "factor_status"), r1, r2):
+        assert torch.equal(a, b), name
+    return r1
+
+
+@pytest.mark.parametrize("precond", ["PARILU0", "PARILU0_ISAI"])
+def test_mlp_solve_fused_matches_split(precond):
+    print(f"case mlp fused solve {precond}", flush=True)
+    pat, A, b = synth.batch("random", 768, first=3 << 20, n=144)
+    P = bilu.Pattern.from_synth(pat)
+    dA, db = to_dev(A), to_dev(b)
+    net = bilu.Mlp(P, hidden=256, seed=5)
+    # a few steps so the guess is not the untrained one
+    dF = exact_factors(P, dA)
+    for step in range(20):
+        net.train_step(dA[:512], dF[:512], 3e-4)
+    it, cv, _, tr, _, fs = _fused_vs_split(P, net, dA, db, getattr(bilu, precond))
+    assert (cv.cpu().numpy() == 1).all()
+    assert (tr.cpu().numpy() / np.linalg.norm(b, axis=1)).max() <= 1e-10
+
+
+def test_mlp_solve_fused_chunks():
+    """more systems than one network pass (65536): the fused solve walks the
+    batch in chunks; a small pattern keeps the case cheap"""
+    print("case mlp fused solve chunks", flush=True)
+    pat = synth.stencil5_pattern(4, 4)
+    B = 65536 + 37
+    A = synth.stencil5_values(pat, np.arange(B))
+    b = synth.rhs(pat.n, np.arange(B))
+    P = bilu.Pattern.from_synth(pat)
+    dA, db = to_dev(A), to_dev(b)
+    net = bilu.Mlp(P, hidden=128, seed=3)
+    _fused_vs_split(P, net, dA, db, bilu.PARILU0, sweeps=12)
+
+
+def test_mlp_solve_rejects():
+    pat, A, b = synth.batch("random", 4, n=144)
+    P = bilu.Pattern.from_synth(pat)
+    dA, db = to_dev(A), to_dev(b)
+    net = bilu.Mlp(P, hidden=128, seed=1)
+    x = torch.empty_like(db)
+    for kw in (dict(precond=bilu.ILU0, precision=bilu.PRECISION_FP32_IR),
+               dict(precond=bilu.PARILU0, precision=bilu.PRECISION_FP64)):
+        with pytest.raises(bilu.BiluError):
+            net.solve(P, dA, db, x, bilu.opts(**kw))
+    pat2 = synth.stencil5_pattern(4, 4)
+    P2 = bilu.Pattern.from_synth(pat2)
+    A2 = synth.stencil5_values(pat2, np.arange(4))
+    with pytest.raises(bilu.BiluError):
+        net.solve(P2, to_dev(A2), to_dev(synth.rhs(pat2.n, np.arange(4))), torch.empty(4, pat2.n, dtype=torch.float64,
+                  device=dev()), bilu.opts(precond=bilu.PARILU0, precision=bilu.PRECISION_FP32_IR))
