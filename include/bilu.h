@@ -74,6 +74,13 @@ extern "C" {
 #define BILU_PRECISION_FP32_IR 1  /* FP32 preconditioner and inner BiCGSTAB inside FP64
                                      iterative refinement (outer residual and its dot in FP64) */
 
+/* exact ILU(0) schedules (the same arithmetic and bits, different parallel orders) */
+#define BILU_ILU_SCHED_AUTO 0     /* the pattern's tuned schedule (bilu_ilu0_tune), else ROWS */
+#define BILU_ILU_SCHED_ROWS 1     /* rows level by level, a lane per row (approach 1 order) */
+#define BILU_ILU_SCHED_ENTRIES 2  /* entries level by level from dependency lists (approach 3) */
+#define BILU_ILU_SCHED_ERAS 3     /* right-looking pivot eras (approach 4) */
+#define BILU_ILU_SCHED_STREAMS 4  /* persistent kernel, list-scheduled lane streams */
+
 #define BILU_ISAI_GENERAL 0
 #define BILU_ISAI_LOWER 1     /* strictly-lower values + unit diagonal */
 #define BILU_ISAI_UPPER 2     /* diagonal + strictly-upper values */
@@ -110,6 +117,8 @@ typedef struct {
     int levels_lower, levels_upper;
     int ilu_updates;           /* number of (i,k,j) ILU(0) updates */
     int max_row_nnz;
+    int levels_entries;        /* levels of the entry (dependency-list) ILU(0) schedule */
+    int ilu_eras;              /* eras of the right-looking pivot ILU(0) schedule */
 } bilu_pattern_info;
 
 /* defaults: ILU(0), 5 sweeps, tol 1e-11 absolute, 500 iterations, merged, no restart,
@@ -172,6 +181,19 @@ int bilu_solve_ex(bilu_pattern_t p, int batch, const double *d_A, const double *
  * (may be NULL) receives 0 or BILU_ZERO_PIVOT. */
 int bilu_ilu0(bilu_pattern_t p, int batch, const double *d_A, double *d_F,
               int *d_status, void *stream);
+/* bilu_ilu0 with a chosen schedule (BILU_ILU_SCHED_*): ILU_0_gpu.cu's
+ * approach switch (:1241-1343; approaches 1/3/4 at :105-186, :693-935,
+ * :947-1229), all of them deterministic here.  BILU_ERR_TOO_LARGE when the
+ * pattern does not fit the schedule's kernel. */
+int bilu_ilu0_ex(bilu_pattern_t p, int batch, const double *d_A, double *d_F, int schedule, int *d_status,
+                 void *stream);
+/* times every schedule the pattern supports on this batch (one warm-up and
+ * two timed runs each, the stream is synchronized), keeps the fastest for
+ * BILU_ILU_SCHED_AUTO and returns it in *best (may be NULL); ms[4] (may be
+ * NULL) gets the time per run of ROWS, ENTRIES, ERAS, STREAMS (-1: not
+ * supported).  d_F holds the factors of d_A on return. */
+int bilu_ilu0_tune(bilu_pattern_t p, int batch, const double *d_A, double *d_F, float *ms, int *best,
+                   void *stream);
 /* ParILU(0): `sweeps` Jacobi sweeps from d_F_init (NULL: from A) */
 int bilu_parilu0(bilu_pattern_t p, int batch, const double *d_A, const double *d_F_init,
                  double *d_F, int sweeps, void *stream);

@@ -24,6 +24,7 @@ MERGED, SEGREGATED = 0, 1
 ISAI_GENERAL, ISAI_LOWER, ISAI_UPPER = 0, 1, 2
 KERNEL_AUTO, KERNEL_WAVE, KERNEL_PERSISTENT = 0, 1, 2
 PRECISION_FP64, PRECISION_FP32_IR = 0, 1
+ILU_SCHED_AUTO, ILU_SCHED_ROWS, ILU_SCHED_ENTRIES, ILU_SCHED_ERAS, ILU_SCHED_STREAMS = 0, 1, 2, 3, 4
 
 # every symbol include/bilu.h declares
 EXPORTS = [
@@ -34,7 +35,7 @@ EXPORTS = [
     "bilu_mlp_get_weights", "bilu_mlp_set_weights", "bilu_mlp_gradient", "bilu_mlp_train_step",
     "bilu_mlp_infer", "bilu_uneven_create", "bilu_uneven_destroy", "bilu_uneven_tune", "bilu_uneven_config",
     "bilu_uneven_solve", "bilu_pattern_add_diagonal", "bilu_expand_values", "bilu_isai_pattern",
-    "bilu_isai_power", "bilu_mlp_solve",
+    "bilu_isai_power", "bilu_mlp_solve", "bilu_ilu0_ex", "bilu_ilu0_tune",
 ]
 
 
@@ -63,7 +64,8 @@ class Opts(C.Structure):
 
 class PatternInfo(C.Structure):
     _fields_ = [(k, C.c_int) for k in ("n", "nnz", "l_nnz", "u_nnz", "levels_lower",
-                                        "levels_upper", "ilu_updates", "max_row_nnz")]
+                                        "levels_upper", "ilu_updates", "max_row_nnz",
+                                        "levels_entries", "ilu_eras")]
 
 
 _lib = None
@@ -113,6 +115,8 @@ def lib():
         L.bilu_mlp_gradient.argtypes = [vp, C.c_int, vp, vp, vp, vp, vp]
         L.bilu_mlp_train_step.argtypes = [vp, C.c_int, vp, vp, C.c_float, vp, vp]
         L.bilu_mlp_infer.argtypes = [vp, C.c_int, vp, vp, vp]
+        L.bilu_ilu0_ex.argtypes = [vp, C.c_int, vp, vp, C.c_int, vp, vp]
+        L.bilu_ilu0_tune.argtypes = [vp, C.c_int, vp, vp, C.POINTER(C.c_float), C.POINTER(C.c_int), vp]
         L.bilu_mlp_solve.argtypes = [vp, vp, C.c_int, vp, vp, vp, C.POINTER(Opts), vp, vp, vp, vp, vp, vp, vp]
         _lib = L
     return _lib
@@ -260,9 +264,25 @@ def solve_ex(pat: Pattern, A, b, x, o: Opts | None = None, F_init=None, iters=No
     return iters, conv, iter_res, true_res, refine, factor_status
 
 
-def ilu0(pat: Pattern, A, F, status=None, stream=None):
-    _check(lib().bilu_ilu0(pat.h, A.shape[0], _ptr(A), _ptr(F), _ptr(status), _stream(stream)),
-           "bilu_ilu0")
+def ilu0(pat: Pattern, A, F, status=None, stream=None, schedule=None):
+    """exact ILU(0) in place layout; schedule = ILU_SCHED_* (None: bilu_ilu0,
+    the pattern's tuned schedule)"""
+    if schedule is None:
+        _check(lib().bilu_ilu0(pat.h, A.shape[0], _ptr(A), _ptr(F), _ptr(status), _stream(stream)),
+               "bilu_ilu0")
+    else:
+        _check(lib().bilu_ilu0_ex(pat.h, A.shape[0], _ptr(A), _ptr(F), schedule, _ptr(status),
+                                  _stream(stream)), "bilu_ilu0_ex")
+
+
+def ilu0_tune(pat: Pattern, A, F, stream=None):
+    """bilu_ilu0_tune: returns (best schedule, [ms per run of ROWS, ENTRIES,
+    ERAS, STREAMS], -1 = not supported)"""
+    ms = (C.c_float * 4)()
+    best = C.c_int()
+    _check(lib().bilu_ilu0_tune(pat.h, A.shape[0], _ptr(A), _ptr(F), ms, C.byref(best), _stream(stream)),
+           "bilu_ilu0_tune")
+    return best.value, list(ms)
 
 
 def parilu0(pat: Pattern, A, F, sweeps, F_init=None, stream=None):

@@ -6,7 +6,9 @@ class PagedCSRMatrices;
 // exact ILU(0) of every page; L (explicit unit diagonal) and U in the
 // reference split layout (factorization.cu:216-344), device arrays
 // allocated here (host pattern arrays too).  `approach_num` (1-4) selects
-// among the reference's equivalent algorithms and is not needed here.  A
+// the schedule of the same arithmetic (identical factors): 1 rows level by
+// level, 2 and 3 entries by dependency lists, 4 right-looking pivot eras;
+// other values throw std::invalid_argument.  A
 // pattern without a full diagonal gets explicit zero diagonal entries first
 // (ILU_0_gpu.cu:1262-1272); L and U are then split from that pattern.
 void ILU_0_Factorization_Gpu(const PagedCSRMatrices &A_pages, PagedCSRMatrices &L_pages, PagedCSRMatrices &U_pages,

@@ -19,8 +19,9 @@
 //          float64 values[pages*nnz], b[pages*n]
 // out.bin (solvers): float64 x[pages*n]; float32 iter[pages]; int32 conv[pages];
 //          float64 iter_res[pages], true_res[pages]; float64 PST, IST, OET
-// out.bin (ilu0 / parilu0, ILU_0_Factorization_Gpu / ParILU_0_Factorization_Gpu
-//          with 5 sweeps): int32 l_nnz, u_nnz; int32 L row_ptrs[n+1], col_ind;
+// out.bin (ilu0 / ilu0_app1..4 / parilu0, ILU_0_Factorization_Gpu with that
+//          approach / ParILU_0_Factorization_Gpu with 5 sweeps): int32 l_nnz,
+//          u_nnz; int32 L row_ptrs[n+1], col_ind;
 //          U row_ptrs, col_ind; float64 L values[pages*l_nnz], U values
 // out.bin (isai_*, GenerateISAI_gpu power 1 of the input's typed view):
 //          float64 values[pages*nnz]
@@ -119,7 +120,10 @@ int main(int argc, char **argv)
     A.CopyFromCpuToGpu();
     b.AllocateMemory(LOCATION::GPU);
     b.CopyFromCpuToGpu();
-    if (variant == "ilu0" || variant == "parilu0" || variant.rfind("isai_", 0) == 0 ||
+    // ilu0_app<k>: ILU_0_Factorization_Gpu with approach k (1..4; "ilu0" = 1)
+    const bool ilu_app = variant.size() == 9 && variant.rfind("ilu0_app", 0) == 0 && variant[8] >= '1' &&
+                         variant[8] <= '4';
+    if (variant == "ilu0" || ilu_app || variant == "parilu0" || variant.rfind("isai_", 0) == 0 ||
         variant.rfind("isai2_", 0) == 0) {
         FILE *o = out_path ? std::fopen(out_path, "wb") : nullptr;
         if (!o) { std::perror("out.bin"); return 2; }
@@ -140,6 +144,7 @@ int main(int argc, char **argv)
             } else {
                 PagedCSRMatrices L, U;
                 if (variant == "ilu0") ILU_0_Factorization_Gpu(A, L, U, 1);
+                else if (ilu_app) ILU_0_Factorization_Gpu(A, L, U, variant[8] - '0');
                 else ParILU_0_Factorization_Gpu(A, L, U, 5);
                 L.CopyFromGpuToCpu();
                 U.CopyFromGpuToCpu();

@@ -258,6 +258,19 @@ double Batched_Preconditioned_BiCGSTAB_Gpu(const PagedCSRMatrices &A_pages, cons
     return ms;
 }
 
+// the reference's approach number (ILU_0_gpu.cu:1280-1317) as a schedule of
+// the same arithmetic: 1 rows, 2 and 3 dependency lists, 4 pivot eras
+static int ilu_schedule(int approach_num)
+{
+    switch (approach_num) {
+    case 1: return BILU_ILU_SCHED_ROWS;
+    case 2:
+    case 3: return BILU_ILU_SCHED_ENTRIES;
+    case 4: return BILU_ILU_SCHED_ERAS;
+    default: throw std::invalid_argument("exact ILU(0): approach must be 1..4");
+    }
+}
+
 void Batched_ILU_Preconditioned_BiCGSTAB_Gpu(const std::vector<std::string> &subdir, const PagedCSRMatrices &A_pages,
                                              const PagedVectors &b_pages, PagedVectors &x_pages, const bool is_scaled,
                                              SolverResults &solver_results, const bool is_parilu, const int approach,
@@ -274,7 +287,8 @@ void Batched_ILU_Preconditioned_BiCGSTAB_Gpu(const std::vector<std::string> &sub
     if (is_parilu)
         check(bilu_parilu0(P.h, B, A.GetPtrToGpuValues(), nullptr, F.p, parilu_sweeps, nullptr), "bilu_parilu0");
     else
-        check(bilu_ilu0(P.h, B, A.GetPtrToGpuValues(), F.p, nullptr, nullptr), "bilu_ilu0");
+        check(bilu_ilu0_ex(P.h, B, A.GetPtrToGpuValues(), F.p, ilu_schedule(approach), nullptr, nullptr),
+              "bilu_ilu0_ex");
     PST = t.stop();
     bilu_opts o;
     bilu_opts_default(&o);
@@ -346,13 +360,14 @@ void Batched_conv_ILU_app1_Preconditioned_BiCGSTAB_segregated_Gpu(const std::vec
 }
 
 void ILU_0_Factorization_Gpu(const PagedCSRMatrices &A_pages, PagedCSRMatrices &L_pages, PagedCSRMatrices &U_pages,
-                             const int)
+                             const int approach_num)
 {
+    const int sched = ilu_schedule(approach_num);
     const WithDiagonal D(A_pages);
     const PagedCSRMatrices &A = D.get(A_pages);
     PatternHandle P(A);
     DeviceBuffer<> F((size_t)A.GetNumPages() * A.GetNumNz());
-    check(bilu_ilu0(P.h, A.GetNumPages(), A.GetPtrToGpuValues(), F.p, nullptr, nullptr), "bilu_ilu0");
+    check(bilu_ilu0_ex(P.h, A.GetNumPages(), A.GetPtrToGpuValues(), F.p, sched, nullptr, nullptr), "bilu_ilu0_ex");
     split_into(P, A, F.p, L_pages, U_pages);
 }
 

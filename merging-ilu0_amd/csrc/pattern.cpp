@@ -211,6 +211,7 @@ int analyse_pattern(int n, int nnz, const int *rp, const int *ci, PatternAnalysi
             pa.dep_ptr[e + 1] = int(pa.dep.size() / 2);
         }
     }
+    build_ilu_schedules(pa);
     // reference L/U split (factorization.cu:216-344)
     pa.l_rp.assign(n + 1, 0);
     pa.u_rp.assign(n + 1, 0);
@@ -227,6 +228,104 @@ int analyse_pattern(int n, int nnz, const int *rp, const int *ci, PatternAnalysi
     return BILU_OK;
 }
 
+
+// Alternative schedules of the same exact ILU(0) arithmetic (each entry:
+// acc = a_ij, acc = fma(-l_ik, u_kj, acc) for k increasing, then acc / u_jj
+// below the diagonal), so every schedule gives the same bits:
+//  - entries: an entry's level is one above the levels of its operands (its
+//    pairs and, below the diagonal, u_jj); the dependency lists of
+//    ILU_0_gpu.cu:693-935 processed level by level instead of by ready flags;
+//  - eras (right-looking, ILU_0_gpu.cu:947-1229): pivot k joins the era after
+//    every pivot k' < k that writes row k or column k ((k,k') or (k',k) in the
+//    pattern), and no earlier era than the previous pivot of any entry it
+//    updates, so each entry sees its updates in increasing k.
+void build_ilu_schedules(PatternAnalysis &pa)
+{
+    const int n = pa.n, nnz = pa.nnz;
+    const auto &rp = pa.rp, &ci = pa.ci, &diag = pa.diag;
+    // entry levels
+    std::vector<int> elev(nnz, 0);
+    int maxlev = 0;
+    for (int i = 0; i < n; i++)
+        for (int e = rp[i]; e < rp[i + 1]; e++) {
+            int l = 0;
+            for (int d = pa.dep_ptr[e]; d < pa.dep_ptr[e + 1]; d++)
+                l = std::max(l, std::max(elev[pa.dep[2 * d]], elev[pa.dep[2 * d + 1]]) + 1);
+            if (ci[e] < i) l = std::max(l, elev[diag[ci[e]]] + 1);
+            elev[e] = l;
+            maxlev = std::max(maxlev, l);
+        }
+    pa.lev_e_ptr.assign(1, 0);
+    pa.lev_e_ent.clear();
+    for (int l = 1; l <= maxlev; l++) {
+        for (int e = 0; e < nnz; e++)
+            if (elev[e] == l) pa.lev_e_ent.push_back(e);
+        pa.lev_e_ptr.push_back(int(pa.lev_e_ent.size()));
+    }
+    // pivot eras
+    std::vector<std::vector<int>> upper_rows(n), prev_piv(n);
+    for (int i = 0; i < n; i++)
+        for (int p = diag[i] + 1; p < rp[i + 1]; p++) upper_rows[ci[p]].push_back(i);
+    for (int e = 0; e < nnz; e++) {
+        int prev = -1;
+        for (int d = pa.dep_ptr[e]; d < pa.dep_ptr[e + 1]; d++) {
+            const int k = ci[pa.dep[2 * d]];
+            if (prev >= 0) prev_piv[k].push_back(prev);
+            prev = k;
+        }
+    }
+    std::vector<int> era(n, 0);
+    int neras = 0;
+    for (int k = 0; k < n; k++) {
+        int r = 0;
+        for (int p = rp[k]; p < diag[k]; p++) r = std::max(r, era[ci[p]] + 1);
+        for (int kk : upper_rows[k]) r = std::max(r, era[kk] + 1);
+        for (int kk : prev_piv[k]) r = std::max(r, era[kk]);
+        era[k] = r;
+        neras = std::max(neras, r + 1);
+    }
+    if (n == 0) neras = 0;
+    // per era: divisions (column k below the diagonal, pivots increasing)
+    std::vector<std::vector<int>> col_lower(n);
+    for (int i = 0; i < n; i++)
+        for (int p = rp[i]; p < diag[i]; p++) col_lower[ci[p]].push_back(p);
+    std::vector<std::vector<int>> era_piv(neras);
+    for (int k = 0; k < n; k++) era_piv[era[k]].push_back(k);
+    pa.era_div_ptr.assign(1, 0);
+    pa.era_div.clear();
+    for (int r = 0; r < neras; r++) {
+        for (int k : era_piv[r])
+            for (int p : col_lower[k]) { pa.era_div.push_back(p); pa.era_div.push_back(diag[k]); }
+        pa.era_div_ptr.push_back(int(pa.era_div.size() / 2));
+    }
+    // per era: targets with their pairs (a target's pivots have non-decreasing
+    // eras, so each era's pairs are one run of its dependency list)
+    std::vector<std::vector<std::pair<int, int>>> runs(neras);   // (target, first dep) per era
+    for (int e = 0; e < nnz; e++) {
+        int d = pa.dep_ptr[e];
+        while (d < pa.dep_ptr[e + 1]) {
+            const int r = era[ci[pa.dep[2 * d]]];
+            runs[r].push_back({e, d});
+            while (d < pa.dep_ptr[e + 1] && era[ci[pa.dep[2 * d]]] == r) d++;
+        }
+    }
+    pa.era_tgt_ptr.assign(1, 0);
+    pa.era_tgt.clear();
+    pa.tgt_pair_ptr.assign(1, 0);
+    pa.tgt_pair.clear();
+    for (int r = 0; r < neras; r++) {
+        for (const auto &td : runs[r]) {
+            const int e = td.first;
+            pa.era_tgt.push_back(e);
+            for (int d = td.second; d < pa.dep_ptr[e + 1] && era[ci[pa.dep[2 * d]]] == r; d++) {
+                pa.tgt_pair.push_back(pa.dep[2 * d]);
+                pa.tgt_pair.push_back(pa.dep[2 * d + 1]);
+            }
+            pa.tgt_pair_ptr.push_back(int(pa.tgt_pair.size() / 2));
+        }
+        pa.era_tgt_ptr.push_back(int(pa.era_tgt.size()));
+    }
+}
 
 void build_fast_plan(PatternAnalysis &pa)
 {

@@ -104,6 +104,16 @@ struct PatternAnalysis {
     // pairs (pos(i,k), pos(k,j)), k < min(i,j) increasing
     std::vector<int> dep_ptr;   // nnz+1
     std::vector<int> dep;       // 2 ints per pair
+    // exact ILU(0) by entries (the dependency-list schedule): the positions
+    // with work (a product or a division), grouped by dependency level
+    std::vector<int> lev_e_ptr, lev_e_ent;
+    // exact ILU(0) by pivot eras (right-looking): era r divides the lower
+    // entries (i,k) of its pivots k by u_kk (pairs pos, pos(k,k)), then
+    // applies its pivots' updates; each updated entry (target) carries its
+    // pairs (pos(i,k), pos(k,j)) of the era, k increasing
+    std::vector<int> era_div_ptr, era_div;    // eras+1; 2 ints per division
+    std::vector<int> era_tgt_ptr, era_tgt;    // eras+1; target positions
+    std::vector<int> tgt_pair_ptr, tgt_pair;  // targets+1; 2 ints per pair
     // reference L/U split (L keeps an explicit unit diagonal)
     std::vector<int> l_rp, l_ci, l_src;   // l_src: pattern position or -1 (unit diagonal)
     std::vector<int> u_rp, u_ci, u_src;
@@ -113,6 +123,7 @@ struct PatternAnalysis {
 };
 
 void build_fast_plan(PatternAnalysis &pa);
+void build_ilu_schedules(PatternAnalysis &pa);
 
 // ISAI with a chosen sparsity: the pattern of |P|^power for the typed view P
 // (LOWER: columns < i plus the unit diagonal, UPPER: columns >= i, GENERAL:

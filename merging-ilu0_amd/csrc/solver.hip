@@ -42,7 +42,7 @@ constexpr int MAX_NR = 5;        // rows per lane in the register layout: n <= 3
 constexpr int ISAI_GROUP = 32;   // lanes per ISAI row (reference row_size_limit 32)
 
 enum PcKind { PC_LU = 0, PC_ISAI_LU = 1, PC_ISAI_GEN = 2, PC_NONE = 3, PC_ISAI_POW = 4, PC_ISAI_GEN_POW = 5 };
-enum FactorKind { FK_ILU = 0, FK_PARILU = 1, FK_LOAD = 2, FK_NONE = 3 };
+enum FactorKind { FK_ILU = 0, FK_PARILU = 1, FK_LOAD = 2, FK_NONE = 3, FK_ILU_ENTRIES = 4, FK_ILU_ERAS = 5 };
 
 struct DevIsai {
     const int *beg, *m, *goff, *gather, *rhs;
@@ -66,6 +66,11 @@ struct DevPattern {
     const int *l_src, *u_src;
     int l_nnz, u_nnz;
     DevIsai isai[3];
+    // alternative exact ILU(0) schedules: entry levels, pivot eras
+    int nlev_e, n_eras;
+    const int *lev_e_ptr, *lev_e_ent;
+    const int *era_div_ptr, *era_tgt_ptr, *era_tgt, *tgt_pair_ptr;
+    const int2 *era_div, *tgt_pair;
 };
 
 struct SolveParams {
@@ -350,6 +355,82 @@ __device__ int ilu_factor(const DevPattern &P, double *F, int lane)
     }
     for (int i = lane; i < P.n; i += WAVE) bad |= (F[P.diag[i]] == 0.0);
     return __any(bad);
+}
+
+// exact ILU(0) in place by entries, level by level (the dependency-list
+// schedule, ILU_0_gpu.cu:693-935, with level barriers instead of ready
+// flags): lane per entry, acc = a_ij, acc = fma(-l_ik, u_kj, acc) for k
+// increasing, then / u_jj below the diagonal -- the bits of ilu_factor
+__device__ int ilu_factor_entries(const DevPattern &P, double *F, int lane)
+{
+    int bad = 0;
+    for (int l = 0; l < P.nlev_e; l++) {
+        const int b = P.lev_e_ptr[l], e = P.lev_e_ptr[l + 1];
+        for (int idx = b + lane; idx < e; idx += WAVE) {
+            const int pos = P.lev_e_ent[idx];
+            double acc = F[pos];
+            const int de = P.dep_ptr[pos + 1];
+            for (int d = P.dep_ptr[pos]; d < de; d++) {
+                const int2 pq = P.dep[d];
+                acc = fma(-F[pq.x], F[pq.y], acc);
+            }
+            const int j = P.ci[pos];
+            if (j < P.erow[pos]) {
+                const double ukk = F[P.diag[j]];
+                bad |= (ukk == 0.0);
+                acc = acc / ukk;
+            }
+            F[pos] = acc;
+        }
+        __syncthreads();
+    }
+    for (int i = lane; i < P.n; i += WAVE) bad |= (F[P.diag[i]] == 0.0);
+    return __any(bad);
+}
+
+// exact ILU(0) in place by pivot eras (right-looking, ILU_0_gpu.cu:947-1229
+// with the era's pivots spread over the lanes): divide the eras' columns,
+// then every updated entry takes its era's updates in increasing k
+__device__ int ilu_factor_eras(const DevPattern &P, double *F, int lane)
+{
+    int bad = 0;
+    for (int r = 0; r < P.n_eras; r++) {
+        const int db = P.era_div_ptr[r], de = P.era_div_ptr[r + 1];
+        for (int idx = db + lane; idx < de; idx += WAVE) {
+            const int2 pd = P.era_div[idx];
+            const double ukk = F[pd.y];
+            bad |= (ukk == 0.0);
+            F[pd.x] = F[pd.x] / ukk;
+        }
+        __syncthreads();
+        const int tb = P.era_tgt_ptr[r], te = P.era_tgt_ptr[r + 1];
+        for (int idx = tb + lane; idx < te; idx += WAVE) {
+            const int t = P.era_tgt[idx];
+            double acc = F[t];
+            const int pe = P.tgt_pair_ptr[idx + 1];
+            for (int d = P.tgt_pair_ptr[idx]; d < pe; d++) {
+                const int2 pq = P.tgt_pair[d];
+                acc = fma(-F[pq.x], F[pq.y], acc);
+            }
+            F[t] = acc;
+        }
+        __syncthreads();
+    }
+    for (int i = lane; i < P.n; i += WAVE) bad |= (F[P.diag[i]] == 0.0);
+    return __any(bad);
+}
+
+// zero-pivot status of in-place factors: one wave per system
+__global__ __launch_bounds__(WAVE) void k_diag_status(DevPattern P, int batch, const double *__restrict__ gF,
+                                                     int *__restrict__ g_status)
+{
+    const int sys = blockIdx.x;
+    if (sys >= batch) return;
+    const double *Fp = gF + (size_t)sys * P.nnz;
+    int bad = 0;
+    for (int i = threadIdx.x; i < P.n; i += WAVE) bad |= (Fp[P.diag[i]] == 0.0);
+    bad = __any(bad);
+    if (threadIdx.x == 0) g_status[sys] = bad ? BILU_ZERO_PIVOT : 0;
 }
 
 // Jacobi ParILU(0) sweeps (parILU_0_gpu.cu:18-57 formula); F holds the
@@ -714,6 +795,8 @@ __global__ __launch_bounds__(WAVE) void k_factor(DevPattern P, int batch, const 
     __syncthreads();
     int bad = 0;
     if (kind == FK_ILU) bad = ilu_factor(P, sF, lane);
+    else if (kind == FK_ILU_ENTRIES) bad = ilu_factor_entries(P, sF, lane);
+    else if (kind == FK_ILU_ERAS) bad = ilu_factor_eras(P, sF, lane);
     else parilu_sweeps(P, sA, sF, sW, sweeps, lane);
     __syncthreads();
     double *Fp = gF + (size_t)sys * nnz;
@@ -878,6 +961,7 @@ struct bilu_pattern_s {
     uint32_t *d_ilu0 = nullptr, *d_ilu1 = nullptr;
     uint32_t *d_parp = nullptr, *d_parf = nullptr;
     int *d_counter = nullptr;      // 256 bytes of work counters: [0] solve kernels, [32] factor kernel
+    int ilu_sched = 0;             // exact ILU(0) schedule chosen by bilu_ilu0_tune (0: none yet)
     // solve workspace (factors between the factor and solve launches), grown on demand
     void *ws = nullptr;
     size_t ws_bytes = 0;
@@ -922,6 +1006,10 @@ int upload(bilu_pattern_s &h)
     size_t o_upp = put(pa.upd_ptr), o_upd = put(pa.upd.empty() ? std::vector<int>{0, 0} : pa.upd);
     size_t o_dpp = put(pa.dep_ptr), o_dep = put(pa.dep.empty() ? std::vector<int>{0, 0} : pa.dep);
     size_t o_ls = put(pa.l_src), o_us = put(pa.u_src);
+    auto nz = [](const std::vector<int> &v) { return v.empty() ? std::vector<int>{0, 0} : v; };
+    size_t o_lep = put(pa.lev_e_ptr), o_lee = put(nz(pa.lev_e_ent)), o_edp = put(pa.era_div_ptr),
+           o_ed = put(nz(pa.era_div)), o_etp = put(pa.era_tgt_ptr), o_et = put(nz(pa.era_tgt)),
+           o_tpp = put(pa.tgt_pair_ptr), o_tp = put(nz(pa.tgt_pair));
     size_t o_is[3][5];
     for (int t = 0; t < 3; t++) {
         const auto &M = pa.isai[t];
@@ -941,6 +1029,12 @@ int upload(bilu_pattern_s &h)
     d.dep_ptr = B + o_dpp; d.dep = reinterpret_cast<const int2 *>(B + o_dep);
     d.l_src = B + o_ls; d.u_src = B + o_us;
     d.l_nnz = int(pa.l_ci.size()); d.u_nnz = int(pa.u_ci.size());
+    d.nlev_e = int(pa.lev_e_ptr.size()) - 1;
+    d.n_eras = int(pa.era_div_ptr.size()) - 1;
+    d.lev_e_ptr = B + o_lep; d.lev_e_ent = B + o_lee;
+    d.era_div_ptr = B + o_edp; d.era_div = reinterpret_cast<const int2 *>(B + o_ed);
+    d.era_tgt_ptr = B + o_etp; d.era_tgt = B + o_et;
+    d.tgt_pair_ptr = B + o_tpp; d.tgt_pair = reinterpret_cast<const int2 *>(B + o_tp);
     for (int t = 0; t < 3; t++) {
         d.isai[t].beg = B + o_is[t][0]; d.isai[t].m = B + o_is[t][1]; d.isai[t].goff = B + o_is[t][2];
         d.isai[t].gather = B + o_is[t][3]; d.isai[t].rhs = B + o_is[t][4];
@@ -1371,6 +1465,8 @@ int bilu_pattern_get_info(bilu_pattern_t p, bilu_pattern_info *info)
     info->levels_upper = p->dev.nlev_u;
     info->ilu_updates = int(p->pa.upd.size() / 2);
     info->max_row_nnz = p->pa.max_row;
+    info->levels_entries = p->dev.nlev_e;
+    info->ilu_eras = p->dev.n_eras;
     return BILU_OK;
 }
 
@@ -1630,17 +1726,75 @@ int bilu_solve(bilu_pattern_t h, int batch, const double *d_A, const double *d_b
                          nullptr, nullptr, stream);
 }
 
-int bilu_ilu0(bilu_pattern_t h, int batch, const double *d_A, double *d_F, int *d_status, void *stream)
+int bilu_ilu0_ex(bilu_pattern_t h, int batch, const double *d_A, double *d_F, int schedule, int *d_status,
+                 void *stream)
 {
-    if (!h || batch < 0 || (batch > 0 && (!d_A || !d_F))) return BILU_ERR_INVALID_ARG;
+    if (!h || batch < 0 || (batch > 0 && (!d_A || !d_F)) || schedule < BILU_ILU_SCHED_AUTO ||
+        schedule > BILU_ILU_SCHED_STREAMS)
+        return BILU_ERR_INVALID_ARG;
+    if (schedule == BILU_ILU_SCHED_AUTO) schedule = h->ilu_sched ? h->ilu_sched : BILU_ILU_SCHED_ROWS;
     if (batch == 0) return BILU_OK;
+    HIP_TRY(hipSetDevice(h->device));
+    hipStream_t st = (hipStream_t)stream;
+    if (schedule == BILU_ILU_SCHED_STREAMS) {
+        if (!h->pa.fast.ok) return BILU_ERR_TOO_LARGE;
+        const int rc = launch_factor_fast<double>(*h, batch, d_A, nullptr, d_F, FS_ILU, 0, 0.0, nullptr, st);
+        if (rc != BILU_OK) return rc;
+        if (d_status) hipLaunchKernelGGL(k_diag_status, dim3(batch), dim3(WAVE), 0, st, h->dev, batch, d_F, d_status);
+        HIP_TRY(hipGetLastError());
+        return BILU_OK;
+    }
     const size_t lds = 2 * (size_t)h->pa.nnz * sizeof(double);
     if (lds > h->max_lds) return BILU_ERR_TOO_LARGE;
-    HIP_TRY(hipSetDevice(h->device));
     if (lds > 65536) HIP_TRY(hipFuncSetAttribute((const void *)k_factor, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL(k_factor, dim3(batch), dim3(WAVE), lds, (hipStream_t)stream, h->dev, batch, d_A,
-                       (const double *)nullptr, d_F, (int)FK_ILU, 0, d_status);
+    const int kind = schedule == BILU_ILU_SCHED_ENTRIES ? FK_ILU_ENTRIES
+                     : schedule == BILU_ILU_SCHED_ERAS  ? FK_ILU_ERAS
+                                                        : FK_ILU;
+    hipLaunchKernelGGL(k_factor, dim3(batch), dim3(WAVE), lds, st, h->dev, batch, d_A,
+                       (const double *)nullptr, d_F, kind, 0, d_status);
     HIP_TRY(hipGetLastError());
+    return BILU_OK;
+}
+
+int bilu_ilu0(bilu_pattern_t h, int batch, const double *d_A, double *d_F, int *d_status, void *stream)
+{
+    return bilu_ilu0_ex(h, batch, d_A, d_F, BILU_ILU_SCHED_AUTO, d_status, stream);
+}
+
+int bilu_ilu0_tune(bilu_pattern_t h, int batch, const double *d_A, double *d_F, float *ms, int *best,
+                   void *stream)
+{
+    if (!h || batch <= 0 || !d_A || !d_F) return BILU_ERR_INVALID_ARG;
+    HIP_TRY(hipSetDevice(h->device));
+    hipStream_t st = (hipStream_t)stream;
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    int choice = 0;
+    float tbest = 0.0f;
+    int rc = BILU_OK;
+    for (int s = BILU_ILU_SCHED_ROWS; s <= BILU_ILU_SCHED_STREAMS && rc == BILU_OK; s++) {
+        float t = -1.0f;
+        int r = bilu_ilu0_ex(h, batch, d_A, d_F, s, nullptr, stream);   // warm-up; also the support check
+        if (r == BILU_OK) {
+            if (hipEventRecord(e0, st) != hipSuccess) { rc = BILU_ERR_HIP; break; }
+            for (int k = 0; k < 2 && r == BILU_OK; k++) r = bilu_ilu0_ex(h, batch, d_A, d_F, s, nullptr, stream);
+            if (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                hipEventElapsedTime(&t, e0, e1) != hipSuccess) { rc = BILU_ERR_HIP; break; }
+            t *= 0.5f;
+            if (r != BILU_OK) rc = r;
+            else if (!choice || t < tbest) { choice = s; tbest = t; }
+        } else if (r != BILU_ERR_TOO_LARGE) {
+            rc = r;
+        }
+        if (ms) ms[s - 1] = r == BILU_OK ? t : -1.0f;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc != BILU_OK) return rc;
+    if (!choice) return BILU_ERR_TOO_LARGE;
+    h->ilu_sched = choice;
+    if (best) *best = choice;
     return BILU_OK;
 }
 
