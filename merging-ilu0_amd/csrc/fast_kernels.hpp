@@ -241,9 +241,6 @@ __device__ __forceinline__ int ilu_stream(const uint32_t *__restrict__ r0p, cons
 #ifndef PAR_AHEAD_ROUNDS
 #define PAR_AHEAD_ROUNDS 2
 #endif
-#ifndef PAR_PRE_LDS
-#define PAR_PRE_LDS 0
-#endif
 constexpr int PAR_AHEAD = PAR_AHEAD_ROUNDS;
 template <bool RES>
 __device__ __forceinline__ double parilu_rounds(const uint32_t *__restrict__ pp, const uint32_t *__restrict__ pf,
@@ -261,17 +258,6 @@ __device__ __forceinline__ double parilu_rounds(const uint32_t *__restrict__ pp,
     for (int k = 0; k <= D; k++) fq[k] = frec(k);
 #pragma unroll
     for (int k = 0; k < D; k++) aq[k] = Ap[(fq[k] & 0xFFF8u) >> 3];
-#if PAR_PRE_LDS
-    // the finish operands (old value, pivot) D rounds ahead: src is read-only here
-    double oq[D], dq[D];
-    auto pre = [&](uint32_t f, double &o, double &d) {
-        const uint32_t pos = f & 0xFFF8u, pv = f >> 16;
-        o = ld(pos);
-        d = ld(pv != 0xFFFFu ? pv : pos);
-    };
-#pragma unroll
-    for (int k = 0; k < D; k++) pre(fq[k], oq[k], dq[k]);
-#endif
     int r = 0;
     uint32_t ra[K];
 #pragma unroll
@@ -293,11 +279,7 @@ __device__ __forceinline__ double parilu_rounds(const uint32_t *__restrict__ pp,
             if (__builtin_amdgcn_readfirstlane(rec) & 1u) {   // the round's last step (all lanes)
                 const uint32_t f = fq[0], pos = f & 0xFFF8u, pv = f >> 16;
                 const bool valid = (f & 1u) != 0, div = pv != 0xFFFFu;
-#if PAR_PRE_LDS
-                const double old = oq[0], dv = dq[0];
-#else
                 const double old = ld(pos), dv = ld(div ? pv : pos);
-#endif
                 const double t = aq[0] - acc;
                 const double val = div ? t / dv : t;
                 if (RES) {
@@ -313,11 +295,6 @@ __device__ __forceinline__ double parilu_rounds(const uint32_t *__restrict__ pp,
 #pragma unroll
                 for (int k = 0; k + 1 < D; k++) aq[k] = aq[k + 1];
                 aq[D - 1] = Ap[(fq[D - 1] & 0xFFF8u) >> 3];
-#if PAR_PRE_LDS
-#pragma unroll
-                for (int k = 0; k + 1 < D; k++) { oq[k] = oq[k + 1]; dq[k] = dq[k + 1]; }
-                pre(fq[D - 1], oq[D - 1], dq[D - 1]);
-#endif
             }
         }
 #pragma unroll
