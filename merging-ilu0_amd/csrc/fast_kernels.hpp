@@ -74,8 +74,8 @@ __global__ void k_zero_counter(int *c) { *c = 0; }
 // (8*col | 8*row << 16, high half 0xFFFF when the slot does not end a row)
 // is read from the workgroup's shared LDS copy, padded to SM slots.  The
 // body is straight-line: every slot stores its running sum, to the row's
-// cell when the slot ends a row and to a scratch cell otherwise, so the
-// loads of all slots can be in flight together.
+// cell when the slot ends a row and to a scratch cell otherwise.
+constexpr int SPMV_G = 3;   // slots per gather group (best of 1..8 measured at n = 144)
 template <int SM, typename T = double>
 __device__ __forceinline__ void spmv_slots(const T (&a)[SM], const uint32_t *__restrict__ meta,
                                            const T *__restrict__ x, T *__restrict__ y,
@@ -86,12 +86,27 @@ __device__ __forceinline__ void spmv_slots(const T (&a)[SM], const uint32_t *__r
     const char *xb = reinterpret_cast<const char *>(x);
     char *yb = reinterpret_cast<char *>(y);
     T acc = T(0);
+    // the meta records and x values of the next SPMV_G slots are read before
+    // this group's row stores (x and y never overlap), so the LDS latency of
+    // the gathers overlaps the fma chain instead of following each store
+    constexpr int G = SPMV_G;
+    uint32_t m[SM];
+    T xv[SM];
+    auto fetch = [&](int s) {
+        m[s] = meta[s * WAVE + lane];
+        xv[s] = *reinterpret_cast<const T *>(xb + ((m[s] & 0xFFFFu) >> SH));
+    };
+#pragma unroll
+    for (int s = 0; s < G && s < SM; s++) fetch(s);
 #pragma unroll
     for (int s = 0; s < SM; s++) {
-        const uint32_t m = meta[s * WAVE + lane];
-        acc = fmaT(a[s], *reinterpret_cast<const T *>(xb + ((m & 0xFFFFu) >> SH)), acc);
-        const bool end = (m >> 16) != 0xFFFFu;
-        T *dst = end ? reinterpret_cast<T *>(yb + ((m >> 16) >> SH)) : scratch;
+        if (s % G == 0) {
+#pragma unroll
+            for (int t = s + G; t < s + 2 * G && t < SM; t++) fetch(t);
+        }
+        acc = fmaT(a[s], xv[s], acc);
+        const bool end = (m[s] >> 16) != 0xFFFFu;
+        T *dst = end ? reinterpret_cast<T *>(yb + ((m[s] >> 16) >> SH)) : scratch;
         *dst = acc;
         acc = end ? T(0) : acc;
     }
