@@ -98,7 +98,7 @@ __global__ __launch_bounds__(KF_MAX_WAVES * WAVE) void k_factor_fast(
             if (gY) {
                 // the warm start from the network's raw outputs y (row sys of gY):
                 // guess = SIGMA sinh(YS (base + y)) for the entries with work, the
-                // same operations as bilu_mlp_infer; computed from A (in F) into W
+                // same operations as bilu_mlp_infer (guess_from); from A (in F) into W
                 // first, then picked, so every base reads an unmodified a_jj
                 const float *Yp = gY + (size_t)sys * P3;
                 for (int i0 = lane; i0 < fp.ne; i0 += 8 * WAVE) {
@@ -115,8 +115,8 @@ __global__ __launch_bounds__(KF_MAX_WAVES * WAVE) void k_factor_fast(
                     for (int j = 0; j < 8; j++)
                         if (pos[j] >= 0) {
                             const bool lower = pv[j] >= 0;
-                            W[pos[j]] = bilu_mlp::guess(
-                                bilu_mlp::base_scaled((float)F[pos[j]], lower ? (float)F[pv[j]] : 1.0f, lower), y[j]);
+                            W[pos[j]] = bilu_mlp::guess_from((float)F[pos[j]], lower ? (float)F[pv[j]] : 1.0f,
+                                                             lower, y[j]);
                         }
                 }
                 wave_sync();
@@ -265,15 +265,31 @@ __device__ __forceinline__ void spmv_coef(const float (&c)[SM], const float *__r
     const char *xb = reinterpret_cast<const char *>(x);
     char *yb = reinterpret_cast<char *>(y);
     float acc = 0.0f;
+    // next group's records, coefficients and x values read before this
+    // group's row stores (x and y never overlap), as in spmv_slots
+    constexpr int G = SPMV_G;
+    uint32_t mr[SM];
+    float xv[SM], cw[SM];
+    auto fetch = [&](int s) {
+        mr[s] = meta[s * WAVE + lane];
+        if (CM == CM_LDS) cw[s] = cl[s * WAVE + lane];
+        xv[s] = *reinterpret_cast<const float *>(xb + ((mr[s] & 0xFFF8u) >> 1));
+    };
+#pragma unroll
+    for (int s = 0; s < G && s < SM; s++) fetch(s);
 #pragma unroll
     for (int s = 0; s < SM; s++) {
-        const uint32_t mm = meta[s * WAVE + lane];
+        if (s % G == 0) {
+#pragma unroll
+            for (int t = s + G; t < s + 2 * G && t < SM; t++) fetch(t);
+        }
+        const uint32_t mm = mr[s];
         const uint32_t kd = mm & 3u;
         float cs;
-        if (CM == CM_LDS) cs = cl[s * WAVE + lane];
+        if (CM == CM_LDS) cs = cw[s];
         else if (CM == CM_ISAI_L) cs = kd == 1u ? c[s] : (kd == 2u ? 1.0f : 0.0f);
         else cs = kd == 1u ? 0.0f : c[s];
-        acc = __builtin_fmaf(cs, *reinterpret_cast<const float *>(xb + ((mm & 0xFFF8u) >> 1)), acc);
+        acc = __builtin_fmaf(cs, xv[s], acc);
         const bool end = (mm >> 16) != 0xFFFFu;
         float *dst = end ? reinterpret_cast<float *>(yb + ((mm >> 16) >> 1)) : scratch;
         *dst = acc;

@@ -300,7 +300,7 @@ __global__ __launch_bounds__(256) void k_outputs(const float *__restrict__ Y, co
             const int e = e0 + j * 256;
             if (e < nnz) {
                 const bool lower = pv[j] >= 0;
-                Fs[e] = bilu_mlp::guess(bilu_mlp::base_scaled(sa[e], lower ? sa[pv[j]] : 1.0f, lower), y[j]);
+                Fs[e] = bilu_mlp::guess_from(sa[e], lower ? sa[pv[j]] : 1.0f, lower, y[j]);
             }
         }
     }

@@ -3,7 +3,8 @@
 // the guess with the same operations:
 //   features / targets: x = asinh(v / SIGMA) / YS
 //   base guess: s = a_ij on / above the diagonal, a_ij / a_jj below it
-//   guess: F = SIGMA sinh(YS (base + y)), y the network output
+//   guess: F = SIGMA sinh(YS (base + y)), y the network output (guess_from,
+//   by an identity that needs no asinh / sinh)
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -22,9 +23,22 @@ __device__ __forceinline__ float base_scaled(float a, float ajj, bool lower)
     return asinhf(sv * INV_SIGMA_F) * INV_YS;
 }
 
-__device__ __forceinline__ double guess(float base, float y)
+// the same guess from a_ij, a_jj and y without the asinh / sinh pair:
+// SIGMA sinh(asinh(u) + b) with u = s / SIGMA, b = YS y equals
+// SIGMA (q - 1/q) / 2 for q = t e^b (u >= 0) or e^b / t (u < 0), where
+// t = |u| + sqrt(1 + u^2) = e^asinh|u| is formed without cancellation
+// (2|u| once u^2 would overflow); the hardware exp, sqrt and reciprocal
+// approximations suffice for a warm start.  Overflow and NaN behave as in
+// the asinh / sinh form.
+__device__ __forceinline__ double guess_from(float a, float ajj, bool lower, float y)
 {
-    return SIGMA * (double)sinhf(YS * (base + y));
+    const float sv = lower ? a / ajj : a;
+    const float u = sv * INV_SIGMA_F;
+    const float au = fabsf(u);
+    const float t = au < 1e18f ? au + __builtin_amdgcn_sqrtf(fmaf(au, au, 1.0f)) : 2.0f * au;
+    const float eb = __expf(YS * y);
+    const float q = u >= 0.0f ? t * eb : eb * __builtin_amdgcn_rcpf(t);
+    return SIGMA * (double)(0.5f * (q - __builtin_amdgcn_rcpf(q)));
 }
 
 }  // namespace bilu_mlp
