@@ -165,7 +165,8 @@ __global__ __launch_bounds__(256) void k_gemm(const __bf16 *__restrict__ A, int 
             }
 }
 
-// features: X[s][e] = asinh(a / SIGMA) / YS (BF16) in FP32 arithmetic; padding
+// features: X[s][e] = asinh(a / SIGMA) / YS (BF16) in FP32 arithmetic
+// (bilu_mlp::feature); padding
 // columns and rows s >= batch are 0.  One block per system row s (its reads
 // and writes stay in one XCD's L2).
 __global__ __launch_bounds__(256) void k_features(const double *__restrict__ A, int nnz, int P0, int batch,
@@ -183,7 +184,7 @@ __global__ __launch_bounds__(256) void k_features(const double *__restrict__ A, 
 #pragma unroll
         for (int j = 0; j < U; j++) {
             const int e = e0 + j * 256;
-            if (e < P0) Xs[e] = (__bf16)(e < lim ? asinhf((float)a[j] * INV_SIGMA_F) * INV_YS : 0.0f);
+            if (e < P0) Xs[e] = (__bf16)(e < lim ? bilu_mlp::feature((float)a[j]) : 0.0f);
         }
     }
 }

@@ -23,6 +23,18 @@ __device__ __forceinline__ float base_scaled(float a, float ajj, bool lower)
     return asinhf(sv * INV_SIGMA_F) * INV_YS;
 }
 
+// network input: asinh(u) / YS for u = a / SIGMA, as sign(u) ln(t) with
+// t = |u| + sqrt(1 + u^2) (2|u| once u^2 would overflow) and the hardware
+// log and sqrt: the value is rounded to BF16 next, far coarser than their
+// error
+__device__ __forceinline__ float feature(float a)
+{
+    const float u = a * INV_SIGMA_F;
+    const float au = fabsf(u);
+    const float t = au < 1e18f ? au + __builtin_amdgcn_sqrtf(fmaf(au, au, 1.0f)) : 2.0f * au;
+    return copysignf(__logf(t) * INV_YS, u);
+}
+
 // the same guess from a_ij, a_jj and y without the asinh / sinh pair:
 // SIGMA sinh(asinh(u) + b) with u = s / SIGMA, b = YS y equals
 // SIGMA (q - 1/q) / 2 for q = t e^b (u >= 0) or e^b / t (u < 0), where
