@@ -346,6 +346,17 @@ def main():
     # (3b) the same ParILU path from the standard start (no network), for the gain
     ms_nolearn = run(o_learn)
     nolearn = summary(ms_nolearn)
+    # (3c) the untrained network (zero output layer: the base guess a_ij, a_ij / a_jj),
+    # what the training adds or loses against its own starting point
+    net0 = bilu.Mlp(P, hidden=512, seed=1)
+    out["fs"].zero_()
+    out["rf"].zero_()
+
+    def untrained_step():
+        net0.solve(P, dA, db, x, o_learn, out["it"], out["cv"], out["ir"], out["tr"], out["rf"], out["fs"],
+                   stream=stream)
+    untrained = summary(time_steps(untrained_step, args.warmup, args.steps, stream))
+    net0.close()
     ms_infer = time_steps(lambda: net.infer(dA, guess, stream=stream), 1, args.steps, stream)
 
     achieved = alg_bytes / (ms_learn * 1e-3) / 1e9
@@ -361,7 +372,8 @@ def main():
         "speedup_vs_plain_fp64": round(ms_plain / ms_learn, 3),
         "variants": {"plain_fp64_ilu0_merged": plain, "mixed_ilu0_isai": mixed,
                      "learned_mixed_parilu0_isai": learned, "learned_mixed_parilu0_isai_split": split,
-                     "mixed_parilu0_isai_no_warm_start": nolearn},
+                     "mixed_parilu0_isai_no_warm_start": nolearn,
+                     "untrained_network_base_guess_mixed_parilu0_isai": untrained},
         "fused_x_equals_split_x": fused_equal,
         "mlp_inference_ms": round(ms_infer, 4),
         "training": {"systems": args.train_systems, "epochs": args.train_epochs, "minibatch": mb,
