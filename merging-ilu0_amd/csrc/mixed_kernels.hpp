@@ -23,6 +23,9 @@ constexpr int FS_ILU = 0, FS_PARILU = 1;
 // target's m_c were written by earlier steps of the same lane, and a wave's
 // LDS accesses execute in program order, so no synchronisation is needed
 // inside the stream.  Returns nonzero when a divisor (a U diagonal) is 0.
+// DIV = false for the ISAI(L) streams, whose records never divide (unit
+// diagonal): the division and the divisor's load are left out.
+template <bool DIV>
 __device__ __forceinline__ int isai_gen(const uint32_t *__restrict__ r0p, const uint32_t *__restrict__ r1p,
                                         int steps, const double *F, float *M, float *mscratch, int lane)
 {
@@ -49,10 +52,14 @@ __device__ __forceinline__ int isai_gen(const uint32_t *__restrict__ r0p, const 
             const double fv = *reinterpret_cast<const double *>(Fb + (a & 0xFFFFu));
             acc = fma(-mv, fv, acc);
             const uint32_t p = b & 0xFFFFu, d = b >> 16;
-            const bool fin = p != 0xFFFFu, div = d != 0xFFFFu;
-            const double dv = *reinterpret_cast<const double *>(Fb + (div ? d : 0u));
-            bad |= (fin && div && dv == 0.0);
-            const double val = div ? acc / dv : acc;
+            const bool fin = p != 0xFFFFu;
+            double val = acc;
+            if constexpr (DIV) {
+                const bool div = d != 0xFFFFu;
+                const double dv = *reinterpret_cast<const double *>(Fb + (div ? d : 0u));
+                bad |= (fin && div && dv == 0.0);
+                val = div ? acc / dv : acc;
+            }
             float *dst = fin ? reinterpret_cast<float *>(Mb + (p >> 1)) : mscratch;
             *dst = (float)val;
             acc = fin ? 0.0 : acc;
@@ -186,8 +193,8 @@ __global__ __launch_bounds__(KF_MAX_WAVES * WAVE) void k_factor_fast(
         if (isai) {
             if (lane == 0) { M[nnz] = 0.0f; M[nnz + 1] = 1.0f; M[nnz + 2] = -1.0f; }
             wave_sync();
-            int bad = isai_gen(fp.isL0, fp.isL1, fp.isL_steps, F, M, M + nnz + 3, lane);
-            bad |= isai_gen(fp.isU0, fp.isU1, fp.isU_steps, F, M, M + nnz + 3, lane);
+            int bad = isai_gen<false>(fp.isL0, fp.isL1, fp.isL_steps, F, M, M + nnz + 3, lane);
+            bad |= isai_gen<true>(fp.isU0, fp.isU1, fp.isU_steps, F, M, M + nnz + 3, lane);
             wave_sync();
             float *Mo = gM + (size_t)sys * nnz;
             wave_copy(Mo, M, nnz, lane);
