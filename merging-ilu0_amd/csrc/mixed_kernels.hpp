@@ -219,15 +219,14 @@ struct MixedParams {
 
 // y (LDS, FP64, by row) = A x with the FP64 values read from global memory
 // slot by slot; x in LDS (FP64).  Same fma chain as spmv_slots.
-// REC2: the two-word records of spmv32 (meta = x records, my = output
-// records; the scratch cell is y[n + 1]) instead of the one-word records.
-template <int SM, bool REC2 = false>
+// PACKED: the records of spmv32 (the scratch cell is y[n + 1]) instead of
+// spmv_slots' records.
+template <int SM, bool PACKED = false>
 __device__ __forceinline__ void spmv_slots_global(const double *__restrict__ Ap,
                                                   const int32_t *__restrict__ slot_pos, int S,
                                                   const uint32_t *__restrict__ meta,
                                                   const double *__restrict__ x, double *__restrict__ y,
-                                                  double *__restrict__ scratch, int lane,
-                                                  const uint32_t *__restrict__ my = nullptr)
+                                                  double *__restrict__ scratch, int lane)
 {
     // groups of G slots: the loads of one group are in flight together, the
     // compiler barrier keeps it from hoisting every group's loads (VGPRs)
@@ -248,12 +247,12 @@ __device__ __forceinline__ void spmv_slots_global(const double *__restrict__ Ap,
         for (int j = 0; j < G; j++) {
             const int s = g + j;
             if (s >= SM) break;
-            if constexpr (REC2) {
+            if constexpr (PACKED) {
                 // FP64 offsets are twice the FP32 ones of the records
-                const uint32_t rx = meta[s * WAVE + lane], ry = my[s * WAVE + lane];
-                acc = fma(av[j], *reinterpret_cast<const double *>(xb + 2u * rx), acc);
-                *reinterpret_cast<double *>(yb + 2u * (ry & 0xFFFFu)) = acc;
-                acc = (ry >> 24) ? acc : 0.0;
+                const uint32_t w = meta[s * WAVE + lane];
+                acc = fma(av[j], *reinterpret_cast<const double *>(xb + 2u * (w & 0xFFFFu)), acc);
+                *reinterpret_cast<double *>(yb + 2u * ((w >> 16) & 0x7FFCu)) = acc;
+                acc = (w >> 31) ? acc : 0.0;
             } else {
                 const uint32_t m = meta[s * WAVE + lane];
                 acc = fma(av[j], *reinterpret_cast<const double *>(xb + (m & 0xFFF8u)), acc);
@@ -273,29 +272,29 @@ __device__ __forceinline__ void spmv_slots_global(const double *__restrict__ Ap,
 // on it, 0 above; ISAI(U) takes c on and above the diagonal, 0 below
 constexpr int CM_LDS = 0, CM_ISAI_L = 1, CM_ISAI_U = 2;
 
-// y (LDS, by row) = C x over the slot layout with two records per slot, so
-// that no field needs a shift or mask beyond a byte / half-word select:
-// mx[s] = byte offset of x (FP32 cells), my[s] = byte offset of the output
-// cell (the row's cell when the slot ends a row, else the output vector's
-// scratch cell n + 1) | triangle kind << 16 (1 strictly lower, 2 diagonal,
-// 0 upper or padding) | 1 << 24 when the slot does not end its row.
+// y (LDS, by row) = C x over the slot layout with one packed record per
+// slot: bits 0-15 the byte offset of x (FP32 cells); bits 16-30 the byte
+// offset of the output cell (the row's cell when the slot ends a row, else
+// the output vector's scratch cell n + 1) with the triangle kind in its two
+// low bits (1 strictly lower, 2 diagonal, 0 upper or padding); bit 31 set
+// when the slot does not end its row.  One LDS word per slot instead of a
+// separate x and output record: the kernel is bound by LDS instructions.
 // The same CSR-order fma chain as spmv_slots.
 template <int SM, int CM>
 __device__ __forceinline__ void spmv32(const float (&c)[SM], const float *__restrict__ cl,
-                                       const uint32_t *__restrict__ mx, const uint32_t *__restrict__ my,
-                                       const float *__restrict__ x, float *__restrict__ y, int lane)
+                                       const uint32_t *__restrict__ meta, const float *__restrict__ x,
+                                       float *__restrict__ y, int lane)
 {
     const char *xb = reinterpret_cast<const char *>(x);
     char *yb = reinterpret_cast<char *>(y);
     float acc = 0.0f;
     constexpr int G = SPMV_G;
-    uint32_t rx[SM], ry[SM];
+    uint32_t rw[SM];
     float xv[SM], cw[SM];
     auto fetch = [&](int s) {
-        rx[s] = mx[s * WAVE + lane];
-        ry[s] = my[s * WAVE + lane];
+        rw[s] = meta[s * WAVE + lane];
         if (CM == CM_LDS) cw[s] = cl[s * WAVE + lane];
-        xv[s] = *reinterpret_cast<const float *>(xb + rx[s]);
+        xv[s] = *reinterpret_cast<const float *>(xb + (rw[s] & 0xFFFFu));
     };
 #pragma unroll
     for (int s = 0; s < G && s < SM; s++) fetch(s);
@@ -305,15 +304,15 @@ __device__ __forceinline__ void spmv32(const float (&c)[SM], const float *__rest
 #pragma unroll
             for (int t = s + G; t < s + 2 * G && t < SM; t++) fetch(t);
         }
-        const uint32_t r = ry[s];
-        const uint32_t kd = (r >> 16) & 0xFFu;
+        const uint32_t r = rw[s];
+        const uint32_t kd = (r >> 16) & 3u;
         float cs;
         if (CM == CM_LDS) cs = cw[s];
         else if (CM == CM_ISAI_L) cs = kd == 1u ? c[s] : (kd == 2u ? 1.0f : 0.0f);
         else cs = kd == 1u ? 0.0f : c[s];
         acc = __builtin_fmaf(cs, xv[s], acc);
-        *reinterpret_cast<float *>(yb + (r & 0xFFFFu)) = acc;
-        acc = (r >> 24) ? acc : 0.0f;
+        *reinterpret_cast<float *>(yb + ((r >> 16) & 0x7FFCu)) = acc;
+        acc = (r >> 31) ? acc : 0.0f;
     }
 }
 
@@ -347,12 +346,11 @@ __global__ __launch_bounds__(km_waves(SM, PCK) * WAVE) void k_mixed(
     const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
     const int n = P.n, nnz = P.nnz, S = fp.S;
     uint32_t *meta = reinterpret_cast<uint32_t *>(smem);
-    uint32_t *meta_y = meta + SM * WAVE;   // PCK_ISAI: the output records of spmv32
     const uint32_t *trL = fp.trL;
     const uint32_t *trU = fp.trU;
-    // LU: the one-word records of spmv_slots.  ISAI: the two-word records of
-    // spmv32, from the one-word ones (8 col | 8 row << 16, 0xFFFF: no row end)
-    // and the slot's triangle; padding slots read x[0] with coefficient 0
+    // LU: the records of spmv_slots.  ISAI: the packed records of spmv32,
+    // from those (8 col | 8 row << 16, 0xFFFF: no row end) and the slot's
+    // triangle; padding slots read x[0] with coefficient 0
     for (int i = threadIdx.x; i < SM * WAVE; i += blockDim.x) {
         const uint32_t m = (i < S * WAVE) ? fp.slot_meta[i] : 0xFFFF0000u;
         if constexpr (PCK == PCK_LU) {
@@ -360,8 +358,8 @@ __global__ __launch_bounds__(km_waves(SM, PCK) * WAVE) void k_mixed(
         } else {
             const uint32_t kd = (i < S * WAVE) ? (uint32_t)fp.slot_kind[i] : 0u;
             const bool end = (m >> 16) != 0xFFFFu;
-            meta[i] = (m & 0xFFFFu) >> 1;
-            meta_y[i] = (end ? (m >> 16) >> 1 : (uint32_t)(n + 1) * 4u) | (kd << 16) | (end ? 0u : 1u << 24);
+            const uint32_t yo = end ? (m >> 16) >> 1 : (uint32_t)(n + 1) * 4u;
+            meta[i] = ((m & 0xFFFFu) >> 1) | ((yo | kd) << 16) | (end ? 0u : 1u << 31);
         }
     }
     // per wave: F[nnz + 2] FP32 (F[nnz] = 0), then two vector regions of
@@ -465,16 +463,16 @@ __global__ __launch_bounds__(km_waves(SM, PCK) * WAVE) void k_mixed(
             } else {
                 to_lds(in, e0, lane, n);
                 wave_sync();
-                spmv32<SM, CM_ISAI_L>(cv, nullptr, meta, meta_y, e0, e1, lane);
+                spmv32<SM, CM_ISAI_L>(cv, nullptr, meta, e0, e1, lane);
                 wave_sync();
-                spmv32<SM, CM_ISAI_U>(cv, nullptr, meta, meta_y, e1, e0, lane);
+                spmv32<SM, CM_ISAI_U>(cv, nullptr, meta, e1, e0, lane);
                 wave_sync();
                 from_lds(out, e0, lane, n);
             }
         };
         auto matvec = [&](Vec<NR, float> &out) {   // out = A32 e0
             if constexpr (PCK == PCK_LU) spmv_slots<SM, float>(a, meta, e0, e1, scratch, lane);
-            else spmv32<SM, CM_LDS>(cv, Asl, meta, meta_y, e0, e1, lane);
+            else spmv32<SM, CM_LDS>(cv, Asl, meta, e0, e1, lane);
             wave_sync();
             from_lds(out, e1, lane, n);
             wave_sync();
@@ -546,7 +544,7 @@ __global__ __launch_bounds__(km_waves(SM, PCK) * WAVE) void k_mixed(
                 for (int k = 0; k < NR; k++) x.e[k] = fma((double)d.e[k], rn, x.e[k]);
                 to_lds(x, d0, lane, n);
                 wave_sync();
-                spmv_slots_global<SM, PCK == PCK_ISAI>(Ap, fp.slot_pos, S, meta, d0, d1, dscratch, lane, meta_y);
+                spmv_slots_global<SM, PCK == PCK_ISAI>(Ap, fp.slot_pos, S, meta, d0, d1, dscratch, lane);
                 wave_sync();
                 Vec<NR, double> w;
                 from_lds(w, d1, lane, n);

@@ -1324,7 +1324,7 @@ int launch_mixed(bilu_pattern_s &h, int batch, const double *A, const double *b,
         ? int(((size_t)(nnz + 2) * 4 + 15) / 16 * 16 + (size_t)2 * (n + 2) * 8 + (size_t)64 * nr * 8)
         : int((size_t)sm * WAVE * 4 + (size_t)2 * (n + 2) * 8);
     fp.wave_bytes = (fp.wave_bytes + 15) / 16 * 16;
-    fp.shared_bytes = sm * WAVE * 4 * (pck == PCK_ISAI ? 2 : 1);   // slot records (ISAI: two words)
+    fp.shared_bytes = sm * WAVE * 4;   // slot records
     if ((size_t)fp.shared_bytes >= h.max_lds) return BILU_ERR_TOO_LARGE;
     if (pck == PCK_ISAI && sm > 48) return BILU_ERR_TOO_LARGE;   // ISAI coefficients beyond the VGPR budget
     int waves = (int)std::min<size_t>(std::min(max_waves, km_waves(sm, pck)),
