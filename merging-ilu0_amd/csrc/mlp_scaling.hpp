@@ -40,11 +40,11 @@ __device__ __forceinline__ float feature(float a)
 // SIGMA (q - 1/q) / 2 for q = t e^b (u >= 0) or e^b / t (u < 0), where
 // t = |u| + sqrt(1 + u^2) = e^asinh|u| is formed without cancellation
 // (2|u| once u^2 would overflow); the hardware exp, sqrt and reciprocal
-// approximations suffice for a warm start.  Overflow and NaN behave as in
-// the asinh / sinh form.
+// approximations (also for a_ij / a_jj) suffice for a warm start.
+// Overflow and NaN behave as in the asinh / sinh form.
 __device__ __forceinline__ double guess_from(float a, float ajj, bool lower, float y)
 {
-    const float sv = lower ? a / ajj : a;
+    const float sv = lower ? a * __builtin_amdgcn_rcpf(ajj) : a;
     const float u = sv * INV_SIGMA_F;
     const float au = fabsf(u);
     const float t = au < 1e18f ? au + __builtin_amdgcn_sqrtf(fmaf(au, au, 1.0f)) : 2.0f * au;
