@@ -41,6 +41,7 @@ struct FastParams {
     int o_trL, o_trU;
     int wave_bytes;
     int par_w;      // 1: the wave region holds a second factor buffer (ParILU Jacobi)
+    int spos_lds;   // k_mixed: 1 when the slot positions follow the records in the shared LDS
     int *counter;
     // ISAI generation streams [step][64] and the slot triangles (ISAI applies)
     const uint32_t *isL0, *isL1, *isU0, *isU1;

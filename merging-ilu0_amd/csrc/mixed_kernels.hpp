@@ -351,6 +351,15 @@ __global__ __launch_bounds__(km_waves(SM, PCK) * WAVE) void k_mixed(
     // LU: the records of spmv_slots.  ISAI: the packed records of spmv32,
     // from those (8 col | 8 row << 16, 0xFFFF: no row end) and the slot's
     // triangle; padding slots read x[0] with coefficient 0
+    // the slot positions (the A gathers of each system's load and of every
+    // FP64 residual) follow the records in the shared LDS when the host found
+    // room for them without losing a wave; else they are read from global
+    // memory, a dependent load before every gather
+    // (two code paths, so that the LDS copy is read by LDS instructions)
+    int32_t *spos_l = reinterpret_cast<int32_t *>(meta + SM * WAVE);
+    const bool sp_lds = fp.spos_lds != 0;
+    if (sp_lds)
+        for (int i = threadIdx.x; i < SM * WAVE; i += blockDim.x) spos_l[i] = (i < S * WAVE) ? fp.slot_pos[i] : -1;
     for (int i = threadIdx.x; i < SM * WAVE; i += blockDim.x) {
         const uint32_t m = (i < S * WAVE) ? fp.slot_meta[i] : 0xFFFF0000u;
         if constexpr (PCK == PCK_LU) {
@@ -408,7 +417,7 @@ __global__ __launch_bounds__(km_waves(SM, PCK) * WAVE) void k_mixed(
             wave_copy(F, Fp, nnz, lane);
 #pragma unroll
         for (int q = 0; q < SM; q++) {
-            const int e = (q < S) ? fp.slot_pos[q * WAVE + lane] : -1;
+            const int e = (q < S) ? (sp_lds ? spos_l[q * WAVE + lane] : fp.slot_pos[q * WAVE + lane]) : -1;
             const float av = (e >= 0) ? (float)Ap[e] : 0.0f;
             if constexpr (PCK == PCK_LU) {
                 a[q] = av;
@@ -544,7 +553,8 @@ __global__ __launch_bounds__(km_waves(SM, PCK) * WAVE) void k_mixed(
                 for (int k = 0; k < NR; k++) x.e[k] = fma((double)d.e[k], rn, x.e[k]);
                 to_lds(x, d0, lane, n);
                 wave_sync();
-                spmv_slots_global<SM, PCK == PCK_ISAI>(Ap, fp.slot_pos, S, meta, d0, d1, dscratch, lane);
+                if (sp_lds) spmv_slots_global<SM, PCK == PCK_ISAI>(Ap, spos_l, S, meta, d0, d1, dscratch, lane);
+                else spmv_slots_global<SM, PCK == PCK_ISAI>(Ap, fp.slot_pos, S, meta, d0, d1, dscratch, lane);
                 wave_sync();
                 Vec<NR, double> w;
                 from_lds(w, d1, lane, n);

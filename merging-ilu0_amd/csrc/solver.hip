@@ -1324,11 +1324,17 @@ int launch_mixed(bilu_pattern_s &h, int batch, const double *A, const double *b,
         ? int(((size_t)(nnz + 2) * 4 + 15) / 16 * 16 + (size_t)2 * (n + 2) * 8 + (size_t)64 * nr * 8)
         : int((size_t)sm * WAVE * 4 + (size_t)2 * (n + 2) * 8);
     fp.wave_bytes = (fp.wave_bytes + 15) / 16 * 16;
-    fp.shared_bytes = sm * WAVE * 4;   // slot records
-    if ((size_t)fp.shared_bytes >= h.max_lds) return BILU_ERR_TOO_LARGE;
+    // shared LDS: the slot records, then the slot positions when they cost no wave
+    const size_t rec = (size_t)sm * WAVE * 4;
+    if (rec >= h.max_lds) return BILU_ERR_TOO_LARGE;
     if (pck == PCK_ISAI && sm > 48) return BILU_ERR_TOO_LARGE;   // ISAI coefficients beyond the VGPR budget
-    int waves = (int)std::min<size_t>(std::min(max_waves, km_waves(sm, pck)),
-                                      (h.max_lds - fp.shared_bytes) / fp.wave_bytes);
+    const int wcap = std::min(max_waves, km_waves(sm, pck));
+    auto fit = [&](size_t shared) {
+        return shared >= h.max_lds ? 0 : (int)std::min<size_t>(wcap, (h.max_lds - shared) / fp.wave_bytes);
+    };
+    fp.spos_lds = fit(2 * rec) >= fit(rec) ? 1 : 0;
+    fp.shared_bytes = int(rec * (fp.spos_lds ? 2 : 1));
+    int waves = fit(fp.shared_bytes);
     if (waves < 1) return BILU_ERR_TOO_LARGE;
     waves = std::min(waves, std::max(1, batch));
     const size_t lds = fp.shared_bytes + (size_t)waves * fp.wave_bytes;
