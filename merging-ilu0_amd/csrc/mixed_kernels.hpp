@@ -75,7 +75,7 @@ __global__ __launch_bounds__(KF_MAX_WAVES * WAVE) void k_factor_fast(
     FastParams fp, int n, int nnz, int batch, const double *__restrict__ gA,
     const double *__restrict__ gFinit, TO *__restrict__ gF, int mode, int sweeps, double stop_tol,
     int *__restrict__ g_status, int isai, float *__restrict__ gM, int *__restrict__ gZ,
-    const int *__restrict__ sysidx, const float *__restrict__ gY, int P3, const int *__restrict__ gpiv)
+    const int *__restrict__ sysidx, const float *__restrict__ gY, int P3)
 {
     // job j factors system sysidx[j] (identity when sysidx is null); outputs are per job
     extern __shared__ __align__(16) unsigned char smem[];
@@ -107,17 +107,21 @@ __global__ __launch_bounds__(KF_MAX_WAVES * WAVE) void k_factor_fast(
                 // guess = SIGMA sinh(YS (base + y)) for the entries with work, the
                 // same operations as bilu_mlp_infer (guess_from); from A (in F) into W
                 // first, then picked, so every base reads an unmodified a_jj
+                // (an entry record carries the position of a_jj for a lower entry,
+                // the base guess's pivot)
                 const float *Yp = gY + (size_t)sys * P3;
                 for (int i0 = lane; i0 < fp.ne; i0 += 8 * WAVE) {
                     int pos[8], pv[8];
                     float y[8];
 #pragma unroll
-                    for (int j = 0; j < 8; j++) pos[j] = (i0 + j * WAVE < fp.ne) ? fp.etab[i0 + j * WAVE].x : -1;
-#pragma unroll
                     for (int j = 0; j < 8; j++) {
-                        pv[j] = pos[j] >= 0 ? gpiv[pos[j]] : -1;
-                        y[j] = pos[j] >= 0 ? Yp[pos[j]] : 0.0f;
+                        const bool in = i0 + j * WAVE < fp.ne;
+                        const ushort4 r = in ? fp.etab[i0 + j * WAVE] : ushort4{0, 0, 0, 0xFFFF};
+                        pos[j] = in ? (int)r.x : -1;
+                        pv[j] = r.w != 0xFFFF ? (int)r.w : -1;
                     }
+#pragma unroll
+                    for (int j = 0; j < 8; j++) y[j] = pos[j] >= 0 ? Yp[pos[j]] : 0.0f;
 #pragma unroll
                     for (int j = 0; j < 8; j++)
                         if (pos[j] >= 0) {

@@ -492,7 +492,7 @@ void copy_params(const bilu_mlp_s &q, const float *src, float *dst, bool to_pack
 int bilu_mlp_forward_rows() { return 65536; }
 
 int bilu_mlp_forward_raw(bilu_mlp_s *q, const void *pattern, int batch, const double *d_A, hipStream_t st,
-                         const float **Y, int *P3, const int **piv)
+                         const float **Y, int *P3)
 {
     if (!q || q->pattern != pattern || batch <= 0 || batch > bilu_mlp_forward_rows() || !d_A)
         return BILU_ERR_INVALID_ARG;
@@ -504,7 +504,6 @@ int bilu_mlp_forward_raw(bilu_mlp_s *q, const void *pattern, int batch, const do
     MLP_TRY(hipGetLastError());
     *Y = q->Y;
     *P3 = q->P3;
-    *piv = q->piv;
     return BILU_OK;
 }
 

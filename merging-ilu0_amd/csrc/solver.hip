@@ -1250,8 +1250,7 @@ template <typename TO>
 int launch_factor_fast(bilu_pattern_s &h, int batch, const double *A, const double *F_init, TO *F,
                        int mode, int sweeps, double stop_tol, int *status, hipStream_t st,
                        int isai = 0, float *M = nullptr, int *Z = nullptr, const int *sysidx = nullptr,
-                       int max_waves = KF_MAX_WAVES, const float *Y = nullptr, int P3 = 0,
-                       const int *piv = nullptr)
+                       int max_waves = KF_MAX_WAVES, const float *Y = nullptr, int P3 = 0)
 {
     const auto &pl = h.pa.fast;
     if (!pl.ok) return BILU_ERR_TOO_LARGE;
@@ -1272,7 +1271,7 @@ int launch_factor_fast(bilu_pattern_s &h, int batch, const double *A, const doub
     if (lds > 65536) HIP_TRY(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, st, fp.counter);
     hipLaunchKernelGGL(kern, dim3(grid), dim3(waves * WAVE), lds, st, fp, h.pa.n, nnz, batch, A, F_init, F,
-                       mode, sweeps, stop_tol, status, isai, M, Z, sysidx, Y, P3, piv);
+                       mode, sweeps, stop_tol, status, isai, M, Z, sysidx, Y, P3);
     HIP_TRY(hipGetLastError());
     return BILU_OK;
 }
@@ -1708,13 +1707,12 @@ int bilu_mlp_solve(bilu_mlp_t m, bilu_pattern_t h, int batch, const double *d_A,
         const int cb = std::min(chunk, batch - s0);
         const double *A = d_A + (size_t)s0 * nnz;
         const float *Y = nullptr;
-        const int *piv = nullptr;
         int P3 = 0;
-        rc = bilu_mlp_forward_raw(m, h, cb, A, st, &Y, &P3, &piv);
+        rc = bilu_mlp_forward_raw(m, h, cb, A, st, &Y, &P3);
         if (rc != BILU_OK) return rc;
         rc = launch_factor_fast<float>(*h, cb, A, nullptr, F, FS_PARILU, o.parilu_sweeps, o.parilu_stop_tol,
                                        d_factor_status ? d_factor_status + s0 : nullptr, st, isai ? 1 : 0, F, Z,
-                                       nullptr, KF_MAX_WAVES, Y, P3, piv);
+                                       nullptr, KF_MAX_WAVES, Y, P3);
         if (rc != BILU_OK) return rc;
         rc = launch_mixed(*h, cb, A, d_b + (size_t)s0 * n, d_x + (size_t)s0 * n, F, Z, isai ? PCK_ISAI : PCK_LU, mp,
                           d_iter + s0, d_conv + s0, d_iter_res ? d_iter_res + s0 : nullptr,
