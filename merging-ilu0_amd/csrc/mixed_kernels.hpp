@@ -83,7 +83,18 @@ __global__ __launch_bounds__(KF_MAX_WAVES * WAVE) void k_factor_fast(
     // per wave: F[nnz + 4] FP64 (F[nnz] = 0, F[nnz + 1] = 1, F[nnz + 2] scratch),
     // then the ParILU buffer W[nnz + 2] FP64 and/or the ISAI M[nnz + 4] FP32
     // (M[nnz] = 0, M[nnz + 1] = 1, M[nnz + 2] = -1, M[nnz + 3] scratch) sharing one region
-    double *F = reinterpret_cast<double *>(smem + (size_t)wave * fp.wave_bytes);
+    // ParILU with room for it: the sweep records (products, then finishes)
+    // in a shared region ahead of the wave regions, read by LDS instructions
+    // instead of global loads (fp.shared_bytes = 0 otherwise)
+    uint32_t *par_l = reinterpret_cast<uint32_t *>(smem);
+    const bool par_lds = fp.shared_bytes > 0;
+    if (par_lds) {
+        const int np = fp.par_psteps * WAVE, nf = fp.par_rounds * WAVE;
+        for (int i = threadIdx.x; i < np; i += blockDim.x) par_l[i] = fp.parp[i];
+        for (int i = threadIdx.x; i < nf; i += blockDim.x) par_l[np + i] = fp.parf[i];
+        __syncthreads();
+    }
+    double *F = reinterpret_cast<double *>(smem + fp.shared_bytes + (size_t)wave * fp.wave_bytes);
     double *W = F + (nnz + 4);
     float *M = reinterpret_cast<float *>(W);
     if (lane == 0) { F[nnz] = 0.0; F[nnz + 1] = 1.0; }
@@ -165,7 +176,10 @@ __global__ __launch_bounds__(KF_MAX_WAVES * WAVE) void k_factor_fast(
             double *src = F, *dst = W;
             while (used < sweeps) {
                 const double rr = wave_sum(fp.par_rounds > 0
-                    ? parilu_rounds<true>(fp.parp, fp.parf, fp.par_rounds, fp.par_psteps, Ap, src, dst, lane)
+                    ? (par_lds ? parilu_rounds<true>(par_l, par_l + fp.par_psteps * WAVE, fp.par_rounds,
+                                                     fp.par_psteps, Ap, src, dst, lane)
+                               : parilu_rounds<true>(fp.parp, fp.parf, fp.par_rounds, fp.par_psteps, Ap, src, dst,
+                                                     lane))
                     : 0.0);
                 wave_sync();
                 used++;

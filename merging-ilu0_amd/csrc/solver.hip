@@ -1262,10 +1262,17 @@ int launch_factor_fast(bilu_pattern_s &h, int batch, const double *A, const doub
     const size_t second = std::max<size_t>(mode == FS_PARILU ? (size_t)(nnz + 2) * sizeof(double) : 0,
                                            isai ? (size_t)(nnz + 4) * sizeof(float) : 0);
     fp.wave_bytes = int(((size_t)(nnz + 4) * sizeof(double) + second + 15) / 16 * 16);
-    int waves = (int)std::min<size_t>(std::min(max_waves, KF_MAX_WAVES), h.max_lds / fp.wave_bytes);
+    const int wcap = std::min(max_waves, KF_MAX_WAVES);
+    int waves = (int)std::min<size_t>(wcap, h.max_lds / fp.wave_bytes);
     if (waves < 1) return BILU_ERR_TOO_LARGE;
+    if (mode == FS_PARILU && sweeps > 0 && pl.par_rounds > 0) {
+        // the sweep records in a shared LDS region when that costs no wave
+        const size_t rec = ((size_t)(pl.par_psteps + pl.par_rounds) * WAVE * 4 + 15) / 16 * 16;
+        if (rec < h.max_lds && (int)std::min<size_t>(wcap, (h.max_lds - rec) / fp.wave_bytes) >= waves)
+            fp.shared_bytes = int(rec);
+    }
     waves = std::min(waves, std::max(1, batch));
-    const size_t lds = (size_t)waves * fp.wave_bytes;
+    const size_t lds = fp.shared_bytes + (size_t)waves * fp.wave_bytes;
     const int grid = persistent_grid(h, lds, waves, batch);
     auto kern = k_factor_fast<TO>;
     if (lds > 65536) HIP_TRY(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
