@@ -148,3 +148,23 @@ def test_bad_schedule_rejected():
     for s in (-1, 5):
         with pytest.raises(bilu.BiluError):
             bilu.ilu0(P, dA, F, schedule=s)
+
+
+def test_tune_without_the_persistent_kernel():
+    """a pattern beyond the persistent kernel's tables (n = 300): the tuner
+    reports streams as unsupported and keeps the fastest of the others"""
+    pat, A, _ = synth.batch("random", 256, n=300)
+    P = bilu.Pattern.from_synth(pat)
+    dA = to_dev(A)
+    F = torch.empty_like(dA)
+    try:
+        bilu.ilu0(P, dA, F, schedule=bilu.ILU_SCHED_STREAMS)
+        streams_ok = True
+    except bilu.BiluError:
+        streams_ok = False
+    best, ms = bilu.ilu0_tune(P, dA, F)
+    print(f"n=300 schedules {ms}, best {best}, streams supported {streams_ok}", flush=True)
+    assert (ms[3] > 0) == streams_ok and all(t > 0 for t in ms[:3])
+    assert ms[best - 1] == min(t for t in ms if t > 0)
+    Fo = np.stack([O.ilu0(pat, A[s])[0] for s in (0, 255)])
+    assert np.array_equal(F.cpu().numpy()[[0, 255]], Fo)
