@@ -41,12 +41,14 @@ struct FastParams {
     int o_trL, o_trU;
     int wave_bytes;
     int par_w;      // 1: the wave region holds a second factor buffer (ParILU Jacobi)
-    int spos_lds;   // k_mixed: 1 when the slot positions follow the records in the shared LDS
     int *counter;
     // ISAI generation streams [step][64] and the slot triangles (ISAI applies)
     const uint32_t *isL0, *isL1, *isU0, *isU1;
     int isL_steps, isU_steps;
-    const uint8_t *slot_kind;
+    // triangle slot layouts of the FP32 ISAI applies [SL or SU][64]
+    const uint32_t *isl_meta, *isu_meta;
+    const int32_t *isl_pos, *isu_pos;
+    int SL, SU;
     // exact ILU(0) lane streams [step][64]
     const uint32_t *ilu0, *ilu1;
     int ilu_steps;

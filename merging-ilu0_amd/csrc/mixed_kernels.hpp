@@ -235,73 +235,58 @@ struct MixedParams {
     int max_refine;      // refinement steps
 };
 
-// y (LDS, FP64, by row) = A x with the FP64 values read from global memory
-// slot by slot; x in LDS (FP64).  Same fma chain as spmv_slots.
-// PACKED: the records of spmv32 (the scratch cell is y[n + 1]) instead of
-// spmv_slots' records.
-template <int SM, bool PACKED = false>
-__device__ __forceinline__ void spmv_slots_global(const double *__restrict__ Ap,
-                                                  const int32_t *__restrict__ slot_pos, int S,
-                                                  const uint32_t *__restrict__ meta,
-                                                  const double *__restrict__ x, double *__restrict__ y,
-                                                  double *__restrict__ scratch, int lane)
+// w = A x in FP64 by CSR rows: lane l computes the rows l + 64 k, each an
+// fma chain left to right over the row's entries (the same chain as the
+// slot SpMVs, whose lanes hold whole rows in CSR order), with A and the
+// column indices read from global memory in batches of 8 entries (the A
+// loads do not wait on each other or on the index loads) and x gathered from
+// LDS (FP64 cells).
+template <int NR>
+__device__ __forceinline__ void residual_rows(const int *__restrict__ rp, const int *__restrict__ ci,
+                                              const double *__restrict__ Ap, const double *__restrict__ x,
+                                              Vec<NR, double> &w, int n, int lane)
 {
-    // groups of G slots: the loads of one group are in flight together, the
-    // compiler barrier keeps it from hoisting every group's loads (VGPRs)
-    constexpr int G = 12;
-    const char *xb = reinterpret_cast<const char *>(x);
-    char *yb = reinterpret_cast<char *>(y);
-    double acc = 0.0;
+    constexpr int K = 8;
 #pragma unroll
-    for (int g = 0; g < SM; g += G) {
-        double av[G];
+    for (int k = 0; k < NR; k++) {
+        const int i = lane + WAVE * k;
+        const int pb = (i < n) ? rp[i] : 0, pe = (i < n) ? rp[i + 1] : 0;
+        int len = pe - pb;
+        // wave-uniform trip count: the longest row of this row slot
+        int lmax = len;
 #pragma unroll
-        for (int j = 0; j < G; j++) {
-            const int s = g + j;
-            const int e = (s < S && s < SM) ? slot_pos[s * WAVE + lane] : -1;
-            av[j] = (e >= 0) ? Ap[e] : 0.0;
-        }
+        for (int o = 32; o >= 1; o >>= 1) lmax = max(lmax, __shfl_xor(lmax, o));
+        double acc = 0.0;
+        for (int q0 = 0; q0 < lmax; q0 += K) {
+            int c[K];
+            double av[K];
 #pragma unroll
-        for (int j = 0; j < G; j++) {
-            const int s = g + j;
-            if (s >= SM) break;
-            if constexpr (PACKED) {
-                // FP64 offsets are twice the FP32 ones of the records
-                const uint32_t w = meta[s * WAVE + lane];
-                acc = fma(av[j], *reinterpret_cast<const double *>(xb + 2u * (w & 0xFFFFu)), acc);
-                *reinterpret_cast<double *>(yb + 2u * ((w >> 16) & 0x7FFCu)) = acc;
-                acc = (w >> 31) ? acc : 0.0;
-            } else {
-                const uint32_t m = meta[s * WAVE + lane];
-                acc = fma(av[j], *reinterpret_cast<const double *>(xb + (m & 0xFFF8u)), acc);
-                const bool end = (m >> 16) != 0xFFFFu;
-                double *dst = end ? reinterpret_cast<double *>(yb + (m >> 16)) : scratch;
-                *dst = acc;
-                acc = end ? 0.0 : acc;
+            for (int j = 0; j < K; j++) {
+                const bool in = q0 + j < len;
+                c[j] = in ? ci[pb + q0 + j] : n;
+                av[j] = in ? Ap[pb + q0 + j] : 0.0;
+            }
+#pragma unroll
+            for (int j = 0; j < K; j++) {
+                const double xv = x[c[j]];
+                if (q0 + j < len) acc = fma(av[j], xv, acc);
             }
         }
-        asm volatile("" ::: "memory");
+        w.e[k] = acc;
     }
 }
 
-// slot coefficients of spmv32: CM_LDS: from the wave's LDS, slot-major
-// (cl[s * 64 + lane]); CM_ISAI_L / CM_ISAI_U: from the slot's ISAI value c[s]
-// (registers) and the slot's triangle: ISAI(L) takes c below the diagonal, 1
-// on it, 0 above; ISAI(U) takes c on and above the diagonal, 0 below
-constexpr int CM_LDS = 0, CM_ISAI_L = 1, CM_ISAI_U = 2;
-
-// y (LDS, by row) = C x over the slot layout with one packed record per
-// slot: bits 0-15 the byte offset of x (FP32 cells); bits 16-30 the byte
+// y (LDS, by row) = C x over A's slot layout with the coefficients read
+// from the wave's LDS, slot-major (cl[s * 64 + lane]) and one packed record
+// per slot: bits 0-15 the byte offset of x (FP32 cells); bits 16-30 the byte
 // offset of the output cell (the row's cell when the slot ends a row, else
-// the output vector's scratch cell n + 1) with the triangle kind in its two
-// low bits (1 strictly lower, 2 diagonal, 0 upper or padding); bit 31 set
-// when the slot does not end its row.  One LDS word per slot instead of a
-// separate x and output record: the kernel is bound by LDS instructions.
-// The same CSR-order fma chain as spmv_slots.
-template <int SM, int CM>
-__device__ __forceinline__ void spmv32(const float (&c)[SM], const float *__restrict__ cl,
-                                       const uint32_t *__restrict__ meta, const float *__restrict__ x,
-                                       float *__restrict__ y, int lane)
+// the output vector's scratch cell n + 1); bit 31 set when the slot does not
+// end its row.  One LDS word per slot instead of a separate x and output
+// record: the kernel is bound by instruction issue.  The same CSR-order fma
+// chain as spmv_slots.
+template <int SM>
+__device__ __forceinline__ void spmv32(const float *__restrict__ cl, const uint32_t *__restrict__ meta,
+                                       const float *__restrict__ x, float *__restrict__ y, int lane)
 {
     const char *xb = reinterpret_cast<const char *>(x);
     char *yb = reinterpret_cast<char *>(y);
@@ -311,7 +296,7 @@ __device__ __forceinline__ void spmv32(const float (&c)[SM], const float *__rest
     float xv[SM], cw[SM];
     auto fetch = [&](int s) {
         rw[s] = meta[s * WAVE + lane];
-        if (CM == CM_LDS) cw[s] = cl[s * WAVE + lane];
+        cw[s] = cl[s * WAVE + lane];
         xv[s] = *reinterpret_cast<const float *>(xb + (rw[s] & 0xFFFFu));
     };
 #pragma unroll
@@ -323,13 +308,43 @@ __device__ __forceinline__ void spmv32(const float (&c)[SM], const float *__rest
             for (int t = s + G; t < s + 2 * G && t < SM; t++) fetch(t);
         }
         const uint32_t r = rw[s];
-        const uint32_t kd = (r >> 16) & 3u;
-        float cs;
-        if (CM == CM_LDS) cs = cw[s];
-        else if (CM == CM_ISAI_L) cs = kd == 1u ? c[s] : (kd == 2u ? 1.0f : 0.0f);
-        else cs = kd == 1u ? 0.0f : c[s];
-        acc = __builtin_fmaf(cs, xv[s], acc);
-        *reinterpret_cast<float *>(yb + ((r >> 16) & 0x7FFCu)) = acc;
+        acc = __builtin_fmaf(cw[s], xv[s], acc);
+        *reinterpret_cast<float *>(yb + ((r >> 16) & 0x7FFFu)) = acc;
+        acc = (r >> 31) ? acc : 0.0f;
+    }
+}
+
+// y (LDS, by row) = C x over one of the triangle slot layouts: the
+// coefficient of slot s is c[s] (registers), the record one packed word
+// (x offset | output offset << 16 | not-a-row-end << 31); only the first
+// `ns` slots are visited (a wave-uniform bound, checked per gather group).
+// Each row sum is the CSR-order fma chain over the triangle's entries.
+template <int SMX>
+__device__ __forceinline__ void spmv_tri(const float (&c)[SMX], int ns, const uint32_t *__restrict__ rec,
+                                         const float *__restrict__ x, float *__restrict__ y, int lane)
+{
+    const char *xb = reinterpret_cast<const char *>(x);
+    char *yb = reinterpret_cast<char *>(y);
+    float acc = 0.0f;
+    constexpr int G = SPMV_G;
+    uint32_t rw[SMX];
+    float xv[SMX];
+    auto fetch = [&](int s) {
+        rw[s] = rec[s * WAVE + lane];
+        xv[s] = *reinterpret_cast<const float *>(xb + (rw[s] & 0xFFFFu));
+    };
+#pragma unroll
+    for (int s = 0; s < G && s < SMX; s++) fetch(s);
+#pragma unroll
+    for (int s = 0; s < SMX; s++) {
+        if (s % G == 0) {
+            if (s >= ns) break;
+#pragma unroll
+            for (int t = s + G; t < s + 2 * G && t < SMX; t++) fetch(t);
+        }
+        const uint32_t r = rw[s];
+        acc = __builtin_fmaf(c[s], xv[s], acc);
+        *reinterpret_cast<float *>(yb + ((r >> 16) & 0x7FFFu)) = acc;
         acc = (r >> 31) ? acc : 0.0f;
     }
 }
@@ -347,10 +362,11 @@ constexpr int PCK_LU = 0, PCK_ISAI = 1;
 // FP32 ISAI(L) / ISAI(U) values on the pattern, gZ = zero-pivot flags of
 // their generation (two SpMVs per apply)
 // waves per workgroup (launch bound) of a k_mixed instantiation: 72 slots
-// (patterns up to ~4600 nnz) need the register budget of 8 waves
-constexpr int km_waves(int SM, int PCK) { return PCK == PCK_LU ? (SM > 48 ? 8 : KM_MAX_WAVES) : KMI_MAX_WAVES; }
+// (patterns up to ~4600 nnz) need the register budget of 8 waves (LU: A in
+// VGPRs; ISAI: 2 x 40 triangle coefficients)
+constexpr int km_waves(int SM, int PCK) { return SM > 48 ? 8 : (PCK == PCK_LU ? KM_MAX_WAVES : KMI_MAX_WAVES); }
 
-template <int NR, int SM, int PCK>
+template <int NR, int SM, int PCK, int SLU = 1>
 __global__ __launch_bounds__(km_waves(SM, PCK) * WAVE) void k_mixed(
     DevPattern P, FastParams fp, int batch, const double *__restrict__ gA,
     const double *__restrict__ gb, double *__restrict__ gx, const float *__restrict__ gF,
@@ -366,27 +382,26 @@ __global__ __launch_bounds__(km_waves(SM, PCK) * WAVE) void k_mixed(
     uint32_t *meta = reinterpret_cast<uint32_t *>(smem);
     const uint32_t *trL = fp.trL;
     const uint32_t *trU = fp.trU;
-    // LU: the records of spmv_slots.  ISAI: the packed records of spmv32,
-    // from those (8 col | 8 row << 16, 0xFFFF: no row end) and the slot's
-    // triangle; padding slots read x[0] with coefficient 0
-    // the slot positions (the A gathers of each system's load and of every
-    // FP64 residual) follow the records in the shared LDS when the host found
-    // room for them without losing a wave; else they are read from global
-    // memory, a dependent load before every gather
-    // (two code paths, so that the LDS copy is read by LDS instructions)
-    int32_t *spos_l = reinterpret_cast<int32_t *>(meta + SM * WAVE);
-    const bool sp_lds = fp.spos_lds != 0;
-    if (sp_lds)
-        for (int i = threadIdx.x; i < SM * WAVE; i += blockDim.x) spos_l[i] = (i < S * WAVE) ? fp.slot_pos[i] : -1;
+    // LU: the records of spmv_slots.  ISAI: the packed records of spmv32
+    // for the A matvec, from those (8 col | 8 row << 16, 0xFFFF: no row end);
+    // padding slots read x[0] with coefficient 0.  ISAI: then the records of
+    // the ISAI(L) and ISAI(U) slot layouts, padded to SLU slots each.
+    uint32_t *lrec = meta + SM * WAVE, *urec = lrec + SLU * WAVE;
     for (int i = threadIdx.x; i < SM * WAVE; i += blockDim.x) {
         const uint32_t m = (i < S * WAVE) ? fp.slot_meta[i] : 0xFFFF0000u;
         if constexpr (PCK == PCK_LU) {
             meta[i] = m;
         } else {
-            const uint32_t kd = (i < S * WAVE) ? (uint32_t)fp.slot_kind[i] : 0u;
             const bool end = (m >> 16) != 0xFFFFu;
             const uint32_t yo = end ? (m >> 16) >> 1 : (uint32_t)(n + 1) * 4u;
-            meta[i] = ((m & 0xFFFFu) >> 1) | ((yo | kd) << 16) | (end ? 0u : 1u << 31);
+            meta[i] = ((m & 0xFFFFu) >> 1) | (yo << 16) | (end ? 0u : 1u << 31);
+        }
+    }
+    if constexpr (PCK == PCK_ISAI) {
+        const uint32_t pad = (uint32_t)n * 4u | ((uint32_t)(n + 1) * 4u << 16) | (1u << 31);
+        for (int i = threadIdx.x; i < SLU * WAVE; i += blockDim.x) {
+            lrec[i] = (i < fp.SL * WAVE) ? fp.isl_meta[i] : pad;
+            urec[i] = (i < fp.SU * WAVE) ? fp.isu_meta[i] : pad;
         }
     }
     // per wave: F[nnz + 2] FP32 (F[nnz] = 0), then two vector regions of
@@ -402,7 +417,6 @@ __global__ __launch_bounds__(km_waves(SM, PCK) * WAVE) void k_mixed(
     double *d1 = d0 + (n + 2);
     float *e0 = reinterpret_cast<float *>(d0), *e1 = reinterpret_cast<float *>(d1);
     float *scratch = e1 + (n + 1);
-    double *dscratch = d1 + (n + 1);
     double *accb = d1 + (n + 2);
     if (PCK == PCK_LU && lane == 0) F[nnz] = 0.0f;
     __syncthreads();
@@ -413,12 +427,11 @@ __global__ __launch_bounds__(km_waves(SM, PCK) * WAVE) void k_mixed(
         rlev[k] = (int)fp.rlevL[k * WAVE + lane] | ((int)fp.rlevU[k * WAVE + lane] << 16);
     const int vcntL = fp.cntL[lane], vcntU = fp.cntU[lane];
 
-    // LU: A (FP32) in VGPRs.  ISAI: one ISAI value per slot in VGPRs (the
-    // strictly lower slots hold ISAI(L), the others ISAI(U); spmv32 derives
-    // each triangle's coefficients from the slot's output record), A in LDS.
-    constexpr int SA = PCK == PCK_LU ? SM : 1, SC = PCK == PCK_ISAI ? SM : 1;
+    // LU: A (FP32) in VGPRs.  ISAI: the ISAI(L) and ISAI(U) coefficients of
+    // the triangle slot layouts in VGPRs (1 on ISAI(L)'s unit diagonal), A in LDS.
+    constexpr int SA = PCK == PCK_LU ? SM : 1, SC = PCK == PCK_ISAI ? SLU : 1;
     float a[SA];
-    float cv[SC];
+    float cL[SC], cU[SC];
 
     for (;;) {
         int sys = 0;
@@ -435,13 +448,18 @@ __global__ __launch_bounds__(km_waves(SM, PCK) * WAVE) void k_mixed(
             wave_copy(F, Fp, nnz, lane);
 #pragma unroll
         for (int q = 0; q < SM; q++) {
-            const int e = (q < S) ? (sp_lds ? spos_l[q * WAVE + lane] : fp.slot_pos[q * WAVE + lane]) : -1;
+            const int e = (q < S) ? fp.slot_pos[q * WAVE + lane] : -1;
             const float av = (e >= 0) ? (float)Ap[e] : 0.0f;
-            if constexpr (PCK == PCK_LU) {
-                a[q] = av;
-            } else {
-                Asl[q * WAVE + lane] = av;
-                cv[q] = (e >= 0) ? Fp[e] : 0.0f;
+            if constexpr (PCK == PCK_LU) a[q] = av;
+            else Asl[q * WAVE + lane] = av;
+        }
+        if constexpr (PCK == PCK_ISAI) {
+#pragma unroll
+            for (int q = 0; q < SLU; q++) {
+                const int el = (q < fp.SL) ? fp.isl_pos[q * WAVE + lane] : -1;
+                const int eu = (q < fp.SU) ? fp.isu_pos[q * WAVE + lane] : -1;
+                cL[q] = el >= 0 ? Fp[el] : (el == -2 ? 1.0f : 0.0f);
+                cU[q] = eu >= 0 ? Fp[eu] : 0.0f;
             }
         }
         wave_sync();
@@ -490,16 +508,16 @@ __global__ __launch_bounds__(km_waves(SM, PCK) * WAVE) void k_mixed(
             } else {
                 to_lds(in, e0, lane, n);
                 wave_sync();
-                spmv32<SM, CM_ISAI_L>(cv, nullptr, meta, e0, e1, lane);
+                spmv_tri<SC>(cL, fp.SL, lrec, e0, e1, lane);
                 wave_sync();
-                spmv32<SM, CM_ISAI_U>(cv, nullptr, meta, e1, e0, lane);
+                spmv_tri<SC>(cU, fp.SU, urec, e1, e0, lane);
                 wave_sync();
                 from_lds(out, e0, lane, n);
             }
         };
         auto matvec = [&](Vec<NR, float> &out) {   // out = A32 e0
             if constexpr (PCK == PCK_LU) spmv_slots<SM, float>(a, meta, e0, e1, scratch, lane);
-            else spmv32<SM, CM_LDS>(cv, Asl, meta, e0, e1, lane);
+            else spmv32<SM>(Asl, meta, e0, e1, lane);
             wave_sync();
             from_lds(out, e1, lane, n);
             wave_sync();
@@ -571,11 +589,8 @@ __global__ __launch_bounds__(km_waves(SM, PCK) * WAVE) void k_mixed(
                 for (int k = 0; k < NR; k++) x.e[k] = fma((double)d.e[k], rn, x.e[k]);
                 to_lds(x, d0, lane, n);
                 wave_sync();
-                if (sp_lds) spmv_slots_global<SM, PCK == PCK_ISAI>(Ap, spos_l, S, meta, d0, d1, dscratch, lane);
-                else spmv_slots_global<SM, PCK == PCK_ISAI>(Ap, fp.slot_pos, S, meta, d0, d1, dscratch, lane);
-                wave_sync();
                 Vec<NR, double> w;
-                from_lds(w, d1, lane, n);
+                residual_rows<NR>(P.rp, P.ci, Ap, d0, w, n, lane);
 #pragma unroll
                 for (int k = 0; k < NR; k++) {
                     const int i = lane + WAVE * k;

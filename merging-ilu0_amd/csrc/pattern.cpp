@@ -327,30 +327,25 @@ void build_ilu_schedules(PatternAnalysis &pa)
     }
 }
 
-void build_fast_plan(PatternAnalysis &pa)
+// Longest-processing-time assignment of whole rows (len(i) entries each) to
+// the 64 lanes of a wave, improved by moves and swaps between the fullest
+// lane and the others; every lane keeps its rows in increasing order.
+// Returns the slots per lane (the fullest lane's load).
+template <typename Len>
+static int balance_rows(int n, Len len, std::vector<std::vector<int>> &rows)
 {
-    FastPlan &fp = pa.fast;
-    fp = FastPlan();
-    const int n = pa.n, nnz = pa.nnz;
-    if (n >= 0xFFFF || nnz >= 0xFFFF) { fp.why = "n or nnz beyond the u16 schedule tables"; return; }
-    // balanced SpMV slot layout: longest-processing-time assignment of whole
-    // rows to 64 lanes; each lane keeps its rows in increasing row order so
-    // every row sum runs left to right over the CSR entries
-    std::vector<int> order(n);
-    for (int i = 0; i < n; i++) order[i] = i;
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
-        return (pa.rp[a + 1] - pa.rp[a]) > (pa.rp[b + 1] - pa.rp[b]);
-    });
+    std::vector<int> order;
+    for (int i = 0; i < n; i++)
+        if (len(i) > 0) order.push_back(i);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return len(a) > len(b); });
     std::vector<int> load(64, 0);
-    std::vector<std::vector<int>> rows(64);
+    rows.assign(64, {});
     for (int i : order) {
         int best = 0;
         for (int l = 1; l < 64; l++) if (load[l] < load[best]) best = l;
-        load[best] += pa.rp[i + 1] - pa.rp[i];
+        load[best] += len(i);
         rows[best].push_back(i);
     }
-    // improve the LPT start by row swaps between the fullest lane and the others
-    auto rlen = [&](int i) { return pa.rp[i + 1] - pa.rp[i]; };
     for (int iter = 0; iter < 4096; iter++) {
         int hi = 0;
         for (int l = 1; l < 64; l++) if (load[l] > load[hi]) hi = l;
@@ -359,16 +354,15 @@ void build_fast_plan(PatternAnalysis &pa)
             if (l == hi) continue;
             for (size_t a = 0; a < rows[hi].size() && !improved; a++) {
                 const int ra = rows[hi][a];
-                // move
-                if (load[l] + rlen(ra) < load[hi]) {
-                    load[l] += rlen(ra); load[hi] -= rlen(ra);
+                if (load[l] + len(ra) < load[hi]) {   // move
+                    load[l] += len(ra); load[hi] -= len(ra);
                     rows[l].push_back(ra); rows[hi].erase(rows[hi].begin() + a);
                     improved = true;
                     break;
                 }
-                for (size_t b = 0; b < rows[l].size(); b++) {
+                for (size_t b = 0; b < rows[l].size(); b++) {   // swap
                     const int rb = rows[l][b];
-                    const int d = rlen(ra) - rlen(rb);
+                    const int d = len(ra) - len(rb);
                     if (d > 0 && load[l] + d < load[hi]) {
                         load[l] += d; load[hi] -= d;
                         std::swap(rows[hi][a], rows[l][b]);
@@ -382,6 +376,19 @@ void build_fast_plan(PatternAnalysis &pa)
     }
     int S = 0;
     for (int l = 0; l < 64; l++) { std::sort(rows[l].begin(), rows[l].end()); S = std::max(S, load[l]); }
+    return S;
+}
+
+void build_fast_plan(PatternAnalysis &pa)
+{
+    FastPlan &fp = pa.fast;
+    fp = FastPlan();
+    const int n = pa.n, nnz = pa.nnz;
+    if (n >= 0xFFFF || nnz >= 0xFFFF) { fp.why = "n or nnz beyond the u16 schedule tables"; return; }
+    // balanced SpMV slot layout of A: whole rows per lane, each row sum an
+    // fma chain left to right over the CSR entries
+    std::vector<std::vector<int>> rows;
+    const int S = balance_rows(n, [&](int i) { return pa.rp[i + 1] - pa.rp[i]; }, rows);
     fp.S = S;
     fp.slot_meta.assign(size_t(S) * 64, 0xFFFF0000u);
     fp.slot_pos.assign(size_t(S) * 64, -1);
@@ -392,6 +399,43 @@ void build_fast_plan(PatternAnalysis &pa)
                 const uint32_t row = (p == pa.rp[i + 1] - 1) ? uint32_t(i) : 0xFFFFu;
                 fp.slot_meta[size_t(s) * 64 + l] = uint32_t(pa.ci[p]) | (row << 16);
                 fp.slot_pos[size_t(s) * 64 + l] = p;
+            }
+        }
+    }
+    // slot layouts of the two triangles for the FP32 ISAI applies: ISAI(L)
+    // over the strictly lower entries and the (unit) diagonal, ISAI(U) over
+    // the diagonal and the strictly upper entries; whole rows per lane, CSR
+    // order.  Records hold FP32 LDS byte offsets: x offset (bits 0-15) | output
+    // offset (bits 16-30: the row's cell at a row end, else the scratch cell
+    // n + 1) | bit 31 when the slot does not end its row; padding slots read
+    // the zero cell n.  Positions: the pattern position of the coefficient,
+    // -2 for ISAI(L)'s unit diagonal, -1 for padding.
+    if (n < 8190) {
+        for (int t = 0; t < 2; t++) {
+            const bool lower = t == 0;
+            auto beg = [&](int i) { return lower ? pa.rp[i] : pa.diag[i]; };
+            auto end = [&](int i) { return lower ? pa.diag[i] + 1 : pa.rp[i + 1]; };
+            std::vector<std::vector<int>> trows;
+            bool diag_ok = true;
+            for (int i = 0; i < n; i++) diag_ok &= pa.diag[i] >= 0;
+            if (!diag_ok) break;
+            const int ST = balance_rows(n, [&](int i) { return end(i) - beg(i); }, trows);
+            const uint32_t pad = uint32_t(n) * 4u | ((uint32_t(n + 1) * 4u) << 16) | (1u << 31);
+            std::vector<uint32_t> &meta = lower ? fp.isl_meta : fp.isu_meta;
+            std::vector<int32_t> &pos = lower ? fp.isl_pos : fp.isu_pos;
+            (lower ? fp.SL : fp.SU) = ST;
+            meta.assign(size_t(ST) * 64, pad);
+            pos.assign(size_t(ST) * 64, -1);
+            for (int l = 0; l < 64; l++) {
+                int s = 0;
+                for (int i : trows[l]) {
+                    for (int p = beg(i); p < end(i); p++, s++) {
+                        const bool last = p == end(i) - 1;
+                        const uint32_t y = last ? uint32_t(i) * 4u : uint32_t(n + 1) * 4u;
+                        meta[size_t(s) * 64 + l] = uint32_t(pa.ci[p]) * 4u | (y << 16) | (last ? 0u : 1u << 31);
+                        pos[size_t(s) * 64 + l] = (lower && p == pa.diag[i]) ? -2 : p;
+                    }
+                }
             }
         }
     }
@@ -520,14 +564,6 @@ void build_fast_plan(PatternAnalysis &pa)
     if (fp.dep.empty()) { fp.dep.assign(2, 0); }
     if (fp.etab.empty()) { fp.etab.assign(4, 0); }
 
-    // triangle of every SpMV slot (ISAI applies): 1 strictly lower, 2 diagonal, 0 upper / padding
-    fp.slot_kind.assign(fp.slot_pos.size(), 0);
-    for (size_t q = 0; q < fp.slot_pos.size(); q++) {
-        const int e = fp.slot_pos[q];
-        if (e < 0) continue;
-        const int row = int(std::upper_bound(pa.rp.begin(), pa.rp.end(), e) - pa.rp.begin()) - 1;
-        fp.slot_kind[q] = uint8_t(e < pa.diag[row] ? 1 : e == pa.diag[row] ? 2 : 0);
-    }
     // ISAI(L), ISAI(U) generation streams (power-1 pattern).  Row i of the
     // ISAI solves A(J,J)^T m = e by the column sweep of the reference
     // (isai_gpu.cu:325-389); in pull form every target m_t is an fma chain

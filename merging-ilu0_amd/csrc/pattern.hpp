@@ -60,10 +60,16 @@ struct FastPlan {
     std::vector<uint16_t> etab, eptr, dep;   // etab: 4 u16 per record, dep: 2 u16 per pair
     int nlev_e = 0, ne = 0;
     // ISAI(L) / ISAI(U) generation streams [step][64] (two u32 records per step)
-    // and the triangle of each SpMV slot (1 lower, 2 diagonal, 0 upper / padding)
     std::vector<uint32_t> isL0, isL1, isU0, isU1;
     int isL_steps = 0, isU_steps = 0;
-    std::vector<uint8_t> slot_kind;
+    // slot layouts of ISAI(L) (strict lower + unit diagonal) and ISAI(U)
+    // (diagonal + strict upper) for the FP32 applies: packed records [S][64]
+    // (x offset | output offset << 16 | not-a-row-end << 31, FP32 LDS byte
+    // offsets) and coefficient positions (-2: unit diagonal, -1: padding);
+    // SL = SU = 0 when the pattern lacks a diagonal entry
+    int SL = 0, SU = 0;
+    std::vector<uint32_t> isl_meta, isu_meta;
+    std::vector<int32_t> isl_pos, isu_pos;
     // exact ILU(0) as per-lane step streams [step][64] (two u32 records per
     // step; list-scheduled over the entry DAG, padded to a multiple of
     // ILU_BATCH steps)

@@ -957,7 +957,8 @@ struct bilu_pattern_s {
     uint16_t *d_rlevL = nullptr, *d_rlevU = nullptr;
     uint16_t *d_etab = nullptr, *d_eptr = nullptr, *d_dep = nullptr;
     uint32_t *d_isL0 = nullptr, *d_isL1 = nullptr, *d_isU0 = nullptr, *d_isU1 = nullptr;
-    uint8_t *d_slot_kind = nullptr;
+    uint32_t *d_isl_meta = nullptr, *d_isu_meta = nullptr;
+    int32_t *d_isl_pos = nullptr, *d_isu_pos = nullptr;
     uint32_t *d_ilu0 = nullptr, *d_ilu1 = nullptr;
     uint32_t *d_parp = nullptr, *d_parf = nullptr;
     int *d_counter = nullptr;      // 256 bytes of work counters: [0] solve kernels, [32] factor kernel
@@ -1065,7 +1066,10 @@ int upload(bilu_pattern_s &h)
         const size_t o_iL1 = put(fp.isL1.data(), fp.isL1.size() * 4);
         const size_t o_iU0 = put(fp.isU0.data(), fp.isU0.size() * 4);
         const size_t o_iU1 = put(fp.isU1.data(), fp.isU1.size() * 4);
-        const size_t o_sk = put(fp.slot_kind.data(), fp.slot_kind.size());
+        const size_t o_lm = put(fp.isl_meta.data(), fp.isl_meta.size() * 4);
+        const size_t o_um = put(fp.isu_meta.data(), fp.isu_meta.size() * 4);
+        const size_t o_lp = put(fp.isl_pos.data(), fp.isl_pos.size() * 4);
+        const size_t o_up = put(fp.isu_pos.data(), fp.isu_pos.size() * 4);
         const size_t o_i0 = put(fp.ilu0.data(), fp.ilu0.size() * 4);
         const size_t o_i1 = put(fp.ilu1.data(), fp.ilu1.size() * 4);
         const size_t o_p0 = put(fp.parp.data(), fp.parp.size() * 4);
@@ -1080,7 +1084,8 @@ int upload(bilu_pattern_s &h)
         h.d_etab = (uint16_t *)(b + o_et); h.d_eptr = (uint16_t *)(b + o_ep); h.d_dep = (uint16_t *)(b + o_dp);
         h.d_isL0 = (uint32_t *)(b + o_iL0); h.d_isL1 = (uint32_t *)(b + o_iL1);
         h.d_isU0 = (uint32_t *)(b + o_iU0); h.d_isU1 = (uint32_t *)(b + o_iU1);
-        h.d_slot_kind = b + o_sk;
+        h.d_isl_meta = (uint32_t *)(b + o_lm); h.d_isu_meta = (uint32_t *)(b + o_um);
+        h.d_isl_pos = (int32_t *)(b + o_lp); h.d_isu_pos = (int32_t *)(b + o_up);
         h.d_ilu0 = (uint32_t *)(b + o_i0); h.d_ilu1 = (uint32_t *)(b + o_i1);
         h.d_parp = (uint32_t *)(b + o_p0); h.d_parf = (uint32_t *)(b + o_p1);
     }
@@ -1144,7 +1149,8 @@ void fast_tables(const bilu_pattern_s &h, FastParams &fp)
     fp.counter = h.d_counter;
     fp.isL0 = h.d_isL0; fp.isL1 = h.d_isL1; fp.isU0 = h.d_isU0; fp.isU1 = h.d_isU1;
     fp.isL_steps = pl.isL_steps; fp.isU_steps = pl.isU_steps;
-    fp.slot_kind = h.d_slot_kind;
+    fp.isl_meta = h.d_isl_meta; fp.isu_meta = h.d_isu_meta; fp.isl_pos = h.d_isl_pos; fp.isu_pos = h.d_isu_pos;
+    fp.SL = pl.SL; fp.SU = pl.SU;
     fp.ilu0 = h.d_ilu0; fp.ilu1 = h.d_ilu1; fp.ilu_steps = pl.ilu_steps;
     fp.parp = h.d_parp; fp.parf = h.d_parf; fp.par_rounds = pl.par_rounds; fp.par_psteps = pl.par_psteps;
 }
@@ -1289,13 +1295,13 @@ struct Jobs {
     const int *sysidx = nullptr, *outidx = nullptr;
 };
 
-template <int NR, int SM, int PCK>
+template <int NR, int SM, int PCK, int SLU>
 int launch_mixed_t(bilu_pattern_s &h, const FastParams &fp, int grid, int waves, size_t lds, int batch,
                    const double *A, const double *b, double *x, const float *F, const int *Z,
                    const MixedParams &mp, float *it, int *cv, double *ir, double *tr, int *refine,
                    hipStream_t st, Jobs jobs)
 {
-    auto kern = k_mixed<NR, SM, PCK>;
+    auto kern = k_mixed<NR, SM, PCK, SLU>;
     if (lds > 65536) HIP_TRY(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, st, fp.counter);
     hipLaunchKernelGGL(kern, dim3(grid), dim3(waves * WAVE), lds, st, h.dev, fp, batch, A, b, x, F, Z, mp, it, cv,
@@ -1320,43 +1326,44 @@ int launch_mixed(bilu_pattern_s &h, int batch, const double *A, const double *b,
     const auto &pl = h.pa.fast;
     const int n = h.pa.n, nnz = h.pa.nnz;
     const int nr = (n + WAVE - 1) / WAVE;
-    const int sm = slot_template_mixed(pl.S);
+    int sm = slot_template_mixed(pl.S), slu = 1;
     if (!pl.ok || nr > MAX_NR || pl.nr != nr || sm < 0) return BILU_ERR_TOO_LARGE;
+    if (pck == PCK_ISAI) {
+        // (A slots, triangle slots) tiers of the ISAI instantiations
+        const int tl = std::max(pl.SL, pl.SU);
+        if (tl <= 0) return BILU_ERR_TOO_LARGE;
+        sm = -1;
+        for (const auto &t : {std::make_pair(24, 20), std::make_pair(36, 24), std::make_pair(48, 32),
+                              std::make_pair(72, 40)})
+            if (pl.S <= t.first && tl <= t.second) { sm = t.first; slu = t.second; break; }
+        if (sm < 0) return BILU_ERR_TOO_LARGE;
+    }
     FastParams fp{};
     fast_tables(h, fp);
-    // per wave: FP32 factors (+2 cells, 16-aligned), two (n + 2)-double vector regions,
-    // FP64 row accumulators of the solves (64 NR)
+    // per wave: LU: FP32 factors (+2 cells, 16-aligned), two (n + 2)-double vector regions,
+    // FP64 row accumulators of the solves (64 NR); ISAI: A (FP32, slot-major), the two vector regions
     fp.wave_bytes = pck == PCK_LU
         ? int(((size_t)(nnz + 2) * 4 + 15) / 16 * 16 + (size_t)2 * (n + 2) * 8 + (size_t)64 * nr * 8)
         : int((size_t)sm * WAVE * 4 + (size_t)2 * (n + 2) * 8);
     fp.wave_bytes = (fp.wave_bytes + 15) / 16 * 16;
-    // shared LDS: the slot records, then the slot positions when they cost no wave
-    const size_t rec = (size_t)sm * WAVE * 4;
+    // shared LDS: A's slot records (ISAI: then the two triangle layouts' records)
+    const size_t rec = (size_t)(sm + (pck == PCK_ISAI ? 2 * slu : 0)) * WAVE * 4;
     if (rec >= h.max_lds) return BILU_ERR_TOO_LARGE;
-    if (pck == PCK_ISAI && sm > 48) return BILU_ERR_TOO_LARGE;   // ISAI coefficients beyond the VGPR budget
     const int wcap = std::min(max_waves, km_waves(sm, pck));
-    auto fit = [&](size_t shared) {
-        return shared >= h.max_lds ? 0 : (int)std::min<size_t>(wcap, (h.max_lds - shared) / fp.wave_bytes);
-    };
-    fp.spos_lds = fit(2 * rec) >= fit(rec) ? 1 : 0;
-    fp.shared_bytes = int(rec * (fp.spos_lds ? 2 : 1));
-    int waves = fit(fp.shared_bytes);
+    fp.shared_bytes = int(rec);
+    int waves = (int)std::min<size_t>(wcap, (h.max_lds - rec) / fp.wave_bytes);
     if (waves < 1) return BILU_ERR_TOO_LARGE;
     waves = std::min(waves, std::max(1, batch));
     const size_t lds = fp.shared_bytes + (size_t)waves * fp.wave_bytes;
     const int grid = persistent_grid(h, lds, waves, batch);
-#define MIXED_CASE(R, M) \
+#define MIXED_CASE(R, M, T) \
     if (nr == R && sm == M) \
         return pck == PCK_LU \
-            ? launch_mixed_t<R, M, PCK_LU>(h, fp, grid, waves, lds, batch, A, b, x, F, Z, mp, it, cv, ir, tr, refine, st, jobs) \
-            : launch_mixed_t<R, M, PCK_ISAI>(h, fp, grid, waves, lds, batch, A, b, x, F, Z, mp, it, cv, ir, tr, refine, st, jobs);
-#define MIXED_CASE_LU(R, M) \
-    if (nr == R && sm == M) \
-        return launch_mixed_t<R, M, PCK_LU>(h, fp, grid, waves, lds, batch, A, b, x, F, Z, mp, it, cv, ir, tr, refine, st, jobs);
-#define MIXED_ROW(R) MIXED_CASE(R, 24) MIXED_CASE(R, 36) MIXED_CASE(R, 48) MIXED_CASE_LU(R, 72)
+            ? launch_mixed_t<R, M, PCK_LU, 1>(h, fp, grid, waves, lds, batch, A, b, x, F, Z, mp, it, cv, ir, tr, refine, st, jobs) \
+            : launch_mixed_t<R, M, PCK_ISAI, T>(h, fp, grid, waves, lds, batch, A, b, x, F, Z, mp, it, cv, ir, tr, refine, st, jobs);
+#define MIXED_ROW(R) MIXED_CASE(R, 24, 20) MIXED_CASE(R, 36, 24) MIXED_CASE(R, 48, 32) MIXED_CASE(R, 72, 40)
     MIXED_ROW(1) MIXED_ROW(2) MIXED_ROW(3) MIXED_ROW(4) MIXED_ROW(5)
 #undef MIXED_ROW
-#undef MIXED_CASE_LU
 #undef MIXED_CASE
     return BILU_ERR_TOO_LARGE;
 }

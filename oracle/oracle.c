@@ -866,22 +866,20 @@ int orc_isai_lu_f32(int n, const int *rp, const int *ci, const int *diag, const 
 }
 
 /* FP32 ISAI apply: out = ISAI(U) (ISAI(L) v), each row a CSR-order fmaf
- * chain over the whole row with the coefficients of the kernel's slots
- * (0 outside the factor's triangle, 1 on the ISAI(L) diagonal) */
+ * chain over the triangle's own entries (ISAI(L): the strictly lower
+ * entries and the unit diagonal; ISAI(U): the diagonal and the strictly
+ * upper entries) -- the kernel's triangle slot layouts */
 static void isai_apply_f(int n, const int *rp, const int *ci, const int *diag, const float *M,
                          const float *v, float *tmp, float *out)
 {
     for (int i = 0; i < n; i++) {
         float s = 0.0f;
-        for (int p = rp[i]; p < rp[i + 1]; p++) {
-            const float c = p < diag[i] ? M[p] : (p == diag[i] ? 1.0f : 0.0f);
-            s = fmaf(c, v[ci[p]], s);
-        }
+        for (int p = rp[i]; p <= diag[i]; p++) s = fmaf(p < diag[i] ? M[p] : 1.0f, v[ci[p]], s);
         tmp[i] = s;
     }
     for (int i = 0; i < n; i++) {
         float s = 0.0f;
-        for (int p = rp[i]; p < rp[i + 1]; p++) s = fmaf(p < diag[i] ? 0.0f : M[p], tmp[ci[p]], s);
+        for (int p = diag[i]; p < rp[i + 1]; p++) s = fmaf(M[p], tmp[ci[p]], s);
         out[i] = s;
     }
 }
