@@ -81,6 +81,13 @@ extern "C" {
 #define BILU_ILU_SCHED_ERAS 3     /* right-looking pivot eras (approach 4) */
 #define BILU_ILU_SCHED_STREAMS 4  /* persistent kernel, list-scheduled lane streams */
 
+/* ParILU(0) sweep order (bilu_opts.parilu_order) */
+#define BILU_PARILU_JACOBI 0   /* every entry reads the iterate the sweep started from */
+#define BILU_PARILU_INPLACE 1  /* rounds of 64 entries (fewest products first), each reading the
+                                  iterate as the earlier rounds of the sweep left it; the
+                                  persistent factor kernel only (FP32_IR, or FP64 ILU-type
+                                  preconditioners through the split factor + solve launches) */
+
 #define BILU_ISAI_GENERAL 0
 #define BILU_ISAI_LOWER 1     /* strictly-lower values + unit diagonal */
 #define BILU_ISAI_UPPER 2     /* diagonal + strictly-upper values */
@@ -109,6 +116,9 @@ typedef struct {
                              The reference's ILU-ISAI solver uses 2 (PrecondBiCGSTAB_ilu_isai_gpu.cu:
                              297-299).  k > 1: FP64, one wave per system, ISAI rows of at most 32
                              entries (the reference's row_size_limit), no warm start */
+    int parilu_order;     /* BILU_PARILU_JACOBI (default) / BILU_PARILU_INPLACE.  With
+                             parilu_stop_tol > 0, an in-place sweep stops on the residuals its
+                             rounds saw and keeps the iterate after the sweep */
 } bilu_opts;
 
 typedef struct {

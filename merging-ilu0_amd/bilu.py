@@ -24,6 +24,7 @@ MERGED, SEGREGATED = 0, 1
 ISAI_GENERAL, ISAI_LOWER, ISAI_UPPER = 0, 1, 2
 KERNEL_AUTO, KERNEL_WAVE, KERNEL_PERSISTENT = 0, 1, 2
 PRECISION_FP64, PRECISION_FP32_IR = 0, 1
+PARILU_JACOBI, PARILU_INPLACE = 0, 1
 ILU_SCHED_AUTO, ILU_SCHED_ROWS, ILU_SCHED_ENTRIES, ILU_SCHED_ERAS, ILU_SCHED_STREAMS = 0, 1, 2, 3, 4
 
 # every symbol include/bilu.h declares
@@ -59,6 +60,7 @@ class Opts(C.Structure):
         ("max_refine", C.c_int),
         ("parilu_stop_tol", C.c_double),
         ("isai_power", C.c_int),
+        ("parilu_order", C.c_int),
     ]
 
 
@@ -131,9 +133,10 @@ def _check(rc, what):
 def opts(precond=ILU0, parilu_sweeps=5, tol=1e-11, tol_mode=TOL_ABSOLUTE, max_iter=500,
          merge=MERGED, true_res_restart=0, max_restarts=3, kernel=KERNEL_AUTO,
          precision=PRECISION_FP64, inner_tol=1e-5, max_refine=10, parilu_stop_tol=0.0,
-         isai_power=1) -> Opts:
+         isai_power=1, parilu_order=PARILU_JACOBI) -> Opts:
     return Opts(precond, parilu_sweeps, tol, tol_mode, max_iter, merge, true_res_restart,
-                max_restarts, kernel, precision, inner_tol, max_refine, parilu_stop_tol, isai_power)
+                max_restarts, kernel, precision, inner_tol, max_refine, parilu_stop_tol, isai_power,
+                parilu_order)
 
 
 def _ptr(t):

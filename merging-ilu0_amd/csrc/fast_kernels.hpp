@@ -321,6 +321,83 @@ __device__ __forceinline__ double parilu_rounds(const uint32_t *__restrict__ pp,
     return rr;
 }
 
+// One in-place ParILU(0) sweep (parILU_0_gpu.cu:18-57 formula; the
+// reference also sweeps in place, in an unspecified order) over the same
+// lane-synchronous rounds: round r reads F as the earlier rounds of the sweep
+// left it and writes its entries at its last step, so its lanes read the
+// round's starting values.  A wave's LDS instructions execute in program
+// order, so the loads of the steps after a round end are issued after that
+// round's writes: at each segment start (the batch start, or the step after
+// a round end) the operands of every step up to the next round end in the
+// batch are loaded together.  The round ends of a batch are wave-uniform
+// (bit 0 of every lane's record).  Returns the lane's sum of squared
+// residuals, each as its round saw it (RES), or 0.
+template <bool RES>
+__device__ __forceinline__ double parilu_inplace(const uint32_t *__restrict__ pp, const uint32_t *__restrict__ pf,
+                                                 int R, int PS, const double *__restrict__ Ap, double *F, int lane)
+{
+    constexpr int K = bilu::PAR_BATCH, D = PAR_AHEAD;
+    char *Fb = reinterpret_cast<char *>(F);
+    auto ld = [&](uint32_t off) { return *reinterpret_cast<const double *>(Fb + off); };
+    auto frec = [&](int r) { return pf[(r < R ? r : R - 1) * WAVE + lane]; };
+    uint32_t fq[D + 1];
+    double aq[D];
+#pragma unroll
+    for (int k = 0; k <= D; k++) fq[k] = frec(k);
+#pragma unroll
+    for (int k = 0; k < D; k++) aq[k] = Ap[(fq[k] & 0xFFF8u) >> 3];
+    int r = 0;
+    uint32_t ra[K];
+#pragma unroll
+    for (int j = 0; j < K; j++) ra[j] = pp[j * WAVE + lane];
+    double acc = 0.0, rr = 0.0;
+    for (int s0 = 0; s0 < PS; s0 += K) {
+        const int nx = (s0 + K < PS) ? s0 + K : s0;
+        uint32_t na[K];
+#pragma unroll
+        for (int j = 0; j < K; j++) na[j] = pp[(nx + j) * WAVE + lane];
+        uint32_t fin = 0;
+#pragma unroll
+        for (int j = 0; j < K; j++) fin |= (__builtin_amdgcn_readfirstlane(ra[j]) & 1u) << j;
+        double lv[K], uv[K];
+#pragma unroll
+        for (int j = 0; j < K; j++) {
+            if (j == 0 || ((fin >> (j - 1)) & 1u)) {   // segment start: load up to the next round end
+#pragma unroll
+                for (int t = j; t < K; t++) {
+                    lv[t] = ld(ra[t] & 0xFFF8u);
+                    uv[t] = ld(ra[t] >> 16);
+                    if ((fin >> t) & 1u) break;
+                }
+            }
+            acc = fma(lv[j], uv[j], acc);
+            if ((fin >> j) & 1u) {   // the round's last step (all lanes)
+                const uint32_t f = fq[0], pos = f & 0xFFF8u, pv = f >> 16;
+                const bool valid = (f & 1u) != 0, div = pv != 0xFFFFu;
+                const double old = ld(pos), dv = ld(div ? pv : pos);
+                const double t = aq[0] - acc;
+                const double val = div ? t / dv : t;
+                if (RES) {
+                    const double res = div ? t - old * dv : t - old;
+                    rr = valid ? fma(res, res, rr) : rr;
+                }
+                if (valid) *reinterpret_cast<double *>(Fb + pos) = val;
+                acc = 0.0;
+                r++;
+#pragma unroll
+                for (int k = 0; k < D; k++) fq[k] = fq[k + 1];
+                fq[D] = frec(r + D);
+#pragma unroll
+                for (int k = 0; k + 1 < D; k++) aq[k] = aq[k + 1];
+                aq[D - 1] = Ap[(fq[D - 1] & 0xFFF8u) >> 3];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < K; j++) ra[j] = na[j];
+    }
+    return rr;
+}
+
 // Jacobi ParILU(0) sweep (parILU_0_gpu.cu:18-57 formula) entry by entry
 // (entry idx on lane idx % 64), W = the old iterate; entries without work
 // keep a_ij.  The register-lean form for the merged FP64 kernel, whose slot

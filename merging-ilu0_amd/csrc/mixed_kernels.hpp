@@ -75,14 +75,15 @@ __global__ __launch_bounds__(KF_MAX_WAVES * WAVE) void k_factor_fast(
     FastParams fp, int n, int nnz, int batch, const double *__restrict__ gA,
     const double *__restrict__ gFinit, TO *__restrict__ gF, int mode, int sweeps, double stop_tol,
     int *__restrict__ g_status, int isai, float *__restrict__ gM, int *__restrict__ gZ,
-    const int *__restrict__ sysidx, const float *__restrict__ gY, int P3)
+    const int *__restrict__ sysidx, const float *__restrict__ gY, int P3, int inplace)
 {
     // job j factors system sysidx[j] (identity when sysidx is null); outputs are per job
     extern __shared__ __align__(16) unsigned char smem[];
     const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
     // per wave: F[nnz + 4] FP64 (F[nnz] = 0, F[nnz + 1] = 1, F[nnz + 2] scratch),
-    // then the ParILU buffer W[nnz + 2] FP64 and/or the ISAI M[nnz + 4] FP32
-    // (M[nnz] = 0, M[nnz + 1] = 1, M[nnz + 2] = -1, M[nnz + 3] scratch) sharing one region
+    // then the Jacobi ParILU buffer W[nnz + 2] FP64 (not for in-place sweeps)
+    // and/or the ISAI M[nnz + 4] FP32 (M[nnz] = 0, M[nnz + 1] = 1, M[nnz + 2] = -1,
+    // M[nnz + 3] scratch) sharing one region
     // ParILU with room for it: the sweep records (products, then finishes)
     // in a shared region ahead of the wave regions, read by LDS instructions
     // instead of global loads (fp.shared_bytes = 0 otherwise)
@@ -116,84 +117,95 @@ __global__ __launch_bounds__(KF_MAX_WAVES * WAVE) void k_factor_fast(
             if (gY) {
                 // the warm start from the network's raw outputs y (row sys of gY):
                 // guess = SIGMA sinh(YS (base + y)) for the entries with work, the
-                // same operations as bilu_mlp_infer (guess_from); from A (in F) into W
-                // first, then picked, so every base reads an unmodified a_jj
-                // (an entry record carries the position of a_jj for a lower entry,
-                // the base guess's pivot)
+                // same operations as bilu_mlp_infer (guess_from), written into F in
+                // place: the lower entries first (each reads its own a_ij and the
+                // still unmodified a_jj, the base guess's pivot; an entry record
+                // carries the position of a_jj for a lower entry), then the upper ones
                 const float *Yp = gY + (size_t)sys * P3;
-                for (int i0 = lane; i0 < fp.ne; i0 += 8 * WAVE) {
-                    int pos[8], pv[8];
-                    float y[8];
+#pragma unroll 1
+                for (int pass = 0; pass < 2; pass++) {
+                    for (int i0 = lane; i0 < fp.ne; i0 += 8 * WAVE) {
+                        int pos[8], pv[8];
+                        float y[8];
 #pragma unroll
-                    for (int j = 0; j < 8; j++) {
-                        const bool in = i0 + j * WAVE < fp.ne;
-                        const ushort4 r = in ? fp.etab[i0 + j * WAVE] : ushort4{0, 0, 0, 0xFFFF};
-                        pos[j] = in ? (int)r.x : -1;
-                        pv[j] = r.w != 0xFFFF ? (int)r.w : -1;
-                    }
-#pragma unroll
-                    for (int j = 0; j < 8; j++) y[j] = pos[j] >= 0 ? Yp[pos[j]] : 0.0f;
-#pragma unroll
-                    for (int j = 0; j < 8; j++)
-                        if (pos[j] >= 0) {
-                            const bool lower = pv[j] >= 0;
-                            W[pos[j]] = bilu_mlp::guess_from((float)F[pos[j]], lower ? (float)F[pv[j]] : 1.0f,
-                                                             lower, y[j]);
+                        for (int j = 0; j < 8; j++) {
+                            const bool in = i0 + j * WAVE < fp.ne;
+                            const ushort4 r = in ? fp.etab[i0 + j * WAVE] : ushort4{0, 0, 0, 0xFFFF};
+                            pv[j] = r.w != 0xFFFF ? (int)r.w : -1;
+                            pos[j] = (in && (pv[j] >= 0) == (pass == 0)) ? (int)r.x : -1;
                         }
-                }
-                wave_sync();
-                for (int i0 = lane; i0 < fp.ne; i0 += 8 * WAVE) {
-                    int pos[8];
 #pragma unroll
-                    for (int j = 0; j < 8; j++) pos[j] = (i0 + j * WAVE < fp.ne) ? fp.etab[i0 + j * WAVE].x : -1;
+                        for (int j = 0; j < 8; j++) y[j] = pos[j] >= 0 ? Yp[pos[j]] : 0.0f;
 #pragma unroll
-                    for (int j = 0; j < 8; j++)
-                        if (pos[j] >= 0) F[pos[j]] = W[pos[j]];
+                        for (int j = 0; j < 8; j++)
+                            if (pos[j] >= 0) {
+                                const bool lower = pv[j] >= 0;
+                                F[pos[j]] = bilu_mlp::guess_from((float)F[pos[j]], lower ? (float)F[pv[j]] : 1.0f,
+                                                                 lower, y[j]);
+                            }
+                    }
+                    wave_sync();
                 }
             } else if (gFinit) {
-                // entries with work start from the guess; the others are exact
-                // (= a).  The guess row comes in through W with coalesced loads.
+                // entries with work start from the guess; the others are exact (= a)
                 const double *Ip = gFinit + (size_t)(sysidx ? sysidx[sys] : sys) * nnz;
-                wave_copy(W, Ip, nnz, lane);
-                wave_sync();
                 for (int i0 = lane; i0 < fp.ne; i0 += 8 * WAVE) {
                     int pos[8];
+                    double g[8];
 #pragma unroll
                     for (int j = 0; j < 8; j++) pos[j] = (i0 + j * WAVE < fp.ne) ? fp.etab[i0 + j * WAVE].x : -1;
 #pragma unroll
+                    for (int j = 0; j < 8; j++) g[j] = pos[j] >= 0 ? Ip[pos[j]] : 0.0;
+#pragma unroll
                     for (int j = 0; j < 8; j++)
-                        if (pos[j] >= 0) F[pos[j]] = W[pos[j]];
+                        if (pos[j] >= 0) F[pos[j]] = g[j];
                 }
             }
             wave_sync();
             const double lim2 = stop_tol * stop_tol * anorm2;
             int used = 0;
             bool met = stop_tol <= 0.0;
-            // Jacobi sweeps between F and W (entries without work and the zero cell in both)
-            wave_copy(W, F, nnz, lane);
-            if (lane == 0) W[nnz] = 0.0;
-            wave_sync();
-            double *src = F, *dst = W;
-            while (used < sweeps) {
-                const double rr = wave_sum(fp.par_rounds > 0
-                    ? (par_lds ? parilu_rounds<true>(par_l, par_l + fp.par_psteps * WAVE, fp.par_rounds,
-                                                     fp.par_psteps, Ap, src, dst, lane)
-                               : parilu_rounds<true>(fp.parp, fp.parf, fp.par_rounds, fp.par_psteps, Ap, src, dst,
-                                                     lane))
-                    : 0.0);
-                wave_sync();
-                used++;
-                if (stop_tol > 0.0 && rr <= lim2) {
-                    // the iterate this sweep started from (src) meets the tolerance: keep it
-                    used--;
-                    met = true;
-                    break;
+            if (inplace) {
+                // in-place sweeps over F; the stop test uses the residuals the
+                // rounds saw, and the iterate after the sweep is kept
+                while (used < sweeps) {
+                    const double rr = wave_sum(fp.par_rounds > 0
+                        ? (par_lds ? parilu_inplace<true>(par_l, par_l + fp.par_psteps * WAVE, fp.par_rounds,
+                                                          fp.par_psteps, Ap, F, lane)
+                                   : parilu_inplace<true>(fp.parp, fp.parf, fp.par_rounds, fp.par_psteps, Ap, F,
+                                                          lane))
+                        : 0.0);
+                    wave_sync();
+                    used++;
+                    if (stop_tol > 0.0 && rr <= lim2) { met = true; break; }
                 }
-                double *tmp = src; src = dst; dst = tmp;
-            }
-            if (src != F) {
-                wave_copy(F, src, nnz, lane);
+            } else {
+                // Jacobi sweeps between F and W (entries without work and the zero cell in both)
+                wave_copy(W, F, nnz, lane);
+                if (lane == 0) W[nnz] = 0.0;
                 wave_sync();
+                double *src = F, *dst = W;
+                while (used < sweeps) {
+                    const double rr = wave_sum(fp.par_rounds > 0
+                        ? (par_lds ? parilu_rounds<true>(par_l, par_l + fp.par_psteps * WAVE, fp.par_rounds,
+                                                         fp.par_psteps, Ap, src, dst, lane)
+                                   : parilu_rounds<true>(fp.parp, fp.parf, fp.par_rounds, fp.par_psteps, Ap, src,
+                                                         dst, lane))
+                        : 0.0);
+                    wave_sync();
+                    used++;
+                    if (stop_tol > 0.0 && rr <= lim2) {
+                        // the iterate this sweep started from (src) meets the tolerance: keep it
+                        used--;
+                        met = true;
+                        break;
+                    }
+                    double *tmp = src; src = dst; dst = tmp;
+                }
+                if (src != F) {
+                    wave_copy(F, src, nnz, lane);
+                    wave_sync();
+                }
             }
             if (!met) {
                 // fallback: exact ILU(0)

@@ -535,10 +535,13 @@ void build_fast_plan(PatternAnalysis &pa)
         std::vector<int> ord(fp.ne);
         for (int q = 0; q < fp.ne; q++) ord[q] = q;
         auto cnt = [&](int q) { return int(fp.etab[4 * q + 2]) - int(fp.etab[4 * q + 1]); };
-        std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return cnt(x) > cnt(y); });
+        std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return cnt(x) < cnt(y); });
         const int R = (fp.ne + 63) / 64;
         std::vector<int> base(R + 1, 0);
-        for (int r = 0; r < R; r++) base[r + 1] = base[r] + std::max(1, cnt(ord[64 * r]));
+        for (int r = 0; r < R; r++) {
+            const int last = std::min(fp.ne, 64 * (r + 1)) - 1;   // the round's largest count
+            base[r + 1] = base[r] + std::max(1, cnt(ord[last]));
+        }
         const int PS = std::max(PAR_BATCH, (base[R] + PAR_BATCH - 1) / PAR_BATCH * PAR_BATCH);
         fp.par_rounds = R;
         fp.par_psteps = PS;

@@ -75,9 +75,10 @@ struct FastPlan {
     // ILU_BATCH steps)
     std::vector<uint32_t> ilu0, ilu1;
     int ilu_steps = 0;
-    // one Jacobi ParILU(0) sweep in lane-synchronous rounds: the entries with
-    // work sorted by product count (most first, stable), entry q of that
-    // order on lane q % 64 in round q / 64.  Products [step][64] (round r
+    // one ParILU(0) sweep (Jacobi or in place) in lane-synchronous rounds:
+    // the entries with work sorted by product count (fewest first, stable:
+    // an in-place sweep then meets most entries after their operands),
+    // entry q of that order on lane q % 64 in round q / 64.  Products [step][64] (round r
     // takes max(1, its largest count) steps; bit 0 marks a round's last
     // step), padded to a multiple of PAR_BATCH; finish records [round][64].
     std::vector<uint32_t> parp, parf;

@@ -1256,7 +1256,7 @@ template <typename TO>
 int launch_factor_fast(bilu_pattern_s &h, int batch, const double *A, const double *F_init, TO *F,
                        int mode, int sweeps, double stop_tol, int *status, hipStream_t st,
                        int isai = 0, float *M = nullptr, int *Z = nullptr, const int *sysidx = nullptr,
-                       int max_waves = KF_MAX_WAVES, const float *Y = nullptr, int P3 = 0)
+                       int max_waves = KF_MAX_WAVES, const float *Y = nullptr, int P3 = 0, int inplace = 0)
 {
     const auto &pl = h.pa.fast;
     if (!pl.ok) return BILU_ERR_TOO_LARGE;
@@ -1264,8 +1264,8 @@ int launch_factor_fast(bilu_pattern_s &h, int batch, const double *A, const doub
     FastParams fp{};
     fast_tables(h, fp);
     fp.counter = h.d_counter + 32;
-    // F (FP64), then the ParILU buffer (FP64) or / and the ISAI values (FP32) in one region
-    const size_t second = std::max<size_t>(mode == FS_PARILU ? (size_t)(nnz + 2) * sizeof(double) : 0,
+    // F (FP64), then the Jacobi ParILU buffer (FP64) or / and the ISAI values (FP32) in one region
+    const size_t second = std::max<size_t>(mode == FS_PARILU && !inplace ? (size_t)(nnz + 2) * sizeof(double) : 0,
                                            isai ? (size_t)(nnz + 4) * sizeof(float) : 0);
     fp.wave_bytes = int(((size_t)(nnz + 4) * sizeof(double) + second + 15) / 16 * 16);
     const int wcap = std::min(max_waves, KF_MAX_WAVES);
@@ -1284,7 +1284,7 @@ int launch_factor_fast(bilu_pattern_s &h, int batch, const double *A, const doub
     if (lds > 65536) HIP_TRY(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, st, fp.counter);
     hipLaunchKernelGGL(kern, dim3(grid), dim3(waves * WAVE), lds, st, fp, h.pa.n, nnz, batch, A, F_init, F,
-                       mode, sweeps, stop_tol, status, isai, M, Z, sysidx, Y, P3);
+                       mode, sweeps, stop_tol, status, isai, M, Z, sysidx, Y, P3, inplace);
     HIP_TRY(hipGetLastError());
     return BILU_OK;
 }
@@ -1417,6 +1417,7 @@ void bilu_opts_default(bilu_opts *o)
     o->max_refine = 10;
     o->parilu_stop_tol = 0.0;
     o->isai_power = 1;
+    o->parilu_order = BILU_PARILU_JACOBI;
 }
 
 const char *bilu_version(void) { return "bilu 0.1 gfx950"; }
@@ -1588,7 +1589,8 @@ int bilu_solve_ex(bilu_pattern_t h, int batch, const double *d_A, const double *
         o.parilu_sweeps < 0 || !(o.tol >= 0.0) ||
         (o.precond == BILU_PRECOND_FACTORS && (!d_F_init || o.precision != BILU_PRECISION_FP64)) ||
         (o.precision != BILU_PRECISION_FP64 && o.precision != BILU_PRECISION_FP32_IR) ||
-        !(o.parilu_stop_tol >= 0.0) || o.isai_power < 1 || o.isai_power > 16)
+        !(o.parilu_stop_tol >= 0.0) || o.isai_power < 1 || o.isai_power > 16 ||
+        (o.parilu_order != BILU_PARILU_JACOBI && o.parilu_order != BILU_PARILU_INPLACE))
         return BILU_ERR_INVALID_ARG;
     if (o.precision == BILU_PRECISION_FP32_IR && (!(o.inner_tol > 0.0 && o.inner_tol < 1.0) || o.max_refine < 1))
         return BILU_ERR_INVALID_ARG;
@@ -1614,8 +1616,9 @@ int bilu_solve_ex(bilu_pattern_t h, int batch, const double *d_A, const double *
     const bool lu = o.precond == BILU_PRECOND_ILU0 || o.precond == BILU_PRECOND_PARILU0;
     const bool isai = o.precond == BILU_PRECOND_ILU0_ISAI || o.precond == BILU_PRECOND_PARILU0_ISAI;
     const bool par = o.precond == BILU_PRECOND_PARILU0 || o.precond == BILU_PRECOND_PARILU0_ISAI;
+    const bool inplace = par && o.parilu_order == BILU_PARILU_INPLACE;
     const bool warm = par && (d_F_init || o.parilu_stop_tol > 0.0);
-    const bool split = o.precision == BILU_PRECISION_FP32_IR || warm || d_factor_status;
+    const bool split = o.precision == BILU_PRECISION_FP32_IR || warm || d_factor_status || inplace;
     if (o.precond == BILU_PRECOND_FACTORS) {
         // caller's factors: the solve launch only (persistent kernel when the pattern fits)
         SolveParams sp{o.tol, o.tol_mode, o.max_iter, o.true_res_restart, o.max_restarts, FK_LOAD, 0};
@@ -1662,7 +1665,8 @@ int bilu_solve_ex(bilu_pattern_t h, int batch, const double *d_A, const double *
         float *F = static_cast<float *>(ws);
         int *Z = reinterpret_cast<int *>(static_cast<char *>(ws) + fbytes);
         rc = launch_factor_fast<float>(*h, batch, d_A, mode == FS_PARILU ? d_F_init : nullptr, F, mode,
-                                       o.parilu_sweeps, stop, d_factor_status, st, isai ? 1 : 0, F, Z);
+                                       o.parilu_sweeps, stop, d_factor_status, st, isai ? 1 : 0, F, Z, nullptr,
+                                       KF_MAX_WAVES, nullptr, 0, inplace ? 1 : 0);
         if (rc != BILU_OK) return rc;
         MixedParams mp{o.tol, o.tol_mode, (float)o.inner_tol, o.max_iter, o.max_refine};
         return launch_mixed(*h, batch, d_A, d_b, d_x, F, Z, isai ? PCK_ISAI : PCK_LU, mp, d_iter, d_conv,
@@ -1672,7 +1676,8 @@ int bilu_solve_ex(bilu_pattern_t h, int batch, const double *d_A, const double *
     if (rc != BILU_OK) return rc;
     double *F = static_cast<double *>(ws);
     rc = launch_factor_fast<double>(*h, batch, d_A, mode == FS_PARILU ? d_F_init : nullptr, F, mode,
-                                    o.parilu_sweeps, stop, d_factor_status, st);
+                                    o.parilu_sweeps, stop, d_factor_status, st, 0, nullptr, nullptr, nullptr,
+                                    KF_MAX_WAVES, nullptr, 0, inplace ? 1 : 0);
     if (rc != BILU_OK) return rc;
     SolveParams sp{o.tol, o.tol_mode, o.max_iter, o.true_res_restart, o.max_restarts, FK_LOAD, 0};
     rc = -1;
@@ -1702,7 +1707,8 @@ int bilu_mlp_solve(bilu_mlp_t m, bilu_pattern_t h, int batch, const double *d_A,
     const bool isai = o.precond == BILU_PRECOND_PARILU0_ISAI;
     if ((o.precond != BILU_PRECOND_PARILU0 && !isai) || o.precision != BILU_PRECISION_FP32_IR ||
         o.parilu_sweeps < 0 || !(o.tol >= 0.0) || o.max_iter < 0 || !(o.parilu_stop_tol >= 0.0) ||
-        !(o.inner_tol > 0.0 && o.inner_tol < 1.0) || o.max_refine < 1 || o.isai_power != 1)
+        !(o.inner_tol > 0.0 && o.inner_tol < 1.0) || o.max_refine < 1 || o.isai_power != 1 ||
+        (o.parilu_order != BILU_PARILU_JACOBI && o.parilu_order != BILU_PARILU_INPLACE))
         return BILU_ERR_INVALID_ARG;
     if (!h->pa.fast.ok) return BILU_ERR_TOO_LARGE;
     const int n = h->pa.n, nnz = h->pa.nnz, nr = (n + WAVE - 1) / WAVE;
@@ -1726,7 +1732,7 @@ int bilu_mlp_solve(bilu_mlp_t m, bilu_pattern_t h, int batch, const double *d_A,
         if (rc != BILU_OK) return rc;
         rc = launch_factor_fast<float>(*h, cb, A, nullptr, F, FS_PARILU, o.parilu_sweeps, o.parilu_stop_tol,
                                        d_factor_status ? d_factor_status + s0 : nullptr, st, isai ? 1 : 0, F, Z,
-                                       nullptr, KF_MAX_WAVES, Y, P3);
+                                       nullptr, KF_MAX_WAVES, Y, P3, o.parilu_order == BILU_PARILU_INPLACE ? 1 : 0);
         if (rc != BILU_OK) return rc;
         rc = launch_mixed(*h, cb, A, d_b + (size_t)s0 * n, d_x + (size_t)s0 * n, F, Z, isai ? PCK_ISAI : PCK_LU, mp,
                           d_iter + s0, d_conv + s0, d_iter_res ? d_iter_res + s0 : nullptr,

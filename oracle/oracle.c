@@ -1029,7 +1029,9 @@ int orc_entry_order(int n, const int *rp, const int *ci, const int *diag, int *o
 
 /* The sweep order of the factor kernel's ParILU rounds: the entries with
  * work (orc_entry_order) stably sorted by their number of products
- * l_ik u_kj, most first; entry q of it is lane q % 64's in round q / 64.
+ * l_ik u_kj, fewest first (the entries early in the dependency order come
+ * first, which the in-place sweeps need); entry q of it is lane q % 64's in
+ * round q / 64.
  * Returns the entry count. */
 int orc_sweep_order(int n, const int *rp, const int *ci, const int *diag, int *order)
 {
@@ -1047,8 +1049,8 @@ int orc_sweep_order(int n, const int *rp, const int *ci, const int *diag, int *o
             cnt[e] = c;
             if (c > maxc) maxc = c;
         }
-    int w = 0;   /* by count, descending; stable in the entry order */
-    for (int c = maxc; c >= 0; c--)
+    int w = 0;   /* by count, ascending; stable in the entry order */
+    for (int c = 0; c <= maxc; c++)
         for (int q = 0; q < ne; q++)
             if (cnt[eo[q]] == c) order[w++] = eo[q];
     free(eo); free(cnt);
@@ -1066,7 +1068,7 @@ int orc_sweep_order(int n, const int *rp, const int *ci, const int *diag, int *o
  * the wave tree.  *status = sweeps used, or -1 for the fallback.  F[nnz]
  * out. */
 int orc_parilu0_stop(int n, const int *rp, const int *ci, const int *diag, const double *A,
-                     const double *init, double *F, int sweeps, double stop_tol, int *status)
+                     const double *init, double *F, int sweeps, double stop_tol, int *status, int inplace)
 {
     const int nnz = rp[n];
     int *order = (int *)malloc(sizeof(int) * (size_t)(nnz > 0 ? nnz : 1));
@@ -1082,10 +1084,17 @@ int orc_parilu0_stop(int n, const int *rp, const int *ci, const int *diag, const
     for (int i = 0; i < n; i++)
         for (int e = rp[i]; e < rp[i + 1]; e++) rowof[e] = i;
     while (used < sweeps) {
+        /* Jacobi: every entry reads the iterate the sweep started from (W).
+         * In place: the sweep's rounds (64 consecutive entries of the order)
+         * run one after the other; a round reads the iterate as the earlier
+         * rounds of the sweep left it (W refreshed at every round start) and
+         * writes its entries when all of them are computed. */
         memcpy(W, F, sizeof(double) * (size_t)nnz);
         double t[64];
         for (int l = 0; l < 64; l++) t[l] = 0.0;
         for (int q = 0; q < ne; q++) {
+            if (inplace && q % 64 == 0 && q > 0)
+                for (int qq = q - 64; qq < q; qq++) W[order[qq]] = F[order[qq]];
             const int e = order[q], i = rowof[e], j = ci[e];
             const int kmax = (i < j ? i : j) - 1;
             double sum = 0.0;
@@ -1109,8 +1118,13 @@ int orc_parilu0_stop(int n, const int *rp, const int *ci, const int *diag, const
             for (int l = 0; l < off; l++) t[l] = t[l] + t[l + off];
         used++;
         if (stop_tol > 0.0 && t[0] <= lim2) {
-            memcpy(F, W, sizeof(double) * (size_t)nnz);
-            used--;
+            /* Jacobi: t is the residual of the iterate the sweep started
+             * from, which is kept.  In place: t sums the residuals each
+             * round saw; the iterate after the sweep is kept. */
+            if (!inplace) {
+                memcpy(F, W, sizeof(double) * (size_t)nnz);
+                used--;
+            }
             met = 1;
             break;
         }
@@ -1132,7 +1146,7 @@ int orc_parilu0_stop(int n, const int *rp, const int *ci, const int *diag, const
  * FP64 BiCGSTAB (availability-order solves), 1 = FP32 inner / FP64
  * refinement over the FP32-rounded factors or the FP32 ISAI. */
 typedef struct {
-    int n, nnz, b0, b1, precond, sweeps, precision, max_refine;
+    int n, nnz, b0, b1, precond, sweeps, precision, max_refine, inplace;
     double stop_tol, inner_tol;
     const int *rp, *ci, *diag, *lev;
     const double *A, *b, *init;
@@ -1154,7 +1168,7 @@ static void *split_worker(void *arg)
         int st = 0;
         if (j->precond == 1 || j->precond == 3) {
             orc_parilu0_stop(n, j->rp, j->ci, j->diag, A, j->init ? j->init + (size_t)s * nnz : NULL, F,
-                             j->sweeps, j->stop_tol, &st);
+                             j->sweeps, j->stop_tol, &st, j->inplace);
         } else {
             memcpy(F, A, sizeof(double) * (size_t)nnz);
             orc_ilu0(n, j->rp, j->ci, j->diag, F);
@@ -1184,7 +1198,8 @@ int orc_batch_solve_split(int n, int nnz, const int *rp, const int *ci, int batc
                           const double *A, const double *b, double *x, int precond, int sweeps,
                           const double *init, double stop_tol, int precision, double inner_tol,
                           int max_refine, const orc_opts *o, float *iter, int *conv,
-                          double *iter_res, double *true_res, int *refine, int *fstat, int threads)
+                          double *iter_res, double *true_res, int *refine, int *fstat, int threads,
+                          int parilu_inplace)
 {
     int *diag = (int *)malloc(sizeof(int) * (size_t)n);
     if (orc_find_diag(n, rp, ci, diag) > 0) { free(diag); return -5; }
@@ -1198,6 +1213,7 @@ int orc_batch_solve_split(int n, int nnz, const int *rp, const int *ci, int batc
         split_job *j = &jobs[t];
         j->n = n; j->nnz = nnz; j->precond = precond; j->sweeps = sweeps; j->precision = precision;
         j->max_refine = max_refine; j->stop_tol = stop_tol; j->inner_tol = inner_tol;
+        j->inplace = parilu_inplace;
         j->b0 = (int)((long long)batch * t / threads);
         j->b1 = (int)((long long)batch * (t + 1) / threads);
         j->rp = rp; j->ci = ci; j->diag = diag; j->lev = lev; j->A = A; j->b = b; j->init = init;

@@ -126,13 +126,22 @@ int orc_bicgstab_mixed_pc(int n, const int *rp, const int *ci, const int *diag,
                           double *true_res, int *refine);
 int orc_isai_lu_f32(int n, const int *rp, const int *ci, const int *diag, const double *F, float *M);
 int orc_entry_order(int n, const int *rp, const int *ci, const int *diag, int *order);
+/* ParILU(0) sweeps in the factor kernel's round order (orc_sweep_order):
+ * inplace = 0 Jacobi (every entry reads the sweep's starting iterate),
+ * inplace = 1 rounds of 64 entries one after the other, each reading the
+ * iterate the earlier rounds left (entries of one round read its start).
+ * stop_tol > 0 stops on the summed squared residual (Jacobi: of the
+ * starting iterate, which is kept; in place: as each round saw it, the
+ * iterate after the sweep is kept) and falls back to exact ILU(0). */
 int orc_parilu0_stop(int n, const int *rp, const int *ci, const int *diag, const double *A,
-                     const double *init, double *F, int sweeps, double stop_tol, int *status);
+                     const double *init, double *F, int sweeps, double stop_tol, int *status,
+                     int inplace);
 int orc_batch_solve_split(int n, int nnz, const int *rp, const int *ci, int batch,
                           const double *A, const double *b, double *x, int precond, int sweeps,
                           const double *init, double stop_tol, int precision, double inner_tol,
                           int max_refine, const orc_opts *o, float *iter, int *conv,
-                          double *iter_res, double *true_res, int *refine, int *fstat, int threads);
+                          double *iter_res, double *true_res, int *refine, int *fstat, int threads,
+                          int parilu_inplace);
 
 #ifdef __cplusplus
 }

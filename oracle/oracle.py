@@ -97,11 +97,11 @@ def lib():
         L.orc_sweep_order.argtypes = [C.c_int, i32, i32, i32, i32]
         L.orc_isai_lu_f32.argtypes = [C.c_int, i32, i32, i32, f64, f32]
         L.orc_parilu0_stop.argtypes = [C.c_int, i32, i32, i32, f64, C.c_void_p, f64, C.c_int,
-                                       C.c_double, C.POINTER(C.c_int)]
+                                       C.c_double, C.POINTER(C.c_int), C.c_int]
         L.orc_batch_solve_split.argtypes = [C.c_int, C.c_int, i32, i32, C.c_int, f64, f64, f64,
                                             C.c_int, C.c_int, C.c_void_p, C.c_double, C.c_int,
                                             C.c_double, C.c_int, C.POINTER(Opts), f32, i32, f64,
-                                            f64, i32, i32, C.c_int]
+                                            f64, i32, i32, C.c_int, C.c_int]
         _lib = L
     return _lib
 
@@ -274,22 +274,24 @@ def entry_order(pat):
 
 
 def sweep_order(pat):
-    """entries with work sorted by product count (most first), stable in
+    """entries with work sorted by product count (fewest first), stable in
     the entry order: the factor kernel's ParILU round order"""
     order = np.empty(pat.nnz, np.int32)
     ne = lib().orc_sweep_order(pat.n, _i32(pat.row_ptrs), _i32(pat.col_idx), _i32(pat.diag), order)
     return order[:ne]
 
-def parilu0_stop(pat, vals, sweeps, stop_tol=0.0, init=None):
-    """ParILU(0) sweeps with warm start / residual stop / exact fallback.
-    Returns (F, status): status = sweeps used or -1 (fallback)."""
+def parilu0_stop(pat, vals, sweeps, stop_tol=0.0, init=None, inplace=False):
+    """ParILU(0) sweeps with warm start / residual stop / exact fallback,
+    Jacobi or in place (rounds of the factor kernel's order, each reading
+    what the earlier rounds wrote).  Returns (F, status): status = sweeps
+    used or -1 (fallback)."""
     A = _f64(vals)
     F = np.empty(pat.nnz)
     st = C.c_int()
     ini = _f64(init) if init is not None else None
     lib().orc_parilu0_stop(pat.n, pat.row_ptrs, pat.col_idx, pat.diag, A,
                            ini.ctypes.data if ini is not None else None, F, sweeps, stop_tol,
-                           C.byref(st))
+                           C.byref(st), int(bool(inplace)))
     return F, st.value
 
 
@@ -298,7 +300,7 @@ PREC_FP64, PREC_FP32_IR = 0, 1
 
 def batch_solve_split(pat, vals, b, precond=ILU0, sweeps=5, init=None, stop_tol=0.0,
                       precision=PREC_FP64, inner_tol=1e-5, max_refine=10, o: Opts | None = None,
-                      threads=1):
+                      threads=1, inplace=False):
     """Split factor + solve paths (persistent factor kernel, then the FP64
     solve in availability order or the FP32-inner / FP64-refinement solve).
     Returns x, iter, conv, iter_res, true_res, refine, factor_status."""
@@ -318,7 +320,7 @@ def batch_solve_split(pat, vals, b, precond=ILU0, sweeps=5, init=None, stop_tol=
                                      {ILU0: 0, PARILU0: 1, ILU0_ISAI: 2, PARILU0_ISAI: 3}[precond], sweeps,
                                      ini.ctypes.data if ini is not None else None, stop_tol,
                                      precision, inner_tol, max_refine, C.byref(o), it, cv, ir, tr,
-                                     rf, fs, threads)
+                                     rf, fs, threads, int(bool(inplace)))
     if st != 0:
         raise RuntimeError(f"orc_batch_solve_split failed: {st}")
     return x, it, cv, ir, tr, rf, fs

@@ -37,7 +37,7 @@ def run_gpu(pat, A, b, F_init=None, **kw):
 
 def run_oracle(pat, A, b, precond=bilu.ILU0, parilu_sweeps=5, tol=1e-10, tol_mode=1,
                precision=0, inner_tol=1e-5, max_refine=10, parilu_stop_tol=0.0, init=None,
-               true_res_restart=0, max_iter=500, **_):
+               true_res_restart=0, max_iter=500, parilu_order=bilu.PARILU_JACOBI, **_):
     o = O.opts(tol=tol, tol_mode=tol_mode, max_iter=max_iter, true_res_restart=true_res_restart,
                trsv_order=O.TRSV_AVAIL)
     pc = {bilu.ILU0: O.ILU0, bilu.PARILU0: O.PARILU0, bilu.ILU0_ISAI: O.ILU0_ISAI,
@@ -45,7 +45,7 @@ def run_oracle(pat, A, b, precond=bilu.ILU0, parilu_sweeps=5, tol=1e-10, tol_mod
     return O.batch_solve_split(pat, A, b, precond=pc,
                                sweeps=parilu_sweeps, init=init, stop_tol=parilu_stop_tol,
                                precision=precision, inner_tol=inner_tol, max_refine=max_refine,
-                               o=o, threads=8)
+                               o=o, threads=8, inplace=parilu_order == bilu.PARILU_INPLACE)
 
 
 NAMES = ["x", "iters", "conv", "iter_res", "true_res", "refine", "factor_status"]
@@ -205,3 +205,41 @@ def test_mixed_options_bitwise(kw, precond):
     assert_same(g, o, f"mixed options {kw}")
     if kw.get("max_refine") == 1:
         assert (o[2] == bilu.NOT_CONVERGED).all() and (o[5] == 1).all()
+
+
+@pytest.mark.parametrize("kind,B,n", [("random", 128, 144), ("stencil5", 64, 100), ("random", 64, 32),
+                                      ("random", 64, 90)])
+@pytest.mark.parametrize("precision,precond", [(bilu.PRECISION_FP64, bilu.PARILU0),
+                                               (bilu.PRECISION_FP32_IR, bilu.PARILU0),
+                                               (bilu.PRECISION_FP32_IR, bilu.PARILU0_ISAI)])
+@pytest.mark.parametrize("stop", [0.0, 1e-8])
+def test_inplace_parilu_bitwise(kind, B, n, precision, precond, stop):
+    """In-place ParILU(0) sweeps (rounds reading what the earlier rounds of
+    the sweep wrote) through the persistent factor kernel, fixed count or
+    stopped on the residual, bit for bit against the oracle's in-place sweep."""
+    print(f"case inplace parilu {kind} n={n} precision={precision} precond={precond} stop={stop}", flush=True)
+    pat, A, b = synth.batch(kind, B, n=n)
+    kw = dict(precond=precond, parilu_sweeps=40 if stop > 0 else 3, parilu_stop_tol=stop, tol=1e-10,
+              tol_mode=bilu.TOL_RELATIVE, precision=precision, parilu_order=bilu.PARILU_INPLACE)
+    g = run_gpu(pat, A, b, **kw)
+    o = run_oracle(pat, A, b, **kw)
+    assert_same(g, o, f"inplace parilu {kind} n={n}")
+    assert (o[2] == 1).all()
+    if stop > 0:
+        assert (o[6] >= 1).all()
+
+
+def test_inplace_parilu_warm_bitwise():
+    """In-place sweeps from a caller's warm start (perturbed exact factors)."""
+    print("case inplace parilu warm", flush=True)
+    pat, A, b = synth.batch("random", 128, n=144)
+    F = np.stack([O.ilu0(pat, A[s])[0] for s in range(A.shape[0])])
+    init = F * (1.0 + 1e-3 * np.random.default_rng(11).standard_normal(F.shape))
+    kw = dict(precond=bilu.PARILU0_ISAI, parilu_sweeps=40, parilu_stop_tol=1e-8, tol=1e-10,
+              tol_mode=bilu.TOL_RELATIVE, precision=bilu.PRECISION_FP32_IR, parilu_order=bilu.PARILU_INPLACE)
+    g = run_gpu(pat, A, b, F_init=init, **kw)
+    assert_same(g, run_oracle(pat, A, b, init=init, **kw), "inplace parilu warm")
+    # fewer in-place sweeps than Jacobi ones from the same start
+    kj = dict(kw, parilu_order=bilu.PARILU_JACOBI)
+    gj = run_gpu(pat, A, b, F_init=init, **kj)
+    assert g[6].mean() < gj[6].mean(), (g[6].mean(), gj[6].mean())

@@ -130,11 +130,11 @@ def test_mlp_training_gain():
     assert eg < ea
 
 
-def _fused_vs_split(P, net, dA, db, precond, sweeps=40, stop_tol=1e-8):
+def _fused_vs_split(P, net, dA, db, precond, sweeps=40, stop_tol=1e-8, order=bilu.PARILU_JACOBI):
     """bilu_mlp_solve against bilu_mlp_infer + bilu_solve_ex(F_init): every
     output bit-equal (the fused warm start runs the same FP32 operations)."""
     o = bilu.opts(precond=precond, parilu_sweeps=sweeps, parilu_stop_tol=stop_tol, tol=1e-10,
-                  tol_mode=bilu.TOL_RELATIVE, precision=bilu.PRECISION_FP32_IR)
+                  tol_mode=bilu.TOL_RELATIVE, precision=bilu.PRECISION_FP32_IR, parilu_order=order)
     x1, x2 = torch.empty_like(db), torch.empty_like(db)
     r1 = net.solve(P, dA, db, x1, o)
     r2 = bilu.solve_ex(P, dA, db, x2, o, F_init=net.infer(dA))
@@ -146,8 +146,9 @@ def _fused_vs_split(P, net, dA, db, precond, sweeps=40, stop_tol=1e-8):
 
 
 @pytest.mark.parametrize("precond", ["PARILU0", "PARILU0_ISAI"])
-def test_mlp_solve_fused_matches_split(precond):
-    print(f"case mlp fused solve {precond}", flush=True)
+@pytest.mark.parametrize("order", ["PARILU_JACOBI", "PARILU_INPLACE"])
+def test_mlp_solve_fused_matches_split(precond, order):
+    print(f"case mlp fused solve {precond} {order}", flush=True)
     pat, A, b = synth.batch("random", 768, first=3 << 20, n=144)
     P = bilu.Pattern.from_synth(pat)
     dA, db = to_dev(A), to_dev(b)
@@ -156,7 +157,7 @@ def test_mlp_solve_fused_matches_split(precond):
     dF = exact_factors(P, dA)
     for step in range(20):
         net.train_step(dA[:512], dF[:512], 3e-4)
-    it, cv, _, tr, _, fs = _fused_vs_split(P, net, dA, db, getattr(bilu, precond))
+    it, cv, _, tr, _, fs = _fused_vs_split(P, net, dA, db, getattr(bilu, precond), order=getattr(bilu, order))
     assert (cv.cpu().numpy() == 1).all()
     assert (tr.cpu().numpy() / np.linalg.norm(b, axis=1)).max() <= 1e-10
 

@@ -63,6 +63,57 @@ def test_parilu_stop_fallback_is_exact_ilu():
     assert st == -1 and np.array_equal(F, Fe)
 
 
+def test_inplace_one_round_equals_jacobi():
+    # with every entry with work in one round of 64, the in-place sweep reads
+    # only the round's starting values: bit for bit a Jacobi sweep
+    pat, A, _ = synth.batch("random", 3, n=16, density=0.2)
+    assert 0 < len(O.sweep_order(pat)) <= 64
+    for s in range(3):
+        for sw in (1, 2, 5):
+            Fi, si = O.parilu0_stop(pat, A[s], sw, inplace=True)
+            Fj, sj = O.parilu0_stop(pat, A[s], sw)
+            assert si == sj == sw and np.array_equal(Fi, Fj)
+
+
+def test_inplace_two_rounds_differ_from_jacobi_and_reach_ilu():
+    # more than one round: the later rounds see the earlier rounds' values, so
+    # one in-place sweep differs from a Jacobi sweep; both share the fixed
+    # point exact ILU(0)
+    pat, A, _ = synth.batch("random", 2, n=144)
+    assert len(O.sweep_order(pat)) > 64
+    for s in range(2):
+        Fi, _ = O.parilu0_stop(pat, A[s], 1, inplace=True)
+        Fj, _ = O.parilu0_stop(pat, A[s], 1)
+        assert not np.array_equal(Fi, Fj)
+        Fe, _ = O.ilu0(pat, A[s])
+        F, st = O.parilu0_stop(pat, A[s], 200, stop_tol=1e-14, inplace=True)
+        assert 0 < st < 200
+        assert np.allclose(F, Fe, rtol=1e-12, atol=0.0)
+
+
+def test_inplace_round_order_contracts_faster():
+    # in-place rounds in the plan's order (fewest products first) reach the
+    # 1e-8 factor residual in fewer sweeps than Jacobi, from A and from the
+    # base guess, and the kept iterate meets the tolerance by the residual's
+    # definition (orc_factor_residual, an independent computation)
+    pat, A, _ = synth.batch("random", 16, n=144)
+    order = O.entry_order(pat)
+    si, sj, sbi = [], [], []
+    for s in range(16):
+        F, k = O.parilu0_stop(pat, A[s], 40, stop_tol=1e-8, inplace=True)
+        assert k > 0
+        assert O.factor_residual(pat, A[s], F) <= 1e-8 * np.linalg.norm(A[s])
+        si.append(k)
+        sj.append(O.parilu0_stop(pat, A[s], 40, stop_tol=1e-8)[1])
+        base = A[s].copy()
+        rows = np.repeat(np.arange(pat.n), np.diff(pat.row_ptrs))
+        low = rows > pat.col_idx
+        base[low] = A[s][low] / A[s][pat.diag[pat.col_idx[low]]]
+        sbi.append(O.parilu0_stop(pat, A[s], 40, stop_tol=1e-8, init=base, inplace=True)[1])
+    assert np.mean(si) < np.mean(sj) - 1.5, (np.mean(si), np.mean(sj))
+    assert np.mean(sbi) <= np.mean(si)
+
+
 def test_entry_order_is_topological():
     # every entry's dependencies (ParILU pairs and the pivot) come at a lower
     # DAG level, so one pass in this order with level barriers is exact ILU(0)
