@@ -15,9 +15,10 @@
 
 constexpr int FS_ILU = 0, FS_PARILU = 1;
 
-// ISAI of the L (or U) factor in F (FP64, in-place factor layout) into M
-// (FP32 on the pattern: strictly lower positions = ISAI(L), diagonal and
-// upper = ISAI(U)), by the lane streams of the pattern plan: each step is
+// ISAI of the L (or U) factor in F (the in-place factor layout rounded to
+// FP32; the records hold 8-byte offsets, halved here) into M (FP32 on the
+// pattern: strictly lower positions = ISAI(L), diagonal and upper =
+// ISAI(U)), by the lane streams of the pattern plan: each step is
 // acc = fma(-m_c, f_q, acc) in FP64; a finishing step writes the target
 // (divided by its diagonal for U) rounded to FP32 and restarts acc.  A
 // target's m_c were written by earlier steps of the same lane, and a wave's
@@ -27,7 +28,7 @@ constexpr int FS_ILU = 0, FS_PARILU = 1;
 // diagonal): the division and the divisor's load are left out.
 template <bool DIV>
 __device__ __forceinline__ int isai_gen(const uint32_t *__restrict__ r0p, const uint32_t *__restrict__ r1p,
-                                        int steps, const double *F, float *M, float *mscratch, int lane)
+                                        int steps, const float *F, float *M, float *mscratch, int lane)
 {
     constexpr int K = bilu::ISAI_BATCH;
     const char *Fb = reinterpret_cast<const char *>(F);
@@ -49,14 +50,14 @@ __device__ __forceinline__ int isai_gen(const uint32_t *__restrict__ r0p, const 
         for (int j = 0; j < K; j++) {
             const uint32_t a = ra[j], b = rb[j];
             const double mv = (double)*reinterpret_cast<const float *>(Mb + ((a >> 16) >> 1));
-            const double fv = *reinterpret_cast<const double *>(Fb + (a & 0xFFFFu));
+            const double fv = (double)*reinterpret_cast<const float *>(Fb + ((a & 0xFFFFu) >> 1));
             acc = fma(-mv, fv, acc);
             const uint32_t p = b & 0xFFFFu, d = b >> 16;
             const bool fin = p != 0xFFFFu;
             double val = acc;
             if constexpr (DIV) {
                 const bool div = d != 0xFFFFu;
-                const double dv = *reinterpret_cast<const double *>(Fb + (div ? d : 0u));
+                const double dv = (double)*reinterpret_cast<const float *>(Fb + ((div ? d : 0u) >> 1));
                 bad |= (fin && div && dv == 0.0);
                 val = div ? acc / dv : acc;
             }
@@ -81,9 +82,10 @@ __global__ __launch_bounds__(KF_MAX_WAVES * WAVE) void k_factor_fast(
     extern __shared__ __align__(16) unsigned char smem[];
     const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
     // per wave: F[nnz + 4] FP64 (F[nnz] = 0, F[nnz + 1] = 1, F[nnz + 2] scratch),
-    // then the Jacobi ParILU buffer W[nnz + 2] FP64 (not for in-place sweeps)
-    // and/or the ISAI M[nnz + 4] FP32 (M[nnz] = 0, M[nnz + 1] = 1, M[nnz + 2] = -1,
-    // M[nnz + 3] scratch) sharing one region
+    // then the Jacobi ParILU buffer W[nnz + 2] FP64 (not for in-place sweeps).
+    // ISAI: the factors are rounded to FP32 in place when the factorization is
+    // done (the first half of F), the ISAI values M[nnz + 4] FP32 (M[nnz] = 0,
+    // M[nnz + 1] = 1, M[nnz + 2] = -1, M[nnz + 3] scratch) take the second half
     // ParILU with room for it: the sweep records (products, then finishes)
     // in a shared region ahead of the wave regions, read by LDS instructions
     // instead of global loads (fp.shared_bytes = 0 otherwise)
@@ -97,8 +99,8 @@ __global__ __launch_bounds__(KF_MAX_WAVES * WAVE) void k_factor_fast(
     }
     double *F = reinterpret_cast<double *>(smem + fp.shared_bytes + (size_t)wave * fp.wave_bytes);
     double *W = F + (nnz + 4);
-    float *M = reinterpret_cast<float *>(W);
-    if (lane == 0) { F[nnz] = 0.0; F[nnz + 1] = 1.0; }
+    float *F32 = reinterpret_cast<float *>(F);
+    float *M = F32 + (nnz + 4);
     for (;;) {
         int sys = 0;
         if (lane == 0) sys = atomicAdd(fp.counter, 1);
@@ -107,6 +109,8 @@ __global__ __launch_bounds__(KF_MAX_WAVES * WAVE) void k_factor_fast(
         const double *Ap = gA + (size_t)(sysidx ? sysidx[sys] : sys) * nnz;
         int status = 0;
         wave_copy(F, Ap, nnz, lane);
+        // the zero and one cells (the previous system's ISAI values may cover them)
+        if (lane == 0) { F[nnz] = 0.0; F[nnz + 1] = 1.0; }
         if (mode == FS_PARILU) {
             wave_sync();
             double anorm2 = 0.0;
@@ -118,33 +122,31 @@ __global__ __launch_bounds__(KF_MAX_WAVES * WAVE) void k_factor_fast(
                 // the warm start from the network's raw outputs y (row sys of gY):
                 // guess = SIGMA sinh(YS (base + y)) for the entries with work, the
                 // same operations as bilu_mlp_infer (guess_from), written into F in
-                // place: the lower entries first (each reads its own a_ij and the
-                // still unmodified a_jj, the base guess's pivot; an entry record
-                // carries the position of a_jj for a lower entry), then the upper ones
+                // place in one pass: a lower entry's pivot a_jj (the base guess
+                // divides by it; an entry record carries its position) is read from
+                // A in global memory, which the pass never modifies.  WK entries per
+                // lane are in flight: their records, then their outputs and pivots.
+                constexpr int WK = 12;
                 const float *Yp = gY + (size_t)sys * P3;
-#pragma unroll 1
-                for (int pass = 0; pass < 2; pass++) {
-                    for (int i0 = lane; i0 < fp.ne; i0 += 8 * WAVE) {
-                        int pos[8], pv[8];
-                        float y[8];
+                for (int i0 = lane; i0 < fp.ne; i0 += WK * WAVE) {
+                    int pos[WK], pv[WK];
 #pragma unroll
-                        for (int j = 0; j < 8; j++) {
-                            const bool in = i0 + j * WAVE < fp.ne;
-                            const ushort4 r = in ? fp.etab[i0 + j * WAVE] : ushort4{0, 0, 0, 0xFFFF};
-                            pv[j] = r.w != 0xFFFF ? (int)r.w : -1;
-                            pos[j] = (in && (pv[j] >= 0) == (pass == 0)) ? (int)r.x : -1;
-                        }
-#pragma unroll
-                        for (int j = 0; j < 8; j++) y[j] = pos[j] >= 0 ? Yp[pos[j]] : 0.0f;
-#pragma unroll
-                        for (int j = 0; j < 8; j++)
-                            if (pos[j] >= 0) {
-                                const bool lower = pv[j] >= 0;
-                                F[pos[j]] = bilu_mlp::guess_from((float)F[pos[j]], lower ? (float)F[pv[j]] : 1.0f,
-                                                                 lower, y[j]);
-                            }
+                    for (int j = 0; j < WK; j++) {
+                        const bool in = i0 + j * WAVE < fp.ne;
+                        const ushort4 r = in ? fp.etab[i0 + j * WAVE] : ushort4{0, 0, 0, 0xFFFF};
+                        pos[j] = in ? (int)r.x : -1;
+                        pv[j] = r.w != 0xFFFF ? (int)r.w : -1;
                     }
-                    wave_sync();
+                    float y[WK], ajj[WK];
+#pragma unroll
+                    for (int j = 0; j < WK; j++) {
+                        y[j] = pos[j] >= 0 ? Yp[pos[j]] : 0.0f;
+                        ajj[j] = pv[j] >= 0 ? (float)Ap[pv[j]] : 1.0f;
+                    }
+#pragma unroll
+                    for (int j = 0; j < WK; j++)
+                        if (pos[j] >= 0)
+                            F[pos[j]] = bilu_mlp::guess_from((float)F[pos[j]], ajj[j], pv[j] >= 0, y[j]);
                 }
             } else if (gFinit) {
                 // entries with work start from the guess; the others are exact (= a)
@@ -221,10 +223,21 @@ __global__ __launch_bounds__(KF_MAX_WAVES * WAVE) void k_factor_fast(
             ilu_stream(fp.ilu0, fp.ilu1, fp.ilu_steps, F, F + nnz + 2, fp.diag, n, lane);
         }
         if (isai) {
+            // round the factors (and the zero / one cells) to FP32 in place,
+            // 64 values at a time: a chunk's values are all read before its
+            // writes, which cover only the halves of chunks already read
+            wave_sync();
+            for (int e0 = 0; e0 < nnz + 4; e0 += WAVE) {
+                const int e = e0 + lane;
+                const double v = e < nnz + 4 ? F[e] : 0.0;
+                asm volatile("" ::: "memory");
+                if (e < nnz + 4) F32[e] = (float)v;
+                asm volatile("" ::: "memory");
+            }
             if (lane == 0) { M[nnz] = 0.0f; M[nnz + 1] = 1.0f; M[nnz + 2] = -1.0f; }
             wave_sync();
-            int bad = isai_gen<false>(fp.isL0, fp.isL1, fp.isL_steps, F, M, M + nnz + 3, lane);
-            bad |= isai_gen<true>(fp.isU0, fp.isU1, fp.isU_steps, F, M, M + nnz + 3, lane);
+            int bad = isai_gen<false>(fp.isL0, fp.isL1, fp.isL_steps, F32, M, M + nnz + 3, lane);
+            bad |= isai_gen<true>(fp.isU0, fp.isU1, fp.isU_steps, F32, M, M + nnz + 3, lane);
             wave_sync();
             float *Mo = gM + (size_t)sys * nnz;
             wave_copy(Mo, M, nnz, lane);

@@ -1264,9 +1264,9 @@ int launch_factor_fast(bilu_pattern_s &h, int batch, const double *A, const doub
     FastParams fp{};
     fast_tables(h, fp);
     fp.counter = h.d_counter + 32;
-    // F (FP64), then the Jacobi ParILU buffer (FP64) or / and the ISAI values (FP32) in one region
-    const size_t second = std::max<size_t>(mode == FS_PARILU && !inplace ? (size_t)(nnz + 2) * sizeof(double) : 0,
-                                           isai ? (size_t)(nnz + 4) * sizeof(float) : 0);
+    // F (FP64; the ISAI values share it once the factors are rounded to FP32), then
+    // the Jacobi ParILU buffer (FP64)
+    const size_t second = mode == FS_PARILU && !inplace ? (size_t)(nnz + 2) * sizeof(double) : 0;
     fp.wave_bytes = int(((size_t)(nnz + 4) * sizeof(double) + second + 15) / 16 * 16);
     const int wcap = std::min(max_waves, KF_MAX_WAVES);
     int waves = (int)std::min<size_t>(wcap, h.max_lds / fp.wave_bytes);

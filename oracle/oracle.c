@@ -826,8 +826,9 @@ static void lu_solve_avail_f(int n, const int *rp, const int *ci, const int *dia
 
 /* ISAI(L) and ISAI(U) of an in-place factor F (power-1 patterns) for the
  * FP32 preconditioner: the column sweeps of orc_isai (isai_gpu.cu:325-389)
- * with every finished m_c rounded to FP32 before it is used, and updates
- * only from entries of the pattern.  M[rp[i] .. diag[i]) = ISAI(L) (its unit
+ * over the factor rounded to FP32, in FP64 arithmetic, with every finished
+ * m_c rounded to FP32 before it is used, and updates only from entries of
+ * the pattern.  M[rp[i] .. diag[i]) = ISAI(L) (its unit
  * diagonal implicit), M[diag[i] .. rp[i+1]) = ISAI(U).  Returns
  * ORC_ZERO_PIVOT if a U diagonal is 0. */
 int orc_isai_lu_f32(int n, const int *rp, const int *ci, const int *diag, const double *F, float *M)
@@ -843,7 +844,7 @@ int orc_isai_lu_f32(int n, const int *rp, const int *ci, const int *diag, const 
             sol[c] = (double)(float)sol[c];   /* L(J_c, J_c) = 1 */
             for (int t = c - 1; t >= 0; t--) {
                 const int q = find_pos(rp, ci, ci[b + c], ci[b + t]);
-                if (q >= 0) sol[t] = fma(-sol[c], F[q], sol[t]);
+                if (q >= 0) sol[t] = fma(-sol[c], (double)(float)F[q], sol[t]);
             }
         }
         for (int t = 0; t < m - 1; t++) M[b + t] = (float)sol[t];
@@ -852,12 +853,12 @@ int orc_isai_lu_f32(int n, const int *rp, const int *ci, const int *diag, const 
         if (m > 64) return -4;
         for (int t = 0; t < m; t++) sol[t] = (t == 0) ? 1.0 : 0.0;
         for (int c = 0; c < m; c++) {
-            const double dv = F[diag[ci[b + c]]];
+            const double dv = (double)(float)F[diag[ci[b + c]]];
             if (dv == 0.0) bad = 1;
             sol[c] = (double)(float)(sol[c] / dv);
             for (int t = c + 1; t < m; t++) {
                 const int q = find_pos(rp, ci, ci[b + c], ci[b + t]);
-                if (q >= 0) sol[t] = fma(-sol[c], F[q], sol[t]);
+                if (q >= 0) sol[t] = fma(-sol[c], (double)(float)F[q], sol[t]);
             }
         }
         for (int t = 0; t < m; t++) M[b + t] = (float)sol[t];
