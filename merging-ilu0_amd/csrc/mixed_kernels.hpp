@@ -101,6 +101,8 @@ __global__ __launch_bounds__(KF_MAX_WAVES * WAVE) void k_factor_fast(
     double *W = F + (nnz + 4);
     float *F32 = reinterpret_cast<float *>(F);
     float *M = F32 + (nnz + 4);
+    // the fused warm start's diagonal of A (FP32), after F (and W)
+    float *Dg = reinterpret_cast<float *>(reinterpret_cast<unsigned char *>(F) + fp.wave_bytes) - fp.dg_floats;
     for (;;) {
         int sys = 0;
         if (lane == 0) sys = atomicAdd(fp.counter, 1);
@@ -120,33 +122,33 @@ __global__ __launch_bounds__(KF_MAX_WAVES * WAVE) void k_factor_fast(
             }
             if (gY) {
                 // the warm start from the network's raw outputs y (row sys of gY):
-                // guess = SIGMA sinh(YS (base + y)) for the entries with work, the
-                // same operations as bilu_mlp_infer (guess_from), written into F in
-                // place in one pass: a lower entry's pivot a_jj (the base guess
-                // divides by it; an entry record carries its position) is read from
-                // A in global memory, which the pass never modifies.  WK entries per
-                // lane are in flight: their records, then their outputs and pivots.
-                constexpr int WK = 12;
+                // guess = SIGMA sinh(YS (base + y)) for the entries with work (a bit
+                // per position), the same operations as bilu_mlp_infer (guess_from);
+                // the others keep a_ij.  The diagonal of A goes to Dg first (FP32, as
+                // the base guess rounds it), so the positions are visited in one
+                // coalesced pass, 8 per lane in flight: a lower entry (i, j)
+                // (column j in lowcol, else 0xFFFF) reads its pivot a_jj from Dg.
+                for (int j = lane; j < n; j += WAVE) Dg[j] = (float)F[fp.diag[j]];
+                wave_sync();
                 const float *Yp = gY + (size_t)sys * P3;
-                for (int i0 = lane; i0 < fp.ne; i0 += WK * WAVE) {
-                    int pos[WK], pv[WK];
+                for (int e0 = lane; e0 < nnz; e0 += 8 * WAVE) {
+                    float y[8];
+                    uint32_t wbit[8], lc[8];
 #pragma unroll
-                    for (int j = 0; j < WK; j++) {
-                        const bool in = i0 + j * WAVE < fp.ne;
-                        const ushort4 r = in ? fp.etab[i0 + j * WAVE] : ushort4{0, 0, 0, 0xFFFF};
-                        pos[j] = in ? (int)r.x : -1;
-                        pv[j] = r.w != 0xFFFF ? (int)r.w : -1;
-                    }
-                    float y[WK], ajj[WK];
-#pragma unroll
-                    for (int j = 0; j < WK; j++) {
-                        y[j] = pos[j] >= 0 ? Yp[pos[j]] : 0.0f;
-                        ajj[j] = pv[j] >= 0 ? (float)Ap[pv[j]] : 1.0f;
+                    for (int j = 0; j < 8; j++) {
+                        const int e = e0 + j * WAVE;
+                        y[j] = e < nnz ? Yp[e] : 0.0f;
+                        wbit[j] = e < nnz ? fp.workbits[e >> 5] : 0u;
+                        lc[j] = e < nnz ? fp.lowcol[e] : 0xFFFFu;
                     }
 #pragma unroll
-                    for (int j = 0; j < WK; j++)
-                        if (pos[j] >= 0)
-                            F[pos[j]] = bilu_mlp::guess_from((float)F[pos[j]], ajj[j], pv[j] >= 0, y[j]);
+                    for (int j = 0; j < 8; j++) {
+                        const int e = e0 + j * WAVE;
+                        if (e < nnz && ((wbit[j] >> (e & 31)) & 1u)) {
+                            const bool lower = lc[j] != 0xFFFFu;
+                            F[e] = bilu_mlp::guess_from((float)F[e], lower ? Dg[lc[j]] : 1.0f, lower, y[j]);
+                        }
+                    }
                 }
             } else if (gFinit) {
                 // entries with work start from the guess; the others are exact (= a)

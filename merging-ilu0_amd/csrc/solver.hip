@@ -961,6 +961,8 @@ struct bilu_pattern_s {
     int32_t *d_isl_pos = nullptr, *d_isu_pos = nullptr;
     uint32_t *d_ilu0 = nullptr, *d_ilu1 = nullptr;
     uint32_t *d_parp = nullptr, *d_parf = nullptr;
+    uint32_t *d_workbits = nullptr;
+    uint16_t *d_lowcol = nullptr;
     int *d_counter = nullptr;      // 256 bytes of work counters: [0] solve kernels, [32] factor kernel
     int ilu_sched = 0;             // exact ILU(0) schedule chosen by bilu_ilu0_tune (0: none yet)
     // solve workspace (factors between the factor and solve launches), grown on demand
@@ -1074,6 +1076,8 @@ int upload(bilu_pattern_s &h)
         const size_t o_i1 = put(fp.ilu1.data(), fp.ilu1.size() * 4);
         const size_t o_p0 = put(fp.parp.data(), fp.parp.size() * 4);
         const size_t o_p1 = put(fp.parf.data(), fp.parf.size() * 4);
+        const size_t o_wb = put(fp.workbits.data(), fp.workbits.size() * 4);
+        const size_t o_lc = put(fp.lowcol.data(), fp.lowcol.size() * 2);
         HIP_TRY(hipMalloc((void **)&h.d_fast, blob.size()));
         HIP_TRY(hipMemcpy(h.d_fast, blob.data(), blob.size(), hipMemcpyHostToDevice));
         uint8_t *b = h.d_fast;
@@ -1088,6 +1092,8 @@ int upload(bilu_pattern_s &h)
         h.d_isl_pos = (int32_t *)(b + o_lp); h.d_isu_pos = (int32_t *)(b + o_up);
         h.d_ilu0 = (uint32_t *)(b + o_i0); h.d_ilu1 = (uint32_t *)(b + o_i1);
         h.d_parp = (uint32_t *)(b + o_p0); h.d_parf = (uint32_t *)(b + o_p1);
+        h.d_workbits = (uint32_t *)(b + o_wb);
+        h.d_lowcol = (uint16_t *)(b + o_lc);
     }
     return BILU_OK;
 }
@@ -1142,6 +1148,8 @@ void fast_tables(const bilu_pattern_s &h, FastParams &fp)
     fp.cntL = h.d_cntL; fp.cntU = h.d_cntU;
     fp.rlevL = h.d_rlevL; fp.rlevU = h.d_rlevU;
     fp.etab = reinterpret_cast<const ushort4 *>(h.d_etab); fp.eptr = h.d_eptr;
+    fp.workbits = h.d_workbits;
+    fp.lowcol = h.d_lowcol;
     fp.dep = reinterpret_cast<const ushort2 *>(h.d_dep);
     fp.diag = h.dev.diag;
     fp.nlev_l = h.dev.nlev_l; fp.nlev_u = h.dev.nlev_u; fp.nlev_e = pl.nlev_e; fp.ne = pl.ne;
@@ -1267,7 +1275,9 @@ int launch_factor_fast(bilu_pattern_s &h, int batch, const double *A, const doub
     // F (FP64; the ISAI values share it once the factors are rounded to FP32), then
     // the Jacobi ParILU buffer (FP64)
     const size_t second = mode == FS_PARILU && !inplace ? (size_t)(nnz + 2) * sizeof(double) : 0;
-    fp.wave_bytes = int(((size_t)(nnz + 4) * sizeof(double) + second + 15) / 16 * 16);
+    // the fused warm start's FP32 diagonal at the end of the region
+    fp.dg_floats = Y ? (h.pa.n + 3) / 4 * 4 : 0;
+    fp.wave_bytes = int(((size_t)(nnz + 4) * sizeof(double) + second + (size_t)fp.dg_floats * 4 + 15) / 16 * 16);
     const int wcap = std::min(max_waves, KF_MAX_WAVES);
     int waves = (int)std::min<size_t>(wcap, h.max_lds / fp.wave_bytes);
     if (waves < 1) return BILU_ERR_TOO_LARGE;
