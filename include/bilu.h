@@ -144,6 +144,16 @@ int bilu_pattern_create(int n, int nnz, const int *row_ptrs, const int *col_idx,
                         bilu_pattern_t *out);
 int bilu_pattern_destroy(bilu_pattern_t p);
 int bilu_pattern_get_info(bilu_pattern_t p, bilu_pattern_info *info);
+/* Sizes the handle's solve workspace for batches of up to `batch` systems on
+ * every solve path (the split factor + solve launches keep the factors or
+ * ISAI values there), so that no later bilu_solve / bilu_solve_ex /
+ * bilu_mlp_solve of at most that many systems allocates or waits for the
+ * device: the calls a caller times or captures into a hipGraph.  Without it
+ * the workspace grows on first use (a device synchronize and hipMalloc),
+ * and a solve under graph capture that would grow it returns
+ * BILU_ERR_INVALID_ARG instead.  No counterpart in the reference, which
+ * allocates and frees its scratch inside every solve (merged.cu:553,565). */
+int bilu_pattern_reserve(bilu_pattern_t p, int batch);
 /* Missing-diagonal insertion (factorization.cu:399-467): the pattern with an
  * explicit entry (i,i) inserted at its sorted place in every row that lacks
  * one.  Host arrays, columns sorted within each row (checked as in

@@ -36,7 +36,7 @@ EXPORTS = [
     "bilu_mlp_get_weights", "bilu_mlp_set_weights", "bilu_mlp_gradient", "bilu_mlp_train_step",
     "bilu_mlp_infer", "bilu_uneven_create", "bilu_uneven_destroy", "bilu_uneven_tune", "bilu_uneven_config",
     "bilu_uneven_solve", "bilu_pattern_add_diagonal", "bilu_expand_values", "bilu_isai_pattern",
-    "bilu_isai_power", "bilu_mlp_solve", "bilu_ilu0_ex", "bilu_ilu0_tune",
+    "bilu_isai_power", "bilu_mlp_solve", "bilu_ilu0_ex", "bilu_ilu0_tune", "bilu_pattern_reserve",
 ]
 
 
@@ -89,6 +89,7 @@ def lib():
         L.bilu_pattern_create.argtypes = [C.c_int, C.c_int, i32p, i32p, C.POINTER(vp)]
         L.bilu_pattern_destroy.argtypes = [vp]
         L.bilu_pattern_get_info.argtypes = [vp, C.POINTER(PatternInfo)]
+        L.bilu_pattern_reserve.argtypes = [vp, C.c_int]
         L.bilu_pattern_lu_split.argtypes = [vp, i32p, i32p, i32p, i32p]
         L.bilu_pattern_add_diagonal.argtypes = [C.c_int, C.c_int, i32p, i32p, i32p, i32p, i32p, i32p]
         L.bilu_expand_values.argtypes = [C.c_int, C.c_int, C.c_int, vp, vp, vp, vp]
@@ -175,6 +176,11 @@ class Pattern:
         inf = PatternInfo()
         _check(lib().bilu_pattern_get_info(self.h, C.byref(inf)), "bilu_pattern_get_info")
         return inf
+
+    def reserve(self, batch):
+        """size the solve workspace for batches of up to `batch` systems (no
+        allocation or device wait inside later solves, e.g. under graph capture)"""
+        _check(lib().bilu_pattern_reserve(self.h, int(batch)), "bilu_pattern_reserve")
 
     def lu_split(self):
         inf = self.info()

