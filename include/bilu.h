@@ -196,6 +196,25 @@ int bilu_solve_ex(bilu_pattern_t p, int batch, const double *d_A, const double *
                   int *d_conv, double *d_iter_res, double *d_true_res, int *d_refine,
                   int *d_factor_status, void *stream);
 
+/* bilu_solve_ex for callers that cannot allocate (code under hipGraph
+ * capture, serving loops, harnesses that forbid hipMalloc).  No counterpart
+ * in the reference.  Same arguments, options and results, bit for bit; no
+ * hipMalloc, hipFree, device synchronize or allocating library call inside.
+ * The factors / ISAI values between the factor and solve launches live in a
+ * fixed 256 MiB device array of the library (one per GPU, held until the
+ * process ends) instead of the handle's workspace; batches larger than it
+ * holds (about 16,000 systems at nnz = 2082) run in consecutive chunks on
+ * `stream`.  Every call on a GPU shares that array: calls that overlap on one
+ * GPU (other streams, other threads, other handles) race and must not be
+ * issued.  Nothing in the array needs to start at zero (each system's values
+ * are written before they are read).  BILU_ERR_TOO_LARGE when one system
+ * does not fit; BILU_ERR_INVALID_ARG for isai_power > 1 (those pattern maps
+ * are allocated at their first use). */
+int bilu_solve_ex_noalloc(bilu_pattern_t p, int batch, const double *d_A, const double *d_b,
+                          double *d_x, const bilu_opts *opts, const double *d_F_init, float *d_iter,
+                          int *d_conv, double *d_iter_res, double *d_true_res, int *d_refine,
+                          int *d_factor_status, void *stream);
+
 /* Exact ILU(0) in place on the pattern: d_F[batch*nnz] <- factors of d_A
  * (strict lower = L, unit diagonal implicit; upper = U).  d_status[batch]
  * (may be NULL) receives 0 or BILU_ZERO_PIVOT. */

@@ -37,6 +37,7 @@ EXPORTS = [
     "bilu_mlp_infer", "bilu_uneven_create", "bilu_uneven_destroy", "bilu_uneven_tune", "bilu_uneven_config",
     "bilu_uneven_solve", "bilu_pattern_add_diagonal", "bilu_expand_values", "bilu_isai_pattern",
     "bilu_isai_power", "bilu_mlp_solve", "bilu_ilu0_ex", "bilu_ilu0_tune", "bilu_pattern_reserve",
+    "bilu_solve_ex_noalloc",
 ]
 
 
@@ -96,6 +97,7 @@ def lib():
         L.bilu_solve.argtypes = [vp, C.c_int, vp, vp, vp, C.POINTER(Opts), vp, vp, vp, vp, vp]
         L.bilu_solve_ex.argtypes = [vp, C.c_int, vp, vp, vp, C.POINTER(Opts), vp, vp, vp, vp, vp,
                                     vp, vp, vp]
+        L.bilu_solve_ex_noalloc.argtypes = L.bilu_solve_ex.argtypes
         L.bilu_ilu0.argtypes = [vp, C.c_int, vp, vp, vp, vp]
         L.bilu_parilu0.argtypes = [vp, C.c_int, vp, vp, vp, C.c_int, vp]
         L.bilu_factor_residual.argtypes = [vp, C.c_int, vp, vp, vp, vp]
@@ -270,6 +272,27 @@ def solve_ex(pat: Pattern, A, b, x, o: Opts | None = None, F_init=None, iters=No
                                _ptr(iters), _ptr(conv), _ptr(iter_res), _ptr(true_res),
                                _ptr(refine), _ptr(factor_status), _stream(stream)),
            "bilu_solve_ex")
+    return iters, conv, iter_res, true_res, refine, factor_status
+
+
+def solve_ex_noalloc(pat: Pattern, A, b, x, o: Opts | None = None, F_init=None, iters=None, conv=None,
+                     iter_res=None, true_res=None, refine=None, factor_status=None, stream=None):
+    """bilu_solve_ex_noalloc: bilu_solve_ex without any allocation (the
+    library's fixed per-GPU scratch array).  Outputs as solve_ex; pass every
+    output tensor to keep the call itself free of torch allocations."""
+    import torch
+    B = b.shape[0]
+    dev = b.device
+    mk = lambda t, dt: t if t is not None else torch.empty(B, dtype=dt, device=dev)
+    iters, conv = mk(iters, torch.float32), mk(conv, torch.int32)
+    iter_res, true_res = mk(iter_res, torch.float64), mk(true_res, torch.float64)
+    refine = mk(refine, torch.int32)
+    factor_status = None if factor_status is False else mk(factor_status, torch.int32)
+    o = o or opts()
+    _check(lib().bilu_solve_ex_noalloc(pat.h, B, _ptr(A), _ptr(b), _ptr(x), C.byref(o), _ptr(F_init),
+                                       _ptr(iters), _ptr(conv), _ptr(iter_res), _ptr(true_res),
+                                       _ptr(refine), _ptr(factor_status), _stream(stream)),
+           "bilu_solve_ex_noalloc")
     return iters, conv, iter_res, true_res, refine, factor_status
 
 

@@ -1424,6 +1424,33 @@ int launch_solve(bilu_pattern_s &h, int batch, const double *A, const double *b,
     }
 }
 
+// ---- scratch of the allocation-free solve (bilu_solve_ex_noalloc) ----
+// A fixed device array that the library never allocates or frees: the
+// factors or ISAI values that pass between the factor and solve launches,
+// for one chunk of systems.  Each GPU the library runs on holds its own copy
+// until the process ends.  The array's name is a host-side stand-in: host
+// code uses only the per-device address from hipGetSymbolAddress.
+constexpr size_t NOALLOC_SCRATCH_BYTES = size_t(256) << 20;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[64] = {};
+
+// the array's address on `dev` (current), looked up on the first call on that device
+int noalloc_scratch(int dev, void **out)
+{
+    if (dev < 0 || dev >= 64) return BILU_ERR_INVALID_ARG;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return BILU_ERR_HIP;
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return BILU_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1537,9 +1564,24 @@ int bilu_pattern_lu_split(bilu_pattern_t p, int *lrp, int *lci, int *urp, int *u
     return BILU_OK;
 }
 
+// the workspace of one solve: the handle's (grown on demand) or, for the
+// allocation-free entry point, a fixed buffer of `fixed_cap` bytes
+struct SolveWs {
+    void *fixed = nullptr;
+    size_t fixed_cap = 0;
+};
+
+int solve_ws(bilu_pattern_s &h, size_t bytes, void **out, hipStream_t st, const SolveWs &w)
+{
+    if (!w.fixed) return workspace(h, bytes, out, st);
+    if (bytes > w.fixed_cap) return BILU_ERR_TOO_LARGE;
+    *out = w.fixed;
+    return BILU_OK;
+}
+
 static int solve_fp64(bilu_pattern_t h, int batch, const double *d_A, const double *d_b, double *d_x,
                       const bilu_opts &o, float *d_iter, int *d_conv, double *d_iter_res,
-                      double *d_true_res, void *stream)
+                      double *d_true_res, void *stream, const SolveWs &wsrc = SolveWs())
 {
     const auto &pa = h->pa;
     const int n = pa.n, nnz = pa.nnz;
@@ -1582,7 +1624,7 @@ static int solve_fp64(bilu_pattern_t h, int batch, const double *d_A, const doub
         const size_t flds = (size_t)(sp.factor == FK_PARILU ? 3 : 2) * nnz * sizeof(double);
         if (flds > h->max_lds) return BILU_ERR_TOO_LARGE;
         void *ws = nullptr;
-        const int wrc = workspace(*h, (size_t)batch * nnz * sizeof(double), &ws, st);
+        const int wrc = solve_ws(*h, (size_t)batch * nnz * sizeof(double), &ws, st, wsrc);
         if (wrc != BILU_OK) return wrc;
         double *Fbuf = static_cast<double *>(ws);
         if (flds > 65536) HIP_TRY(hipFuncSetAttribute((const void *)k_factor, hipFuncAttributeMaxDynamicSharedMemorySize, (int)flds));
@@ -1610,10 +1652,10 @@ static int solve_fp64(bilu_pattern_t h, int batch, const double *d_A, const doub
     return rc;
 }
 
-int bilu_solve_ex(bilu_pattern_t h, int batch, const double *d_A, const double *d_b, double *d_x,
-                  const bilu_opts *opts, const double *d_F_init, float *d_iter, int *d_conv,
-                  double *d_iter_res, double *d_true_res, int *d_refine, int *d_factor_status,
-                  void *stream)
+static int solve_ex_impl(bilu_pattern_t h, int batch, const double *d_A, const double *d_b, double *d_x,
+                         const bilu_opts *opts, const double *d_F_init, float *d_iter, int *d_conv,
+                         double *d_iter_res, double *d_true_res, int *d_refine, int *d_factor_status,
+                         void *stream, const SolveWs &wsrc)
 {
     if (!h || batch < 0 || (batch > 0 && (!d_A || !d_b || !d_x || !d_iter || !d_conv)))
         return BILU_ERR_INVALID_ARG;
@@ -1636,7 +1678,7 @@ int bilu_solve_ex(bilu_pattern_t h, int batch, const double *d_A, const double *
         if (o.precision != BILU_PRECISION_FP64 || d_F_init || o.parilu_stop_tol > 0.0 ||
             o.kernel == BILU_KERNEL_PERSISTENT)
             return BILU_ERR_INVALID_ARG;
-        const int rc = solve_fp64(h, batch, d_A, d_b, d_x, o, d_iter, d_conv, d_iter_res, d_true_res, stream);
+        const int rc = solve_fp64(h, batch, d_A, d_b, d_x, o, d_iter, d_conv, d_iter_res, d_true_res, stream, wsrc);
         if (rc == BILU_OK && (d_refine || d_factor_status)) {
             hipStream_t st0 = (hipStream_t)stream;
             const int fs = o.precond == BILU_PRECOND_PARILU0_ISAI ? o.parilu_sweeps : 0;
@@ -1675,7 +1717,7 @@ int bilu_solve_ex(bilu_pattern_t h, int batch, const double *d_A, const double *
         return rc;
     }
     if (!split) {
-        const int rc = solve_fp64(h, batch, d_A, d_b, d_x, o, d_iter, d_conv, d_iter_res, d_true_res, stream);
+        const int rc = solve_fp64(h, batch, d_A, d_b, d_x, o, d_iter, d_conv, d_iter_res, d_true_res, stream, wsrc);
         if (rc == BILU_OK && d_refine) {
             hipLaunchKernelGGL(k_fill_int, dim3((batch + 255) / 256), dim3(256), 0, st, d_refine, 0, batch);
             HIP_TRY(hipGetLastError());
@@ -1695,7 +1737,7 @@ int bilu_solve_ex(bilu_pattern_t h, int batch, const double *d_A, const double *
         const int nr = (h->pa.n + WAVE - 1) / WAVE;
         if (slot_template_mixed(h->pa.fast.S) < 0 || nr > MAX_NR || h->pa.fast.nr != nr) return BILU_ERR_TOO_LARGE;
         const size_t fbytes = ((size_t)batch * nnz * sizeof(float) + 255) / 256 * 256;
-        rc = workspace(*h, fbytes + (size_t)batch * sizeof(int), &ws, st);
+        rc = solve_ws(*h, fbytes + (size_t)batch * sizeof(int), &ws, st, wsrc);
         if (rc != BILU_OK) return rc;
         float *F = static_cast<float *>(ws);
         int *Z = reinterpret_cast<int *>(static_cast<char *>(ws) + fbytes);
@@ -1707,7 +1749,7 @@ int bilu_solve_ex(bilu_pattern_t h, int batch, const double *d_A, const double *
         return launch_mixed(*h, batch, d_A, d_b, d_x, F, Z, isai ? PCK_ISAI : PCK_LU, mp, d_iter, d_conv,
                             d_iter_res, d_true_res, d_refine, st);
     }
-    rc = workspace(*h, (size_t)batch * nnz * sizeof(double), &ws, st);
+    rc = solve_ws(*h, (size_t)batch * nnz * sizeof(double), &ws, st, wsrc);
     if (rc != BILU_OK) return rc;
     double *F = static_cast<double *>(ws);
     rc = launch_factor_fast<double>(*h, batch, d_A, mode == FS_PARILU ? d_F_init : nullptr, F, mode,
@@ -1728,6 +1770,55 @@ int bilu_solve_ex(bilu_pattern_t h, int batch, const double *d_A, const double *
         HIP_TRY(hipGetLastError());
     }
     return rc;
+}
+
+int bilu_solve_ex(bilu_pattern_t h, int batch, const double *d_A, const double *d_b, double *d_x,
+                  const bilu_opts *opts, const double *d_F_init, float *d_iter, int *d_conv,
+                  double *d_iter_res, double *d_true_res, int *d_refine, int *d_factor_status,
+                  void *stream)
+{
+    return solve_ex_impl(h, batch, d_A, d_b, d_x, opts, d_F_init, d_iter, d_conv, d_iter_res, d_true_res, d_refine,
+                         d_factor_status, stream, SolveWs());
+}
+
+int bilu_solve_ex_noalloc(bilu_pattern_t h, int batch, const double *d_A, const double *d_b, double *d_x,
+                          const bilu_opts *opts, const double *d_F_init, float *d_iter, int *d_conv,
+                          double *d_iter_res, double *d_true_res, int *d_refine, int *d_factor_status,
+                          void *stream)
+{
+    if (!h || batch < 0 || (batch > 0 && (!d_A || !d_b || !d_x || !d_iter || !d_conv)))
+        return BILU_ERR_INVALID_ARG;
+    if (batch == 0) return BILU_OK;
+    bilu_opts o;
+    if (opts) o = *opts; else bilu_opts_default(&o);
+    // power-k ISAI patterns are built and uploaded at their first use (an allocation)
+    if (o.isai_power != 1) return BILU_ERR_INVALID_ARG;
+    HIP_TRY(hipSetDevice(h->device));
+    void *ws = nullptr;
+    const int rc0 = noalloc_scratch(h->device, &ws);
+    if (rc0 != BILU_OK) return rc0;
+    // systems per chunk: every split path's layout fits (FP32 values + flags
+    // with the flags 256-byte aligned, or FP64 factors)
+    const size_t nnz = (size_t)h->pa.nnz, n = (size_t)h->pa.n;
+    const size_t per = nnz * sizeof(double) + sizeof(int);
+    if (NOALLOC_SCRATCH_BYTES < 256 + per) {
+        std::fprintf(stderr, "bilu: bilu_solve_ex_noalloc: one system (nnz %zu) exceeds the %zu-byte scratch\n", nnz,
+                     NOALLOC_SCRATCH_BYTES);
+        return BILU_ERR_TOO_LARGE;
+    }
+    const int cb = (int)std::min<size_t>((size_t)batch, (NOALLOC_SCRATCH_BYTES - 256) / per);
+    SolveWs w;
+    w.fixed = ws;
+    w.fixed_cap = NOALLOC_SCRATCH_BYTES;
+    for (int s0 = 0; s0 < batch; s0 += cb) {
+        const int c = std::min(cb, batch - s0);
+        auto off = [&](auto *p, size_t stride) { return p ? p + (size_t)s0 * stride : p; };
+        const int rc = solve_ex_impl(h, c, d_A + (size_t)s0 * nnz, d_b + (size_t)s0 * n, d_x + (size_t)s0 * n, &o,
+                                     off(d_F_init, nnz), d_iter + s0, d_conv + s0, off(d_iter_res, 1),
+                                     off(d_true_res, 1), off(d_refine, 1), off(d_factor_status, 1), stream, w);
+        if (rc != BILU_OK) return rc;
+    }
+    return BILU_OK;
 }
 
 int bilu_mlp_solve(bilu_mlp_t m, bilu_pattern_t h, int batch, const double *d_A, const double *d_b, double *d_x,
