@@ -116,6 +116,10 @@ typedef struct {
                              The reference's ILU-ISAI solver uses 2 (PrecondBiCGSTAB_ilu_isai_gpu.cu:
                              297-299).  k > 1: FP64, one wave per system, ISAI rows of at most 32
                              entries (the reference's row_size_limit), no warm start */
+    double inner_adapt;   /* FP32_IR: an inner solve stops at the relative residual
+                             max(inner_tol, inner_adapt * tol / |r|), |r| the FP64 residual it
+                             starts from, so the last refinement step solves no further than the
+                             outer tolerance needs (default 0.5; 0 = inner_tol always) */
     int parilu_order;     /* BILU_PARILU_JACOBI (default) / BILU_PARILU_INPLACE.  With
                              parilu_stop_tol > 0, an in-place sweep stops on the residuals its
                              rounds saw and keeps the iterate after the sweep */
@@ -214,6 +218,17 @@ int bilu_solve_ex_noalloc(bilu_pattern_t p, int batch, const double *d_A, const 
                           double *d_x, const bilu_opts *opts, const double *d_F_init, float *d_iter,
                           int *d_conv, double *d_iter_res, double *d_true_res, int *d_refine,
                           int *d_factor_status, void *stream);
+
+/* Kernel timing for benchmarks (no counterpart in the reference): between
+ * bilu_timing_begin and bilu_timing_end, the first `max_launches` launches
+ * of one kernel kind by any solve entry point are bracketed by HIP events
+ * recorded on the stream the kernel is launched on (events created by
+ * _begin).  _end waits for those events and returns the average kernel
+ * duration (ms) and the number of launches timed.  One window at a time. */
+#define BILU_TIMING_FACTOR 1   /* the persistent factorization kernel of the split paths */
+#define BILU_TIMING_SOLVE 2    /* the solve kernel (merged FP64, mixed precision) */
+int bilu_timing_begin(int kernel, int max_launches);
+int bilu_timing_end(float *avg_ms, int *launches);
 
 /* Exact ILU(0) in place on the pattern: d_F[batch*nnz] <- factors of d_A
  * (strict lower = L, unit diagonal implicit; upper = U).  d_status[batch]

@@ -25,6 +25,7 @@ ISAI_GENERAL, ISAI_LOWER, ISAI_UPPER = 0, 1, 2
 KERNEL_AUTO, KERNEL_WAVE, KERNEL_PERSISTENT = 0, 1, 2
 PRECISION_FP64, PRECISION_FP32_IR = 0, 1
 PARILU_JACOBI, PARILU_INPLACE = 0, 1
+TIMING_FACTOR, TIMING_SOLVE = 1, 2
 ILU_SCHED_AUTO, ILU_SCHED_ROWS, ILU_SCHED_ENTRIES, ILU_SCHED_ERAS, ILU_SCHED_STREAMS = 0, 1, 2, 3, 4
 
 # every symbol include/bilu.h declares
@@ -37,7 +38,7 @@ EXPORTS = [
     "bilu_mlp_infer", "bilu_uneven_create", "bilu_uneven_destroy", "bilu_uneven_tune", "bilu_uneven_config",
     "bilu_uneven_solve", "bilu_pattern_add_diagonal", "bilu_expand_values", "bilu_isai_pattern",
     "bilu_isai_power", "bilu_mlp_solve", "bilu_ilu0_ex", "bilu_ilu0_tune", "bilu_pattern_reserve",
-    "bilu_solve_ex_noalloc",
+    "bilu_solve_ex_noalloc", "bilu_timing_begin", "bilu_timing_end",
 ]
 
 
@@ -61,6 +62,7 @@ class Opts(C.Structure):
         ("max_refine", C.c_int),
         ("parilu_stop_tol", C.c_double),
         ("isai_power", C.c_int),
+        ("inner_adapt", C.c_double),
         ("parilu_order", C.c_int),
     ]
 
@@ -98,6 +100,8 @@ def lib():
         L.bilu_solve_ex.argtypes = [vp, C.c_int, vp, vp, vp, C.POINTER(Opts), vp, vp, vp, vp, vp,
                                     vp, vp, vp]
         L.bilu_solve_ex_noalloc.argtypes = L.bilu_solve_ex.argtypes
+        L.bilu_timing_begin.argtypes = [C.c_int, C.c_int]
+        L.bilu_timing_end.argtypes = [C.POINTER(C.c_float), C.POINTER(C.c_int)]
         L.bilu_ilu0.argtypes = [vp, C.c_int, vp, vp, vp, vp]
         L.bilu_parilu0.argtypes = [vp, C.c_int, vp, vp, vp, C.c_int, vp]
         L.bilu_factor_residual.argtypes = [vp, C.c_int, vp, vp, vp, vp]
@@ -136,10 +140,10 @@ def _check(rc, what):
 def opts(precond=ILU0, parilu_sweeps=5, tol=1e-11, tol_mode=TOL_ABSOLUTE, max_iter=500,
          merge=MERGED, true_res_restart=0, max_restarts=3, kernel=KERNEL_AUTO,
          precision=PRECISION_FP64, inner_tol=1e-5, max_refine=10, parilu_stop_tol=0.0,
-         isai_power=1, parilu_order=PARILU_JACOBI) -> Opts:
+         isai_power=1, parilu_order=PARILU_JACOBI, inner_adapt=0.5) -> Opts:
     return Opts(precond, parilu_sweeps, tol, tol_mode, max_iter, merge, true_res_restart,
                 max_restarts, kernel, precision, inner_tol, max_refine, parilu_stop_tol, isai_power,
-                parilu_order)
+                inner_adapt, parilu_order)
 
 
 def _ptr(t):
@@ -294,6 +298,26 @@ def solve_ex_noalloc(pat: Pattern, A, b, x, o: Opts | None = None, F_init=None, 
                                        _ptr(refine), _ptr(factor_status), _stream(stream)),
            "bilu_solve_ex_noalloc")
     return iters, conv, iter_res, true_res, refine, factor_status
+
+
+class KernelTiming:
+    """with KernelTiming(TIMING_SOLVE, n) as t: ... -> t.avg_ms, t.launches: the
+    average duration of that kernel's launches inside the block, from HIP
+    events the library records on the kernel's stream"""
+
+    def __init__(self, kernel, max_launches):
+        self.kernel, self.max_launches = kernel, max_launches
+        self.avg_ms, self.launches = None, 0
+
+    def __enter__(self):
+        _check(lib().bilu_timing_begin(self.kernel, self.max_launches), "bilu_timing_begin")
+        return self
+
+    def __exit__(self, *exc):
+        ms, n = C.c_float(), C.c_int()
+        _check(lib().bilu_timing_end(C.byref(ms), C.byref(n)), "bilu_timing_end")
+        self.avg_ms, self.launches = ms.value, n.value
+        return False
 
 
 def ilu0(pat: Pattern, A, F, status=None, stream=None, schedule=None):

@@ -82,6 +82,7 @@ struct BiluSolveOptions {
     double parilu_stop_tol = 0.0;
     int isai_power = 1;           // ISAI sparsity |L|^k, |U|^k (|A|^k); the reference's ILU-ISAI uses 2
     int parilu_order = 0;         // BILU_PARILU_JACOBI / BILU_PARILU_INPLACE
+    double inner_adapt = 0.5;     // FP32_IR: inner solves stop at max(inner_tol, inner_adapt tol / |r|)
     const double *d_F_init = nullptr;   // ParILU warm start (device, in-place factor layout)
     void *stream = nullptr;             // hipStream_t; nullptr = default stream
     bool copy_back = true;              // copy x and the results to the host arrays

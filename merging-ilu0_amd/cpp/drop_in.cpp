@@ -150,6 +150,7 @@ bilu_opts to_opts(const BiluSolveOptions &s)
     o.parilu_stop_tol = s.parilu_stop_tol;
     o.isai_power = s.isai_power;
     o.parilu_order = s.parilu_order;
+    o.inner_adapt = s.inner_adapt;
     return o;
 }
 

@@ -260,6 +260,8 @@ struct MixedParams {
     float inner_tol;     // inner relative tolerance (FP32 BiCGSTAB)
     int max_iter;        // inner iterations per refinement step
     int max_refine;      // refinement steps
+    double inner_tol_d;  // inner_tol as given (FP64)
+    double inner_adapt;  // stop the inner solve at max(inner_tol, inner_adapt * tol / |r|) (0: inner_tol)
 };
 
 // w = A x in FP64 by CSR rows: lane l computes the rows l + 64 k, each an
@@ -571,7 +573,12 @@ __global__ __launch_bounds__(km_waves(SM, PCK) * WAVE) void k_mixed(
                     d.e[k] = 0.0f; p.e[k] = 0.0f; v.e[k] = 0.0f;
                 }
                 const float r0 = sqrtf(vdot(ri, ri, lane, n));
-                const float itol = mp.inner_tol * r0;
+                // the inner solve stops at the relative residual inner_tol, or looser
+                // when that already takes |r| below the outer tolerance with the
+                // margin inner_adapt (the last refinement step of most systems)
+                double want = mp.inner_tol_d;
+                if (mp.inner_adapt > 0.0 && mp.inner_adapt * tol_abs / rn > want) want = mp.inner_adapt * tol_abs / rn;
+                const float itol = (float)want * r0;
                 float ires = r0;
                 float it = 0.0f;
                 float rho_old = 1.0f, omega_old = 1.0f, alpha = 1.0f;

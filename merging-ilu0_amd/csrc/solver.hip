@@ -1163,6 +1163,31 @@ void fast_tables(const bilu_pattern_s &h, FastParams &fp)
     fp.parp = h.d_parp; fp.parf = h.d_parf; fp.par_rounds = pl.par_rounds; fp.par_psteps = pl.par_psteps;
 }
 
+// Kernel timing (bilu_timing_begin / _end): while a window is open, the
+// launches of one kernel kind are bracketed by HIP events recorded on their
+// stream, from a pool created when the window opens.
+struct Timing {
+    std::mutex mu;
+    int kind = 0;                                  // BILU_TIMING_* or 0 (closed)
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+    size_t used = 0;
+};
+Timing g_timing;
+
+// record the start (before = true) or the stop event of a kernel of `kind` on st
+void timing_mark(int kind, hipStream_t st, bool before)
+{
+    if (g_timing.kind != kind) return;
+    std::lock_guard<std::mutex> lock(g_timing.mu);
+    if (g_timing.kind != kind) return;
+    if (before) {
+        if (g_timing.used < g_timing.ev.size()) (void)hipEventRecord(g_timing.ev[g_timing.used].first, st);
+    } else {
+        if (g_timing.used < g_timing.ev.size()) (void)hipEventRecord(g_timing.ev[g_timing.used].second, st);
+        g_timing.used++;
+    }
+}
+
 template <int NR, int SM>
 int launch_fast_t(bilu_pattern_s &h, const FastParams &fp, int grid, int threads, size_t lds, int batch,
                   const double *A, const double *b, double *x, const double *F, const SolveParams &sp,
@@ -1171,7 +1196,9 @@ int launch_fast_t(bilu_pattern_s &h, const FastParams &fp, int grid, int threads
     auto kern = k_fast<NR, SM>;
     if (lds > 65536) HIP_TRY(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, st, fp.counter);
+    timing_mark(BILU_TIMING_SOLVE, st, true);
     hipLaunchKernelGGL(kern, dim3(grid), dim3(threads), lds, st, h.dev, fp, batch, A, b, x, F, sp, it, cv, ir, tr);
+    timing_mark(BILU_TIMING_SOLVE, st, false);
     HIP_TRY(hipGetLastError());
     return BILU_OK;
 }
@@ -1311,8 +1338,10 @@ int launch_factor_fast(bilu_pattern_s &h, int batch, const double *A, const doub
     auto kern = k_factor_fast<TO>;
     if (lds > 65536) HIP_TRY(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, st, fp.counter);
+    timing_mark(BILU_TIMING_FACTOR, st, true);
     hipLaunchKernelGGL(kern, dim3(grid), dim3(waves * WAVE), lds, st, fp, h.pa.n, nnz, batch, A, F_init, F,
                        mode, sweeps, stop_tol, status, isai, M, Z, sysidx, Y, P3, inplace);
+    timing_mark(BILU_TIMING_FACTOR, st, false);
     HIP_TRY(hipGetLastError());
     return BILU_OK;
 }
@@ -1332,8 +1361,10 @@ int launch_mixed_t(bilu_pattern_s &h, const FastParams &fp, int grid, int waves,
     auto kern = k_mixed<NR, SM, PCK, SLU>;
     if (lds > 65536) HIP_TRY(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, st, fp.counter);
+    timing_mark(BILU_TIMING_SOLVE, st, true);
     hipLaunchKernelGGL(kern, dim3(grid), dim3(waves * WAVE), lds, st, h.dev, fp, batch, A, b, x, F, Z, mp, it, cv,
                        ir, tr, refine, jobs.sysidx, jobs.outidx);
+    timing_mark(BILU_TIMING_SOLVE, st, false);
     HIP_TRY(hipGetLastError());
     return BILU_OK;
 }
@@ -1469,6 +1500,7 @@ void bilu_opts_default(bilu_opts *o)
     o->kernel = BILU_KERNEL_AUTO;
     o->precision = BILU_PRECISION_FP64;
     o->inner_tol = 1e-5;
+    o->inner_adapt = 0.5;
     o->max_refine = 10;
     o->parilu_stop_tol = 0.0;
     o->isai_power = 1;
@@ -1542,6 +1574,40 @@ int bilu_pattern_get_info(bilu_pattern_t p, bilu_pattern_info *info)
     info->max_row_nnz = p->pa.max_row;
     info->levels_entries = p->dev.nlev_e;
     info->ilu_eras = p->dev.n_eras;
+    return BILU_OK;
+}
+
+int bilu_timing_begin(int kernel, int max_launches)
+{
+    if ((kernel != BILU_TIMING_FACTOR && kernel != BILU_TIMING_SOLVE) || max_launches < 1) return BILU_ERR_INVALID_ARG;
+    std::lock_guard<std::mutex> lock(g_timing.mu);
+    if (g_timing.kind) return BILU_ERR_INVALID_ARG;   // one window at a time
+    for (auto &e : g_timing.ev) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
+    g_timing.ev.assign((size_t)max_launches, {nullptr, nullptr});
+    for (auto &e : g_timing.ev) {
+        HIP_TRY(hipEventCreate(&e.first));
+        HIP_TRY(hipEventCreate(&e.second));
+    }
+    g_timing.used = 0;
+    g_timing.kind = kernel;
+    return BILU_OK;
+}
+
+int bilu_timing_end(float *avg_ms, int *launches)
+{
+    std::lock_guard<std::mutex> lock(g_timing.mu);
+    if (!g_timing.kind) return BILU_ERR_INVALID_ARG;
+    g_timing.kind = 0;
+    const size_t n = std::min(g_timing.used, g_timing.ev.size());
+    double sum = 0.0;
+    for (size_t k = 0; k < n; k++) {
+        HIP_TRY(hipEventSynchronize(g_timing.ev[k].second));
+        float ms = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&ms, g_timing.ev[k].first, g_timing.ev[k].second));
+        sum += ms;
+    }
+    if (avg_ms) *avg_ms = n ? (float)(sum / (double)n) : 0.0f;
+    if (launches) *launches = (int)n;
     return BILU_OK;
 }
 
@@ -1669,7 +1735,8 @@ static int solve_ex_impl(bilu_pattern_t h, int batch, const double *d_A, const d
         !(o.parilu_stop_tol >= 0.0) || o.isai_power < 1 || o.isai_power > 16 ||
         (o.parilu_order != BILU_PARILU_JACOBI && o.parilu_order != BILU_PARILU_INPLACE))
         return BILU_ERR_INVALID_ARG;
-    if (o.precision == BILU_PRECISION_FP32_IR && (!(o.inner_tol > 0.0 && o.inner_tol < 1.0) || o.max_refine < 1))
+    if (o.precision == BILU_PRECISION_FP32_IR &&
+        (!(o.inner_tol > 0.0 && o.inner_tol < 1.0) || o.max_refine < 1 || !(o.inner_adapt >= 0.0 && o.inner_adapt < 1.0)))
         return BILU_ERR_INVALID_ARG;
     const bool isai_any = o.precond == BILU_PRECOND_ILU0_ISAI || o.precond == BILU_PRECOND_PARILU0_ISAI ||
                           o.precond == BILU_PRECOND_GENERAL_ISAI;
@@ -1745,7 +1812,7 @@ static int solve_ex_impl(bilu_pattern_t h, int batch, const double *d_A, const d
                                        o.parilu_sweeps, stop, d_factor_status, st, isai ? 1 : 0, F, Z, nullptr,
                                        KF_MAX_WAVES, nullptr, 0, inplace ? 1 : 0);
         if (rc != BILU_OK) return rc;
-        MixedParams mp{o.tol, o.tol_mode, (float)o.inner_tol, o.max_iter, o.max_refine};
+        MixedParams mp{o.tol, o.tol_mode, (float)o.inner_tol, o.max_iter, o.max_refine, o.inner_tol, o.inner_adapt};
         return launch_mixed(*h, batch, d_A, d_b, d_x, F, Z, isai ? PCK_ISAI : PCK_LU, mp, d_iter, d_conv,
                             d_iter_res, d_true_res, d_refine, st);
     }
@@ -1834,6 +1901,7 @@ int bilu_mlp_solve(bilu_mlp_t m, bilu_pattern_t h, int batch, const double *d_A,
     if ((o.precond != BILU_PRECOND_PARILU0 && !isai) || o.precision != BILU_PRECISION_FP32_IR ||
         o.parilu_sweeps < 0 || !(o.tol >= 0.0) || o.max_iter < 0 || !(o.parilu_stop_tol >= 0.0) ||
         !(o.inner_tol > 0.0 && o.inner_tol < 1.0) || o.max_refine < 1 || o.isai_power != 1 ||
+        !(o.inner_adapt >= 0.0 && o.inner_adapt < 1.0) ||
         (o.parilu_order != BILU_PARILU_JACOBI && o.parilu_order != BILU_PARILU_INPLACE))
         return BILU_ERR_INVALID_ARG;
     if (!h->pa.fast.ok) return BILU_ERR_TOO_LARGE;
@@ -1848,7 +1916,7 @@ int bilu_mlp_solve(bilu_mlp_t m, bilu_pattern_t h, int batch, const double *d_A,
     if (rc != BILU_OK) return rc;
     float *F = static_cast<float *>(ws);
     int *Z = reinterpret_cast<int *>(static_cast<char *>(ws) + fbytes);
-    MixedParams mp{o.tol, o.tol_mode, (float)o.inner_tol, o.max_iter, o.max_refine};
+    MixedParams mp{o.tol, o.tol_mode, (float)o.inner_tol, o.max_iter, o.max_refine, o.inner_tol, o.inner_adapt};
     for (int s0 = 0; s0 < batch; s0 += chunk) {
         const int cb = std::min(chunk, batch - s0);
         const double *A = d_A + (size_t)s0 * nnz;

@@ -70,7 +70,8 @@ int uneven_branch(bilu_uneven_s &u, int p, const double *A, const double *b, dou
                                        jobs.sysidx, fw, nullptr, 0,
                                        par && u.o.parilu_order == BILU_PARILU_INPLACE ? 1 : 0);
     if (rc != BILU_OK) return rc;
-    MixedParams mp{u.o.tol, u.o.tol_mode, (float)u.o.inner_tol, u.o.max_iter, u.o.max_refine};
+    MixedParams mp{u.o.tol, u.o.tol_mode, (float)u.o.inner_tol, u.o.max_iter, u.o.max_refine, u.o.inner_tol,
+                   u.o.inner_adapt};
     return launch_mixed(h, cnt, A, b, x, F, Z, isai ? PCK_ISAI : PCK_LU, mp, it, cv, ir, tr, rf, st, jobs, sw);
 }
 
@@ -117,7 +118,8 @@ int bilu_uneven_create(int npat, const bilu_pattern_t *patterns, int nreq, const
         !(o.precond == BILU_PRECOND_ILU0 || o.precond == BILU_PRECOND_PARILU0 ||
           o.precond == BILU_PRECOND_ILU0_ISAI || o.precond == BILU_PRECOND_PARILU0_ISAI) ||
         !(o.inner_tol > 0.0 && o.inner_tol < 1.0) || o.max_refine < 1 || !(o.tol >= 0.0) || o.max_iter < 0 ||
-        o.isai_power != 1 || (o.parilu_order != BILU_PARILU_JACOBI && o.parilu_order != BILU_PARILU_INPLACE))
+        o.isai_power != 1 || (o.parilu_order != BILU_PARILU_JACOBI && o.parilu_order != BILU_PARILU_INPLACE) ||
+        !(o.inner_adapt >= 0.0 && o.inner_adapt < 1.0))
         return BILU_ERR_INVALID_ARG;
     for (int p = 0; p < npat; p++) {
         if (!patterns[p]) return BILU_ERR_INVALID_ARG;

@@ -938,7 +938,12 @@ int orc_bicgstab_mixed_pc(int n, const int *rp, const int *ci, const int *diag,
                 d[i] = 0.0f; p[i] = 0.0f; v[i] = 0.0f;
             }
             const float r0 = sqrtf(dotf64w(n, ri, ri));
-            const float itol = (float)inner_tol * r0;
+            /* the inner solve stops at the relative residual inner_tol, or looser
+             * when that already takes |r| below the outer tolerance with the
+             * margin inner_adapt (the last refinement step of most systems) */
+            double want = inner_tol;
+            if (o->inner_adapt > 0.0 && o->inner_adapt * tol_abs / rn > want) want = o->inner_adapt * tol_abs / rn;
+            const float itol = (float)want * r0;
             float ires = r0, it = 0.0f;
             float rho_old = 1.0f, omega_old = 1.0f, alpha = 1.0f;
             int inner_bad = 0;

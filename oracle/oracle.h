@@ -63,6 +63,7 @@ typedef struct {
     int true_res_restart;    /* restart from b - A x when the true residual misses tol */
     int max_restarts;
     int trsv_order;          /* 0: CSR order (reference), 1: availability order (see orc_lu_solve_avail) */
+    double inner_adapt;      /* FP32 inner solves: stop at max(inner_tol, inner_adapt * tol / |r|) */
 } orc_opts;
 
 int orc_find_diag(int n, const int *rp, const int *ci, int *diag);

@@ -32,6 +32,7 @@ class Opts(C.Structure):
         ("true_res_restart", C.c_int),
         ("max_restarts", C.c_int),
         ("trsv_order", C.c_int),
+        ("inner_adapt", C.c_double),
     ]
 
 
@@ -39,12 +40,14 @@ TRSV_CSR, TRSV_AVAIL = 0, 1
 
 
 def opts(tol=1e-11, tol_mode=TOL_ABSOLUTE, max_iter=500, dot_width=64, spmv_width=1,
-         true_res_restart=0, max_restarts=3, trsv_order=TRSV_CSR) -> Opts:
+         true_res_restart=0, max_restarts=3, trsv_order=TRSV_CSR, inner_adapt=0.5) -> Opts:
     """Defaults: reference tolerance semantics (ATOL=1e-11, MAX_ITER=500,
     PrecondBiCGSTAB.h:10-12) with the wave64 reduction order.  trsv_order 1
-    is the row-sum order of the persistent kernel."""
+    is the row-sum order of the persistent kernel.  inner_adapt: the FP32
+    inner solves of the mixed path stop at the relative residual
+    max(inner_tol, inner_adapt * tol / |r|) (0: inner_tol)."""
     return Opts(tol, tol_mode, max_iter, dot_width, spmv_width, true_res_restart, max_restarts,
-                trsv_order)
+                trsv_order, inner_adapt)
 
 
 def reference_opts(**kw) -> Opts:

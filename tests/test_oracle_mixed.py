@@ -114,6 +114,23 @@ def test_inplace_round_order_contracts_faster():
     assert np.mean(sbi) <= np.mean(si)
 
 
+def test_inner_adapt_keeps_convergence_and_saves_inner_steps():
+    # the adaptive inner tolerance (stop each FP32 inner solve once it takes
+    # the FP64 residual below half the outer tolerance) keeps every system at
+    # relres <= 1e-10 and spends fewer inner half steps than a fixed 1e-5
+    pat, A, b = synth.batch("random", 96, n=144)
+    res = {}
+    for ad in (0.0, 0.5):
+        o = O.opts(tol=1e-10, tol_mode=O.TOL_RELATIVE, trsv_order=O.TRSV_AVAIL, inner_adapt=ad)
+        x, it, cv, ir, tr, rf, fs = O.batch_solve_split(pat, A, b, precond=O.ILU0_ISAI, precision=O.PREC_FP32_IR,
+                                                       o=o, threads=8)
+        assert (cv == 1).all()
+        assert max(relres(pat, A[s], b[s], x[s]) for s in range(A.shape[0])) <= 1e-10
+        res[ad] = (it.mean(), rf.mean())
+    assert res[0.5][0] < res[0.0][0]
+    assert res[0.5][1] <= res[0.0][1] + 0.05
+
+
 def test_entry_order_is_topological():
     # every entry's dependencies (ParILU pairs and the pivot) come at a lower
     # DAG level, so one pass in this order with level barriers is exact ILU(0)
