@@ -7,17 +7,24 @@ inference) + BiCGSTAB + FP64 true residual.  Prints one JSON line (see the
 repository's bench contract).
 
 Default (headline): the learned + mixed-precision solve -- MLP warm start,
-ParILU(0) sweeps to a factor residual of 1e-8 (exact ILU(0) fallback),
-ISAI(L, U) of the factors applied in FP32, FP32 BiCGSTAB inside FP64
-iterative refinement -- timed beside the plain FP64 merged exact-ILU(0)
+in-place ParILU(0) sweeps to a factor residual of 1e-8 (exact ILU(0)
+fallback), ISAI(L, U) of the factors applied in FP32, FP32 BiCGSTAB inside
+FP64 iterative refinement -- timed beside the plain FP64 merged exact-ILU(0)
 BiCGSTAB kernel and the mixed-precision solve without the network.  The
-network is trained before the timed region (reported under "training").
---mode single times one variant chosen by the flags.
+network is trained before the timed region (reported under "training", with
+held-out sweep counts against its untrained base guess and against no warm
+start).  The line also carries the uneven-batch result (4 patterns), the
+FP32-IR solution error against dense FP64 solves, the dominant kernel's
+duration from HIP events the library records on its stream, and the CPU
+baseline (the oracle on every core of this host).
+--mode single times one variant chosen by the flags; --mode uneven only the
+uneven batch.
 """
 from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -46,24 +53,50 @@ def parse():
                     help="fp32_ir: FP32 preconditioner and inner BiCGSTAB inside FP64 refinement")
     ap.add_argument("--inner-tol", type=float, default=1e-5)
     ap.add_argument("--train-systems", type=int, default=262144)
-    ap.add_argument("--train-epochs", type=int, default=3)
+    ap.add_argument("--train-epochs", type=int, default=30)
+    ap.add_argument("--no-uneven", action="store_true", help="skip the uneven-batch leg")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run only a 1 GiB device copy (HBM counter calibration)")
     return ap.parse_args()
 
 
-def measured_traffic(key):
-    """HBM bytes per launch of the dominant kernel from the committed
-    rocprofv3 FETCH_SIZE / WRITE_SIZE runs (profiles/traffic.json), each
-    scaled by what the counter reads for a 1 GiB device copy; None when that
-    workload has not been profiled."""
+def committed_profile(key):
+    """Counter figures of a workload from the committed rocprofv3 --pmc runs
+    (profiles/counters.json: HBM traffic per launch from FETCH_SIZE /
+    WRITE_SIZE scaled by a device copy's reading, LDS bytes per launch from
+    the SQ_INSTS_LDS_*_BANDWIDTH counters).  These are not measured by this
+    run; the record says where they come from.  None when not profiled."""
     try:
-        with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
-            rec = json.load(f).get(key)
+        with open(os.path.join(ROOT, "profiles", "counters.json")) as f:
+            return json.load(f).get(key)
     except (OSError, ValueError):
         return None
-    return None if rec is None else rec["traffic_bytes_per_launch"]
+
+
+def traffic_fields(key):
+    rec = committed_profile(key)
+    if rec is None:
+        return {"traffic": None, "traffic_source": None}
+    return {"traffic": rec.get("traffic_bytes_per_launch"),
+            "traffic_source": f"profiles/counters.json[{key!r}] (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE; "
+                              f"committed, not measured by this run)"}
+
+
+# LDS peak of MI355X: 256 CUs x 128 bytes per clock x 2.4 GHz
+LDS_PEAK_GBS = 256 * 128 * 2.4
+
+
+def lds_fields(key, kernel_ms):
+    """the kernel's LDS bytes per launch (committed counters) over its
+    duration measured in this run, against the chip's LDS peak"""
+    rec = committed_profile(key)
+    if rec is None or kernel_ms is None or "lds_bytes_per_launch" not in rec:
+        return None
+    gbs = rec["lds_bytes_per_launch"] / (kernel_ms * 1e-3) / 1e9
+    return {"achieved": round(gbs, 1), "peak": LDS_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / LDS_PEAK_GBS, 4),
+            "lds_bytes_per_launch": rec["lds_bytes_per_launch"],
+            "source": f"LDS bytes: profiles/counters.json[{key!r}] (committed); duration: this run"}
 
 
 def cpu_model():
@@ -76,16 +109,24 @@ def cpu_model():
     return "unknown"
 
 
+def host_cores():
+    try:
+        return len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        return os.cpu_count() or 1
+
+
 def cpu_baseline(pat, A, b, precond):
     """The oracle (CPU restatement of the reference path, same algorithm and
-    stopping rule) timed on this host: the whole batch on all threads, plus a
-    single-core figure on a prefix."""
+    stopping rule) timed on this host: the whole batch on every core this
+    process may use (a few seconds of CPU work), plus a single-core figure
+    on a prefix."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     code = {"ilu0": O.ILU0, "parilu0": O.PARILU0, "ilu0_isai": O.ILU0_ISAI,
             "parilu0_isai": O.PARILU0_ISAI, "general_isai": O.GENERAL_ISAI}[precond]
     oo = O.opts(tol=1e-10, tol_mode=O.TOL_RELATIVE, true_res_restart=1)
-    threads = max(1, min(os.cpu_count() or 1, 16))
+    threads = host_cores()
     s_all = A.shape[0]
     t = time.perf_counter()
     O.batch_solve(pat, A[:s_all], b[:s_all], code, sweeps=5, o=oo, threads=threads)
@@ -97,26 +138,58 @@ def cpu_baseline(pat, A, b, precond):
     return {"value": round(s_all / dt_all, 1), "unit": "systems/s", "cores": threads,
             "kind": "port",
             "sample": f"{s_all} systems (the whole batch), plain FP64 {precond} + BiCGSTAB, "
-                      f"{threads} threads, {dt_all:.2f} s wall",
+                      f"{threads} threads (every core of the affinity mask), {dt_all:.2f} s wall",
             "single_core_value": round(s_one / dt_one, 1),
             "single_core_sample": f"first {s_one} systems, 1 thread, {dt_one:.2f} s",
-            "cpu_model": cpu_model()}
+            "cpu_model": cpu_model(), "host_logical_cpus": os.cpu_count()}
 
 
-def time_steps(step, warmup, steps, stream):
-    """HIP events on the launch stream around exactly `steps` steps."""
+def hbm_copy_peak(dev):
+    """device-to-device copy of 2 GiB, timed with HIP events: the HBM rate a
+    plain streaming kernel reaches on this GPU (read + write bytes / time)"""
     import torch
+    a = torch.empty(1 << 28, dtype=torch.float64, device=dev)
+    c = torch.empty_like(a)
+    a.fill_(1.0)
+    c.copy_(a)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(5):
+        c.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / 5
+    gbs = 2 * a.numel() * 8 / (ms * 1e-3) / 1e9
+    del a, c
+    torch.cuda.empty_cache()
+    return round(gbs, 1)
+
+
+def time_steps(step, warmup, steps, stream, kernel=None):
+    """HIP events on the launch stream around exactly `steps` steps; with
+    `kernel` (bilu.TIMING_*) also the average duration of that kernel's
+    launches inside the timed region (events the library records on the
+    kernel's stream).  Returns ms per step, or (ms per step, kernel ms)."""
+    import torch
+    import bilu
     for _ in range(warmup):
         step()
     torch.cuda.synchronize()
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
+    kt = bilu.KernelTiming(kernel, 4 * steps + 4) if kernel else None
+    if kt:
+        kt.__enter__()
     e0.record(stream)
     for _ in range(steps):
         step()
     e1.record(stream)
     torch.cuda.synchronize()
-    return e0.elapsed_time(e1) / max(steps, 1)
+    if kt:
+        kt.__exit__(None, None, None)
+    ms = e0.elapsed_time(e1) / max(steps, 1)
+    return (ms, kt.avg_ms) if kt else ms
 
 
 def uneven_bench(args, dev):
@@ -235,7 +308,7 @@ def main():
                 "parilu_sweeps_mean": float(fs[fs >= 0].mean()) if (fs >= 0).any() else None,
                 "parilu_fallbacks": int((fs < 0).sum())}
 
-    def run(o, F_init=None, pre=None):
+    def run(o, F_init=None, pre=None, kernel=None):
         # the factor status is requested only from the ParILU paths (asking for
         # it routes a plain solve through the split factor + solve launches)
         par = o.precond in (bilu.PARILU0, bilu.PARILU0_ISAI)
@@ -247,7 +320,19 @@ def main():
                 pre()
             bilu.solve_ex(P, dA, db, x, o, F_init, out["it"], out["cv"], out["ir"], out["tr"],
                           out["rf"], out["fs"] if par else False, stream=stream)
-        return time_steps(step, args.warmup, args.steps, stream)
+        return time_steps(step, args.warmup, args.steps, stream, kernel=kernel)
+
+    def roofline(kname, kernel_ms, key, step_ms):
+        # achieved = the path's algorithmic bytes per launch over the kernel's
+        # own duration (HIP events on its stream, this run)
+        ach = alg_bytes / (kernel_ms * 1e-3) / 1e9
+        r = {"bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+             "frac": round(ach / HBM_PEAK_GBS, 5), "kernel": kname, "kernel_ms": round(kernel_ms, 4),
+             "alg_bytes_per_launch": alg_bytes,
+             "step_achieved": round(alg_bytes / (step_ms * 1e-3) / 1e9, 2)}
+        r.update(traffic_fields(key))
+        r["lds"] = lds_fields(key, kernel_ms)
+        return r
 
     rec = {"metric": METRIC, "unit": "systems/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
@@ -265,11 +350,10 @@ def main():
                               "persistent": bilu.KERNEL_PERSISTENT}[args.kernel],
                       precision=bilu.PRECISION_FP32_IR if mixed else bilu.PRECISION_FP64,
                       inner_tol=args.inner_tol)
-        ms = run(o)
+        ms, kms = run(o, kernel=bilu.TIMING_SOLVE)
         s = summary(ms)
         kname = ("k_mixed" if mixed else
-                 "k_fast" if (args.merge == "merged" and args.kernel != "wave") else None)
-        achieved = alg_bytes / (ms * 1e-3) / 1e9
+                 "k_fast" if (args.merge == "merged" and args.kernel != "wave") else "k_solve")
         rec.update({"value": s["systems_per_s"], "ms_per_step": s["ms_per_step"],
                     "dtype": "f32+f64" if mixed else "f64",
                     "config": dict(base_cfg, workload=f"random pattern n={n} nnz={nnz}, batch {B}, "
@@ -277,29 +361,47 @@ def main():
                                    precond=args.precond, merge=args.merge, kernel=args.kernel,
                                    precision=args.precision),
                     "result": s,
-                    "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
-                                 "traffic": measured_traffic(f"{kname} {args.precond} {args.merge} n{n} b{B}")
-                                 if kname else None,
-                                 "kernel": kname or "k_solve", "alg_bytes_per_launch": alg_bytes}})
+                    "roofline": roofline(kname, kms, f"{kname} {args.precond} {args.precision} n{n} b{B}", ms)})
         rec["cpu_baseline"] = None if args.no_cpu else cpu_baseline(pat, A, b, args.precond)
         print(json.dumps(rec), flush=True)
         return
 
     # ---------------------------------------------------------------- headline
     common = dict(tol=1e-10, tol_mode=bilu.TOL_RELATIVE)
+    P.reserve(B)
+    copy_peak = hbm_copy_peak(dev)
     # (1) plain FP64 merged exact ILU(0) + BiCGSTAB (the persistent k_fast kernel)
     o_plain = bilu.opts(precond=bilu.ILU0, true_res_restart=1, **common)
-    ms_plain = run(o_plain)
+    ms_plain, kms_plain = run(o_plain, kernel=bilu.TIMING_SOLVE)
     plain = summary(ms_plain)
+    x_plain = x.clone()
     # (2) mixed precision without the network: exact ILU(0), ISAI(L, U) in FP32, FP32 inner / FP64 IR
     o_mixed = bilu.opts(precond=bilu.ILU0_ISAI, precision=bilu.PRECISION_FP32_IR,
                         inner_tol=args.inner_tol, **common)
     ms_mixed = run(o_mixed)
     mixed = summary(ms_mixed)
+    x_mixed = x.clone()
     # (3) learned + mixed: train the warm start (untimed), then time inference + solve
+    o_learn = bilu.opts(precond=bilu.PARILU0_ISAI, parilu_sweeps=40, parilu_stop_tol=1e-8,
+                        parilu_order=bilu.PARILU_INPLACE, precision=bilu.PRECISION_FP32_IR,
+                        inner_tol=args.inner_tol, **common)
+    # held-out systems for the training curve (ParILU sweeps to 1e-8 from the network's guess)
+    _, Ah, bh = synth.batch("random", 8192, first=(1 << 26), n=n)
+    dAh, dbh = torch.from_numpy(Ah).to(dev), torch.from_numpy(bh).to(dev)
+    xh = torch.empty_like(dbh)
+
+    def held_sweeps(net=None):
+        fs = torch.empty(dAh.shape[0], dtype=torch.int32, device=dev)
+        if net is None:
+            bilu.solve_ex(P, dAh, dbh, xh, o_learn, factor_status=fs)
+        else:
+            net.solve(P, dAh, dbh, xh, o_learn, factor_status=fs)
+        torch.cuda.synchronize()
+        return round(float(fs.float().mean().item()), 4)
+
     t0 = time.time()
     net = bilu.Mlp(P, hidden=512, seed=1)
+    net0 = bilu.Mlp(P, hidden=512, seed=1)   # untrained: the base guess
     chunk = 65536
     loss = torch.zeros(1, dtype=torch.float32, device=dev)
     tr_A, tr_F = [], []
@@ -312,43 +414,45 @@ def main():
         tr_F.append(dFt)
     torch.cuda.synchronize()
     gen_train_s = time.time() - t0
+    held_from_A, held_base = held_sweeps(None), held_sweeps(net0)
     t1 = time.time()
-    mb, steps_done, losses = 4096, 0, []
+    mb, steps_done, curve = 4096, 0, []
     total_steps = max(1, args.train_epochs * args.train_systems // mb)
+    every = max(1, args.train_epochs // 6)
     for ep in range(args.train_epochs):
         for ci in range(len(tr_A)):
             for s0 in range(0, tr_A[ci].shape[0], mb):
-                lr = 3e-4 if steps_done < 0.7 * total_steps else 1e-4
+                lr = 3e-4 * 0.5 * (1.0 + math.cos(math.pi * steps_done / total_steps))   # cosine decay
                 net.train_step(tr_A[ci][s0:s0 + mb], tr_F[ci][s0:s0 + mb], lr, loss)
                 steps_done += 1
-        losses.append(round(loss.item(), 6))
+        if ep % every == 0 or ep == args.train_epochs - 1:
+            curve.append({"epoch": ep + 1, "loss": round(loss.item(), 6), "held_out_sweeps": held_sweeps(net)})
     torch.cuda.synchronize()
     train_s = time.time() - t1
     del tr_A, tr_F
     guess = torch.empty_like(dA)
-    o_learn = bilu.opts(precond=bilu.PARILU0_ISAI, parilu_sweeps=40, parilu_stop_tol=1e-8,
-                        precision=bilu.PRECISION_FP32_IR, inner_tol=args.inner_tol, **common)
     # (3a) split: bilu_mlp_infer writes the FP64 guess, bilu_solve_ex reads it
     ms_split = run(o_learn, F_init=guess, pre=lambda: net.infer(dA, guess, stream=stream))
     split = summary(ms_split)
     x_split = x.clone()
     # (3) fused (bilu_mlp_solve): the factor kernel forms the guess from the
-    # network's raw outputs; the same results bit for bit
+    # network's raw outputs; the same results bit for bit.  The solve kernel
+    # (k_mixed, the step's longest) is timed with the library's HIP events.
     out["fs"].zero_()
     out["rf"].zero_()
 
     def fused_step():
         net.solve(P, dA, db, x, o_learn, out["it"], out["cv"], out["ir"], out["tr"], out["rf"], out["fs"],
                   stream=stream)
-    ms_learn = time_steps(fused_step, args.warmup, args.steps, stream)
+    ms_learn, kms_learn = time_steps(fused_step, args.warmup, args.steps, stream, kernel=bilu.TIMING_SOLVE)
     learned = summary(ms_learn)
     fused_equal = bool(torch.equal(x, x_split))
+    x_learn = x.clone()
+    _, kms_factor = time_steps(fused_step, 0, max(2, args.steps // 2), stream, kernel=bilu.TIMING_FACTOR)
     # (3b) the same ParILU path from the standard start (no network), for the gain
     ms_nolearn = run(o_learn)
     nolearn = summary(ms_nolearn)
-    # (3c) the untrained network (zero output layer: the base guess a_ij, a_ij / a_jj),
-    # what the training adds or loses against its own starting point
-    net0 = bilu.Mlp(P, hidden=512, seed=1)
+    # (3c) the untrained network (zero output layer: the base guess a_ij, a_ij / a_jj)
     out["fs"].zero_()
     out["rf"].zero_()
 
@@ -359,15 +463,28 @@ def main():
     net0.close()
     ms_infer = time_steps(lambda: net.infer(dA, guess, stream=stream), 1, args.steps, stream)
 
-    achieved = alg_bytes / (ms_learn * 1e-3) / 1e9
-    achieved_plain = alg_bytes / (ms_plain * 1e-3) / 1e9
+    # FP32-IR solution error against dense FP64 solves, on a sample of systems
+    smp = np.arange(0, B, max(1, B // 256))[:256]
+    xs = np.stack([np.linalg.solve(pat.dense(A[i]), b[i]) for i in smp])
+    def err(xg):
+        xg = xg.cpu().numpy()[smp]
+        e = np.linalg.norm(xg - xs, axis=1) / np.linalg.norm(xs, axis=1)
+        return {"max": float(e.max()), "mean": float(e.mean())}
+    c2 = {"sample": f"{len(smp)} systems (every {B // len(smp)}th), reference numpy.linalg.solve in FP64",
+          "plain_fp64": err(x_plain), "mixed_ilu0_isai": err(x_mixed), "learned_mixed": err(x_learn),
+          "refine_steps_mean": {"mixed_ilu0_isai": mixed["refine_steps_mean"],
+                                "learned_mixed": learned["refine_steps_mean"]},
+          "refine_steps_max": {"mixed_ilu0_isai": mixed["refine_steps_max"],
+                               "learned_mixed": learned["refine_steps_max"]}}
+
+    dom = "k_mixed" if kms_learn >= kms_factor else "k_factor_fast"
     rec.update({
         "value": learned["systems_per_s"], "ms_per_step": learned["ms_per_step"], "dtype": "f32+f64",
         "config": dict(base_cfg,
                        workload=(f"random pattern n={n} nnz={nnz}, batch {B}: MLP warm start (BF16) -> "
-                                 f"ParILU(0) to factor residual 1e-8 (exact ILU(0) fallback) -> ISAI(L,U) "
-                                 f"FP32 -> FP32 BiCGSTAB (inner tol {args.inner_tol:g}) in FP64 refinement "
-                                 f"to relres 1e-10"),
+                                 f"in-place ParILU(0) to factor residual 1e-8 (exact ILU(0) fallback) -> "
+                                 f"ISAI(L,U) FP32 -> FP32 BiCGSTAB (inner tol {args.inner_tol:g}, adaptive) in "
+                                 f"FP64 refinement to relres 1e-10"),
                        variant="learned+mixed"),
         "speedup_vs_plain_fp64": round(ms_plain / ms_learn, 3),
         "variants": {"plain_fp64_ilu0_merged": plain, "mixed_ilu0_isai": mixed,
@@ -376,21 +493,23 @@ def main():
                      "untrained_network_base_guess_mixed_parilu0_isai": untrained},
         "fused_x_equals_split_x": fused_equal,
         "mlp_inference_ms": round(ms_infer, 4),
+        "kernel_ms": {"k_mixed": round(kms_learn, 4), "k_factor_fast": round(kms_factor, 4),
+                      "k_fast_plain_fp64": round(kms_plain, 4),
+                      "source": "HIP events recorded by the library around each launch on its stream"},
         "training": {"systems": args.train_systems, "epochs": args.train_epochs, "minibatch": mb,
-                     "label_generation_s": round(gen_train_s, 2), "train_s": round(train_s, 2),
-                     "loss_per_epoch": losses},
+                     "schedule": "Adam, lr 3e-4 cosine to 0", "label_generation_s": round(gen_train_s, 2),
+                     "train_s": round(train_s, 2),
+                     "held_out": {"systems": int(dAh.shape[0]), "sweeps_from_A": held_from_A,
+                                  "sweeps_untrained_base_guess": held_base, "curve": curve}},
         "all_converged_relres_le_1e-10": learned["all_converged_relres_le_1e-10"],
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
-                     "traffic": measured_traffic(f"learned_mixed n{n} b{B}"),
-                     "kernel": "whole learned+mixed step (MLP features + GEMMs, k_factor_fast, k_mixed)",
-                     "alg_bytes_per_launch": alg_bytes},
-        "roofline_plain_fp64": {"bound": "hbm", "achieved": round(achieved_plain, 2), "peak": HBM_PEAK_GBS,
-                                "unit": "GB/s", "frac": round(achieved_plain / HBM_PEAK_GBS, 5),
-                                "traffic": measured_traffic(f"k_fast ilu0 merged n{n} b{B}"),
-                                "kernel": "k_fast (one launch per step)", "alg_bytes_per_launch": alg_bytes},
+        "roofline": roofline(dom, kms_learn if dom == "k_mixed" else kms_factor,
+                             f"{dom} learned n{n} b{B}", ms_learn),
+        "roofline_plain_fp64": roofline("k_fast", kms_plain, f"k_fast ilu0 fp64 n{n} b{B}", ms_plain),
+        "hbm_copy_peak_gbs": copy_peak,
+        "fp32_ir_error_vs_fp64": c2,
         "gen_seconds": round(gen_s, 2),
     })
+    rec["uneven"] = None if args.no_uneven else uneven_bench(args, dev)
     rec["cpu_baseline"] = None if args.no_cpu else cpu_baseline(pat, A, b, "ilu0")
     print(json.dumps(rec), flush=True)
 
