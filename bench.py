@@ -43,7 +43,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--batch", type=int, default=65536)
     ap.add_argument("--n", type=int, default=144)
-    ap.add_argument("--mode", default="headline", choices=["headline", "single", "uneven"])
+    ap.add_argument("--mode", default="headline", choices=["headline", "single", "uneven", "learned"],
+                    help="learned: only the learned + mixed step (training as --train-*), for profiling")
     ap.add_argument("--uneven-each", type=int, default=16384)
     ap.add_argument("--precond", default="ilu0",
                     choices=["ilu0", "parilu0", "ilu0_isai", "parilu0_isai", "general_isai"])
@@ -363,6 +364,40 @@ def main():
                     "result": s,
                     "roofline": roofline(kname, kms, f"{kname} {args.precond} {args.precision} n{n} b{B}", ms)})
         rec["cpu_baseline"] = None if args.no_cpu else cpu_baseline(pat, A, b, args.precond)
+        print(json.dumps(rec), flush=True)
+        return
+
+    if args.mode == "learned":
+        # the headline's learned step alone: network trained on --train-systems
+        # for --train-epochs, then bilu_mlp_solve timed (both kernels' durations
+        # from the library's HIP events)
+        o_learn = bilu.opts(precond=bilu.PARILU0_ISAI, parilu_sweeps=40, parilu_stop_tol=1e-8,
+                            parilu_order=bilu.PARILU_INPLACE, precision=bilu.PRECISION_FP32_IR,
+                            inner_tol=args.inner_tol, tol=1e-10, tol_mode=bilu.TOL_RELATIVE)
+        P.reserve(B)
+        net = bilu.Mlp(P, hidden=512, seed=1)
+        if args.train_epochs > 0 and args.train_systems > 0:
+            _, At, _ = synth.batch("random", args.train_systems, first=1 << 24, n=n)
+            dAt = torch.from_numpy(At).to(dev)
+            dFt = torch.empty_like(dAt)
+            bilu.ilu0(P, dAt, dFt)
+            for ep in range(args.train_epochs):
+                for s0 in range(0, dAt.shape[0], 4096):
+                    net.train_step(dAt[s0:s0 + 4096], dFt[s0:s0 + 4096], 3e-4)
+            del dAt, dFt
+
+        def learned_step():
+            net.solve(P, dA, db, x, o_learn, out["it"], out["cv"], out["ir"], out["tr"], out["rf"], out["fs"],
+                      stream=stream)
+        ms, kms = time_steps(learned_step, args.warmup, args.steps, stream, kernel=bilu.TIMING_SOLVE)
+        s = summary(ms)
+        _, kmf = time_steps(learned_step, 0, args.steps, stream, kernel=bilu.TIMING_FACTOR)
+        rec.update({"value": s["systems_per_s"], "ms_per_step": s["ms_per_step"], "dtype": "f32+f64",
+                    "config": dict(base_cfg, workload=f"random pattern n={n} nnz={nnz}, batch {B}, learned + "
+                                                      f"mixed step (trained {args.train_epochs} epochs on "
+                                                      f"{args.train_systems} systems)"),
+                    "result": s, "kernel_ms": {"k_mixed": round(kms, 4), "k_factor_fast": round(kmf, 4)},
+                    "roofline": roofline("k_mixed", kms, f"k_mixed learned n{n} b{B}", ms)})
         print(json.dumps(rec), flush=True)
         return
 

@@ -1,7 +1,8 @@
 """Uneven batches (several patterns, one interleaved request list) through
 the hipGraph-captured plan: bitwise the same as solving each pattern's
 systems on their own, unchanged by the autotuner's choice, stable over graph
-replays, and the oracle's results for the 72-slot (n = 300) kernels."""
+replays, and the oracle's results for the 72-slot (n = 300) kernels, LU
+and ISAI."""
 import numpy as np
 import pytest
 
@@ -50,11 +51,6 @@ def test_uneven_matches_per_pattern_and_tuning(precond):
     B_each = 384
     pats, As, bs, req_p, req_i = c3(B_each)
     kw = dict(OPTS, precond=precond)
-    if precond == bilu.ILU0_ISAI:   # the ISAI coefficients of n = 300 exceed the kernel's register budget
-        keep = [0, 1, 2]
-        pats, As, bs = [pats[k] for k in keep], [As[k] for k in keep], [bs[k] for k in keep]
-        sel = req_p < 3
-        req_p, req_i = req_p[sel], req_i[sel]
     P = [bilu.Pattern.from_synth(p) for p in pats]
     dA = [torch.from_numpy(A).to(dev()) for A in As]
     db = [torch.from_numpy(b).to(dev()) for b in bs]
@@ -123,18 +119,20 @@ def test_uneven_graph_replays_and_recapture():
     plan.close()
 
 
-def test_uneven_n300_against_oracle():
-    print("case uneven n300 oracle", flush=True)
+@pytest.mark.parametrize("precond", [bilu.ILU0, bilu.ILU0_ISAI])
+def test_uneven_n300_against_oracle(precond):
+    print(f"case uneven n300 oracle precond={precond}", flush=True)
     pat, A, b = synth.batch("random", 48, n=300, density=13.0 / 299)
     P = bilu.Pattern.from_synth(pat)
     dA, db = torch.from_numpy(A).to(dev()), torch.from_numpy(b).to(dev())
     dx = torch.empty_like(db)
     plan = bilu.Uneven([P], np.zeros(48, np.int32), np.arange(48, dtype=np.int32),
-                       bilu.opts(precond=bilu.ILU0, **OPTS))
+                       bilu.opts(precond=precond, **OPTS))
     out = plan.outputs()
     plan.solve([dA], [db], [dx], out)
     torch.cuda.synchronize()
-    o = O.batch_solve_split(pat, A, b, precision=O.PREC_FP32_IR,
+    o = O.batch_solve_split(pat, A, b, precond=O.ILU0 if precond == bilu.ILU0 else O.ILU0_ISAI,
+                            precision=O.PREC_FP32_IR,
                             o=O.opts(tol=1e-10, tol_mode=O.TOL_RELATIVE, trsv_order=O.TRSV_AVAIL), threads=8)
     assert np.array_equal(dx.cpu().numpy(), o[0])
     for u, v in zip(out, (o[1], o[2], o[3], o[4], o[5])):
