@@ -326,7 +326,9 @@ int bilu_mlp_gradient(bilu_mlp_t m, int batch, const double *d_A, const double *
 /* one Adam step (lr) on the minibatch d_A[batch*nnz] with exact factors d_F */
 int bilu_mlp_train_step(bilu_mlp_t m, int batch, const double *d_A, const double *d_F, float lr,
                         float *d_loss, void *stream);
-/* predicted factors d_F_init[batch*nnz] (FP64, in-place factor layout) */
+/* predicted factors d_F_init[batch*nnz] (FP64, in-place factor layout); the
+ * network's output y is rounded to FP16 before it enters the guess (as in
+ * bilu_mlp_solve's fused warm start) */
 int bilu_mlp_infer(bilu_mlp_t m, int batch, const double *d_A, double *d_F_init, void *stream);
 /* learned + mixed-precision solve in one call: the same results, bit for bit,
  * as bilu_mlp_infer followed by bilu_solve_ex with that d_F_init and the

@@ -30,8 +30,8 @@ struct FastParams {
     const uint16_t *cntL, *cntU;          // per level: chunks | row-slot mask << 8, [64]
     const uint16_t *rlevL, *rlevU;        // [NR][64] level of the lane's rows
     const ushort4 *etab;                  // factorization entry records
-    const uint32_t *workbits;             // bit e set: entry e has work (a product or a division)
-    const uint16_t *lowcol;               // per position: column j of a strictly lower entry, else 0xFFFF
+    const uint16_t *wcode;                // per position: column j of a strictly lower entry, 0xFFFE an
+                                          // upper entry with work, 0xFFFF no work
     int dg_floats;                        // k_factor_fast: floats at the end of a wave region for the diagonal
     const uint16_t *eptr;                 // per entry-DAG level
     const ushort2 *dep;                   // (pos(i,k), pos(k,j)) pairs

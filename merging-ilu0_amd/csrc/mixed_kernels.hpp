@@ -76,7 +76,7 @@ __global__ __launch_bounds__(KF_MAX_WAVES * WAVE) void k_factor_fast(
     FastParams fp, int n, int nnz, int batch, const double *__restrict__ gA,
     const double *__restrict__ gFinit, TO *__restrict__ gF, int mode, int sweeps, double stop_tol,
     int *__restrict__ g_status, int isai, float *__restrict__ gM, int *__restrict__ gZ,
-    const int *__restrict__ sysidx, const float *__restrict__ gY, int P3, int inplace)
+    const int *__restrict__ sysidx, const _Float16 *__restrict__ gY, int P3, int inplace)
 {
     // job j factors system sysidx[j] (identity when sysidx is null); outputs are per job
     extern __shared__ __align__(16) unsigned char smem[];
@@ -110,7 +110,7 @@ __global__ __launch_bounds__(KF_MAX_WAVES * WAVE) void k_factor_fast(
         if (sys >= batch) break;
         const double *Ap = gA + (size_t)(sysidx ? sysidx[sys] : sys) * nnz;
         int status = 0;
-        wave_copy(F, Ap, nnz, lane);
+        wave_copy_wide<24>(F, Ap, nnz, lane);
         // the zero and one cells (the previous system's ISAI values may cover them)
         if (lane == 0) { F[nnz] = 0.0; F[nnz + 1] = 1.0; }
         if (mode == FS_PARILU) {
@@ -121,32 +121,33 @@ __global__ __launch_bounds__(KF_MAX_WAVES * WAVE) void k_factor_fast(
                 anorm2 = wave_sum(anorm2);
             }
             if (gY) {
-                // the warm start from the network's raw outputs y (row sys of gY):
-                // guess = SIGMA sinh(YS (base + y)) for the entries with work (a bit
-                // per position), the same operations as bilu_mlp_infer (guess_from);
-                // the others keep a_ij.  The diagonal of A goes to Dg first (FP32, as
-                // the base guess rounds it), so the positions are visited in one
-                // coalesced pass, 8 per lane in flight: a lower entry (i, j)
-                // (column j in lowcol, else 0xFFFF) reads its pivot a_jj from Dg.
+                // the warm start from the network's raw outputs y (row sys of gY, FP16):
+                // guess = SIGMA sinh(YS (base + y)) for the entries with work, the
+                // same operations as bilu_mlp_infer (guess_from); the others keep
+                // a_ij.  The diagonal of A goes to Dg first (FP32, as the base guess
+                // rounds it), so the positions are visited in one coalesced pass
+                // with WU outputs and position codes per lane in flight: a lower
+                // entry (i, j) (code = j) reads its pivot a_jj from Dg, an upper
+                // entry with work has code 0xFFFE, one without work 0xFFFF.
+                constexpr int WU = 24;
                 for (int j = lane; j < n; j += WAVE) Dg[j] = (float)F[fp.diag[j]];
                 wave_sync();
-                const float *Yp = gY + (size_t)sys * P3;
-                for (int e0 = lane; e0 < nnz; e0 += 8 * WAVE) {
-                    float y[8];
-                    uint32_t wbit[8], lc[8];
+                const _Float16 *Yp = gY + (size_t)sys * P3;
+                for (int e0 = lane; e0 < nnz; e0 += WU * WAVE) {
+                    float y[WU];
+                    uint32_t cd[WU];
 #pragma unroll
-                    for (int j = 0; j < 8; j++) {
+                    for (int j = 0; j < WU; j++) {
                         const int e = e0 + j * WAVE;
-                        y[j] = e < nnz ? Yp[e] : 0.0f;
-                        wbit[j] = e < nnz ? fp.workbits[e >> 5] : 0u;
-                        lc[j] = e < nnz ? fp.lowcol[e] : 0xFFFFu;
+                        y[j] = e < nnz ? (float)Yp[e] : 0.0f;
+                        cd[j] = e < nnz ? fp.wcode[e] : 0xFFFFu;
                     }
 #pragma unroll
-                    for (int j = 0; j < 8; j++) {
+                    for (int j = 0; j < WU; j++) {
                         const int e = e0 + j * WAVE;
-                        if (e < nnz && ((wbit[j] >> (e & 31)) & 1u)) {
-                            const bool lower = lc[j] != 0xFFFFu;
-                            F[e] = bilu_mlp::guess_from((float)F[e], lower ? Dg[lc[j]] : 1.0f, lower, y[j]);
+                        if (cd[j] != 0xFFFFu) {
+                            const bool lower = cd[j] != 0xFFFEu;
+                            F[e] = bilu_mlp::guess_from((float)F[e], lower ? Dg[cd[j]] : 1.0f, lower, y[j]);
                         }
                     }
                 }

@@ -45,7 +45,7 @@ using bilu_mlp::YS;
 using bilu_mlp::INV_SIGMA_F;
 using bilu_mlp::INV_YS;
 
-enum Epi { EPI_F32 = 0, EPI_BIAS_GELU = 1, EPI_BIAS = 2, EPI_GELU_BWD = 3 };
+enum Epi { EPI_F32 = 0, EPI_BIAS_GELU = 1, EPI_BIAS = 2, EPI_GELU_BWD = 3, EPI_BIAS_F16 = 4 };
 
 __device__ __forceinline__ float gelu(float z)
 {
@@ -63,7 +63,8 @@ __device__ __forceinline__ float gelu_grad(float z)
 
 // EPI_F32: C = acc; EPI_BIAS: C = acc + bias[n]; EPI_BIAS_GELU: Z = acc +
 // bias[n] (FP32, kept for the backward pass), H = gelu(Z) (BF16);
-// EPI_GELU_BWD: H = acc * gelu'(Z[m][n]) (BF16)
+// EPI_GELU_BWD: H = acc * gelu'(Z[m][n]) (BF16); EPI_BIAS_F16: C (as FP16,
+// pitch ldc) = acc + bias[n] rounded to FP16
 template <bool TA, bool TB, int EPI>
 __global__ __launch_bounds__(256) void k_gemm(const __bf16 *__restrict__ A, int lda, const __bf16 *__restrict__ B,
                                               int ldb, int K, float *__restrict__ C, int ldc,
@@ -155,6 +156,8 @@ __global__ __launch_bounds__(256) void k_gemm(const __bf16 *__restrict__ A, int 
                     C[o] = v;
                 } else if (EPI == EPI_BIAS) {
                     C[o] = v + bias[n];
+                } else if (EPI == EPI_BIAS_F16) {
+                    reinterpret_cast<_Float16 *>(C)[o] = (_Float16)(v + bias[n]);
                 } else if (EPI == EPI_BIAS_GELU) {
                     const float z = v + bias[n];
                     Z[o] = z;
@@ -285,7 +288,7 @@ __global__ __launch_bounds__(256) void k_outputs(const float *__restrict__ Y, co
             if (e0 + j * 256 < nnz) sa[e0 + j * 256] = (float)a[j];
     }
     __syncthreads();
-    const float *Ys = Y + (size_t)s * P3;
+    const float *Ys = Y + (size_t)s * P3;   // y is used rounded to FP16, as by the fused warm start
     double *Fs = Fo + (size_t)s * nnz;
     for (int e0 = threadIdx.x; e0 < nnz; e0 += U * 256) {
         float y[U];
@@ -301,7 +304,7 @@ __global__ __launch_bounds__(256) void k_outputs(const float *__restrict__ Y, co
             const int e = e0 + j * 256;
             if (e < nnz) {
                 const bool lower = pv[j] >= 0;
-                Fs[e] = bilu_mlp::guess_from(sa[e], lower ? sa[pv[j]] : 1.0f, lower, y[j]);
+                Fs[e] = bilu_mlp::guess_from(sa[e], lower ? sa[pv[j]] : 1.0f, lower, (float)(_Float16)y[j]);
             }
         }
     }
@@ -492,7 +495,7 @@ void copy_params(const bilu_mlp_s &q, const float *src, float *dst, bool to_pack
 int bilu_mlp_forward_rows() { return 65536; }
 
 int bilu_mlp_forward_raw(bilu_mlp_s *q, const void *pattern, int batch, const double *d_A, hipStream_t st,
-                         const float **Y, int *P3)
+                         const _Float16 **Y, int *P3)
 {
     if (!q || q->pattern != pattern || batch <= 0 || batch > bilu_mlp_forward_rows() || !d_A)
         return BILU_ERR_INVALID_ARG;
@@ -500,9 +503,14 @@ int bilu_mlp_forward_raw(bilu_mlp_s *q, const void *pattern, int batch, const do
     const int rc = reserve(*q, rows);
     if (rc != BILU_OK) return rc;
     hipLaunchKernelGGL(k_features, dim3(rows), dim3(256), 0, st, d_A, q->nnz, q->P0, batch, q->X);
-    forward(*q, rows, st);
+    // the three layers, the last one writing the outputs in FP16
+    const __bf16 *W1 = q->wb + q->off[0], *W2 = q->wb + q->off[2], *W3 = q->wb + q->off[4];
+    const float *b1 = q->w + q->off[1], *b2 = q->w + q->off[3], *b3 = q->w + q->off[5];
+    gemm<false, true, EPI_BIAS_GELU>(st, rows, q->H, q->P0, q->X, q->P0, W1, q->P0, nullptr, q->H, b1, q->Z1, q->H1);
+    gemm<false, true, EPI_BIAS_GELU>(st, rows, q->H, q->H, q->H1, q->H, W2, q->H, nullptr, q->H, b2, q->Z2, q->H2);
+    gemm<false, true, EPI_BIAS_F16>(st, rows, q->P3, q->H, q->H2, q->H, W3, q->H, q->Y, q->P3, b3);
     MLP_TRY(hipGetLastError());
-    *Y = q->Y;
+    *Y = reinterpret_cast<const _Float16 *>(q->Y);
     *P3 = q->P3;
     return BILU_OK;
 }

@@ -59,10 +59,12 @@ struct bilu_mlp_s;
 
 // Network outputs for the fused learned solve (defined in mlp.hip): the
 // feature map and the three GEMMs for batch <= bilu_mlp_forward_rows()
-// systems of d_A on `st`; *Y = the raw outputs [batch][*P3] (FP32, valid
-// until the network's next call).  Rejects (BILU_ERR_INVALID_ARG) a network
+// systems of d_A on `st`; *Y = the raw outputs [batch][*P3] rounded to FP16
+// (valid until the network's next call; the warm start uses y only through
+// e^(YS y), and FP16 keeps it to ~1e-3 relative, far inside the network's
+// error).  bilu_mlp_infer rounds y the same way.  Rejects (BILU_ERR_INVALID_ARG) a network
 // created on another pattern handle than `pattern`, before any launch reads
 // d_A.
 int bilu_mlp_forward_raw(bilu_mlp_s *q, const void *pattern, int batch, const double *d_A, hipStream_t st,
-                         const float **Y, int *P3);
+                         const _Float16 **Y, int *P3);
 int bilu_mlp_forward_rows();

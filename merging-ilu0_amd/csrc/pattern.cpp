@@ -525,12 +525,13 @@ void build_fast_plan(PatternAnalysis &pa)
     }
     fp.nlev_e = maxlev;
     fp.ne = int(fp.etab.size() / 4);
-    fp.workbits.assign(size_t(nnz + 31) / 32 + 1, 0u);
-    for (int q = 0; q < fp.ne; q++) fp.workbits[fp.etab[4 * q] >> 5] |= 1u << (fp.etab[4 * q] & 31);
-    fp.lowcol.assign(size_t(nnz) + 1, 0xFFFF);
+    // per position: the column j of a strictly lower entry (every one has
+    // work: a division), 0xFFFE for an upper entry with work, 0xFFFF without
+    fp.wcode.assign(size_t(nnz) + 1, 0xFFFF);
+    for (int q = 0; q < fp.ne; q++) fp.wcode[fp.etab[4 * q]] = 0xFFFE;
     for (int i = 0; i < n; i++)
         for (int p = pa.rp[i]; p < pa.rp[i + 1]; p++)
-            if (pa.ci[p] < i) fp.lowcol[p] = uint16_t(pa.ci[p]);
+            if (pa.ci[p] < i) fp.wcode[p] = uint16_t(pa.ci[p]);
 
     // ParILU sweep rounds: a product record holds the byte offsets of the pair
     // (l, u) in the source iterate (the zero cell nnz for padding) | 1 on the

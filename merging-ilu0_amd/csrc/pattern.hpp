@@ -59,8 +59,8 @@ struct FastPlan {
     std::vector<uint16_t> rlevL, rlevU;
     std::vector<uint16_t> etab, eptr, dep;   // etab: 4 u16 per record, dep: 2 u16 per pair
     int nlev_e = 0, ne = 0;
-    std::vector<uint32_t> workbits;   // bit e: position e is an entry with work (etab)
-    std::vector<uint16_t> lowcol;     // per position: column of a strictly lower entry, else 0xFFFF
+    std::vector<uint16_t> wcode;      // per position: column of a strictly lower entry, 0xFFFE an upper
+                                      // entry with work, 0xFFFF no work
     // ISAI(L) / ISAI(U) generation streams [step][64] (two u32 records per step)
     std::vector<uint32_t> isL0, isL1, isU0, isU1;
     int isL_steps = 0, isU_steps = 0;

@@ -105,6 +105,22 @@ __device__ __forceinline__ void wave_copy(TD *dst, const TS *__restrict__ src, i
     }
 }
 
+// wave_copy with U values per lane in flight: one memory round trip per
+// 64 U values (the persistent kernels run few waves per SIMD, so a copy's
+// latency is not hidden by other waves)
+template <int U, typename TD, typename TS>
+__device__ __forceinline__ void wave_copy_wide(TD *dst, const TS *__restrict__ src, int cnt, int lane)
+{
+    for (int e0 = lane; e0 < cnt; e0 += U * WAVE) {
+        TS v[U];
+#pragma unroll
+        for (int j = 0; j < U; j++) v[j] = (e0 + j * WAVE < cnt) ? src[e0 + j * WAVE] : TS(0);
+#pragma unroll
+        for (int j = 0; j < U; j++)
+            if (e0 + j * WAVE < cnt) dst[e0 + j * WAVE] = (TD)v[j];
+    }
+}
+
 // ---------------------------------------------------------------- wave ops
 // 64-bit lane exchanges without the LDS pipe: DPP within 16-lane rows,
 // gfx950 permlane swaps across rows / halves
@@ -961,8 +977,7 @@ struct bilu_pattern_s {
     int32_t *d_isl_pos = nullptr, *d_isu_pos = nullptr;
     uint32_t *d_ilu0 = nullptr, *d_ilu1 = nullptr;
     uint32_t *d_parp = nullptr, *d_parf = nullptr;
-    uint32_t *d_workbits = nullptr;
-    uint16_t *d_lowcol = nullptr;
+    uint16_t *d_wcode = nullptr;
     int *d_counter = nullptr;      // 256 bytes of work counters: [0] solve kernels, [32] factor kernel
     int ilu_sched = 0;             // exact ILU(0) schedule chosen by bilu_ilu0_tune (0: none yet)
     // solve workspace (factors between the factor and solve launches), grown on demand
@@ -1076,8 +1091,7 @@ int upload(bilu_pattern_s &h)
         const size_t o_i1 = put(fp.ilu1.data(), fp.ilu1.size() * 4);
         const size_t o_p0 = put(fp.parp.data(), fp.parp.size() * 4);
         const size_t o_p1 = put(fp.parf.data(), fp.parf.size() * 4);
-        const size_t o_wb = put(fp.workbits.data(), fp.workbits.size() * 4);
-        const size_t o_lc = put(fp.lowcol.data(), fp.lowcol.size() * 2);
+        const size_t o_wc = put(fp.wcode.data(), fp.wcode.size() * 2);
         HIP_TRY(hipMalloc((void **)&h.d_fast, blob.size()));
         HIP_TRY(hipMemcpy(h.d_fast, blob.data(), blob.size(), hipMemcpyHostToDevice));
         uint8_t *b = h.d_fast;
@@ -1092,8 +1106,7 @@ int upload(bilu_pattern_s &h)
         h.d_isl_pos = (int32_t *)(b + o_lp); h.d_isu_pos = (int32_t *)(b + o_up);
         h.d_ilu0 = (uint32_t *)(b + o_i0); h.d_ilu1 = (uint32_t *)(b + o_i1);
         h.d_parp = (uint32_t *)(b + o_p0); h.d_parf = (uint32_t *)(b + o_p1);
-        h.d_workbits = (uint32_t *)(b + o_wb);
-        h.d_lowcol = (uint16_t *)(b + o_lc);
+        h.d_wcode = (uint16_t *)(b + o_wc);
     }
     return BILU_OK;
 }
@@ -1148,8 +1161,7 @@ void fast_tables(const bilu_pattern_s &h, FastParams &fp)
     fp.cntL = h.d_cntL; fp.cntU = h.d_cntU;
     fp.rlevL = h.d_rlevL; fp.rlevU = h.d_rlevU;
     fp.etab = reinterpret_cast<const ushort4 *>(h.d_etab); fp.eptr = h.d_eptr;
-    fp.workbits = h.d_workbits;
-    fp.lowcol = h.d_lowcol;
+    fp.wcode = h.d_wcode;
     fp.dep = reinterpret_cast<const ushort2 *>(h.d_dep);
     fp.diag = h.dev.diag;
     fp.nlev_l = h.dev.nlev_l; fp.nlev_u = h.dev.nlev_u; fp.nlev_e = pl.nlev_e; fp.ne = pl.ne;
@@ -1309,7 +1321,7 @@ template <typename TO>
 int launch_factor_fast(bilu_pattern_s &h, int batch, const double *A, const double *F_init, TO *F,
                        int mode, int sweeps, double stop_tol, int *status, hipStream_t st,
                        int isai = 0, float *M = nullptr, int *Z = nullptr, const int *sysidx = nullptr,
-                       int max_waves = KF_MAX_WAVES, const float *Y = nullptr, int P3 = 0, int inplace = 0)
+                       int max_waves = KF_MAX_WAVES, const _Float16 *Y = nullptr, int P3 = 0, int inplace = 0)
 {
     const auto &pl = h.pa.fast;
     if (!pl.ok) return BILU_ERR_TOO_LARGE;
@@ -1920,7 +1932,7 @@ int bilu_mlp_solve(bilu_mlp_t m, bilu_pattern_t h, int batch, const double *d_A,
     for (int s0 = 0; s0 < batch; s0 += chunk) {
         const int cb = std::min(chunk, batch - s0);
         const double *A = d_A + (size_t)s0 * nnz;
-        const float *Y = nullptr;
+        const _Float16 *Y = nullptr;
         int P3 = 0;
         rc = bilu_mlp_forward_raw(m, h, cb, A, st, &Y, &P3);
         if (rc != BILU_OK) return rc;

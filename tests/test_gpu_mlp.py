@@ -84,16 +84,24 @@ def test_mlp_forward_matches_oracle():
 
 
 def test_mlp_training_gain():
-    """Train on exact factors of fresh systems (16384 here; the benchmark's
-    learned mode trains on 262144), then ParILU(0) sweeps to a factor
-    residual of 1e-8 from the network's guess against the standard start
-    (A): the warm start must save sweeps, and the solves stay exact."""
+    """Train on exact factors of fresh systems (16384 here; the benchmark
+    trains on 262144 for 30 epochs), then count in-place ParILU(0) sweeps to
+    a factor residual of 1e-8 on held-out systems from three starts: A (no
+    warm start), the untrained network (its base guess a_ij, a_ij / a_jj)
+    and the trained network.  What is asserted: the warm start saves sweeps
+    against A, training lowers the loss, the trained guess stays within a
+    small margin of its base guess, and every solve stays exact.  What is
+    reported: trained against base -- on these i.i.d. systems the network
+    does not beat its own base guess (the base guess is already within
+    1e-4 relative on most entries; even exact values on every entry that
+    the base guess gets wrong by 0.1 % save only ~0.1 sweep)."""
     print("case mlp training gain", flush=True)
     P_pat, A, b = synth.batch("random", 16384, first=1 << 20, n=144)
     P = bilu.Pattern.from_synth(P_pat)
     dA = to_dev(A)
     dF = exact_factors(P, dA)
     net = bilu.Mlp(P, hidden=512, seed=11)
+    net0 = bilu.Mlp(P, hidden=512, seed=11)
     loss = torch.zeros(1, dtype=torch.float32, device=dev())
     losses = []
     mb = 1024
@@ -105,29 +113,37 @@ def test_mlp_training_gain():
     torch.cuda.synchronize()
     print(f"training loss {losses}", flush=True)
     assert losses[-1] < losses[0]
-    pat, At, bt = synth.batch("random", 512, n=144)        # held out
+    pat, At, bt = synth.batch("random", 2048, n=144)        # held out
     dAt, dbt = to_dev(At), to_dev(bt)
-    guess = net.infer(dAt)
-    kw = dict(precond=bilu.PARILU0, parilu_sweeps=60, parilu_stop_tol=1e-8, tol=1e-10,
-              tol_mode=bilu.TOL_RELATIVE, precision=bilu.PRECISION_FP32_IR)
+    kw = dict(precond=bilu.PARILU0_ISAI, parilu_sweeps=60, parilu_stop_tol=1e-8, tol=1e-10,
+              tol_mode=bilu.TOL_RELATIVE, precision=bilu.PRECISION_FP32_IR, parilu_order=bilu.PARILU_INPLACE)
+    o = bilu.opts(**kw)
     x = torch.empty_like(dbt)
-    _, cv1, _, tr1, _, fs1 = bilu.solve_ex(P, dAt, dbt, x, bilu.opts(**kw), F_init=guess)
-    _, cv0, _, tr0, _, fs0 = bilu.solve_ex(P, dAt, dbt, x, bilu.opts(**kw))
-    torch.cuda.synchronize()
-    s1, s0 = fs1.cpu().numpy(), fs0.cpu().numpy()
-    print(f"ParILU sweeps to 1e-8: from A {s0.mean():.2f}, from the network {s1.mean():.2f}", flush=True)
-    assert (s0 > 0).all() and (s1 >= 0).all()
-    assert s1.mean() < s0.mean()
     bn = np.linalg.norm(bt, axis=1)
-    assert (cv1.cpu().numpy() == 1).all() and (tr1.cpu().numpy() / bn).max() <= 1e-10
-    # the guess is closer to the exact factors than the standard start (A);
-    # its error against the untrained network's base guess is reported
+
+    def check(r):
+        torch.cuda.synchronize()
+        assert (r[1].cpu().numpy() == 1).all() and (r[3].cpu().numpy() / bn).max() <= 1e-10
+        fs = r[5].cpu().numpy()
+        assert (fs > 0).all()
+        return fs.mean()
+    s_a = check(bilu.solve_ex(P, dAt, dbt, x, o, factor_status=torch.empty(2048, dtype=torch.int32, device=dev())))
+    s_base = check(net0.solve(P, dAt, dbt, x, o))
+    s_net = check(net.solve(P, dAt, dbt, x, o))
+    print(f"in-place ParILU sweeps to 1e-8: from A {s_a:.3f}, base guess {s_base:.3f}, trained network "
+          f"{s_net:.3f}", flush=True)
+    assert s_base < s_a
+    assert s_net < s_base + 0.25
+    # the scaled error of the guess: trained network vs base guess vs A (reported)
+    guess = net.infer(dAt)
     Fe = exact_factors(P, dAt).cpu().numpy()
     eg = np.abs(MO.targets(guess.cpu().numpy()) - MO.targets(Fe)).mean()
     eb = np.abs(MO.base(pat, At) - MO.targets(Fe)).mean()
     ea = np.abs(MO.targets(At) - MO.targets(Fe)).mean()
     print(f"mean scaled error: trained guess {eg:.4f}, base guess {eb:.4f}, A {ea:.4f}", flush=True)
     assert eg < ea
+    net.close()
+    net0.close()
 
 
 def _fused_vs_split(P, net, dA, db, precond, sweeps=40, stop_tol=1e-8, order=bilu.PARILU_JACOBI):
