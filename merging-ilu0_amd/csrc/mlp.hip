@@ -15,7 +15,8 @@
 // relative errors in F).  Weights are kept in FP32 (master, Adam moments)
 // and copied to BF16 for the GEMMs.
 //
-// GEMM: C[m][n] = sum_k A(m, k) B(k, n), A(m, k) = TA ? A[k lda + m] :
+// GEMMs: the forward products (both operands k-contiguous) run k_gemm_nt,
+// the backward ones k_gemm.  k_gemm: C[m][n] = sum_k A(m, k) B(k, n), A(m, k) = TA ? A[k lda + m] :
 // A[m lda + k], B(k, n) = TB ? B[n ldb + k] : B[k ldb + n], all dimensions
 // multiples of the tile (buffers are padded).  256 threads = 4 waves, tile
 // 128 x 128 x 64: each wave owns a 64 x 64 quarter as 4 x 4 MFMA
@@ -29,6 +30,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/bilu.h"
@@ -45,7 +47,7 @@ using bilu_mlp::YS;
 using bilu_mlp::INV_SIGMA_F;
 using bilu_mlp::INV_YS;
 
-enum Epi { EPI_F32 = 0, EPI_BIAS_GELU = 1, EPI_BIAS = 2, EPI_GELU_BWD = 3, EPI_BIAS_F16 = 4 };
+enum Epi { EPI_F32 = 0, EPI_BIAS_GELU = 1, EPI_BIAS = 2, EPI_GELU_BWD = 3, EPI_BIAS_F16 = 4, EPI_BIAS_GELU_H = 5 };
 
 __device__ __forceinline__ float gelu(float z)
 {
@@ -64,7 +66,8 @@ __device__ __forceinline__ float gelu_grad(float z)
 // EPI_F32: C = acc; EPI_BIAS: C = acc + bias[n]; EPI_BIAS_GELU: Z = acc +
 // bias[n] (FP32, kept for the backward pass), H = gelu(Z) (BF16);
 // EPI_GELU_BWD: H = acc * gelu'(Z[m][n]) (BF16); EPI_BIAS_F16: C (as FP16,
-// pitch ldc) = acc + bias[n] rounded to FP16
+// pitch ldc) = acc + bias[n] rounded to FP16; EPI_BIAS_GELU_H: H only
+// (inference keeps no Z)
 template <bool TA, bool TB, int EPI>
 __global__ __launch_bounds__(256) void k_gemm(const __bf16 *__restrict__ A, int lda, const __bf16 *__restrict__ B,
                                               int ldb, int K, float *__restrict__ C, int ldc,
@@ -149,7 +152,7 @@ __global__ __launch_bounds__(256) void k_gemm(const __bf16 *__restrict__ A, int 
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 const int m = m0 + wm + 16 * i + 4 * (lane / 16) + r;
-                const int n = n0 + wn + 16 * j + lane % 16;
+                const int n = n0 + 16 * j + lane % 16;
                 const float v = acc[i][j][r];
                 const size_t o = (size_t)m * ldc + n;
                 if (EPI == EPI_F32) {
@@ -164,6 +167,131 @@ __global__ __launch_bounds__(256) void k_gemm(const __bf16 *__restrict__ A, int 
                     H[o] = (__bf16)gelu(z);
                 } else {
                     H[o] = (__bf16)(v * gelu_grad(Z[o]));
+                }
+            }
+}
+
+// Forward GEMM C = A B^T (+ epilogue) with both operands k-contiguous: A
+// [M][K] (lda), B [N][K] (ldb); M, N, K multiples of FM, FN, FK.  256
+// threads = 4 waves, block tile FM x FN = 256 x 128: wave w owns rows
+// 64 w .. 64 w + 63 and all 128 columns as 4 x 8 MFMA 16x16x32 blocks, so
+// each LDS fragment of B feeds 4 MFMAs and each register fragment of A 8.
+// A's fragments belong to one wave only: they are loaded straight from
+// global memory into registers in the MFMA layout (one 16-byte load per
+// fragment), two 32-wide k steps ahead.  B's 128 x 32 step tile is shared
+// by the four waves and staged in LDS, double buffered (one barrier per k
+// step), its global loads also issued two steps ahead.  Measured at 65536
+// rows the kernel spends most of its time waiting on those loads (about 27%
+// of the BF16 MFMA peak; a 256 x 256 block tile with 8 waves was no faster).
+// Every accumulator receives the same MFMAs in the same k order as in
+// k_gemm, so both kernels give the same bits.  Blocks are renumbered so that
+// consecutive tiles (the column tiles of one row tile) run on one XCD and
+// share its L2 (dispatch places block b on XCD b mod 8).
+constexpr int FM = 256, FN = 128, FK = 32, FKP = FK + 8;
+template <int EPI>
+__global__ __launch_bounds__(256, 2) void k_gemm_nt(const __bf16 *__restrict__ A, int lda,
+                                                    const __bf16 *__restrict__ B, int ldb, int K, int ntn,
+                                                    float *__restrict__ C, int ldc, const float *__restrict__ bias,
+                                                    float *__restrict__ Z, __bf16 *__restrict__ H)
+{
+    __shared__ __align__(16) __bf16 Bs[2][FN * FKP];
+    const int t = threadIdx.x, lane = t % 64, wave = t / 64;
+    const int nb = (int)gridDim.x, b = (int)blockIdx.x;
+    const int lg = (nb % 8 == 0) ? (b % 8) * (nb / 8) + b / 8 : b;
+    const int m0 = (lg / ntn) * FM + wave * 64, n0 = (lg % ntn) * FN;
+    floatx4 acc[4][8];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 8; j++) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+    // lane's A fragment i of k step s: row m0 + 16 i + lane % 16, k = 32 s + 8 (lane / 16) .. + 7
+    const __bf16 *Aw = A + (size_t)(m0 + lane % 16) * lda + 8 * (lane / 16);
+    constexpr int KV = FK / 8, VB = FN * KV / 256;   // B step tile: FN rows x KV vectors, VB per thread
+    bf16x8 fa[3][4], gb[2][VB];
+    auto aload = [&](int s, bf16x8 (&f)[4]) {
+#pragma unroll
+        for (int i = 0; i < 4; i++) f[i] = *reinterpret_cast<const bf16x8 *>(Aw + (size_t)16 * i * lda + FK * s);
+    };
+    auto bload = [&](int s, bf16x8 (&g)[VB]) {
+#pragma unroll
+        for (int v = 0; v < VB; v++) {
+            const int id = t + 256 * v, r = id / KV, c = (id % KV) * 8;
+            g[v] = *reinterpret_cast<const bf16x8 *>(&B[(size_t)(n0 + r) * ldb + FK * s + c]);
+        }
+    };
+    auto bstore = [&](const bf16x8 (&g)[VB], __bf16 *S) {
+#pragma unroll
+        for (int v = 0; v < VB; v++) {
+            const int id = t + 256 * v, r = id / KV, c = (id % KV) * 8;
+            *reinterpret_cast<bf16x8 *>(&S[r * FKP + c]) = g[v];
+        }
+    };
+    // k step s: A(s + 2) and B(s + 2) loads issued (two steps of MFMAs hide
+    // their latency), the MFMAs of step s, B(s + 1) (loaded in step s - 1)
+    // stored to the LDS buffer that step s - 1 read, one barrier.  The loads
+    // and stores are unconditional (steps past the end reload step S - 1 and
+    // store into the idle buffer), so every step issues the same memory
+    // operations and the compiler's wait counts stay exact across steps.
+    const int S = K / FK;
+    auto step = [&](auto R, int s) {
+        constexpr int r = decltype(R)::value;
+        const int sn = min(s + 2, S - 1);
+        aload(sn, fa[(r + 2) % 3]);
+        bload(sn, gb[r % 2]);
+        __builtin_amdgcn_sched_barrier(0);   // the loads stay ahead of the MFMAs
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const bf16x8 bf =
+                *reinterpret_cast<const bf16x8 *>(&Bs[r % 2][(16 * j + lane % 16) * FKP + 8 * (lane / 16)]);
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[r % 3][i], bf, acc[i][j], 0, 0, 0);
+        }
+        bstore(gb[(r + 1) % 2], Bs[(r + 1) % 2]);
+        __syncthreads();
+    };
+    aload(0, fa[0]);
+    bload(0, gb[0]);
+    aload(min(1, S - 1), fa[1]);
+    bload(min(1, S - 1), gb[1]);
+    bstore(gb[0], Bs[0]);
+    __syncthreads();
+    int s0 = 0;
+    for (; s0 + 6 <= S; s0 += 6) {
+        step(std::integral_constant<int, 0>(), s0);
+        step(std::integral_constant<int, 1>(), s0 + 1);
+        step(std::integral_constant<int, 2>(), s0 + 2);
+        step(std::integral_constant<int, 3>(), s0 + 3);
+        step(std::integral_constant<int, 4>(), s0 + 4);
+        step(std::integral_constant<int, 5>(), s0 + 5);
+    }
+    if (s0 < S) step(std::integral_constant<int, 0>(), s0);
+    if (s0 + 1 < S) step(std::integral_constant<int, 1>(), s0 + 1);
+    if (s0 + 2 < S) step(std::integral_constant<int, 2>(), s0 + 2);
+    if (s0 + 3 < S) step(std::integral_constant<int, 3>(), s0 + 3);
+    if (s0 + 4 < S) step(std::integral_constant<int, 4>(), s0 + 4);
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 8; j++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int m = m0 + 16 * i + 4 * (lane / 16) + r;
+                const int n = n0 + 16 * j + lane % 16;
+                const float v = acc[i][j][r];
+                const size_t o = (size_t)m * ldc + n;
+                if (EPI == EPI_BIAS) {
+                    C[o] = v + bias[n];
+                } else if (EPI == EPI_BIAS_F16) {
+                    reinterpret_cast<_Float16 *>(C)[o] = (_Float16)(v + bias[n]);
+                } else if (EPI == EPI_BIAS_GELU) {
+                    const float z = v + bias[n];
+                    Z[o] = z;
+                    H[o] = (__bf16)gelu(z);
+                } else if (EPI == EPI_BIAS_GELU_H) {
+                    H[o] = (__bf16)gelu(v + bias[n]);
+                } else {
+                    C[o] = v;
                 }
             }
 }
@@ -411,6 +539,14 @@ template <bool TA, bool TB, int EPI>
 void gemm(hipStream_t st, int M, int N, int K, const __bf16 *A, int lda, const __bf16 *B, int ldb, float *C,
           int ldc, const float *bias = nullptr, float *Z = nullptr, __bf16 *H = nullptr)
 {
+    if constexpr (!TA && TB) {
+        if (M % FM == 0 && N % FN == 0 && K % FK == 0) {
+            hipLaunchKernelGGL((k_gemm_nt<EPI>), dim3((M / FM) * (N / FN)), dim3(256), 0, st, A, lda, B, ldb, K,
+                               N / FN, C, ldc, bias, Z, H);
+            return;
+        }
+    }
+    static_assert((EPI != EPI_BIAS_GELU_H && EPI != EPI_BIAS_F16) || (!TA && TB), "forward epilogues");
     hipLaunchKernelGGL((k_gemm<TA, TB, EPI>), dim3(N / TN, M / TM), dim3(256), 0, st, A, lda, B, ldb, K, C, ldc,
                        bias, Z, H);
 }
@@ -421,13 +557,19 @@ void colsum(hipStream_t st, const __bf16 *D, int M, int N, float *part, float *g
     hipLaunchKernelGGL(k_colsum_final, dim3(blocks(N)), dim3(256), 0, st, part, N, g);
 }
 
-// forward of `rows` padded rows (features already in X): Y
-void forward(bilu_mlp_s &q, int rows, hipStream_t st)
+// forward of `rows` padded rows (features already in X): Y (FP32); `keep`:
+// the pre-activations Z1, Z2 for the backward pass
+void forward(bilu_mlp_s &q, int rows, hipStream_t st, bool keep = true)
 {
     const __bf16 *W1 = q.wb + q.off[0], *W2 = q.wb + q.off[2], *W3 = q.wb + q.off[4];
     const float *b1 = q.w + q.off[1], *b2 = q.w + q.off[3], *b3 = q.w + q.off[5];
-    gemm<false, true, EPI_BIAS_GELU>(st, rows, q.H, q.P0, q.X, q.P0, W1, q.P0, nullptr, q.H, b1, q.Z1, q.H1);
-    gemm<false, true, EPI_BIAS_GELU>(st, rows, q.H, q.H, q.H1, q.H, W2, q.H, nullptr, q.H, b2, q.Z2, q.H2);
+    if (keep) {
+        gemm<false, true, EPI_BIAS_GELU>(st, rows, q.H, q.P0, q.X, q.P0, W1, q.P0, nullptr, q.H, b1, q.Z1, q.H1);
+        gemm<false, true, EPI_BIAS_GELU>(st, rows, q.H, q.H, q.H1, q.H, W2, q.H, nullptr, q.H, b2, q.Z2, q.H2);
+    } else {
+        gemm<false, true, EPI_BIAS_GELU_H>(st, rows, q.H, q.P0, q.X, q.P0, W1, q.P0, nullptr, q.H, b1, nullptr, q.H1);
+        gemm<false, true, EPI_BIAS_GELU_H>(st, rows, q.H, q.H, q.H1, q.H, W2, q.H, nullptr, q.H, b2, nullptr, q.H2);
+    }
     gemm<false, true, EPI_BIAS>(st, rows, q.P3, q.H, q.H2, q.H, W3, q.H, q.Y, q.P3, b3);
 }
 
@@ -435,7 +577,7 @@ void forward(bilu_mlp_s &q, int rows, hipStream_t st)
 // and zero loss gradient)
 int loss_and_grad(bilu_mlp_s &q, int batch, const double *A, const double *F, float *d_loss, hipStream_t st)
 {
-    const int rows = pad_to(batch, TM);
+    const int rows = pad_to(batch, FM);
     int rc = reserve(q, rows);
     if (rc != BILU_OK) return rc;
     hipLaunchKernelGGL(k_features, dim3(rows), dim3(256), 0, st, A, q.nnz, q.P0, batch, q.X);
@@ -499,15 +641,15 @@ int bilu_mlp_forward_raw(bilu_mlp_s *q, const void *pattern, int batch, const do
 {
     if (!q || q->pattern != pattern || batch <= 0 || batch > bilu_mlp_forward_rows() || !d_A)
         return BILU_ERR_INVALID_ARG;
-    const int rows = pad_to(batch, TM);
+    const int rows = pad_to(batch, FM);
     const int rc = reserve(*q, rows);
     if (rc != BILU_OK) return rc;
     hipLaunchKernelGGL(k_features, dim3(rows), dim3(256), 0, st, d_A, q->nnz, q->P0, batch, q->X);
     // the three layers, the last one writing the outputs in FP16
     const __bf16 *W1 = q->wb + q->off[0], *W2 = q->wb + q->off[2], *W3 = q->wb + q->off[4];
     const float *b1 = q->w + q->off[1], *b2 = q->w + q->off[3], *b3 = q->w + q->off[5];
-    gemm<false, true, EPI_BIAS_GELU>(st, rows, q->H, q->P0, q->X, q->P0, W1, q->P0, nullptr, q->H, b1, q->Z1, q->H1);
-    gemm<false, true, EPI_BIAS_GELU>(st, rows, q->H, q->H, q->H1, q->H, W2, q->H, nullptr, q->H, b2, q->Z2, q->H2);
+    gemm<false, true, EPI_BIAS_GELU_H>(st, rows, q->H, q->P0, q->X, q->P0, W1, q->P0, nullptr, q->H, b1, nullptr, q->H1);
+    gemm<false, true, EPI_BIAS_GELU_H>(st, rows, q->H, q->H, q->H1, q->H, W2, q->H, nullptr, q->H, b2, nullptr, q->H2);
     gemm<false, true, EPI_BIAS_F16>(st, rows, q->P3, q->H, q->H2, q->H, W3, q->H, q->Y, q->P3, b3);
     MLP_TRY(hipGetLastError());
     *Y = reinterpret_cast<const _Float16 *>(q->Y);
@@ -670,13 +812,13 @@ int bilu_mlp_infer(bilu_mlp_t q, int batch, const double *d_A, double *d_F_init,
     hipStream_t st = (hipStream_t)stream;
     // in chunks, so the activations stay bounded
     const int chunk = bilu_mlp_forward_rows();
-    int rc = reserve(*q, std::min(pad_to(batch, TM), chunk));
+    int rc = reserve(*q, std::min(pad_to(batch, FM), chunk));
     if (rc != BILU_OK) return rc;
     for (int s0 = 0; s0 < batch; s0 += chunk) {
-        const int nb = std::min(chunk, batch - s0), rows = pad_to(nb, TM);
+        const int nb = std::min(chunk, batch - s0), rows = pad_to(nb, FM);
         hipLaunchKernelGGL(k_features, dim3(rows), dim3(256), 0, st,
                            d_A + (size_t)s0 * q->nnz, q->nnz, q->P0, nb, q->X);
-        forward(*q, rows, st);
+        forward(*q, rows, st, false);
         hipLaunchKernelGGL(k_outputs, dim3(nb), dim3(256), (size_t)q->nnz * sizeof(float), st, q->Y,
                            d_A + (size_t)s0 * q->nnz, q->piv, q->nnz, q->P3, nb, d_F_init + (size_t)s0 * q->nnz);
     }
