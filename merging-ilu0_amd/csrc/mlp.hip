@@ -16,8 +16,9 @@
 // and copied to BF16 for the GEMMs.
 //
 // GEMMs: the forward products (both operands k-contiguous) run k_gemm_nt,
-// the backward ones k_gemm.  k_gemm: C[m][n] = sum_k A(m, k) B(k, n), A(m, k) = TA ? A[k lda + m] :
-// A[m lda + k], B(k, n) = TB ? B[n ldb + k] : B[k ldb + n], all dimensions
+// the backward ones k_gemm.  k_gemm: C[m][n] = sum_k A(m, k) B(k, n),
+// A(m, k) = TA ? A[k lda + m] : A[m lda + k], B(k, n) = TB ? B[n ldb + k] :
+// B[k ldb + n], all dimensions
 // multiples of the tile (buffers are padded).  256 threads = 4 waves, tile
 // 128 x 128 x 64: each wave owns a 64 x 64 quarter as 4 x 4 MFMA
 // 16x16x32 blocks; the next k tile's global loads are issued before the
@@ -152,7 +153,7 @@ __global__ __launch_bounds__(256) void k_gemm(const __bf16 *__restrict__ A, int 
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 const int m = m0 + wm + 16 * i + 4 * (lane / 16) + r;
-                const int n = n0 + 16 * j + lane % 16;
+                const int n = n0 + wn + 16 * j + lane % 16;
                 const float v = acc[i][j][r];
                 const size_t o = (size_t)m * ldc + n;
                 if (EPI == EPI_F32) {

@@ -38,7 +38,7 @@
 namespace {
 
 constexpr int WAVE = 64;
-constexpr int MAX_NR = 5;        // rows per lane in the register layout: n <= 320
+constexpr int MAX_NR = 8;        // rows per lane in the register layout: n <= 512
 constexpr int ISAI_GROUP = 32;   // lanes per ISAI row (reference row_size_limit 32)
 
 enum PcKind { PC_LU = 0, PC_ISAI_LU = 1, PC_ISAI_GEN = 2, PC_NONE = 3, PC_ISAI_POW = 4, PC_ISAI_GEN_POW = 5 };
@@ -1463,6 +1463,9 @@ int launch_solve(bilu_pattern_s &h, int batch, const double *A, const double *b,
     case 3: return launch_solve_nr<3, PC>(h, batch, A, b, x, F, sp, it, cv, ir, tr, lds, st, QL, QU);
     case 4: return launch_solve_nr<4, PC>(h, batch, A, b, x, F, sp, it, cv, ir, tr, lds, st, QL, QU);
     case 5: return launch_solve_nr<5, PC>(h, batch, A, b, x, F, sp, it, cv, ir, tr, lds, st, QL, QU);
+    case 6: return launch_solve_nr<6, PC>(h, batch, A, b, x, F, sp, it, cv, ir, tr, lds, st, QL, QU);
+    case 7: return launch_solve_nr<7, PC>(h, batch, A, b, x, F, sp, it, cv, ir, tr, lds, st, QL, QU);
+    case 8: return launch_solve_nr<8, PC>(h, batch, A, b, x, F, sp, it, cv, ir, tr, lds, st, QL, QU);
     default: return BILU_ERR_TOO_LARGE;
     }
 }
@@ -2148,7 +2151,7 @@ int bilu_true_residual(bilu_pattern_t h, int batch, const double *d_A, const dou
         hipLaunchKernelGGL(k_true_residual<R>, dim3(batch), dim3(WAVE), lds, st, h->dev, batch, d_A, d_b, d_x, d_res); \
         break;
     switch (nr) {
-        TR_CASE(1) TR_CASE(2) TR_CASE(3) TR_CASE(4) TR_CASE(5)
+        TR_CASE(1) TR_CASE(2) TR_CASE(3) TR_CASE(4) TR_CASE(5) TR_CASE(6) TR_CASE(7) TR_CASE(8)
     default: return BILU_ERR_TOO_LARGE;
     }
 #undef TR_CASE

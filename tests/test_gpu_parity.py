@@ -225,7 +225,7 @@ def test_empty_batch_and_bad_args():
         bilu.Pattern(3, [0, 1, 2, 3], [0, 2, 1])        # missing diagonal in rows 1, 2
     with pytest.raises(bilu.BiluError):
         bilu.Pattern(2, [0, 2, 3], [1, 0, 1])           # unsorted row
-    big = synth.random_pattern(400, 0.01)
+    big = synth.random_pattern(513, 0.01)                 # beyond the 512-row cap
     with pytest.raises(bilu.BiluError):
         Pb = bilu.Pattern.from_synth(big)
         bilu.solve(Pb, torch.zeros(1, big.nnz, dtype=torch.float64, device=dev()),
