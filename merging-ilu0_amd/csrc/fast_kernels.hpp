@@ -43,6 +43,7 @@ struct FastParams {
     int shared_bytes;           // shared LDS: SpMV meta [SM][64], trL, trU (16-byte aligned)
     int o_trL, o_trU;
     int wave_bytes;
+    int f_cells;    // k_fast: FP64 cells of a wave region ahead of its vectors
     int par_w;      // 1: the wave region holds a second factor buffer (ParILU Jacobi)
     int *counter;
     // ISAI generation streams [step][64] and the slot triangles (ISAI applies)
@@ -115,6 +116,54 @@ __device__ __forceinline__ void spmv_slots(const T (&a)[SM], const uint32_t *__r
         T *dst = end ? reinterpret_cast<T *>(yb + ((m[s] >> 16) >> SH)) : scratch;
         *dst = acc;
         acc = end ? T(0) : acc;
+    }
+}
+
+// y = C x over a slot layout whose FP64 coefficients C sit in LDS
+// slot-major (C[s 64 + lane]), ns slots (wave-uniform); lanes hold whole
+// rows, each row an fma chain left to right from 0.  TRI: the triangle
+// records of the ISAI applies (FP32 byte offsets, doubled here: x in bits
+// 0-15, the output cell in bits 16-30 -- the scratch cell n + 1 of y while
+// the row continues -- and bit 31 set while it continues); otherwise A's slot
+// records (FP64 byte offsets: x in bits 0-15, bits 16-31 the row's cell or
+// 0xFFFF while the row continues, its running sum then going to `scratch`).
+// Records, coefficients and x values of G slots are read ahead of their fmas.
+template <bool TRI>
+__device__ __forceinline__ void spmv_coef(const double *__restrict__ C, int ns, const uint32_t *__restrict__ rec,
+                                          const double *__restrict__ x, double *__restrict__ y,
+                                          double *__restrict__ scratch, int lane)
+{
+    constexpr int G = 4;
+    const char *xb = reinterpret_cast<const char *>(x);
+    char *yb = reinterpret_cast<char *>(y);
+    double acc = 0.0;
+    for (int s0 = 0; s0 < ns; s0 += G) {
+        uint32_t r[G];
+        double xv[G], cv[G];
+#pragma unroll
+        for (int j = 0; j < G; j++) {
+            const int s = min(s0 + j, ns - 1);   // past the end: the last slot again, not used
+            r[j] = rec[s * WAVE + lane];
+            cv[j] = C[s * WAVE + lane];
+            xv[j] = *reinterpret_cast<const double *>(xb + (TRI ? (r[j] & 0xFFFFu) << 1 : (r[j] & 0xFFFFu)));
+        }
+#pragma unroll
+        for (int j = 0; j < G; j++) {
+            if (s0 + j < ns) {
+                acc = fma(cv[j], xv[j], acc);
+                bool end;
+                double *dst;
+                if constexpr (TRI) {
+                    end = !(r[j] >> 31);
+                    dst = reinterpret_cast<double *>(yb + (((r[j] >> 16) & 0x7FFFu) << 1));
+                } else {
+                    end = (r[j] >> 16) != 0xFFFFu;
+                    dst = end ? reinterpret_cast<double *>(yb + (r[j] >> 16)) : scratch;
+                }
+                *dst = acc;
+                acc = end ? 0.0 : acc;
+            }
+        }
     }
 }
 
@@ -426,31 +475,53 @@ __device__ __forceinline__ void parilu_sweep(const ushort4 *__restrict__ etab, i
     wave_sync();
 }
 
-template <int NR, int SM>
+// Preconditioners of k_fast: FPC_LU = the in-place LU factors (built in the
+// kernel or loaded), applied by triangular solves; FPC_ISAI = ISAI(L) and
+// ISAI(U) values on the pattern (strictly lower = ISAI(L), its unit diagonal
+// implicit; diagonal and upper = ISAI(U)) with zero-pivot flags, applied as
+// two SpMVs over the triangle slot layouts; FPC_GEN = the general ISAI of A
+// on its pattern with flags, applied as an SpMV over A's slot layout.  The
+// ISAI values come from a separate generation launch.
+constexpr int FPC_LU = 0, FPC_ISAI = 1, FPC_GEN = 2;
+
+template <int NR, int SM, int PC = FPC_LU>
 __global__ __launch_bounds__(KF_MAX_WAVES * WAVE) void k_fast(DevPattern P, FastParams fp, int batch,
                                               const double *__restrict__ gA, const double *__restrict__ gb,
                                               double *__restrict__ gx, const double *__restrict__ gF,
                                               SolveParams sp, float *__restrict__ g_iter,
                                               int *__restrict__ g_conv, double *__restrict__ g_ires,
-                                              double *__restrict__ g_tres)
+                                              double *__restrict__ g_tres, const int *__restrict__ gZ)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
     const int n = P.n, nnz = P.nnz, S = fp.S;
-    // shared: SpMV slot metadata [SM][64] (padded); solve chunk records are read
-    // from global memory (cache-resident tables shared by every wave)
+    // shared: SpMV slot metadata [SM][64] (padded), FPC_ISAI: then the
+    // records of the two triangle layouts [SL + SU][64]; LU solve chunk records
+    // are read from global memory (cache-resident tables shared by every wave)
     uint32_t *meta = reinterpret_cast<uint32_t *>(smem);
+    uint32_t *trec = meta + SM * WAVE;
     const uint32_t *trL = fp.trL;
     const uint32_t *trU = fp.trU;
     for (int i = threadIdx.x; i < SM * WAVE; i += blockDim.x) meta[i] = (i < S * WAVE) ? fp.slot_meta[i] : 0xFFFF0000u;
-    // per wave: F[nnz + 2] (F[nnz] = 0), (ParILU: W[nnz + 2]), e0[n + 2], e1[n + 2],
-    // acc[64 NR] (e[n] = 0 is the zero cell of padding records, e1[n + 1] a scratch cell)
+    if constexpr (PC == FPC_ISAI) {
+        const int nl = fp.SL * WAVE, nu = fp.SU * WAVE;
+        for (int i = threadIdx.x; i < nl; i += blockDim.x) trec[i] = fp.isl_meta[i];
+        for (int i = threadIdx.x; i < nu; i += blockDim.x) trec[nl + i] = fp.isu_meta[i];
+    }
+    // per wave: f_cells FP64 cells -- FPC_LU: F[nnz + 2] (F[nnz] = 0), (ParILU:
+    // W[nnz + 2]); FPC_ISAI / FPC_GEN: the coefficients slot-major -- then
+    // e0[n + 2], e1[n + 2], (FPC_LU: acc[64 NR]) (e[n] = 0 is the zero cell of
+    // padding records, e[n + 1] a scratch cell)
     double *F = reinterpret_cast<double *>(smem + fp.shared_bytes + (size_t)wave * fp.wave_bytes);
     double *Wb = F + (nnz + 2);                       // ParILU only
-    double *e0 = Wb + (fp.par_w ? nnz + 2 : 0), *e1 = e0 + (n + 2);
+    double *e0 = F + fp.f_cells, *e1 = e0 + (n + 2);
     double *scratch = e1 + (n + 1);
     double *accb = e1 + (n + 2);
-    if (lane == 0) { F[nnz] = 0.0; e0[n] = 0.0; e1[n] = 0.0; }
+    if (lane == 0) {
+        if constexpr (PC == FPC_LU) F[nnz] = 0.0;
+        e0[n] = 0.0;
+        e1[n] = 0.0;
+    }
     __syncthreads();
     // level of the lane's rows in the lower (low half) and upper (high half) solve
     int rlev[NR];
@@ -477,7 +548,39 @@ __global__ __launch_bounds__(KF_MAX_WAVES * WAVE) void k_fast(DevPattern P, Fast
             r.e[k] = (i < n) ? bp[i] : 0.0;
         }
         int bad = 0;
-        if (sp.factor == FK_LOAD) {
+        if constexpr (PC != FPC_LU) {
+            // A's slot values, and the ISAI values gathered slot-major into the
+            // wave's region (ISAI(L)'s unit diagonal = 1, padding = 0)
+            const double *Mp = gF + (size_t)sys * nnz;
+#pragma unroll
+            for (int q = 0; q < SM; q++) {
+                a[q] = 0.0;
+                const int e = (q < S) ? fp.slot_pos[q * WAVE + lane] : -1;
+                if (e >= 0) a[q] = Ap[e];
+            }
+            auto gather = [&](const int32_t *__restrict__ pos, int cnt, double *dst) {
+                constexpr int U = 8;
+                for (int s0 = 0; s0 < cnt; s0 += U) {
+                    int ps[U];
+                    double v[U];
+#pragma unroll
+                    for (int j = 0; j < U; j++) ps[j] = s0 + j < cnt ? pos[(s0 + j) * WAVE + lane] : -1;
+#pragma unroll
+                    for (int j = 0; j < U; j++) v[j] = ps[j] >= 0 ? Mp[ps[j]] : (ps[j] == -2 ? 1.0 : 0.0);
+#pragma unroll
+                    for (int j = 0; j < U; j++)
+                        if (s0 + j < cnt) dst[(s0 + j) * WAVE + lane] = v[j];
+                }
+            };
+            if constexpr (PC == FPC_ISAI) {
+                gather(fp.isl_pos, fp.SL, F);
+                gather(fp.isu_pos, fp.SU, F + fp.SL * WAVE);
+            } else {
+                gather(fp.slot_pos, S, F);
+            }
+            bad = gZ[sys] != 0;
+            wave_sync();
+        } else if (sp.factor == FK_LOAD) {
             const double *Fp = gF + (size_t)sys * nnz;
             wave_copy(F, Fp, nnz, lane);
 #pragma unroll
@@ -502,7 +605,7 @@ __global__ __launch_bounds__(KF_MAX_WAVES * WAVE) void k_fast(DevPattern P, Fast
                 for (int sw = 0; sw < sp.sweeps; sw++) parilu_sweep(fp.etab, fp.ne, fp.dep, Ap, F, Wb, nnz, lane);
             }
         }
-        if (sp.factor != FK_ILU) {
+        if (PC == FPC_LU && sp.factor != FK_ILU) {
             // a zero on U's diagonal is a zero pivot, whatever produced the factors
             // (exact ILU(0) checks its pivots as it divides)
             wave_sync();
@@ -527,14 +630,35 @@ __global__ __launch_bounds__(KF_MAX_WAVES * WAVE) void k_fast(DevPattern P, Fast
 #pragma unroll
         for (int k = 0; k < NR; k++) {
             const int i = lane + WAVE * k;
-            rinv.e[k] = (i < n) ? 1.0 / F[fp.diag[i]] : 1.0;
+            rinv.e[k] = (PC == FPC_LU && i < n) ? 1.0 / F[fp.diag[i]] : 1.0;
         }
-        // out = U^{-1} L^{-1} in, in registers; also left in e0 for the following SpMV
+        // out = M in (LU: U^{-1} L^{-1} in), in registers; also left in e0 for
+        // the following SpMV
         auto apply = [&](const Vec<NR> &in, Vec<NR> &out) {
-            Vec<NR> y;
-            trsv_avail<NR, true>(trL, vcntL, fp.nlev_l, rlev, F, in, rinv, e1, accb, scratch, y, lane);
-            trsv_avail<NR, false>(trU, vcntU, fp.nlev_u, rlev, F, y, rinv, e0, accb, scratch, out, lane);
-            wave_sync();
+            if constexpr (PC == FPC_LU) {
+                Vec<NR> y;
+                trsv_avail<NR, true>(trL, vcntL, fp.nlev_l, rlev, F, in, rinv, e1, accb, scratch, y, lane);
+                trsv_avail<NR, false>(trU, vcntU, fp.nlev_u, rlev, F, y, rinv, e0, accb, scratch, out, lane);
+                wave_sync();
+            } else if constexpr (PC == FPC_ISAI) {
+                // ilu_isai_solve.cu:25-45: t = ISAI(L) in, out = ISAI(U) t
+                to_lds(in, e0, lane, n);
+                wave_sync();
+                spmv_coef<true>(F, fp.SL, trec, e0, e1, nullptr, lane);
+                wave_sync();
+                spmv_coef<true>(F + fp.SL * WAVE, fp.SU, trec + fp.SL * WAVE, e1, e0, nullptr, lane);
+                wave_sync();
+                from_lds(out, e0, lane, n);
+                wave_sync();
+            } else {
+                // gisai_solve.cu:23-30: out = G in
+                to_lds(in, e1, lane, n);
+                wave_sync();
+                spmv_coef<false>(F, S, meta, e1, e0, e1 + (n + 1), lane);
+                wave_sync();
+                from_lds(out, e0, lane, n);
+                wave_sync();
+            }
         };
         auto matvec = [&](Vec<NR> &out) {  // out = A e0
             spmv_slots<SM>(a, meta, e0, e1, scratch, lane);

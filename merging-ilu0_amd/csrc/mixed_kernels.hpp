@@ -15,21 +15,22 @@
 
 constexpr int FS_ILU = 0, FS_PARILU = 1;
 
-// ISAI of the L (or U) factor in F (the in-place factor layout rounded to
-// FP32; the records hold 8-byte offsets, halved here) into M (FP32 on the
-// pattern: strictly lower positions = ISAI(L), diagonal and upper =
-// ISAI(U)), by the lane streams of the pattern plan: each step is
-// acc = fma(-m_c, f_q, acc) in FP64; a finishing step writes the target
-// (divided by its diagonal for U) rounded to FP32 and restarts acc.  A
+// ISAI of the L (or U) factor in F (the in-place factor layout, T = float:
+// rounded to FP32, the records' 8-byte offsets halved here; T = double:
+// FP64) into M (type T on the pattern: strictly lower positions = ISAI(L),
+// diagonal and upper = ISAI(U)), by the lane streams of the pattern plan:
+// each step is acc = fma(-m_c, f_q, acc) in FP64; a finishing step writes
+// the target (divided by its diagonal for U) rounded to T and restarts acc.  A
 // target's m_c were written by earlier steps of the same lane, and a wave's
 // LDS accesses execute in program order, so no synchronisation is needed
 // inside the stream.  Returns nonzero when a divisor (a U diagonal) is 0.
 // DIV = false for the ISAI(L) streams, whose records never divide (unit
 // diagonal): the division and the divisor's load are left out.
-template <bool DIV>
+template <bool DIV, typename T = float>
 __device__ __forceinline__ int isai_gen(const uint32_t *__restrict__ r0p, const uint32_t *__restrict__ r1p,
-                                        int steps, const float *F, float *M, float *mscratch, int lane)
+                                        int steps, const T *F, T *M, T *mscratch, int lane)
 {
+    constexpr int SH = sizeof(T) == 4 ? 1 : 0;
     constexpr int K = bilu::ISAI_BATCH;
     const char *Fb = reinterpret_cast<const char *>(F);
     char *Mb = reinterpret_cast<char *>(M);
@@ -49,20 +50,20 @@ __device__ __forceinline__ int isai_gen(const uint32_t *__restrict__ r0p, const 
 #pragma unroll
         for (int j = 0; j < K; j++) {
             const uint32_t a = ra[j], b = rb[j];
-            const double mv = (double)*reinterpret_cast<const float *>(Mb + ((a >> 16) >> 1));
-            const double fv = (double)*reinterpret_cast<const float *>(Fb + ((a & 0xFFFFu) >> 1));
+            const double mv = (double)*reinterpret_cast<const T *>(Mb + ((a >> 16) >> SH));
+            const double fv = (double)*reinterpret_cast<const T *>(Fb + ((a & 0xFFFFu) >> SH));
             acc = fma(-mv, fv, acc);
             const uint32_t p = b & 0xFFFFu, d = b >> 16;
             const bool fin = p != 0xFFFFu;
             double val = acc;
             if constexpr (DIV) {
                 const bool div = d != 0xFFFFu;
-                const double dv = (double)*reinterpret_cast<const float *>(Fb + ((div ? d : 0u) >> 1));
+                const double dv = (double)*reinterpret_cast<const T *>(Fb + ((div ? d : 0u) >> SH));
                 bad |= (fin && div && dv == 0.0);
                 val = div ? acc / dv : acc;
             }
-            float *dst = fin ? reinterpret_cast<float *>(Mb + (p >> 1)) : mscratch;
-            *dst = (float)val;
+            T *dst = fin ? reinterpret_cast<T *>(Mb + (p >> SH)) : mscratch;
+            *dst = (T)val;
             acc = fin ? 0.0 : acc;
         }
 #pragma unroll
@@ -75,7 +76,7 @@ template <typename TO>
 __global__ __launch_bounds__(KF_MAX_WAVES * WAVE) void k_factor_fast(
     FastParams fp, int n, int nnz, int batch, const double *__restrict__ gA,
     const double *__restrict__ gFinit, TO *__restrict__ gF, int mode, int sweeps, double stop_tol,
-    int *__restrict__ g_status, int isai, float *__restrict__ gM, int *__restrict__ gZ,
+    int *__restrict__ g_status, int isai, void *__restrict__ gM, int *__restrict__ gZ,
     const int *__restrict__ sysidx, const _Float16 *__restrict__ gY, int P3, int inplace)
 {
     // job j factors system sysidx[j] (identity when sysidx is null); outputs are per job
@@ -83,9 +84,12 @@ __global__ __launch_bounds__(KF_MAX_WAVES * WAVE) void k_factor_fast(
     const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
     // per wave: F[nnz + 4] FP64 (F[nnz] = 0, F[nnz + 1] = 1, F[nnz + 2] scratch),
     // then the Jacobi ParILU buffer W[nnz + 2] FP64 (not for in-place sweeps).
-    // ISAI: the factors are rounded to FP32 in place when the factorization is
-    // done (the first half of F), the ISAI values M[nnz + 4] FP32 (M[nnz] = 0,
-    // M[nnz + 1] = 1, M[nnz + 2] = -1, M[nnz + 3] scratch) take the second half
+    // ISAI (TO = float): the factors are rounded to FP32 in place when the
+    // factorization is done (the first half of F), the ISAI values M[nnz + 4]
+    // FP32 (M[nnz] = 0, M[nnz + 1] = 1, M[nnz + 2] = -1, M[nnz + 3] scratch)
+    // take the second half.  ISAI (TO = double): the factors stay FP64 and
+    // the ISAI values, FP64 with the same cells, take the second region (the
+    // Jacobi buffer, free once the factorization is done).
     // ParILU with room for it: the sweep records (products, then finishes)
     // in a shared region ahead of the wave regions, read by LDS instructions
     // instead of global loads (fp.shared_bytes = 0 otherwise)
@@ -226,24 +230,35 @@ __global__ __launch_bounds__(KF_MAX_WAVES * WAVE) void k_factor_fast(
             ilu_stream(fp.ilu0, fp.ilu1, fp.ilu_steps, F, F + nnz + 2, fp.diag, n, lane);
         }
         if (isai) {
-            // round the factors (and the zero / one cells) to FP32 in place,
-            // 64 values at a time: a chunk's values are all read before its
-            // writes, which cover only the halves of chunks already read
-            wave_sync();
-            for (int e0 = 0; e0 < nnz + 4; e0 += WAVE) {
-                const int e = e0 + lane;
-                const double v = e < nnz + 4 ? F[e] : 0.0;
-                asm volatile("" ::: "memory");
-                if (e < nnz + 4) F32[e] = (float)v;
-                asm volatile("" ::: "memory");
+            int bad;
+            if constexpr (sizeof(TO) == 4) {
+                // round the factors (and the zero / one cells) to FP32 in place,
+                // 64 values at a time: a chunk's values are all read before its
+                // writes, which cover only the halves of chunks already read
+                wave_sync();
+                for (int e0 = 0; e0 < nnz + 4; e0 += WAVE) {
+                    const int e = e0 + lane;
+                    const double v = e < nnz + 4 ? F[e] : 0.0;
+                    asm volatile("" ::: "memory");
+                    if (e < nnz + 4) F32[e] = (float)v;
+                    asm volatile("" ::: "memory");
+                }
+                if (lane == 0) { M[nnz] = 0.0f; M[nnz + 1] = 1.0f; M[nnz + 2] = -1.0f; }
+                wave_sync();
+                bad = isai_gen<false>(fp.isL0, fp.isL1, fp.isL_steps, F32, M, M + nnz + 3, lane);
+                bad |= isai_gen<true>(fp.isU0, fp.isU1, fp.isU_steps, F32, M, M + nnz + 3, lane);
+                wave_sync();
+                wave_copy(static_cast<float *>(gM) + (size_t)sys * nnz, M, nnz, lane);
+            } else {
+                double *M64 = W;
+                wave_sync();
+                if (lane == 0) { M64[nnz] = 0.0; M64[nnz + 1] = 1.0; M64[nnz + 2] = -1.0; }
+                wave_sync();
+                bad = isai_gen<false, double>(fp.isL0, fp.isL1, fp.isL_steps, F, M64, M64 + nnz + 3, lane);
+                bad |= isai_gen<true, double>(fp.isU0, fp.isU1, fp.isU_steps, F, M64, M64 + nnz + 3, lane);
+                wave_sync();
+                wave_copy(static_cast<double *>(gM) + (size_t)sys * nnz, M64, nnz, lane);
             }
-            if (lane == 0) { M[nnz] = 0.0f; M[nnz + 1] = 1.0f; M[nnz + 2] = -1.0f; }
-            wave_sync();
-            int bad = isai_gen<false>(fp.isL0, fp.isL1, fp.isL_steps, F32, M, M + nnz + 3, lane);
-            bad |= isai_gen<true>(fp.isU0, fp.isU1, fp.isU_steps, F32, M, M + nnz + 3, lane);
-            wave_sync();
-            float *Mo = gM + (size_t)sys * nnz;
-            wave_copy(Mo, M, nnz, lane);
             bad = __any(bad);
             if (lane == 0) gZ[sys] = bad;
         } else {

@@ -1200,46 +1200,54 @@ void timing_mark(int kind, hipStream_t st, bool before)
     }
 }
 
-template <int NR, int SM>
+template <int NR, int SM, int PC>
 int launch_fast_t(bilu_pattern_s &h, const FastParams &fp, int grid, int threads, size_t lds, int batch,
                   const double *A, const double *b, double *x, const double *F, const SolveParams &sp,
-                  float *it, int *cv, double *ir, double *tr, hipStream_t st)
+                  float *it, int *cv, double *ir, double *tr, const int *Z, hipStream_t st)
 {
-    auto kern = k_fast<NR, SM>;
+    auto kern = k_fast<NR, SM, PC>;
     if (lds > 65536) HIP_TRY(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, st, fp.counter);
     timing_mark(BILU_TIMING_SOLVE, st, true);
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(threads), lds, st, h.dev, fp, batch, A, b, x, F, sp, it, cv, ir, tr);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(threads), lds, st, h.dev, fp, batch, A, b, x, F, sp, it, cv, ir, tr, Z);
     timing_mark(BILU_TIMING_SOLVE, st, false);
     HIP_TRY(hipGetLastError());
     return BILU_OK;
 }
 
-// returns -1 when the persistent kernel cannot take this pattern/option set
+// the persistent FP64 solve; pc = FPC_LU (F: factors, or null to factor in
+// the kernel), FPC_ISAI / FPC_GEN (F: ISAI values on the pattern, Z: their
+// zero-pivot flags).  Returns -1 when the kernel cannot take this
+// pattern/option set; dry: only that check, nothing launched.
 int launch_fast(bilu_pattern_s &h, int batch, const double *A, const double *b, double *x, const double *F,
-                const SolveParams &sp, float *it, int *cv, double *ir, double *tr, hipStream_t st)
+                const SolveParams &sp, float *it, int *cv, double *ir, double *tr, hipStream_t st,
+                int pc = FPC_LU, const int *Z = nullptr, bool dry = false)
 {
     const auto &pl = h.pa.fast;
     const int n = h.pa.n, nnz = h.pa.nnz;
     const int nr = (n + WAVE - 1) / WAVE;
-    if (!pl.ok || nr > MAX_NR) return -1;
+    if (!pl.ok || nr > 5) return -1;   // the instantiated row counts
     int sm;
     if (pl.S <= 24) sm = 24;
     else if (pl.S <= 36) sm = 36;
     else if (pl.S <= 48) sm = 48;
     else return -1;
-    const bool par = sp.factor == FK_PARILU;
+    if (pc == FPC_ISAI && (pl.SL <= 0 || pl.SU <= 0)) return -1;
+    const bool par = pc == FPC_LU && sp.factor == FK_PARILU;
     if (pl.nr != nr) return -1;
     FastParams fp{};
     fast_tables(h, fp);
-    // per wave: F (+2 cells), (ParILU: the previous sweep's factors), e0, e1 (+2 cells each),
-    // row accumulators of the solves (64 NR)
-    fp.wave_bytes = int(((size_t)((par ? 2 : 1) * (nnz + 2) + 2 * (n + 2) + 64 * nr) * sizeof(double) + 15) / 16 * 16);
+    // per wave: LU: F (+2 cells), (ParILU: the previous sweep's factors); ISAI: the
+    // coefficients slot-major; then e0, e1 (+2 cells each); LU: the row
+    // accumulators of the solves (64 NR)
+    fp.f_cells = pc == FPC_LU ? (par ? 2 : 1) * (nnz + 2) : pc == FPC_ISAI ? (pl.SL + pl.SU) * WAVE : pl.S * WAVE;
+    fp.wave_bytes = int(((size_t)fp.f_cells + 2 * (n + 2) + (pc == FPC_LU ? 64 * nr : 0)) * sizeof(double));
+    fp.wave_bytes = (fp.wave_bytes + 15) / 16 * 16;
     fp.par_w = par ? 1 : 0;
     fp.counter = h.d_counter;
     auto a16 = [](size_t x) { return (x + 15) / 16 * 16; };
     fp.o_trL = fp.o_trU = 0;
-    fp.shared_bytes = int(a16((size_t)sm * WAVE * 4));
+    fp.shared_bytes = int(a16((size_t)(sm + (pc == FPC_ISAI ? pl.SL + pl.SU : 0)) * WAVE * 4));
     if ((size_t)fp.shared_bytes >= h.max_lds) return -1;
     int waves = (int)std::min<size_t>(KF_MAX_WAVES, (h.max_lds - fp.shared_bytes) / fp.wave_bytes);
     if (waves < 1) return -1;
@@ -1249,8 +1257,18 @@ int launch_fast(bilu_pattern_s &h, int batch, const double *A, const double *b, 
     int grid = h.num_cus * per_cu;
     grid = std::max(1, std::min(grid, (batch + waves - 1) / waves));
     const int threads = waves * WAVE;
-#define FAST_CASE(R, M) \
-    if (nr == R && sm == M) return launch_fast_t<R, M>(h, fp, grid, threads, lds, batch, A, b, x, F, sp, it, cv, ir, tr, st);
+    if (dry) return BILU_OK;
+#define FAST_CASE(R, M)                                                                                       \
+    if (nr == R && sm == M) {                                                                                 \
+        if (pc == FPC_ISAI)                                                                                   \
+            return launch_fast_t<R, M, FPC_ISAI>(h, fp, grid, threads, lds, batch, A, b, x, F, sp, it, cv, ir, \
+                                                 tr, Z, st);                                                  \
+        if (pc == FPC_GEN)                                                                                    \
+            return launch_fast_t<R, M, FPC_GEN>(h, fp, grid, threads, lds, batch, A, b, x, F, sp, it, cv, ir,  \
+                                                tr, Z, st);                                                   \
+        return launch_fast_t<R, M, FPC_LU>(h, fp, grid, threads, lds, batch, A, b, x, F, sp, it, cv, ir, tr,   \
+                                           Z, st);                                                            \
+    }
 #define FAST_ROW(R) FAST_CASE(R, 24) FAST_CASE(R, 36) FAST_CASE(R, 48)
     FAST_ROW(1) FAST_ROW(2) FAST_ROW(3) FAST_ROW(4) FAST_ROW(5)
 #undef FAST_ROW
@@ -1310,17 +1328,17 @@ int workspace(bilu_pattern_s &h, size_t bytes, void **out, hipStream_t st)
 size_t workspace_need(const bilu_pattern_s &h, int batch)
 {
     const size_t nnz = (size_t)h.pa.nnz, b = (size_t)std::max(batch, 0);
-    return std::max(((b * nnz * sizeof(float) + 255) / 256) * 256 + b * sizeof(int), b * nnz * sizeof(double));
+    return ((b * nnz * sizeof(double) + 255) / 256) * 256 + b * sizeof(int);
 }
 
 // persistent factorization: exact ILU(0) or ParILU(0) sweeps (optionally
 // from F_init, optionally stopped on the factor residual) -> F (TO)
-// isai: write ISAI(L) / ISAI(U) of the factors (FP32) to M and zero-pivot
+// isai: write ISAI(L) / ISAI(U) of the factors (TO) to M and zero-pivot
 // flags to Z instead of the factors
 template <typename TO>
 int launch_factor_fast(bilu_pattern_s &h, int batch, const double *A, const double *F_init, TO *F,
                        int mode, int sweeps, double stop_tol, int *status, hipStream_t st,
-                       int isai = 0, float *M = nullptr, int *Z = nullptr, const int *sysidx = nullptr,
+                       int isai = 0, void *M = nullptr, int *Z = nullptr, const int *sysidx = nullptr,
                        int max_waves = KF_MAX_WAVES, const _Float16 *Y = nullptr, int P3 = 0, int inplace = 0)
 {
     const auto &pl = h.pa.fast;
@@ -1329,9 +1347,11 @@ int launch_factor_fast(bilu_pattern_s &h, int batch, const double *A, const doub
     FastParams fp{};
     fast_tables(h, fp);
     fp.counter = h.d_counter + 32;
-    // F (FP64; the ISAI values share it once the factors are rounded to FP32), then
-    // the Jacobi ParILU buffer (FP64)
-    const size_t second = mode == FS_PARILU && !inplace ? (size_t)(nnz + 2) * sizeof(double) : 0;
+    // F (FP64; FP32 ISAI values share it once the factors are rounded to FP32), then
+    // the Jacobi ParILU buffer (FP64), which FP64 ISAI values take afterwards
+    const bool isai64 = isai && sizeof(TO) == 8;
+    const size_t second = isai64 ? (size_t)(nnz + 4) * sizeof(double)
+                                 : mode == FS_PARILU && !inplace ? (size_t)(nnz + 2) * sizeof(double) : 0;
     // the fused warm start's FP32 diagonal at the end of the region
     fp.dg_floats = Y ? (h.pa.n + 3) / 4 * 4 : 0;
     fp.wave_bytes = int(((size_t)(nnz + 4) * sizeof(double) + second + (size_t)fp.dg_floats * 4 + 15) / 16 * 16);
@@ -1660,6 +1680,46 @@ int solve_ws(bilu_pattern_s &h, size_t bytes, void **out, hipStream_t st, const 
     return BILU_OK;
 }
 
+// FP64 ISAI solvers on the persistent kernel: a generation launch writes
+// the ISAI values and their zero-pivot flags into the workspace -- ILU(0) or
+// ParILU(0) factors then ISAI(L), ISAI(U) (k_factor_fast; FP64 throughout),
+// or the general ISAI of A (k_isai) -- then the persistent solve applies them
+// (ilu_isai_solve.cu, gisai_solve.cu).  -1, with nothing launched, when the
+// persistent solve cannot take the pattern.
+static int solve_isai_fast(bilu_pattern_s &h, int batch, const double *A, const double *b, double *x, int pc,
+                           int mode, int sweeps, double stop, int inplace, const double *F_init, SolveParams sp,
+                           float *it, int *cv, double *ir, double *tr, int *fstatus, hipStream_t st,
+                           const SolveWs &wsrc)
+{
+    const int fpc = pc == PC_ISAI_GEN ? FPC_GEN : FPC_ISAI;
+    if (launch_fast(h, batch, A, b, x, nullptr, sp, it, cv, ir, tr, st, fpc, nullptr, true) != BILU_OK) return -1;
+    const int nnz = h.pa.nnz;
+    if (pc == PC_ISAI_GEN && (h.pa.isai[BILU_ISAI_GENERAL].max_m > ISAI_GROUP ||
+                              2 * (size_t)nnz * sizeof(double) > h.max_lds))
+        return -1;
+    const size_t mbytes = ((size_t)batch * nnz * sizeof(double) + 255) / 256 * 256;
+    void *ws = nullptr;
+    int rc = solve_ws(h, mbytes + (size_t)batch * sizeof(int), &ws, st, wsrc);
+    if (rc != BILU_OK) return rc;
+    double *M = static_cast<double *>(ws);
+    int *Z = reinterpret_cast<int *>(static_cast<char *>(ws) + mbytes);
+    if (pc == PC_ISAI_GEN) {
+        const size_t lds = 2 * (size_t)nnz * sizeof(double);
+        if (lds > 65536) HIP_TRY(hipFuncSetAttribute((const void *)k_isai, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        timing_mark(BILU_TIMING_FACTOR, st, true);
+        hipLaunchKernelGGL(k_isai, dim3(batch), dim3(WAVE), lds, st, h.dev, batch, A, (int)BILU_ISAI_GENERAL, M, Z);
+        timing_mark(BILU_TIMING_FACTOR, st, false);
+        HIP_TRY(hipGetLastError());
+        if (fstatus) hipLaunchKernelGGL(k_fill_int, dim3((batch + 255) / 256), dim3(256), 0, st, fstatus, 0, batch);
+    } else {
+        rc = launch_factor_fast<double>(h, batch, A, F_init, M, mode, sweeps, stop, fstatus, st, 1, M, Z, nullptr,
+                                        KF_MAX_WAVES, nullptr, 0, inplace);
+        if (rc != BILU_OK) return rc;
+    }
+    sp.factor = FK_LOAD;
+    return launch_fast(h, batch, A, b, x, M, sp, it, cv, ir, tr, st, fpc, Z);
+}
+
 static int solve_fp64(bilu_pattern_t h, int batch, const double *d_A, const double *d_b, double *d_x,
                       const bilu_opts &o, float *d_iter, int *d_conv, double *d_iter_res,
                       double *d_true_res, void *stream, const SolveWs &wsrc = SolveWs())
@@ -1719,8 +1779,13 @@ static int solve_fp64(bilu_pattern_t h, int batch, const double *d_A, const doub
     if (pc == PC_LU && o.kernel != BILU_KERNEL_WAVE) {
         rc = launch_fast(*h, batch, d_A, d_b, d_x, F, sp, d_iter, d_conv, d_iter_res, d_true_res, st);
         if (rc == -1 && o.kernel == BILU_KERNEL_PERSISTENT) rc = BILU_ERR_TOO_LARGE;
+    } else if ((pc == PC_ISAI_LU || pc == PC_ISAI_GEN) && o.kernel != BILU_KERNEL_WAVE) {
+        const int mode = sp.factor == FK_PARILU ? FS_PARILU : FS_ILU;
+        rc = solve_isai_fast(*h, batch, d_A, d_b, d_x, pc, mode, sp.sweeps, 0.0, 0, nullptr, sp, d_iter, d_conv,
+                             d_iter_res, d_true_res, nullptr, st, wsrc);
+        if (rc == -1 && o.kernel == BILU_KERNEL_PERSISTENT) rc = BILU_ERR_TOO_LARGE;
     } else if (o.kernel == BILU_KERNEL_PERSISTENT) {
-        rc = BILU_ERR_INVALID_ARG;   // the persistent kernel serves the LU preconditioners only
+        rc = BILU_ERR_INVALID_ARG;   // no persistent kernel without a preconditioner
     }
     if (rc != -1) return rc;
     const bool need_w = (sp.factor == FK_PARILU) || pc == PC_ISAI_LU || pc == PC_ISAI_GEN;
@@ -1806,9 +1871,10 @@ static int solve_ex_impl(bilu_pattern_t h, int batch, const double *d_A, const d
         }
         return rc;
     }
-    // factor launch (persistent), factors through the handle's workspace, then the solve;
-    // these paths serve ILU(0) / ParILU(0), and their ISAI(L, U) in FP32_IR
-    if (!lu && !(isai && o.precision == BILU_PRECISION_FP32_IR)) return BILU_ERR_INVALID_ARG;
+    // factor launch (persistent), factors (or their ISAI(L, U)) through the
+    // handle's workspace, then the solve; these paths serve ILU(0) / ParILU(0)
+    // and their ISAI(L, U)
+    if (!lu && !isai) return BILU_ERR_INVALID_ARG;
     if (!h->pa.fast.ok) return BILU_ERR_TOO_LARGE;
     const int nnz = h->pa.nnz;
     const int mode = par ? FS_PARILU : FS_ILU;
@@ -1830,6 +1896,18 @@ static int solve_ex_impl(bilu_pattern_t h, int batch, const double *d_A, const d
         MixedParams mp{o.tol, o.tol_mode, (float)o.inner_tol, o.max_iter, o.max_refine, o.inner_tol, o.inner_adapt};
         return launch_mixed(*h, batch, d_A, d_b, d_x, F, Z, isai ? PCK_ISAI : PCK_LU, mp, d_iter, d_conv,
                             d_iter_res, d_true_res, d_refine, st);
+    }
+    if (isai) {
+        SolveParams sp{o.tol, o.tol_mode, o.max_iter, o.true_res_restart, o.max_restarts, FK_LOAD, 0};
+        rc = solve_isai_fast(*h, batch, d_A, d_b, d_x, PC_ISAI_LU, mode, o.parilu_sweeps, stop, inplace ? 1 : 0,
+                             mode == FS_PARILU ? d_F_init : nullptr, sp, d_iter, d_conv, d_iter_res, d_true_res,
+                             d_factor_status, st, wsrc);
+        if (rc == -1) return BILU_ERR_TOO_LARGE;
+        if (rc == BILU_OK && d_refine) {
+            hipLaunchKernelGGL(k_fill_int, dim3((batch + 255) / 256), dim3(256), 0, st, d_refine, 0, batch);
+            HIP_TRY(hipGetLastError());
+        }
+        return rc;
     }
     rc = solve_ws(*h, (size_t)batch * nnz * sizeof(double), &ws, st, wsrc);
     if (rc != BILU_OK) return rc;

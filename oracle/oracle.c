@@ -1148,9 +1148,10 @@ int orc_parilu0_stop(int n, const int *rp, const int *ci, const int *diag, const
 
 /* whole-batch driver of the split factor + solve paths: precond 0 = exact
  * ILU(0), 1 = ParILU(0) (with init / stop_tol as orc_parilu0_stop), 2 / 3 =
- * the same followed by ISAI(L), ISAI(U) (precision 1 only); precision 0 =
- * FP64 BiCGSTAB (availability-order solves), 1 = FP32 inner / FP64
- * refinement over the FP32-rounded factors or the FP32 ISAI. */
+ * the same followed by ISAI(L), ISAI(U); precision 0 = FP64 BiCGSTAB
+ * (availability-order solves; FP64 ISAI applies, a zero pivot of their
+ * generation failing the system as in the batch driver), 1 = FP32 inner /
+ * FP64 refinement over the FP32-rounded factors or the FP32 ISAI. */
 typedef struct {
     int n, nnz, b0, b1, precond, sweeps, precision, max_refine, inplace;
     double stop_tol, inner_tol;
@@ -1166,6 +1167,8 @@ static void *split_worker(void *arg)
     split_job *j = (split_job *)arg;
     const int n = j->n, nnz = j->nnz;
     double *F = (double *)malloc(sizeof(double) * (size_t)nnz);
+    double *P1 = (double *)malloc(sizeof(double) * (size_t)nnz);
+    double *P2 = (double *)malloc(sizeof(double) * (size_t)nnz);
     float *F32 = (float *)malloc(sizeof(float) * (size_t)nnz);
     for (int s = j->b0; s < j->b1; s++) {
         const double *A = j->A + (size_t)s * nnz;
@@ -1190,13 +1193,25 @@ static void *split_worker(void *arg)
             orc_bicgstab_mixed(n, j->rp, j->ci, j->diag, j->lev, j->lev + n, A, b, F32, j->o,
                                j->inner_tol, j->max_refine, x, &j->iter[s], &j->conv[s],
                                &j->iter_res[s], &j->true_res[s], &j->refine[s]);
+        } else if (j->precond >= 2) {
+            int zs = orc_isai(n, j->rp, j->ci, j->diag, F, ORC_ISAI_LOWER, P1);
+            if (zs == 0) zs = orc_isai(n, j->rp, j->ci, j->diag, F, ORC_ISAI_UPPER, P2);
+            if (zs != 0) {
+                for (int i = 0; i < n; i++) x[i] = 0.0;
+                j->iter[s] = 0.0f; j->conv[s] = zs; j->iter_res[s] = 0.0;
+                j->true_res[s] = sqrt(orc_dot(n, b, b, j->o->dot_width));
+            } else {
+                orc_bicgstab(n, j->rp, j->ci, j->diag, A, b, x, ORC_PC_ISAI_LU, P1, P2, j->o,
+                             &j->iter[s], &j->conv[s], &j->iter_res[s], &j->true_res[s]);
+            }
+            j->refine[s] = 0;
         } else {
             orc_bicgstab(n, j->rp, j->ci, j->diag, A, b, x, ORC_PC_LU, F, NULL, j->o,
                          &j->iter[s], &j->conv[s], &j->iter_res[s], &j->true_res[s]);
             j->refine[s] = 0;
         }
     }
-    free(F); free(F32);
+    free(F); free(P1); free(P2); free(F32);
     return NULL;
 }
 

@@ -243,3 +243,18 @@ def test_inplace_parilu_warm_bitwise():
     kj = dict(kw, parilu_order=bilu.PARILU_JACOBI)
     gj = run_gpu(pat, A, b, F_init=init, **kj)
     assert g[6].mean() < gj[6].mean(), (g[6].mean(), gj[6].mean())
+
+
+@pytest.mark.parametrize("inplace", [False, True])
+def test_fp64_isai_split_bitwise(inplace):
+    """FP64 ISAI(L, U) after residual-stopped ParILU(0) (Jacobi or in place):
+    the persistent factor + ISAI launch, then the persistent FP64 solve."""
+    print(f"case fp64 isai split inplace={inplace}", flush=True)
+    pat, A, b = synth.batch("random", 128, n=144)
+    kw = dict(precond=bilu.PARILU0_ISAI, parilu_sweeps=40, parilu_stop_tol=1e-8, tol=1e-10,
+              tol_mode=bilu.TOL_RELATIVE,
+              parilu_order=bilu.PARILU_INPLACE if inplace else bilu.PARILU_JACOBI)
+    g = run_gpu(pat, A, b, **kw)
+    o = run_oracle(pat, A, b, **kw)
+    assert_same(g, o, f"fp64 isai split inplace={inplace}")
+    assert (o[2] == 1).all() and (o[6] > 0).all()

@@ -107,6 +107,24 @@ def test_precond_variants_bitwise(precond):
     assert_same(g, o, f"precond {precond}")
 
 
+@pytest.mark.parametrize("precond", [bilu.ILU0_ISAI, bilu.PARILU0_ISAI, bilu.GENERAL_ISAI])
+@pytest.mark.parametrize("kind,B,n", [("stencil5", 64, 100), ("random", 128, 144), ("random", 64, 32),
+                                      ("random", 64, 90)])
+def test_persistent_isai_bitwise(precond, kind, B, n):
+    """FP64 ISAI solvers on the persistent kernel (generation launch, then the
+    solve applying ISAI(L), ISAI(U) or the general ISAI as SpMVs): bitwise the
+    oracle and the one-wave-per-system kernel."""
+    print(f"case persistent isai precond={precond} {kind} B={B} n={n}", flush=True)
+    pat, A, b = synth.batch(kind, B, n=n)
+    kw = dict(precond=precond, parilu_sweeps=5, tol=1e-10, tol_mode=bilu.TOL_RELATIVE)
+    g = run_gpu(pat, A, b, kernel=bilu.KERNEL_PERSISTENT, **kw)
+    o = run_oracle(pat, A, b, **kw)
+    assert (o[2] == 1).all()
+    assert_same(g, o, f"persistent isai {precond} {kind} n={n}")
+    w = run_gpu(pat, A, b, kernel=bilu.KERNEL_WAVE, **kw)
+    assert_same(g, w, f"persistent vs wave isai {precond} {kind} n={n}")
+
+
 def test_relative_tol_with_restart_meets_1e10():
     pat, A, b = synth.batch("random", 512, n=144)
     kw = dict(tol=1e-10, tol_mode=bilu.TOL_RELATIVE, true_res_restart=1)

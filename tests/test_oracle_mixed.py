@@ -181,6 +181,33 @@ def test_mixed_split_fp64_equals_merged_oracle():
     assert (g[5] == 0).all() and (g[6] == 0).all()
 
 
+@pytest.mark.parametrize("pc", [O.ILU0_ISAI, O.PARILU0_ISAI])
+def test_split_fp64_isai_equals_merged_oracle(pc):
+    """The split driver's FP64 ISAI path (the persistent ISAI solve's
+    oracle) is the batch driver's ISAI solver for the same factors, and its
+    converged solutions agree with dense FP64 solves."""
+    pat, A, b = synth.batch("random", 16, n=144)
+    o = O.opts(tol=1e-10, tol_mode=O.TOL_RELATIVE)
+    g = O.batch_solve_split(pat, A, b, precond=pc, sweeps=5, precision=O.PREC_FP64, o=o, threads=4)
+    m = O.batch_solve(pat, A, b, pc, sweeps=5, o=o, threads=4)
+    for u, v in zip(g[:5], m):
+        assert np.array_equal(u, v)
+    assert (g[2] == 1).all()
+    for s in range(4):
+        xd = np.linalg.solve(pat.dense(A[s]), b[s])
+        assert np.linalg.norm(g[0][s] - xd) <= 1e-6 * np.linalg.norm(xd)
+
+
+def test_split_fp64_isai_zero_pivot():
+    pat, A, b = synth.batch("random", 2, n=64)
+    A = A.copy()
+    A[1][pat.diag[0]] = 0.0
+    o = O.opts(tol=1e-10, tol_mode=O.TOL_RELATIVE)
+    x, it, cv, ir, tr, rf, fs = O.batch_solve_split(pat, A, b, precond=O.ILU0_ISAI, precision=O.PREC_FP64, o=o)
+    assert cv[0] == 1 and cv[1] == O.ZERO_PIVOT
+    assert (x[1] == 0).all() and it[1] == 0 and tr[1] == np.sqrt(O.dot(b[1], b[1]))
+
+
 def test_mixed_special_cases():
     pat, A, b = synth.batch("random", 3, n=64)
     b = b.copy()
