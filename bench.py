@@ -309,10 +309,10 @@ def main():
                 "parilu_sweeps_mean": float(fs[fs >= 0].mean()) if (fs >= 0).any() else None,
                 "parilu_fallbacks": int((fs < 0).sum())}
 
-    def run(o, F_init=None, pre=None, kernel=None):
+    def run(o, F_init=None, pre=None, kernel=None, status=True):
         # the factor status is requested only from the ParILU paths (asking for
         # it routes a plain solve through the split factor + solve launches)
-        par = o.precond in (bilu.PARILU0, bilu.PARILU0_ISAI)
+        par = status and o.precond in (bilu.PARILU0, bilu.PARILU0_ISAI)
         out["fs"].zero_()
         out["rf"].zero_()
 
@@ -416,6 +416,19 @@ def main():
     ms_mixed = run(o_mixed)
     mixed = summary(ms_mixed)
     x_mixed = x.clone()
+    # (2b) the other FP64 solvers of the plain method on the persistent kernels:
+    # ParILU(0) 5 sweeps merged into the solve; ILU(0) / ParILU(0) + ISAI(L, U)
+    # and the general ISAI of A (generation launch, then the persistent solve)
+    c1 = {}
+    for name, pc in (("parilu0_5_sweeps_merged", bilu.PARILU0), ("ilu0_isai", bilu.ILU0_ISAI),
+                     ("parilu0_5_sweeps_isai", bilu.PARILU0_ISAI), ("general_isai", bilu.GENERAL_ISAI)):
+        o_v = bilu.opts(precond=pc, parilu_sweeps=5, true_res_restart=1, kernel=bilu.KERNEL_PERSISTENT, **common)
+        ms_v, kms_v = run(o_v, kernel=bilu.TIMING_SOLVE, status=False)
+        c1[name] = dict(summary(ms_v), solve_kernel_ms=round(kms_v, 4))
+    for name, pc in (("ilu0_segregated", bilu.ILU0), ("parilu0_5_sweeps_segregated", bilu.PARILU0)):
+        o_seg = bilu.opts(precond=pc, parilu_sweeps=5, true_res_restart=1, merge=bilu.SEGREGATED, **common)
+        ms_v, kms_v = run(o_seg, kernel=bilu.TIMING_SOLVE, status=False)
+        c1[name] = dict(summary(ms_v), solve_kernel_ms=round(kms_v, 4))
     # (3) learned + mixed: train the warm start (untimed), then time inference + solve
     o_learn = bilu.opts(precond=bilu.PARILU0_ISAI, parilu_sweeps=40, parilu_stop_tol=1e-8,
                         parilu_order=bilu.PARILU_INPLACE, precision=bilu.PRECISION_FP32_IR,
@@ -526,6 +539,7 @@ def main():
                      "learned_mixed_parilu0_isai": learned, "learned_mixed_parilu0_isai_split": split,
                      "mixed_parilu0_isai_no_warm_start": nolearn,
                      "untrained_network_base_guess_mixed_parilu0_isai": untrained},
+        "fp64_solver_variants": c1,
         "fused_x_equals_split_x": fused_equal,
         "mlp_inference_ms": round(ms_infer, 4),
         "kernel_ms": {"k_mixed": round(kms_learn, 4), "k_factor_fast": round(kms_factor, 4),

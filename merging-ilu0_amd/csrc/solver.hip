@@ -530,6 +530,9 @@ __device__ int isai_tri(const DevIsai &M, int n, int type, const double *vals, d
     return __any(bad);
 }
 
+// General ISAI rows by Gauss-Jordan, two rows per wave (32 lanes each, lane t
+// holding row t of the m x m block); steps and columns stop at the larger m of
+// the two rows (beyond a row's m nothing changes its solution)
 __device__ int isai_general(const DevIsai &M, int n, const double *vals, double *out, int lane)
 {
     const int grp = lane / ISAI_GROUP, t = lane % ISAI_GROUP;
@@ -538,6 +541,8 @@ __device__ int isai_general(const DevIsai &M, int n, const double *vals, double 
         const int i = i0 + grp;
         const bool act = i < n;
         const int m = act ? M.m[i] : 0;
+        const int mo = __shfl_xor(m, ISAI_GROUP, WAVE);
+        const int mmax = m > mo ? m : mo;
         const int *g = act ? M.gather + M.goff[i] : M.gather;
         double row[ISAI_GROUP];
 #pragma unroll
@@ -546,6 +551,7 @@ __device__ int isai_general(const DevIsai &M, int n, const double *vals, double 
         double rhs = (act && t == M.rhs[i]) ? 1.0 : 0.0;
 #pragma unroll
         for (int dp = 0; dp < ISAI_GROUP; dp++) {
+            if (dp >= mmax) break;
             const bool on = dp < m;  // uniform within the group
             // pivot: reference 32-lane shfl_down tree (choose_pivot_row_1)
             const double dval = __shfl(row[dp], dp, ISAI_GROUP);
@@ -562,6 +568,7 @@ __device__ int isai_general(const DevIsai &M, int n, const double *vals, double 
             if (piv != dp) {
 #pragma unroll
                 for (int c = 0; c < ISAI_GROUP; c++) {
+                    if (c >= mmax) break;
                     const double a = __shfl(row[c], dp, ISAI_GROUP);
                     const double b = __shfl(row[c], piv, ISAI_GROUP);
                     if (t == dp) row[c] = b;
@@ -578,6 +585,7 @@ __device__ int isai_general(const DevIsai &M, int n, const double *vals, double 
                 const double mult = row[dp] / de;
 #pragma unroll
                 for (int c = 0; c < ISAI_GROUP; c++) {
+                    if (c >= mmax) break;
                     const double key = __shfl(row[c], dp, ISAI_GROUP);
                     if (t != dp && c < m) row[c] = fma(-mult, key, row[c]);
                 }
@@ -833,13 +841,18 @@ __global__ __launch_bounds__(WAVE) void k_isai(DevPattern P, int batch, const do
     const double *Vp = gV + (size_t)sys * nnz;
     double *Op = gOut + (size_t)sys * nnz;
     wave_copy(sV, Vp, nnz, lane);
-    wave_copy(sO, Op, nnz, lane);
-    __syncthreads();
     int bad;
-    if (type == BILU_ISAI_GENERAL) bad = isai_general(P.isai[type], P.n, sV, sO, lane);
-    else bad = isai_tri(P.isai[type], P.n, type, sV, sO, nullptr, lane);
-    __syncthreads();
-    for (int e = lane; e < nnz; e += WAVE) Op[e] = sO[e];
+    if (type == BILU_ISAI_GENERAL) {
+        // every position is written once: straight to global memory (LDS holds the values only)
+        __syncthreads();
+        bad = isai_general(P.isai[type], P.n, sV, Op, lane);
+    } else {
+        wave_copy(sO, Op, nnz, lane);
+        __syncthreads();
+        bad = isai_tri(P.isai[type], P.n, type, sV, sO, nullptr, lane);
+        __syncthreads();
+        for (int e = lane; e < nnz; e += WAVE) Op[e] = sO[e];
+    }
     if (g_status && lane == 0) g_status[sys] = bad ? BILU_ZERO_PIVOT : 0;
 }
 
@@ -1695,7 +1708,7 @@ static int solve_isai_fast(bilu_pattern_s &h, int batch, const double *A, const 
     if (launch_fast(h, batch, A, b, x, nullptr, sp, it, cv, ir, tr, st, fpc, nullptr, true) != BILU_OK) return -1;
     const int nnz = h.pa.nnz;
     if (pc == PC_ISAI_GEN && (h.pa.isai[BILU_ISAI_GENERAL].max_m > ISAI_GROUP ||
-                              2 * (size_t)nnz * sizeof(double) > h.max_lds))
+                              (size_t)nnz * sizeof(double) > h.max_lds))
         return -1;
     const size_t mbytes = ((size_t)batch * nnz * sizeof(double) + 255) / 256 * 256;
     void *ws = nullptr;
@@ -1704,7 +1717,7 @@ static int solve_isai_fast(bilu_pattern_s &h, int batch, const double *A, const 
     double *M = static_cast<double *>(ws);
     int *Z = reinterpret_cast<int *>(static_cast<char *>(ws) + mbytes);
     if (pc == PC_ISAI_GEN) {
-        const size_t lds = 2 * (size_t)nnz * sizeof(double);
+        const size_t lds = (size_t)nnz * sizeof(double);
         if (lds > 65536) HIP_TRY(hipFuncSetAttribute((const void *)k_isai, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         timing_mark(BILU_TIMING_FACTOR, st, true);
         hipLaunchKernelGGL(k_isai, dim3(batch), dim3(WAVE), lds, st, h.dev, batch, A, (int)BILU_ISAI_GENERAL, M, Z);
@@ -1763,15 +1776,23 @@ static int solve_fp64(bilu_pattern_t h, int batch, const double *d_A, const doub
         // factor launch, factors through HBM (the handle's workspace), then the
         // solve launch (segregated.cu:429-490)
         const size_t flds = (size_t)(sp.factor == FK_PARILU ? 3 : 2) * nnz * sizeof(double);
-        if (flds > h->max_lds) return BILU_ERR_TOO_LARGE;
+        if (!pa.fast.ok && flds > h->max_lds) return BILU_ERR_TOO_LARGE;
         void *ws = nullptr;
         const int wrc = solve_ws(*h, (size_t)batch * nnz * sizeof(double), &ws, st, wsrc);
         if (wrc != BILU_OK) return wrc;
         double *Fbuf = static_cast<double *>(ws);
-        if (flds > 65536) HIP_TRY(hipFuncSetAttribute((const void *)k_factor, hipFuncAttributeMaxDynamicSharedMemorySize, (int)flds));
-        hipLaunchKernelGGL(k_factor, dim3(batch), dim3(WAVE), flds, st, h->dev, batch, d_A,
-                           (const double *)nullptr, Fbuf, sp.factor, sp.sweeps, (int *)nullptr);
-        HIP_TRY(hipGetLastError());
+        if (pa.fast.ok) {
+            // the persistent factorization (exact ILU(0) lane streams or Jacobi sweeps)
+            const int frc = launch_factor_fast<double>(*h, batch, d_A, nullptr, Fbuf,
+                                                       sp.factor == FK_PARILU ? FS_PARILU : FS_ILU, sp.sweeps, 0.0,
+                                                       nullptr, st);
+            if (frc != BILU_OK) return frc;
+        } else {
+            if (flds > 65536) HIP_TRY(hipFuncSetAttribute((const void *)k_factor, hipFuncAttributeMaxDynamicSharedMemorySize, (int)flds));
+            hipLaunchKernelGGL(k_factor, dim3(batch), dim3(WAVE), flds, st, h->dev, batch, d_A,
+                               (const double *)nullptr, Fbuf, sp.factor, sp.sweeps, (int *)nullptr);
+            HIP_TRY(hipGetLastError());
+        }
         F = Fbuf;
         sp.factor = FK_LOAD;
     }
@@ -2170,7 +2191,7 @@ int bilu_isai(bilu_pattern_t h, int batch, const double *d_vals, int type, doubl
     if (!h || batch < 0 || type < 0 || type > 2 || (batch > 0 && (!d_vals || !d_out))) return BILU_ERR_INVALID_ARG;
     if (batch == 0) return BILU_OK;
     if (h->pa.isai[type].max_m > ISAI_GROUP) return BILU_ERR_TOO_LARGE;
-    const size_t lds = 2 * (size_t)h->pa.nnz * sizeof(double);
+    const size_t lds = (type == BILU_ISAI_GENERAL ? 1 : 2) * (size_t)h->pa.nnz * sizeof(double);
     if (lds > h->max_lds) return BILU_ERR_TOO_LARGE;
     HIP_TRY(hipSetDevice(h->device));
     if (lds > 65536) HIP_TRY(hipFuncSetAttribute((const void *)k_isai, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
