@@ -70,6 +70,60 @@ __device__ __forceinline__ void wave_sync()
 
 __global__ void k_zero_counter(int *c) { *c = 0; }
 
+// A zero the compiler cannot see through.  Table addresses formed from it
+// inside the persistent loop are recomputed there: otherwise the loop-
+// invariant per-slot addresses of the slot tables are hoisted out of the
+// loop, outnumber the registers, and every load first waits for its
+// address to come back from scratch memory.
+__device__ __forceinline__ int opaque_zero()
+{
+    int z;
+    asm volatile("s_mov_b32 %0, 0" : "=s"(z));
+    return z;
+}
+
+// A system's values in a slot layout: for slots q < SMX, put(q, v) with
+// v = V[pos[q][lane]] for q < ns (position -1: 0, position -2: 1, the unit
+// diagonal of ISAI(L)) and v = 0 for ns <= q.  Every load is unconditional
+// -- slots past ns re-read record ns - 1, negative positions read V[0] --
+// and a batch of KB slots issues its position loads, then its value loads,
+// together: a load that sits inside a branch on its own value waits alone,
+// one memory round trip per slot.  pos holds at least one slot when ns > 0.
+template <int SMX, typename TV, typename PUT>
+__device__ __forceinline__ void slot_gather(const int32_t *__restrict__ pos, int ns, const TV *V, int lane,
+                                            PUT &&put)
+{
+    constexpr int KB = 12;
+    if (ns <= 0) {
+#pragma unroll
+        for (int q = 0; q < SMX; q++) put(q, TV(0));
+        return;
+    }
+    // the last record's index is opaque, so the (uniform) record addresses are
+    // formed per use
+    const int last = ns - 1 + opaque_zero();
+#pragma unroll
+    for (int q0 = 0; q0 < SMX; q0 += KB) {
+        int ps[KB];
+        TV v[KB];
+#pragma unroll
+        for (int j = 0; j < KB; j++)
+            if (q0 + j < SMX) {
+                const int32_t *rec = pos + min(q0 + j, last) * WAVE;
+                ps[j] = rec[lane];
+            }
+#pragma unroll
+        for (int j = 0; j < KB; j++)
+            if (q0 + j < SMX) v[j] = V[max(ps[j], 0)];
+#pragma unroll
+        for (int j = 0; j < KB; j++)
+            if (q0 + j < SMX) {
+                const bool in = q0 + j < ns;
+                put(q0 + j, (in && ps[j] >= 0) ? v[j] : (in && ps[j] == -2) ? TV(1) : TV(0));
+            }
+    }
+}
+
 // waves per workgroup of the persistent kernel (its launch bound; the
 // register budget per lane is 512 / (waves / 4) rounded to the allocation
 // granule)
@@ -552,21 +606,20 @@ __global__ __launch_bounds__(KF_MAX_WAVES * WAVE) void k_fast(DevPattern P, Fast
             // A's slot values, and the ISAI values gathered slot-major into the
             // wave's region (ISAI(L)'s unit diagonal = 1, padding = 0)
             const double *Mp = gF + (size_t)sys * nnz;
-#pragma unroll
-            for (int q = 0; q < SM; q++) {
-                a[q] = 0.0;
-                const int e = (q < S) ? fp.slot_pos[q * WAVE + lane] : -1;
-                if (e >= 0) a[q] = Ap[e];
-            }
+            slot_gather<SM>(fp.slot_pos, S, Ap, lane, [&](int q, double v) { a[q] = v; });
+            // runtime slot counts: batches of U slots, loads unconditional (clamped
+            // record and position) so that a batch's loads are in flight together
             auto gather = [&](const int32_t *__restrict__ pos, int cnt, double *dst) {
                 constexpr int U = 8;
                 for (int s0 = 0; s0 < cnt; s0 += U) {
                     int ps[U];
                     double v[U];
 #pragma unroll
-                    for (int j = 0; j < U; j++) ps[j] = s0 + j < cnt ? pos[(s0 + j) * WAVE + lane] : -1;
+                    for (int j = 0; j < U; j++) ps[j] = pos[min(s0 + j, cnt - 1) * WAVE + lane];
 #pragma unroll
-                    for (int j = 0; j < U; j++) v[j] = ps[j] >= 0 ? Mp[ps[j]] : (ps[j] == -2 ? 1.0 : 0.0);
+                    for (int j = 0; j < U; j++) v[j] = Mp[max(ps[j], 0)];
+#pragma unroll
+                    for (int j = 0; j < U; j++) v[j] = ps[j] >= 0 ? v[j] : (ps[j] == -2 ? 1.0 : 0.0);
 #pragma unroll
                     for (int j = 0; j < U; j++)
                         if (s0 + j < cnt) dst[(s0 + j) * WAVE + lane] = v[j];
@@ -583,22 +636,12 @@ __global__ __launch_bounds__(KF_MAX_WAVES * WAVE) void k_fast(DevPattern P, Fast
         } else if (sp.factor == FK_LOAD) {
             const double *Fp = gF + (size_t)sys * nnz;
             wave_copy(F, Fp, nnz, lane);
-#pragma unroll
-            for (int q = 0; q < SM; q++) {
-                a[q] = 0.0;
-                const int e = (q < S) ? fp.slot_pos[q * WAVE + lane] : -1;
-                if (e >= 0) a[q] = Ap[e];
-            }
+            slot_gather<SM>(fp.slot_pos, S, Ap, lane, [&](int q, double v) { a[q] = v; });
             wave_sync();
         } else {
             wave_copy(F, Ap, nnz, lane);
             wave_sync();
-#pragma unroll
-            for (int q = 0; q < SM; q++) {
-                a[q] = 0.0;
-                const int e = (q < S) ? fp.slot_pos[q * WAVE + lane] : -1;
-                if (e >= 0) a[q] = F[e];
-            }
+            slot_gather<SM>(fp.slot_pos, S, (const double *)F, lane, [&](int q, double v) { a[q] = v; });
             if (sp.factor == FK_ILU) {
                 bad = ilu_stream(fp.ilu0, fp.ilu1, fp.ilu_steps, F, F + nnz + 1, fp.diag, n, lane);
             } else if (sp.factor == FK_PARILU) {

@@ -137,37 +137,40 @@ __global__ __launch_bounds__(KF_MAX_WAVES * WAVE) void k_factor_fast(
                 for (int j = lane; j < n; j += WAVE) Dg[j] = (float)F[fp.diag[j]];
                 wave_sync();
                 const _Float16 *Yp = gY + (size_t)sys * P3;
+                // the loads are unconditional (positions past nnz re-read the last
+                // one), so all 2 WU of a lane are in flight together
                 for (int e0 = lane; e0 < nnz; e0 += WU * WAVE) {
-                    float y[WU];
+                    _Float16 yh[WU];
                     uint32_t cd[WU];
 #pragma unroll
                     for (int j = 0; j < WU; j++) {
-                        const int e = e0 + j * WAVE;
-                        y[j] = e < nnz ? (float)Yp[e] : 0.0f;
-                        cd[j] = e < nnz ? fp.wcode[e] : 0xFFFFu;
+                        const int e = min(e0 + j * WAVE, nnz - 1);
+                        yh[j] = Yp[e];
+                        cd[j] = fp.wcode[e];
                     }
 #pragma unroll
                     for (int j = 0; j < WU; j++) {
                         const int e = e0 + j * WAVE;
-                        if (cd[j] != 0xFFFFu) {
+                        if (e < nnz && cd[j] != 0xFFFFu) {
                             const bool lower = cd[j] != 0xFFFEu;
-                            F[e] = bilu_mlp::guess_from((float)F[e], lower ? Dg[cd[j]] : 1.0f, lower, y[j]);
+                            F[e] = bilu_mlp::guess_from((float)F[e], lower ? Dg[cd[j]] : 1.0f, lower, (float)yh[j]);
                         }
                     }
                 }
             } else if (gFinit) {
                 // entries with work start from the guess; the others are exact (= a)
                 const double *Ip = gFinit + (size_t)(sysidx ? sysidx[sys] : sys) * nnz;
+                // unconditional loads (entries past ne re-read the last one)
                 for (int i0 = lane; i0 < fp.ne; i0 += 8 * WAVE) {
                     int pos[8];
                     double g[8];
 #pragma unroll
-                    for (int j = 0; j < 8; j++) pos[j] = (i0 + j * WAVE < fp.ne) ? fp.etab[i0 + j * WAVE].x : -1;
+                    for (int j = 0; j < 8; j++) pos[j] = fp.etab[min(i0 + j * WAVE, fp.ne - 1)].x;
 #pragma unroll
-                    for (int j = 0; j < 8; j++) g[j] = pos[j] >= 0 ? Ip[pos[j]] : 0.0;
+                    for (int j = 0; j < 8; j++) g[j] = Ip[pos[j]];
 #pragma unroll
                     for (int j = 0; j < 8; j++)
-                        if (pos[j] >= 0) F[pos[j]] = g[j];
+                        if (i0 + j * WAVE < fp.ne) F[pos[j]] = g[j];
                 }
             }
             wave_sync();
@@ -491,21 +494,14 @@ __global__ __launch_bounds__(km_waves(SM, PCK) * WAVE) void k_mixed(
         const float *Fp = gF + (size_t)sys * nnz;
         if (PCK == PCK_LU)
             wave_copy(F, Fp, nnz, lane);
-#pragma unroll
-        for (int q = 0; q < SM; q++) {
-            const int e = (q < S) ? fp.slot_pos[q * WAVE + lane] : -1;
-            const float av = (e >= 0) ? (float)Ap[e] : 0.0f;
-            if constexpr (PCK == PCK_LU) a[q] = av;
-            else Asl[q * WAVE + lane] = av;
-        }
+        slot_gather<SM>(fp.slot_pos, S, Ap, lane, [&](int q, double v) {
+            if constexpr (PCK == PCK_LU) a[q] = (float)v;
+            else Asl[q * WAVE + lane] = (float)v;
+        });
         if constexpr (PCK == PCK_ISAI) {
-#pragma unroll
-            for (int q = 0; q < SLU; q++) {
-                const int el = (q < fp.SL) ? fp.isl_pos[q * WAVE + lane] : -1;
-                const int eu = (q < fp.SU) ? fp.isu_pos[q * WAVE + lane] : -1;
-                cL[q] = el >= 0 ? Fp[el] : (el == -2 ? 1.0f : 0.0f);
-                cU[q] = eu >= 0 ? Fp[eu] : 0.0f;
-            }
+            // ISAI(L)'s unit diagonal (position -2) reads as 1
+            slot_gather<SLU>(fp.isl_pos, fp.SL, Fp, lane, [&](int q, float v) { cL[q] = v; });
+            slot_gather<SLU>(fp.isu_pos, fp.SU, Fp, lane, [&](int q, float v) { cU[q] = v; });
         }
         wave_sync();
         // zero cells of the FP32 views (the FP64 residual overwrites them)
