@@ -326,15 +326,16 @@ __device__ __forceinline__ void residual_rows(const int *__restrict__ rp, const 
 
 // y (LDS, by row) = C x over A's slot layout with the coefficients read
 // from the wave's LDS, slot-major (cl[s * 64 + lane]) and one packed record
-// per slot: bits 0-15 the byte offset of x (FP32 cells); bits 16-30 the byte
+// per slot: bits 0-15 the byte offset of x (FP32 cells); bits 16-31 the byte
 // offset of the output cell (the row's cell when the slot ends a row, else
-// the output vector's scratch cell n + 1); bit 31 set when the slot does not
-// end its row.  One LDS word per slot instead of a separate x and output
-// record: the kernel is bound by instruction issue.  The same CSR-order fma
-// chain as spmv_slots.
+// the output vector's scratch cell n + 1, the largest offset in use).  A
+// record below thr = (4 (n + 1)) << 16 ends a row: the running sum restarts.
+// One LDS word per slot instead of a separate x and output record (the
+// kernel is bound by instruction issue), each half taken by one add.  The
+// same CSR-order fma chain as spmv_slots.
 template <int SM>
 __device__ __forceinline__ void spmv32(const float *__restrict__ cl, const uint32_t *__restrict__ meta,
-                                       const float *__restrict__ x, float *__restrict__ y, int lane)
+                                       const float *__restrict__ x, float *__restrict__ y, uint32_t thr, int lane)
 {
     const char *xb = reinterpret_cast<const char *>(x);
     char *yb = reinterpret_cast<char *>(y);
@@ -357,19 +358,20 @@ __device__ __forceinline__ void spmv32(const float *__restrict__ cl, const uint3
         }
         const uint32_t r = rw[s];
         acc = __builtin_fmaf(cw[s], xv[s], acc);
-        *reinterpret_cast<float *>(yb + ((r >> 16) & 0x7FFFu)) = acc;
-        acc = (r >> 31) ? acc : 0.0f;
+        *reinterpret_cast<float *>(yb + (r >> 16)) = acc;
+        acc = r < thr ? 0.0f : acc;
     }
 }
 
 // y (LDS, by row) = C x over one of the triangle slot layouts: the
 // coefficient of slot s is c[s] (registers), the record one packed word
-// (x offset | output offset << 16 | not-a-row-end << 31); only the first
-// `ns` slots are visited (a wave-uniform bound, checked per gather group).
+// (x offset | output offset << 16, as for spmv32); only the first `ns`
+// slots are visited (a wave-uniform bound, checked per gather group).
 // Each row sum is the CSR-order fma chain over the triangle's entries.
 template <int SMX>
 __device__ __forceinline__ void spmv_tri(const float (&c)[SMX], int ns, const uint32_t *__restrict__ rec,
-                                         const float *__restrict__ x, float *__restrict__ y, int lane)
+                                         const float *__restrict__ x, float *__restrict__ y, uint32_t thr,
+                                         int lane)
 {
     const char *xb = reinterpret_cast<const char *>(x);
     char *yb = reinterpret_cast<char *>(y);
@@ -392,8 +394,8 @@ __device__ __forceinline__ void spmv_tri(const float (&c)[SMX], int ns, const ui
         }
         const uint32_t r = rw[s];
         acc = __builtin_fmaf(c[s], xv[s], acc);
-        *reinterpret_cast<float *>(yb + ((r >> 16) & 0x7FFFu)) = acc;
-        acc = (r >> 31) ? acc : 0.0f;
+        *reinterpret_cast<float *>(yb + (r >> 16)) = acc;
+        acc = r < thr ? 0.0f : acc;
     }
 }
 
@@ -433,7 +435,8 @@ __global__ __launch_bounds__(km_waves(SM, PCK) * WAVE) void k_mixed(
     // LU: the records of spmv_slots.  ISAI: the packed records of spmv32
     // for the A matvec, from those (8 col | 8 row << 16, 0xFFFF: no row end);
     // padding slots read x[0] with coefficient 0.  ISAI: then the records of
-    // the ISAI(L) and ISAI(U) slot layouts, padded to SLU slots each.
+    // the ISAI(L) and ISAI(U) slot layouts (their not-a-row-end bit 31
+    // dropped: the output offset tells), padded to SLU slots each.
     uint32_t *lrec = meta + SM * WAVE, *urec = lrec + SLU * WAVE;
     for (int i = threadIdx.x; i < SM * WAVE; i += blockDim.x) {
         const uint32_t m = (i < S * WAVE) ? fp.slot_meta[i] : 0xFFFF0000u;
@@ -442,14 +445,16 @@ __global__ __launch_bounds__(km_waves(SM, PCK) * WAVE) void k_mixed(
         } else {
             const bool end = (m >> 16) != 0xFFFFu;
             const uint32_t yo = end ? (m >> 16) >> 1 : (uint32_t)(n + 1) * 4u;
-            meta[i] = ((m & 0xFFFFu) >> 1) | (yo << 16) | (end ? 0u : 1u << 31);
+            meta[i] = ((m & 0xFFFFu) >> 1) | (yo << 16);
         }
     }
+    // FP32 records: an output offset below that of the scratch cell n + 1 ends a row
+    const uint32_t thr = (uint32_t)(n + 1) * 4u << 16;
     if constexpr (PCK == PCK_ISAI) {
-        const uint32_t pad = (uint32_t)n * 4u | ((uint32_t)(n + 1) * 4u << 16) | (1u << 31);
+        const uint32_t pad = (uint32_t)n * 4u | thr;
         for (int i = threadIdx.x; i < SLU * WAVE; i += blockDim.x) {
-            lrec[i] = (i < fp.SL * WAVE) ? fp.isl_meta[i] : pad;
-            urec[i] = (i < fp.SU * WAVE) ? fp.isu_meta[i] : pad;
+            lrec[i] = (i < fp.SL * WAVE) ? fp.isl_meta[i] & 0x7FFFFFFFu : pad;
+            urec[i] = (i < fp.SU * WAVE) ? fp.isu_meta[i] & 0x7FFFFFFFu : pad;
         }
     }
     // per wave: F[nnz + 2] FP32 (F[nnz] = 0), then two vector regions of
@@ -549,16 +554,16 @@ __global__ __launch_bounds__(km_waves(SM, PCK) * WAVE) void k_mixed(
             } else {
                 to_lds(in, e0, lane, n);
                 wave_sync();
-                spmv_tri<SC>(cL, fp.SL, lrec, e0, e1, lane);
+                spmv_tri<SC>(cL, fp.SL, lrec, e0, e1, thr, lane);
                 wave_sync();
-                spmv_tri<SC>(cU, fp.SU, urec, e1, e0, lane);
+                spmv_tri<SC>(cU, fp.SU, urec, e1, e0, thr, lane);
                 wave_sync();
                 from_lds(out, e0, lane, n);
             }
         };
         auto matvec = [&](Vec<NR, float> &out) {   // out = A32 e0
             if constexpr (PCK == PCK_LU) spmv_slots<SM, float>(a, meta, e0, e1, scratch, lane);
-            else spmv32<SM>(Asl, meta, e0, e1, lane);
+            else spmv32<SM>(Asl, meta, e0, e1, thr, lane);
             wave_sync();
             from_lds(out, e1, lane, n);
             wave_sync();
