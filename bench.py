@@ -46,6 +46,8 @@ def parse():
     ap.add_argument("--mode", default="headline", choices=["headline", "single", "uneven", "learned"],
                     help="learned: only the learned + mixed step (training as --train-*), for profiling")
     ap.add_argument("--uneven-each", type=int, default=16384)
+    ap.add_argument("--uneven-precond", default="ilu0_isai", choices=["ilu0", "ilu0_isai"],
+                    help="preconditioner of the uneven leg (ISAI(L, U) applies or triangular solves)")
     ap.add_argument("--precond", default="ilu0",
                     choices=["ilu0", "parilu0", "ilu0_isai", "parilu0_isai", "general_isai"])
     ap.add_argument("--merge", default="merged", choices=["merged", "segregated"])
@@ -193,10 +195,12 @@ def time_steps(step, warmup, steps, stream, kernel=None):
     return (ms, kt.avg_ms) if kt else ms
 
 
-def uneven_bench(args, dev):
+def uneven_bench(args, dev, precond=None):
     """Uneven batch: 4 patterns (n = 32, 90, 144, 300, nnz ~ 14 n), `uneven_each`
-    systems each, interleaved in one request list, FP32 inner / FP64
-    refinement; the plan is autotuned, then its hipGraph is replayed."""
+    systems each, interleaved in one request list, exact ILU(0) with its
+    ISAI(L, U) applies (or its triangular solves, precond="ilu0") in FP32,
+    FP32 inner / FP64 refinement; the plan is autotuned, then its hipGraph
+    is replayed."""
     import numpy as np
     import torch
 
@@ -215,7 +219,9 @@ def uneven_bench(args, dev):
     req_i = np.tile(np.arange(E), len(ns)).astype(np.int32)
     perm = np.random.default_rng(0).permutation(req_p.size)
     req_p, req_i = req_p[perm], req_i[perm]
-    plan = bilu.Uneven(pats, req_p, req_i, bilu.opts(precond=bilu.ILU0, tol=1e-10, tol_mode=bilu.TOL_RELATIVE,
+    precond = precond or args.uneven_precond
+    pc = {"ilu0": bilu.ILU0, "ilu0_isai": bilu.ILU0_ISAI}[precond]
+    plan = bilu.Uneven(pats, req_p, req_i, bilu.opts(precond=pc, tol=1e-10, tol_mode=bilu.TOL_RELATIVE,
                                                      precision=bilu.PRECISION_FP32_IR, inner_tol=args.inner_tol))
     out = plan.outputs()
     t0 = time.time()
@@ -236,8 +242,10 @@ def uneven_bench(args, dev):
             "value": round(B / (ms * 1e-3), 1), "unit": "systems/s", "n_gpus": 1, "steps": args.steps,
             "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak",
             "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
-            "config": {"workload": f"{len(ns)} patterns x {E} systems interleaved, exact ILU(0) FP32 + FP32 "
-                                   f"BiCGSTAB in FP64 refinement, one hipGraph", "n": list(ns), "nnz": nnzs},
+            "config": {"workload": f"{len(ns)} patterns x {E} systems interleaved, exact ILU(0) "
+                                   f"{'with ISAI(L, U) applies' if precond == 'ilu0_isai' else 'triangular solves'} "
+                                   f"in FP32 + FP32 BiCGSTAB in FP64 refinement, one hipGraph",
+                       "precond": precond, "n": list(ns), "nnz": nnzs},
             "all_converged_relres_le_1e-10": bool((cv == 1).all() and (relres <= 1e-10).all()),
             "tuned": {str(n): dict(zip(("factor_waves", "solve_waves", "jobs"), plan.config(p)))
                       for p, n in enumerate(ns)},
@@ -559,6 +567,11 @@ def main():
         "gen_seconds": round(gen_s, 2),
     })
     rec["uneven"] = None if args.no_uneven else uneven_bench(args, dev)
+    if not args.no_uneven:
+        other = "ilu0" if args.uneven_precond == "ilu0_isai" else "ilu0_isai"
+        u2 = uneven_bench(args, dev, precond=other)
+        rec["uneven"]["other_precond"] = {k: u2[k] for k in ("value", "ms_per_step", "all_converged_relres_le_1e-10",
+                                                             "tuned", "config")}
     rec["cpu_baseline"] = None if args.no_cpu else cpu_baseline(pat, A, b, "ilu0")
     print(json.dumps(rec), flush=True)
 
