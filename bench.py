@@ -247,8 +247,9 @@ def uneven_bench(args, dev, precond=None):
                                    f"in FP32 + FP32 BiCGSTAB in FP64 refinement, one hipGraph",
                        "precond": precond, "n": list(ns), "nnz": nnzs},
             "all_converged_relres_le_1e-10": bool((cv == 1).all() and (relres <= 1e-10).all()),
-            "tuned": {str(n): dict(zip(("factor_waves", "solve_waves", "jobs"), plan.config(p)))
-                      for p, n in enumerate(ns)},
+            "tuned": {str(n): dict(zip(("factor_waves", "solve_waves", "jobs"), plan.config(p)),
+                                   merged=plan.merged(p)) for p, n in enumerate(ns)},
+            "tune_table": {str(n): plan.tune_table(p) for p, n in enumerate(ns)},
             "tune_seconds": round(tune_s, 2),
             "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
@@ -424,6 +425,11 @@ def main():
     ms_mixed = run(o_mixed)
     mixed = summary(ms_mixed)
     x_mixed = x.clone()
+    # (2a) the same in one launch: the merged kernel factors each system in LDS, then solves it
+    o_mixed_m = bilu.opts(precond=bilu.ILU0_ISAI, precision=bilu.PRECISION_FP32_IR, merge=bilu.MERGED,
+                          inner_tol=args.inner_tol, **common)
+    mixed_merged = summary(run(o_mixed_m))
+    mixed_merged["x_equals_segregated"] = bool(torch.equal(x, x_mixed))
     # (2b) the other FP64 solvers of the plain method on the persistent kernels:
     # ParILU(0) 5 sweeps merged into the solve; ILU(0) / ParILU(0) + ISAI(L, U)
     # and the general ISAI of A (generation launch, then the persistent solve)
@@ -504,6 +510,18 @@ def main():
     learned = summary(ms_learn)
     fused_equal = bool(torch.equal(x, x_split))
     x_learn = x.clone()
+    # (3m) the learned step with the merged kernel (warm start, sweeps, ISAI and solve in one launch)
+    o_learn_m = bilu.opts(precond=bilu.PARILU0_ISAI, parilu_sweeps=40, parilu_stop_tol=1e-8,
+                          parilu_order=bilu.PARILU_INPLACE, precision=bilu.PRECISION_FP32_IR, merge=bilu.MERGED,
+                          inner_tol=args.inner_tol, **common)
+    out["fs"].zero_()
+    out["rf"].zero_()
+
+    def merged_step():
+        net.solve(P, dA, db, x, o_learn_m, out["it"], out["cv"], out["ir"], out["tr"], out["rf"], out["fs"],
+                  stream=stream)
+    learned_merged = summary(time_steps(merged_step, args.warmup, args.steps, stream))
+    learned_merged["x_equals_segregated"] = bool(torch.equal(x, x_learn))
     _, kms_factor = time_steps(fused_step, 0, max(2, args.steps // 2), stream, kernel=bilu.TIMING_FACTOR)
     # (3b) the same ParILU path from the standard start (no network), for the gain
     ms_nolearn = run(o_learn)
@@ -544,7 +562,9 @@ def main():
                        variant="learned+mixed"),
         "speedup_vs_plain_fp64": round(ms_plain / ms_learn, 3),
         "variants": {"plain_fp64_ilu0_merged": plain, "mixed_ilu0_isai": mixed,
-                     "learned_mixed_parilu0_isai": learned, "learned_mixed_parilu0_isai_split": split,
+                     "mixed_ilu0_isai_merged_kernel": mixed_merged,
+                     "learned_mixed_parilu0_isai": learned, "learned_mixed_parilu0_isai_merged_kernel": learned_merged,
+                     "learned_mixed_parilu0_isai_split": split,
                      "mixed_parilu0_isai_no_warm_start": nolearn,
                      "untrained_network_base_guess_mixed_parilu0_isai": untrained},
         "fp64_solver_variants": c1,

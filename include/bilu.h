@@ -65,6 +65,8 @@ extern "C" {
 
 #define BILU_MERGED 0         /* factor and solve in one launch (merged.cu) */
 #define BILU_SEGREGATED 1     /* factor launch, factors through HBM, solve launch (segregated.cu) */
+#define BILU_MERGE_AUTO 2     /* FP64: merged; FP32_IR: segregated (the faster at n = 144 when measured);
+                                 bilu_uneven_tune picks per pattern.  Results do not depend on it */
 
 #define BILU_KERNEL_AUTO 0        /* fastest kernel that supports the pattern and options */
 #define BILU_KERNEL_WAVE 1        /* one wave64 workgroup per system, schedules read from global */
@@ -100,7 +102,10 @@ typedef struct {
     double tol;           /* default 1e-11 */
     int tol_mode;         /* BILU_TOL_* */
     int max_iter;         /* full BiCGSTAB iterations, default 500 */
-    int merge;            /* BILU_MERGED / BILU_SEGREGATED */
+    int merge;            /* BILU_MERGED / BILU_SEGREGATED / BILU_MERGE_AUTO (default).  FP32_IR:
+                             merged = one persistent kernel factors (and generates the ISAI
+                             of) each system in LDS and solves it; segregated = the factor
+                             launch writes the FP32 factors / ISAI values to the workspace */
     int true_res_restart; /* 1: restart from b - A x when the true residual misses tol */
     int max_restarts;     /* bound on such restarts (default 3) */
     int kernel;           /* BILU_KERNEL_*: which kernel family runs the solve */
@@ -291,14 +296,28 @@ typedef struct bilu_uneven_s *bilu_uneven_t;
 int bilu_uneven_create(int npat, const bilu_pattern_t *patterns, int nreq, const int *req_pattern,
                        const int *req_index, const bilu_opts *opts, bilu_uneven_t *out);
 int bilu_uneven_destroy(bilu_uneven_t u);
-/* autotuner: times the candidate workgroup sizes (waves per workgroup of the
- * factor and of the solve kernel) of every pattern on its own requests with
- * these arrays and keeps the fastest (results do not depend on the choice) */
+/* autotuner: times every pattern's candidates -- the merge point (merged:
+ * one kernel factors each system in LDS and solves it; segregated: factor
+ * kernel, FP32 factors / ISAI values through HBM, solve kernel; opts.merge
+ * BILU_MERGED / BILU_SEGREGATED fixes it, BILU_MERGE_AUTO lets the tuner
+ * choose) and the workgroup sizes (waves per factor workgroup {2, 4, 8}, per
+ * solve workgroup {4, 8, 12}) -- first alone on the pattern's requests, then
+ * pattern by pattern (slowest first) inside the whole launch with the other
+ * patterns running, and keeps the fastest (results do not depend on the
+ * choice) */
 int bilu_uneven_tune(bilu_uneven_t u, const double *const *d_A, const double *const *d_b, double *const *d_x,
                      float *d_iter, int *d_conv, double *d_iter_res, double *d_true_res, int *d_refine,
                      void *stream);
-/* chosen configuration of pattern p and its number of requests */
+/* chosen configuration of pattern p and its number of requests (factor_waves
+ * 0: the merged kernel) */
 int bilu_uneven_config(bilu_uneven_t u, int p, int *factor_waves, int *solve_waves, int *jobs);
+/* 1 when pattern p runs the merged kernel */
+int bilu_uneven_merge_point(bilu_uneven_t u, int p, int *merged);
+/* the candidates the last bilu_uneven_tune timed for pattern p (up to cap of
+ * them; *count = how many): merge point, waves per factor / solve workgroup,
+ * ms alone and inside the whole launch (-1: not timed there) */
+int bilu_uneven_tune_table(bilu_uneven_t u, int p, int cap, int *merged, int *factor_waves, int *solve_waves,
+                           float *alone_ms, float *corun_ms, int *count);
 int bilu_uneven_solve(bilu_uneven_t u, const double *const *d_A, const double *const *d_b, double *const *d_x,
                       float *d_iter, int *d_conv, double *d_iter_res, double *d_true_res, int *d_refine,
                       void *stream);

@@ -20,7 +20,7 @@ ERR_INVALID_ARG, ERR_INVALID_PATTERN, ERR_MISSING_DIAGONAL, ERR_TOO_LARGE, ERR_H
 CONVERGED, NOT_CONVERGED, ZERO_PIVOT, NONFINITE = 1, -1, -2, -3
 ILU0, PARILU0, ILU0_ISAI, PARILU0_ISAI, GENERAL_ISAI, NONE, FACTORS = 0, 1, 2, 3, 4, 5, 6
 TOL_ABSOLUTE, TOL_RELATIVE = 0, 1
-MERGED, SEGREGATED = 0, 1
+MERGED, SEGREGATED, MERGE_AUTO = 0, 1, 2
 ISAI_GENERAL, ISAI_LOWER, ISAI_UPPER = 0, 1, 2
 KERNEL_AUTO, KERNEL_WAVE, KERNEL_PERSISTENT = 0, 1, 2
 PRECISION_FP64, PRECISION_FP32_IR = 0, 1
@@ -36,6 +36,7 @@ EXPORTS = [
     "bilu_true_residual", "bilu_mlp_create", "bilu_mlp_destroy", "bilu_mlp_num_params",
     "bilu_mlp_get_weights", "bilu_mlp_set_weights", "bilu_mlp_gradient", "bilu_mlp_train_step",
     "bilu_mlp_infer", "bilu_uneven_create", "bilu_uneven_destroy", "bilu_uneven_tune", "bilu_uneven_config",
+    "bilu_uneven_merge_point", "bilu_uneven_tune_table",
     "bilu_uneven_solve", "bilu_pattern_add_diagonal", "bilu_expand_values", "bilu_isai_pattern",
     "bilu_isai_power", "bilu_mlp_solve", "bilu_ilu0_ex", "bilu_ilu0_tune", "bilu_pattern_reserve",
     "bilu_solve_ex_noalloc", "bilu_timing_begin", "bilu_timing_end",
@@ -115,6 +116,9 @@ def lib():
         L.bilu_uneven_destroy.argtypes = [vp]
         L.bilu_uneven_tune.argtypes = [vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp), vp, vp, vp, vp, vp, vp]
         L.bilu_uneven_config.argtypes = [vp, C.c_int, i32p, i32p, i32p]
+        L.bilu_uneven_merge_point.argtypes = [vp, C.c_int, i32p]
+        L.bilu_uneven_tune_table.argtypes = [vp, C.c_int, C.c_int, i32p, i32p, i32p, C.POINTER(C.c_float),
+                                             C.POINTER(C.c_float), i32p]
         L.bilu_uneven_solve.argtypes = [vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp), vp, vp, vp, vp, vp, vp]
         L.bilu_mlp_create.argtypes = [vp, C.c_int, C.c_ulonglong, C.POINTER(vp)]
         L.bilu_mlp_destroy.argtypes = [vp]
@@ -138,7 +142,7 @@ def _check(rc, what):
 
 
 def opts(precond=ILU0, parilu_sweeps=5, tol=1e-11, tol_mode=TOL_ABSOLUTE, max_iter=500,
-         merge=MERGED, true_res_restart=0, max_restarts=3, kernel=KERNEL_AUTO,
+         merge=MERGE_AUTO, true_res_restart=0, max_restarts=3, kernel=KERNEL_AUTO,
          precision=PRECISION_FP64, inner_tol=1e-5, max_refine=10, parilu_stop_tol=0.0,
          isai_power=1, parilu_order=PARILU_JACOBI, inner_adapt=0.5) -> Opts:
     return Opts(precond, parilu_sweeps, tol, tol_mode, max_iter, merge, true_res_restart,
@@ -496,9 +500,29 @@ class Uneven:
         self._call(lib().bilu_uneven_solve, A, b, x, out, stream)
 
     def config(self, p):
+        """(waves per factor workgroup, 0 for the merged kernel; waves per solve
+        workgroup; requests) of pattern p"""
         fw, sw, jobs = C.c_int(), C.c_int(), C.c_int()
         _check(lib().bilu_uneven_config(self.h, p, C.byref(fw), C.byref(sw), C.byref(jobs)), "bilu_uneven_config")
         return fw.value, sw.value, jobs.value
+
+    def merged(self, p):
+        m = C.c_int()
+        _check(lib().bilu_uneven_merge_point(self.h, p, C.byref(m)), "bilu_uneven_merge_point")
+        return bool(m.value)
+
+    def tune_table(self, p):
+        """the candidates the last tune() timed for pattern p: dicts of merged,
+        factor_waves, solve_waves, alone_ms, corun_ms"""
+        cnt = C.c_int()
+        _check(lib().bilu_uneven_tune_table(self.h, p, 0, None, None, None, None, None, C.byref(cnt)),
+               "bilu_uneven_tune_table")
+        k = cnt.value
+        mg, fw, sw = (C.c_int * max(k, 1))(), (C.c_int * max(k, 1))(), (C.c_int * max(k, 1))()
+        al, co = (C.c_float * max(k, 1))(), (C.c_float * max(k, 1))()
+        _check(lib().bilu_uneven_tune_table(self.h, p, k, mg, fw, sw, al, co, C.byref(cnt)), "bilu_uneven_tune_table")
+        return [{"merged": bool(mg[i]), "factor_waves": fw[i], "solve_waves": sw[i],
+                 "alone_ms": round(al[i], 4), "corun_ms": round(co[i], 4)} for i in range(k)]
 
     def close(self):
         if getattr(self, "h", None):

@@ -72,6 +72,178 @@ __device__ __forceinline__ int isai_gen(const uint32_t *__restrict__ r0p, const 
     return bad;
 }
 
+// F[0 .. cnt) (FP64) rounded to FP32 in place (the first half of the
+// region), 64 values at a time: a chunk's values are all read before its
+// writes, which cover only the halves of chunks already read
+__device__ __forceinline__ void round_in_place(double *F, int cnt, int lane)
+{
+    float *F32 = reinterpret_cast<float *>(F);
+    for (int e0 = 0; e0 < cnt; e0 += WAVE) {
+        const int e = e0 + lane;
+        const double v = e < cnt ? F[e] : 0.0;
+        asm volatile("" ::: "memory");
+        if (e < cnt) F32[e] = (float)v;
+        asm volatile("" ::: "memory");
+    }
+}
+
+// One system's factorization in the wave's LDS region: A (FP64, Ap) into
+// F[nnz + 4] (F[nnz] = 0, F[nnz + 1] = 1, F[nnz + 2] scratch), then exact
+// ILU(0) (FS_ILU) or ParILU(0) sweeps (FS_PARILU: warm start from the
+// network's raw outputs Yp or from the guess Ip, in place or Jacobi between F
+// and W, residual stop with exact ILU(0) as the fallback).  isai: ISAI(L)
+// and ISAI(U) of the factors into the ISAI region (TO = float: the factors
+// rounded to FP32 in place, M = the second half of F's region; TO = double:
+// FP64 factors, M = W's region).  Returns the ISAI generation's zero-divisor
+// flag (wave-uniform); status = sweeps used, -1 on the fallback (0 for FS_ILU).
+template <typename TO>
+__device__ __forceinline__ int factor_system(const FastParams &fp, int n, int nnz, const double *__restrict__ Ap,
+                                             const double *__restrict__ Ip, const _Float16 *__restrict__ Yp,
+                                             int mode, int sweeps, double stop_tol, int isai, int inplace,
+                                             const uint32_t *par_l, bool par_lds, double *F, double *W, float *Dg,
+                                             int lane, int &status)
+{
+    float *F32 = reinterpret_cast<float *>(F);
+    float *M = F32 + (nnz + 4);
+    status = 0;
+    wave_copy_wide<24>(F, Ap, nnz, lane);
+    // the zero and one cells (the previous system's ISAI values may cover them)
+    if (lane == 0) { F[nnz] = 0.0; F[nnz + 1] = 1.0; }
+    if (mode == FS_PARILU) {
+        wave_sync();
+        double anorm2 = 0.0;
+        if (stop_tol > 0.0) {
+            for (int e = lane; e < nnz; e += WAVE) anorm2 = fma(F[e], F[e], anorm2);
+            anorm2 = wave_sum(anorm2);
+        }
+        if (Yp) {
+            // the warm start from the network's raw outputs y (FP16):
+            // guess = SIGMA sinh(YS (base + y)) for the entries with work, the
+            // same operations as bilu_mlp_infer (guess_from); the others keep
+            // a_ij.  The diagonal of A goes to Dg first (FP32, as the base guess
+            // rounds it), so the positions are visited in one coalesced pass
+            // with WU outputs and position codes per lane in flight: a lower
+            // entry (i, j) (code = j) reads its pivot a_jj from Dg, an upper
+            // entry with work has code 0xFFFE, one without work 0xFFFF.
+            constexpr int WU = 24;
+            for (int j = lane; j < n; j += WAVE) Dg[j] = (float)F[fp.diag[j]];
+            wave_sync();
+            // the loads are unconditional (positions past nnz re-read the last
+            // one), so all 2 WU of a lane are in flight together
+            for (int e0 = lane; e0 < nnz; e0 += WU * WAVE) {
+                _Float16 yh[WU];
+                uint32_t cd[WU];
+#pragma unroll
+                for (int j = 0; j < WU; j++) {
+                    const int e = min(e0 + j * WAVE, nnz - 1);
+                    yh[j] = Yp[e];
+                    cd[j] = fp.wcode[e];
+                }
+#pragma unroll
+                for (int j = 0; j < WU; j++) {
+                    const int e = e0 + j * WAVE;
+                    if (e < nnz && cd[j] != 0xFFFFu) {
+                        const bool lower = cd[j] != 0xFFFEu;
+                        F[e] = bilu_mlp::guess_from((float)F[e], lower ? Dg[cd[j]] : 1.0f, lower, (float)yh[j]);
+                    }
+                }
+            }
+        } else if (Ip) {
+            // entries with work start from the guess; the others are exact (= a)
+            // unconditional loads (entries past ne re-read the last one)
+            for (int i0 = lane; i0 < fp.ne; i0 += 8 * WAVE) {
+                int pos[8];
+                double g[8];
+#pragma unroll
+                for (int j = 0; j < 8; j++) pos[j] = fp.etab[min(i0 + j * WAVE, fp.ne - 1)].x;
+#pragma unroll
+                for (int j = 0; j < 8; j++) g[j] = Ip[pos[j]];
+#pragma unroll
+                for (int j = 0; j < 8; j++)
+                    if (i0 + j * WAVE < fp.ne) F[pos[j]] = g[j];
+            }
+        }
+        wave_sync();
+        const double lim2 = stop_tol * stop_tol * anorm2;
+        int used = 0;
+        bool met = stop_tol <= 0.0;
+        if (inplace) {
+            // in-place sweeps over F; the stop test uses the residuals the
+            // rounds saw, and the iterate after the sweep is kept
+            while (used < sweeps) {
+                const double rr = wave_sum(fp.par_rounds > 0
+                    ? (par_lds ? parilu_inplace<true>(par_l, par_l + fp.par_psteps * WAVE, fp.par_rounds,
+                                                      fp.par_psteps, Ap, F, lane)
+                               : parilu_inplace<true>(fp.parp, fp.parf, fp.par_rounds, fp.par_psteps, Ap, F,
+                                                      lane))
+                    : 0.0);
+                wave_sync();
+                used++;
+                if (stop_tol > 0.0 && rr <= lim2) { met = true; break; }
+            }
+        } else {
+            // Jacobi sweeps between F and W (entries without work and the zero cell in both)
+            wave_copy(W, F, nnz, lane);
+            if (lane == 0) W[nnz] = 0.0;
+            wave_sync();
+            double *src = F, *dst = W;
+            while (used < sweeps) {
+                const double rr = wave_sum(fp.par_rounds > 0
+                    ? (par_lds ? parilu_rounds<true>(par_l, par_l + fp.par_psteps * WAVE, fp.par_rounds,
+                                                     fp.par_psteps, Ap, src, dst, lane)
+                               : parilu_rounds<true>(fp.parp, fp.parf, fp.par_rounds, fp.par_psteps, Ap, src,
+                                                     dst, lane))
+                    : 0.0);
+                wave_sync();
+                used++;
+                if (stop_tol > 0.0 && rr <= lim2) {
+                    // the iterate this sweep started from (src) meets the tolerance: keep it
+                    used--;
+                    met = true;
+                    break;
+                }
+                double *tmp = src; src = dst; dst = tmp;
+            }
+            if (src != F) {
+                wave_copy(F, src, nnz, lane);
+                wave_sync();
+            }
+        }
+        if (!met) {
+            // fallback: exact ILU(0)
+            wave_copy(F, Ap, nnz, lane);
+            wave_sync();
+            ilu_stream(fp.ilu0, fp.ilu1, fp.ilu_steps, F, F + nnz + 2, fp.diag, n, lane);
+            status = -1;
+        } else {
+            status = used;
+        }
+    } else {
+        wave_sync();
+        ilu_stream(fp.ilu0, fp.ilu1, fp.ilu_steps, F, F + nnz + 2, fp.diag, n, lane);
+    }
+    int bad = 0;
+    if (isai) {
+        if constexpr (sizeof(TO) == 4) {
+            wave_sync();
+            round_in_place(F, nnz + 4, lane);
+            if (lane == 0) { M[nnz] = 0.0f; M[nnz + 1] = 1.0f; M[nnz + 2] = -1.0f; }
+            wave_sync();
+            bad = isai_gen<false>(fp.isL0, fp.isL1, fp.isL_steps, F32, M, M + nnz + 3, lane);
+            bad |= isai_gen<true>(fp.isU0, fp.isU1, fp.isU_steps, F32, M, M + nnz + 3, lane);
+        } else {
+            double *M64 = W;
+            wave_sync();
+            if (lane == 0) { M64[nnz] = 0.0; M64[nnz + 1] = 1.0; M64[nnz + 2] = -1.0; }
+            wave_sync();
+            bad = isai_gen<false, double>(fp.isL0, fp.isL1, fp.isL_steps, F, M64, M64 + nnz + 3, lane);
+            bad |= isai_gen<true, double>(fp.isU0, fp.isU1, fp.isU_steps, F, M64, M64 + nnz + 3, lane);
+        }
+        bad = __any(bad);
+    }
+    return bad;
+}
+
 template <typename TO>
 __global__ __launch_bounds__(KF_MAX_WAVES * WAVE) void k_factor_fast(
     FastParams fp, int n, int nnz, int batch, const double *__restrict__ gA,
@@ -103,8 +275,7 @@ __global__ __launch_bounds__(KF_MAX_WAVES * WAVE) void k_factor_fast(
     }
     double *F = reinterpret_cast<double *>(smem + fp.shared_bytes + (size_t)wave * fp.wave_bytes);
     double *W = F + (nnz + 4);
-    float *F32 = reinterpret_cast<float *>(F);
-    float *M = F32 + (nnz + 4);
+    float *M = reinterpret_cast<float *>(F) + (nnz + 4);
     // the fused warm start's diagonal of A (FP32), after F (and W)
     float *Dg = reinterpret_cast<float *>(reinterpret_cast<unsigned char *>(F) + fp.wave_bytes) - fp.dg_floats;
     for (;;) {
@@ -112,157 +283,16 @@ __global__ __launch_bounds__(KF_MAX_WAVES * WAVE) void k_factor_fast(
         if (lane == 0) sys = atomicAdd(fp.counter, 1);
         sys = __builtin_amdgcn_readfirstlane(sys);
         if (sys >= batch) break;
-        const double *Ap = gA + (size_t)(sysidx ? sysidx[sys] : sys) * nnz;
-        int status = 0;
-        wave_copy_wide<24>(F, Ap, nnz, lane);
-        // the zero and one cells (the previous system's ISAI values may cover them)
-        if (lane == 0) { F[nnz] = 0.0; F[nnz + 1] = 1.0; }
-        if (mode == FS_PARILU) {
-            wave_sync();
-            double anorm2 = 0.0;
-            if (stop_tol > 0.0) {
-                for (int e = lane; e < nnz; e += WAVE) anorm2 = fma(F[e], F[e], anorm2);
-                anorm2 = wave_sum(anorm2);
-            }
-            if (gY) {
-                // the warm start from the network's raw outputs y (row sys of gY, FP16):
-                // guess = SIGMA sinh(YS (base + y)) for the entries with work, the
-                // same operations as bilu_mlp_infer (guess_from); the others keep
-                // a_ij.  The diagonal of A goes to Dg first (FP32, as the base guess
-                // rounds it), so the positions are visited in one coalesced pass
-                // with WU outputs and position codes per lane in flight: a lower
-                // entry (i, j) (code = j) reads its pivot a_jj from Dg, an upper
-                // entry with work has code 0xFFFE, one without work 0xFFFF.
-                constexpr int WU = 24;
-                for (int j = lane; j < n; j += WAVE) Dg[j] = (float)F[fp.diag[j]];
-                wave_sync();
-                const _Float16 *Yp = gY + (size_t)sys * P3;
-                // the loads are unconditional (positions past nnz re-read the last
-                // one), so all 2 WU of a lane are in flight together
-                for (int e0 = lane; e0 < nnz; e0 += WU * WAVE) {
-                    _Float16 yh[WU];
-                    uint32_t cd[WU];
-#pragma unroll
-                    for (int j = 0; j < WU; j++) {
-                        const int e = min(e0 + j * WAVE, nnz - 1);
-                        yh[j] = Yp[e];
-                        cd[j] = fp.wcode[e];
-                    }
-#pragma unroll
-                    for (int j = 0; j < WU; j++) {
-                        const int e = e0 + j * WAVE;
-                        if (e < nnz && cd[j] != 0xFFFFu) {
-                            const bool lower = cd[j] != 0xFFFEu;
-                            F[e] = bilu_mlp::guess_from((float)F[e], lower ? Dg[cd[j]] : 1.0f, lower, (float)yh[j]);
-                        }
-                    }
-                }
-            } else if (gFinit) {
-                // entries with work start from the guess; the others are exact (= a)
-                const double *Ip = gFinit + (size_t)(sysidx ? sysidx[sys] : sys) * nnz;
-                // unconditional loads (entries past ne re-read the last one)
-                for (int i0 = lane; i0 < fp.ne; i0 += 8 * WAVE) {
-                    int pos[8];
-                    double g[8];
-#pragma unroll
-                    for (int j = 0; j < 8; j++) pos[j] = fp.etab[min(i0 + j * WAVE, fp.ne - 1)].x;
-#pragma unroll
-                    for (int j = 0; j < 8; j++) g[j] = Ip[pos[j]];
-#pragma unroll
-                    for (int j = 0; j < 8; j++)
-                        if (i0 + j * WAVE < fp.ne) F[pos[j]] = g[j];
-                }
-            }
-            wave_sync();
-            const double lim2 = stop_tol * stop_tol * anorm2;
-            int used = 0;
-            bool met = stop_tol <= 0.0;
-            if (inplace) {
-                // in-place sweeps over F; the stop test uses the residuals the
-                // rounds saw, and the iterate after the sweep is kept
-                while (used < sweeps) {
-                    const double rr = wave_sum(fp.par_rounds > 0
-                        ? (par_lds ? parilu_inplace<true>(par_l, par_l + fp.par_psteps * WAVE, fp.par_rounds,
-                                                          fp.par_psteps, Ap, F, lane)
-                                   : parilu_inplace<true>(fp.parp, fp.parf, fp.par_rounds, fp.par_psteps, Ap, F,
-                                                          lane))
-                        : 0.0);
-                    wave_sync();
-                    used++;
-                    if (stop_tol > 0.0 && rr <= lim2) { met = true; break; }
-                }
-            } else {
-                // Jacobi sweeps between F and W (entries without work and the zero cell in both)
-                wave_copy(W, F, nnz, lane);
-                if (lane == 0) W[nnz] = 0.0;
-                wave_sync();
-                double *src = F, *dst = W;
-                while (used < sweeps) {
-                    const double rr = wave_sum(fp.par_rounds > 0
-                        ? (par_lds ? parilu_rounds<true>(par_l, par_l + fp.par_psteps * WAVE, fp.par_rounds,
-                                                         fp.par_psteps, Ap, src, dst, lane)
-                                   : parilu_rounds<true>(fp.parp, fp.parf, fp.par_rounds, fp.par_psteps, Ap, src,
-                                                         dst, lane))
-                        : 0.0);
-                    wave_sync();
-                    used++;
-                    if (stop_tol > 0.0 && rr <= lim2) {
-                        // the iterate this sweep started from (src) meets the tolerance: keep it
-                        used--;
-                        met = true;
-                        break;
-                    }
-                    double *tmp = src; src = dst; dst = tmp;
-                }
-                if (src != F) {
-                    wave_copy(F, src, nnz, lane);
-                    wave_sync();
-                }
-            }
-            if (!met) {
-                // fallback: exact ILU(0)
-                wave_copy(F, Ap, nnz, lane);
-                wave_sync();
-                ilu_stream(fp.ilu0, fp.ilu1, fp.ilu_steps, F, F + nnz + 2, fp.diag, n, lane);
-                status = -1;
-            } else {
-                status = used;
-            }
-        } else {
-            wave_sync();
-            ilu_stream(fp.ilu0, fp.ilu1, fp.ilu_steps, F, F + nnz + 2, fp.diag, n, lane);
-        }
+        const int si = sysidx ? sysidx[sys] : sys;
+        int status;
+        const int bad = factor_system<TO>(fp, n, nnz, gA + (size_t)si * nnz,
+                                          gFinit ? gFinit + (size_t)si * nnz : nullptr,
+                                          gY ? gY + (size_t)sys * P3 : nullptr, mode, sweeps, stop_tol, isai,
+                                          inplace, par_l, par_lds, F, W, Dg, lane, status);
         if (isai) {
-            int bad;
-            if constexpr (sizeof(TO) == 4) {
-                // round the factors (and the zero / one cells) to FP32 in place,
-                // 64 values at a time: a chunk's values are all read before its
-                // writes, which cover only the halves of chunks already read
-                wave_sync();
-                for (int e0 = 0; e0 < nnz + 4; e0 += WAVE) {
-                    const int e = e0 + lane;
-                    const double v = e < nnz + 4 ? F[e] : 0.0;
-                    asm volatile("" ::: "memory");
-                    if (e < nnz + 4) F32[e] = (float)v;
-                    asm volatile("" ::: "memory");
-                }
-                if (lane == 0) { M[nnz] = 0.0f; M[nnz + 1] = 1.0f; M[nnz + 2] = -1.0f; }
-                wave_sync();
-                bad = isai_gen<false>(fp.isL0, fp.isL1, fp.isL_steps, F32, M, M + nnz + 3, lane);
-                bad |= isai_gen<true>(fp.isU0, fp.isU1, fp.isU_steps, F32, M, M + nnz + 3, lane);
-                wave_sync();
-                wave_copy(static_cast<float *>(gM) + (size_t)sys * nnz, M, nnz, lane);
-            } else {
-                double *M64 = W;
-                wave_sync();
-                if (lane == 0) { M64[nnz] = 0.0; M64[nnz + 1] = 1.0; M64[nnz + 2] = -1.0; }
-                wave_sync();
-                bad = isai_gen<false, double>(fp.isL0, fp.isL1, fp.isL_steps, F, M64, M64 + nnz + 3, lane);
-                bad |= isai_gen<true, double>(fp.isU0, fp.isU1, fp.isU_steps, F, M64, M64 + nnz + 3, lane);
-                wave_sync();
-                wave_copy(static_cast<double *>(gM) + (size_t)sys * nnz, M64, nnz, lane);
-            }
-            bad = __any(bad);
+            wave_sync();
+            if constexpr (sizeof(TO) == 4) wave_copy(static_cast<float *>(gM) + (size_t)sys * nnz, M, nnz, lane);
+            else wave_copy(static_cast<double *>(gM) + (size_t)sys * nnz, W, nnz, lane);
             if (lane == 0) gZ[sys] = bad;
         } else {
             TO *Fo = gF + (size_t)sys * nnz;
@@ -272,6 +302,18 @@ __global__ __launch_bounds__(KF_MAX_WAVES * WAVE) void k_factor_fast(
         wave_sync();
     }
 }
+
+// the factorization of the merged mixed kernel (FM = 1): the arguments of
+// factor_system, and where the ParILU status goes (by result index; null:
+// nowhere)
+struct FactorArgs {
+    int mode, sweeps, inplace;
+    double stop_tol;
+    const double *Finit;    // ParILU guess [system][nnz] or null
+    const _Float16 *Y;      // network outputs [job][P3] or null
+    int P3;
+    int *status;
+};
 
 struct MixedParams {
     double tol;          // outer tolerance (relative or absolute, tol_mode)
@@ -416,13 +458,19 @@ constexpr int PCK_LU = 0, PCK_ISAI = 1;
 // VGPRs; ISAI: 2 x 40 triangle coefficients)
 constexpr int km_waves(int SM, int PCK) { return SM > 48 ? 8 : (PCK == PCK_LU ? KM_MAX_WAVES : KMI_MAX_WAVES); }
 
-template <int NR, int SM, int PCK, int SLU = 1>
+// FM = 0 (segregated): the FP32 factors / ISAI values come from gF (a
+// factor launch before).  FM = 1 (merged): each system is factored in the
+// wave's own region first (factor_system, FP64 in LDS, then FP32), and the
+// solve reads the factors / ISAI values there; the region is the larger of
+// the two layouts, the solve's overlaying the factorization's once its
+// values are in registers.  Same arithmetic, same results bit for bit.
+template <int NR, int SM, int PCK, int SLU = 1, int FM = 0>
 __global__ __launch_bounds__(km_waves(SM, PCK) * WAVE) void k_mixed(
     DevPattern P, FastParams fp, int batch, const double *__restrict__ gA,
     const double *__restrict__ gb, double *__restrict__ gx, const float *__restrict__ gF,
     const int *__restrict__ gZ, MixedParams mp, float *__restrict__ g_iter, int *__restrict__ g_conv,
     double *__restrict__ g_ires, double *__restrict__ g_tres, int *__restrict__ g_refine,
-    const int *__restrict__ sysidx, const int *__restrict__ outidx)
+    const int *__restrict__ sysidx, const int *__restrict__ outidx, FactorArgs fa)
 {
     // job j solves system sysidx[j] (A, b, x) and writes its results at
     // outidx[j] (identity when null); factors / ISAI values are per job
@@ -497,17 +545,39 @@ __global__ __launch_bounds__(km_waves(SM, PCK) * WAVE) void k_mixed(
         const double *Ap = gA + (size_t)si * nnz;
         const double *bp = gb + (size_t)si * n;
         const float *Fp = gF + (size_t)sys * nnz;
-        if (PCK == PCK_LU)
+        int fbad = 0;
+        if constexpr (FM == 1) {
+            // merged: the factors (LU: FP32 in place at F) or the ISAI values
+            // (FP32 at M, the second half of the FP64 factor region) in LDS
+            double *F64 = reinterpret_cast<double *>(wb);
+            float *Dg = reinterpret_cast<float *>(wb + fp.wave_bytes) - fp.dg_floats;
+            int status;
+            fbad = factor_system<float>(fp, n, nnz, Ap, fa.Finit ? fa.Finit + (size_t)si * nnz : nullptr,
+                                        fa.Y ? fa.Y + (size_t)sys * fa.P3 : nullptr, fa.mode, fa.sweeps,
+                                        fa.stop_tol, PCK == PCK_ISAI ? 1 : 0, fa.inplace, nullptr, false, F64,
+                                        F64 + (nnz + 4), Dg, lane, status);
+            if (fa.status && lane == 0) fa.status[oi] = status;
+            wave_sync();
+            if constexpr (PCK == PCK_LU) {
+                // the factors as the solve reads them: FP32 in place (F[nnz] = 0)
+                round_in_place(F64, nnz + 4, lane);
+                wave_sync();
+            }
+            Fp = reinterpret_cast<const float *>(wb) + (PCK == PCK_ISAI ? nnz + 4 : 0);
+        } else if (PCK == PCK_LU) {
             wave_copy(F, Fp, nnz, lane);
+        }
+        if constexpr (PCK == PCK_ISAI) {
+            // ISAI(L)'s unit diagonal (position -2) reads as 1; gathered before
+            // A's slot values, which overlay them in the merged kernel
+            slot_gather<SLU>(fp.isl_pos, fp.SL, Fp, lane, [&](int q, float v) { cL[q] = v; });
+            slot_gather<SLU>(fp.isu_pos, fp.SU, Fp, lane, [&](int q, float v) { cU[q] = v; });
+            if constexpr (FM == 1) wave_sync();
+        }
         slot_gather<SM>(fp.slot_pos, S, Ap, lane, [&](int q, double v) {
             if constexpr (PCK == PCK_LU) a[q] = (float)v;
             else Asl[q * WAVE + lane] = (float)v;
         });
-        if constexpr (PCK == PCK_ISAI) {
-            // ISAI(L)'s unit diagonal (position -2) reads as 1
-            slot_gather<SLU>(fp.isl_pos, fp.SL, Fp, lane, [&](int q, float v) { cL[q] = v; });
-            slot_gather<SLU>(fp.isu_pos, fp.SU, Fp, lane, [&](int q, float v) { cU[q] = v; });
-        }
         wave_sync();
         // zero cells of the FP32 views (the FP64 residual overwrites them)
         auto zero_cells = [&]() {
@@ -527,7 +597,7 @@ __global__ __launch_bounds__(km_waves(SM, PCK) * WAVE) void k_mixed(
             }
             bad = __any(bad);
         } else {
-            bad = gZ[sys];
+            bad = FM == 1 ? fbad : gZ[sys];
         }
 
         Vec<NR, double> x, r;

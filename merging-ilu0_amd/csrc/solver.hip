@@ -1401,14 +1401,14 @@ template <int NR, int SM, int PCK, int SLU>
 int launch_mixed_t(bilu_pattern_s &h, const FastParams &fp, int grid, int waves, size_t lds, int batch,
                    const double *A, const double *b, double *x, const float *F, const int *Z,
                    const MixedParams &mp, float *it, int *cv, double *ir, double *tr, int *refine,
-                   hipStream_t st, Jobs jobs)
+                   hipStream_t st, Jobs jobs, const FactorArgs *fa)
 {
-    auto kern = k_mixed<NR, SM, PCK, SLU>;
+    auto kern = fa ? k_mixed<NR, SM, PCK, SLU, 1> : k_mixed<NR, SM, PCK, SLU, 0>;
     if (lds > 65536) HIP_TRY(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, st, fp.counter);
     timing_mark(BILU_TIMING_SOLVE, st, true);
     hipLaunchKernelGGL(kern, dim3(grid), dim3(waves * WAVE), lds, st, h.dev, fp, batch, A, b, x, F, Z, mp, it, cv,
-                       ir, tr, refine, jobs.sysidx, jobs.outidx);
+                       ir, tr, refine, jobs.sysidx, jobs.outidx, fa ? *fa : FactorArgs{});
     timing_mark(BILU_TIMING_SOLVE, st, false);
     HIP_TRY(hipGetLastError());
     return BILU_OK;
@@ -1422,10 +1422,11 @@ int slot_template_mixed(int S)
 
 // FP32 inner / FP64 refinement solve over FP32 factors F (pck = PCK_LU) or
 // FP32 ISAI values F with zero-pivot flags Z (pck = PCK_ISAI); at most
-// max_waves waves per workgroup
+// max_waves waves per workgroup.  fa non-null: the merged kernel, which
+// factors each system itself (F and Z unused)
 int launch_mixed(bilu_pattern_s &h, int batch, const double *A, const double *b, double *x, const float *F,
                  const int *Z, int pck, const MixedParams &mp, float *it, int *cv, double *ir, double *tr,
-                 int *refine, hipStream_t st, Jobs jobs = Jobs(), int max_waves = 64)
+                 int *refine, hipStream_t st, Jobs jobs = Jobs(), int max_waves = 64, const FactorArgs *fa = nullptr)
 {
     const auto &pl = h.pa.fast;
     const int n = h.pa.n, nnz = h.pa.nnz;
@@ -1449,6 +1450,15 @@ int launch_mixed(bilu_pattern_s &h, int batch, const double *A, const double *b,
     fp.wave_bytes = pck == PCK_LU
         ? int(((size_t)(nnz + 2) * 4 + 15) / 16 * 16 + (size_t)2 * (n + 2) * 8 + (size_t)64 * nr * 8)
         : int((size_t)sm * WAVE * 4 + (size_t)2 * (n + 2) * 8);
+    if (fa) {
+        // merged: the factorization's region (FP64 factors, the Jacobi buffer, the
+        // warm start's FP32 diagonal at the end) when it is the larger
+        fp.dg_floats = fa->Y ? (n + 3) / 4 * 4 : 0;
+        const size_t fr = (size_t)(nnz + 4) * 8 +
+                          (fa->mode == FS_PARILU && !fa->inplace ? (size_t)(nnz + 2) * 8 : 0) +
+                          (size_t)fp.dg_floats * 4;
+        fp.wave_bytes = std::max<int>(fp.wave_bytes, (int)fr);
+    }
     fp.wave_bytes = (fp.wave_bytes + 15) / 16 * 16;
     // shared LDS: A's slot records (ISAI: then the two triangle layouts' records)
     const size_t rec = (size_t)(sm + (pck == PCK_ISAI ? 2 * slu : 0)) * WAVE * 4;
@@ -1463,8 +1473,8 @@ int launch_mixed(bilu_pattern_s &h, int batch, const double *A, const double *b,
 #define MIXED_CASE(R, M, T) \
     if (nr == R && sm == M) \
         return pck == PCK_LU \
-            ? launch_mixed_t<R, M, PCK_LU, 1>(h, fp, grid, waves, lds, batch, A, b, x, F, Z, mp, it, cv, ir, tr, refine, st, jobs) \
-            : launch_mixed_t<R, M, PCK_ISAI, T>(h, fp, grid, waves, lds, batch, A, b, x, F, Z, mp, it, cv, ir, tr, refine, st, jobs);
+            ? launch_mixed_t<R, M, PCK_LU, 1>(h, fp, grid, waves, lds, batch, A, b, x, F, Z, mp, it, cv, ir, tr, refine, st, jobs, fa) \
+            : launch_mixed_t<R, M, PCK_ISAI, T>(h, fp, grid, waves, lds, batch, A, b, x, F, Z, mp, it, cv, ir, tr, refine, st, jobs, fa);
 #define MIXED_ROW(R) MIXED_CASE(R, 24, 20) MIXED_CASE(R, 36, 24) MIXED_CASE(R, 48, 32) MIXED_CASE(R, 72, 40)
     MIXED_ROW(1) MIXED_ROW(2) MIXED_ROW(3) MIXED_ROW(4) MIXED_ROW(5)
 #undef MIXED_ROW
@@ -1542,7 +1552,7 @@ void bilu_opts_default(bilu_opts *o)
     o->tol = 1e-11;
     o->tol_mode = BILU_TOL_ABSOLUTE;
     o->max_iter = 500;
-    o->merge = BILU_MERGED;
+    o->merge = BILU_MERGE_AUTO;
     o->true_res_restart = 0;
     o->max_restarts = 3;
     o->kernel = BILU_KERNEL_AUTO;
@@ -1905,6 +1915,14 @@ static int solve_ex_impl(bilu_pattern_t h, int batch, const double *d_A, const d
     if (o.precision == BILU_PRECISION_FP32_IR) {
         const int nr = (h->pa.n + WAVE - 1) / WAVE;
         if (slot_template_mixed(h->pa.fast.S) < 0 || nr > MAX_NR || h->pa.fast.nr != nr) return BILU_ERR_TOO_LARGE;
+        MixedParams mp{o.tol, o.tol_mode, (float)o.inner_tol, o.max_iter, o.max_refine, o.inner_tol, o.inner_adapt};
+        if (o.merge == BILU_MERGED) {
+            // one launch: each wave factors its system in LDS, then solves it
+            FactorArgs fa{mode, o.parilu_sweeps, inplace ? 1 : 0, stop, mode == FS_PARILU ? d_F_init : nullptr,
+                          nullptr, 0, d_factor_status};
+            return launch_mixed(*h, batch, d_A, d_b, d_x, nullptr, nullptr, isai ? PCK_ISAI : PCK_LU, mp, d_iter,
+                                d_conv, d_iter_res, d_true_res, d_refine, st, Jobs(), 64, &fa);
+        }
         const size_t fbytes = ((size_t)batch * nnz * sizeof(float) + 255) / 256 * 256;
         rc = solve_ws(*h, fbytes + (size_t)batch * sizeof(int), &ws, st, wsrc);
         if (rc != BILU_OK) return rc;
@@ -1914,7 +1932,6 @@ static int solve_ex_impl(bilu_pattern_t h, int batch, const double *d_A, const d
                                        o.parilu_sweeps, stop, d_factor_status, st, isai ? 1 : 0, F, Z, nullptr,
                                        KF_MAX_WAVES, nullptr, 0, inplace ? 1 : 0);
         if (rc != BILU_OK) return rc;
-        MixedParams mp{o.tol, o.tol_mode, (float)o.inner_tol, o.max_iter, o.max_refine, o.inner_tol, o.inner_adapt};
         return launch_mixed(*h, batch, d_A, d_b, d_x, F, Z, isai ? PCK_ISAI : PCK_LU, mp, d_iter, d_conv,
                             d_iter_res, d_true_res, d_refine, st);
     }
@@ -2024,12 +2041,13 @@ int bilu_mlp_solve(bilu_mlp_t m, bilu_pattern_t h, int batch, const double *d_A,
     hipStream_t st = (hipStream_t)stream;
     HIP_TRY(hipSetDevice(h->device));
     const int chunk = std::min(batch, bilu_mlp_forward_rows());
+    const bool merged = o.merge == BILU_MERGED;
     const size_t fbytes = ((size_t)chunk * nnz * sizeof(float) + 255) / 256 * 256;
     void *ws = nullptr;
-    int rc = workspace(*h, fbytes + (size_t)chunk * sizeof(int), &ws, st);
+    int rc = merged ? BILU_OK : workspace(*h, fbytes + (size_t)chunk * sizeof(int), &ws, st);
     if (rc != BILU_OK) return rc;
     float *F = static_cast<float *>(ws);
-    int *Z = reinterpret_cast<int *>(static_cast<char *>(ws) + fbytes);
+    int *Z = merged ? nullptr : reinterpret_cast<int *>(static_cast<char *>(ws) + fbytes);
     MixedParams mp{o.tol, o.tol_mode, (float)o.inner_tol, o.max_iter, o.max_refine, o.inner_tol, o.inner_adapt};
     for (int s0 = 0; s0 < batch; s0 += chunk) {
         const int cb = std::min(chunk, batch - s0);
@@ -2038,6 +2056,17 @@ int bilu_mlp_solve(bilu_mlp_t m, bilu_pattern_t h, int batch, const double *d_A,
         int P3 = 0;
         rc = bilu_mlp_forward_raw(m, h, cb, A, st, &Y, &P3);
         if (rc != BILU_OK) return rc;
+        if (merged) {
+            // one launch: the warm start, sweeps and ISAI in each wave's LDS, then the solve
+            FactorArgs fa{FS_PARILU, o.parilu_sweeps, o.parilu_order == BILU_PARILU_INPLACE ? 1 : 0,
+                          o.parilu_stop_tol, nullptr, Y, P3, d_factor_status ? d_factor_status + s0 : nullptr};
+            rc = launch_mixed(*h, cb, A, d_b + (size_t)s0 * n, d_x + (size_t)s0 * n, nullptr, nullptr,
+                              isai ? PCK_ISAI : PCK_LU, mp, d_iter + s0, d_conv + s0,
+                              d_iter_res ? d_iter_res + s0 : nullptr, d_true_res ? d_true_res + s0 : nullptr,
+                              d_refine ? d_refine + s0 : nullptr, st, Jobs(), 64, &fa);
+            if (rc != BILU_OK) return rc;
+            continue;
+        }
         rc = launch_factor_fast<float>(*h, cb, A, nullptr, F, FS_PARILU, o.parilu_sweeps, o.parilu_stop_tol,
                                        d_factor_status ? d_factor_status + s0 : nullptr, st, isai ? 1 : 0, F, Z,
                                        nullptr, KF_MAX_WAVES, Y, P3, o.parilu_order == BILU_PARILU_INPLACE ? 1 : 0);

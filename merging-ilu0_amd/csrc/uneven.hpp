@@ -11,11 +11,18 @@
 // caller's stream), so the persistent kernels of all patterns share the
 // machine, each taking work from its own counter until its jobs run out.
 // The whole launch sequence is captured into a hipGraph on the first solve
-// and replayed while the arguments stay the same.  The autotuner times the
-// candidate workgroup sizes (waves per workgroup of the factor and of the
-// solve kernel, which trade occupancy against LDS per CU) of each pattern on
-// its own jobs and keeps the fastest; results do not depend on the choice
-// (every system is processed by one wave with the same arithmetic).
+// and replayed while the arguments stay the same.
+//
+// Per pattern the plan runs either the segregated chain (factor kernel, FP32
+// factors / ISAI values through HBM, solve kernel) or the merged solve kernel
+// alone (each wave factors its system in LDS and solves it: the factors never
+// leave the CU, at the cost of the larger per-wave LDS region).  The
+// autotuner times the candidates of each pattern -- segregated with {2, 4, 8}
+// waves per factor workgroup x {4, 8, 12} per solve workgroup, merged with
+// {4, 8, 12} -- first alone on the pattern's requests, then, pattern by
+// pattern from the slowest, inside the whole launch with every other pattern
+// running its current choice, and keeps the fastest.  Results do not depend on
+// the choice (every system is processed by one wave with the same arithmetic).
 
 struct bilu_uneven_s {
     int npat = 0, nreq = 0, device = 0;
@@ -24,7 +31,10 @@ struct bilu_uneven_s {
     int *d_jobs = nullptr;           // per pattern: sysidx[count] then outidx[count]
     std::vector<size_t> job_off;
     bilu_opts o{};
-    std::vector<int> fwaves, swaves; // tuned waves per workgroup (factor, solve)
+    std::vector<int> fwaves, swaves; // tuned waves per workgroup (factor, solve; merged: solve only)
+    std::vector<int> merged;         // per pattern: 1 = the merged kernel
+    struct Cand { int merged, fw, sw; float alone_ms, corun_ms; };
+    std::vector<std::vector<Cand>> table;   // per pattern: the candidates of the last tuning
     std::vector<void *> ws;          // per pattern: FP32 factors / ISAI [count * nnz], then flags [count]
     std::vector<size_t> fbytes;
     std::vector<hipStream_t> streams;
@@ -51,9 +61,9 @@ void free_uneven(bilu_uneven_s *u)
     delete u;
 }
 
-// the factor + solve chain of pattern p on stream st
+// the factor + solve chain of pattern p on stream st (mg: the merged kernel alone)
 int uneven_branch(bilu_uneven_s &u, int p, const double *A, const double *b, double *x, float *it, int *cv,
-                  double *ir, double *tr, int *rf, hipStream_t st, int fw, int sw)
+                  double *ir, double *tr, int *rf, hipStream_t st, int fw, int sw, int mg)
 {
     bilu_pattern_s &h = *u.pats[p];
     const int cnt = u.count[p];
@@ -63,6 +73,15 @@ int uneven_branch(bilu_uneven_s &u, int p, const double *A, const double *b, dou
     Jobs jobs;
     jobs.sysidx = u.d_jobs + u.job_off[p];
     jobs.outidx = jobs.sysidx + cnt;
+    MixedParams mp{u.o.tol, u.o.tol_mode, (float)u.o.inner_tol, u.o.max_iter, u.o.max_refine, u.o.inner_tol,
+                   u.o.inner_adapt};
+    if (mg) {
+        FactorArgs fa{par ? FS_PARILU : FS_ILU, u.o.parilu_sweeps,
+                      par && u.o.parilu_order == BILU_PARILU_INPLACE ? 1 : 0, par ? u.o.parilu_stop_tol : 0.0,
+                      nullptr, nullptr, 0, nullptr};
+        return launch_mixed(h, cnt, A, b, x, nullptr, nullptr, isai ? PCK_ISAI : PCK_LU, mp, it, cv, ir, tr, rf, st,
+                            jobs, sw, &fa);
+    }
     float *F = static_cast<float *>(u.ws[p]);
     int *Z = reinterpret_cast<int *>(static_cast<char *>(u.ws[p]) + u.fbytes[p]);
     int rc = launch_factor_fast<float>(h, cnt, A, nullptr, F, par ? FS_PARILU : FS_ILU, u.o.parilu_sweeps,
@@ -70,8 +89,6 @@ int uneven_branch(bilu_uneven_s &u, int p, const double *A, const double *b, dou
                                        jobs.sysidx, fw, nullptr, 0,
                                        par && u.o.parilu_order == BILU_PARILU_INPLACE ? 1 : 0);
     if (rc != BILU_OK) return rc;
-    MixedParams mp{u.o.tol, u.o.tol_mode, (float)u.o.inner_tol, u.o.max_iter, u.o.max_refine, u.o.inner_tol,
-                   u.o.inner_adapt};
     return launch_mixed(h, cnt, A, b, x, F, Z, isai ? PCK_ISAI : PCK_LU, mp, it, cv, ir, tr, rf, st, jobs, sw);
 }
 
@@ -82,7 +99,8 @@ int uneven_launch(bilu_uneven_s &u, const double *const *A, const double *const 
     HIP_TRY(hipEventRecord(u.events[0], st));
     for (int p = 0; p < u.npat; p++) {
         HIP_TRY(hipStreamWaitEvent(u.streams[p], u.events[0], 0));
-        const int rc = uneven_branch(u, p, A[p], b[p], x[p], it, cv, ir, tr, rf, u.streams[p], u.fwaves[p], u.swaves[p]);
+        const int rc = uneven_branch(u, p, A[p], b[p], x[p], it, cv, ir, tr, rf, u.streams[p], u.fwaves[p],
+                                     u.swaves[p], u.merged[p]);
         if (rc != BILU_OK) return rc;
         HIP_TRY(hipEventRecord(u.events[1 + p], u.streams[p]));
         HIP_TRY(hipStreamWaitEvent(st, u.events[1 + p], 0));
@@ -99,7 +117,11 @@ std::vector<const void *> uneven_args(const bilu_uneven_s &u, const double *cons
     for (const void *q : {(const void *)it, (const void *)cv, (const void *)ir, (const void *)tr, (const void *)rf})
         a.push_back(q);
     (void)st;
-    for (int p = 0; p < u.npat; p++) { a.push_back((const void *)(intptr_t)u.fwaves[p]); a.push_back((const void *)(intptr_t)u.swaves[p]); }
+    for (int p = 0; p < u.npat; p++) {
+        a.push_back((const void *)(intptr_t)u.fwaves[p]);
+        a.push_back((const void *)(intptr_t)u.swaves[p]);
+        a.push_back((const void *)(intptr_t)u.merged[p]);
+    }
     return a;
 }
 
@@ -119,6 +141,7 @@ int bilu_uneven_create(int npat, const bilu_pattern_t *patterns, int nreq, const
           o.precond == BILU_PRECOND_ILU0_ISAI || o.precond == BILU_PRECOND_PARILU0_ISAI) ||
         !(o.inner_tol > 0.0 && o.inner_tol < 1.0) || o.max_refine < 1 || !(o.tol >= 0.0) || o.max_iter < 0 ||
         o.isai_power != 1 || (o.parilu_order != BILU_PARILU_JACOBI && o.parilu_order != BILU_PARILU_INPLACE) ||
+        (o.merge != BILU_MERGED && o.merge != BILU_SEGREGATED && o.merge != BILU_MERGE_AUTO) ||
         !(o.inner_adapt >= 0.0 && o.inner_adapt < 1.0))
         return BILU_ERR_INVALID_ARG;
     for (int p = 0; p < npat; p++) {
@@ -151,6 +174,7 @@ int bilu_uneven_create(int npat, const bilu_pattern_t *patterns, int nreq, const
         flat.insert(flat.end(), outi[p].begin(), outi[p].end());
         u->fwaves.push_back(KF_MAX_WAVES);
         u->swaves.push_back(64);
+        u->merged.push_back(o.merge == BILU_MERGED ? 1 : 0);
     }
     if (hipMalloc((void **)&u->d_jobs, std::max<size_t>(1, flat.size()) * sizeof(int)) != hipSuccess ||
         (!flat.empty() && hipMemcpy(u->d_jobs, flat.data(), flat.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess)) {
@@ -195,29 +219,76 @@ int bilu_uneven_tune(bilu_uneven_t u, const double *const *d_A, const double *co
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
-    const int fcand[] = {2, 4, 8}, scand[] = {4, 8, 12};
+    // candidates of a pattern: segregated (factor waves, solve waves) and merged (solve waves)
+    std::vector<bilu_uneven_s::Cand> cands;
+    if (u->o.merge != BILU_MERGED)
+        for (int fw : {2, 4, 8})
+            for (int sw : {4, 8, 12}) cands.push_back({0, fw, sw, -1.0f, -1.0f});
+    if (u->o.merge != BILU_SEGREGATED)
+        for (int sw : {4, 8, 12}) cands.push_back({1, 0, sw, -1.0f, -1.0f});
+    // one untimed run (first-use costs), then the timed one
+    auto timed = [&](auto &&launch, float &ms) -> int {
+        for (int rep = 0; rep < 2; rep++) {
+            if (rep == 1 && hipEventRecord(e0, st) != hipSuccess) return BILU_ERR_HIP;
+            const int rc = launch();
+            if (rc != BILU_OK) return rc;
+        }
+        if (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+            hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+            return BILU_ERR_HIP;
+        return BILU_OK;
+    };
     int rc = BILU_OK;
+    u->table.assign(u->npat, {});
+    // 1: every pattern alone on its own requests
+    std::vector<std::pair<float, int>> order;
     for (int p = 0; p < u->npat && rc == BILU_OK; p++) {
         if (u->count[p] == 0) continue;
+        auto &tab = u->table[p];
+        tab = cands;
         float best = 1e30f;
-        int bf = u->fwaves[p], bs = u->swaves[p];
-        for (int fw : fcand)
-            for (int sw : scand) {
-                // one untimed run (first-use costs), then the timed one
-                for (int rep = 0; rep < 2 && rc == BILU_OK; rep++) {
-                    if (rep == 1) { if (hipEventRecord(e0, st) != hipSuccess) rc = BILU_ERR_HIP; }
-                    if (rc == BILU_OK)
-                        rc = uneven_branch(*u, p, d_A[p], d_b[p], d_x[p], d_iter, d_conv, d_iter_res, d_true_res,
-                                           d_refine, st, fw, sw);
-                }
-                if (rc != BILU_OK) break;
-                float ms = 0.0f;
-                if (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
-                    hipEventElapsedTime(&ms, e0, e1) != hipSuccess) { rc = BILU_ERR_HIP; break; }
-                if (ms < best) { best = ms; bf = fw; bs = sw; }
+        for (size_t c = 0; c < tab.size() && rc == BILU_OK; c++) {
+            float ms = 0.0f;
+            rc = timed([&]() {
+                return uneven_branch(*u, p, d_A[p], d_b[p], d_x[p], d_iter, d_conv, d_iter_res, d_true_res,
+                                     d_refine, st, tab[c].fw, tab[c].sw, tab[c].merged);
+            }, ms);
+            tab[c].alone_ms = ms;
+            if (rc == BILU_OK && ms < best) {
+                best = ms;
+                u->fwaves[p] = tab[c].fw ? tab[c].fw : u->fwaves[p];
+                u->swaves[p] = tab[c].sw;
+                u->merged[p] = tab[c].merged;
             }
-        u->fwaves[p] = bf;
-        u->swaves[p] = bs;
+        }
+        order.push_back({best, p});
+    }
+    // 2: pattern by pattern, slowest first, inside the whole launch (the other
+    // patterns' branches running concurrently with their current choice)
+    std::sort(order.begin(), order.end(), [](const auto &a, const auto &b) { return a.first > b.first; });
+    for (const auto &op : order) {
+        if (rc != BILU_OK || u->npat < 2) break;
+        const int p = op.second;
+        auto &tab = u->table[p];
+        float best = 1e30f;
+        bilu_uneven_s::Cand keep{u->merged[p], u->fwaves[p], u->swaves[p], 0.0f, 0.0f};
+        for (size_t c = 0; c < tab.size() && rc == BILU_OK; c++) {
+            u->fwaves[p] = tab[c].fw ? tab[c].fw : keep.fw;
+            u->swaves[p] = tab[c].sw;
+            u->merged[p] = tab[c].merged;
+            float ms = 0.0f;
+            rc = timed([&]() {
+                return uneven_launch(*u, d_A, d_b, d_x, d_iter, d_conv, d_iter_res, d_true_res, d_refine, st);
+            }, ms);
+            tab[c].corun_ms = ms;
+            if (rc == BILU_OK && ms < best) {
+                best = ms;
+                keep = {tab[c].merged, u->fwaves[p], tab[c].sw, 0.0f, 0.0f};
+            }
+        }
+        u->fwaves[p] = keep.fw;
+        u->swaves[p] = keep.sw;
+        u->merged[p] = keep.merged;
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
@@ -227,9 +298,32 @@ int bilu_uneven_tune(bilu_uneven_t u, const double *const *d_A, const double *co
 int bilu_uneven_config(bilu_uneven_t u, int p, int *factor_waves, int *solve_waves, int *jobs)
 {
     if (!u || p < 0 || p >= u->npat) return BILU_ERR_INVALID_ARG;
-    if (factor_waves) *factor_waves = u->fwaves[p];
+    if (factor_waves) *factor_waves = u->merged[p] ? 0 : u->fwaves[p];
     if (solve_waves) *solve_waves = u->swaves[p];
     if (jobs) *jobs = u->count[p];
+    return BILU_OK;
+}
+
+int bilu_uneven_merge_point(bilu_uneven_t u, int p, int *merged)
+{
+    if (!u || p < 0 || p >= u->npat || !merged) return BILU_ERR_INVALID_ARG;
+    *merged = u->merged[p];
+    return BILU_OK;
+}
+
+int bilu_uneven_tune_table(bilu_uneven_t u, int p, int cap, int *merged, int *factor_waves, int *solve_waves,
+                           float *alone_ms, float *corun_ms, int *count)
+{
+    if (!u || p < 0 || p >= u->npat || !count || cap < 0) return BILU_ERR_INVALID_ARG;
+    const auto &tab = p < (int)u->table.size() ? u->table[p] : std::vector<bilu_uneven_s::Cand>();
+    *count = (int)tab.size();
+    for (int c = 0; c < (int)tab.size() && c < cap; c++) {
+        if (merged) merged[c] = tab[c].merged;
+        if (factor_waves) factor_waves[c] = tab[c].fw;
+        if (solve_waves) solve_waves[c] = tab[c].sw;
+        if (alone_ms) alone_ms[c] = tab[c].alone_ms;
+        if (corun_ms) corun_ms[c] = tab[c].corun_ms;
+    }
     return BILU_OK;
 }
 

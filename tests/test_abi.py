@@ -38,7 +38,7 @@ def test_host_only_entry_points(lib):
     o = bilu.Opts()
     lib.bilu_opts_default(ctypes.byref(o))
     assert o.precond == bilu.ILU0 and o.max_iter == 500 and o.tol == 1e-11
-    assert o.tol_mode == bilu.TOL_ABSOLUTE and o.merge == bilu.MERGED and o.parilu_sweeps == 5
+    assert o.tol_mode == bilu.TOL_ABSOLUTE and o.merge == bilu.MERGE_AUTO and o.parilu_sweeps == 5
     assert lib.bilu_version().decode().startswith("bilu")
     assert lib.bilu_status_string(3).decode() == "missing diagonal"
     # invalid arguments are rejected before any device work

@@ -258,3 +258,61 @@ def test_fp64_isai_split_bitwise(inplace):
     o = run_oracle(pat, A, b, **kw)
     assert_same(g, o, f"fp64 isai split inplace={inplace}")
     assert (o[2] == 1).all() and (o[6] > 0).all()
+
+
+MERGE_CASES = [
+    ("ilu0", dict(precond=bilu.ILU0)),
+    ("ilu0_isai", dict(precond=bilu.ILU0_ISAI)),
+    ("parilu0_jacobi_5", dict(precond=bilu.PARILU0, parilu_sweeps=5)),
+    ("parilu0_inplace_stop", dict(precond=bilu.PARILU0, parilu_sweeps=40, parilu_stop_tol=1e-8,
+                                  parilu_order=bilu.PARILU_INPLACE)),
+    ("parilu0_isai_jacobi_stop", dict(precond=bilu.PARILU0_ISAI, parilu_sweeps=40, parilu_stop_tol=1e-8)),
+    ("parilu0_isai_inplace_stop", dict(precond=bilu.PARILU0_ISAI, parilu_sweeps=40, parilu_stop_tol=1e-8,
+                                       parilu_order=bilu.PARILU_INPLACE)),
+]
+
+
+@pytest.mark.parametrize("kind,B,n", [("random", 192, 144), ("stencil5", 64, 100), ("random", 64, 32),
+                                      ("random", 48, 300)])
+@pytest.mark.parametrize("name,kw", MERGE_CASES, ids=[c[0] for c in MERGE_CASES])
+def test_mixed_merged_bitwise(kind, B, n, name, kw):
+    """the merged kernel (each system factored in the wave's LDS, then solved)
+    against the oracle and the segregated launches: every output bit for bit"""
+    print(f"case mixed merged {kind} B={B} n={n} {name}", flush=True)
+    dens = {} if kind != "random" or n == 144 else dict(density=13.0 / (n - 1))
+    pat, A, b = synth.batch(kind, B, n=n, **dens)
+    base = dict(kw, tol=1e-10, tol_mode=bilu.TOL_RELATIVE, precision=bilu.PRECISION_FP32_IR)
+    P = bilu.Pattern.from_synth(pat)
+    dA, db = to_dev(A), to_dev(b)
+    res = {}
+    for merge in (bilu.MERGED, bilu.SEGREGATED):
+        x = torch.empty_like(db)
+        fs = torch.full((B,), -7, dtype=torch.int32, device=dev())
+        out = bilu.solve_ex(P, dA, db, x, bilu.opts(**base, merge=merge), factor_status=fs)
+        torch.cuda.synchronize()
+        res[merge] = (x.cpu().numpy(),) + tuple(t.cpu().numpy() for t in out)
+    assert_same(res[bilu.MERGED], res[bilu.SEGREGATED], f"merged vs segregated {name}")
+    assert_same(res[bilu.MERGED], run_oracle(pat, A, b, **base), f"merged vs oracle {name}")
+    assert (res[bilu.MERGED][2] == 1).all()
+
+
+def test_mixed_merged_warm_and_zero_pivot():
+    """merged: the caller's ParILU guess (F_init) and a zero pivot, as in the segregated launches"""
+    print("case mixed merged warm start + zero pivot", flush=True)
+    pat, A, b = synth.batch("random", 96, n=144)
+    F = np.stack([O.ilu0(pat, A[s])[0] for s in range(A.shape[0])])
+    init = F * (1.0 + 1e-3 * np.random.default_rng(11).standard_normal(F.shape))
+    for pc in (bilu.PARILU0, bilu.PARILU0_ISAI):
+        kw = dict(precond=pc, parilu_sweeps=40, parilu_stop_tol=1e-8, tol=1e-10, tol_mode=bilu.TOL_RELATIVE,
+                  precision=bilu.PRECISION_FP32_IR)
+        g = run_gpu(pat, A, b, F_init=init, merge=bilu.MERGED, **kw)
+        assert_same(g, run_gpu(pat, A, b, F_init=init, merge=bilu.SEGREGATED, **kw), f"merged warm {pc}")
+        assert_same(g, run_oracle(pat, A, b, init=init, **kw), f"merged warm vs oracle {pc}")
+    A = A.copy()
+    A[5][pat.diag[3]] = 0.0
+    for pc in (bilu.ILU0, bilu.ILU0_ISAI):
+        kw = dict(precond=pc, tol=1e-10, tol_mode=bilu.TOL_RELATIVE, precision=bilu.PRECISION_FP32_IR)
+        g = run_gpu(pat, A, b, merge=bilu.MERGED, **kw)
+        assert_same(g, run_gpu(pat, A, b, merge=bilu.SEGREGATED, **kw), f"merged zero pivot {pc}")
+        assert_same(g, run_oracle(pat, A, b, **kw), f"merged zero pivot vs oracle {pc}")
+        assert g[2][5] == bilu.ZERO_PIVOT and (np.delete(g[2], 5) == 1).all()

@@ -208,3 +208,32 @@ def test_mlp_solve_rejects():
     with pytest.raises(bilu.BiluError):
         net.solve(P2, to_dev(A2), to_dev(synth.rhs(pat2.n, np.arange(4))), torch.empty(4, pat2.n, dtype=torch.float64,
                   device=dev()), bilu.opts(precond=bilu.PARILU0, precision=bilu.PRECISION_FP32_IR))
+
+
+@pytest.mark.parametrize("precond", ["PARILU0", "PARILU0_ISAI"])
+@pytest.mark.parametrize("order", ["PARILU_JACOBI", "PARILU_INPLACE"])
+def test_mlp_solve_merged_matches_segregated(precond, order):
+    """bilu_mlp_solve with the merged kernel (warm start, sweeps, ISAI and the
+    solve in one launch) equals the segregated launches bit for bit"""
+    print(f"case mlp merged solve {precond} {order}", flush=True)
+    pat, A, b = synth.batch("random", 640, first=5 << 20, n=144)
+    P = bilu.Pattern.from_synth(pat)
+    dA, db = to_dev(A), to_dev(b)
+    net = bilu.Mlp(P, hidden=256, seed=9)
+    dF = exact_factors(P, dA)
+    for step in range(10):
+        net.train_step(dA[:512], dF[:512], 3e-4)
+    res = {}
+    for merge in (bilu.MERGED, bilu.SEGREGATED):
+        o = bilu.opts(precond=getattr(bilu, precond), parilu_sweeps=40, parilu_stop_tol=1e-8, tol=1e-10,
+                      tol_mode=bilu.TOL_RELATIVE, precision=bilu.PRECISION_FP32_IR,
+                      parilu_order=getattr(bilu, order), merge=merge)
+        x = torch.empty_like(db)
+        r = net.solve(P, dA, db, x, o)
+        torch.cuda.synchronize()
+        res[merge] = (x,) + tuple(r)
+    for name, u, v in zip(("x", "iters", "conv", "iter_res", "true_res", "refine", "factor_status"),
+                          res[bilu.MERGED], res[bilu.SEGREGATED]):
+        assert torch.equal(u, v), name
+    assert (res[bilu.MERGED][2].cpu().numpy() == 1).all()
+    net.close()

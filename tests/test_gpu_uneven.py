@@ -68,11 +68,17 @@ def test_uneven_matches_per_pattern_and_tuning(precond):
         for name, u, v in zip(["iters", "conv", "iter_res", "true_res", "refine"], first, res):
             assert np.array_equal(u[rows], v[req_i[rows]]), f"pattern {p}: {name} differs"
     assert (first[1] == 1).all()
-    # the autotuner picks workgroup sizes; the results stay bit for bit
+    # the autotuner picks the merge point and the workgroup sizes (alone, then
+    # inside the whole launch); the results stay bit for bit
     plan.tune(dA, db, dx, out)
     for p in range(len(P)):
         fw, sw, jobs = plan.config(p)
-        assert fw in (2, 4, 8) and sw in (4, 8, 12) and jobs == B_each
+        assert (fw == 0) == plan.merged(p) and sw in (4, 8, 12) and jobs == B_each
+        assert fw in (0, 2, 4, 8)
+        tab = plan.tune_table(p)
+        assert len(tab) == 12 and sum(c["merged"] for c in tab) == 3
+        assert all(c["alone_ms"] > 0 and c["corun_ms"] > 0 for c in tab)
+        print(f"pattern n={NS[p]}: merged={plan.merged(p)} factor_waves={fw} solve_waves={sw}", flush=True)
     for x in dx:
         x.zero_()
     plan.solve(dA, db, dx, out)
@@ -81,6 +87,34 @@ def test_uneven_matches_per_pattern_and_tuning(precond):
         assert np.array_equal(dx[p].cpu().numpy(), xs[p])
     for u, v in zip(first, out):
         assert np.array_equal(u, v.cpu().numpy())
+    plan.close()
+
+
+@pytest.mark.parametrize("precond", [bilu.ILU0, bilu.ILU0_ISAI])
+def test_uneven_merged_plan(precond):
+    """a plan fixed on the merged kernel (opts.merge): bitwise equal to the
+    per-pattern segregated solves; the tuner keeps the merge point fixed"""
+    print(f"case uneven merged precond={precond}", flush=True)
+    pats, As, bs, req_p, req_i = c3(256, seed=9)
+    P = [bilu.Pattern.from_synth(p) for p in pats]
+    dA = [torch.from_numpy(A).to(dev()) for A in As]
+    db = [torch.from_numpy(b).to(dev()) for b in bs]
+    dx = [torch.empty_like(b) for b in db]
+    plan = bilu.Uneven(P, req_p, req_i, bilu.opts(precond=precond, merge=bilu.MERGED, **OPTS))
+    out = plan.outputs()
+    for tuned in (False, True):
+        if tuned:
+            plan.tune(dA, db, dx, out)
+        plan.solve(dA, db, dx, out)
+        torch.cuda.synchronize()
+        for p in range(len(P)):
+            assert plan.merged(p)
+            xr, res = per_pattern(P[p], dA[p], db[p], precond=precond, merge=bilu.SEGREGATED, **OPTS)
+            assert np.array_equal(dx[p].cpu().numpy(), xr), f"pattern {p}: x differs"
+            rows = np.nonzero(req_p == p)[0]
+            for name, u, v in zip(["iters", "conv", "iter_res", "true_res", "refine"], out, res):
+                assert np.array_equal(u.cpu().numpy()[rows], v[req_i[rows]]), f"pattern {p}: {name} differs"
+    assert len(plan.tune_table(0)) == 3
     plan.close()
 
 
