@@ -301,7 +301,7 @@ int bilu_uneven_destroy(bilu_uneven_t u);
  * kernel, FP32 factors / ISAI values through HBM, solve kernel; opts.merge
  * BILU_MERGED / BILU_SEGREGATED fixes it, BILU_MERGE_AUTO lets the tuner
  * choose) and the workgroup sizes (waves per factor workgroup {2, 4, 8}, per
- * solve workgroup {4, 8, 12}) -- first alone on the pattern's requests, then
+ * solve workgroup {4, 8, 12}, for either kernel) -- first alone on the pattern's requests, then
  * pattern by pattern (slowest first) inside the whole launch with the other
  * patterns running, and keeps the fastest (results do not depend on the
  * choice) */

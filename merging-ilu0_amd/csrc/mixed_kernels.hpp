@@ -456,7 +456,13 @@ constexpr int PCK_LU = 0, PCK_ISAI = 1;
 // waves per workgroup (launch bound) of a k_mixed instantiation: 72 slots
 // (patterns up to ~4600 nnz) need the register budget of 8 waves (LU: A in
 // VGPRs; ISAI: 2 x 40 triangle coefficients)
-constexpr int km_waves(int SM, int PCK) { return SM > 48 ? 8 : (PCK == PCK_LU ? KM_MAX_WAVES : KMI_MAX_WAVES); }
+constexpr int km_waves(int SM, int PCK, int FM = 0)
+{
+    // the merged kernel holds the factorization's and the solve's registers:
+    // the budget of 8 waves beyond 24 slots (its LDS region seldom fits more;
+    // small patterns keep 12, measured faster at n = 32)
+    return (SM > 48 || (FM && SM > 24)) ? 8 : (PCK == PCK_LU ? KM_MAX_WAVES : KMI_MAX_WAVES);
+}
 
 // FM = 0 (segregated): the FP32 factors / ISAI values come from gF (a
 // factor launch before).  FM = 1 (merged): each system is factored in the
@@ -465,7 +471,7 @@ constexpr int km_waves(int SM, int PCK) { return SM > 48 ? 8 : (PCK == PCK_LU ? 
 // the two layouts, the solve's overlaying the factorization's once its
 // values are in registers.  Same arithmetic, same results bit for bit.
 template <int NR, int SM, int PCK, int SLU = 1, int FM = 0>
-__global__ __launch_bounds__(km_waves(SM, PCK) * WAVE) void k_mixed(
+__global__ __launch_bounds__(km_waves(SM, PCK, FM) * WAVE) void k_mixed(
     DevPattern P, FastParams fp, int batch, const double *__restrict__ gA,
     const double *__restrict__ gb, double *__restrict__ gx, const float *__restrict__ gF,
     const int *__restrict__ gZ, MixedParams mp, float *__restrict__ g_iter, int *__restrict__ g_conv,

@@ -1463,7 +1463,7 @@ int launch_mixed(bilu_pattern_s &h, int batch, const double *A, const double *b,
     // shared LDS: A's slot records (ISAI: then the two triangle layouts' records)
     const size_t rec = (size_t)(sm + (pck == PCK_ISAI ? 2 * slu : 0)) * WAVE * 4;
     if (rec >= h.max_lds) return BILU_ERR_TOO_LARGE;
-    const int wcap = std::min(max_waves, km_waves(sm, pck));
+    const int wcap = std::min(max_waves, km_waves(sm, pck, fa ? 1 : 0));
     fp.shared_bytes = int(rec);
     int waves = (int)std::min<size_t>(wcap, (h.max_lds - rec) / fp.wave_bytes);
     if (waves < 1) return BILU_ERR_TOO_LARGE;

@@ -19,7 +19,7 @@
 // leave the CU, at the cost of the larger per-wave LDS region).  The
 // autotuner times the candidates of each pattern -- segregated with {2, 4, 8}
 // waves per factor workgroup x {4, 8, 12} per solve workgroup, merged with
-// {4, 8, 12} -- first alone on the pattern's requests, then, pattern by
+// {4, 8, 12} (capped by the kernel's register budget) -- first alone on the pattern's requests, then, pattern by
 // pattern from the slowest, inside the whole launch with every other pattern
 // running its current choice, and keeps the fastest.  Results do not depend on
 // the choice (every system is processed by one wave with the same arithmetic).
