@@ -140,7 +140,7 @@ typedef struct {
     int ilu_eras;              /* eras of the right-looking pivot ILU(0) schedule */
 } bilu_pattern_info;
 
-/* defaults: ILU(0), 5 sweeps, tol 1e-11 absolute, 500 iterations, merged, no restart,
+/* defaults: ILU(0), 5 sweeps, tol 1e-11 absolute, 500 iterations, merge auto, no restart,
  * FP64, inner_tol 1e-5, 10 refinement steps, fixed ParILU sweeps */
 void bilu_opts_default(bilu_opts *o);
 const char *bilu_version(void);
@@ -148,7 +148,12 @@ const char *bilu_status_string(int status);
 
 /* Host arrays; columns sorted within each row.  Analyses the pattern once
  * (diagonal pointers, level schedules, ILU update lists, ParILU dependency
- * lists, ISAI gather maps) and uploads it to the current device. */
+ * lists, ISAI gather maps) and uploads it to the current device.  A handle's
+ * launches share its work counters and its workspace: calls on one handle
+ * must be ordered (one stream, or the caller's events); two solves on one
+ * handle running concurrently on different streams would take systems from
+ * each other's counter.  Different handles are independent (an uneven plan
+ * takes each of its patterns' handles for its own branches). */
 int bilu_pattern_create(int n, int nnz, const int *row_ptrs, const int *col_idx,
                         bilu_pattern_t *out);
 int bilu_pattern_destroy(bilu_pattern_t p);

@@ -421,15 +421,47 @@ void ParILU_0_Factorization_Gpu(const PagedCSRMatrices &A_pages, PagedCSRMatrice
     split_into(P, A, F.p, L_pages, U_pages);
 }
 
+// true unless the matrix is lower triangular with a stored diagonal entry
+// other than 1 on some page (a matrix with entries above the diagonal keeps
+// the LOWER view: its strictly lower part and a unit diagonal)
+static bool unit_lower(const PagedCSRMatrices &A)
+{
+    const int n = A.GetNumRows(), nnz = A.GetNumNz(), pages = A.GetNumPages();
+    std::vector<int> rp(n + 1), ci(nnz);
+    hip_check(hipMemcpy(rp.data(), A.GetPtrToGpuRowPtrs(), sizeof(int) * (n + 1), hipMemcpyDeviceToHost), "D2H");
+    hip_check(hipMemcpy(ci.data(), A.GetPtrToGpuColInd(), sizeof(int) * nnz, hipMemcpyDeviceToHost), "D2H");
+    std::vector<int> dpos;
+    for (int i = 0; i < n; i++)
+        for (int p = rp[i]; p < rp[i + 1]; p++) {
+            if (ci[p] > i) return true;
+            if (ci[p] == i) dpos.push_back(p);
+        }
+    std::vector<double> v((size_t)pages * nnz);
+    hip_check(hipMemcpy(v.data(), A.GetPtrToGpuValues(), sizeof(double) * v.size(), hipMemcpyDeviceToHost), "D2H");
+    for (int pg = 0; pg < pages; pg++)
+        for (int p : dpos)
+            if (v[(size_t)pg * nnz + p] != 1.0) return false;
+    return true;
+}
+
 void GenerateISAI_gpu(PagedCSRMatrices &aiA_pages, const PagedCSRMatrices &A_pages, const mat_type A_type,
                       const int power)
 {
     if (power < 1) throw std::invalid_argument("GenerateISAI_gpu: power must be >= 1");
     if (!A_pages.ExistsGPU()) const_cast<PagedCSRMatrices &>(A_pages).CopyFromCpuToGpu();
     const PatternHandle &P = cached_pattern(A_pages);
-    const int type = A_type == mat_type::lower_tri ? BILU_ISAI_LOWER
-                   : A_type == mat_type::upper_tri ? BILU_ISAI_UPPER : BILU_ISAI_GENERAL;
+    int type = A_type == mat_type::lower_tri ? BILU_ISAI_LOWER
+             : A_type == mat_type::upper_tri ? BILU_ISAI_UPPER : BILU_ISAI_GENERAL;
     const int n = A_pages.GetNumRows(), nnz = A_pages.GetNumNz();
+    if (type == BILU_ISAI_LOWER && !unit_lower(A_pages)) {
+        // a lower-triangular input whose stored diagonal is not all ones: the
+        // reference's lower solve divides by it (isai_gpu.cu:359-389); the
+        // LOWER view assumes the factor L's unit diagonal, so take the ISAI of
+        // the input on its own (lower) pattern, which divides by it as well
+        // (the same systems, eliminated without pivoting: their transposes are
+        // upper triangular)
+        type = BILU_ISAI_GENERAL;
+    }
     if (power > 1) {
         int mnnz = 0;
         check(bilu_isai_pattern(P.h, type, power, &mnnz, nullptr, nullptr), "bilu_isai_pattern");

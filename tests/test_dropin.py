@@ -173,3 +173,30 @@ def test_driver_isai_matches_oracle(tmp_path, variant, typ):
                 mask[list(rng)] = True
             assert np.array_equal(M[s][mask], want[mask])
             assert np.all(M[s][~mask] == 0.0)
+
+
+@pytest.mark.gpu
+def test_driver_isai_lower_nonunit_diagonal(tmp_path):
+    """GenerateISAI_gpu(lower_tri) of a lower-triangular matrix whose stored
+    diagonal is not 1 divides by it, as the reference's lower solve does
+    (isai_gpu.cu:359-389): the oracle's ISAI of the matrix on its own pattern,
+    bit for bit, and (M L)(i, j) = delta_ij on the pattern"""
+    import oracle as O
+    print("case drop-in isai lower, non-unit diagonal", flush=True)
+    full, A, b = synth.batch("random", 16, n=90)
+    keep = full.col_idx <= full.rows
+    rp = np.concatenate([[0], np.cumsum(np.bincount(full.rows[keep], minlength=full.n))]).astype(np.int32)
+    lpat = synth.Pattern(full.n, rp, full.col_idx[keep])
+    L = np.ascontiguousarray(A[:, keep])
+    L[:, lpat.diag] *= np.linspace(0.5, 3.0, L.shape[0])[:, None]     # stored diagonal far from 1
+    fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
+    write_problem(fin, lpat, L, b)
+    r = subprocess.run([DRIVER, "isai_lower", str(fin), str(fout)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    M = np.fromfile(fout, dtype=np.float64).reshape(16, lpat.nnz)
+    for s in range(16):
+        want, st = O.isai(lpat, L[s], 0)
+        assert st == 0 and np.array_equal(M[s], want)
+        P = lpat.dense(M[s]) @ lpat.dense(L[s])
+        I = np.eye(lpat.n)
+        assert np.abs((P - I)[lpat.rows, lpat.col_idx]).max() < 1e-10
