@@ -492,6 +492,19 @@ def main():
     torch.cuda.synchronize()
     train_s = time.time() - t1
     del tr_A, tr_F
+    # how accurate a warm start must be to save sweeps: the exact ILU(0) factors of
+    # the held-out systems with relative noise eps on every entry
+    dFh = torch.empty_like(dAh)
+    bilu.ilu0(P, dAh, dFh)
+    gen = torch.Generator(device=dev).manual_seed(1234)
+    noise = torch.randn(dFh.shape, generator=gen, device=dev, dtype=torch.float64)
+    fsh = torch.empty(dAh.shape[0], dtype=torch.int32, device=dev)
+    accuracy = {}
+    for eps in (1e-2, 1e-3, 1e-4, 1e-5, 1e-6):
+        bilu.solve_ex(P, dAh, dbh, xh, o_learn, F_init=dFh * (1.0 + eps * noise), factor_status=fsh)
+        torch.cuda.synchronize()
+        accuracy[f"{eps:g}"] = round(float(fsh.float().mean().item()), 4)
+    del dFh, noise
     guess = torch.empty_like(dA)
     # (3a) split: bilu_mlp_infer writes the FP64 guess, bilu_solve_ex reads it
     ms_split = run(o_learn, F_init=guess, pre=lambda: net.infer(dA, guess, stream=stream))
@@ -577,7 +590,8 @@ def main():
                      "schedule": "Adam, lr 3e-4 cosine to 0", "label_generation_s": round(gen_train_s, 2),
                      "train_s": round(train_s, 2),
                      "held_out": {"systems": int(dAh.shape[0]), "sweeps_from_A": held_from_A,
-                                  "sweeps_untrained_base_guess": held_base, "curve": curve}},
+                                  "sweeps_untrained_base_guess": held_base, "curve": curve,
+                                  "sweeps_from_exact_factors_with_relative_noise": accuracy}},
         "all_converged_relres_le_1e-10": learned["all_converged_relres_le_1e-10"],
         "roofline": roofline(dom, kms_learn if dom == "k_mixed" else kms_factor,
                              f"{dom} learned n{n} b{B}", ms_learn),

@@ -306,7 +306,9 @@ int bilu_uneven_destroy(bilu_uneven_t u);
  * kernel, FP32 factors / ISAI values through HBM, solve kernel; opts.merge
  * BILU_MERGED / BILU_SEGREGATED fixes it, BILU_MERGE_AUTO lets the tuner
  * choose) and the workgroup sizes (waves per factor workgroup {2, 4, 8}, per
- * solve workgroup {4, 8, 12}, for either kernel) -- first alone on the pattern's requests, then
+ * solve workgroup {4, 8, 12}, for either kernel; for the ISAI templates of 48 and 72 slots per
+ * lane also the placement of the solve kernel's slot records: the workgroup's LDS or global
+ * memory) -- first alone on the pattern's requests, then
  * pattern by pattern (slowest first) inside the whole launch with the other
  * patterns running, and keeps the fastest (results do not depend on the
  * choice) */
@@ -318,11 +320,18 @@ int bilu_uneven_tune(bilu_uneven_t u, const double *const *d_A, const double *co
 int bilu_uneven_config(bilu_uneven_t u, int p, int *factor_waves, int *solve_waves, int *jobs);
 /* 1 when pattern p runs the merged kernel */
 int bilu_uneven_merge_point(bilu_uneven_t u, int p, int *merged);
+/* pins pattern p's configuration instead of the tuner's choice (merged 0/1,
+ * a merge point the plan's opts.merge allows; waves per workgroup, capped by
+ * what the kernel and the LDS hold; records_global 1 takes effect for the ISAI
+ * templates of 48 and 72 slots).  Results do not depend on it. */
+int bilu_uneven_configure(bilu_uneven_t u, int p, int merged, int factor_waves, int solve_waves, int records_global);
+/* 1 when pattern p's solve kernel reads its slot records from global memory */
+int bilu_uneven_placement(bilu_uneven_t u, int p, int *records_global);
 /* the candidates the last bilu_uneven_tune timed for pattern p (up to cap of
  * them; *count = how many): merge point, waves per factor / solve workgroup,
- * ms alone and inside the whole launch (-1: not timed there) */
+ * record placement, ms alone and inside the whole launch (-1: not timed there) */
 int bilu_uneven_tune_table(bilu_uneven_t u, int p, int cap, int *merged, int *factor_waves, int *solve_waves,
-                           float *alone_ms, float *corun_ms, int *count);
+                           int *records_global, float *alone_ms, float *corun_ms, int *count);
 int bilu_uneven_solve(bilu_uneven_t u, const double *const *d_A, const double *const *d_b, double *const *d_x,
                       float *d_iter, int *d_conv, double *d_iter_res, double *d_true_res, int *d_refine,
                       void *stream);

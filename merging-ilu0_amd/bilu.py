@@ -36,7 +36,7 @@ EXPORTS = [
     "bilu_true_residual", "bilu_mlp_create", "bilu_mlp_destroy", "bilu_mlp_num_params",
     "bilu_mlp_get_weights", "bilu_mlp_set_weights", "bilu_mlp_gradient", "bilu_mlp_train_step",
     "bilu_mlp_infer", "bilu_uneven_create", "bilu_uneven_destroy", "bilu_uneven_tune", "bilu_uneven_config",
-    "bilu_uneven_merge_point", "bilu_uneven_tune_table",
+    "bilu_uneven_merge_point", "bilu_uneven_placement", "bilu_uneven_tune_table", "bilu_uneven_configure",
     "bilu_uneven_solve", "bilu_pattern_add_diagonal", "bilu_expand_values", "bilu_isai_pattern",
     "bilu_isai_power", "bilu_mlp_solve", "bilu_ilu0_ex", "bilu_ilu0_tune", "bilu_pattern_reserve",
     "bilu_solve_ex_noalloc", "bilu_timing_begin", "bilu_timing_end",
@@ -117,7 +117,9 @@ def lib():
         L.bilu_uneven_tune.argtypes = [vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp), vp, vp, vp, vp, vp, vp]
         L.bilu_uneven_config.argtypes = [vp, C.c_int, i32p, i32p, i32p]
         L.bilu_uneven_merge_point.argtypes = [vp, C.c_int, i32p]
-        L.bilu_uneven_tune_table.argtypes = [vp, C.c_int, C.c_int, i32p, i32p, i32p, C.POINTER(C.c_float),
+        L.bilu_uneven_placement.argtypes = [vp, C.c_int, i32p]
+        L.bilu_uneven_configure.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.bilu_uneven_tune_table.argtypes = [vp, C.c_int, C.c_int, i32p, i32p, i32p, i32p, C.POINTER(C.c_float),
                                              C.POINTER(C.c_float), i32p]
         L.bilu_uneven_solve.argtypes = [vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp), vp, vp, vp, vp, vp, vp]
         L.bilu_mlp_create.argtypes = [vp, C.c_int, C.c_ulonglong, C.POINTER(vp)]
@@ -511,17 +513,27 @@ class Uneven:
         _check(lib().bilu_uneven_merge_point(self.h, p, C.byref(m)), "bilu_uneven_merge_point")
         return bool(m.value)
 
+    def configure(self, p, merged, factor_waves, solve_waves, records_global=False):
+        _check(lib().bilu_uneven_configure(self.h, p, int(merged), factor_waves, solve_waves, int(records_global)),
+               "bilu_uneven_configure")
+
+    def records_global(self, p):
+        g = C.c_int()
+        _check(lib().bilu_uneven_placement(self.h, p, C.byref(g)), "bilu_uneven_placement")
+        return bool(g.value)
+
     def tune_table(self, p):
         """the candidates the last tune() timed for pattern p: dicts of merged,
-        factor_waves, solve_waves, alone_ms, corun_ms"""
+        factor_waves, solve_waves, records_global, alone_ms, corun_ms"""
         cnt = C.c_int()
-        _check(lib().bilu_uneven_tune_table(self.h, p, 0, None, None, None, None, None, C.byref(cnt)),
+        _check(lib().bilu_uneven_tune_table(self.h, p, 0, None, None, None, None, None, None, C.byref(cnt)),
                "bilu_uneven_tune_table")
         k = cnt.value
-        mg, fw, sw = (C.c_int * max(k, 1))(), (C.c_int * max(k, 1))(), (C.c_int * max(k, 1))()
+        mg, fw, sw, rg = ((C.c_int * max(k, 1))() for _ in range(4))
         al, co = (C.c_float * max(k, 1))(), (C.c_float * max(k, 1))()
-        _check(lib().bilu_uneven_tune_table(self.h, p, k, mg, fw, sw, al, co, C.byref(cnt)), "bilu_uneven_tune_table")
-        return [{"merged": bool(mg[i]), "factor_waves": fw[i], "solve_waves": sw[i],
+        _check(lib().bilu_uneven_tune_table(self.h, p, k, mg, fw, sw, rg, al, co, C.byref(cnt)),
+               "bilu_uneven_tune_table")
+        return [{"merged": bool(mg[i]), "factor_waves": fw[i], "solve_waves": sw[i], "records_global": bool(rg[i]),
                  "alone_ms": round(al[i], 4), "corun_ms": round(co[i], 4)} for i in range(k)]
 
     def close(self):

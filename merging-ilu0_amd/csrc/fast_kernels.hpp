@@ -59,6 +59,9 @@ struct FastParams {
     // Jacobi ParILU(0) sweep rounds: products [step][64], finishes [round][64]
     const uint32_t *parp, *parf;
     int par_rounds, par_psteps;
+    // the FP32 mixed ISAI kernel's slot records in the layout of its LDS copy
+    // (null unless the pattern's template has 48 or 72 slots)
+    const uint32_t *rec32;
 };
 
 __device__ __forceinline__ void wave_sync()

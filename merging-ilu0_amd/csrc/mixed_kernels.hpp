@@ -470,7 +470,9 @@ constexpr int km_waves(int SM, int PCK, int FM = 0)
 // solve reads the factors / ISAI values there; the region is the larger of
 // the two layouts, the solve's overlaying the factorization's once its
 // values are in registers.  Same arithmetic, same results bit for bit.
-template <int NR, int SM, int PCK, int SLU = 1, int FM = 0>
+// RG = 1: the slot records are read from global memory (fp.rec32, the same
+// layout as the shared LDS copy) instead, freeing that LDS for more waves.
+template <int NR, int SM, int PCK, int SLU = 1, int FM = 0, int RG = 0>
 __global__ __launch_bounds__(km_waves(SM, PCK, FM) * WAVE) void k_mixed(
     DevPattern P, FastParams fp, int batch, const double *__restrict__ gA,
     const double *__restrict__ gb, double *__restrict__ gx, const float *__restrict__ gF,
@@ -483,7 +485,8 @@ __global__ __launch_bounds__(km_waves(SM, PCK, FM) * WAVE) void k_mixed(
     extern __shared__ __align__(16) unsigned char smem[];
     const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
     const int n = P.n, nnz = P.nnz, S = fp.S;
-    uint32_t *meta = reinterpret_cast<uint32_t *>(smem);
+    uint32_t *meta_l = reinterpret_cast<uint32_t *>(smem);
+    const uint32_t *meta = RG ? fp.rec32 : meta_l;
     const uint32_t *trL = fp.trL;
     const uint32_t *trU = fp.trU;
     // LU: the records of spmv_slots.  ISAI: the packed records of spmv32
@@ -491,24 +494,25 @@ __global__ __launch_bounds__(km_waves(SM, PCK, FM) * WAVE) void k_mixed(
     // padding slots read x[0] with coefficient 0.  ISAI: then the records of
     // the ISAI(L) and ISAI(U) slot layouts (their not-a-row-end bit 31
     // dropped: the output offset tells), padded to SLU slots each.
-    uint32_t *lrec = meta + SM * WAVE, *urec = lrec + SLU * WAVE;
-    for (int i = threadIdx.x; i < SM * WAVE; i += blockDim.x) {
+    const uint32_t *lrec = meta + SM * WAVE, *urec = lrec + SLU * WAVE;
+    uint32_t *lrec_l = meta_l + SM * WAVE, *urec_l = lrec_l + SLU * WAVE;
+    for (int i = threadIdx.x; i < (RG ? 0 : SM * WAVE); i += blockDim.x) {
         const uint32_t m = (i < S * WAVE) ? fp.slot_meta[i] : 0xFFFF0000u;
         if constexpr (PCK == PCK_LU) {
-            meta[i] = m;
+            meta_l[i] = m;
         } else {
             const bool end = (m >> 16) != 0xFFFFu;
             const uint32_t yo = end ? (m >> 16) >> 1 : (uint32_t)(n + 1) * 4u;
-            meta[i] = ((m & 0xFFFFu) >> 1) | (yo << 16);
+            meta_l[i] = ((m & 0xFFFFu) >> 1) | (yo << 16);
         }
     }
     // FP32 records: an output offset below that of the scratch cell n + 1 ends a row
     const uint32_t thr = (uint32_t)(n + 1) * 4u << 16;
-    if constexpr (PCK == PCK_ISAI) {
+    if constexpr (PCK == PCK_ISAI && !RG) {
         const uint32_t pad = (uint32_t)n * 4u | thr;
         for (int i = threadIdx.x; i < SLU * WAVE; i += blockDim.x) {
-            lrec[i] = (i < fp.SL * WAVE) ? fp.isl_meta[i] & 0x7FFFFFFFu : pad;
-            urec[i] = (i < fp.SU * WAVE) ? fp.isu_meta[i] & 0x7FFFFFFFu : pad;
+            lrec_l[i] = (i < fp.SL * WAVE) ? fp.isl_meta[i] & 0x7FFFFFFFu : pad;
+            urec_l[i] = (i < fp.SU * WAVE) ? fp.isu_meta[i] & 0x7FFFFFFFu : pad;
         }
     }
     // per wave: F[nnz + 2] FP32 (F[nnz] = 0), then two vector regions of

@@ -991,6 +991,10 @@ struct bilu_pattern_s {
     uint32_t *d_ilu0 = nullptr, *d_ilu1 = nullptr;
     uint32_t *d_parp = nullptr, *d_parf = nullptr;
     uint16_t *d_wcode = nullptr;
+    // the FP32 mixed kernel's slot records (A, then ISAI(L), ISAI(U) layouts)
+    // in their LDS layout, for the launches that read them from global memory
+    // (ISAI templates of 48 and 72 slots; null otherwise)
+    uint32_t *d_rec32 = nullptr;
     int *d_counter = nullptr;      // 256 bytes of work counters: [0] solve kernels, [32] factor kernel
     int ilu_sched = 0;             // exact ILU(0) schedule chosen by bilu_ilu0_tune (0: none yet)
     // solve workspace (factors between the factor and solve launches), grown on demand
@@ -1017,6 +1021,39 @@ namespace {
             return BILU_ERR_HIP;                                                         \
         }                                                                                \
     } while (0)
+
+// (A slots, triangle slots) template of the mixed ISAI kernel for a pattern's
+// slot counts (A's S, ISAI(L)'s SL, ISAI(U)'s SU); false when none fits
+bool isai_tier(int S, int SL, int SU, int &sm, int &slu)
+{
+    const int tl = std::max(SL, SU);
+    if (tl <= 0) return false;
+    for (const auto &t : {std::make_pair(24, 20), std::make_pair(36, 24), std::make_pair(48, 32),
+                          std::make_pair(72, 40)})
+        if (S <= t.first && tl <= t.second) { sm = t.first; slu = t.second; return true; }
+    return false;
+}
+
+// the mixed ISAI kernel's slot records as its LDS copy holds them: A's
+// (FP32 x offset | FP32 output offset << 16, the scratch cell n + 1 for a
+// slot that ends no row) padded to sm slots, then ISAI(L)'s and ISAI(U)'s
+// without their row-end bit, padded to slu slots with zero-cell records
+void mixed_isai_records(int n, const bilu::FastPlan &fp, int sm, int slu, std::vector<uint32_t> &rec)
+{
+    rec.assign((size_t)(sm + 2 * slu) * WAVE, 0);
+    const uint32_t scratch = (uint32_t)(n + 1) * 4u;
+    for (size_t i = 0; i < (size_t)sm * WAVE; i++) {
+        const uint32_t m = i < fp.slot_meta.size() ? fp.slot_meta[i] : 0xFFFF0000u;
+        const bool end = (m >> 16) != 0xFFFFu;
+        const uint32_t yo = end ? (m >> 16) >> 1 : scratch;
+        rec[i] = ((m & 0xFFFFu) >> 1) | (yo << 16);
+    }
+    const uint32_t pad = (uint32_t)n * 4u | (scratch << 16);
+    for (size_t i = 0; i < (size_t)slu * WAVE; i++) {
+        rec[(size_t)sm * WAVE + i] = i < fp.isl_meta.size() ? fp.isl_meta[i] & 0x7FFFFFFFu : pad;
+        rec[(size_t)(sm + slu) * WAVE + i] = i < fp.isu_meta.size() ? fp.isu_meta[i] & 0x7FFFFFFFu : pad;
+    }
+}
 
 int upload(bilu_pattern_s &h)
 {
@@ -1105,6 +1142,15 @@ int upload(bilu_pattern_s &h)
         const size_t o_p0 = put(fp.parp.data(), fp.parp.size() * 4);
         const size_t o_p1 = put(fp.parf.data(), fp.parf.size() * 4);
         const size_t o_wc = put(fp.wcode.data(), fp.wcode.size() * 2);
+        // the mixed ISAI kernel's records in global memory (its LDS copy's layout)
+        int sm = 0, slu = 0;
+        size_t o_r32 = 0;
+        const bool r32 = isai_tier(fp.S, fp.SL, fp.SU, sm, slu) && sm >= 48;
+        if (r32) {
+            std::vector<uint32_t> rec;
+            mixed_isai_records(pa.n, fp, sm, slu, rec);
+            o_r32 = put(rec.data(), rec.size() * 4);
+        }
         HIP_TRY(hipMalloc((void **)&h.d_fast, blob.size()));
         HIP_TRY(hipMemcpy(h.d_fast, blob.data(), blob.size(), hipMemcpyHostToDevice));
         uint8_t *b = h.d_fast;
@@ -1120,6 +1166,7 @@ int upload(bilu_pattern_s &h)
         h.d_ilu0 = (uint32_t *)(b + o_i0); h.d_ilu1 = (uint32_t *)(b + o_i1);
         h.d_parp = (uint32_t *)(b + o_p0); h.d_parf = (uint32_t *)(b + o_p1);
         h.d_wcode = (uint16_t *)(b + o_wc);
+        h.d_rec32 = r32 ? (uint32_t *)(b + o_r32) : nullptr;
     }
     return BILU_OK;
 }
@@ -1186,6 +1233,7 @@ void fast_tables(const bilu_pattern_s &h, FastParams &fp)
     fp.SL = pl.SL; fp.SU = pl.SU;
     fp.ilu0 = h.d_ilu0; fp.ilu1 = h.d_ilu1; fp.ilu_steps = pl.ilu_steps;
     fp.parp = h.d_parp; fp.parf = h.d_parf; fp.par_rounds = pl.par_rounds; fp.par_psteps = pl.par_psteps;
+    fp.rec32 = h.d_rec32;
 }
 
 // Kernel timing (bilu_timing_begin / _end): while a window is open, the
@@ -1397,13 +1445,13 @@ struct Jobs {
     const int *sysidx = nullptr, *outidx = nullptr;
 };
 
-template <int NR, int SM, int PCK, int SLU>
+template <int NR, int SM, int PCK, int SLU, int RG = 0>
 int launch_mixed_t(bilu_pattern_s &h, const FastParams &fp, int grid, int waves, size_t lds, int batch,
                    const double *A, const double *b, double *x, const float *F, const int *Z,
                    const MixedParams &mp, float *it, int *cv, double *ir, double *tr, int *refine,
                    hipStream_t st, Jobs jobs, const FactorArgs *fa)
 {
-    auto kern = fa ? k_mixed<NR, SM, PCK, SLU, 1> : k_mixed<NR, SM, PCK, SLU, 0>;
+    auto kern = fa ? k_mixed<NR, SM, PCK, SLU, 1, RG> : k_mixed<NR, SM, PCK, SLU, 0, RG>;
     if (lds > 65536) HIP_TRY(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, st, fp.counter);
     timing_mark(BILU_TIMING_SOLVE, st, true);
@@ -1412,6 +1460,23 @@ int launch_mixed_t(bilu_pattern_s &h, const FastParams &fp, int grid, int waves,
     timing_mark(BILU_TIMING_SOLVE, st, false);
     HIP_TRY(hipGetLastError());
     return BILU_OK;
+}
+
+// the ISAI instantiations: records in global memory only for the 48- and
+// 72-slot templates (the smaller ones are bound by registers, not LDS)
+template <int NR, int SM, int SLU>
+int launch_mixed_isai(bool rg, bilu_pattern_s &h, const FastParams &fp, int grid, int waves, size_t lds, int batch,
+                      const double *A, const double *b, double *x, const float *F, const int *Z,
+                      const MixedParams &mp, float *it, int *cv, double *ir, double *tr, int *refine,
+                      hipStream_t st, Jobs jobs, const FactorArgs *fa)
+{
+    if constexpr (SM >= 48) {
+        if (rg)
+            return launch_mixed_t<NR, SM, PCK_ISAI, SLU, 1>(h, fp, grid, waves, lds, batch, A, b, x, F, Z, mp, it, cv,
+                                                             ir, tr, refine, st, jobs, fa);
+    }
+    return launch_mixed_t<NR, SM, PCK_ISAI, SLU, 0>(h, fp, grid, waves, lds, batch, A, b, x, F, Z, mp, it, cv, ir,
+                                                     tr, refine, st, jobs, fa);
 }
 
 // SpMV slot templates of the mixed kernel (FP32 A: up to 72 slots per lane)
@@ -1426,23 +1491,17 @@ int slot_template_mixed(int S)
 // factors each system itself (F and Z unused)
 int launch_mixed(bilu_pattern_s &h, int batch, const double *A, const double *b, double *x, const float *F,
                  const int *Z, int pck, const MixedParams &mp, float *it, int *cv, double *ir, double *tr,
-                 int *refine, hipStream_t st, Jobs jobs = Jobs(), int max_waves = 64, const FactorArgs *fa = nullptr)
+                 int *refine, hipStream_t st, Jobs jobs = Jobs(), int max_waves = 64, const FactorArgs *fa = nullptr,
+                 int rec_global = 0)
 {
     const auto &pl = h.pa.fast;
     const int n = h.pa.n, nnz = h.pa.nnz;
     const int nr = (n + WAVE - 1) / WAVE;
     int sm = slot_template_mixed(pl.S), slu = 1;
     if (!pl.ok || nr > MAX_NR || pl.nr != nr || sm < 0) return BILU_ERR_TOO_LARGE;
-    if (pck == PCK_ISAI) {
-        // (A slots, triangle slots) tiers of the ISAI instantiations
-        const int tl = std::max(pl.SL, pl.SU);
-        if (tl <= 0) return BILU_ERR_TOO_LARGE;
-        sm = -1;
-        for (const auto &t : {std::make_pair(24, 20), std::make_pair(36, 24), std::make_pair(48, 32),
-                              std::make_pair(72, 40)})
-            if (pl.S <= t.first && tl <= t.second) { sm = t.first; slu = t.second; break; }
-        if (sm < 0) return BILU_ERR_TOO_LARGE;
-    }
+    if (pck == PCK_ISAI && !isai_tier(pl.S, pl.SL, pl.SU, sm, slu)) return BILU_ERR_TOO_LARGE;
+    // records in global memory: the ISAI templates of 48 and 72 slots, when asked
+    const bool rg = rec_global && pck == PCK_ISAI && sm >= 48 && h.d_rec32;
     FastParams fp{};
     fast_tables(h, fp);
     // per wave: LU: FP32 factors (+2 cells, 16-aligned), two (n + 2)-double vector regions,
@@ -1460,8 +1519,9 @@ int launch_mixed(bilu_pattern_s &h, int batch, const double *A, const double *b,
         fp.wave_bytes = std::max<int>(fp.wave_bytes, (int)fr);
     }
     fp.wave_bytes = (fp.wave_bytes + 15) / 16 * 16;
-    // shared LDS: A's slot records (ISAI: then the two triangle layouts' records)
-    const size_t rec = (size_t)(sm + (pck == PCK_ISAI ? 2 * slu : 0)) * WAVE * 4;
+    // shared LDS: A's slot records (ISAI: then the two triangle layouts' records),
+    // unless they are read from global memory
+    const size_t rec = rg ? 0 : (size_t)(sm + (pck == PCK_ISAI ? 2 * slu : 0)) * WAVE * 4;
     if (rec >= h.max_lds) return BILU_ERR_TOO_LARGE;
     const int wcap = std::min(max_waves, km_waves(sm, pck, fa ? 1 : 0));
     fp.shared_bytes = int(rec);
@@ -1474,7 +1534,7 @@ int launch_mixed(bilu_pattern_s &h, int batch, const double *A, const double *b,
     if (nr == R && sm == M) \
         return pck == PCK_LU \
             ? launch_mixed_t<R, M, PCK_LU, 1>(h, fp, grid, waves, lds, batch, A, b, x, F, Z, mp, it, cv, ir, tr, refine, st, jobs, fa) \
-            : launch_mixed_t<R, M, PCK_ISAI, T>(h, fp, grid, waves, lds, batch, A, b, x, F, Z, mp, it, cv, ir, tr, refine, st, jobs, fa);
+            : launch_mixed_isai<R, M, T>(rg, h, fp, grid, waves, lds, batch, A, b, x, F, Z, mp, it, cv, ir, tr, refine, st, jobs, fa);
 #define MIXED_ROW(R) MIXED_CASE(R, 24, 20) MIXED_CASE(R, 36, 24) MIXED_CASE(R, 48, 32) MIXED_CASE(R, 72, 40)
     MIXED_ROW(1) MIXED_ROW(2) MIXED_ROW(3) MIXED_ROW(4) MIXED_ROW(5)
 #undef MIXED_ROW

@@ -19,7 +19,9 @@
 // leave the CU, at the cost of the larger per-wave LDS region).  The
 // autotuner times the candidates of each pattern -- segregated with {2, 4, 8}
 // waves per factor workgroup x {4, 8, 12} per solve workgroup, merged with
-// {4, 8, 12} (capped by the kernel's register budget) -- first alone on the pattern's requests, then, pattern by
+// {4, 8, 12} (capped by the kernel's register budget), and for the ISAI
+// templates of 48 and 72 slots the solve kernel's slot records in LDS or in
+// global memory (the LDS they free makes room for more waves) -- first alone on the pattern's requests, then, pattern by
 // pattern from the slowest, inside the whole launch with every other pattern
 // running its current choice, and keeps the fastest.  Results do not depend on
 // the choice (every system is processed by one wave with the same arithmetic).
@@ -33,7 +35,8 @@ struct bilu_uneven_s {
     bilu_opts o{};
     std::vector<int> fwaves, swaves; // tuned waves per workgroup (factor, solve; merged: solve only)
     std::vector<int> merged;         // per pattern: 1 = the merged kernel
-    struct Cand { int merged, fw, sw; float alone_ms, corun_ms; };
+    std::vector<int> recg;           // per pattern: 1 = the solve kernel's slot records in global memory
+    struct Cand { int merged, fw, sw, recg; float alone_ms, corun_ms; };
     std::vector<std::vector<Cand>> table;   // per pattern: the candidates of the last tuning
     std::vector<void *> ws;          // per pattern: FP32 factors / ISAI [count * nnz], then flags [count]
     std::vector<size_t> fbytes;
@@ -63,7 +66,7 @@ void free_uneven(bilu_uneven_s *u)
 
 // the factor + solve chain of pattern p on stream st (mg: the merged kernel alone)
 int uneven_branch(bilu_uneven_s &u, int p, const double *A, const double *b, double *x, float *it, int *cv,
-                  double *ir, double *tr, int *rf, hipStream_t st, int fw, int sw, int mg)
+                  double *ir, double *tr, int *rf, hipStream_t st, int fw, int sw, int mg, int rg)
 {
     bilu_pattern_s &h = *u.pats[p];
     const int cnt = u.count[p];
@@ -80,7 +83,7 @@ int uneven_branch(bilu_uneven_s &u, int p, const double *A, const double *b, dou
                       par && u.o.parilu_order == BILU_PARILU_INPLACE ? 1 : 0, par ? u.o.parilu_stop_tol : 0.0,
                       nullptr, nullptr, 0, nullptr};
         return launch_mixed(h, cnt, A, b, x, nullptr, nullptr, isai ? PCK_ISAI : PCK_LU, mp, it, cv, ir, tr, rf, st,
-                            jobs, sw, &fa);
+                            jobs, sw, &fa, rg);
     }
     float *F = static_cast<float *>(u.ws[p]);
     int *Z = reinterpret_cast<int *>(static_cast<char *>(u.ws[p]) + u.fbytes[p]);
@@ -89,7 +92,8 @@ int uneven_branch(bilu_uneven_s &u, int p, const double *A, const double *b, dou
                                        jobs.sysidx, fw, nullptr, 0,
                                        par && u.o.parilu_order == BILU_PARILU_INPLACE ? 1 : 0);
     if (rc != BILU_OK) return rc;
-    return launch_mixed(h, cnt, A, b, x, F, Z, isai ? PCK_ISAI : PCK_LU, mp, it, cv, ir, tr, rf, st, jobs, sw);
+    return launch_mixed(h, cnt, A, b, x, F, Z, isai ? PCK_ISAI : PCK_LU, mp, it, cv, ir, tr, rf, st, jobs, sw,
+                        nullptr, rg);
 }
 
 // all branches: fork from st, one chain per pattern on its stream, join into st
@@ -100,7 +104,7 @@ int uneven_launch(bilu_uneven_s &u, const double *const *A, const double *const 
     for (int p = 0; p < u.npat; p++) {
         HIP_TRY(hipStreamWaitEvent(u.streams[p], u.events[0], 0));
         const int rc = uneven_branch(u, p, A[p], b[p], x[p], it, cv, ir, tr, rf, u.streams[p], u.fwaves[p],
-                                     u.swaves[p], u.merged[p]);
+                                     u.swaves[p], u.merged[p], u.recg[p]);
         if (rc != BILU_OK) return rc;
         HIP_TRY(hipEventRecord(u.events[1 + p], u.streams[p]));
         HIP_TRY(hipStreamWaitEvent(st, u.events[1 + p], 0));
@@ -121,6 +125,7 @@ std::vector<const void *> uneven_args(const bilu_uneven_s &u, const double *cons
         a.push_back((const void *)(intptr_t)u.fwaves[p]);
         a.push_back((const void *)(intptr_t)u.swaves[p]);
         a.push_back((const void *)(intptr_t)u.merged[p]);
+        a.push_back((const void *)(intptr_t)u.recg[p]);
     }
     return a;
 }
@@ -175,6 +180,7 @@ int bilu_uneven_create(int npat, const bilu_pattern_t *patterns, int nreq, const
         u->fwaves.push_back(KF_MAX_WAVES);
         u->swaves.push_back(64);
         u->merged.push_back(o.merge == BILU_MERGED ? 1 : 0);
+        u->recg.push_back(0);
     }
     if (hipMalloc((void **)&u->d_jobs, std::max<size_t>(1, flat.size()) * sizeof(int)) != hipSuccess ||
         (!flat.empty() && hipMemcpy(u->d_jobs, flat.data(), flat.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess)) {
@@ -219,13 +225,24 @@ int bilu_uneven_tune(bilu_uneven_t u, const double *const *d_A, const double *co
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
-    // candidates of a pattern: segregated (factor waves, solve waves) and merged (solve waves)
-    std::vector<bilu_uneven_s::Cand> cands;
-    if (u->o.merge != BILU_MERGED)
-        for (int fw : {2, 4, 8})
-            for (int sw : {4, 8, 12}) cands.push_back({0, fw, sw, -1.0f, -1.0f});
-    if (u->o.merge != BILU_SEGREGATED)
-        for (int sw : {4, 8, 12}) cands.push_back({1, 0, sw, -1.0f, -1.0f});
+    // candidates of a pattern: segregated (factor waves, solve waves) and merged
+    // (solve waves), each with the solve kernel's slot records in LDS and, for
+    // the ISAI templates of 48 and 72 slots, in global memory
+    const bool isai = u->o.precond == BILU_PRECOND_ILU0_ISAI || u->o.precond == BILU_PRECOND_PARILU0_ISAI;
+    auto cands_of = [&](int p) {
+        std::vector<bilu_uneven_s::Cand> cands;
+        const auto &pl = u->pats[p]->pa.fast;
+        int sm = 0, slu = 0;
+        const bool place = isai && u->pats[p]->d_rec32 && isai_tier(pl.S, pl.SL, pl.SU, sm, slu) && sm >= 48;
+        for (int rg = 0; rg <= (place ? 1 : 0); rg++) {
+            if (u->o.merge != BILU_MERGED)
+                for (int fw : {2, 4, 8})
+                    for (int sw : {4, 8, 12}) cands.push_back({0, fw, sw, rg, -1.0f, -1.0f});
+            if (u->o.merge != BILU_SEGREGATED)
+                for (int sw : {4, 8, 12}) cands.push_back({1, 0, sw, rg, -1.0f, -1.0f});
+        }
+        return cands;
+    };
     // one untimed run (first-use costs), then the timed one
     auto timed = [&](auto &&launch, float &ms) -> int {
         for (int rep = 0; rep < 2; rep++) {
@@ -245,13 +262,13 @@ int bilu_uneven_tune(bilu_uneven_t u, const double *const *d_A, const double *co
     for (int p = 0; p < u->npat && rc == BILU_OK; p++) {
         if (u->count[p] == 0) continue;
         auto &tab = u->table[p];
-        tab = cands;
+        tab = cands_of(p);
         float best = 1e30f;
         for (size_t c = 0; c < tab.size() && rc == BILU_OK; c++) {
             float ms = 0.0f;
             rc = timed([&]() {
                 return uneven_branch(*u, p, d_A[p], d_b[p], d_x[p], d_iter, d_conv, d_iter_res, d_true_res,
-                                     d_refine, st, tab[c].fw, tab[c].sw, tab[c].merged);
+                                     d_refine, st, tab[c].fw, tab[c].sw, tab[c].merged, tab[c].recg);
             }, ms);
             tab[c].alone_ms = ms;
             if (rc == BILU_OK && ms < best) {
@@ -259,6 +276,7 @@ int bilu_uneven_tune(bilu_uneven_t u, const double *const *d_A, const double *co
                 u->fwaves[p] = tab[c].fw ? tab[c].fw : u->fwaves[p];
                 u->swaves[p] = tab[c].sw;
                 u->merged[p] = tab[c].merged;
+                u->recg[p] = tab[c].recg;
             }
         }
         order.push_back({best, p});
@@ -271,11 +289,12 @@ int bilu_uneven_tune(bilu_uneven_t u, const double *const *d_A, const double *co
         const int p = op.second;
         auto &tab = u->table[p];
         float best = 1e30f;
-        bilu_uneven_s::Cand keep{u->merged[p], u->fwaves[p], u->swaves[p], 0.0f, 0.0f};
+        bilu_uneven_s::Cand keep{u->merged[p], u->fwaves[p], u->swaves[p], u->recg[p], 0.0f, 0.0f};
         for (size_t c = 0; c < tab.size() && rc == BILU_OK; c++) {
             u->fwaves[p] = tab[c].fw ? tab[c].fw : keep.fw;
             u->swaves[p] = tab[c].sw;
             u->merged[p] = tab[c].merged;
+            u->recg[p] = tab[c].recg;
             float ms = 0.0f;
             rc = timed([&]() {
                 return uneven_launch(*u, d_A, d_b, d_x, d_iter, d_conv, d_iter_res, d_true_res, d_refine, st);
@@ -283,12 +302,13 @@ int bilu_uneven_tune(bilu_uneven_t u, const double *const *d_A, const double *co
             tab[c].corun_ms = ms;
             if (rc == BILU_OK && ms < best) {
                 best = ms;
-                keep = {tab[c].merged, u->fwaves[p], tab[c].sw, 0.0f, 0.0f};
+                keep = {tab[c].merged, u->fwaves[p], tab[c].sw, tab[c].recg, 0.0f, 0.0f};
             }
         }
         u->fwaves[p] = keep.fw;
         u->swaves[p] = keep.sw;
         u->merged[p] = keep.merged;
+        u->recg[p] = keep.recg;
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
@@ -311,8 +331,28 @@ int bilu_uneven_merge_point(bilu_uneven_t u, int p, int *merged)
     return BILU_OK;
 }
 
+int bilu_uneven_configure(bilu_uneven_t u, int p, int merged, int factor_waves, int solve_waves, int records_global)
+{
+    if (!u || p < 0 || p >= u->npat || (merged != 0 && merged != 1) || solve_waves < 1 ||
+        (!merged && factor_waves < 1) || (records_global != 0 && records_global != 1))
+        return BILU_ERR_INVALID_ARG;
+    if ((merged && u->o.merge == BILU_SEGREGATED) || (!merged && u->o.merge == BILU_MERGED)) return BILU_ERR_INVALID_ARG;
+    u->merged[p] = merged;
+    if (!merged) u->fwaves[p] = factor_waves;
+    u->swaves[p] = solve_waves;
+    u->recg[p] = records_global;
+    return BILU_OK;
+}
+
+int bilu_uneven_placement(bilu_uneven_t u, int p, int *records_global)
+{
+    if (!u || p < 0 || p >= u->npat || !records_global) return BILU_ERR_INVALID_ARG;
+    *records_global = u->recg[p];
+    return BILU_OK;
+}
+
 int bilu_uneven_tune_table(bilu_uneven_t u, int p, int cap, int *merged, int *factor_waves, int *solve_waves,
-                           float *alone_ms, float *corun_ms, int *count)
+                           int *records_global, float *alone_ms, float *corun_ms, int *count)
 {
     if (!u || p < 0 || p >= u->npat || !count || cap < 0) return BILU_ERR_INVALID_ARG;
     const auto &tab = p < (int)u->table.size() ? u->table[p] : std::vector<bilu_uneven_s::Cand>();
@@ -321,6 +361,7 @@ int bilu_uneven_tune_table(bilu_uneven_t u, int p, int cap, int *merged, int *fa
         if (merged) merged[c] = tab[c].merged;
         if (factor_waves) factor_waves[c] = tab[c].fw;
         if (solve_waves) solve_waves[c] = tab[c].sw;
+        if (records_global) records_global[c] = tab[c].recg;
         if (alone_ms) alone_ms[c] = tab[c].alone_ms;
         if (corun_ms) corun_ms[c] = tab[c].corun_ms;
     }

@@ -76,9 +76,13 @@ def test_uneven_matches_per_pattern_and_tuning(precond):
         assert (fw == 0) == plan.merged(p) and sw in (4, 8, 12) and jobs == B_each
         assert fw in (0, 2, 4, 8)
         tab = plan.tune_table(p)
-        assert len(tab) == 12 and sum(c["merged"] for c in tab) == 3
+        # the record placement is a candidate for the ISAI templates of 48 / 72 slots (n = 300 here)
+        place = precond == bilu.ILU0_ISAI and NS[p] == 300
+        assert len(tab) == (24 if place else 12) and sum(c["merged"] for c in tab) == (6 if place else 3)
+        assert sum(c["records_global"] for c in tab) == (12 if place else 0)
         assert all(c["alone_ms"] > 0 and c["corun_ms"] > 0 for c in tab)
-        print(f"pattern n={NS[p]}: merged={plan.merged(p)} factor_waves={fw} solve_waves={sw}", flush=True)
+        print(f"pattern n={NS[p]}: merged={plan.merged(p)} factor_waves={fw} solve_waves={sw} "
+              f"records_global={plan.records_global(p)}", flush=True)
     for x in dx:
         x.zero_()
     plan.solve(dA, db, dx, out)
@@ -115,6 +119,33 @@ def test_uneven_merged_plan(precond):
             for name, u, v in zip(["iters", "conv", "iter_res", "true_res", "refine"], out, res):
                 assert np.array_equal(u.cpu().numpy()[rows], v[req_i[rows]]), f"pattern {p}: {name} differs"
     assert len(plan.tune_table(0)) == 3
+    plan.close()
+
+
+def test_uneven_records_global_n300_bitwise():
+    """the ISAI solve kernel with its slot records in global memory (the
+    placement the tuner may pick for 72-slot patterns): bitwise the oracle's
+    and the LDS placement's results, merged and segregated"""
+    print("case uneven n300 records in global memory", flush=True)
+    pat, A, b = synth.batch("random", 96, n=300, density=13.0 / 299)
+    P = bilu.Pattern.from_synth(pat)
+    dA, db = torch.from_numpy(A).to(dev()), torch.from_numpy(b).to(dev())
+    o = O.batch_solve_split(pat, A, b, precond=O.ILU0_ISAI, precision=O.PREC_FP32_IR,
+                            o=O.opts(tol=1e-10, tol_mode=O.TOL_RELATIVE, trsv_order=O.TRSV_AVAIL), threads=8)
+    plan = bilu.Uneven([P], np.zeros(96, np.int32), np.arange(96, dtype=np.int32),
+                       bilu.opts(precond=bilu.ILU0_ISAI, merge=bilu.MERGE_AUTO, **OPTS))
+    out = plan.outputs()
+    for merged, fw, sw, rg in ((False, 4, 8, True), (True, 0, 8, True), (False, 8, 4, False), (True, 0, 4, False)):
+        dx = torch.zeros_like(db)
+        for t in out:
+            t.zero_()
+        plan.configure(0, merged, fw, sw, rg)
+        assert plan.records_global(0) == rg and plan.merged(0) == merged
+        plan.solve([dA], [db], [dx], out)
+        torch.cuda.synchronize()
+        assert np.array_equal(dx.cpu().numpy(), o[0]), f"merged={merged} records_global={rg}: x differs"
+        for u, v in zip(out, (o[1], o[2], o[3], o[4], o[5])):
+            assert np.array_equal(u.cpu().numpy(), v), f"merged={merged} records_global={rg}"
     plan.close()
 
 

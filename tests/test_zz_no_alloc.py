@@ -2,7 +2,8 @@
 handle workspace) against bilu_solve_ex through the C ABI: every output bit
 for bit, two calls in a row, the second with a smaller batch, on the paths
 that pass factors or ISAI values through the scratch (mixed precision,
-split FP64 ParILU, segregated FP64) and one that needs none (merged FP64).
+split FP64 ParILU, segregated FP64) and ones that need none (merged FP64,
+the merged mixed-precision kernel).
 The first call's batch spans several scratch chunks."""
 import numpy as np
 import pytest
@@ -27,6 +28,7 @@ CASES = {
     "fp64_split_parilu": dict(precond=bilu.PARILU0, parilu_sweeps=40, parilu_stop_tol=1e-8),
     "fp64_segregated_ilu0": dict(precond=bilu.ILU0, merge=bilu.SEGREGATED),
     "fp64_merged_ilu0": dict(precond=bilu.ILU0, true_res_restart=1),
+    "mixed_merged_kernel": dict(precond=bilu.ILU0_ISAI, precision=bilu.PRECISION_FP32_IR, merge=bilu.MERGED),
 }
 
 # 36000 systems at nnz 2082 need three chunks of the 256 MiB scratch
