@@ -434,9 +434,13 @@ def main():
     # ParILU(0) 5 sweeps merged into the solve; ILU(0) / ParILU(0) + ISAI(L, U)
     # and the general ISAI of A (generation launch, then the persistent solve)
     c1 = {}
-    for name, pc in (("parilu0_5_sweeps_merged", bilu.PARILU0), ("ilu0_isai", bilu.ILU0_ISAI),
-                     ("parilu0_5_sweeps_isai", bilu.PARILU0_ISAI), ("general_isai", bilu.GENERAL_ISAI)):
-        o_v = bilu.opts(precond=pc, parilu_sweeps=5, true_res_restart=1, kernel=bilu.KERNEL_PERSISTENT, **common)
+    for name, pc, mg in (("parilu0_5_sweeps", bilu.PARILU0, bilu.MERGE_AUTO),
+                         ("parilu0_5_sweeps_merged", bilu.PARILU0, bilu.MERGED),
+                         ("ilu0_isai", bilu.ILU0_ISAI, bilu.MERGE_AUTO),
+                         ("parilu0_5_sweeps_isai", bilu.PARILU0_ISAI, bilu.MERGE_AUTO),
+                         ("general_isai", bilu.GENERAL_ISAI, bilu.MERGE_AUTO)):
+        o_v = bilu.opts(precond=pc, parilu_sweeps=5, true_res_restart=1, kernel=bilu.KERNEL_PERSISTENT, merge=mg,
+                        **common)
         ms_v, kms_v = run(o_v, kernel=bilu.TIMING_SOLVE, status=False)
         c1[name] = dict(summary(ms_v), solve_kernel_ms=round(kms_v, 4))
     for name, pc in (("ilu0_segregated", bilu.ILU0), ("parilu0_5_sweeps_segregated", bilu.PARILU0)):

@@ -65,8 +65,9 @@ extern "C" {
 
 #define BILU_MERGED 0         /* factor and solve in one launch (merged.cu) */
 #define BILU_SEGREGATED 1     /* factor launch, factors through HBM, solve launch (segregated.cu) */
-#define BILU_MERGE_AUTO 2     /* FP64: merged; FP32_IR: segregated (the faster at n = 144 when measured);
-                                 bilu_uneven_tune picks per pattern.  Results do not depend on it */
+#define BILU_MERGE_AUTO 2     /* FP64: merged (ParILU(0): segregated); FP32_IR: segregated (the faster at
+                                 n = 144 when measured); bilu_uneven_tune picks per pattern.  Results do
+                                 not depend on it */
 
 #define BILU_KERNEL_AUTO 0        /* fastest kernel that supports the pattern and options */
 #define BILU_KERNEL_WAVE 1        /* one wave64 workgroup per system, schedules read from global */

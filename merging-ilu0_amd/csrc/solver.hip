@@ -532,70 +532,85 @@ __device__ int isai_tri(const DevIsai &M, int n, int type, const double *vals, d
 
 // General ISAI rows by Gauss-Jordan, two rows per wave (32 lanes each, lane t
 // holding row t of the m x m block); steps and columns stop at the larger m of
-// the two rows (beyond a row's m nothing changes its solution)
+// the two rows (beyond a row's m nothing changes its solution).  Every loop over
+// the block's columns is unrolled with constant indices, so the block stays in
+// registers; the lane's entry in the pivot column is carried in cdp (picked up
+// as the previous step's update passes it) instead of indexing row[] by dp.
 __device__ int isai_general(const DevIsai &M, int n, const double *vals, double *out, int lane)
 {
-    const int grp = lane / ISAI_GROUP, t = lane % ISAI_GROUP;
+    constexpr int G = ISAI_GROUP;
+    const int grp = lane / G, t = lane % G;
     int bad = 0;
     for (int i0 = 0; i0 < n; i0 += 2) {
         const int i = i0 + grp;
         const bool act = i < n;
         const int m = act ? M.m[i] : 0;
-        const int mo = __shfl_xor(m, ISAI_GROUP, WAVE);
+        const int mo = __shfl_xor(m, G, WAVE);
         const int mmax = m > mo ? m : mo;
-        const int *g = act ? M.gather + M.goff[i] : M.gather;
-        double row[ISAI_GROUP];
+        const int *g = M.gather + (act ? M.goff[i] : 0);
+        // unconditional loads at clamped positions (all in flight at once)
+        const int last = m > 0 ? m * m - 1 : 0;
+        int gi[G];
 #pragma unroll
-        for (int k = 0; k < ISAI_GROUP; k++)
-            row[k] = (t < m && k < m) ? view_val(vals, g[t * m + k]) : 0.0;
+        for (int k = 0; k < G; k++) gi[k] = g[min(t * m + k, last)];
+        double row[G];
+#pragma unroll
+        for (int k = 0; k < G; k++) row[k] = (t < m && k < m) ? view_val(vals, gi[k]) : 0.0;
         double rhs = (act && t == M.rhs[i]) ? 1.0 : 0.0;
-#pragma unroll
-        for (int dp = 0; dp < ISAI_GROUP; dp++) {
-            if (dp >= mmax) break;
+        double cdp = row[0];
+        for (int dp = 0; dp < mmax; dp++) {
             const bool on = dp < m;  // uniform within the group
             // pivot: reference 32-lane shfl_down tree (choose_pivot_row_1)
-            const double dval = __shfl(row[dp], dp, ISAI_GROUP);
-            double val = (t >= m || t < dp) ? dval : row[dp];
+            const double dval = __shfl(cdp, dp, G);
+            double val = (t >= m || t < dp) ? dval : cdp;
             int piv = (t >= m || t < dp) ? dp : t;
 #pragma unroll
-            for (int off = ISAI_GROUP / 2; off > 0; off >>= 1) {
-                const double vo = __shfl_down(val, off, ISAI_GROUP);
-                const int po = __shfl_down(piv, off, ISAI_GROUP);
+            for (int off = G / 2; off > 0; off >>= 1) {
+                const double vo = __shfl_down(val, off, G);
+                const int po = __shfl_down(piv, off, G);
                 if (fabs(vo) > fabs(val)) { val = vo; piv = po; }
             }
-            piv = __shfl(piv, 0, ISAI_GROUP);
+            piv = __shfl(piv, 0, G);
             if (!on) piv = dp;
             if (piv != dp) {
 #pragma unroll
-                for (int c = 0; c < ISAI_GROUP; c++) {
-                    if (c >= mmax) break;
-                    const double a = __shfl(row[c], dp, ISAI_GROUP);
-                    const double b = __shfl(row[c], piv, ISAI_GROUP);
-                    if (t == dp) row[c] = b;
-                    if (t == piv) row[c] = a;
+                for (int c = 0; c < G; c++) {
+                    if (c < mmax) {
+                        const double a = __shfl(row[c], dp, G);
+                        const double b = __shfl(row[c], piv, G);
+                        if (t == dp) row[c] = b;
+                        if (t == piv) row[c] = a;
+                    }
                 }
-                const double a = __shfl(rhs, dp, ISAI_GROUP);
-                const double b = __shfl(rhs, piv, ISAI_GROUP);
+                const double a = __shfl(rhs, dp, G), b = __shfl(rhs, piv, G);
                 if (t == dp) rhs = b;
                 if (t == piv) rhs = a;
+                const double ca = __shfl(cdp, dp, G), cb = __shfl(cdp, piv, G);
+                if (t == dp) cdp = cb;
+                if (t == piv) cdp = ca;
             }
-            const double de = __shfl(row[dp], dp, ISAI_GROUP);
+            const double de = __shfl(cdp, dp, G);
             if (on) {
                 bad |= (de == 0.0);
-                const double mult = row[dp] / de;
+                const double mult = cdp / de;
 #pragma unroll
-                for (int c = 0; c < ISAI_GROUP; c++) {
-                    if (c >= mmax) break;
-                    const double key = __shfl(row[c], dp, ISAI_GROUP);
-                    if (t != dp && c < m) row[c] = fma(-mult, key, row[c]);
+                for (int c = 0; c < G; c++) {
+                    if (c < mmax) {
+                        const double key = __shfl(row[c], dp, G);
+                        if (t != dp && c < m) row[c] = fma(-mult, key, row[c]);
+                    }
                 }
-                const double rk = __shfl(rhs, dp, ISAI_GROUP);
+                const double rk = __shfl(rhs, dp, G);
                 if (t != dp) rhs = fma(-mult, rk, rhs);
             }
+            // the lane's entry in the next pivot column, after this step
+#pragma unroll
+            for (int c = 1; c < G; c++)
+                if (c == dp + 1) cdp = row[c];
         }
         double d = 1.0;
 #pragma unroll
-        for (int k = 0; k < ISAI_GROUP; k++)
+        for (int k = 0; k < G; k++)
             if (k == t) d = row[k];
         if (act && t < m) out[M.beg[i] + t] = rhs / d;
     }
@@ -1842,7 +1857,12 @@ static int solve_fp64(bilu_pattern_t h, int batch, const double *d_A, const doub
         return BILU_ERR_TOO_LARGE;
     if (pc == PC_ISAI_GEN && pa.isai[BILU_ISAI_GENERAL].max_m > ISAI_GROUP) return BILU_ERR_TOO_LARGE;
     const double *F = nullptr;
-    if (o.merge == BILU_SEGREGATED && pc == PC_LU) {
+    // BILU_MERGE_AUTO splits ParILU(0): the factor kernel's lane-synchronous
+    // sweeps beat the merged kernel's register-lean ones (n = 144, batch
+    // 65536, 5 sweeps: 13.6 vs 25.6 ms), with the same results
+    const bool seg = o.merge == BILU_SEGREGATED ||
+                     (o.merge == BILU_MERGE_AUTO && sp.factor == FK_PARILU && pa.fast.ok);
+    if (seg && pc == PC_LU) {
         // factor launch, factors through HBM (the handle's workspace), then the
         // solve launch (segregated.cu:429-490)
         const size_t flds = (size_t)(sp.factor == FK_PARILU ? 3 : 2) * nnz * sizeof(double);
