@@ -482,12 +482,14 @@ __device__ __forceinline__ double view_val(const double *vals, int g)
     return g >= 0 ? vals[g] : (g == -2 ? 1.0 : 0.0);
 }
 
+// The wave takes row pairs (i0, i0 + 1) for i0 = p0, p0 + pstep, ... (several
+// waves of one workgroup share a system by taking interleaved pairs).
 __device__ int isai_tri(const DevIsai &M, int n, int type, const double *vals, double *out,
-                        double *udiag, int lane)
+                        double *udiag, int lane, int p0 = 0, int pstep = 2)
 {
     const int grp = lane / ISAI_GROUP, t = lane % ISAI_GROUP;
     int bad = 0;
-    for (int i0 = 0; i0 < n; i0 += 2) {
+    for (int i0 = p0; i0 < n; i0 += pstep) {
         const int i = i0 + grp;
         const bool act = i < n;
         const int m = act ? M.m[i] : 0;
@@ -530,48 +532,93 @@ __device__ int isai_tri(const DevIsai &M, int n, int type, const double *vals, d
     return __any(bad);
 }
 
+// DPP row_shl:S -- lane l reads lane l + S of its row of 16 lanes (a lane whose
+// source falls outside the row keeps its own value)
+template <int S>
+__device__ __forceinline__ int row_shl_i32(int v)
+{
+    return __builtin_amdgcn_update_dpp(v, v, 0x100 | S, 0xF, 0xF, false);
+}
+
+template <int S>
+__device__ __forceinline__ double row_shl_f64(double v)
+{
+    return __hiloint2double(row_shl_i32<S>(__double2hiint(v)), row_shl_i32<S>(__double2loint(v)));
+}
+
 // General ISAI rows by Gauss-Jordan, two rows per wave (32 lanes each, lane t
 // holding row t of the m x m block); steps and columns stop at the larger m of
-// the two rows (beyond a row's m nothing changes its solution).  Every loop over
-// the block's columns is unrolled with constant indices, so the block stays in
-// registers; the lane's entry in the pivot column is carried in cdp (picked up
-// as the previous step's update passes it) instead of indexing row[] by dp.
-__device__ int isai_general(const DevIsai &M, int n, const double *vals, double *out, int lane)
+// the two rows (beyond a row's m nothing changes its solution).  Row pairs as
+// in isai_tri.  Every loop over the block's columns is unrolled with constant
+// indices, so the block stays in registers; the lane's entry in the pivot
+// column is carried in cdp (picked up as the previous step's update passes
+// it) instead of indexing row[] by dp.  The pivot search is the reference's
+// 32-lane shfl_down tree: the step at distance 16 crosses rows of 16 lanes
+// (ds_bpermute), the steps at 8, 4, 2, 1 stay inside a row (DPP row_shl).
+// Only lanes below the distance are read by later steps, and each of them
+// reads a source inside its own row, so the DPP steps give the tree's values.
+__device__ int isai_general(const DevIsai &M, int n, const double *vals, double *out, int lane,
+                            int p0 = 0, int pstep = 2)
 {
     constexpr int G = ISAI_GROUP;
+    constexpr int CH = 8;  // columns per batch of pivot-row broadcasts
     const int grp = lane / G, t = lane % G;
     int bad = 0;
-    for (int i0 = 0; i0 < n; i0 += 2) {
+    for (int i0 = p0; i0 < n; i0 += pstep) {
         const int i = i0 + grp;
         const bool act = i < n;
         const int m = act ? M.m[i] : 0;
         const int mo = __shfl_xor(m, G, WAVE);
-        const int mmax = m > mo ? m : mo;
+        const int mmax = __builtin_amdgcn_readfirstlane(m > mo ? m : mo);  // same on every lane
         const int *g = M.gather + (act ? M.goff[i] : 0);
-        // unconditional loads at clamped positions (all in flight at once)
+        // unconditional loads at clamped positions (a batch in flight at once)
         const int last = m > 0 ? m * m - 1 : 0;
-        int gi[G];
-#pragma unroll
-        for (int k = 0; k < G; k++) gi[k] = g[min(t * m + k, last)];
         double row[G];
 #pragma unroll
-        for (int k = 0; k < G; k++) row[k] = (t < m && k < m) ? view_val(vals, gi[k]) : 0.0;
+        for (int k0 = 0; k0 < G; k0 += CH) {
+            if (k0 < mmax) {
+                int gi[CH];
+#pragma unroll
+                for (int j = 0; j < CH; j++) gi[j] = g[min(t * m + k0 + j, last)];
+#pragma unroll
+                for (int j = 0; j < CH; j++) {
+                    const double v = vals[max(gi[j], 0)];
+                    row[k0 + j] = (t < m && k0 + j < m) ? (gi[j] >= 0 ? v : (gi[j] == -2 ? 1.0 : 0.0)) : 0.0;
+                }
+            } else {
+#pragma unroll
+                for (int j = 0; j < CH; j++) row[k0 + j] = 0.0;
+            }
+        }
         double rhs = (act && t == M.rhs[i]) ? 1.0 : 0.0;
         double cdp = row[0];
         for (int dp = 0; dp < mmax; dp++) {
             const bool on = dp < m;  // uniform within the group
-            // pivot: reference 32-lane shfl_down tree (choose_pivot_row_1)
             const double dval = __shfl(cdp, dp, G);
             double val = (t >= m || t < dp) ? dval : cdp;
             int piv = (t >= m || t < dp) ? dp : t;
-#pragma unroll
-            for (int off = G / 2; off > 0; off >>= 1) {
-                const double vo = __shfl_down(val, off, G);
-                const int po = __shfl_down(piv, off, G);
+            {
+                const double vo = __shfl_down(val, 16, G);
+                const int po = __shfl_down(piv, 16, G);
                 if (fabs(vo) > fabs(val)) { val = vo; piv = po; }
             }
-            piv = __shfl(piv, 0, G);
+#define BILU_PIVOT_DPP(S)                                                                          \
+            {                                                                                      \
+                const double vo = row_shl_f64<S>(val);                                             \
+                const int po = row_shl_i32<S>(piv);                                                \
+                if (fabs(vo) > fabs(val)) { val = vo; piv = po; }                                  \
+            }
+            BILU_PIVOT_DPP(8)
+            BILU_PIVOT_DPP(4)
+            BILU_PIVOT_DPP(2)
+            BILU_PIVOT_DPP(1)
+#undef BILU_PIVOT_DPP
+            {
+                const int pa = __builtin_amdgcn_readlane(piv, 0), pb = __builtin_amdgcn_readlane(piv, G);
+                piv = grp ? pb : pa;
+            }
             if (!on) piv = dp;
+            double de = dval;
             if (piv != dp) {
 #pragma unroll
                 for (int c = 0; c < G; c++) {
@@ -588,25 +635,35 @@ __device__ int isai_general(const DevIsai &M, int n, const double *vals, double 
                 const double ca = __shfl(cdp, dp, G), cb = __shfl(cdp, piv, G);
                 if (t == dp) cdp = cb;
                 if (t == piv) cdp = ca;
+                de = cb;  // the pivot row's entry, now on lane dp
             }
-            const double de = __shfl(cdp, dp, G);
             if (on) {
                 bad |= (de == 0.0);
                 const double mult = cdp / de;
+                const bool upd = t != dp;
+                const int nx = dp + 1;
 #pragma unroll
-                for (int c = 0; c < G; c++) {
-                    if (c < mmax) {
-                        const double key = __shfl(row[c], dp, G);
-                        if (t != dp && c < m) row[c] = fma(-mult, key, row[c]);
+                for (int c0 = 0; c0 < G; c0 += CH) {
+                    if (c0 < mmax) {
+                        double key[CH];
+#pragma unroll
+                        for (int j = 0; j < CH; j++) key[j] = __shfl(row[c0 + j], dp, G);
+                        // columns at and beyond m never reach this row's solution, the
+                        // pivot search or another row's columns below m: updated as well
+#pragma unroll
+                        for (int j = 0; j < CH; j++)
+                            if (upd) row[c0 + j] = fma(-mult, key[j], row[c0 + j]);
+                        // the lane's entry in the next pivot column, after this step
+                        if (nx >= c0 && nx < c0 + CH) {
+#pragma unroll
+                            for (int j = 0; j < CH; j++)
+                                if (nx == c0 + j) cdp = row[c0 + j];
+                        }
                     }
                 }
                 const double rk = __shfl(rhs, dp, G);
-                if (t != dp) rhs = fma(-mult, rk, rhs);
+                if (upd) rhs = fma(-mult, rk, rhs);
             }
-            // the lane's entry in the next pivot column, after this step
-#pragma unroll
-            for (int c = 1; c < G; c++)
-                if (c == dp + 1) cdp = row[c];
         }
         double d = 1.0;
 #pragma unroll
@@ -843,54 +900,66 @@ __global__ __launch_bounds__(WAVE) void k_factor(DevPattern P, int batch, const 
     if (g_status && lane == 0) g_status[sys] = bad ? BILU_ZERO_PIVOT : 0;
 }
 
-// ISAI of a batch of typed views; out positions outside the view untouched
-__global__ __launch_bounds__(WAVE) void k_isai(DevPattern P, int batch, const double *__restrict__ gV,
-                                              int type, double *__restrict__ gOut,
-                                              int *__restrict__ g_status)
+// ISAI of a batch of typed views; out positions outside the view untouched.
+// One system per workgroup of ISAI_WAVES waves sharing the values in LDS; wave
+// w takes row pairs (2w, 2w + 1), (2w + 2 ISAI_WAVES, ...), ...
+constexpr int ISAI_WAVES = 4;
+
+__global__ __launch_bounds__(WAVE * ISAI_WAVES) void k_isai(DevPattern P, int batch, const double *__restrict__ gV,
+                                                           int type, double *__restrict__ gOut,
+                                                           int *__restrict__ g_status)
 {
     extern __shared__ double lds[];
+    __shared__ int s_bad;
     const int sys = blockIdx.x;
     if (sys >= batch) return;
-    const int lane = threadIdx.x, nnz = P.nnz;
+    const int tid = threadIdx.x, lane = tid % WAVE, w = tid / WAVE, nnz = P.nnz;
+    constexpr int NT = WAVE * ISAI_WAVES;
     double *sV = lds, *sO = lds + nnz;
     const double *Vp = gV + (size_t)sys * nnz;
     double *Op = gOut + (size_t)sys * nnz;
-    wave_copy(sV, Vp, nnz, lane);
+    if (tid == 0) s_bad = 0;
+    for (int e = tid; e < nnz; e += NT) sV[e] = Vp[e];
     int bad;
     if (type == BILU_ISAI_GENERAL) {
         // every position is written once: straight to global memory (LDS holds the values only)
         __syncthreads();
-        bad = isai_general(P.isai[type], P.n, sV, Op, lane);
+        bad = isai_general(P.isai[type], P.n, sV, Op, lane, 2 * w, 2 * ISAI_WAVES);
     } else {
-        wave_copy(sO, Op, nnz, lane);
+        for (int e = tid; e < nnz; e += NT) sO[e] = Op[e];
         __syncthreads();
-        bad = isai_tri(P.isai[type], P.n, type, sV, sO, nullptr, lane);
-        __syncthreads();
-        for (int e = lane; e < nnz; e += WAVE) Op[e] = sO[e];
+        bad = isai_tri(P.isai[type], P.n, type, sV, sO, nullptr, lane, 2 * w, 2 * ISAI_WAVES);
+        for (int e = tid; e < nnz; e += NT) Op[e] = sO[e];
     }
-    if (g_status && lane == 0) g_status[sys] = bad ? BILU_ZERO_PIVOT : 0;
+    if (lane == 0 && bad) atomicOr(&s_bad, 1);
+    __syncthreads();
+    if (g_status && tid == 0) g_status[sys] = s_bad ? BILU_ZERO_PIVOT : 0;
 }
 
-// ISAI on a chosen pattern: sV[nnz] | sO[Q.nnz]
-__global__ __launch_bounds__(WAVE) void k_isai_pow(DevPattern P, DevIsaiPow Q, int batch,
-                                                  const double *__restrict__ gV, int type,
-                                                  double *__restrict__ gOut, int *__restrict__ g_status)
+// ISAI on a chosen pattern: sV[nnz] | sO[Q.nnz], workgroups as in k_isai
+__global__ __launch_bounds__(WAVE * ISAI_WAVES) void k_isai_pow(DevPattern P, DevIsaiPow Q, int batch,
+                                                               const double *__restrict__ gV, int type,
+                                                               double *__restrict__ gOut, int *__restrict__ g_status)
 {
     extern __shared__ double lds[];
+    __shared__ int s_bad;
     const int sys = blockIdx.x;
     if (sys >= batch) return;
-    const int lane = threadIdx.x, nnz = P.nnz;
+    const int tid = threadIdx.x, lane = tid % WAVE, w = tid / WAVE, nnz = P.nnz;
+    constexpr int NT = WAVE * ISAI_WAVES;
     double *sV = lds, *sO = lds + nnz;
     const double *Vp = gV + (size_t)sys * nnz;
     double *Op = gOut + (size_t)sys * Q.nnz;
-    wave_copy(sV, Vp, nnz, lane);
+    if (tid == 0) s_bad = 0;
+    for (int e = tid; e < nnz; e += NT) sV[e] = Vp[e];
     __syncthreads();
     int bad;
-    if (type == BILU_ISAI_GENERAL) bad = isai_general(Q.map, P.n, sV, sO, lane);
-    else bad = isai_tri(Q.map, P.n, type, sV, sO, nullptr, lane);
+    if (type == BILU_ISAI_GENERAL) bad = isai_general(Q.map, P.n, sV, sO, lane, 2 * w, 2 * ISAI_WAVES);
+    else bad = isai_tri(Q.map, P.n, type, sV, sO, nullptr, lane, 2 * w, 2 * ISAI_WAVES);
+    for (int e = tid; e < Q.nnz; e += NT) Op[e] = sO[e];
+    if (lane == 0 && bad) atomicOr(&s_bad, 1);
     __syncthreads();
-    for (int e = lane; e < Q.nnz; e += WAVE) Op[e] = sO[e];
-    if (g_status && lane == 0) g_status[sys] = bad ? BILU_ZERO_PIVOT : 0;
+    if (g_status && tid == 0) g_status[sys] = s_bad ? BILU_ZERO_PIVOT : 0;
 }
 
 // ||(L U - A)|_P||_F per system, F in place (unit-diagonal L)
@@ -1805,7 +1874,7 @@ static int solve_isai_fast(bilu_pattern_s &h, int batch, const double *A, const 
         const size_t lds = (size_t)nnz * sizeof(double);
         if (lds > 65536) HIP_TRY(hipFuncSetAttribute((const void *)k_isai, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         timing_mark(BILU_TIMING_FACTOR, st, true);
-        hipLaunchKernelGGL(k_isai, dim3(batch), dim3(WAVE), lds, st, h.dev, batch, A, (int)BILU_ISAI_GENERAL, M, Z);
+        hipLaunchKernelGGL(k_isai, dim3(batch), dim3(WAVE * ISAI_WAVES), lds, st, h.dev, batch, A, (int)BILU_ISAI_GENERAL, M, Z);
         timing_mark(BILU_TIMING_FACTOR, st, false);
         HIP_TRY(hipGetLastError());
         if (fstatus) hipLaunchKernelGGL(k_fill_int, dim3((batch + 255) / 256), dim3(256), 0, st, fstatus, 0, batch);
@@ -2304,7 +2373,7 @@ int bilu_isai(bilu_pattern_t h, int batch, const double *d_vals, int type, doubl
     if (lds > h->max_lds) return BILU_ERR_TOO_LARGE;
     HIP_TRY(hipSetDevice(h->device));
     if (lds > 65536) HIP_TRY(hipFuncSetAttribute((const void *)k_isai, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL(k_isai, dim3(batch), dim3(WAVE), lds, (hipStream_t)stream, h->dev, batch, d_vals, type, d_out, d_status);
+    hipLaunchKernelGGL(k_isai, dim3(batch), dim3(WAVE * ISAI_WAVES), lds, (hipStream_t)stream, h->dev, batch, d_vals, type, d_out, d_status);
     HIP_TRY(hipGetLastError());
     return BILU_OK;
 }
@@ -2337,7 +2406,7 @@ int bilu_isai_power(bilu_pattern_t h, int batch, const double *d_vals, int type,
     if (lds > h->max_lds) return BILU_ERR_TOO_LARGE;
     HIP_TRY(hipSetDevice(h->device));
     if (lds > 65536) HIP_TRY(hipFuncSetAttribute((const void *)k_isai_pow, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL(k_isai_pow, dim3(batch), dim3(WAVE), lds, (hipStream_t)stream, h->dev, q->dev, batch, d_vals,
+    hipLaunchKernelGGL(k_isai_pow, dim3(batch), dim3(WAVE * ISAI_WAVES), lds, (hipStream_t)stream, h->dev, q->dev, batch, d_vals,
                        type, d_out, d_status);
     HIP_TRY(hipGetLastError());
     return BILU_OK;

@@ -197,6 +197,33 @@ def test_isai_bitwise(typ):
     assert np.array_equal(out.cpu().numpy(), want) and not st.cpu().numpy().any()
 
 
+@pytest.mark.parametrize("n", [32, 144, 300])
+def test_isai_general_pivoting_bitwise(n):
+    """general ISAI on blocks without diagonal dominance, so Gauss-Jordan exchanges rows: values
+    +-1..9 (equal magnitudes tie and the pivot tree's order decides; at n = 144 the tree picks
+    another row than the first largest in about 1% of the steps) and standard normal"""
+    print(f"case isai general pivoting n={n}", flush=True)
+    pat, _, _ = synth.batch("random", 1, n=n)
+    rng = np.random.default_rng(7)
+    B = 32
+    ints = np.concatenate([np.arange(1.0, 10.0), -np.arange(1.0, 10.0)])
+    A = np.concatenate([rng.choice(ints, size=(B // 2, pat.nnz)),
+                        rng.standard_normal((B // 2, pat.nnz))])
+    P = bilu.Pattern.from_synth(pat)
+    out = torch.zeros(B, pat.nnz, dtype=torch.float64, device=dev())
+    st = torch.empty(B, dtype=torch.int32, device=dev())
+    bilu.isai(P, to_dev(A), bilu.ISAI_GENERAL, out, st)
+    torch.cuda.synchronize()
+    res = [O.isai(pat, A[s], O.ISAI_GENERAL) for s in range(B)]
+    want = np.stack([r[0] for r in res])
+    ost = np.array([r[1] for r in res])
+    gst = st.cpu().numpy()
+    assert np.array_equal(gst != 0, ost != 0)
+    ok = ost == 0
+    assert ok.sum() >= B - 4
+    assert np.array_equal(out.cpu().numpy()[ok].view(np.int64), want[ok].view(np.int64))
+
+
 def test_true_residual_kernel():
     pat, A, b = synth.batch("random", 16, n=144)
     x = np.random.default_rng(0).standard_normal(b.shape)
