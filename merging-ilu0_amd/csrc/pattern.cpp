@@ -35,6 +35,17 @@ static void build_levels(int n, const std::vector<int> &lev, std::vector<int> &p
     for (int i = 0; i < n; i++) rows[fill[lev[i]]++] = i;
 }
 
+void isai_row_order(IsaiMap &M)
+{
+    const int n = int(M.m.size());
+    M.order.resize(n);
+    for (int i = 0; i < n; i++) M.order[i] = i;
+    std::stable_sort(M.order.begin(), M.order.end(), [&](int a, int b) { return M.m[a] > M.m[b]; });
+    int big = 0;
+    while (big < n && M.m[M.order[big]] > 16) big++;
+    M.n_pair = std::min(n, (big + 1) / 2 * 2);
+}
+
 static void build_isai(const PatternAnalysis &pa, int type, IsaiMap &M)
 {
     const int n = pa.n;
@@ -68,6 +79,7 @@ static void build_isai(const PatternAnalysis &pa, int type, IsaiMap &M)
         }
         M.goff[i + 1] = int(M.gather.size());
     }
+    isai_row_order(M);
 }
 
 // Rows of P^k by unions: row i of P^k = the union over j in row i of P^(k-1)
@@ -140,6 +152,7 @@ void build_isai_on(const PatternAnalysis &pa, int type, const std::vector<int> &
         }
         M.goff[i + 1] = int(M.gather.size());
     }
+    isai_row_order(M);
 }
 
 int analyse_pattern(int n, int nnz, const int *rp, const int *ci, PatternAnalysis &pa,

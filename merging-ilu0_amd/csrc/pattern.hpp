@@ -16,7 +16,15 @@ struct IsaiMap {
     // -1 for a structural zero, -2 for the implicit unit diagonal of L.
     std::vector<int> beg, m, goff, gather, rhs_pos;
     int max_m = 0;
+    // rows by decreasing m (ties by row index); the first n_pair rows (those
+    // with m > 16, rounded up to an even count) are solved two per wave, the
+    // rest four per wave
+    std::vector<int> order;
+    int n_pair = 0;
 };
+
+// fills M.order / M.n_pair from M.m
+void isai_row_order(IsaiMap &M);
 
 // Schedule tables of the persistent multi-wave kernels (global memory; the
 // waves read them through the caches), plus the per-lane SpMV slot layout

@@ -46,6 +46,8 @@ enum FactorKind { FK_ILU = 0, FK_PARILU = 1, FK_LOAD = 2, FK_NONE = 3, FK_ILU_EN
 
 struct DevIsai {
     const int *beg, *m, *goff, *gather, *rhs;
+    const int *order;  // rows by decreasing m: n_pair rows two per wave, then four per wave
+    int n_pair;
 };
 
 // an ISAI on a chosen pattern (powers > 1): the pattern and its gather map
@@ -546,30 +548,39 @@ __device__ __forceinline__ double row_shl_f64(double v)
     return __hiloint2double(row_shl_i32<S>(__double2hiint(v)), row_shl_i32<S>(__double2loint(v)));
 }
 
-// General ISAI rows by Gauss-Jordan, two rows per wave (32 lanes each, lane t
-// holding row t of the m x m block); steps and columns stop at the larger m of
-// the two rows (beyond a row's m nothing changes its solution).  Row pairs as
-// in isai_tri.  Every loop over the block's columns is unrolled with constant
-// indices, so the block stays in registers; the lane's entry in the pivot
-// column is carried in cdp (picked up as the previous step's update passes
-// it) instead of indexing row[] by dp.  The pivot search is the reference's
-// 32-lane shfl_down tree: the step at distance 16 crosses rows of 16 lanes
-// (ds_bpermute), the steps at 8, 4, 2, 1 stay inside a row (DPP row_shl).
-// Only lanes below the distance are read by later steps, and each of them
-// reads a source inside its own row, so the DPP steps give the tree's values.
-__device__ int isai_general(const DevIsai &M, int n, const double *vals, double *out, int lane,
-                            int p0 = 0, int pstep = 2)
+// General ISAI rows by Gauss-Jordan, one row per group of G lanes (lane t
+// holding row t of the m x m block, m <= G): rows order[r0 .. r1) taken
+// WAVE / G at a time, batch q = item0, item0 + istep, ...; steps and columns
+// stop at the largest m of the wave's rows (beyond a row's m nothing changes
+// its solution).  Every loop over the block's columns is unrolled with
+// constant indices, so the block stays in registers; the lane's entry in the
+// pivot column is carried in cdp (picked up as the previous step's update
+// passes it) instead of indexing row[] by dp.  The pivot search is the
+// reference's 32-lane shfl_down tree: only lanes below the distance are read
+// by later steps, and each of them reads a source inside its own row of 16
+// lanes at distances 8, 4, 2, 1 (DPP row_shl).  At distance 16 a 32-lane group
+// reads across rows (ds_bpermute); a 16-lane group (m <= 16) would read lanes
+// t + 16 >= m, which all hold the stand-in (dval, dp): a local comparison.
+template <int G>
+__device__ int isai_general_g(const DevIsai &M, const double *vals, double *out, int lane, int r0, int r1,
+                              int item0, int istep)
 {
-    constexpr int G = ISAI_GROUP;
-    constexpr int CH = 8;  // columns per batch of pivot-row broadcasts
+    constexpr int RPW = WAVE / G;
+    constexpr int CH = 8;  // columns per batch of loads and pivot-row broadcasts
     const int grp = lane / G, t = lane % G;
     int bad = 0;
-    for (int i0 = p0; i0 < n; i0 += pstep) {
-        const int i = i0 + grp;
-        const bool act = i < n;
+    for (int q = item0; r0 + q * RPW < r1; q += istep) {
+        const int ri = r0 + q * RPW + grp;
+        const bool act = ri < r1;
+        const int i = act ? M.order[ri] : 0;
         const int m = act ? M.m[i] : 0;
-        const int mo = __shfl_xor(m, G, WAVE);
-        const int mmax = __builtin_amdgcn_readfirstlane(m > mo ? m : mo);  // same on every lane
+        int mx = m;
+#pragma unroll
+        for (int o = G; o < WAVE; o <<= 1) {
+            const int mo = __shfl_xor(mx, o, WAVE);
+            mx = mx > mo ? mx : mo;
+        }
+        const int mmax = __builtin_amdgcn_readfirstlane(mx);  // same on every lane
         const int *g = M.gather + (act ? M.goff[i] : 0);
         // unconditional loads at clamped positions (a batch in flight at once)
         const int last = m > 0 ? m * m - 1 : 0;
@@ -597,10 +608,12 @@ __device__ int isai_general(const DevIsai &M, int n, const double *vals, double 
             const double dval = __shfl(cdp, dp, G);
             double val = (t >= m || t < dp) ? dval : cdp;
             int piv = (t >= m || t < dp) ? dp : t;
-            {
+            if constexpr (G == 32) {
                 const double vo = __shfl_down(val, 16, G);
                 const int po = __shfl_down(piv, 16, G);
                 if (fabs(vo) > fabs(val)) { val = vo; piv = po; }
+            } else {
+                if (fabs(dval) > fabs(val)) { val = dval; piv = dp; }
             }
 #define BILU_PIVOT_DPP(S)                                                                          \
             {                                                                                      \
@@ -613,9 +626,11 @@ __device__ int isai_general(const DevIsai &M, int n, const double *vals, double 
             BILU_PIVOT_DPP(2)
             BILU_PIVOT_DPP(1)
 #undef BILU_PIVOT_DPP
-            {
+            if constexpr (G == 32) {
                 const int pa = __builtin_amdgcn_readlane(piv, 0), pb = __builtin_amdgcn_readlane(piv, G);
                 piv = grp ? pb : pa;
+            } else {
+                piv = __shfl(piv, 0, G);
             }
             if (!on) piv = dp;
             double de = dval;
@@ -671,6 +686,16 @@ __device__ int isai_general(const DevIsai &M, int n, const double *vals, double 
             if (k == t) d = row[k];
         if (act && t < m) out[M.beg[i] + t] = rhs / d;
     }
+    return bad;
+}
+
+// every row of the map: wave w of nw (of one workgroup, or w = 0, nw = 1)
+// takes every nw-th batch of the paired rows, then of the four-per-wave rows
+__device__ int isai_general(const DevIsai &M, int n, const double *vals, double *out, int lane, int w = 0,
+                            int nw = 1)
+{
+    int bad = isai_general_g<32>(M, vals, out, lane, 0, M.n_pair, w, nw);
+    bad |= isai_general_g<16>(M, vals, out, lane, M.n_pair, n, w, nw);
     __syncthreads();
     return __any(bad);
 }
@@ -901,8 +926,9 @@ __global__ __launch_bounds__(WAVE) void k_factor(DevPattern P, int batch, const 
 }
 
 // ISAI of a batch of typed views; out positions outside the view untouched.
-// One system per workgroup of ISAI_WAVES waves sharing the values in LDS; wave
-// w takes row pairs (2w, 2w + 1), (2w + 2 ISAI_WAVES, ...), ...
+// One system per workgroup of ISAI_WAVES waves sharing the values in LDS; the
+// waves split the rows (general: every ISAI_WAVES-th batch of the row order;
+// lower / upper: wave w takes row pairs (2w, 2w + 1), (2w + 2 ISAI_WAVES, ...), ...)
 constexpr int ISAI_WAVES = 4;
 
 __global__ __launch_bounds__(WAVE * ISAI_WAVES) void k_isai(DevPattern P, int batch, const double *__restrict__ gV,
@@ -924,7 +950,7 @@ __global__ __launch_bounds__(WAVE * ISAI_WAVES) void k_isai(DevPattern P, int ba
     if (type == BILU_ISAI_GENERAL) {
         // every position is written once: straight to global memory (LDS holds the values only)
         __syncthreads();
-        bad = isai_general(P.isai[type], P.n, sV, Op, lane, 2 * w, 2 * ISAI_WAVES);
+        bad = isai_general(P.isai[type], P.n, sV, Op, lane, w, ISAI_WAVES);
     } else {
         for (int e = tid; e < nnz; e += NT) sO[e] = Op[e];
         __syncthreads();
@@ -954,7 +980,7 @@ __global__ __launch_bounds__(WAVE * ISAI_WAVES) void k_isai_pow(DevPattern P, De
     for (int e = tid; e < nnz; e += NT) sV[e] = Vp[e];
     __syncthreads();
     int bad;
-    if (type == BILU_ISAI_GENERAL) bad = isai_general(Q.map, P.n, sV, sO, lane, 2 * w, 2 * ISAI_WAVES);
+    if (type == BILU_ISAI_GENERAL) bad = isai_general(Q.map, P.n, sV, sO, lane, w, ISAI_WAVES);
     else bad = isai_tri(Q.map, P.n, type, sV, sO, nullptr, lane, 2 * w, 2 * ISAI_WAVES);
     for (int e = tid; e < Q.nnz; e += NT) Op[e] = sO[e];
     if (lane == 0 && bad) atomicOr(&s_bad, 1);
@@ -1162,11 +1188,12 @@ int upload(bilu_pattern_s &h)
     size_t o_lep = put(pa.lev_e_ptr), o_lee = put(nz(pa.lev_e_ent)), o_edp = put(pa.era_div_ptr),
            o_ed = put(nz(pa.era_div)), o_etp = put(pa.era_tgt_ptr), o_et = put(nz(pa.era_tgt)),
            o_tpp = put(pa.tgt_pair_ptr), o_tp = put(nz(pa.tgt_pair));
-    size_t o_is[3][5];
+    size_t o_is[3][6];
     for (int t = 0; t < 3; t++) {
         const auto &M = pa.isai[t];
         o_is[t][0] = put(M.beg); o_is[t][1] = put(M.m); o_is[t][2] = put(M.goff);
         o_is[t][3] = put(M.gather.empty() ? std::vector<int>{0} : M.gather); o_is[t][4] = put(M.rhs_pos);
+        o_is[t][5] = put(M.order);
     }
     HIP_TRY(hipMalloc((void **)&h.dbuf, buf.size() * sizeof(int)));
     HIP_TRY(hipMemcpy(h.dbuf, buf.data(), buf.size() * sizeof(int), hipMemcpyHostToDevice));
@@ -1190,6 +1217,7 @@ int upload(bilu_pattern_s &h)
     for (int t = 0; t < 3; t++) {
         d.isai[t].beg = B + o_is[t][0]; d.isai[t].m = B + o_is[t][1]; d.isai[t].goff = B + o_is[t][2];
         d.isai[t].gather = B + o_is[t][3]; d.isai[t].rhs = B + o_is[t][4];
+        d.isai[t].order = B + o_is[t][5]; d.isai[t].n_pair = pa.isai[t].n_pair;
     }
     HIP_TRY(hipMalloc((void **)&h.d_counter, 256));
     HIP_TRY(hipMemset(h.d_counter, 0, 256));
@@ -1283,7 +1311,7 @@ int get_pow_isai(bilu_pattern_s &h, int type, int power, const bilu_pattern_s::P
     std::vector<int> buf;
     auto put = [&](const std::vector<int> &v) { const size_t o = buf.size(); buf.insert(buf.end(), v.begin(), v.end()); return o; };
     const size_t o_rp = put(q.rp), o_ci = put(q.ci), o_beg = put(M.beg), o_m = put(M.m), o_goff = put(M.goff),
-                 o_g = put(M.gather), o_rhs = put(M.rhs_pos);
+                 o_g = put(M.gather), o_rhs = put(M.rhs_pos), o_ord = put(M.order);
     HIP_TRY(hipSetDevice(h.device));
     HIP_TRY(hipMalloc((void **)&q.dbuf, std::max<size_t>(buf.size(), 1) * sizeof(int)));
     HIP_TRY(hipMemcpy(q.dbuf, buf.data(), buf.size() * sizeof(int), hipMemcpyHostToDevice));
@@ -1291,6 +1319,7 @@ int get_pow_isai(bilu_pattern_s &h, int type, int power, const bilu_pattern_s::P
     q.dev.rp = q.dbuf + o_rp; q.dev.ci = q.dbuf + o_ci;
     q.dev.map.beg = q.dbuf + o_beg; q.dev.map.m = q.dbuf + o_m; q.dev.map.goff = q.dbuf + o_goff;
     q.dev.map.gather = q.dbuf + o_g; q.dev.map.rhs = q.dbuf + o_rhs;
+    q.dev.map.order = q.dbuf + o_ord; q.dev.map.n_pair = M.n_pair;
     h.pow_isai.push_back(std::move(q));
     out = &h.pow_isai.back();
     return BILU_OK;
