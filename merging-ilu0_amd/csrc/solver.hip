@@ -535,11 +535,11 @@ __device__ int isai_tri(const DevIsai &M, int n, int type, const double *vals, d
 }
 
 // DPP row_shl:S -- lane l reads lane l + S of its row of 16 lanes (a lane whose
-// source falls outside the row keeps its own value)
+// source falls outside the row reads 0; callers use only lanes l < 16 - S)
 template <int S>
 __device__ __forceinline__ int row_shl_i32(int v)
 {
-    return __builtin_amdgcn_update_dpp(v, v, 0x100 | S, 0xF, 0xF, false);
+    return __builtin_amdgcn_mov_dpp(v, 0x100 | S, 0xF, 0xF, true);
 }
 
 template <int S>
@@ -665,9 +665,10 @@ __device__ int isai_general_g(const DevIsai &M, const double *vals, double *out,
                         for (int j = 0; j < CH; j++) key[j] = __shfl(row[c0 + j], dp, G);
                         // columns at and beyond m never reach this row's solution, the
                         // pivot search or another row's columns below m: updated as well
+                        if (upd) {
 #pragma unroll
-                        for (int j = 0; j < CH; j++)
-                            if (upd) row[c0 + j] = fma(-mult, key[j], row[c0 + j]);
+                            for (int j = 0; j < CH; j++) row[c0 + j] = fma(-mult, key[j], row[c0 + j]);
+                        }
                         // the lane's entry in the next pivot column, after this step
                         if (nx >= c0 && nx < c0 + CH) {
 #pragma unroll
