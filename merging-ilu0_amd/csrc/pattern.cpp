@@ -66,8 +66,8 @@ static void build_isai(const PatternAnalysis &pa, int type, IsaiMap &M)
         M.max_m = std::max(M.max_m, m);
         for (int q = 0; q < m; q++)
             if (pa.ci[b + q] == i) M.rhs_pos[i] = q;
-        for (int t = 0; t < m; t++) {
-            for (int k = 0; k < m; k++) {
+        for (int k = 0; k < m; k++) {      // k-major: lane t reads k * m + t
+            for (int t = 0; t < m; t++) {
                 const int r = pa.ci[b + k], c = pa.ci[b + t];  // entry (J[k], J[t])
                 int g;
                 if (type == BILU_ISAI_LOWER && r == c) g = -2;
@@ -139,8 +139,8 @@ void build_isai_on(const PatternAnalysis &pa, int type, const std::vector<int> &
         M.max_m = std::max(M.max_m, m);
         for (int q = 0; q < m; q++)
             if (mci[b + q] == i) M.rhs_pos[i] = q;
-        for (int t = 0; t < m; t++) {
-            for (int k = 0; k < m; k++) {
+        for (int k = 0; k < m; k++) {      // k-major: lane t reads k * m + t
+            for (int t = 0; t < m; t++) {
                 const int r = mci[b + k], c = mci[b + t];  // entry (J[k], J[t]) of the typed view
                 int g;
                 if (type == BILU_ISAI_LOWER && r == c) g = -2;

@@ -11,8 +11,9 @@ namespace bilu {
 // one triangular / general view of the pattern for ISAI (power 1)
 struct IsaiMap {
     // rows: J = columns of row i inside the view, stored as positions
-    // beg[i]..beg[i]+m[i]-1 of the pattern; gather[goff[i] + t*m + k] is the
-    // pattern position of (J[k], J[t]) (the transposed dense block A(J,J)^T),
+    // beg[i]..beg[i]+m[i]-1 of the pattern; gather[goff[i] + k*m + t] is the
+    // pattern position of (J[k], J[t]) (the transposed dense block A(J,J)^T:
+    // lane t holds its row t, and the lanes of one k read consecutive ints),
     // -1 for a structural zero, -2 for the implicit unit diagonal of L.
     std::vector<int> beg, m, goff, gather, rhs_pos;
     int max_m = 0;

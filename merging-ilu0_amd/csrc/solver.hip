@@ -29,6 +29,7 @@
 #include <mutex>
 #include <new>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/bilu.h"
@@ -509,7 +510,7 @@ __device__ int isai_tri(const DevIsai &M, int n, int type, const double *vals, d
                     s = s / d;
                 }
                 const double bot = __shfl(s, c, ISAI_GROUP);
-                if (on && t < c) s = fma(-bot, view_val(vals, g[t * m + c]), s);
+                if (on && t < c) s = fma(-bot, view_val(vals, g[c * m + t]), s);
             }
         } else {
             s = (t == 0) ? 1.0 : 0.0;
@@ -521,7 +522,7 @@ __device__ int isai_tri(const DevIsai &M, int n, int type, const double *vals, d
                     s = s / d;
                 }
                 const double top = __shfl(s, c, ISAI_GROUP);
-                if (on && t > c && t < m) s = fma(-top, view_val(vals, g[t * m + c]), s);
+                if (on && t > c && t < m) s = fma(-top, view_val(vals, g[c * m + t]), s);
             }
         }
         if (act && t < m) {
@@ -548,25 +549,81 @@ __device__ __forceinline__ double row_shl_f64(double v)
     return __hiloint2double(row_shl_i32<S>(__double2hiint(v)), row_shl_i32<S>(__double2loint(v)));
 }
 
+// DPP row_newbcast:N -- every lane reads lane N of its row of 16 lanes
+template <int N>
+__device__ __forceinline__ int row_bcast_i32(int v)
+{
+    return __builtin_amdgcn_mov_dpp(v, 0x150 | N, 0xF, 0xF, false);
+}
+
+template <int N>
+__device__ __forceinline__ double row_bcast_f64(double v)
+{
+    return __hiloint2double(row_bcast_i32<N>(__double2hiint(v)), row_bcast_i32<N>(__double2loint(v)));
+}
+
+// value of lane N of the lane's group of G lanes (G = 16: a DPP row; G = 32:
+// rows 2r and 2r + 1 -- row_newbcast within each row, then the permlane16 swap
+// hands the row that holds lane N to both rows)
+template <int G, int N>
+__device__ __forceinline__ int group_bcast_i32(int v)
+{
+    const int b = row_bcast_i32<N % 16>(v);
+    if constexpr (G == 16) {
+        return b;
+    } else {
+        const auto sw = __builtin_amdgcn_permlane16_swap(b, b, false, false);
+        return N < 16 ? (int)sw[0] : (int)sw[1];
+    }
+}
+
+template <int G, int N>
+__device__ __forceinline__ double group_bcast(double v)
+{
+    return __hiloint2double(group_bcast_i32<G, N>(__double2hiint(v)), group_bcast_i32<G, N>(__double2loint(v)));
+}
+
+// int value of lane (lane ^ 16)
+__device__ __forceinline__ int lane_xor16_i32(int v)
+{
+    const auto a = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+    return (__lane_id() & 16) == 0 ? (int)a[1] : (int)a[0];
+}
+
+// f(std::integral_constant<int, I>()) for I = B .. E - 1 (an unrolled loop with constant indices)
+template <int B, int E, class F>
+__device__ __forceinline__ void static_for(F &&f)
+{
+    if constexpr (B < E) {
+        f(std::integral_constant<int, B>());
+        static_for<B + 1, E>(f);
+    }
+}
+
 // General ISAI rows by Gauss-Jordan, one row per group of G lanes (lane t
 // holding row t of the m x m block, m <= G): rows order[r0 .. r1) taken
 // WAVE / G at a time, batch q = item0, item0 + istep, ...; steps and columns
 // stop at the largest m of the wave's rows (beyond a row's m nothing changes
 // its solution).  Every loop over the block's columns is unrolled with
-// constant indices, so the block stays in registers; the lane's entry in the
-// pivot column is carried in cdp (picked up as the previous step's update
-// passes it) instead of indexing row[] by dp.  The pivot search is the
-// reference's 32-lane shfl_down tree: only lanes below the distance are read
-// by later steps, and each of them reads a source inside its own row of 16
-// lanes at distances 8, 4, 2, 1 (DPP row_shl).  At distance 16 a 32-lane group
-// reads across rows (ds_bpermute); a 16-lane group (m <= 16) would read lanes
-// t + 16 >= m, which all hold the stand-in (dval, dp): a local comparison.
-template <int G>
-__device__ int isai_general_g(const DevIsai &M, const double *vals, double *out, int lane, int r0, int r1,
+// constant indices, so the block stays in registers.  UNR: the G steps are
+// unrolled too, and every broadcast from the pivot lane is a DPP / permlane
+// move (group_bcast) -- no LDS traffic.  Otherwise (the one-wave solve
+// kernels, G = 32) the step loop is rolled, broadcasts go through ds_bpermute
+// and the lane's entry in the pivot column is carried in cdp (picked up as the
+// previous step's update passes it).  The pivot search is the reference's
+// 32-lane shfl_down tree: only lanes below the distance are read by later
+// steps, and each of them reads a source inside its own row of 16 lanes at
+// distances 8, 4, 2, 1 (DPP row_shl).  At distance 16 a 32-lane group reads
+// the other row (lanes t < 16 read t + 16: ds_bpermute, or the permlane16
+// swap); a 16-lane group (m <= 16) would read lanes t + 16 >= m, which all
+// hold the stand-in (dval, dp): a local comparison.
+template <int G, bool UNR>
+__device__ __forceinline__ int isai_general_g(const DevIsai &M, const double *vals, double *out, int lane, int r0, int r1,
                               int item0, int istep)
 {
+    static_assert(UNR || G == 32, "the rolled step loop is the 32-lane one");
     constexpr int RPW = WAVE / G;
-    constexpr int CH = 8;  // columns per batch of loads and pivot-row broadcasts
+    constexpr int CH = (UNR && G == 32) ? 4 : 8;  // columns per batch of loads and pivot-row broadcasts
     const int grp = lane / G, t = lane % G;
     int bad = 0;
     for (int q = item0; r0 + q * RPW < r1; q += istep) {
@@ -590,7 +647,7 @@ __device__ int isai_general_g(const DevIsai &M, const double *vals, double *out,
             if (k0 < mmax) {
                 int gi[CH];
 #pragma unroll
-                for (int j = 0; j < CH; j++) gi[j] = g[min(t * m + k0 + j, last)];
+                for (int j = 0; j < CH; j++) gi[j] = g[min((k0 + j) * m + t, last)];
 #pragma unroll
                 for (int j = 0; j < CH; j++) {
                     const double v = vals[max(gi[j], 0)];
@@ -602,84 +659,149 @@ __device__ int isai_general_g(const DevIsai &M, const double *vals, double *out,
             }
         }
         double rhs = (act && t == M.rhs[i]) ? 1.0 : 0.0;
-        double cdp = row[0];
-        for (int dp = 0; dp < mmax; dp++) {
-            const bool on = dp < m;  // uniform within the group
-            const double dval = __shfl(cdp, dp, G);
-            double val = (t >= m || t < dp) ? dval : cdp;
-            int piv = (t >= m || t < dp) ? dp : t;
-            if constexpr (G == 32) {
-                const double vo = __shfl_down(val, 16, G);
-                const int po = __shfl_down(piv, 16, G);
-                if (fabs(vo) > fabs(val)) { val = vo; piv = po; }
-            } else {
-                if (fabs(dval) > fabs(val)) { val = dval; piv = dp; }
-            }
+        if constexpr (!UNR) {
+            double cdp = row[0];
+            for (int dp = 0; dp < mmax; dp++) {
+                const bool on = dp < m;  // uniform within the group
+                const double dval = __shfl(cdp, dp, G);
+                double val = (t >= m || t < dp) ? dval : cdp;
+                int piv = (t >= m || t < dp) ? dp : t;
+                {
+                    const double vo = __shfl_down(val, 16, G);
+                    const int po = __shfl_down(piv, 16, G);
+                    if (fabs(vo) > fabs(val)) { val = vo; piv = po; }
+                }
 #define BILU_PIVOT_DPP(S)                                                                          \
-            {                                                                                      \
-                const double vo = row_shl_f64<S>(val);                                             \
-                const int po = row_shl_i32<S>(piv);                                                \
-                if (fabs(vo) > fabs(val)) { val = vo; piv = po; }                                  \
-            }
-            BILU_PIVOT_DPP(8)
-            BILU_PIVOT_DPP(4)
-            BILU_PIVOT_DPP(2)
-            BILU_PIVOT_DPP(1)
+                {                                                                                  \
+                    const double vo = row_shl_f64<S>(val);                                         \
+                    const int po = row_shl_i32<S>(piv);                                            \
+                    if (fabs(vo) > fabs(val)) { val = vo; piv = po; }                              \
+                }
+                BILU_PIVOT_DPP(8)
+                BILU_PIVOT_DPP(4)
+                BILU_PIVOT_DPP(2)
+                BILU_PIVOT_DPP(1)
 #undef BILU_PIVOT_DPP
-            if constexpr (G == 32) {
-                const int pa = __builtin_amdgcn_readlane(piv, 0), pb = __builtin_amdgcn_readlane(piv, G);
-                piv = grp ? pb : pa;
-            } else {
-                piv = __shfl(piv, 0, G);
-            }
-            if (!on) piv = dp;
-            double de = dval;
-            if (piv != dp) {
-#pragma unroll
-                for (int c = 0; c < G; c++) {
-                    if (c < mmax) {
-                        const double a = __shfl(row[c], dp, G);
-                        const double b = __shfl(row[c], piv, G);
-                        if (t == dp) row[c] = b;
-                        if (t == piv) row[c] = a;
-                    }
+                {
+                    const int pa = __builtin_amdgcn_readlane(piv, 0), pb = __builtin_amdgcn_readlane(piv, G);
+                    piv = grp ? pb : pa;
                 }
-                const double a = __shfl(rhs, dp, G), b = __shfl(rhs, piv, G);
-                if (t == dp) rhs = b;
-                if (t == piv) rhs = a;
-                const double ca = __shfl(cdp, dp, G), cb = __shfl(cdp, piv, G);
-                if (t == dp) cdp = cb;
-                if (t == piv) cdp = ca;
-                de = cb;  // the pivot row's entry, now on lane dp
-            }
-            if (on) {
-                bad |= (de == 0.0);
-                const double mult = cdp / de;
-                const bool upd = t != dp;
-                const int nx = dp + 1;
+                if (!on) piv = dp;
+                double de = dval;
+                if (piv != dp) {
 #pragma unroll
-                for (int c0 = 0; c0 < G; c0 += CH) {
-                    if (c0 < mmax) {
-                        double key[CH];
-#pragma unroll
-                        for (int j = 0; j < CH; j++) key[j] = __shfl(row[c0 + j], dp, G);
-                        // columns at and beyond m never reach this row's solution, the
-                        // pivot search or another row's columns below m: updated as well
-                        if (upd) {
-#pragma unroll
-                            for (int j = 0; j < CH; j++) row[c0 + j] = fma(-mult, key[j], row[c0 + j]);
-                        }
-                        // the lane's entry in the next pivot column, after this step
-                        if (nx >= c0 && nx < c0 + CH) {
-#pragma unroll
-                            for (int j = 0; j < CH; j++)
-                                if (nx == c0 + j) cdp = row[c0 + j];
+                    for (int c = 0; c < G; c++) {
+                        if (c < mmax) {
+                            const double a = __shfl(row[c], dp, G);
+                            const double b = __shfl(row[c], piv, G);
+                            if (t == dp) row[c] = b;
+                            if (t == piv) row[c] = a;
                         }
                     }
+                    const double a = __shfl(rhs, dp, G), b = __shfl(rhs, piv, G);
+                    if (t == dp) rhs = b;
+                    if (t == piv) rhs = a;
+                    const double ca = __shfl(cdp, dp, G), cb = __shfl(cdp, piv, G);
+                    if (t == dp) cdp = cb;
+                    if (t == piv) cdp = ca;
+                    de = cb;  // the pivot row's entry, now on lane dp
                 }
-                const double rk = __shfl(rhs, dp, G);
-                if (upd) rhs = fma(-mult, rk, rhs);
+                if (on) {
+                    bad |= (de == 0.0);
+                    const double mult = cdp / de;
+                    const bool upd = t != dp;
+                    const int nx = dp + 1;
+#pragma unroll
+                    for (int c0 = 0; c0 < G; c0 += CH) {
+                        if (c0 < mmax) {
+                            double key[CH];
+#pragma unroll
+                            for (int j = 0; j < CH; j++) key[j] = __shfl(row[c0 + j], dp, G);
+                            // columns at and beyond m never reach this row's solution, the
+                            // pivot search or another row's columns below m: updated as well
+                            if (upd) {
+#pragma unroll
+                                for (int j = 0; j < CH; j++) row[c0 + j] = fma(-mult, key[j], row[c0 + j]);
+                            }
+                            // the lane's entry in the next pivot column, after this step
+                            if (nx >= c0 && nx < c0 + CH) {
+#pragma unroll
+                                for (int j = 0; j < CH; j++)
+                                    if (nx == c0 + j) cdp = row[c0 + j];
+                            }
+                        }
+                    }
+                    const double rk = __shfl(rhs, dp, G);
+                    if (upd) rhs = fma(-mult, rk, rhs);
+                }
             }
+        } else {
+            // steps unrolled (dp a constant): every broadcast from lane dp is DPP
+            // row_newbcast, across the two rows of a 32-lane group with a
+            // permlane16 swap (group_bcast)
+            static_for<0, G>([&](auto DP) {
+                constexpr int dp = decltype(DP)::value;
+                if (dp >= mmax) return;
+                const bool on = dp < m;  // uniform within the group
+                const double dval = group_bcast<G, dp>(row[dp]);
+                double val = (t >= m || t < dp) ? dval : row[dp];
+                int piv = (t >= m || t < dp) ? dp : t;
+                if constexpr (G == 32) {
+                    const double vo = lane_xor16(val);  // lanes t < 16 read t + 16
+                    const int po = lane_xor16_i32(piv);
+                    if (fabs(vo) > fabs(val)) { val = vo; piv = po; }
+                } else {
+                    if (fabs(dval) > fabs(val)) { val = dval; piv = dp; }
+                }
+#define BILU_PIVOT_DPP(S)                                                                          \
+                {                                                                                  \
+                    const double vo = row_shl_f64<S>(val);                                         \
+                    const int po = row_shl_i32<S>(piv);                                            \
+                    if (fabs(vo) > fabs(val)) { val = vo; piv = po; }                              \
+                }
+                BILU_PIVOT_DPP(8)
+                BILU_PIVOT_DPP(4)
+                BILU_PIVOT_DPP(2)
+                BILU_PIVOT_DPP(1)
+#undef BILU_PIVOT_DPP
+                piv = group_bcast_i32<G, 0>(piv);
+                if (!on) piv = dp;
+                double de = dval;
+                if (piv != dp) {
+#pragma unroll
+                    for (int c = 0; c < G; c++) {
+                        if (c < mmax) {
+                            const double a = group_bcast<G, dp>(row[c]);
+                            const double b = __shfl(row[c], piv, G);
+                            if (t == dp) row[c] = b;
+                            if (t == piv) row[c] = a;
+                        }
+                    }
+                    const double a = group_bcast<G, dp>(rhs), b = __shfl(rhs, piv, G);
+                    if (t == dp) rhs = b;
+                    if (t == piv) rhs = a;
+                    de = group_bcast<G, dp>(row[dp]);  // the pivot row's entry, now on lane dp
+                }
+                if (on) {
+                    bad |= (de == 0.0);
+                    const double mult = row[dp] / de;
+                    const bool upd = t != dp;
+#pragma unroll
+                    for (int c0 = 0; c0 < G; c0 += CH) {
+                        if (c0 < mmax) {
+                            double key[CH];
+#pragma unroll
+                            for (int j = 0; j < CH; j++) key[j] = group_bcast<G, dp>(row[c0 + j]);
+                            if (upd) {
+#pragma unroll
+                                for (int j = 0; j < CH; j++) row[c0 + j] = fma(-mult, key[j], row[c0 + j]);
+                            }
+                        }
+                    }
+                    const double rk = group_bcast<G, dp>(rhs);
+                    if (upd) rhs = fma(-mult, rk, rhs);
+                }
+            });
         }
         double d = 1.0;
 #pragma unroll
@@ -690,13 +812,14 @@ __device__ int isai_general_g(const DevIsai &M, const double *vals, double *out,
     return bad;
 }
 
-// every row of the map: wave w of nw (of one workgroup, or w = 0, nw = 1)
-// takes every nw-th batch of the paired rows, then of the four-per-wave rows
-__device__ int isai_general(const DevIsai &M, int n, const double *vals, double *out, int lane, int w = 0,
+// every row of the map, two per wave in the map's row order (the fused
+// one-wave solve kernels and the power-k kernel): wave w of nw (of one
+// workgroup, or w = 0, nw = 1) takes every nw-th pair.  The batch ISAI
+// (k_isai_gen) runs the rows with m <= 16 four per wave in a launch of its own.
+__device__ __forceinline__ int isai_general(const DevIsai &M, int n, const double *vals, double *out, int lane, int w = 0,
                             int nw = 1)
 {
-    int bad = isai_general_g<32>(M, vals, out, lane, 0, M.n_pair, w, nw);
-    bad |= isai_general_g<16>(M, vals, out, lane, M.n_pair, n, w, nw);
+    const int bad = isai_general_g<32, false>(M, vals, out, lane, 0, n, w, nw);
     __syncthreads();
     return __any(bad);
 }
@@ -926,10 +1049,10 @@ __global__ __launch_bounds__(WAVE) void k_factor(DevPattern P, int batch, const 
     if (g_status && lane == 0) g_status[sys] = bad ? BILU_ZERO_PIVOT : 0;
 }
 
-// ISAI of a batch of typed views; out positions outside the view untouched.
-// One system per workgroup of ISAI_WAVES waves sharing the values in LDS; the
-// waves split the rows (general: every ISAI_WAVES-th batch of the row order;
-// lower / upper: wave w takes row pairs (2w, 2w + 1), (2w + 2 ISAI_WAVES, ...), ...)
+// ISAI(L) or ISAI(U) of a batch of typed views; out positions outside the
+// view untouched (the general ISAI: k_isai_gen).  One system per workgroup of
+// ISAI_WAVES waves sharing the values in LDS; wave w takes row pairs
+// (2w, 2w + 1), (2w + 2 ISAI_WAVES, ...), ...
 constexpr int ISAI_WAVES = 4;
 
 __global__ __launch_bounds__(WAVE * ISAI_WAVES) void k_isai(DevPattern P, int batch, const double *__restrict__ gV,
@@ -947,20 +1070,44 @@ __global__ __launch_bounds__(WAVE * ISAI_WAVES) void k_isai(DevPattern P, int ba
     double *Op = gOut + (size_t)sys * nnz;
     if (tid == 0) s_bad = 0;
     for (int e = tid; e < nnz; e += NT) sV[e] = Vp[e];
-    int bad;
-    if (type == BILU_ISAI_GENERAL) {
-        // every position is written once: straight to global memory (LDS holds the values only)
-        __syncthreads();
-        bad = isai_general(P.isai[type], P.n, sV, Op, lane, w, ISAI_WAVES);
-    } else {
-        for (int e = tid; e < nnz; e += NT) sO[e] = Op[e];
-        __syncthreads();
-        bad = isai_tri(P.isai[type], P.n, type, sV, sO, nullptr, lane, 2 * w, 2 * ISAI_WAVES);
-        for (int e = tid; e < nnz; e += NT) Op[e] = sO[e];
-    }
+    for (int e = tid; e < nnz; e += NT) sO[e] = Op[e];
+    __syncthreads();
+    const int bad = isai_tri(P.isai[type], P.n, type, sV, sO, nullptr, lane, 2 * w, 2 * ISAI_WAVES);
+    for (int e = tid; e < nnz; e += NT) Op[e] = sO[e];
     if (lane == 0 && bad) atomicOr(&s_bad, 1);
     __syncthreads();
     if (g_status && tid == 0) g_status[sys] = s_bad ? BILU_ZERO_PIVOT : 0;
+}
+
+// General ISAI of a batch, one launch per row group: rows order[r0 .. r1) of
+// the map with G lanes per row (r0 = 0, r1 = n_pair with G = 32; then the
+// rest with G = 16, a launch of its own so that its smaller register block
+// sets its occupancy).  Workgroups as in k_isai.  The status of the second
+// launch (or_status = 1) keeps a zero pivot the first one found.
+template <int G>
+__global__ __launch_bounds__(WAVE * ISAI_WAVES) void k_isai_gen(DevIsai M, int nnz, int batch,
+                                                               const double *__restrict__ gV,
+                                                               double *__restrict__ gOut,
+                                                               int *__restrict__ g_status, int r0, int r1,
+                                                               int or_status)
+{
+    extern __shared__ double lds[];
+    __shared__ int s_bad;
+    const int sys = blockIdx.x;
+    if (sys >= batch) return;
+    const int tid = threadIdx.x, lane = tid % WAVE, w = tid / WAVE;
+    constexpr int NT = WAVE * ISAI_WAVES;
+    const double *Vp = gV + (size_t)sys * nnz;
+    if (tid == 0) s_bad = 0;
+    for (int e = tid; e < nnz; e += NT) lds[e] = Vp[e];
+    __syncthreads();
+    const int bad = isai_general_g<G, true>(M, lds, gOut + (size_t)sys * nnz, lane, r0, r1, w, ISAI_WAVES);
+    if (__any(bad) && lane == 0) atomicOr(&s_bad, 1);
+    __syncthreads();
+    if (g_status && tid == 0) {
+        const bool prev = or_status && g_status[sys] != 0;
+        g_status[sys] = (s_bad || prev) ? BILU_ZERO_PIVOT : 0;
+    }
 }
 
 // ISAI on a chosen pattern: sV[nnz] | sO[Q.nnz], workgroups as in k_isai
@@ -1877,6 +2024,28 @@ int solve_ws(bilu_pattern_s &h, size_t bytes, void **out, hipStream_t st, const 
     return BILU_OK;
 }
 
+// general ISAI of a batch (values V, output on the pattern): the paired rows,
+// then the four-per-wave rows, two launches on the stream
+static int launch_isai_general(const bilu_pattern_s &h, int batch, const double *V, double *out, int *status,
+                               hipStream_t st)
+{
+    const int n = h.pa.n, nnz = h.pa.nnz, np = h.pa.isai[BILU_ISAI_GENERAL].n_pair;
+    const DevIsai &M = h.dev.isai[BILU_ISAI_GENERAL];
+    const size_t lds = (size_t)nnz * sizeof(double);
+    if (lds > 65536) {
+        HIP_TRY(hipFuncSetAttribute((const void *)k_isai_gen<32>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        HIP_TRY(hipFuncSetAttribute((const void *)k_isai_gen<16>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    }
+    if (np > 0)
+        hipLaunchKernelGGL(k_isai_gen<32>, dim3(batch), dim3(WAVE * ISAI_WAVES), lds, st, M, nnz, batch, V, out, status,
+                           0, np, 0);
+    if (n > np)
+        hipLaunchKernelGGL(k_isai_gen<16>, dim3(batch), dim3(WAVE * ISAI_WAVES), lds, st, M, nnz, batch, V, out, status,
+                           np, n, np > 0 ? 1 : 0);
+    HIP_TRY(hipGetLastError());
+    return BILU_OK;
+}
+
 // FP64 ISAI solvers on the persistent kernel: a generation launch writes
 // the ISAI values and their zero-pivot flags into the workspace -- ILU(0) or
 // ParILU(0) factors then ISAI(L), ISAI(U) (k_factor_fast; FP64 throughout),
@@ -1901,12 +2070,10 @@ static int solve_isai_fast(bilu_pattern_s &h, int batch, const double *A, const 
     double *M = static_cast<double *>(ws);
     int *Z = reinterpret_cast<int *>(static_cast<char *>(ws) + mbytes);
     if (pc == PC_ISAI_GEN) {
-        const size_t lds = (size_t)nnz * sizeof(double);
-        if (lds > 65536) HIP_TRY(hipFuncSetAttribute((const void *)k_isai, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         timing_mark(BILU_TIMING_FACTOR, st, true);
-        hipLaunchKernelGGL(k_isai, dim3(batch), dim3(WAVE * ISAI_WAVES), lds, st, h.dev, batch, A, (int)BILU_ISAI_GENERAL, M, Z);
+        rc = launch_isai_general(h, batch, A, M, Z, st);
         timing_mark(BILU_TIMING_FACTOR, st, false);
-        HIP_TRY(hipGetLastError());
+        if (rc != BILU_OK) return rc;
         if (fstatus) hipLaunchKernelGGL(k_fill_int, dim3((batch + 255) / 256), dim3(256), 0, st, fstatus, 0, batch);
     } else {
         rc = launch_factor_fast<double>(h, batch, A, F_init, M, mode, sweeps, stop, fstatus, st, 1, M, Z, nullptr,
@@ -2402,6 +2569,7 @@ int bilu_isai(bilu_pattern_t h, int batch, const double *d_vals, int type, doubl
     const size_t lds = (type == BILU_ISAI_GENERAL ? 1 : 2) * (size_t)h->pa.nnz * sizeof(double);
     if (lds > h->max_lds) return BILU_ERR_TOO_LARGE;
     HIP_TRY(hipSetDevice(h->device));
+    if (type == BILU_ISAI_GENERAL) return launch_isai_general(*h, batch, d_vals, d_out, d_status, (hipStream_t)stream);
     if (lds > 65536) HIP_TRY(hipFuncSetAttribute((const void *)k_isai, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     hipLaunchKernelGGL(k_isai, dim3(batch), dim3(WAVE * ISAI_WAVES), lds, (hipStream_t)stream, h->dev, batch, d_vals, type, d_out, d_status);
     HIP_TRY(hipGetLastError());
