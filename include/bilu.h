@@ -120,8 +120,9 @@ typedef struct {
     int isai_power;       /* ISAI preconditioners: sparsity of |L|^k, |U|^k (|A|^k for the
                              general ISAI); 1 = the factors' / matrix's own pattern (default).
                              The reference's ILU-ISAI solver uses 2 (PrecondBiCGSTAB_ilu_isai_gpu.cu:
-                             297-299).  k > 1: FP64, one wave per system, ISAI rows of at most 32
-                             entries (the reference's row_size_limit), no warm start */
+                             297-299).  k > 1: FP64, one wave per system, no warm start; rows of
+                             the general ISAI at most 32 entries (the reference's row_size_limit),
+                             of ISAI(L) / ISAI(U) at most 128 (beyond the reference's assert) */
     double inner_adapt;   /* FP32_IR: an inner solve stops at the relative residual
                              max(inner_tol, inner_adapt * tol / |r|), |r| the FP64 residual it
                              starts from, so the last refinement step solves no further than the
@@ -277,8 +278,9 @@ int bilu_isai(bilu_pattern_t p, int batch, const double *d_vals, int type, doubl
  * all) -- the reference's chosen-pattern ISAI (GenerateISAI_gpu(aiA, A, type,
  * power), isai.h:9, pattern isai_gpu.cu:40-178, taken structurally).
  * bilu_isai_pattern: host copy of that pattern (row_ptrs == NULL: *nnz only).
- * bilu_isai_power: values d_out[batch * nnz] on it; rows of more than 32
- * entries give BILU_ERR_TOO_LARGE (the reference's row_size_limit). */
+ * bilu_isai_power: values d_out[batch * nnz] on it; BILU_ERR_TOO_LARGE for a
+ * GENERAL row of more than 32 entries (the reference's row_size_limit) or a
+ * LOWER / UPPER row of more than 128 (the reference asserts beyond 32). */
 int bilu_isai_pattern(bilu_pattern_t p, int type, int power, int *nnz, int *row_ptrs, int *col_idx);
 int bilu_isai_power(bilu_pattern_t p, int batch, const double *d_vals, int type, int power,
                     double *d_out, int *d_status, void *stream);

@@ -41,6 +41,7 @@ namespace {
 constexpr int WAVE = 64;
 constexpr int MAX_NR = 8;        // rows per lane in the register layout: n <= 512
 constexpr int ISAI_GROUP = 32;   // lanes per ISAI row (reference row_size_limit 32)
+constexpr int ISAI_TRI_MAX = 2 * 64; // longest ISAI(L) / ISAI(U) row: two unknowns per lane
 
 enum PcKind { PC_LU = 0, PC_ISAI_LU = 1, PC_ISAI_GEN = 2, PC_NONE = 3, PC_ISAI_POW = 4, PC_ISAI_GEN_POW = 5 };
 enum FactorKind { FK_ILU = 0, FK_PARILU = 1, FK_LOAD = 2, FK_NONE = 3, FK_ILU_ENTRIES = 4, FK_ILU_ERAS = 5 };
@@ -49,6 +50,7 @@ struct DevIsai {
     const int *beg, *m, *goff, *gather, *rhs;
     const int *order;  // rows by decreasing m: n_pair rows two per wave, then four per wave
     int n_pair;
+    int max_m;         // longest row (lower / upper beyond ISAI_GROUP: isai_tri_wide)
 };
 
 // an ISAI on a chosen pattern (powers > 1): the pattern and its gather map
@@ -487,9 +489,85 @@ __device__ __forceinline__ double view_val(const double *vals, int g)
 
 // The wave takes row pairs (i0, i0 + 1) for i0 = p0, p0 + pstep, ... (several
 // waves of one workgroup share a system by taking interleaved pairs).
+// view_val with an unconditional LDS read (loads issue together)
+__device__ __forceinline__ double view_val_u(const double *vals, int g)
+{
+    const double a = vals[g > 0 ? g : 0];
+    return g >= 0 ? a : (g == -2 ? 1.0 : 0.0);
+}
+
+__device__ __forceinline__ double lane_value(double v, int src)  // src wave-uniform
+{
+    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), src),
+                            __builtin_amdgcn_readlane(__double2loint(v), src));
+}
+
+// ISAI(L) / ISAI(U) rows of up to ISAI_TRI_MAX entries (power-k patterns;
+// the reference stops at 32 on an assert): one row per wave, lane t holding
+// unknowns t and t + 64, each receiving the column sweep's operations in the
+// order isai_tri applies them.  The finished unknown of column c is read with
+// v_readlane; the gather entries and values of 8 columns are loaded ahead, at
+// clamped positions.  The wave takes rows w, w + nw, ...
+__device__ int isai_tri_wide(const DevIsai &M, int n, int type, const double *vals, double *out,
+                             double *udiag, int lane, int w, int nw)
+{
+    constexpr int CH = 8;
+    const bool lower = type == BILU_ISAI_LOWER;
+    const int t0 = lane, t1 = lane + WAVE;
+    int bad = 0;
+    for (int i = w; i < n; i += nw) {
+        const int m = M.m[i];
+        const int *g = M.gather + M.goff[i];
+        const int last = m * m - 1;
+        double s0 = (t0 == (lower ? m - 1 : 0)) ? 1.0 : 0.0;
+        double s1 = (t1 == (lower ? m - 1 : 0)) ? 1.0 : 0.0;
+        for (int q0 = 0; q0 < m; q0 += CH) {
+            // columns of this chunk in sweep order: c = m-1-q (lower) or q (upper)
+            double v0[CH], v1[CH], dg[CH];
+#pragma unroll
+            for (int j = 0; j < CH; j++) {
+                const int q = min(q0 + j, m - 1);
+                const int c = lower ? m - 1 - q : q;
+                const int g0 = g[min(c * m + t0, last)], g1 = g[min(c * m + t1, last)], gd = g[c * m + c];
+                v0[j] = view_val_u(vals, g0);
+                v1[j] = view_val_u(vals, g1);
+                dg[j] = view_val_u(vals, gd);
+            }
+#pragma unroll
+            for (int j = 0; j < CH; j++) {
+                const int q = q0 + j;
+                if (q < m) {
+                    const int c = lower ? m - 1 - q : q;
+                    if (lane == (c & (WAVE - 1))) {
+                        bad |= (dg[j] == 0.0);
+                        if (c < WAVE) s0 = s0 / dg[j];
+                        else s1 = s1 / dg[j];
+                    }
+                    const double f = lane_value(c < WAVE ? s0 : s1, c & (WAVE - 1));
+                    if (lower) {
+                        if (t0 < c) s0 = fma(-f, v0[j], s0);
+                        if (t1 < c) s1 = fma(-f, v1[j], s1);
+                    } else {
+                        if (t0 > c && t0 < m) s0 = fma(-f, v0[j], s0);
+                        if (t1 > c && t1 < m) s1 = fma(-f, v1[j], s1);
+                    }
+                }
+            }
+        }
+        if (t0 < m) {
+            if (udiag != nullptr && t0 == 0) udiag[i] = s0;
+            else out[M.beg[i] + t0] = s0;
+        }
+        if (t1 < m) out[M.beg[i] + t1] = s1;
+    }
+    __syncthreads();
+    return __any(bad);
+}
+
 __device__ int isai_tri(const DevIsai &M, int n, int type, const double *vals, double *out,
                         double *udiag, int lane, int p0 = 0, int pstep = 2)
 {
+    if (M.max_m > ISAI_GROUP) return isai_tri_wide(M, n, type, vals, out, udiag, lane, p0 / 2, pstep / 2);
     const int grp = lane / ISAI_GROUP, t = lane % ISAI_GROUP;
     int bad = 0;
     for (int i0 = p0; i0 < n; i0 += pstep) {
@@ -1366,6 +1444,7 @@ int upload(bilu_pattern_s &h)
         d.isai[t].beg = B + o_is[t][0]; d.isai[t].m = B + o_is[t][1]; d.isai[t].goff = B + o_is[t][2];
         d.isai[t].gather = B + o_is[t][3]; d.isai[t].rhs = B + o_is[t][4];
         d.isai[t].order = B + o_is[t][5]; d.isai[t].n_pair = pa.isai[t].n_pair;
+        d.isai[t].max_m = pa.isai[t].max_m;
     }
     HIP_TRY(hipMalloc((void **)&h.d_counter, 256));
     HIP_TRY(hipMemset(h.d_counter, 0, 256));
@@ -1467,7 +1546,7 @@ int get_pow_isai(bilu_pattern_s &h, int type, int power, const bilu_pattern_s::P
     q.dev.rp = q.dbuf + o_rp; q.dev.ci = q.dbuf + o_ci;
     q.dev.map.beg = q.dbuf + o_beg; q.dev.map.m = q.dbuf + o_m; q.dev.map.goff = q.dbuf + o_goff;
     q.dev.map.gather = q.dbuf + o_g; q.dev.map.rhs = q.dbuf + o_rhs;
-    q.dev.map.order = q.dbuf + o_ord; q.dev.map.n_pair = M.n_pair;
+    q.dev.map.order = q.dbuf + o_ord; q.dev.map.n_pair = M.n_pair; q.dev.map.max_m = M.max_m;
     h.pow_isai.push_back(std::move(q));
     out = &h.pow_isai.back();
     return BILU_OK;
@@ -2110,7 +2189,7 @@ static int solve_fp64(bilu_pattern_t h, int batch, const double *d_A, const doub
         int rc = get_pow_isai(*h, gen ? BILU_ISAI_GENERAL : BILU_ISAI_LOWER, o.isai_power, q1);
         if (rc == BILU_OK && !gen) rc = get_pow_isai(*h, BILU_ISAI_UPPER, o.isai_power, q2);
         if (rc != BILU_OK) return rc;
-        if (q1->max_m > ISAI_GROUP || (q2 && q2->max_m > ISAI_GROUP)) return BILU_ERR_TOO_LARGE;
+        if (q1->max_m > (gen ? ISAI_GROUP : ISAI_TRI_MAX) || (q2 && q2->max_m > ISAI_TRI_MAX)) return BILU_ERR_TOO_LARGE;
         const size_t lds = sizeof(double) * ((size_t)nnz * (2 + (sp.factor == FK_PARILU ? 1 : 0)) +
                                              (size_t)q1->dev.nnz + (q2 ? (size_t)q2->dev.nnz : 0) + (size_t)n * 3);
         if (lds > h->max_lds) return BILU_ERR_TOO_LARGE;
@@ -2599,7 +2678,7 @@ int bilu_isai_power(bilu_pattern_t h, int batch, const double *d_vals, int type,
     const bilu_pattern_s::PowIsai *q = nullptr;
     const int rc = get_pow_isai(*h, type, power, q);
     if (rc != BILU_OK) return rc;
-    if (q->max_m > ISAI_GROUP) return BILU_ERR_TOO_LARGE;
+    if (q->max_m > (type == BILU_ISAI_GENERAL ? ISAI_GROUP : ISAI_TRI_MAX)) return BILU_ERR_TOO_LARGE;
     const size_t lds = ((size_t)h->pa.nnz + (size_t)q->dev.nnz) * sizeof(double);
     if (lds > h->max_lds) return BILU_ERR_TOO_LARGE;
     HIP_TRY(hipSetDevice(h->device));

@@ -331,11 +331,18 @@ static double view_at(const int *rp, const int *ci, const double *vals,
     return p >= 0 ? vals[p] : 0.0;
 }
 
+/* Largest ISAI row: 32 for the general (Gauss-Jordan) solve, as the
+ * reference's row_size_limit (isai_gpu.cu:17,223); the triangular sweeps take
+ * rows up to 128 entries (power-k patterns, where the reference stops on its
+ * assert). */
+#define ORC_ISAI_GEN_MAX 32
+#define ORC_ISAI_TRI_MAX 128
+
 /* Reference pivot choice (isai_gpu.cu:417-451, choose_pivot_row_1): 32
  * lanes; lane t outside [dp, m) stands in with the diagonal candidate; a
  * shfl_down tree keeps the other lane's candidate only if strictly larger in
  * magnitude. */
-static int ref_pivot(double Dt[][64], int m, int dp)
+static int ref_pivot(double Dt[][ORC_ISAI_TRI_MAX], int m, int dp)
 {
     double val[32];
     int piv[32];
@@ -361,7 +368,7 @@ static int ref_pivot(double Dt[][64], int m, int dp)
  * rhs e_ipos; lower: upper-triangular column sweep from the last position
  * (isai_gpu.cu:325-355), upper: lower-triangular sweep from 0 (:359-389),
  * general: Gauss-Jordan with the reference pivot choice (:417-554). */
-static int isai_row_solve(double Dt[][64], int m, int type, int ipos, double *sol)
+static int isai_row_solve(double Dt[][ORC_ISAI_TRI_MAX], int m, int type, int ipos, double *sol)
 {
     if (type == ORC_ISAI_LOWER) {
         for (int t = 0; t < m; t++) sol[t] = (t == m - 1) ? 1.0 : 0.0;
@@ -405,11 +412,11 @@ static int isai_row_solve(double Dt[][64], int m, int type, int ipos, double *so
 static int isai_on_rows(int n, const int *rp, const int *ci, const double *vals, int type,
                         const int *mrp, const int *mci, double *out)
 {
-    static __thread double Dt[64][64];
-    double sol[64];
+    static __thread double Dt[ORC_ISAI_TRI_MAX][ORC_ISAI_TRI_MAX];
+    double sol[ORC_ISAI_TRI_MAX];
     for (int i = 0; i < n; i++) {
         const int beg = mrp[i], m = mrp[i + 1] - beg;
-        if (m > 64 || (type == ORC_ISAI_GENERAL && m > 32)) return -4;
+        if (m > ORC_ISAI_TRI_MAX || (type == ORC_ISAI_GENERAL && m > ORC_ISAI_GEN_MAX)) return -4;
         int ipos = 0;
         for (int q = 0; q < m; q++) if (mci[beg + q] == i) ipos = q;
         for (int k = 0; k < m; k++)
@@ -430,15 +437,15 @@ static int isai_on_rows(int n, const int *rp, const int *ci, const double *vals,
 int orc_isai(int n, const int *rp, const int *ci, const int *diag,
              const double *vals, int type, double *out)
 {
-    static __thread double Dt[64][64];
-    double sol[64];
+    static __thread double Dt[ORC_ISAI_TRI_MAX][ORC_ISAI_TRI_MAX];
+    double sol[ORC_ISAI_TRI_MAX];
     for (int i = 0; i < n; i++) {
         int beg, end;
         if (type == ORC_ISAI_LOWER) { beg = rp[i]; end = diag[i] + 1; }
         else if (type == ORC_ISAI_UPPER) { beg = diag[i]; end = rp[i + 1]; }
         else { beg = rp[i]; end = rp[i + 1]; }
         const int m = end - beg;
-        if (m > 64 || (type == ORC_ISAI_GENERAL && m > 32)) return -4;
+        if (m > ORC_ISAI_TRI_MAX || (type == ORC_ISAI_GENERAL && m > ORC_ISAI_GEN_MAX)) return -4;
         int ipos = 0;
         for (int q = 0; q < m; q++) if (ci[beg + q] == i) ipos = q;
         /* Dt[t][k] = D[k][t] = A(J[k], J[t]) */

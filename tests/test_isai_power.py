@@ -58,7 +58,8 @@ def test_oracle_pattern_is_the_boolean_power(typ, power):
 
 
 @pytest.mark.parametrize("typ", [O.ISAI_GENERAL, O.ISAI_LOWER, O.ISAI_UPPER])
-@pytest.mark.parametrize("kind,n,power", [("stencil5", 100, 2), ("stencil5", 100, 3), ("random", 32, 2)])
+@pytest.mark.parametrize("kind,n,power", [("stencil5", 100, 2), ("stencil5", 100, 3), ("random", 32, 2),
+                                          ("random", 144, 2)])
 def test_oracle_values_satisfy_the_isai_identity(typ, kind, n, power):
     pat, A, _ = synth.batch(kind, 3, n=n)
     mrp, mci = O.isai_pattern(pat.n, pat.row_ptrs, pat.col_idx, typ, power)
@@ -138,11 +139,18 @@ def test_oracle_power_k_solves_converge(power):
 # ------------------------------------------------------------------ GPU
 
 GPU_CASES = [("stencil5", 64, 100, 2), ("stencil5", 32, 100, 3), ("random", 64, 32, 2)]
+# ISAI(L) / ISAI(U) rows of 84 / 86 entries (one row per wave, two unknowns per lane)
+LONG_CASES = [("random", 16, 144, 2)]
+
+
+def row_limit(typ):
+    """longest ISAI row the library takes: 32 for the general ISAI, 128 for the triangles"""
+    return 32 if typ == bilu.ISAI_GENERAL else 128
 
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("typ", [bilu.ISAI_GENERAL, bilu.ISAI_LOWER, bilu.ISAI_UPPER])
-@pytest.mark.parametrize("kind,B,n,power", GPU_CASES)
+@pytest.mark.parametrize("kind,B,n,power", GPU_CASES + LONG_CASES)
 def test_library_isai_power_matches_oracle(kind, B, n, power, typ):
     import torch
     print(f"case isai power {power} type {typ} {kind} B={B} n={n}", flush=True)
@@ -156,7 +164,7 @@ def test_library_isai_power_matches_oracle(kind, B, n, power, typ):
     dv = torch.from_numpy(vals).to(dev)
     out = torch.full((B, len(mci)), float("nan"), dtype=torch.float64, device=dev)
     st = torch.empty(B, dtype=torch.int32, device=dev)
-    if np.diff(mrp).max() > 32:
+    if np.diff(mrp).max() > row_limit(typ):
         with pytest.raises(bilu.BiluError):
             bilu.isai_power(P, dv, typ, power, out, st)
         return
@@ -172,23 +180,25 @@ def test_library_isai_power_matches_oracle(kind, B, n, power, typ):
 @pytest.mark.gpu
 def test_library_isai_power_rejects_long_rows():
     import torch
-    pat, A, _ = synth.batch("random", 4, n=144)
-    P = bilu.Pattern.from_synth(pat)
-    mrp, _ = bilu.isai_pattern(P, bilu.ISAI_LOWER, 2)
-    assert np.diff(mrp).max() > 32
-    dv = torch.from_numpy(A).cuda()
-    out = torch.empty((4, int(mrp[-1])), dtype=torch.float64, device="cuda")
-    with pytest.raises(bilu.BiluError, match="4"):
-        bilu.isai_power(P, dv, bilu.ISAI_LOWER, 2, out)
-    o = bilu.opts(precond=bilu.ILU0_ISAI, isai_power=2)
-    with pytest.raises(bilu.BiluError):
-        bilu.solve(P, dv, torch.ones(4, pat.n, dtype=torch.float64, device="cuda"),
-                   torch.empty(4, pat.n, dtype=torch.float64, device="cuda"), o)
+    # ISAI(L) rows beyond 128 entries (|L|^3 at n = 300: 193) and general rows beyond 32 (|A|^2 at n = 144)
+    for n, typ, power, pc in ((300, bilu.ISAI_LOWER, 3, bilu.ILU0_ISAI), (144, bilu.ISAI_GENERAL, 2, bilu.GENERAL_ISAI)):
+        pat, A, _ = synth.batch("random", 4, n=n)
+        P = bilu.Pattern.from_synth(pat)
+        mrp, _ = bilu.isai_pattern(P, typ, power)
+        assert np.diff(mrp).max() > row_limit(typ)
+        dv = torch.from_numpy(A).cuda()
+        out = torch.empty((4, int(mrp[-1])), dtype=torch.float64, device="cuda")
+        with pytest.raises(bilu.BiluError, match="4"):
+            bilu.isai_power(P, dv, typ, power, out)
+        o = bilu.opts(precond=pc, isai_power=power)
+        with pytest.raises(bilu.BiluError):
+            bilu.solve(P, dv, torch.ones(4, pat.n, dtype=torch.float64, device="cuda"),
+                       torch.empty(4, pat.n, dtype=torch.float64, device="cuda"), o)
 
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("precond", [bilu.ILU0_ISAI, bilu.PARILU0_ISAI, bilu.GENERAL_ISAI])
-@pytest.mark.parametrize("kind,B,n,power", GPU_CASES)
+@pytest.mark.parametrize("kind,B,n,power", GPU_CASES + LONG_CASES)
 def test_solve_with_isai_power_matches_oracle(kind, B, n, power, precond):
     import torch
     print(f"case solve isai power {power} precond {precond} {kind} B={B} n={n}", flush=True)
