@@ -55,6 +55,8 @@ def parse():
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32_ir"],
                     help="fp32_ir: FP32 preconditioner and inner BiCGSTAB inside FP64 refinement")
     ap.add_argument("--inner-tol", type=float, default=1e-5)
+    ap.add_argument("--isai-power", type=int, default=1,
+                    help="--mode single, ISAI preconditioners: sparsity of |L|^k, |U|^k (|A|^k)")
     ap.add_argument("--train-systems", type=int, default=262144)
     ap.add_argument("--train-epochs", type=int, default=30)
     ap.add_argument("--no-uneven", action="store_true", help="skip the uneven-batch leg")
@@ -335,6 +337,8 @@ def main():
     def roofline(kname, kernel_ms, key, step_ms):
         # achieved = the path's algorithmic bytes per launch over the kernel's
         # own duration (HIP events on its stream, this run)
+        if not kernel_ms > 0:
+            return None
         ach = alg_bytes / (kernel_ms * 1e-3) / 1e9
         r = {"bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
              "frac": round(ach / HBM_PEAK_GBS, 5), "kernel": kname, "kernel_ms": round(kernel_ms, 4),
@@ -359,17 +363,18 @@ def main():
                       kernel={"auto": bilu.KERNEL_AUTO, "wave": bilu.KERNEL_WAVE,
                               "persistent": bilu.KERNEL_PERSISTENT}[args.kernel],
                       precision=bilu.PRECISION_FP32_IR if mixed else bilu.PRECISION_FP64,
-                      inner_tol=args.inner_tol)
+                      inner_tol=args.inner_tol, isai_power=args.isai_power)
         ms, kms = run(o, kernel=bilu.TIMING_SOLVE)
         s = summary(ms)
         kname = ("k_mixed" if mixed else
-                 "k_fast" if (args.merge == "merged" and args.kernel != "wave") else "k_solve")
+                 "k_fast" if (args.merge == "merged" and args.kernel != "wave" and args.isai_power == 1)
+                 else "k_solve")
         rec.update({"value": s["systems_per_s"], "ms_per_step": s["ms_per_step"],
                     "dtype": "f32+f64" if mixed else "f64",
                     "config": dict(base_cfg, workload=f"random pattern n={n} nnz={nnz}, batch {B}, "
                                                       f"{args.precond} {args.merge} {args.precision}",
                                    precond=args.precond, merge=args.merge, kernel=args.kernel,
-                                   precision=args.precision),
+                                   precision=args.precision, isai_power=args.isai_power),
                     "result": s,
                     "roofline": roofline(kname, kms, f"{kname} {args.precond} {args.precision} n{n} b{B}", ms)})
         rec["cpu_baseline"] = None if args.no_cpu else cpu_baseline(pat, A, b, args.precond)

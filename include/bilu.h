@@ -238,7 +238,7 @@ int bilu_solve_ex_noalloc(bilu_pattern_t p, int batch, const double *d_A, const 
  * _begin).  _end waits for those events and returns the average kernel
  * duration (ms) and the number of launches timed.  One window at a time. */
 #define BILU_TIMING_FACTOR 1   /* the persistent factorization kernel of the split paths */
-#define BILU_TIMING_SOLVE 2    /* the solve kernel (merged FP64, mixed precision) */
+#define BILU_TIMING_SOLVE 2    /* the solve kernel (persistent FP64, mixed precision, one wave per system) */
 int bilu_timing_begin(int kernel, int max_launches);
 int bilu_timing_end(float *avg_ms, int *launches);
 

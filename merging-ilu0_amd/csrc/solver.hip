@@ -1889,7 +1889,9 @@ int launch_solve_nr(bilu_pattern_s &h, int batch, const double *A, const double 
 {
     auto kern = k_solve<NR, PC>;
     if (lds > 65536) HIP_TRY(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    timing_mark(BILU_TIMING_SOLVE, st, true);
     hipLaunchKernelGGL(kern, dim3(batch), dim3(WAVE), lds, st, h.dev, batch, A, b, x, F, sp, it, cv, ir, tr, QL, QU);
+    timing_mark(BILU_TIMING_SOLVE, st, false);
     HIP_TRY(hipGetLastError());
     return BILU_OK;
 }
