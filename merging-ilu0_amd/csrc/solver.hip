@@ -487,8 +487,6 @@ __device__ __forceinline__ double view_val(const double *vals, int g)
     return g >= 0 ? vals[g] : (g == -2 ? 1.0 : 0.0);
 }
 
-// The wave takes row pairs (i0, i0 + 1) for i0 = p0, p0 + pstep, ... (several
-// waves of one workgroup share a system by taking interleaved pairs).
 // view_val with an unconditional LDS read (loads issue together)
 __device__ __forceinline__ double view_val_u(const double *vals, int g)
 {
@@ -564,6 +562,11 @@ __device__ int isai_tri_wide(const DevIsai &M, int n, int type, const double *va
     return __any(bad);
 }
 
+// ISAI(L) / ISAI(U) rows of at most 32 entries: the wave takes row pairs
+// (i0, i0 + 1) for i0 = p0, p0 + pstep, ... (several waves of one workgroup
+// share a system by taking interleaved pairs), 32 lanes per row.  (Loading 8
+// sweep columns ahead and reading the finished unknown with v_readlane was
+// slower on the one-wave solve kernel: 15.8 against 14.4 ms at n = 144.)
 __device__ int isai_tri(const DevIsai &M, int n, int type, const double *vals, double *out,
                         double *udiag, int lane, int p0 = 0, int pstep = 2)
 {
