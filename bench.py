@@ -435,6 +435,10 @@ def main():
                           inner_tol=args.inner_tol, **common)
     mixed_merged = summary(run(o_mixed_m))
     mixed_merged["x_equals_segregated"] = bool(torch.equal(x, x_mixed))
+    # (2a') FP32 general ISAI of A (FP64 generation launch, FP32 values) in the same refinement
+    o_mixed_g = bilu.opts(precond=bilu.GENERAL_ISAI, precision=bilu.PRECISION_FP32_IR, inner_tol=args.inner_tol,
+                          **common)
+    mixed_general = summary(run(o_mixed_g))
     # (2b) the other FP64 solvers of the plain method on the persistent kernels:
     # ParILU(0) 5 sweeps merged into the solve; ILU(0) / ParILU(0) + ISAI(L, U)
     # and the general ISAI of A (generation launch, then the persistent solve)
@@ -584,7 +588,7 @@ def main():
                        variant="learned+mixed"),
         "speedup_vs_plain_fp64": round(ms_plain / ms_learn, 3),
         "variants": {"plain_fp64_ilu0_merged": plain, "mixed_ilu0_isai": mixed,
-                     "mixed_ilu0_isai_merged_kernel": mixed_merged,
+                     "mixed_ilu0_isai_merged_kernel": mixed_merged, "mixed_general_isai": mixed_general,
                      "learned_mixed_parilu0_isai": learned, "learned_mixed_parilu0_isai_merged_kernel": learned_merged,
                      "learned_mixed_parilu0_isai_split": split,
                      "mixed_parilu0_isai_no_warm_start": nolearn,

@@ -75,7 +75,9 @@ extern "C" {
 
 #define BILU_PRECISION_FP64 0     /* everything in FP64 (the reference arithmetic) */
 #define BILU_PRECISION_FP32_IR 1  /* FP32 preconditioner and inner BiCGSTAB inside FP64
-                                     iterative refinement (outer residual and its dot in FP64) */
+                                     iterative refinement (outer residual and its dot in FP64);
+                                     ILU(0) / ParILU(0) factors, their ISAI(L, U), or the general
+                                     ISAI of A (generated in FP64, stored FP32, segregated only) */
 
 /* exact ILU(0) schedules (the same arithmetic and bits, different parallel orders) */
 #define BILU_ILU_SCHED_AUTO 0     /* the pattern's tuned schedule (bilu_ilu0_tune), else ROWS */
@@ -106,7 +108,8 @@ typedef struct {
     int merge;            /* BILU_MERGED / BILU_SEGREGATED / BILU_MERGE_AUTO (default).  FP32_IR:
                              merged = one persistent kernel factors (and generates the ISAI
                              of) each system in LDS and solves it; segregated = the factor
-                             launch writes the FP32 factors / ISAI values to the workspace */
+                             launch writes the FP32 factors / ISAI values to the workspace
+                             (the general ISAI: segregated only, merged gives INVALID_ARG) */
     int true_res_restart; /* 1: restart from b - A x when the true residual misses tol */
     int max_restarts;     /* bound on such restarts (default 3) */
     int kernel;           /* BILU_KERNEL_*: which kernel family runs the solve */

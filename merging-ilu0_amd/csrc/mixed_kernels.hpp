@@ -448,11 +448,13 @@ __device__ __forceinline__ void spmv_tri(const float (&c)[SMX], int ns, const ui
 #define KMI_MAX_WAVES 12     // ISAI variant (two slot coefficient sets in VGPRs; 12 waves fill the LDS)
 #endif
 
-constexpr int PCK_LU = 0, PCK_ISAI = 1;
+constexpr int PCK_LU = 0, PCK_ISAI = 1, PCK_GEN = 2;
 
 // PCK_LU: gF = FP32 in-place factors (triangular solves); PCK_ISAI: gF =
 // FP32 ISAI(L) / ISAI(U) values on the pattern, gZ = zero-pivot flags of
-// their generation (two SpMVs per apply)
+// their generation (two SpMVs per apply); PCK_GEN: gF = FP32 general ISAI of
+// A on the pattern, gZ likewise (one SpMV per apply, over A's slot records;
+// segregated only)
 // waves per workgroup (launch bound) of a k_mixed instantiation: 72 slots
 // (patterns up to ~4600 nnz) need the register budget of 8 waves (LU: A in
 // VGPRs; ISAI: 2 x 40 triangle coefficients)
@@ -540,9 +542,11 @@ __global__ __launch_bounds__(km_waves(SM, PCK, FM) * WAVE) void k_mixed(
 
     // LU: A (FP32) in VGPRs.  ISAI: the ISAI(L) and ISAI(U) coefficients of
     // the triangle slot layouts in VGPRs (1 on ISAI(L)'s unit diagonal), A in LDS.
-    constexpr int SA = PCK == PCK_LU ? SM : 1, SC = PCK == PCK_ISAI ? SLU : 1;
+    // GEN: the general ISAI's coefficients in A's slot layout in VGPRs, A in LDS.
+    static_assert(PCK != PCK_GEN || FM == 0, "the general ISAI is generated by its own launch");
+    constexpr int SA = PCK == PCK_LU ? SM : 1, SC = PCK == PCK_ISAI ? SLU : 1, SG = PCK == PCK_GEN ? SM : 1;
     float a[SA];
-    float cL[SC], cU[SC];
+    float cL[SC], cU[SC], cG[SG];
 
     for (;;) {
         int sys = 0;
@@ -584,6 +588,7 @@ __global__ __launch_bounds__(km_waves(SM, PCK, FM) * WAVE) void k_mixed(
             slot_gather<SLU>(fp.isu_pos, fp.SU, Fp, lane, [&](int q, float v) { cU[q] = v; });
             if constexpr (FM == 1) wave_sync();
         }
+        if constexpr (PCK == PCK_GEN) slot_gather<SM>(fp.slot_pos, S, Fp, lane, [&](int q, float v) { cG[q] = v; });
         slot_gather<SM>(fp.slot_pos, S, Ap, lane, [&](int q, double v) {
             if constexpr (PCK == PCK_LU) a[q] = (float)v;
             else Asl[q * WAVE + lane] = (float)v;
@@ -631,12 +636,19 @@ __global__ __launch_bounds__(km_waves(SM, PCK, FM) * WAVE) void k_mixed(
                 trsv_avail<NR, true, float>(trL, vcntL, fp.nlev_l, rlev, F, in, rinv, e1, accb, scratch, y, lane);
                 trsv_avail<NR, false, float>(trU, vcntU, fp.nlev_u, rlev, F, y, rinv, e0, accb, scratch, out, lane);
                 wave_sync();
-            } else {
+            } else if constexpr (PCK == PCK_ISAI) {
                 to_lds(in, e0, lane, n);
                 wave_sync();
                 spmv_tri<SC>(cL, fp.SL, lrec, e0, e1, thr, lane);
                 wave_sync();
                 spmv_tri<SC>(cU, fp.SU, urec, e1, e0, thr, lane);
+                wave_sync();
+                from_lds(out, e0, lane, n);
+            } else {
+                // rows of M v with A's records (row ends, scratch cell n + 1), e1 -> e0
+                to_lds(in, e1, lane, n);
+                wave_sync();
+                spmv_tri<SG>(cG, S, meta, e1, e0, thr, lane);
                 wave_sync();
                 from_lds(out, e0, lane, n);
             }

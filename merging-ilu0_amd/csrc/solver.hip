@@ -698,8 +698,8 @@ __device__ __forceinline__ void static_for(F &&f)
 // the other row (lanes t < 16 read t + 16: ds_bpermute, or the permlane16
 // swap); a 16-lane group (m <= 16) would read lanes t + 16 >= m, which all
 // hold the stand-in (dval, dp): a local comparison.
-template <int G, bool UNR>
-__device__ __forceinline__ int isai_general_g(const DevIsai &M, const double *vals, double *out, int lane, int r0, int r1,
+template <int G, bool UNR, typename TO = double>
+__device__ __forceinline__ int isai_general_g(const DevIsai &M, const double *vals, TO *out, int lane, int r0, int r1,
                               int item0, int istep)
 {
     static_assert(UNR || G == 32, "the rolled step loop is the 32-lane one");
@@ -888,7 +888,7 @@ __device__ __forceinline__ int isai_general_g(const DevIsai &M, const double *va
 #pragma unroll
         for (int k = 0; k < G; k++)
             if (k == t) d = row[k];
-        if (act && t < m) out[M.beg[i] + t] = rhs / d;
+        if (act && t < m) out[M.beg[i] + t] = (TO)(rhs / d);
     }
     return bad;
 }
@@ -1164,11 +1164,12 @@ __global__ __launch_bounds__(WAVE * ISAI_WAVES) void k_isai(DevPattern P, int ba
 // the map with G lanes per row (r0 = 0, r1 = n_pair with G = 32; then the
 // rest with G = 16, a launch of its own so that its smaller register block
 // sets its occupancy).  Workgroups as in k_isai.  The status of the second
-// launch (or_status = 1) keeps a zero pivot the first one found.
-template <int G>
+// launch (or_status = 1) keeps a zero pivot the first one found.  TO = float:
+// the FP64 values rounded once, as the FP32 preconditioner stores them.
+template <int G, typename TO>
 __global__ __launch_bounds__(WAVE * ISAI_WAVES) void k_isai_gen(DevIsai M, int nnz, int batch,
                                                                const double *__restrict__ gV,
-                                                               double *__restrict__ gOut,
+                                                               TO *__restrict__ gOut,
                                                                int *__restrict__ g_status, int r0, int r1,
                                                                int or_status)
 {
@@ -1182,7 +1183,7 @@ __global__ __launch_bounds__(WAVE * ISAI_WAVES) void k_isai_gen(DevIsai M, int n
     if (tid == 0) s_bad = 0;
     for (int e = tid; e < nnz; e += NT) lds[e] = Vp[e];
     __syncthreads();
-    const int bad = isai_general_g<G, true>(M, lds, gOut + (size_t)sys * nnz, lane, r0, r1, w, ISAI_WAVES);
+    const int bad = isai_general_g<G, true, TO>(M, lds, gOut + (size_t)sys * nnz, lane, r0, r1, w, ISAI_WAVES);
     if (__any(bad) && lane == 0) atomicOr(&s_bad, 1);
     __syncthreads();
     if (g_status && tid == 0) {
@@ -1794,7 +1795,10 @@ int launch_mixed_t(bilu_pattern_s &h, const FastParams &fp, int grid, int waves,
                    const MixedParams &mp, float *it, int *cv, double *ir, double *tr, int *refine,
                    hipStream_t st, Jobs jobs, const FactorArgs *fa)
 {
-    auto kern = fa ? k_mixed<NR, SM, PCK, SLU, 1, RG> : k_mixed<NR, SM, PCK, SLU, 0, RG>;
+    auto kern = k_mixed<NR, SM, PCK, SLU, 0, RG>;
+    if constexpr (PCK != PCK_GEN) {
+        if (fa) kern = k_mixed<NR, SM, PCK, SLU, 1, RG>;
+    }
     if (lds > 65536) HIP_TRY(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, st, fp.counter);
     timing_mark(BILU_TIMING_SOLVE, st, true);
@@ -1843,6 +1847,7 @@ int launch_mixed(bilu_pattern_s &h, int batch, const double *A, const double *b,
     int sm = slot_template_mixed(pl.S), slu = 1;
     if (!pl.ok || nr > MAX_NR || pl.nr != nr || sm < 0) return BILU_ERR_TOO_LARGE;
     if (pck == PCK_ISAI && !isai_tier(pl.S, pl.SL, pl.SU, sm, slu)) return BILU_ERR_TOO_LARGE;
+    if (pck == PCK_GEN && fa) return BILU_ERR_INVALID_ARG;   // generated by its own launch
     // records in global memory: the ISAI templates of 48 and 72 slots, when asked
     const bool rg = rec_global && pck == PCK_ISAI && sm >= 48 && h.d_rec32;
     FastParams fp{};
@@ -1877,6 +1882,8 @@ int launch_mixed(bilu_pattern_s &h, int batch, const double *A, const double *b,
     if (nr == R && sm == M) \
         return pck == PCK_LU \
             ? launch_mixed_t<R, M, PCK_LU, 1>(h, fp, grid, waves, lds, batch, A, b, x, F, Z, mp, it, cv, ir, tr, refine, st, jobs, fa) \
+            : pck == PCK_GEN \
+            ? launch_mixed_t<R, M, PCK_GEN, 1>(h, fp, grid, waves, lds, batch, A, b, x, F, Z, mp, it, cv, ir, tr, refine, st, jobs, nullptr) \
             : launch_mixed_isai<R, M, T>(rg, h, fp, grid, waves, lds, batch, A, b, x, F, Z, mp, it, cv, ir, tr, refine, st, jobs, fa);
 #define MIXED_ROW(R) MIXED_CASE(R, 24, 20) MIXED_CASE(R, 36, 24) MIXED_CASE(R, 48, 32) MIXED_CASE(R, 72, 40)
     MIXED_ROW(1) MIXED_ROW(2) MIXED_ROW(3) MIXED_ROW(4) MIXED_ROW(5)
@@ -2110,22 +2117,32 @@ int solve_ws(bilu_pattern_s &h, size_t bytes, void **out, hipStream_t st, const 
 
 // general ISAI of a batch (values V, output on the pattern): the paired rows,
 // then the four-per-wave rows, two launches on the stream
-static int launch_isai_general(const bilu_pattern_s &h, int batch, const double *V, double *out, int *status,
-                               hipStream_t st)
+static int launch_isai_general(const bilu_pattern_s &h, int batch, const double *V, void *out, bool f32,
+                               int *status, hipStream_t st)
 {
     const int n = h.pa.n, nnz = h.pa.nnz, np = h.pa.isai[BILU_ISAI_GENERAL].n_pair;
     const DevIsai &M = h.dev.isai[BILU_ISAI_GENERAL];
     const size_t lds = (size_t)nnz * sizeof(double);
+    const void *k32 = f32 ? (const void *)k_isai_gen<32, float> : (const void *)k_isai_gen<32, double>;
+    const void *k16 = f32 ? (const void *)k_isai_gen<16, float> : (const void *)k_isai_gen<16, double>;
     if (lds > 65536) {
-        HIP_TRY(hipFuncSetAttribute((const void *)k_isai_gen<32>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        HIP_TRY(hipFuncSetAttribute((const void *)k_isai_gen<16>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        HIP_TRY(hipFuncSetAttribute(k32, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        HIP_TRY(hipFuncSetAttribute(k16, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     }
-    if (np > 0)
-        hipLaunchKernelGGL(k_isai_gen<32>, dim3(batch), dim3(WAVE * ISAI_WAVES), lds, st, M, nnz, batch, V, out, status,
-                           0, np, 0);
-    if (n > np)
-        hipLaunchKernelGGL(k_isai_gen<16>, dim3(batch), dim3(WAVE * ISAI_WAVES), lds, st, M, nnz, batch, V, out, status,
-                           np, n, np > 0 ? 1 : 0);
+    float *o32 = static_cast<float *>(out);
+    double *o64 = static_cast<double *>(out);
+    if (np > 0) {
+        if (f32) hipLaunchKernelGGL((k_isai_gen<32, float>), dim3(batch), dim3(WAVE * ISAI_WAVES), lds, st, M, nnz,
+                                    batch, V, o32, status, 0, np, 0);
+        else hipLaunchKernelGGL((k_isai_gen<32, double>), dim3(batch), dim3(WAVE * ISAI_WAVES), lds, st, M, nnz,
+                                batch, V, o64, status, 0, np, 0);
+    }
+    if (n > np) {
+        if (f32) hipLaunchKernelGGL((k_isai_gen<16, float>), dim3(batch), dim3(WAVE * ISAI_WAVES), lds, st, M, nnz,
+                                    batch, V, o32, status, np, n, np > 0 ? 1 : 0);
+        else hipLaunchKernelGGL((k_isai_gen<16, double>), dim3(batch), dim3(WAVE * ISAI_WAVES), lds, st, M, nnz,
+                                batch, V, o64, status, np, n, np > 0 ? 1 : 0);
+    }
     HIP_TRY(hipGetLastError());
     return BILU_OK;
 }
@@ -2155,7 +2172,7 @@ static int solve_isai_fast(bilu_pattern_s &h, int batch, const double *A, const 
     int *Z = reinterpret_cast<int *>(static_cast<char *>(ws) + mbytes);
     if (pc == PC_ISAI_GEN) {
         timing_mark(BILU_TIMING_FACTOR, st, true);
-        rc = launch_isai_general(h, batch, A, M, Z, st);
+        rc = launch_isai_general(h, batch, A, M, false, Z, st);
         timing_mark(BILU_TIMING_FACTOR, st, false);
         if (rc != BILU_OK) return rc;
         if (fstatus) hipLaunchKernelGGL(k_fill_int, dim3((batch + 255) / 256), dim3(256), 0, st, fstatus, 0, batch);
@@ -2331,6 +2348,34 @@ static int solve_ex_impl(bilu_pattern_t h, int batch, const double *d_A, const d
             HIP_TRY(hipGetLastError());
         }
         return rc;
+    }
+    if (o.precond == BILU_PRECOND_GENERAL_ISAI && o.precision == BILU_PRECISION_FP32_IR) {
+        // the general ISAI of A (FP64 Gauss-Jordan, stored FP32) through the
+        // workspace, then the FP32 inner / FP64 refinement solve applying it
+        // as one SpMV; always the two launches
+        if (o.merge == BILU_MERGED) return BILU_ERR_INVALID_ARG;
+        const auto &pa = h->pa;
+        const int nr = (pa.n + WAVE - 1) / WAVE;
+        if (!pa.fast.ok || slot_template_mixed(pa.fast.S) < 0 || nr > MAX_NR || pa.fast.nr != nr ||
+            pa.isai[BILU_ISAI_GENERAL].max_m > ISAI_GROUP || (size_t)pa.nnz * sizeof(double) > h->max_lds)
+            return BILU_ERR_TOO_LARGE;
+        MixedParams mp{o.tol, o.tol_mode, (float)o.inner_tol, o.max_iter, o.max_refine, o.inner_tol, o.inner_adapt};
+        const size_t fbytes = ((size_t)batch * pa.nnz * sizeof(float) + 255) / 256 * 256;
+        void *gws = nullptr;
+        int rc = solve_ws(*h, fbytes + (size_t)batch * sizeof(int), &gws, st, wsrc);
+        if (rc != BILU_OK) return rc;
+        float *M = static_cast<float *>(gws);
+        int *Z = reinterpret_cast<int *>(static_cast<char *>(gws) + fbytes);
+        timing_mark(BILU_TIMING_FACTOR, st, true);
+        rc = launch_isai_general(*h, batch, d_A, M, true, Z, st);
+        timing_mark(BILU_TIMING_FACTOR, st, false);
+        if (rc != BILU_OK) return rc;
+        if (d_factor_status) {
+            hipLaunchKernelGGL(k_fill_int, dim3((batch + 255) / 256), dim3(256), 0, st, d_factor_status, 0, batch);
+            HIP_TRY(hipGetLastError());
+        }
+        return launch_mixed(*h, batch, d_A, d_b, d_x, M, Z, PCK_GEN, mp, d_iter, d_conv, d_iter_res, d_true_res,
+                            d_refine, st);
     }
     // factor launch (persistent), factors (or their ISAI(L, U)) through the
     // handle's workspace, then the solve; these paths serve ILU(0) / ParILU(0)
@@ -2653,7 +2698,7 @@ int bilu_isai(bilu_pattern_t h, int batch, const double *d_vals, int type, doubl
     const size_t lds = (type == BILU_ISAI_GENERAL ? 1 : 2) * (size_t)h->pa.nnz * sizeof(double);
     if (lds > h->max_lds) return BILU_ERR_TOO_LARGE;
     HIP_TRY(hipSetDevice(h->device));
-    if (type == BILU_ISAI_GENERAL) return launch_isai_general(*h, batch, d_vals, d_out, d_status, (hipStream_t)stream);
+    if (type == BILU_ISAI_GENERAL) return launch_isai_general(*h, batch, d_vals, d_out, false, d_status, (hipStream_t)stream);
     if (lds > 65536) HIP_TRY(hipFuncSetAttribute((const void *)k_isai, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     hipLaunchKernelGGL(k_isai, dim3(batch), dim3(WAVE * ISAI_WAVES), lds, (hipStream_t)stream, h->dev, batch, d_vals, type, d_out, d_status);
     HIP_TRY(hipGetLastError());

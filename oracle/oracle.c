@@ -904,7 +904,9 @@ int orc_bicgstab_mixed(int n, const int *rp, const int *ci, const int *diag,
 }
 
 /* isai = 0: P32 is the FP32 in-place LU factor (availability-order solves);
- * isai = 1: P32 is the FP32 ISAI(L)/ISAI(U) array, zp its zero-pivot flag */
+ * isai = 1: P32 is the FP32 ISAI(L)/ISAI(U) array, zp its zero-pivot flag;
+ * isai = 2: P32 is the FP32 general ISAI of A on A's pattern, applied as one
+ * FP32 SpMV (rows left to right), zp its zero-pivot flag */
 int orc_bicgstab_mixed_pc(int n, const int *rp, const int *ci, const int *diag,
                           const int *lev_l, const int *lev_u, const double *A, const double *b,
                           const float *F32, int isai, int zp, const orc_opts *o, double inner_tol,
@@ -959,7 +961,8 @@ int orc_bicgstab_mixed_pc(int n, const int *rp, const int *ci, const int *diag,
                 if (rho_new == 0.0f) break;
                 const float beta = (rho_new / rho_old) * (alpha / omega_old);
                 for (int i = 0; i < n; i++) p[i] = fmaf(beta, fmaf(-omega_old, v[i], p[i]), ri[i]);
-                if (isai) isai_apply_f(n, rp, ci, diag, F32, p, tmp, ph);
+                if (isai == 2) spmvf(n, rp, ci, F32, p, ph);
+                else if (isai) isai_apply_f(n, rp, ci, diag, F32, p, tmp, ph);
                 else lu_solve_avail_f(n, rp, ci, diag, lev_l, lev_u, F32, p, tmp, ph);
                 spmvf(n, rp, ci, Af, ph, v);
                 alpha = rho_new / dotf64w(n, rh, v);
@@ -969,7 +972,8 @@ int orc_bicgstab_mixed_pc(int n, const int *rp, const int *ci, const int *diag,
                 it += 0.5f;
                 if (!isfinite(ires)) { inner_bad = 1; break; }
                 if (ires < itol) break;
-                if (isai) isai_apply_f(n, rp, ci, diag, F32, s, tmp, sh);
+                if (isai == 2) spmvf(n, rp, ci, F32, s, sh);
+                else if (isai) isai_apply_f(n, rp, ci, diag, F32, s, tmp, sh);
                 else lu_solve_avail_f(n, rp, ci, diag, lev_l, lev_u, F32, s, tmp, sh);
                 spmvf(n, rp, ci, Af, sh, t);
                 const float ts = dotf64w(n, t, s);
@@ -1155,7 +1159,7 @@ int orc_parilu0_stop(int n, const int *rp, const int *ci, const int *diag, const
 
 /* whole-batch driver of the split factor + solve paths: precond 0 = exact
  * ILU(0), 1 = ParILU(0) (with init / stop_tol as orc_parilu0_stop), 2 / 3 =
- * the same followed by ISAI(L), ISAI(U); precision 0 = FP64 BiCGSTAB
+ * the same followed by ISAI(L), ISAI(U), 4 = the general ISAI of A; precision 0 = FP64 BiCGSTAB
  * (availability-order solves; FP64 ISAI applies, a zero pivot of their
  * generation failing the system as in the batch driver), 1 = FP32 inner /
  * FP64 refinement over the FP32-rounded factors or the FP32 ISAI. */
@@ -1182,6 +1186,27 @@ static void *split_worker(void *arg)
         const double *b = j->b + (size_t)s * n;
         double *x = j->x + (size_t)s * n;
         int st = 0;
+        if (j->precond == 4) {
+            /* general ISAI of A (no factorization): FP64 values, then FP32 (precision 1) */
+            const int zs = orc_isai(n, j->rp, j->ci, j->diag, A, ORC_ISAI_GENERAL, P1);
+            if (j->fstat) j->fstat[s] = 0;
+            if (j->precision == 1) {
+                for (int q = 0; q < nnz; q++) F32[q] = zs == 0 ? (float)P1[q] : 0.0f;
+                orc_bicgstab_mixed_pc(n, j->rp, j->ci, j->diag, j->lev, j->lev + n, A, b, F32, 2, zs != 0, j->o,
+                                      j->inner_tol, j->max_refine, x, &j->iter[s], &j->conv[s],
+                                      &j->iter_res[s], &j->true_res[s], &j->refine[s]);
+            } else if (zs != 0) {
+                for (int i = 0; i < n; i++) x[i] = 0.0;
+                j->iter[s] = 0.0f; j->conv[s] = zs; j->iter_res[s] = 0.0;
+                j->true_res[s] = sqrt(orc_dot(n, b, b, j->o->dot_width));
+                j->refine[s] = 0;
+            } else {
+                orc_bicgstab(n, j->rp, j->ci, j->diag, A, b, x, ORC_PC_ISAI_GEN, P1, NULL, j->o,
+                             &j->iter[s], &j->conv[s], &j->iter_res[s], &j->true_res[s]);
+                j->refine[s] = 0;
+            }
+            continue;
+        }
         if (j->precond == 1 || j->precond == 3) {
             orc_parilu0_stop(n, j->rp, j->ci, j->diag, A, j->init ? j->init + (size_t)s * nnz : NULL, F,
                              j->sweeps, j->stop_tol, &st, j->inplace);

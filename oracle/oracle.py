@@ -320,7 +320,7 @@ def batch_solve_split(pat, vals, b, precond=ILU0, sweeps=5, init=None, stop_tol=
     fs = np.empty(B, dtype=np.int32)
     ini = _f64(init) if init is not None else None
     st = lib().orc_batch_solve_split(pat.n, pat.nnz, pat.row_ptrs, pat.col_idx, B, vals, b, x,
-                                     {ILU0: 0, PARILU0: 1, ILU0_ISAI: 2, PARILU0_ISAI: 3}[precond], sweeps,
+                                     {ILU0: 0, PARILU0: 1, ILU0_ISAI: 2, PARILU0_ISAI: 3, GENERAL_ISAI: 4}[precond], sweeps,
                                      ini.ctypes.data if ini is not None else None, stop_tol,
                                      precision, inner_tol, max_refine, C.byref(o), it, cv, ir, tr,
                                      rf, fs, threads, int(bool(inplace)))

@@ -41,7 +41,7 @@ def run_oracle(pat, A, b, precond=bilu.ILU0, parilu_sweeps=5, tol=1e-10, tol_mod
     o = O.opts(tol=tol, tol_mode=tol_mode, max_iter=max_iter, true_res_restart=true_res_restart,
                trsv_order=O.TRSV_AVAIL)
     pc = {bilu.ILU0: O.ILU0, bilu.PARILU0: O.PARILU0, bilu.ILU0_ISAI: O.ILU0_ISAI,
-          bilu.PARILU0_ISAI: O.PARILU0_ISAI}[precond]
+          bilu.PARILU0_ISAI: O.PARILU0_ISAI, bilu.GENERAL_ISAI: O.GENERAL_ISAI}[precond]
     return O.batch_solve_split(pat, A, b, precond=pc,
                                sweeps=parilu_sweeps, init=init, stop_tol=parilu_stop_tol,
                                precision=precision, inner_tol=inner_tol, max_refine=max_refine,
@@ -151,13 +151,42 @@ def test_mixed_special_cases():
     assert g[2][2] == bilu.ZERO_PIVOT
 
 
+@pytest.mark.parametrize("kind,B,n", [("random", 256, 144), ("stencil5", 64, 100), ("random", 64, 32),
+                                      ("random", 64, 300)])
+def test_mixed_general_isai_bitwise(kind, B, n):
+    """FP32 general ISAI of A (generated in FP64 with the reference pivot tree, stored FP32, applied as
+    one SpMV over A's slot records) inside FP64 refinement: bitwise the oracle, every system to 1e-10"""
+    print(f"case mixed general isai {kind} B={B} n={n}", flush=True)
+    pat, A, b = synth.batch(kind, B, n=n)
+    kw = dict(precond=bilu.GENERAL_ISAI, tol=1e-10, tol_mode=bilu.TOL_RELATIVE, precision=bilu.PRECISION_FP32_IR)
+    g = run_gpu(pat, A, b, **kw)
+    o = run_oracle(pat, A, b, **kw)
+    assert_same(g, o, f"mixed general isai {kind} n={n}")
+    assert (o[2] == 1).all()
+    relres = np.array([np.linalg.norm(b[s] - pat.dense(A[s]) @ g[0][s]) / np.linalg.norm(b[s])
+                       for s in range(0, B, max(1, B // 16))])
+    assert relres.max() <= 1e-10 * 1.0001
+
+
+def test_mixed_general_isai_zero_pivot():
+    print("case mixed general isai zero pivot", flush=True)
+    pat, A, b = synth.batch("random", 8, n=32)
+    A = A.copy()
+    row0 = slice(pat.row_ptrs[0], pat.row_ptrs[1])
+    A[3, row0] = 0.0   # row 0 of A zero: every block holding it is singular
+    kw = dict(precond=bilu.GENERAL_ISAI, tol=1e-10, tol_mode=bilu.TOL_RELATIVE, precision=bilu.PRECISION_FP32_IR)
+    g = run_gpu(pat, A, b, **kw)
+    assert_same(g, run_oracle(pat, A, b, **kw), "mixed general isai zero pivot")
+    assert g[2][3] == bilu.ZERO_PIVOT and not g[0][3].any()
+
+
 def test_mixed_rejects_unsupported():
     pat, A, b = synth.batch("random", 4, n=300, density=0.09)   # ~8400 nnz: beyond the persistent kernels' tables
     with pytest.raises(bilu.BiluError):
         run_gpu(pat, A, b, precision=bilu.PRECISION_FP32_IR, tol=1e-10, tol_mode=1)
     pat, A, b = synth.batch("random", 4, n=144)
-    with pytest.raises(bilu.BiluError):
-        run_gpu(pat, A, b, precond=bilu.GENERAL_ISAI, precision=bilu.PRECISION_FP32_IR)
+    with pytest.raises(bilu.BiluError):   # the general ISAI has its own generation launch
+        run_gpu(pat, A, b, precond=bilu.GENERAL_ISAI, precision=bilu.PRECISION_FP32_IR, merge=bilu.MERGED)
 
 
 def test_mixed_full_batch_properties():
