@@ -313,3 +313,27 @@ def test_mixed_isai_converges():
         assert (cv == 1).all()
         for s in range(0, 32, 5):
             assert relres(pat, A[s], b[s], x[s]) <= 1e-10
+
+
+def test_split_general_isai():
+    """The split driver's general ISAI (precond 4): FP64 = the batch driver's general-ISAI solver
+    bit for bit; FP32 refinement over the FP32-rounded general ISAI reaches relres <= 1e-10
+    (recomputed with dense products here) with 1 or 2 inner half steps per refinement on A = I"""
+    pat, A, b = synth.batch("random", 16, n=144)
+    o = O.opts(tol=1e-10, tol_mode=O.TOL_RELATIVE)
+    g = O.batch_solve_split(pat, A, b, precond=O.GENERAL_ISAI, precision=O.PREC_FP64, o=o, threads=4)
+    m = O.batch_solve(pat, A, b, O.GENERAL_ISAI, o=o, threads=4)
+    for u, v in zip(g[:5], m):
+        assert np.array_equal(u, v)
+    x, it, cv, ir, tr, rf, fs = O.batch_solve_split(pat, A, b, precond=O.GENERAL_ISAI,
+                                                    precision=O.PREC_FP32_IR, o=o, threads=4)
+    assert (cv == 1).all() and (fs == 0).all() and (rf >= 1).all()
+    for s in range(16):
+        assert relres(pat, A[s], b[s], x[s]) <= 1e-10
+    # A = I: the general ISAI is I, so every inner solve ends after one half step
+    I = np.zeros_like(A[:2])
+    I[:, pat.diag] = 1.0
+    x, it, cv, ir, tr, rf, fs = O.batch_solve_split(pat, I, b[:2], precond=O.GENERAL_ISAI,
+                                                    precision=O.PREC_FP32_IR, o=o)
+    assert (cv == 1).all() and np.allclose(x, b[:2], rtol=1e-10, atol=0)
+    assert (it == 0.5 * rf).all()
