@@ -700,7 +700,7 @@ __device__ __forceinline__ void static_for(F &&f)
 // hold the stand-in (dval, dp): a local comparison.
 template <int G, bool UNR, typename TO = double>
 __device__ __forceinline__ int isai_general_g(const DevIsai &M, const double *vals, TO *out, int lane, int r0, int r1,
-                              int item0, int istep)
+                              int item0, int istep, double *pv = nullptr)
 {
     static_assert(UNR || G == 32, "the rolled step loop is the 32-lane one");
     constexpr int RPW = WAVE / G;
@@ -867,12 +867,27 @@ __device__ __forceinline__ int isai_general_g(const DevIsai &M, const double *va
                     bad |= (de == 0.0);
                     const double mult = row[dp] / de;
                     const bool upd = t != dp;
+                    // the pivot row's columns from G / 2 on go through the wave's LDS cells
+                    // pv (stored by the pivot lane, read back as a broadcast), the rest by
+                    // DPP -- the two pipes share the broadcasts (each alone sets the pace)
+                    constexpr int CL = G / 2;
+                    if constexpr (CL < G) {
+                        if (t == dp) {
+#pragma unroll
+                            for (int c = CL; c < G; c++)
+                                if (c < mmax) pv[grp * G + c] = row[c];
+                        }
+                        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                        __builtin_amdgcn_wave_barrier();
+                        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                    }
 #pragma unroll
                     for (int c0 = 0; c0 < G; c0 += CH) {
                         if (c0 < mmax) {
                             double key[CH];
 #pragma unroll
-                            for (int j = 0; j < CH; j++) key[j] = group_bcast<G, dp>(row[c0 + j]);
+                            for (int j = 0; j < CH; j++)
+                                key[j] = c0 + j >= CL ? pv[grp * G + c0 + j] : group_bcast<G, dp>(row[c0 + j]);
                             if (upd) {
 #pragma unroll
                                 for (int j = 0; j < CH; j++) row[c0 + j] = fma(-mult, key[j], row[c0 + j]);
@@ -1183,7 +1198,11 @@ __global__ __launch_bounds__(WAVE * ISAI_WAVES) void k_isai_gen(DevIsai M, int n
     if (tid == 0) s_bad = 0;
     for (int e = tid; e < nnz; e += NT) lds[e] = Vp[e];
     __syncthreads();
-    const int bad = isai_general_g<G, true, TO>(M, lds, gOut + (size_t)sys * nnz, lane, r0, r1, w, ISAI_WAVES);
+    // after the values: each wave's pivot-row cells (G per row group), zeroed
+    // once (a cell a step does not store holds an earlier step's value)
+    double *pv = lds + nnz + (size_t)w * WAVE;
+    pv[lane] = 0.0;
+    const int bad = isai_general_g<G, true, TO>(M, lds, gOut + (size_t)sys * nnz, lane, r0, r1, w, ISAI_WAVES, pv);
     if (__any(bad) && lane == 0) atomicOr(&s_bad, 1);
     __syncthreads();
     if (g_status && tid == 0) {
@@ -2122,7 +2141,8 @@ static int launch_isai_general(const bilu_pattern_s &h, int batch, const double 
 {
     const int n = h.pa.n, nnz = h.pa.nnz, np = h.pa.isai[BILU_ISAI_GENERAL].n_pair;
     const DevIsai &M = h.dev.isai[BILU_ISAI_GENERAL];
-    const size_t lds = (size_t)nnz * sizeof(double);
+    const size_t lds = (size_t)nnz * sizeof(double) + (size_t)ISAI_WAVES * WAVE * sizeof(double);  // + pivot-row cells
+    if (lds > h.max_lds) return BILU_ERR_TOO_LARGE;
     const void *k32 = f32 ? (const void *)k_isai_gen<32, float> : (const void *)k_isai_gen<32, double>;
     const void *k16 = f32 ? (const void *)k_isai_gen<16, float> : (const void *)k_isai_gen<16, double>;
     if (lds > 65536) {
@@ -2162,7 +2182,7 @@ static int solve_isai_fast(bilu_pattern_s &h, int batch, const double *A, const 
     if (launch_fast(h, batch, A, b, x, nullptr, sp, it, cv, ir, tr, st, fpc, nullptr, true) != BILU_OK) return -1;
     const int nnz = h.pa.nnz;
     if (pc == PC_ISAI_GEN && (h.pa.isai[BILU_ISAI_GENERAL].max_m > ISAI_GROUP ||
-                              (size_t)nnz * sizeof(double) > h.max_lds))
+                              (size_t)(nnz + ISAI_WAVES * WAVE) * sizeof(double) > h.max_lds))
         return -1;
     const size_t mbytes = ((size_t)batch * nnz * sizeof(double) + 255) / 256 * 256;
     void *ws = nullptr;
@@ -2357,7 +2377,8 @@ static int solve_ex_impl(bilu_pattern_t h, int batch, const double *d_A, const d
         const auto &pa = h->pa;
         const int nr = (pa.n + WAVE - 1) / WAVE;
         if (!pa.fast.ok || slot_template_mixed(pa.fast.S) < 0 || nr > MAX_NR || pa.fast.nr != nr ||
-            pa.isai[BILU_ISAI_GENERAL].max_m > ISAI_GROUP || (size_t)pa.nnz * sizeof(double) > h->max_lds)
+            pa.isai[BILU_ISAI_GENERAL].max_m > ISAI_GROUP ||
+            (size_t)(pa.nnz + ISAI_WAVES * WAVE) * sizeof(double) > h->max_lds)
             return BILU_ERR_TOO_LARGE;
         MixedParams mp{o.tol, o.tol_mode, (float)o.inner_tol, o.max_iter, o.max_refine, o.inner_tol, o.inner_adapt};
         const size_t fbytes = ((size_t)batch * pa.nnz * sizeof(float) + 255) / 256 * 256;
