@@ -867,10 +867,12 @@ __device__ __forceinline__ int isai_general_g(const DevIsai &M, const double *va
                     bad |= (de == 0.0);
                     const double mult = row[dp] / de;
                     const bool upd = t != dp;
-                    // the pivot row's columns from G / 2 on go through the wave's LDS cells
+                    // the pivot row's columns from CL on go through the wave's LDS cells
                     // pv (stored by the pivot lane, read back as a broadcast), the rest by
-                    // DPP -- the two pipes share the broadcasts (each alone sets the pace)
-                    constexpr int CL = G / 2;
+                    // DPP -- the two pipes share the broadcasts (each alone sets the pace;
+                    // CL measured: 32-lane rows 16 / 8 / 4 -> 10.4 / 8.6 / 8.5 ms, 16-lane
+                    // rows 8 / 4 -> 4.15 / 4.56 ms at n = 144, 65536 systems)
+                    constexpr int CL = G == 32 ? 4 : 8;
                     if constexpr (CL < G) {
                         if (t == dp) {
 #pragma unroll
