@@ -44,7 +44,10 @@ constexpr int ISAI_GROUP = 32;   // lanes per ISAI row (reference row_size_limit
 constexpr int ISAI_TRI_MAX = 2 * 64; // longest ISAI(L) / ISAI(U) row: two unknowns per lane
 
 enum PcKind { PC_LU = 0, PC_ISAI_LU = 1, PC_ISAI_GEN = 2, PC_NONE = 3, PC_ISAI_POW = 4, PC_ISAI_GEN_POW = 5 };
-enum FactorKind { FK_ILU = 0, FK_PARILU = 1, FK_LOAD = 2, FK_NONE = 3, FK_ILU_ENTRIES = 4, FK_ILU_ERAS = 5 };
+// FK_ISAI_LOADED (one-wave solve kernel, ISAI preconditioners): the ISAI values
+// come from the generation launches (gF, SolveParams::m2, ::zstatus)
+enum FactorKind { FK_ILU = 0, FK_PARILU = 1, FK_LOAD = 2, FK_NONE = 3, FK_ILU_ENTRIES = 4, FK_ILU_ERAS = 5,
+                  FK_ISAI_LOADED = 6 };
 
 struct DevIsai {
     const int *beg, *m, *goff, *gather, *rhs;
@@ -82,6 +85,8 @@ struct SolveParams {
     double tol;
     int tol_mode, max_iter, restart, max_restarts;
     int factor, sweeps;
+    const int *zstatus = nullptr;   // FK_ISAI_LOADED: zero-pivot flags of the generation, per system
+    const double *m2 = nullptr;     // FK_ISAI_LOADED, ISAI(L, U) on |L|^k, |U|^k: the ISAI(U) values
 };
 
 // ---------------------------------------------------------------- copies
@@ -951,7 +956,18 @@ __global__ __launch_bounds__(WAVE) void k_solve(DevPattern P, int batch, const d
 
     const double *Ap = gA + (size_t)sys * nnz;
     wave_copy(sA, Ap, nnz, lane);
-    if (sp.factor == FK_LOAD) {
+    // the ISAI values generated by their own launches (gF: ISAI(A) on A's pattern,
+    // or ISAI(L) / ISAI(A) on its chosen pattern; sp.m2: ISAI(U) on its pattern)
+    constexpr bool ISAI_LOADABLE = PC == PC_ISAI_GEN || PC == PC_ISAI_POW || PC == PC_ISAI_GEN_POW;
+    const bool loaded = ISAI_LOADABLE && sp.factor == FK_ISAI_LOADED;
+    if (loaded) {
+        if (PC == PC_ISAI_GEN) {
+            wave_copy(sW, gF + (size_t)sys * nnz, nnz, lane);
+        } else {
+            wave_copy(sML, gF + (size_t)sys * QL.nnz, QL.nnz, lane);
+            if (PC == PC_ISAI_POW) wave_copy(sMU, sp.m2 + (size_t)sys * QU.nnz, QU.nnz, lane);
+        }
+    } else if (sp.factor == FK_LOAD) {
         const double *Fp = gF + (size_t)sys * nnz;
         wave_copy(sF, Fp, nnz, lane);
     } else if (PC != PC_NONE) {
@@ -968,7 +984,9 @@ __global__ __launch_bounds__(WAVE) void k_solve(DevPattern P, int batch, const d
 
     // ---- preconditioner generation (fused: no HBM round trip) ----
     int bad = 0;
-    if (PC == PC_LU || PC == PC_ISAI_LU || PC == PC_ISAI_POW) {
+    if (loaded) {
+        bad = sp.zstatus[sys] != 0;
+    } else if (PC == PC_LU || PC == PC_ISAI_LU || PC == PC_ISAI_POW) {
         if (sp.factor == FK_ILU) {
             bad = ilu_factor(P, sF, lane);
         } else {
@@ -979,7 +997,9 @@ __global__ __launch_bounds__(WAVE) void k_solve(DevPattern P, int batch, const d
             bad = __any(bad);
         }
     }
-    if (PC == PC_ISAI_LU && !bad) {
+    if (loaded) {
+        // generated before this launch
+    } else if (PC == PC_ISAI_LU && !bad) {
         bad |= isai_tri(P.isai[BILU_ISAI_LOWER], P.n, BILU_ISAI_LOWER, sF, sW, nullptr, lane);
         bad |= isai_tri(P.isai[BILU_ISAI_UPPER], P.n, BILU_ISAI_UPPER, sF, sW, sU, lane);
     } else if (PC == PC_ISAI_GEN) {
@@ -2237,6 +2257,49 @@ static int solve_fp64(bilu_pattern_t h, int batch, const double *d_A, const doub
         const size_t lds = sizeof(double) * ((size_t)nnz * (2 + (sp.factor == FK_PARILU ? 1 : 0)) +
                                              (size_t)q1->dev.nnz + (q2 ? (size_t)q2->dev.nnz : 0) + (size_t)n * 3);
         if (lds > h->max_lds) return BILU_ERR_TOO_LARGE;
+        // the factors (one wave per system) and the ISAI values (k_isai_pow, four
+        // waves per system) by launches of their own, through the workspace, when
+        // their LDS fits; the solve kernel then loads them
+        const size_t glds = sizeof(double) * ((size_t)nnz + std::max<size_t>(q1->dev.nnz, q2 ? q2->dev.nnz : 0));
+        const size_t flds = sizeof(double) * (size_t)nnz * (sp.factor == FK_PARILU ? 3 : 2);
+        if (glds <= h->max_lds && flds <= h->max_lds) {
+            auto al = [](size_t b) { return (b + 255) / 256 * 256; };
+            const size_t bF = gen ? 0 : al((size_t)batch * nnz * sizeof(double));
+            const size_t b1 = al((size_t)batch * q1->dev.nnz * sizeof(double));
+            const size_t b2 = gen ? 0 : al((size_t)batch * q2->dev.nnz * sizeof(double));
+            void *ws = nullptr;
+            rc = solve_ws(*h, bF + b1 + b2 + (size_t)batch * sizeof(int), &ws, st, wsrc);
+            if (rc != BILU_OK) return rc;
+            char *wb = static_cast<char *>(ws);
+            double *Fw = reinterpret_cast<double *>(wb);
+            double *M1 = reinterpret_cast<double *>(wb + bF);
+            double *M2 = reinterpret_cast<double *>(wb + bF + b1);
+            int *Z = reinterpret_cast<int *>(wb + bF + b1 + b2);
+            if (glds > 65536) HIP_TRY(hipFuncSetAttribute((const void *)k_isai_pow, hipFuncAttributeMaxDynamicSharedMemorySize, (int)glds));
+            timing_mark(BILU_TIMING_FACTOR, st, true);
+            if (gen) {
+                hipLaunchKernelGGL(k_isai_pow, dim3(batch), dim3(WAVE * ISAI_WAVES), glds, st, h->dev, q1->dev, batch,
+                                   d_A, (int)BILU_ISAI_GENERAL, M1, Z);
+            } else {
+                if (flds > 65536) HIP_TRY(hipFuncSetAttribute((const void *)k_factor, hipFuncAttributeMaxDynamicSharedMemorySize, (int)flds));
+                hipLaunchKernelGGL(k_factor, dim3(batch), dim3(WAVE), flds, st, h->dev, batch, d_A,
+                                   (const double *)nullptr, Fw, sp.factor, sp.sweeps, (int *)nullptr);
+                // ISAI(L) has a unit diagonal: a zero pivot shows on U's diagonal
+                hipLaunchKernelGGL(k_isai_pow, dim3(batch), dim3(WAVE * ISAI_WAVES), glds, st, h->dev, q1->dev, batch,
+                                   (const double *)Fw, (int)BILU_ISAI_LOWER, M1, (int *)nullptr);
+                hipLaunchKernelGGL(k_isai_pow, dim3(batch), dim3(WAVE * ISAI_WAVES), glds, st, h->dev, q2->dev, batch,
+                                   (const double *)Fw, (int)BILU_ISAI_UPPER, M2, Z);
+            }
+            timing_mark(BILU_TIMING_FACTOR, st, false);
+            HIP_TRY(hipGetLastError());
+            sp.factor = FK_ISAI_LOADED;
+            sp.zstatus = Z;
+            sp.m2 = gen ? nullptr : M2;
+            if (gen) return launch_solve<PC_ISAI_GEN_POW>(*h, batch, d_A, d_b, d_x, M1, sp, d_iter, d_conv,
+                                                          d_iter_res, d_true_res, lds, st, q1->dev, DevIsaiPow{});
+            return launch_solve<PC_ISAI_POW>(*h, batch, d_A, d_b, d_x, M1, sp, d_iter, d_conv, d_iter_res,
+                                             d_true_res, lds, st, q1->dev, q2->dev);
+        }
         if (gen) return launch_solve<PC_ISAI_GEN_POW>(*h, batch, d_A, d_b, d_x, nullptr, sp, d_iter, d_conv,
                                                       d_iter_res, d_true_res, lds, st, q1->dev, DevIsaiPow{});
         return launch_solve<PC_ISAI_POW>(*h, batch, d_A, d_b, d_x, nullptr, sp, d_iter, d_conv, d_iter_res,
@@ -2293,7 +2356,26 @@ static int solve_fp64(bilu_pattern_t h, int batch, const double *d_A, const doub
     if (lds > h->max_lds) rc = BILU_ERR_TOO_LARGE;
     else if (pc == PC_LU) rc = launch_solve<PC_LU>(*h, batch, d_A, d_b, d_x, F, sp, d_iter, d_conv, d_iter_res, d_true_res, lds, st);
     else if (pc == PC_ISAI_LU) rc = launch_solve<PC_ISAI_LU>(*h, batch, d_A, d_b, d_x, F, sp, d_iter, d_conv, d_iter_res, d_true_res, lds, st);
-    else if (pc == PC_ISAI_GEN) rc = launch_solve<PC_ISAI_GEN>(*h, batch, d_A, d_b, d_x, F, sp, d_iter, d_conv, d_iter_res, d_true_res, lds, st);
+    else if (pc == PC_ISAI_GEN) {
+        // the general ISAI by its multi-wave launches (k_isai_gen), through the
+        // workspace, when their LDS fits; else generated inside the solve kernel
+        if ((size_t)(nnz + ISAI_WAVES * WAVE) * sizeof(double) <= h->max_lds) {
+            const size_t mb = ((size_t)batch * nnz * sizeof(double) + 255) / 256 * 256;
+            void *ws = nullptr;
+            rc = solve_ws(*h, mb + (size_t)batch * sizeof(int), &ws, st, wsrc);
+            if (rc != BILU_OK) return rc;
+            double *M = static_cast<double *>(ws);
+            int *Z = reinterpret_cast<int *>(static_cast<char *>(ws) + mb);
+            timing_mark(BILU_TIMING_FACTOR, st, true);
+            rc = launch_isai_general(*h, batch, d_A, M, false, Z, st);
+            timing_mark(BILU_TIMING_FACTOR, st, false);
+            if (rc != BILU_OK) return rc;
+            sp.factor = FK_ISAI_LOADED;
+            sp.zstatus = Z;
+            F = M;
+        }
+        rc = launch_solve<PC_ISAI_GEN>(*h, batch, d_A, d_b, d_x, F, sp, d_iter, d_conv, d_iter_res, d_true_res, lds, st);
+    }
     else rc = launch_solve<PC_NONE>(*h, batch, d_A, d_b, d_x, F, sp, d_iter, d_conv, d_iter_res, d_true_res, lds, st);
     return rc;
 }
