@@ -958,11 +958,20 @@ __global__ __launch_bounds__(WAVE) void k_solve(DevPattern P, int batch, const d
     wave_copy(sA, Ap, nnz, lane);
     // the ISAI values generated by their own launches (gF: ISAI(A) on A's pattern,
     // or ISAI(L) / ISAI(A) on its chosen pattern; sp.m2: ISAI(U) on its pattern)
-    constexpr bool ISAI_LOADABLE = PC == PC_ISAI_GEN || PC == PC_ISAI_POW || PC == PC_ISAI_GEN_POW;
+    constexpr bool ISAI_LOADABLE = PC == PC_ISAI_GEN || PC == PC_ISAI_POW || PC == PC_ISAI_GEN_POW || PC == PC_ISAI_LU;
     const bool loaded = ISAI_LOADABLE && sp.factor == FK_ISAI_LOADED;
     if (loaded) {
         if (PC == PC_ISAI_GEN) {
             wave_copy(sW, gF + (size_t)sys * nnz, nnz, lane);
+        } else if (PC == PC_ISAI_LU) {
+            // ISAI(L) strictly lower, ISAI(U) from the diagonal (the factor kernel's
+            // layout): U's diagonal moves to sU, ISAI(L)'s unit diagonal takes its place
+            wave_copy(sW, gF + (size_t)sys * nnz, nnz, lane);
+            __syncthreads();
+            for (int i = lane; i < n; i += WAVE) {
+                sU[i] = sW[P.diag[i]];
+                sW[P.diag[i]] = 1.0;
+            }
         } else {
             wave_copy(sML, gF + (size_t)sys * QL.nnz, QL.nnz, lane);
             if (PC == PC_ISAI_POW) wave_copy(sMU, sp.m2 + (size_t)sys * QU.nnz, QU.nnz, lane);
@@ -2355,7 +2364,25 @@ static int solve_fp64(bilu_pattern_t h, int batch, const double *d_A, const doub
     const size_t lds = sizeof(double) * ((size_t)nnz * (2 + (need_w ? 1 : 0)) + (size_t)n * (3 + (pc == PC_ISAI_LU ? 1 : 0)));
     if (lds > h->max_lds) rc = BILU_ERR_TOO_LARGE;
     else if (pc == PC_LU) rc = launch_solve<PC_LU>(*h, batch, d_A, d_b, d_x, F, sp, d_iter, d_conv, d_iter_res, d_true_res, lds, st);
-    else if (pc == PC_ISAI_LU) rc = launch_solve<PC_ISAI_LU>(*h, batch, d_A, d_b, d_x, F, sp, d_iter, d_conv, d_iter_res, d_true_res, lds, st);
+    else if (pc == PC_ISAI_LU) {
+        // the factors and ISAI(L), ISAI(U) by the persistent factor kernel (FP64),
+        // through the workspace, when its plan exists; else inside the solve kernel
+        if (pa.fast.ok) {
+            const size_t mb = ((size_t)batch * nnz * sizeof(double) + 255) / 256 * 256;
+            void *ws = nullptr;
+            rc = solve_ws(*h, mb + (size_t)batch * sizeof(int), &ws, st, wsrc);
+            if (rc != BILU_OK) return rc;
+            double *M = static_cast<double *>(ws);
+            int *Z = reinterpret_cast<int *>(static_cast<char *>(ws) + mb);
+            rc = launch_factor_fast<double>(*h, batch, d_A, nullptr, M, sp.factor == FK_PARILU ? FS_PARILU : FS_ILU,
+                                            sp.sweeps, 0.0, nullptr, st, 1, M, Z);
+            if (rc != BILU_OK) return rc;
+            sp.factor = FK_ISAI_LOADED;
+            sp.zstatus = Z;
+            F = M;
+        }
+        rc = launch_solve<PC_ISAI_LU>(*h, batch, d_A, d_b, d_x, F, sp, d_iter, d_conv, d_iter_res, d_true_res, lds, st);
+    }
     else if (pc == PC_ISAI_GEN) {
         // the general ISAI by its multi-wave launches (k_isai_gen), through the
         // workspace, when their LDS fits; else generated inside the solve kernel
