@@ -65,7 +65,8 @@ extern "C" {
 
 #define BILU_MERGED 0         /* factor and solve in one launch (merged.cu) */
 #define BILU_SEGREGATED 1     /* factor launch, factors through HBM, solve launch (segregated.cu) */
-#define BILU_MERGE_AUTO 2     /* FP64: merged (ParILU(0): segregated); FP32_IR: segregated (the faster at
+#define BILU_MERGE_AUTO 2     /* FP64: merged (ParILU(0), and ILU(0) solved on the one-wave kernel:
+                                 segregated); FP32_IR: segregated (the faster at
                                  n = 144 when measured); bilu_uneven_tune picks per pattern.  Results do
                                  not depend on it */
 

@@ -2320,9 +2320,15 @@ static int solve_fp64(bilu_pattern_t h, int batch, const double *d_A, const doub
     const double *F = nullptr;
     // BILU_MERGE_AUTO splits ParILU(0): the factor kernel's lane-synchronous
     // sweeps beat the merged kernel's register-lean ones (n = 144, batch
-    // 65536, 5 sweeps: 13.6 vs 25.6 ms), with the same results
+    // 65536, 5 sweeps: 13.6 vs 25.6 ms), with the same results; and exact
+    // ILU(0) when the solve runs on the one-wave kernel (the persistent factor
+    // kernel's lane streams beat its row levels: n = 400, batch 8192: 51.2 vs
+    // 55.3 ms)
+    const bool fast_solve = pc == PC_LU && o.kernel != BILU_KERNEL_WAVE &&
+                            launch_fast(*h, batch, d_A, d_b, d_x, nullptr, sp, d_iter, d_conv, d_iter_res,
+                                        d_true_res, st, FPC_LU, nullptr, true) == BILU_OK;
     const bool seg = o.merge == BILU_SEGREGATED ||
-                     (o.merge == BILU_MERGE_AUTO && sp.factor == FK_PARILU && pa.fast.ok);
+                     (o.merge == BILU_MERGE_AUTO && pa.fast.ok && (sp.factor == FK_PARILU || !fast_solve));
     if (seg && pc == PC_LU) {
         // factor launch, factors through HBM (the handle's workspace), then the
         // solve launch (segregated.cu:429-490)
