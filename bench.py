@@ -50,7 +50,8 @@ def parse():
                     help="preconditioner of the uneven leg (ISAI(L, U) applies or triangular solves)")
     ap.add_argument("--precond", default="ilu0",
                     choices=["ilu0", "parilu0", "ilu0_isai", "parilu0_isai", "general_isai"])
-    ap.add_argument("--merge", default="merged", choices=["merged", "segregated"])
+    ap.add_argument("--merge", default="auto", choices=["auto", "merged", "segregated"],
+                    help="--mode single: the merge point (auto = the library's default rule)")
     ap.add_argument("--kernel", default="auto", choices=["auto", "wave", "persistent"])
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32_ir"],
                     help="fp32_ir: FP32 preconditioner and inner BiCGSTAB inside FP64 refinement")
@@ -359,7 +360,7 @@ def main():
         mixed = args.precision == "fp32_ir"
         o = bilu.opts(precond=PC[args.precond], parilu_sweeps=5, tol=1e-10, tol_mode=bilu.TOL_RELATIVE,
                       true_res_restart=1, max_restarts=3,
-                      merge=bilu.MERGED if args.merge == "merged" else bilu.SEGREGATED,
+                      merge={"auto": bilu.MERGE_AUTO, "merged": bilu.MERGED, "segregated": bilu.SEGREGATED}[args.merge],
                       kernel={"auto": bilu.KERNEL_AUTO, "wave": bilu.KERNEL_WAVE,
                               "persistent": bilu.KERNEL_PERSISTENT}[args.kernel],
                       precision=bilu.PRECISION_FP32_IR if mixed else bilu.PRECISION_FP64,
@@ -367,8 +368,7 @@ def main():
         ms, kms = run(o, kernel=bilu.TIMING_SOLVE)
         s = summary(ms)
         kname = ("k_mixed" if mixed else
-                 "k_fast" if (args.merge == "merged" and args.kernel != "wave" and args.isai_power == 1)
-                 else "k_solve")
+                 "k_fast" if (args.kernel != "wave" and args.isai_power == 1 and n <= 192) else "k_solve")
         rec.update({"value": s["systems_per_s"], "ms_per_step": s["ms_per_step"],
                     "dtype": "f32+f64" if mixed else "f64",
                     "config": dict(base_cfg, workload=f"random pattern n={n} nnz={nnz}, batch {B}, "
