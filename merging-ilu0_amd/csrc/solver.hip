@@ -1215,7 +1215,7 @@ __global__ __launch_bounds__(WAVE * ISAI_WAVES) void k_isai(DevPattern P, int ba
 template <int G, typename TO>
 __global__ __launch_bounds__(WAVE * ISAI_WAVES) void k_isai_gen(DevIsai M, int nnz, int batch,
                                                                const double *__restrict__ gV,
-                                                               TO *__restrict__ gOut,
+                                                               TO *__restrict__ gOut, int ostride,
                                                                int *__restrict__ g_status, int r0, int r1,
                                                                int or_status)
 {
@@ -1233,7 +1233,7 @@ __global__ __launch_bounds__(WAVE * ISAI_WAVES) void k_isai_gen(DevIsai M, int n
     // once (a cell a step does not store holds an earlier step's value)
     double *pv = lds + nnz + (size_t)w * WAVE;
     pv[lane] = 0.0;
-    const int bad = isai_general_g<G, true, TO>(M, lds, gOut + (size_t)sys * nnz, lane, r0, r1, w, ISAI_WAVES, pv);
+    const int bad = isai_general_g<G, true, TO>(M, lds, gOut + (size_t)sys * ostride, lane, r0, r1, w, ISAI_WAVES, pv);
     if (__any(bad) && lane == 0) atomicOr(&s_bad, 1);
     __syncthreads();
     if (g_status && tid == 0) {
@@ -2165,13 +2165,13 @@ int solve_ws(bilu_pattern_s &h, size_t bytes, void **out, hipStream_t st, const 
     return BILU_OK;
 }
 
-// general ISAI of a batch (values V, output on the pattern): the paired rows,
-// then the four-per-wave rows, two launches on the stream
-static int launch_isai_general(const bilu_pattern_s &h, int batch, const double *V, void *out, bool f32,
-                               int *status, hipStream_t st)
+// general ISAI of a batch (values V on the pattern, output on the map's
+// pattern: ostride values per system): the paired rows, then the
+// four-per-wave rows, two launches on the stream
+static int launch_isai_gen(const bilu_pattern_s &h, const DevIsai &M, int np, int ostride, int batch,
+                           const double *V, void *out, bool f32, int *status, hipStream_t st)
 {
-    const int n = h.pa.n, nnz = h.pa.nnz, np = h.pa.isai[BILU_ISAI_GENERAL].n_pair;
-    const DevIsai &M = h.dev.isai[BILU_ISAI_GENERAL];
+    const int n = h.pa.n, nnz = h.pa.nnz;
     const size_t lds = (size_t)nnz * sizeof(double) + (size_t)ISAI_WAVES * WAVE * sizeof(double);  // + pivot-row cells
     if (lds > h.max_lds) return BILU_ERR_TOO_LARGE;
     const void *k32 = f32 ? (const void *)k_isai_gen<32, float> : (const void *)k_isai_gen<32, double>;
@@ -2184,18 +2184,26 @@ static int launch_isai_general(const bilu_pattern_s &h, int batch, const double 
     double *o64 = static_cast<double *>(out);
     if (np > 0) {
         if (f32) hipLaunchKernelGGL((k_isai_gen<32, float>), dim3(batch), dim3(WAVE * ISAI_WAVES), lds, st, M, nnz,
-                                    batch, V, o32, status, 0, np, 0);
+                                    batch, V, o32, ostride, status, 0, np, 0);
         else hipLaunchKernelGGL((k_isai_gen<32, double>), dim3(batch), dim3(WAVE * ISAI_WAVES), lds, st, M, nnz,
-                                batch, V, o64, status, 0, np, 0);
+                                batch, V, o64, ostride, status, 0, np, 0);
     }
     if (n > np) {
         if (f32) hipLaunchKernelGGL((k_isai_gen<16, float>), dim3(batch), dim3(WAVE * ISAI_WAVES), lds, st, M, nnz,
-                                    batch, V, o32, status, np, n, np > 0 ? 1 : 0);
+                                    batch, V, o32, ostride, status, np, n, np > 0 ? 1 : 0);
         else hipLaunchKernelGGL((k_isai_gen<16, double>), dim3(batch), dim3(WAVE * ISAI_WAVES), lds, st, M, nnz,
-                                batch, V, o64, status, np, n, np > 0 ? 1 : 0);
+                                batch, V, o64, ostride, status, np, n, np > 0 ? 1 : 0);
     }
     HIP_TRY(hipGetLastError());
     return BILU_OK;
+}
+
+// the general ISAI on the pattern itself
+static int launch_isai_general(const bilu_pattern_s &h, int batch, const double *V, void *out, bool f32,
+                               int *status, hipStream_t st)
+{
+    return launch_isai_gen(h, h.dev.isai[BILU_ISAI_GENERAL], h.pa.isai[BILU_ISAI_GENERAL].n_pair, h.pa.nnz, batch,
+                           V, out, f32, status, st);
 }
 
 // FP64 ISAI solvers on the persistent kernel: a generation launch writes
@@ -2287,8 +2295,8 @@ static int solve_fp64(bilu_pattern_t h, int batch, const double *d_A, const doub
             if (glds > 65536) HIP_TRY(hipFuncSetAttribute((const void *)k_isai_pow, hipFuncAttributeMaxDynamicSharedMemorySize, (int)glds));
             timing_mark(BILU_TIMING_FACTOR, st, true);
             if (gen) {
-                hipLaunchKernelGGL(k_isai_pow, dim3(batch), dim3(WAVE * ISAI_WAVES), glds, st, h->dev, q1->dev, batch,
-                                   d_A, (int)BILU_ISAI_GENERAL, M1, Z);
+                rc = launch_isai_gen(*h, q1->dev.map, q1->dev.map.n_pair, q1->dev.nnz, batch, d_A, M1, false, Z, st);
+                if (rc != BILU_OK) return rc;
             } else {
                 if (flds > 65536) HIP_TRY(hipFuncSetAttribute((const void *)k_factor, hipFuncAttributeMaxDynamicSharedMemorySize, (int)flds));
                 hipLaunchKernelGGL(k_factor, dim3(batch), dim3(WAVE), flds, st, h->dev, batch, d_A,
@@ -2870,6 +2878,9 @@ int bilu_isai_power(bilu_pattern_t h, int batch, const double *d_vals, int type,
     const size_t lds = ((size_t)h->pa.nnz + (size_t)q->dev.nnz) * sizeof(double);
     if (lds > h->max_lds) return BILU_ERR_TOO_LARGE;
     HIP_TRY(hipSetDevice(h->device));
+    if (type == BILU_ISAI_GENERAL)   // the general ISAI's own kernels, on the chosen pattern's map
+        return launch_isai_gen(*h, q->dev.map, q->dev.map.n_pair, q->dev.nnz, batch, d_vals, d_out, false, d_status,
+                               (hipStream_t)stream);
     if (lds > 65536) HIP_TRY(hipFuncSetAttribute((const void *)k_isai_pow, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     hipLaunchKernelGGL(k_isai_pow, dim3(batch), dim3(WAVE * ISAI_WAVES), lds, (hipStream_t)stream, h->dev, q->dev, batch, d_vals,
                        type, d_out, d_status);
