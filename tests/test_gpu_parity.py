@@ -236,6 +236,25 @@ def test_true_residual_kernel():
     assert np.array_equal(res.cpu().numpy(), want)
 
 
+@pytest.mark.parametrize("precond", [bilu.ILU0_ISAI, bilu.PARILU0_ISAI, bilu.GENERAL_ISAI])
+def test_wave_kernel_loaded_isai_zero_pivot(precond):
+    """one-wave solve kernel with ISAI values from their own generation launches: a zero pivot
+    (a zero diagonal of A: the first pivot for ILU(0), a singular block for the general ISAI) flags the
+    system (-2, x = 0) as the oracle does; the other systems bitwise the oracle"""
+    print(f"case wave loaded isai zero pivot precond={precond}", flush=True)
+    pat, A, b = synth.batch("random", 8, n=144)
+    A = A.copy()
+    if precond == bilu.GENERAL_ISAI:
+        A[2, pat.row_ptrs[0]:pat.row_ptrs[1]] = 0.0      # row 0 of A zero
+    else:
+        A[2, pat.diag[0]] = 0.0                          # u_00 = 0
+    kw = dict(precond=precond, parilu_sweeps=5, tol=1e-10, tol_mode=bilu.TOL_RELATIVE)
+    g = run_gpu(pat, A, b, kernel=bilu.KERNEL_WAVE, **kw)
+    o = run_oracle(pat, A, b, kernel=bilu.KERNEL_WAVE, **kw)
+    assert g[2][2] == bilu.ZERO_PIVOT and not g[0][2].any()
+    assert_same(g, o, f"wave loaded isai zero pivot {precond}")
+
+
 def test_edge_cases_no_hang():
     pat, A, b = synth.batch("random", 8, n=144)
     A = A.copy()
