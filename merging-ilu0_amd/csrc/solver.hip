@@ -2209,7 +2209,7 @@ static int launch_isai_general(const bilu_pattern_s &h, int batch, const double 
 // FP64 ISAI solvers on the persistent kernel: a generation launch writes
 // the ISAI values and their zero-pivot flags into the workspace -- ILU(0) or
 // ParILU(0) factors then ISAI(L), ISAI(U) (k_factor_fast; FP64 throughout),
-// or the general ISAI of A (k_isai) -- then the persistent solve applies them
+// or the general ISAI of A (k_isai_gen) -- then the persistent solve applies them
 // (ilu_isai_solve.cu, gisai_solve.cu).  -1, with nothing launched, when the
 // persistent solve cannot take the pattern.
 static int solve_isai_fast(bilu_pattern_s &h, int batch, const double *A, const double *b, double *x, int pc,
