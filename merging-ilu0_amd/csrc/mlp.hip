@@ -271,6 +271,44 @@ __global__ __launch_bounds__(256, 2) void k_gemm_nt(const __bf16 *__restrict__ A
     if (s0 + 2 < S) step(std::integral_constant<int, 2>(), s0 + 2);
     if (s0 + 3 < S) step(std::integral_constant<int, 3>(), s0 + 3);
     if (s0 + 4 < S) step(std::integral_constant<int, 4>(), s0 + 4);
+    if constexpr (EPI == EPI_BIAS_F16 || EPI == EPI_BIAS_GELU_H) {
+        // 16-bit outputs leave through the LDS: in the MFMA layout a lane
+        // holds four rows of one column, so direct stores write 32-byte
+        // pieces of rows.  Each wave converts its 64 x 128 tile in two
+        // 64 x 64 halves into its own LDS region, then writes whole 128-byte
+        // row pieces, 16 bytes per lane.  Values and rounding as below.
+        using T16 = std::conditional_t<EPI == EPI_BIAS_F16, _Float16, __bf16>;
+        constexpr int CP = 64 + 8;   // row pitch (elements): 144 bytes
+        __shared__ __align__(16) T16 Cs[4][64 * CP];
+        T16 *cw = Cs[wave];
+        T16 *out = EPI == EPI_BIAS_F16 ? reinterpret_cast<T16 *>(C) : reinterpret_cast<T16 *>(H);
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+#pragma unroll
+                for (int jj = 0; jj < 4; jj++) {
+                    const float bn = bias[n0 + 64 * h + 16 * jj + lane % 16];
+#pragma unroll
+                    for (int r = 0; r < 4; r++) {
+                        const float z = acc[i][4 * h + jj][r] + bn;
+                        T16 y;
+                        if constexpr (EPI == EPI_BIAS_F16) y = (T16)z;
+                        else y = (T16)gelu(z);
+                        cw[(16 * i + 4 * (lane / 16) + r) * CP + 16 * jj + lane % 16] = y;
+                    }
+                }
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                const int row = 8 * q + lane / 8, c = 8 * (lane % 8);
+                const bf16x8 v = *reinterpret_cast<const bf16x8 *>(&cw[row * CP + c]);
+                *reinterpret_cast<bf16x8 *>(&out[(size_t)(m0 + row) * ldc + n0 + 64 * h + c]) = v;
+            }
+            __syncthreads();
+        }
+        return;
+    }
 #pragma unroll
     for (int i = 0; i < 4; i++)
 #pragma unroll
@@ -541,7 +579,9 @@ void gemm(hipStream_t st, int M, int N, int K, const __bf16 *A, int lda, const _
           int ldc, const float *bias = nullptr, float *Z = nullptr, __bf16 *H = nullptr)
 {
     if constexpr (!TA && TB) {
-        if (M % FM == 0 && N % FN == 0 && K % FK == 0) {
+        // the 16-bit epilogues store 8 outputs per lane: rows 16-byte aligned
+        constexpr bool w16 = EPI == EPI_BIAS_F16 || EPI == EPI_BIAS_GELU_H;
+        if (M % FM == 0 && N % FN == 0 && K % FK == 0 && (!w16 || ldc % 8 == 0)) {
             hipLaunchKernelGGL((k_gemm_nt<EPI>), dim3((M / FM) * (N / FN)), dim3(256), 0, st, A, lda, B, ldb, K,
                                N / FN, C, ldc, bias, Z, H);
             return;
